@@ -1,0 +1,176 @@
+/* od.h — C-ABI of libodhip.so, the MI355X (gfx950) implicit-step hot path.
+ *
+ * Conventions (no reference counterpart: the reference has no ABI, no error
+ * codes and no device memory):
+ *   - every call returns 0 on success and a negative code on error; the text
+ *     of the last error of the calling thread is od_last_error();
+ *   - pointers named *_host are host memory borrowed for the call; pointers
+ *     named *_dev are device memory (od_dev_alloc or any HIP allocation on the
+ *     context's device);
+ *   - each handle is bound to its context's HIP stream; calls are ordered on
+ *     that stream and handles must not be used from several threads at once;
+ *   - DOF vectors are over free DOFs in the reference's compacted numbering
+ *     (core/nodeIndexer.cpp:12-25): a constrained DOF is removed and later
+ *     DOFs shift down.  The device layout needs whole-node constraints.
+ *
+ * Each entry point cites the reference interface it replaces
+ * (paths relative to OpenDeformer/ in the reference tree).
+ */
+#ifndef OD_H
+#define OD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct od_ctx od_ctx;
+typedef struct od_mesh od_mesh;
+typedef struct od_matrix od_matrix;
+typedef struct od_solver od_solver;
+typedef struct od_integrator od_integrator;
+
+enum {
+    OD_OK = 0,
+    OD_ERR_ARG = -1,       /* invalid argument */
+    OD_ERR_HIP = -2,       /* HIP runtime error */
+    OD_ERR_STATE = -3,     /* call not valid in the handle's state */
+    OD_ERR_UNSUPPORTED = -4
+};
+
+/* material kinds per tet */
+enum { OD_MAT_COROTATIONAL = 0, OD_MAT_NEOHOOKEAN = 1 };
+/* preconditioners (core/preconditioner.h:12-22 family) */
+enum { OD_PRE_NONE = 0, OD_PRE_JACOBI = 1, OD_PRE_IC0 = 3 };
+/* arithmetic of the inner PCG iterations */
+enum { OD_FP64 = 0, OD_MIXED = 1 };
+
+const char *od_last_error(void);
+int od_version(void);
+
+/* ---- context and device memory (no reference counterpart) ---------------- */
+int od_ctx_create(int device, od_ctx **out);
+int od_ctx_destroy(od_ctx *ctx);
+int od_ctx_sync(od_ctx *ctx);
+int od_dev_alloc(od_ctx *ctx, long long bytes, void **dev);
+int od_dev_free(od_ctx *ctx, void *dev);
+int od_copy_h2d(od_ctx *ctx, void *dev, const void *host, long long bytes);
+int od_copy_d2h(od_ctx *ctx, void *host, const void *dev, long long bytes);
+int od_memset_d(od_ctx *ctx, void *dev, int byte, long long bytes);
+
+/* ---- mesh, DOF indexing and the canonical pattern --------------------------
+ * Replaces: Mesh/TetMesh set-up (core/mesh.cpp:7-40), NodeIndexer
+ * (core/nodeIndexer.cpp:8-74), FullOrderNonlinearMaterial::getMatrixStructure
+ * (material/fullOrderNonlinear.h:46-71) with BlockedSymSparseMatrix's
+ * constructor (core/sparseMatrix.h:186-274), and getMatrixIndicesPerElement
+ * (material/fullOrderNonlinear.h:73-110).
+ * X_host: 3*nNodes rest positions; tets_host: 4*nTets node ids;
+ * constrained_host: sorted DOF ids 3*node+axis (Constrainer::outputConstrains,
+ * core/constrainer.cpp:74-79).  nInstances > 1 concatenates independent
+ * meshes: instNodeStart/instTetStart (nInstances+1 entries) give each one's
+ * node and tet range (NULL when nInstances == 1). */
+int od_mesh_create(od_ctx *ctx, int nNodes, const double *X_host, int nTets, const int *tets_host,
+                   int nConstrained, const int *constrained_host, int nInstances,
+                   const int *instNodeStart_host, const int *instTetStart_host, od_mesh **out);
+int od_mesh_destroy(od_mesh *mesh);
+/* info[16]: ndof, nBlockRows, nBlocks, nValues, nLevels, nInstances, nTets,
+ * nNodes, nChunks, nPairContrib, nNodeContrib, device bytes, 0... */
+int od_mesh_info(od_mesh *mesh, long long *info_host);
+/* Canonical layout, identical to BlockedSymSparseMatrix<3,3>'s blockPcol,
+ * blockColumnOris, blockRows (core/sparseMatrix.h:627-635), and the 78-entry
+ * per-tet map (SparseSymMatrixIndicesPerElementCache,
+ * core/sparseMatrix.cpp:97-118), the latter expanded on the device from the
+ * block-level map the assembly kernels use.  elemMap78_host may be NULL. */
+int od_pattern_export(od_mesh *mesh, int *blockPcol_host, int *blockColumnOris_host, int *blockRows_host,
+                      int *elemMap78_host);
+/* Per-instance row range and IC(0) level structure for tests:
+ * instRowStart_host (nInstances+1), level_host (nBlockRows). */
+int od_mesh_rows(od_mesh *mesh, int *instRowStart_host, int *level_host);
+
+/* Material per tet.  Replaces the constructors of
+ * CorotationalHyperelasticMaterial (material/corotationalHyperelastic.cpp:8-18)
+ * and InvertibleNeoHookeanMaterial (material/invertibleNeoHookean.cpp:5-9),
+ * extended to per-tet (kind, E, nu, rho).  corotThreshold is the compression
+ * threshold selecting polar vs QR (element/tetelement.cpp:571-572),
+ * neoThreshold the singular-value clamp (material/invertibleHyperelastic.cpp:239-240). */
+int od_mesh_set_material(od_mesh *mesh, const int *kind_host, const double *E_host, const double *nu_host,
+                         const double *rho_host, double corotThreshold, double neoThreshold);
+
+/* Free-DOF body-force virtual works (Forcer::addBodyForce + getVirtualWorks,
+ * core/forcer.cpp:8-33, 80-96): f_dev = sum over tets of V/4 * body. */
+int od_body_force(od_mesh *mesh, const double *bodyPerTet_host /* 3*nTets */, double *f_dev);
+
+/* ---- matrices in the canonical layout ---------------------------------------- */
+int od_matrix_create(od_mesh *mesh, od_matrix **out);
+int od_matrix_destroy(od_matrix *m);
+int od_matrix_values(od_matrix *m, double **values_dev);
+int od_matrix_upload(od_matrix *m, const double *values_host);
+int od_matrix_download(od_matrix *m, double *values_host);
+/* BlockedSymSparseMatrix::setZeros (core/sparseMatrix.h:277-279) */
+int od_matrix_zero(od_matrix *m);
+/* values = a*values + b*other  (Scale + Add, core/sparseMatrix.h:281-295) */
+int od_matrix_axpby(od_matrix *m, double a, double b, const od_matrix *other);
+/* MechMaterial::generateMassMatrix (core/mechMaterial.cpp:69-98): M += mass */
+int od_mass_matrix(od_mesh *mesh, od_matrix *M);
+/* FullOrderNonlinearMaterial::generateMatrixAndVirtualWorks for the per-tet
+ * materials (material/corotationalHyperelastic.cpp:43-102,
+ * material/invertibleHyperelastic.cpp:34-114): accumulates K += K(u) and
+ * fint += f(u) at free-DOF displacement u_dev. */
+int od_assemble(od_mesh *mesh, const double *u_dev, od_matrix *K, double *fint_dev);
+/* SpMDV (core/numerMethod.h:134-262): y = A x, or y += A x when accumulate. */
+int od_spmv(od_matrix *A, const double *x_dev, double *y_dev, int accumulate);
+/* FP32 form on an FP32 copy of A refreshed by od_matrix_sync_f32 */
+int od_matrix_sync_f32(od_matrix *A);
+int od_spmv_f32(od_matrix *A, const float *x_dev, float *y_dev, int accumulate);
+
+/* ---- PCG ---------------------------------------------------------------------
+ * Replaces CGSolver::solveLinearSystem (solver/cgSolver.cpp:13-57) with its
+ * preconditioner (JacobiPreconditioner, solver/jacobiPreconditioner.cpp:6-26;
+ * IC(0) in the reference's Preconditioner role, solver/inCholeskyPreconditioner.cpp):
+ * warm start from x_dev, preconditioner rebuilt from A on every solve, stop
+ * when ||r||^2 <= tol^2 ||b||^2 or after maxIter iterations, per instance.
+ * iters_host / relres_host (nInstances entries each) may be NULL. */
+int od_solver_create(od_mesh *mesh, int precond, double precondParam, int precision, od_solver **out);
+int od_solver_destroy(od_solver *s);
+int od_pcg_solve(od_solver *s, od_matrix *A, const double *b_dev, double *x_dev, double tol, int maxIter,
+                 int *iters_host, double *relres_host);
+/* Preconditioner::Preprocess + solvePreconditionerSystem once: z = P(A)^-1 r */
+int od_precond_apply(od_solver *s, od_matrix *A, const double *r_dev, double *z_dev);
+/* IC(0) factor of A in the canonical layout (U = L^T with U^T U ~ A). */
+int od_ic0_factor(od_solver *s, od_matrix *A, double *L_host);
+
+/* ---- integrators ---------------------------------------------------------------
+ * Full-order linearised Newmark (the reference's Newmark update,
+ * intergrator/linearNewmark.cpp:27-30, 57-79, at full order with the assembly
+ * structure of intergrator/nonlinearImplicitBackwardEuler.h:82-101) and the
+ * reference's NonlinearImplicitBackwardEuler (same file, :48-146), both behind
+ * the Intergrator interface (core/intergrator.h:13-35). */
+int od_newmark_create(od_mesh *mesh, double beta, double gamma, double dt, double alphaM, double alphaK,
+                      int precond, double precondParam, double tol, int maxIter, int precision, int initAccel,
+                      od_integrator **out);
+int od_backward_euler_create(od_mesh *mesh, double dt, double alphaM, double alphaK, int precond,
+                             double precondParam, double tol, int maxIter, od_integrator **out);
+int od_integrator_destroy(od_integrator *it);
+/* Intergrator::setExternalVirtualWork (core/intergrator.h:18) */
+int od_integrator_set_external(od_integrator *it, const double *f_host);
+/* one-step load added to the next step's external force only */
+int od_integrator_set_impulse(od_integrator *it, const double *f_host);
+/* per-instance mask (nInstances bytes): inactive instances do not advance */
+int od_integrator_set_active(od_integrator *it, const unsigned char *active_host);
+/* Intergrator::runOneTimeStep x nSteps (core/intergrator.h:25) */
+int od_integrator_step(od_integrator *it, int nSteps);
+int od_integrator_get_state(od_integrator *it, double *d_host, double *v_host, double *a_host);
+int od_integrator_set_state(od_integrator *it, const double *d_host, const double *v_host, const double *a_host);
+/* device pointers of the state vectors (d, v, a) for device-side consumers */
+int od_integrator_state_dev(od_integrator *it, double **d_dev, double **v_dev, double **a_dev);
+/* last step: PCG iterations and relative residual per instance, outer
+ * refinement sweeps (mixed precision), initial-acceleration iterations */
+int od_integrator_stats(od_integrator *it, int *iters_host, double *relres_host, int *outer_host, int *initIters_host);
+/* NonlinearImplicitBackwardEuler::updateMeshVerticesDisplacements
+ * (nonlinearImplicitBackwardEuler.h:103-116): u += h v on free DOFs */
+int od_backward_euler_update_mesh(od_integrator *it);
+/* Simulator::getVertexPositions (core/simulator.cpp:24-33): float X + u per node */
+int od_integrator_positions(od_integrator *it, float *pos_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

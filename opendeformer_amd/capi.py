@@ -1,0 +1,100 @@
+"""ctypes binding of libodhip.so (the C-ABI declared in include/od.h).
+
+There is no CPU fallback: if the HIP library is missing or fails to load,
+`lib()` raises, and every handle constructor goes through it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libodhip.so")
+_LIB = None
+
+OD_MAT_COROTATIONAL, OD_MAT_NEOHOOKEAN = 0, 1
+OD_PRE_NONE, OD_PRE_JACOBI, OD_PRE_IC0 = 0, 1, 3
+OD_FP64, OD_MIXED = 0, 1
+
+# every symbol include/od.h declares
+EXPORTS = [
+    "od_last_error", "od_version", "od_ctx_create", "od_ctx_destroy", "od_ctx_sync", "od_dev_alloc", "od_dev_free",
+    "od_copy_h2d", "od_copy_d2h", "od_memset_d", "od_mesh_create", "od_mesh_destroy", "od_mesh_info",
+    "od_pattern_export", "od_mesh_rows", "od_mesh_set_material", "od_body_force", "od_matrix_create",
+    "od_matrix_destroy", "od_matrix_values", "od_matrix_upload", "od_matrix_download", "od_matrix_zero",
+    "od_matrix_axpby", "od_mass_matrix", "od_assemble", "od_spmv", "od_matrix_sync_f32", "od_spmv_f32",
+    "od_solver_create", "od_solver_destroy", "od_pcg_solve", "od_precond_apply", "od_ic0_factor",
+    "od_newmark_create", "od_backward_euler_create", "od_integrator_destroy", "od_integrator_set_external",
+    "od_integrator_set_impulse", "od_integrator_set_active", "od_integrator_step", "od_integrator_get_state",
+    "od_integrator_set_state", "od_integrator_state_dev", "od_integrator_stats", "od_backward_euler_update_mesh",
+    "od_integrator_positions",
+]
+
+
+class ODError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"od error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ODError(-4, f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C opendeformer_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, dp, i, d, ll = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int, C.c_double, C.c_longlong
+        pvp = C.POINTER(C.c_void_p)
+        L.od_last_error.restype = C.c_char_p
+        sig = {
+            "od_ctx_create": [i, pvp], "od_ctx_destroy": [vp], "od_ctx_sync": [vp],
+            "od_dev_alloc": [vp, ll, pvp], "od_dev_free": [vp, vp], "od_copy_h2d": [vp, vp, vp, ll],
+            "od_copy_d2h": [vp, vp, vp, ll], "od_memset_d": [vp, vp, i, ll],
+            "od_mesh_create": [vp, i, dp, i, ip, i, ip, i, ip, ip, pvp], "od_mesh_destroy": [vp],
+            "od_mesh_info": [vp, C.POINTER(ll)], "od_pattern_export": [vp, ip, ip, ip, ip], "od_mesh_rows": [vp, ip, ip],
+            "od_mesh_set_material": [vp, ip, dp, dp, dp, d, d], "od_body_force": [vp, dp, vp],
+            "od_matrix_create": [vp, pvp], "od_matrix_destroy": [vp], "od_matrix_values": [vp, pvp],
+            "od_matrix_upload": [vp, dp], "od_matrix_download": [vp, dp], "od_matrix_zero": [vp],
+            "od_matrix_axpby": [vp, d, d, vp], "od_mass_matrix": [vp, vp], "od_assemble": [vp, vp, vp, vp],
+            "od_spmv": [vp, vp, vp, i], "od_matrix_sync_f32": [vp], "od_spmv_f32": [vp, vp, vp, i],
+            "od_solver_create": [vp, i, d, i, pvp], "od_solver_destroy": [vp],
+            "od_pcg_solve": [vp, vp, vp, vp, d, i, ip, dp], "od_precond_apply": [vp, vp, vp, vp],
+            "od_ic0_factor": [vp, vp, dp],
+            "od_newmark_create": [vp, d, d, d, d, d, i, d, d, i, i, i, pvp],
+            "od_backward_euler_create": [vp, d, d, d, i, d, d, i, pvp], "od_integrator_destroy": [vp],
+            "od_integrator_set_external": [vp, dp], "od_integrator_set_impulse": [vp, dp],
+            "od_integrator_set_active": [vp, C.POINTER(C.c_ubyte)], "od_integrator_step": [vp, i],
+            "od_integrator_get_state": [vp, dp, dp, dp], "od_integrator_set_state": [vp, dp, dp, dp],
+            "od_integrator_state_dev": [vp, pvp, pvp, pvp], "od_integrator_stats": [vp, ip, dp, ip, ip],
+            "od_backward_euler_update_mesh": [vp], "od_integrator_positions": [vp, C.POINTER(C.c_float)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def check(code):
+    if code != 0:
+        raise ODError(code, lib().od_last_error().decode())
+    return code
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

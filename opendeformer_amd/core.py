@@ -1,0 +1,300 @@
+"""Python host mirror of the reference's interfaces over the C-ABI.
+
+Class names follow the reference (OpenDeformer/): the Newmark and
+backward-Euler integrators stand behind the `Intergrator` interface
+(core/intergrator.h:13-35), `CGSolver` behind `LinearSolver`
+(core/linearSolver.h:12-20, solver/cgSolver.cpp:13-57), the material call
+behind `FullOrderNonlinearMaterial::generateMatrixAndVirtualWorks`
+(material/fullOrderNonlinear.h:36-44), and `spmdv` is `SpMDV`
+(core/numerMethod.h:134-262).  Compute happens in libodhip.so only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check, dptr, f64, i32, lib
+
+
+class Context:
+    """One HIP device and stream."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().od_ctx_create(device, C.byref(self.h)))
+
+    def sync(self):
+        check(lib().od_ctx_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().od_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device buffer owned by the context (float64/float32/int32)."""
+
+    def __init__(self, ctx: Context, n: int, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self.ptr = C.c_void_p()
+        check(lib().od_dev_alloc(ctx.h, self.n * self.dtype.itemsize, C.byref(self.ptr)))
+
+    @classmethod
+    def from_host(cls, ctx, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(ctx, a.size, dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().od_copy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        check(lib().od_copy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().od_memset_d(self.ctx.h, self.ptr, 0, self.n * self.dtype.itemsize))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().od_dev_free(self.ctx.h, self.ptr)
+                self.ptr = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Mesh:
+    """Device-resident mesh + NodeIndexer + canonical pattern (+ instances)."""
+
+    def __init__(self, ctx: Context, X, tets, constrained_dofs, inst_node_start=None, inst_tet_start=None):
+        self.ctx = ctx
+        X = f64(X).reshape(-1, 3)
+        tets = i32(tets).reshape(-1, 4)
+        con = i32(constrained_dofs)
+        self.X, self.tets = X, tets
+        n_inst = 1 if inst_node_start is None else len(inst_node_start) - 1
+        ins = i32(inst_node_start) if n_inst > 1 else None
+        its = i32(inst_tet_start) if n_inst > 1 else None
+        self.h = C.c_void_p()
+        check(lib().od_mesh_create(ctx.h, X.shape[0], dptr(X), tets.shape[0], capi.iptr(tets), con.size,
+                                   capi.iptr(con) if con.size else None, n_inst,
+                                   capi.iptr(ins) if ins is not None else None,
+                                   capi.iptr(its) if its is not None else None, C.byref(self.h)))
+        info = np.zeros(16, np.int64)
+        check(lib().od_mesh_info(self.h, info.ctypes.data_as(C.POINTER(C.c_longlong))))
+        (self.ndof, self.n_rows, self.n_blocks, self.n_values, self.n_levels, self.n_inst, self.n_tets,
+         self.n_nodes, self.n_chunks, self.n_pair_contrib, self.n_node_contrib, self.device_bytes) = (int(v) for v in info[:12])
+
+    def close(self):
+        if self.h:
+            lib().od_mesh_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pattern(self, with_map=True):
+        pcol = np.zeros(self.n_rows + 1, np.int32); oris = np.zeros(self.n_rows + 1, np.int32)
+        rows = np.zeros(self.n_blocks, np.int32)
+        emap = np.zeros(78 * self.n_tets, np.int32) if with_map else None
+        check(lib().od_pattern_export(self.h, capi.iptr(pcol), capi.iptr(oris), capi.iptr(rows),
+                                      capi.iptr(emap) if with_map else None))
+        return pcol, oris, rows, (emap.reshape(-1, 78) if with_map else None)
+
+    def rows(self):
+        irs = np.zeros(self.n_inst + 1, np.int32); lv = np.zeros(self.n_rows, np.int32)
+        check(lib().od_mesh_rows(self.h, capi.iptr(irs), capi.iptr(lv)))
+        return irs, lv
+
+    def set_material(self, kind, E, nu, rho, corot_threshold=0.0, neo_threshold=0.1):
+        n = self.n_tets
+        kind = i32(np.broadcast_to(kind, (n,))); E = f64(np.broadcast_to(E, (n,)))
+        nu = f64(np.broadcast_to(nu, (n,))); rho = f64(np.broadcast_to(rho, (n,)))
+        check(lib().od_mesh_set_material(self.h, capi.iptr(kind), dptr(E), dptr(nu), dptr(rho), corot_threshold,
+                                         neo_threshold))
+
+    def body_force(self, body):
+        body = f64(np.broadcast_to(np.asarray(body, np.float64).reshape(-1, 3), (self.n_tets, 3)))
+        f = DeviceArray(self.ctx, self.ndof)
+        check(lib().od_body_force(self.h, dptr(body), f.ptr))
+        return f.download()
+
+
+class Matrix:
+    """Values of a BlockedSymSparseMatrix<3,3> in the canonical layout, on the device."""
+
+    def __init__(self, mesh: Mesh):
+        self.mesh = mesh
+        self.h = C.c_void_p()
+        check(lib().od_matrix_create(mesh.h, C.byref(self.h)))
+        p = C.c_void_p()
+        check(lib().od_matrix_values(self.h, C.byref(p)))
+        self.values_ptr = p
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().od_matrix_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+    def upload(self, values):
+        v = f64(values)
+        assert v.size == self.mesh.n_values
+        check(lib().od_matrix_upload(self.h, dptr(v)))
+
+    def download(self):
+        v = np.zeros(self.mesh.n_values)
+        check(lib().od_matrix_download(self.h, dptr(v)))
+        return v
+
+    def set_zeros(self):
+        check(lib().od_matrix_zero(self.h))
+
+    def scale_add(self, a, b=0.0, other=None):
+        check(lib().od_matrix_axpby(self.h, a, b, other.h if other is not None else None))
+
+    def add_mass(self):
+        check(lib().od_mass_matrix(self.mesh.h, self.h))
+
+    def sync_f32(self):
+        check(lib().od_matrix_sync_f32(self.h))
+
+
+def generate_matrix_and_virtual_works(mesh: Mesh, u_free, K: Matrix, fint: DeviceArray):
+    """FullOrderNonlinearMaterial::generateMatrixAndVirtualWorks: K += K(u), fint += f(u)."""
+    u = u_free if isinstance(u_free, DeviceArray) else DeviceArray.from_host(mesh.ctx, u_free)
+    check(lib().od_assemble(mesh.h, u.ptr, K.h, fint.ptr if fint is not None else None))
+
+
+def spmdv(A: Matrix, x: DeviceArray, y: DeviceArray, accumulate=True):
+    """SpMDV: y += A x (accumulate) or y = A x."""
+    check(lib().od_spmv(A.h, x.ptr, y.ptr, 1 if accumulate else 0))
+
+
+def spmdv_f32(A: Matrix, x: DeviceArray, y: DeviceArray, accumulate=True):
+    check(lib().od_spmv_f32(A.h, x.ptr, y.ptr, 1 if accumulate else 0))
+
+
+class CGSolver:
+    """PCG with a per-solve preconditioner rebuild (cgSolver.cpp:13-57)."""
+
+    def __init__(self, mesh: Mesh, tol=1e-6, max_iter=10000, precond=capi.OD_PRE_JACOBI, precond_param=1e-3,
+                 precision=capi.OD_FP64):
+        self.mesh, self.tol, self.max_iter = mesh, tol, max_iter
+        self.h = C.c_void_p()
+        check(lib().od_solver_create(mesh.h, precond, precond_param, precision, C.byref(self.h)))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().od_solver_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+    def solve(self, A: Matrix, b: DeviceArray, x: DeviceArray):
+        it = np.zeros(self.mesh.n_inst, np.int32); rr = np.zeros(self.mesh.n_inst)
+        check(lib().od_pcg_solve(self.h, A.h, b.ptr, x.ptr, self.tol, self.max_iter, capi.iptr(it), dptr(rr)))
+        return it, rr
+
+    def precond_apply(self, A: Matrix, r: DeviceArray, z: DeviceArray):
+        check(lib().od_precond_apply(self.h, A.h, r.ptr, z.ptr))
+
+    def ic0_factor(self, A: Matrix):
+        L = np.zeros(self.mesh.n_values)
+        check(lib().od_ic0_factor(self.h, A.h, dptr(L)))
+        return L
+
+
+class _Integrator:
+    def __del__(self):
+        try:
+            if self.h:
+                lib().od_integrator_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+    def set_external_virtual_work(self, f):
+        check(lib().od_integrator_set_external(self.h, dptr(f64(f))))
+
+    def set_impulse(self, f):
+        check(lib().od_integrator_set_impulse(self.h, dptr(f64(f))))
+
+    def set_active(self, mask):
+        if mask is None:
+            check(lib().od_integrator_set_active(self.h, None))
+        else:
+            m = np.ascontiguousarray(mask, dtype=np.uint8)
+            check(lib().od_integrator_set_active(self.h, m.ctypes.data_as(C.POINTER(C.c_ubyte))))
+
+    def run_one_time_step(self):
+        check(lib().od_integrator_step(self.h, 1))
+
+    def run_time_steps(self, n):
+        check(lib().od_integrator_step(self.h, int(n)))
+
+    def state(self):
+        n = self.mesh.ndof
+        d, v, a = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(lib().od_integrator_get_state(self.h, dptr(d), dptr(v), dptr(a)))
+        return d, v, a
+
+    def set_state(self, d=None, v=None, a=None):
+        args = [dptr(f64(x)) if x is not None else None for x in (d, v, a)]
+        check(lib().od_integrator_set_state(self.h, *args))
+
+    def stats(self):
+        it = np.zeros(self.mesh.n_inst, np.int32); rr = np.zeros(self.mesh.n_inst)
+        outer = C.c_int(0); init = C.c_int(0)
+        check(lib().od_integrator_stats(self.h, capi.iptr(it), dptr(rr), C.byref(outer), C.byref(init)))
+        return it, rr, outer.value, init.value
+
+    def vertex_positions(self):
+        pos = np.zeros(3 * self.mesh.n_nodes, np.float32)
+        check(lib().od_integrator_positions(self.h, pos.ctypes.data_as(C.POINTER(C.c_float))))
+        return pos.reshape(-1, 3)
+
+
+class NewmarkIntegrator(_Integrator):
+    """Full-order linearised Newmark (linearNewmark.cpp:27-30, 57-79 at full order)."""
+
+    def __init__(self, mesh: Mesh, dt, beta=0.25, gamma=0.5, alpha_m=0.0, alpha_k=0.0, precond=capi.OD_PRE_IC0,
+                 precond_param=1e-3, tol=1e-6, max_iter=10000, precision=capi.OD_FP64, init_accel=True):
+        self.mesh = mesh
+        self.h = C.c_void_p()
+        check(lib().od_newmark_create(mesh.h, beta, gamma, dt, alpha_m, alpha_k, precond, precond_param, tol, max_iter,
+                                      precision, 1 if init_accel else 0, C.byref(self.h)))
+
+
+class BackwardEulerIntegrator(_Integrator):
+    """NonlinearImplicitBackwardEuler (intergrator/nonlinearImplicitBackwardEuler.h:48-146)."""
+
+    def __init__(self, mesh: Mesh, dt, alpha_m=0.0, alpha_k=0.0, precond=capi.OD_PRE_JACOBI, precond_param=1e-3,
+                 tol=1e-6, max_iter=10000):
+        self.mesh = mesh
+        self.h = C.c_void_p()
+        check(lib().od_backward_euler_create(mesh.h, dt, alpha_m, alpha_k, precond, precond_param, tol, max_iter,
+                                             C.byref(self.h)))
+
+    def update_mesh_vertices_displacements(self):
+        check(lib().od_backward_euler_update_mesh(self.h))

@@ -1,0 +1,933 @@
+// C-ABI of libodhip.so (declared in include/od.h): handles, device memory,
+// PCG driver (FP64 and FP32-inner / FP64-refinement) and the integrators.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/od.h"
+#include "od_internal.h"
+
+using namespace od;
+
+namespace {
+thread_local std::string g_err;
+
+struct ApiError : std::runtime_error {
+    int code;
+    ApiError(int c, const std::string &s) : std::runtime_error(s), code(c) {}
+};
+
+#define API_BEGIN try {
+#define API_END                                                        \
+    }                                                                  \
+    catch (const od::HipError &e) { g_err = e.what(); return OD_ERR_HIP; } \
+    catch (const ApiError &e) { g_err = e.what(); return e.code; }     \
+    catch (const std::exception &e) { g_err = e.what(); return OD_ERR_ARG; } \
+    return OD_OK;
+
+void require(bool c, const char *msg, int code = OD_ERR_ARG) {
+    if (!c) throw ApiError(code, msg);
+}
+
+__global__ void k_done_to_mask(int n, const InstScalars *sc, const uint8_t *extra, uint8_t *mask) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) mask[i] = (sc[i].done || (extra && extra[i])) ? 1 : 0;
+}
+}  // namespace
+
+struct od_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int *pinned = nullptr;   // small host-pinned mailbox for convergence polling
+};
+
+struct od_mesh {
+    od_ctx *ctx = nullptr;
+    Topology topo;
+    DevMesh dm{};
+    std::vector<void *> allocs;
+    size_t bytes = 0;
+    bool hasMaterial = false;
+    double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
+    template <typename T>
+    T *upload(const std::vector<T> &v) {
+        if (v.empty()) return nullptr;
+        void *p = nullptr;
+        OD_HIP(hipMalloc(&p, v.size() * sizeof(T)));
+        allocs.push_back(p);
+        bytes += v.size() * sizeof(T);
+        OD_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+        return (T *)p;
+    }
+    void *alloc(size_t b) {
+        void *p = nullptr;
+        OD_HIP(hipMalloc(&p, std::max<size_t>(b, 16)));
+        allocs.push_back(p);
+        bytes += b;
+        return p;
+    }
+    void ensureContrib() {
+        if (!blkC) blkC = (double *)alloc(sizeof(double) * 9 * std::max<int64_t>(1, topo.nPairContrib));
+        if (!nodeC) nodeC = (double *)alloc(sizeof(double) * 3 * std::max<int64_t>(1, topo.nNodeContrib));
+    }
+    ~od_mesh() {
+        for (void *p : allocs) (void)hipFree(p);
+    }
+};
+
+struct od_matrix {
+    od_mesh *mesh = nullptr;
+    double *vals = nullptr;
+    float *vals32 = nullptr;
+    ~od_matrix() {
+        if (vals) (void)hipFree(vals);
+        if (vals32) (void)hipFree(vals32);
+    }
+};
+
+struct od_solver {
+    od_mesh *mesh = nullptr;
+    int precond = OD_PRE_JACOBI;
+    double param = 1e-3;        // IC(0) pivot epsilon
+    int precision = OD_FP64;
+    double innerTol = 1e-3;     // mixed: inner FP32 relative tolerance
+    int maxOuter = 12;
+    std::vector<void *> allocs;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *invDiag = nullptr, *U = nullptr;
+    float *r32 = nullptr, *ri32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+    float *invDiag32 = nullptr, *U32 = nullptr;
+    double *Ax = nullptr;
+    double *partials = nullptr;
+    int *counter = nullptr, *notDoneIn = nullptr, *notDoneOut = nullptr;
+    InstScalars *scIn = nullptr, *scOut = nullptr;
+    uint8_t *mask = nullptr, *zeroMask = nullptr;
+    std::vector<int> iters;
+    std::vector<double> relres;
+    int outer = 0;
+    void *alloc(size_t b) {
+        void *ptr = nullptr;
+        OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
+        allocs.push_back(ptr);
+        return ptr;
+    }
+    ~od_solver() {
+        for (void *ptr : allocs) (void)hipFree(ptr);
+    }
+};
+
+struct od_integrator {
+    od_mesh *mesh = nullptr;
+    int kind = 0;  // 0 Newmark, 1 backward Euler
+    double beta = 0.25, gamma = 0.5, h = 1.0 / 60, alphaM = 0, alphaK = 0, tol = 1e-6;
+    int maxIter = 10000;
+    bool initAccel = true;
+    int steps = 0;
+    od_solver *solver = nullptr;
+    od_matrix M, A;
+    double *d = nullptr, *v = nullptr, *a = nullptr, *aNew = nullptr, *fext = nullptr, *imp = nullptr, *rhs = nullptr;
+    uint8_t *active = nullptr, *inactive = nullptr;
+    bool hasImpulse = false, hasActive = false;
+    std::vector<void *> allocs;
+    std::vector<int> lastIters;
+    std::vector<double> lastRelres;
+    int lastOuter = 0, initIters = 0;
+    void *alloc(size_t b) {
+        void *ptr = nullptr;
+        OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
+        OD_HIP(hipMemset(ptr, 0, std::max<size_t>(b, 16)));
+        allocs.push_back(ptr);
+        return ptr;
+    }
+    ~od_integrator() {
+        M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr;   // owned by allocs
+        for (void *ptr : allocs) (void)hipFree(ptr);
+        delete solver;
+    }
+};
+
+// =====================================================================================
+// solver internals
+// =====================================================================================
+static od_solver *make_solver(od_mesh *mesh, int precond, double param, int precision) {
+    require(precond == OD_PRE_NONE || precond == OD_PRE_JACOBI || precond == OD_PRE_IC0, "unknown preconditioner",
+            OD_ERR_UNSUPPORTED);
+    require(precision == OD_FP64 || precision == OD_MIXED, "unknown precision", OD_ERR_UNSUPPORTED);
+    od_solver *s = new od_solver();
+    try {
+        s->mesh = mesh;
+        s->precond = precond;
+        s->param = param > 0 ? param : 1e-3;
+        s->precision = precision;
+        const Topology &t = mesh->topo;
+        size_t n = (size_t)t.ndof, nv = (size_t)t.nValues;
+        s->r = (double *)s->alloc(8 * n); s->z = (double *)s->alloc(8 * n);
+        s->p = (double *)s->alloc(8 * n); s->q = (double *)s->alloc(8 * n);
+        s->Ax = (double *)s->alloc(8 * n);
+        if (precond == OD_PRE_JACOBI) s->invDiag = (double *)s->alloc(8 * n);
+        if (precond == OD_PRE_IC0) s->U = (double *)s->alloc(8 * nv);
+        if (precision == OD_MIXED) {
+            s->r32 = (float *)s->alloc(4 * n); s->ri32 = (float *)s->alloc(4 * n); s->z32 = (float *)s->alloc(4 * n);
+            s->p32 = (float *)s->alloc(4 * n); s->q32 = (float *)s->alloc(4 * n);
+            s->x32 = (float *)s->alloc(4 * n);
+            if (precond == OD_PRE_JACOBI) s->invDiag32 = (float *)s->alloc(4 * n);
+            if (precond == OD_PRE_IC0) s->U32 = (float *)s->alloc(4 * nv);
+        }
+        s->partials = (double *)s->alloc(sizeof(double) * 4 * (size_t)t.chunkRow0.size());
+        s->counter = (int *)s->alloc(sizeof(int) * 4);
+        OD_HIP(hipMemset(s->counter, 0, sizeof(int) * 4));
+        s->notDoneIn = s->counter + 1;
+        s->notDoneOut = s->counter + 2;
+        s->scIn = (InstScalars *)s->alloc(sizeof(InstScalars) * t.nInst);
+        s->scOut = (InstScalars *)s->alloc(sizeof(InstScalars) * t.nInst);
+        OD_HIP(hipMemset(s->scIn, 0, sizeof(InstScalars) * t.nInst));
+        OD_HIP(hipMemset(s->scOut, 0, sizeof(InstScalars) * t.nInst));
+        s->mask = (uint8_t *)s->alloc(t.nInst);
+        s->zeroMask = (uint8_t *)s->alloc(t.nInst);
+        OD_HIP(hipMemset(s->zeroMask, 0, t.nInst));
+        s->iters.assign(t.nInst, 0);
+        s->relres.assign(t.nInst, 0.0);
+    } catch (...) {
+        delete s;
+        throw;
+    }
+    return s;
+}
+
+static int read_flag(od_ctx *ctx, const int *dev) {
+    OD_HIP(hipMemcpyAsync(ctx->pinned, dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    OD_HIP(hipStreamSynchronize(ctx->stream));
+    return *ctx->pinned;
+}
+
+// Builds the preconditioner from A (Preconditioner::Preprocess, cgSolver.cpp:23).
+static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip) {
+    od_mesh *mesh = s->mesh;
+    const DevMesh &m = mesh->dm;
+    hipStream_t st = mesh->ctx->stream;
+    if (s->precond == OD_PRE_JACOBI) {
+        launch_jacobi_diag<double>(m, st, A->vals, s->invDiag);
+        if (s->invDiag32) launch_jacobi_diag<float>(m, st, A->vals, s->invDiag32);
+    } else if (s->precond == OD_PRE_IC0) {
+        if (s->U32) launch_ic0_factor<float>(m, st, mesh->topo.levelPtr, A->vals, s->U, s->U32, s->param, skip);
+        else launch_ic0_factor<double>(m, st, mesh->topo.levelPtr, A->vals, s->U, (double *)nullptr, s->param, skip);
+    }
+}
+
+// PCG on the chunked instances: x in/out, b fixed.  preDone marks instances to skip.
+template <typename T>
+static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *invDiag, const T *b, T *x, T *r, T *z, T *p, T *q,
+                    double tol, int maxIt, const uint8_t *preDone, InstScalars *sc, int *notDone, bool xZero) {
+    od_mesh *mesh = s->mesh;
+    const DevMesh &m = mesh->dm;
+    hipStream_t st = mesh->ctx->stream;
+    PcgWork w{s->partials, sc, s->counter, notDone};
+    const size_t n = (size_t)mesh->topo.ndof;
+    if (xZero) OD_HIP(hipMemsetAsync(q, 0, sizeof(T) * n, st));
+    else launch_spmv<T>(m, st, vals, x, q, 0, nullptr, nullptr);
+    pcg_init_residual<T>(m, st, b, q, r, w);
+    const bool ic = s->precond == OD_PRE_IC0;
+    if (ic) launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, r, z, nullptr, preDone);
+    else pcg_jacobi<T>(m, st, invDiag, r, z, w);
+    pcg_dot_rz_init<T>(m, st, r, z, p, w, tol, maxIt, preDone);
+    int launched = 0, batch = 4;
+    while (launched < maxIt) {
+        if (read_flag(mesh->ctx, notDone) == 0) break;
+        const int k = std::min(batch, maxIt - launched);
+        for (int j = 0; j < k; j++) {
+            pcg_alpha<T>(m, st, vals, p, q, w);
+            if (ic) {
+                pcg_update_xr<T>(m, st, p, q, x, r, (const T *)nullptr, (T *)nullptr, w);
+                launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, r, z, sc, preDone);
+                pcg_dot_rz<T>(m, st, r, z, w);
+            } else {
+                pcg_update_xr<T>(m, st, p, q, x, r, invDiag, z, w);
+            }
+            pcg_update_p<T>(m, st, z, p, w);
+        }
+        launched += k;
+        batch = std::min(batch * 2, 64);
+    }
+}
+
+static void collect_stats(od_solver *s, const InstScalars *scDev, bool accumulate) {
+    int nInst = s->mesh->topo.nInst;
+    std::vector<InstScalars> h(nInst);
+    OD_HIP(hipMemcpyAsync(h.data(), scDev, sizeof(InstScalars) * nInst, hipMemcpyDeviceToHost, s->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
+    for (int i = 0; i < nInst; i++) {
+        if (accumulate) s->iters[i] += h[i].it;
+        else s->iters[i] = h[i].it;
+    }
+}
+
+static void collect_relres(od_solver *s, const InstScalars *scDev) {
+    int nInst = s->mesh->topo.nInst;
+    std::vector<InstScalars> h(nInst);
+    OD_HIP(hipMemcpyAsync(h.data(), scDev, sizeof(InstScalars) * nInst, hipMemcpyDeviceToHost, s->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
+    for (int i = 0; i < nInst; i++) s->relres[i] = h[i].bb > 0 ? std::sqrt(h[i].rr / h[i].bb) : std::sqrt(h[i].rr);
+}
+
+// Full solve A x = b (x warm start).  skip: per-instance mask of instances left untouched.
+static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double tol, int maxIt, const uint8_t *skip) {
+    od_mesh *mesh = s->mesh;
+    const DevMesh &m = mesh->dm;
+    hipStream_t st = mesh->ctx->stream;
+    const Topology &t = mesh->topo;
+    const uint8_t *pre = skip ? skip : s->zeroMask;
+    if (s->precision == OD_FP64) {
+        build_precond(s, A, pre);
+        run_pcg<double>(s, A->vals, s->U, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre, s->scIn,
+                        s->notDoneIn, false);
+        collect_stats(s, s->scIn, false);
+        collect_relres(s, s->scIn);
+        s->outer = 0;
+        return;
+    }
+    // ---- mixed: FP32 inner PCG, FP64 residual and correction (iterative refinement)
+    require(A->vals32 != nullptr, "mixed precision needs the FP32 copy of A (od_matrix_sync_f32)", OD_ERR_STATE);
+    build_precond(s, A, pre);
+    std::fill(s->iters.begin(), s->iters.end(), 0);
+    PcgWork wo{s->partials, s->scOut, s->counter, s->notDoneOut};
+    launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr, nullptr);
+    launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 1);
+    int outer = 0;
+    for (; outer < s->maxOuter; outer++) {
+        if (read_flag(mesh->ctx, s->notDoneOut) == 0) break;
+        hipLaunchKernelGGL(k_done_to_mask, dim3((t.nInst + 255) / 256), dim3(256), 0, st, t.nInst, s->scOut, pre, s->mask);
+        OD_HIP(hipGetLastError());
+        OD_HIP(hipMemsetAsync(s->x32, 0, sizeof(float) * t.ndof, st));
+        run_pcg<float>(s, A->vals32, s->U32, s->invDiag32, s->r32, s->x32, s->ri32, s->z32, s->p32, s->q32, s->innerTol,
+                       maxIt, s->mask, s->scIn, s->notDoneIn, true);
+        collect_stats(s, s->scIn, true);
+        launch_add_correction(m, st, s->x32, x, s->scOut);
+        launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr, nullptr);
+        launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 0);
+    }
+    s->outer = outer;
+    collect_relres(s, s->scOut);
+}
+
+// =====================================================================================
+// C-ABI
+// =====================================================================================
+extern "C" {
+
+const char *od_last_error(void) { return g_err.c_str(); }
+int od_version(void) { return 1; }
+
+int od_ctx_create(int device, od_ctx **out) {
+    API_BEGIN
+    require(out != nullptr, "out is NULL");
+    int count = 0;
+    OD_HIP(hipGetDeviceCount(&count));
+    require(device >= 0 && device < count, "device index out of range");
+    OD_HIP(hipSetDevice(device));
+    od_ctx *c = new od_ctx();
+    c->device = device;
+    OD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    OD_HIP(hipHostMalloc((void **)&c->pinned, 64, hipHostMallocDefault));
+    *out = c;
+    API_END
+}
+
+int od_ctx_destroy(od_ctx *ctx) {
+    API_BEGIN
+    if (!ctx) return OD_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipHostFree(ctx->pinned);
+    delete ctx;
+    API_END
+}
+
+int od_ctx_sync(od_ctx *ctx) {
+    API_BEGIN
+    require(ctx != nullptr, "ctx is NULL");
+    OD_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int od_dev_alloc(od_ctx *ctx, long long bytes, void **dev) {
+    API_BEGIN
+    require(ctx && dev && bytes >= 0, "bad arguments");
+    OD_HIP(hipSetDevice(ctx->device));
+    OD_HIP(hipMalloc(dev, (size_t)std::max<long long>(bytes, 16)));
+    API_END
+}
+int od_dev_free(od_ctx *ctx, void *dev) {
+    API_BEGIN
+    require(ctx != nullptr, "ctx is NULL");
+    if (dev) OD_HIP(hipFree(dev));
+    API_END
+}
+int od_copy_h2d(od_ctx *ctx, void *dev, const void *host, long long bytes) {
+    API_BEGIN
+    require(ctx && dev && host && bytes >= 0, "bad arguments");
+    OD_HIP(hipMemcpyAsync(dev, host, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+    OD_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+int od_copy_d2h(od_ctx *ctx, void *host, const void *dev, long long bytes) {
+    API_BEGIN
+    require(ctx && dev && host && bytes >= 0, "bad arguments");
+    OD_HIP(hipMemcpyAsync(host, dev, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
+    OD_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+int od_memset_d(od_ctx *ctx, void *dev, int byte, long long bytes) {
+    API_BEGIN
+    require(ctx && dev && bytes >= 0, "bad arguments");
+    OD_HIP(hipMemsetAsync(dev, byte, (size_t)bytes, ctx->stream));
+    API_END
+}
+
+int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const int *tets, int nConstrained,
+                   const int *constrained, int nInstances, const int *instNodeStart, const int *instTetStart,
+                   od_mesh **out) {
+    API_BEGIN
+    require(ctx && X && tets && out, "NULL argument");
+    require(nNodes > 0 && nTets > 0 && nConstrained >= 0, "empty mesh");
+    require(nConstrained == 0 || constrained != nullptr, "constrained list is NULL");
+    require(nInstances >= 1, "nInstances must be >= 1");
+    require(nInstances == 1 || (instNodeStart && instTetStart), "instance ranges are NULL");
+    od_mesh *mesh = new od_mesh();
+    try {
+        mesh->ctx = ctx;
+        Topology &t = mesh->topo;
+        t.nNodes = nNodes;
+        t.nTets = nTets;
+        t.X.assign(X, X + 3 * (size_t)nNodes);
+        t.tets.assign(tets, tets + 4 * (size_t)nTets);
+        t.constrained.assign(constrained, constrained + nConstrained);
+        if (nInstances > 1) {
+            t.instNodeStart.assign(instNodeStart, instNodeStart + nInstances + 1);
+            t.instTetStart.assign(instTetStart, instTetStart + nInstances + 1);
+        }
+        std::string err = buildTopology(t, kChunkRows);
+        require(err.empty(), err.c_str());
+        OD_HIP(hipSetDevice(ctx->device));
+        DevMesh &m = mesh->dm;
+        m.nNodes = t.nNodes; m.nTets = t.nTets; m.nRows = t.nRows; m.nInst = t.nInst;
+        m.nChunks = (int)t.chunkRow0.size(); m.nLevels = t.nLevels;
+        m.nBlocks = t.nBlocks; m.nValues = t.nValues;
+        m.X = mesh->upload(t.X);
+        m.tets = mesh->upload(t.tets);
+        m.nodeRow = mesh->upload(t.nodeRow);
+        m.rowPtr = mesh->upload(t.rowPtr);
+        m.colIdx = mesh->upload(t.colIdx);
+        m.blkOff = mesh->upload(t.blkOff);
+        std::vector<int> blkRow(t.nBlocks);
+        for (int r = 0; r < t.nRows; r++)
+            for (int b = t.rowPtr[r]; b < t.rowPtr[r + 1]; b++) blkRow[b] = r;
+        m.blkRow = mesh->upload(blkRow);
+        m.tPtr = mesh->upload(t.tPtr);
+        m.tRow = mesh->upload(t.tRow);
+        m.tBlk = mesh->upload(t.tBlk);
+        m.pairBlk = mesh->upload(t.pairBlk);
+        m.pairSlot = mesh->upload(t.pairSlot);
+        m.pairT = mesh->upload(t.pairT);
+        m.blkCStart = mesh->upload(t.blkCStart);
+        m.nodeSlot = mesh->upload(t.nodeSlot);
+        m.rowCStart = mesh->upload(t.rowCStart);
+        m.levelRows = mesh->upload(t.levelRows);
+        m.chunkRow0 = mesh->upload(t.chunkRow0);
+        m.chunkRow1 = mesh->upload(t.chunkRow1);
+        m.chunkInst = mesh->upload(t.chunkInst);
+        m.instChunkStart = mesh->upload(t.instChunkStart);
+        m.instRowStart = mesh->upload(t.instRowStart);
+        m.tetInst = mesh->upload(t.tetInst);
+        m.rowInst = mesh->upload(t.rowInst);
+        m.kind = (int *)mesh->alloc(sizeof(int) * nTets);
+        m.E = (double *)mesh->alloc(sizeof(double) * nTets);
+        m.nu = (double *)mesh->alloc(sizeof(double) * nTets);
+        m.rho = (double *)mesh->alloc(sizeof(double) * nTets);
+        m.thrCorot = 0.0;
+        m.thrNeo = 0.1;
+        OD_HIP(hipStreamSynchronize(ctx->stream));
+        // host copies no longer needed on the per-step path
+        t.elemMap78.clear(); t.elemMap78.shrink_to_fit();
+        t.pairSlot.clear(); t.pairSlot.shrink_to_fit();
+        t.nodeSlot.clear(); t.nodeSlot.shrink_to_fit();
+    } catch (...) {
+        delete mesh;
+        throw;
+    }
+    *out = mesh;
+    API_END
+}
+
+int od_mesh_destroy(od_mesh *mesh) {
+    API_BEGIN
+    if (mesh) {
+        (void)hipStreamSynchronize(mesh->ctx->stream);
+        delete mesh;
+    }
+    API_END
+}
+
+int od_mesh_info(od_mesh *mesh, long long *info) {
+    API_BEGIN
+    require(mesh && info, "NULL argument");
+    const Topology &t = mesh->topo;
+    for (int i = 0; i < 16; i++) info[i] = 0;
+    info[0] = t.ndof; info[1] = t.nRows; info[2] = t.nBlocks; info[3] = t.nValues; info[4] = t.nLevels;
+    info[5] = t.nInst; info[6] = t.nTets; info[7] = t.nNodes; info[8] = (long long)t.chunkRow0.size();
+    info[9] = t.nPairContrib; info[10] = t.nNodeContrib; info[11] = (long long)mesh->bytes;
+    API_END
+}
+
+int od_pattern_export(od_mesh *mesh, int *blockPcol, int *blockColumnOris, int *blockRows, int *elemMap78) {
+    API_BEGIN
+    require(mesh != nullptr, "mesh is NULL");
+    const Topology &t = mesh->topo;
+    if (blockPcol) std::memcpy(blockPcol, t.rowPtr.data(), sizeof(int) * (t.nRows + 1));
+    if (blockColumnOris) std::memcpy(blockColumnOris, t.valOff.data(), sizeof(int) * (t.nRows + 1));
+    if (blockRows)
+        for (int64_t b = 0; b < t.nBlocks; b++) blockRows[b] = 3 * t.colIdx[b];
+    if (elemMap78) {
+        int *dmap = nullptr;
+        OD_HIP(hipMalloc(&dmap, sizeof(int) * 78 * (size_t)t.nTets));
+        try {
+            launch_expand_map(mesh->dm, mesh->ctx->stream, dmap);
+            OD_HIP(hipMemcpyAsync(elemMap78, dmap, sizeof(int) * 78 * (size_t)t.nTets, hipMemcpyDeviceToHost, mesh->ctx->stream));
+            OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
+        } catch (...) {
+            (void)hipFree(dmap);
+            throw;
+        }
+        OD_HIP(hipFree(dmap));
+    }
+    API_END
+}
+
+int od_mesh_rows(od_mesh *mesh, int *instRowStart, int *level) {
+    API_BEGIN
+    require(mesh != nullptr, "mesh is NULL");
+    const Topology &t = mesh->topo;
+    if (instRowStart) std::memcpy(instRowStart, t.instRowStart.data(), sizeof(int) * (t.nInst + 1));
+    if (level) std::memcpy(level, t.level.data(), sizeof(int) * t.nRows);
+    API_END
+}
+
+int od_mesh_set_material(od_mesh *mesh, const int *kind, const double *E, const double *nu, const double *rho,
+                         double corotThreshold, double neoThreshold) {
+    API_BEGIN
+    require(mesh && kind && E && nu && rho, "NULL argument");
+    const int n = mesh->topo.nTets;
+    for (int i = 0; i < n; i++) {
+        require(kind[i] == OD_MAT_COROTATIONAL || kind[i] == OD_MAT_NEOHOOKEAN, "unknown material kind", OD_ERR_UNSUPPORTED);
+        require(E[i] > 0 && nu[i] > -1.0 && nu[i] < 0.5 && rho[i] >= 0, "material parameters out of range");
+    }
+    hipStream_t st = mesh->ctx->stream;
+    OD_HIP(hipMemcpyAsync(mesh->dm.kind, kind, sizeof(int) * n, hipMemcpyHostToDevice, st));
+    OD_HIP(hipMemcpyAsync(mesh->dm.E, E, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    OD_HIP(hipMemcpyAsync(mesh->dm.nu, nu, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    OD_HIP(hipMemcpyAsync(mesh->dm.rho, rho, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    OD_HIP(hipStreamSynchronize(st));
+    mesh->dm.thrCorot = corotThreshold;
+    mesh->dm.thrNeo = neoThreshold;
+    mesh->hasMaterial = true;
+    API_END
+}
+
+int od_body_force(od_mesh *mesh, const double *body, double *f_dev) {
+    API_BEGIN
+    require(mesh && body && f_dev, "NULL argument");
+    mesh->ensureContrib();
+    double *db = nullptr;
+    size_t bytes = sizeof(double) * 3 * (size_t)mesh->topo.nTets;
+    OD_HIP(hipMalloc(&db, bytes));
+    OD_HIP(hipMemcpyAsync(db, body, bytes, hipMemcpyHostToDevice, mesh->ctx->stream));
+    launch_body_force(mesh->dm, mesh->ctx->stream, db, mesh->nodeC, f_dev);
+    OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
+    OD_HIP(hipFree(db));
+    API_END
+}
+
+int od_matrix_create(od_mesh *mesh, od_matrix **out) {
+    API_BEGIN
+    require(mesh && out, "NULL argument");
+    od_matrix *m = new od_matrix();
+    m->mesh = mesh;
+    size_t b = sizeof(double) * (size_t)mesh->topo.nValues;
+    if (hipMalloc(&m->vals, b) != hipSuccess) { delete m; throw ApiError(OD_ERR_HIP, "hipMalloc failed for matrix values"); }
+    OD_HIP(hipMemsetAsync(m->vals, 0, b, mesh->ctx->stream));
+    *out = m;
+    API_END
+}
+int od_matrix_destroy(od_matrix *m) {
+    API_BEGIN
+    if (m) { (void)hipStreamSynchronize(m->mesh->ctx->stream); delete m; }
+    API_END
+}
+int od_matrix_values(od_matrix *m, double **values) {
+    API_BEGIN
+    require(m && values, "NULL argument");
+    *values = m->vals;
+    API_END
+}
+int od_matrix_upload(od_matrix *m, const double *host) {
+    API_BEGIN
+    require(m && host, "NULL argument");
+    OD_HIP(hipMemcpyAsync(m->vals, host, sizeof(double) * m->mesh->topo.nValues, hipMemcpyHostToDevice, m->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(m->mesh->ctx->stream));
+    API_END
+}
+int od_matrix_download(od_matrix *m, double *host) {
+    API_BEGIN
+    require(m && host, "NULL argument");
+    OD_HIP(hipMemcpyAsync(host, m->vals, sizeof(double) * m->mesh->topo.nValues, hipMemcpyDeviceToHost, m->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(m->mesh->ctx->stream));
+    API_END
+}
+int od_matrix_zero(od_matrix *m) {
+    API_BEGIN
+    require(m != nullptr, "NULL argument");
+    OD_HIP(hipMemsetAsync(m->vals, 0, sizeof(double) * m->mesh->topo.nValues, m->mesh->ctx->stream));
+    API_END
+}
+
+__global__ static void k_axpby(double *y, double a, double b, const double *x, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * y[i] + (x ? b * x[i] : 0.0);
+}
+int od_matrix_axpby(od_matrix *m, double a, double b, const od_matrix *o) {
+    API_BEGIN
+    require(m != nullptr, "NULL argument");
+    require(!o || o->mesh == m->mesh, "matrices of different meshes");
+    int64_t n = m->mesh->topo.nValues;
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->mesh->ctx->stream, m->vals, a, b,
+                       o ? o->vals : nullptr, n);
+    OD_HIP(hipGetLastError());
+    API_END
+}
+
+int od_mass_matrix(od_mesh *mesh, od_matrix *M) {
+    API_BEGIN
+    require(mesh && M && M->mesh == mesh, "bad arguments");
+    require(mesh->hasMaterial, "od_mesh_set_material must be called first (density)", OD_ERR_STATE);
+    mesh->ensureContrib();
+    ElemParams P{1, 0, 0, 0, 0, 0, 0};
+    launch_element(mesh->dm, mesh->ctx->stream, nullptr, nullptr, nullptr, P, mesh->blkC, mesh->nodeC, nullptr);
+    launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, M->vals, 1, nullptr, nullptr);
+    API_END
+}
+
+int od_assemble(od_mesh *mesh, const double *u_dev, od_matrix *K, double *fint_dev) {
+    API_BEGIN
+    require(mesh && K && K->mesh == mesh, "bad arguments");
+    require(mesh->hasMaterial, "od_mesh_set_material must be called first", OD_ERR_STATE);
+    mesh->ensureContrib();
+    ElemParams P{0, fint_dev ? 1 : 0, 0, 0, 0, 0, 0};
+    launch_element(mesh->dm, mesh->ctx->stream, u_dev, nullptr, nullptr, P, mesh->blkC, mesh->nodeC, nullptr);
+    launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, K->vals, 1, nullptr, nullptr);
+    if (fint_dev) launch_gather_rhs(mesh->dm, mesh->ctx->stream, mesh->nodeC, nullptr, nullptr, 1.0, fint_dev, 1, nullptr);
+    API_END
+}
+
+int od_spmv(od_matrix *A, const double *x, double *y, int accumulate) {
+    API_BEGIN
+    require(A && x && y, "NULL argument");
+    require(x != y, "x and y must differ (SpMDV asserts src != dest)");
+    launch_spmv<double>(A->mesh->dm, A->mesh->ctx->stream, A->vals, x, y, accumulate, nullptr, nullptr);
+    API_END
+}
+
+int od_matrix_sync_f32(od_matrix *A) {
+    API_BEGIN
+    require(A != nullptr, "NULL argument");
+    int64_t n = A->mesh->topo.nValues;
+    if (!A->vals32) OD_HIP(hipMalloc(&A->vals32, sizeof(float) * (size_t)n));
+    launch_cast_f64_f32(A->mesh->ctx->stream, A->vals, A->vals32, n);
+    API_END
+}
+
+int od_spmv_f32(od_matrix *A, const float *x, float *y, int accumulate) {
+    API_BEGIN
+    require(A && x && y && x != y, "bad arguments");
+    require(A->vals32 != nullptr, "call od_matrix_sync_f32 first", OD_ERR_STATE);
+    launch_spmv<float>(A->mesh->dm, A->mesh->ctx->stream, A->vals32, x, y, accumulate, nullptr, nullptr);
+    API_END
+}
+
+int od_solver_create(od_mesh *mesh, int precond, double param, int precision, od_solver **out) {
+    API_BEGIN
+    require(mesh && out, "NULL argument");
+    *out = make_solver(mesh, precond, param, precision);
+    API_END
+}
+int od_solver_destroy(od_solver *s) {
+    API_BEGIN
+    if (s) { (void)hipStreamSynchronize(s->mesh->ctx->stream); delete s; }
+    API_END
+}
+
+int od_pcg_solve(od_solver *s, od_matrix *A, const double *b, double *x, double tol, int maxIter, int *iters,
+                 double *relres) {
+    API_BEGIN
+    require(s && A && b && x && A->mesh == s->mesh, "bad arguments");
+    require(tol >= 0 && maxIter >= 0, "tol and maxIter must be non-negative");
+    if (s->precision == OD_MIXED && !A->vals32) {
+        int64_t n = A->mesh->topo.nValues;
+        OD_HIP(hipMalloc(&A->vals32, sizeof(float) * (size_t)n));
+        launch_cast_f64_f32(A->mesh->ctx->stream, A->vals, A->vals32, n);
+    }
+    solve(s, A, b, x, tol, maxIter, nullptr);
+    OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
+    int nInst = s->mesh->topo.nInst;
+    if (iters) std::memcpy(iters, s->iters.data(), sizeof(int) * nInst);
+    if (relres) std::memcpy(relres, s->relres.data(), sizeof(double) * nInst);
+    API_END
+}
+
+int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
+    API_BEGIN
+    require(s && A && r && z && A->mesh == s->mesh, "bad arguments");
+    od_mesh *mesh = s->mesh;
+    build_precond(s, A, s->zeroMask);
+    PcgWork w{s->partials, s->scIn, s->counter, s->notDoneIn};
+    if (s->precond == OD_PRE_IC0) launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U, r, z, nullptr, nullptr);
+    else pcg_jacobi<double>(mesh->dm, mesh->ctx->stream, s->precond == OD_PRE_JACOBI ? s->invDiag : nullptr, r, z, w);
+    OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
+    API_END
+}
+
+int od_ic0_factor(od_solver *s, od_matrix *A, double *L) {
+    API_BEGIN
+    require(s && A && L && A->mesh == s->mesh, "bad arguments");
+    require(s->precond == OD_PRE_IC0, "solver was not created with OD_PRE_IC0", OD_ERR_STATE);
+    build_precond(s, A, s->zeroMask);
+    OD_HIP(hipMemcpyAsync(L, s->U, sizeof(double) * s->mesh->topo.nValues, hipMemcpyDeviceToHost, s->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
+    API_END
+}
+
+// ---- integrators -------------------------------------------------------------------
+static od_integrator *make_integrator(od_mesh *mesh, int kind, int precond, double param, int precision) {
+    require(mesh->hasMaterial, "od_mesh_set_material must be called first", OD_ERR_STATE);
+    od_integrator *it = new od_integrator();
+    try {
+        it->mesh = mesh;
+        it->kind = kind;
+        it->solver = make_solver(mesh, precond, param, precision);
+        mesh->ensureContrib();
+        size_t n = (size_t)mesh->topo.ndof, nv = (size_t)mesh->topo.nValues;
+        it->M.mesh = mesh;
+        it->A.mesh = mesh;
+        it->M.vals = (double *)it->alloc(8 * nv);
+        it->A.vals = (double *)it->alloc(8 * nv);
+        if (precision == OD_MIXED) it->A.vals32 = (float *)it->alloc(4 * nv);
+        it->d = (double *)it->alloc(8 * n); it->v = (double *)it->alloc(8 * n);
+        it->a = (double *)it->alloc(8 * n); it->aNew = (double *)it->alloc(8 * n);
+        it->fext = (double *)it->alloc(8 * n); it->imp = (double *)it->alloc(8 * n);
+        it->rhs = (double *)it->alloc(8 * n);
+        it->active = (uint8_t *)it->alloc(mesh->topo.nInst);
+        it->inactive = (uint8_t *)it->alloc(mesh->topo.nInst);
+        it->lastIters.assign(mesh->topo.nInst, 0);
+        it->lastRelres.assign(mesh->topo.nInst, 0.0);
+        ElemParams P{1, 0, 0, 0, 0, 0, 0};
+        launch_element(mesh->dm, mesh->ctx->stream, nullptr, nullptr, nullptr, P, mesh->blkC, mesh->nodeC, nullptr);
+        launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, it->M.vals, 0, nullptr, nullptr);
+        OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
+    } catch (...) {
+        delete it;
+        throw;
+    }
+    return it;
+}
+
+int od_newmark_create(od_mesh *mesh, double beta, double gamma, double dt, double alphaM, double alphaK, int precond,
+                      double param, double tol, int maxIter, int precision, int initAccel, od_integrator **out) {
+    API_BEGIN
+    require(mesh && out, "NULL argument");
+    require(dt > 0 && beta > 0 && gamma > 0 && tol >= 0 && maxIter >= 0, "bad integrator parameters");
+    od_integrator *it = make_integrator(mesh, 0, precond, param, precision);
+    it->beta = beta; it->gamma = gamma; it->h = dt; it->alphaM = alphaM; it->alphaK = alphaK;
+    it->tol = tol; it->maxIter = maxIter; it->initAccel = initAccel != 0;
+    *out = it;
+    API_END
+}
+
+int od_backward_euler_create(od_mesh *mesh, double dt, double alphaM, double alphaK, int precond, double param,
+                             double tol, int maxIter, od_integrator **out) {
+    API_BEGIN
+    require(mesh && out, "NULL argument");
+    require(dt > 0 && tol >= 0 && maxIter >= 0, "bad integrator parameters");
+    od_integrator *it = make_integrator(mesh, 1, precond, param, OD_FP64);
+    it->h = dt; it->alphaM = alphaM; it->alphaK = alphaK; it->tol = tol; it->maxIter = maxIter;
+    *out = it;
+    API_END
+}
+
+int od_integrator_destroy(od_integrator *it) {
+    API_BEGIN
+    if (it) {
+        (void)hipStreamSynchronize(it->mesh->ctx->stream);
+        delete it;
+    }
+    API_END
+}
+
+int od_integrator_set_external(od_integrator *it, const double *f) {
+    API_BEGIN
+    require(it && f, "NULL argument");
+    OD_HIP(hipMemcpyAsync(it->fext, f, sizeof(double) * it->mesh->topo.ndof, hipMemcpyHostToDevice, it->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    API_END
+}
+int od_integrator_set_impulse(od_integrator *it, const double *f) {
+    API_BEGIN
+    require(it && f, "NULL argument");
+    OD_HIP(hipMemcpyAsync(it->imp, f, sizeof(double) * it->mesh->topo.ndof, hipMemcpyHostToDevice, it->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    it->hasImpulse = true;
+    API_END
+}
+int od_integrator_set_active(od_integrator *it, const unsigned char *active) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    if (!active) { it->hasActive = false; return OD_OK; }
+    OD_HIP(hipMemcpyAsync(it->active, active, it->mesh->topo.nInst, hipMemcpyHostToDevice, it->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    it->hasActive = true;
+    API_END
+}
+
+static void integrator_step(od_integrator *it) {
+    od_mesh *mesh = it->mesh;
+    const DevMesh &m = mesh->dm;
+    hipStream_t st = mesh->ctx->stream;
+    const uint8_t *act = it->hasActive ? it->active : nullptr;
+    od_solver *s = it->solver;
+    const uint8_t *skipArg = nullptr;
+    if (act) {
+        launch_set_done_from_active(m, st, it->inactive, act);
+        skipArg = it->inactive;
+    }
+    const double h = it->h;
+    if (it->kind == 0) {
+        const double beta = it->beta, gamma = it->gamma, aM = it->alphaM, aK = it->alphaK;
+        if (it->steps == 0 && it->initAccel) {
+            // consistent initial acceleration: M a0 = f_ext - f_int(d0) - C v0
+            ElemParams P0{0, 1, aK, 0.0, 1.0, 0.0, aM};
+            launch_element(m, st, it->d, it->v, nullptr, P0, mesh->blkC, mesh->nodeC, act);
+            launch_gather_rhs(m, st, mesh->nodeC, it->fext, nullptr, 1.0, it->rhs, 0, act);
+            od_solver *ms = make_solver(mesh, OD_PRE_JACOBI, 0.0, OD_FP64);
+            try {
+                solve(ms, &it->M, it->rhs, it->a, it->tol, it->maxIter, skipArg);
+                it->initIters = ms->iters.empty() ? 0 : *std::max_element(ms->iters.begin(), ms->iters.end());
+            } catch (...) {
+                delete ms;
+                throw;
+            }
+            delete ms;
+        }
+        ElemParams P{0, 1, h + aK, (0.5 - beta) * h * h + aK * (1.0 - gamma) * h, 1.0, (1.0 - gamma) * h, aM};
+        launch_element(m, st, it->d, it->v, it->a, P, mesh->blkC, mesh->nodeC, act);
+        const double cK = gamma * h * aK + beta * h * h, cM = 1.0 + gamma * h * aM;
+        launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act);
+        launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, 1.0, it->rhs, 0, act);
+        OD_HIP(hipMemcpyAsync(it->aNew, it->a, sizeof(double) * mesh->topo.ndof, hipMemcpyDeviceToDevice, st));
+        solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);
+        launch_newmark_correct(m, st, it->d, it->v, it->a, it->aNew, h, beta, gamma, act);
+        // inactive instances: aNew is the copy of a made before the solve, so the swap keeps their state
+        std::swap(it->a, it->aNew);
+    } else {
+        // backward Euler (nonlinearImplicitBackwardEuler.h:82-101); d is the mesh displacement state
+        const double aM = it->alphaM, aK = it->alphaK;
+        ElemParams P{0, 1, h + aK, 0.0, 1.0, 0.0, aM};
+        launch_element(m, st, it->d, it->v, nullptr, P, mesh->blkC, mesh->nodeC, act);
+        launch_gather_blocks(m, st, mesh->blkC, h * (h + aK), it->M.vals, 1.0 + h * aM, it->A.vals, 0, nullptr, act);
+        launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, h, it->rhs, 0, act);
+        // delta_v persists across steps as the warm start (aNew holds it)
+        solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);
+        launch_be_correct(m, st, it->v, it->aNew, act);
+    }
+    it->hasImpulse = false;
+    it->lastIters = s->iters;
+    it->lastRelres = s->relres;
+    it->lastOuter = s->outer;
+    it->steps++;
+}
+
+int od_integrator_step(od_integrator *it, int nSteps) {
+    API_BEGIN
+    require(it != nullptr && nSteps >= 0, "bad arguments");
+    for (int k = 0; k < nSteps; k++) integrator_step(it);
+    OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    API_END
+}
+
+int od_integrator_get_state(od_integrator *it, double *d, double *v, double *a) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    size_t b = sizeof(double) * it->mesh->topo.ndof;
+    hipStream_t st = it->mesh->ctx->stream;
+    if (d) OD_HIP(hipMemcpyAsync(d, it->d, b, hipMemcpyDeviceToHost, st));
+    if (v) OD_HIP(hipMemcpyAsync(v, it->v, b, hipMemcpyDeviceToHost, st));
+    if (a) OD_HIP(hipMemcpyAsync(a, it->kind == 0 ? it->a : it->aNew, b, hipMemcpyDeviceToHost, st));
+    OD_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int od_integrator_set_state(od_integrator *it, const double *d, const double *v, const double *a) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    size_t b = sizeof(double) * it->mesh->topo.ndof;
+    hipStream_t st = it->mesh->ctx->stream;
+    if (d) OD_HIP(hipMemcpyAsync(it->d, d, b, hipMemcpyHostToDevice, st));
+    if (v) OD_HIP(hipMemcpyAsync(it->v, v, b, hipMemcpyHostToDevice, st));
+    if (a) OD_HIP(hipMemcpyAsync(it->kind == 0 ? it->a : it->aNew, a, b, hipMemcpyHostToDevice, st));
+    OD_HIP(hipStreamSynchronize(st));
+    if (a) it->steps = std::max(it->steps, 1);
+    API_END
+}
+
+int od_integrator_state_dev(od_integrator *it, double **d, double **v, double **a) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    if (d) *d = it->d;
+    if (v) *v = it->v;
+    if (a) *a = it->kind == 0 ? it->a : it->aNew;
+    API_END
+}
+
+int od_integrator_stats(od_integrator *it, int *iters, double *relres, int *outer, int *initIters) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    int nInst = it->mesh->topo.nInst;
+    if (iters) std::memcpy(iters, it->lastIters.data(), sizeof(int) * nInst);
+    if (relres) std::memcpy(relres, it->lastRelres.data(), sizeof(double) * nInst);
+    if (outer) *outer = it->lastOuter;
+    if (initIters) *initIters = it->initIters;
+    API_END
+}
+
+int od_backward_euler_update_mesh(od_integrator *it) {
+    API_BEGIN
+    require(it != nullptr && it->kind == 1, "not a backward-Euler integrator", OD_ERR_STATE);
+    launch_axpy_rows(it->mesh->dm, it->mesh->ctx->stream, it->d, it->h, it->v, it->hasActive ? it->active : nullptr);
+    OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    API_END
+}
+
+int od_integrator_positions(od_integrator *it, float *pos) {
+    API_BEGIN
+    require(it && pos, "NULL argument");
+    const Topology &t = it->mesh->topo;
+    std::vector<double> d(t.ndof);
+    OD_HIP(hipMemcpyAsync(d.data(), it->d, sizeof(double) * t.ndof, hipMemcpyDeviceToHost, it->mesh->ctx->stream));
+    OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    for (int n = 0; n < t.nNodes; n++) {
+        int r = t.nodeRow[n];
+        for (int a = 0; a < 3; a++) pos[3 * n + a] = float(t.X[3 * n + a] + (r >= 0 ? d[3 * r + a] : 0.0));
+    }
+    API_END
+}
+
+}  // extern "C"

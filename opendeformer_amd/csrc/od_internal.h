@@ -1,0 +1,117 @@
+// Internal structures shared by the kernels and the C-ABI implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "od_setup.h"
+
+namespace od {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define OD_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            throw ::od::HipError(std::string(#call) + " failed: " + hipGetErrorString(e_));            \
+    } while (0)
+
+constexpr int kChunkRows = 256;     // block rows per workgroup chunk (never straddles an instance)
+constexpr int kBlock = 256;         // threads per workgroup
+
+// Device copy of the topology and material.
+struct DevMesh {
+    int nNodes, nTets, nRows, nInst, nChunks, nLevels;
+    int64_t nBlocks, nValues;
+    double *X;            // 3*nNodes
+    int *tets;            // 4*nTets
+    int *nodeRow;         // nNodes
+    int *rowPtr, *colIdx, *blkOff, *blkRow;     // pattern
+    int *tPtr, *tRow, *tBlk;                    // transposed index
+    int *pairBlk, *pairSlot; uint8_t *pairT;    // 10 per tet
+    int *blkCStart;                              // nBlocks+1
+    int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
+    int *levelRows;                              // nRows
+    int *chunkRow0, *chunkRow1, *chunkInst;      // nChunks
+    int *instChunkStart, *instRowStart;          // nInst+1
+    int *tetInst, *rowInst;
+    // material (per tet)
+    int *kind; double *E, *nu, *rho;
+    double thrCorot, thrNeo;
+};
+
+// Per-instance PCG scalars (device resident).
+struct InstScalars {
+    double delta, bb, rr, eps, alpha, beta, pq, rz;
+    int it, done, maxIt, pad;
+};
+
+// What the element kernel writes per node (see ElemParams).
+struct ElemParams {
+    int mode;             // 0: stiffness blocks + node terms, 1: mass blocks only
+    int nodeTerms;        // write node contributions
+    double cyv, cya;      // y = cyv*v + cya*a  (multiplied by K_e)
+    double czv, cza, cm;  // z = czv*v + cza*a  (multiplied by cm*M_e)
+};
+
+// ---- kernel launchers (od_kernels.hip) ------------------------------------------
+void launch_element(const DevMesh &m, hipStream_t s, const double *d, const double *v, const double *a,
+                    const ElemParams &p, double *blkC, double *nodeC, const uint8_t *instActive);
+void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
+                          double *out, int accumulate, float *out32, const uint8_t *instActive);
+void launch_gather_rhs(const DevMesh &m, hipStream_t s, const double *nodeC, const double *fext, const double *imp,
+                       double cf, double *out, int mode, const uint8_t *instActive);
+void launch_body_force(const DevMesh &m, hipStream_t s, const double *bodyPerTet, double *nodeC, double *f);
+void launch_expand_map(const DevMesh &m, hipStream_t s, int *map78);
+template <typename T>
+void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
+                 double *partials, const InstScalars *sc);
+
+// PCG building blocks (see od_kernels.hip for the exact phase semantics)
+struct PcgWork {
+    double *partials;     // nChunks * 4
+    InstScalars *sc;      // nInst
+    int *counter;         // last-workgroup counter
+    int *notDone;         // number of instances not done (device)
+};
+template <typename T>
+void pcg_init_residual(const DevMesh &m, hipStream_t s, const T *b, const T *q, T *r, PcgWork &w);
+template <typename T>
+void pcg_jacobi(const DevMesh &m, hipStream_t s, const T *invDiag, const T *r, T *z, PcgWork &w);
+template <typename T>
+void pcg_dot_rz_init(const DevMesh &m, hipStream_t s, const T *r, const T *z, T *p, PcgWork &w, double tol, int maxIt,
+                     const uint8_t *preDone);
+template <typename T>
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w);
+template <typename T>
+void pcg_update_xr(const DevMesh &m, hipStream_t s, const T *p, const T *q, T *x, T *r, const T *invDiag, T *z,
+                   PcgWork &w);
+template <typename T>
+void pcg_dot_rz(const DevMesh &m, hipStream_t s, const T *r, const T *z, PcgWork &w);
+template <typename T>
+void pcg_update_p(const DevMesh &m, hipStream_t s, const T *z, T *p, PcgWork &w);
+template <typename T>
+void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *invDiag);
+template <typename T>
+void launch_ic0_factor(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const double *A, double *U,
+                       T *U32, double pivotEps, const uint8_t *skip);
+template <typename T>
+void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *U, const T *r, T *z,
+                      const InstScalars *sc, const uint8_t *skip);
+void launch_cast_f64_f32(hipStream_t s, const double *a, float *b, int64_t n);
+void launch_residual_f64(const DevMesh &m, hipStream_t s, const double *b, const double *Ax, double *r, float *r32,
+                         PcgWork &w, double tol, const uint8_t *preDone, int first);
+void launch_add_correction(const DevMesh &m, hipStream_t s, const float *dx, double *x, const InstScalars *outer);
+void launch_newmark_correct(const DevMesh &m, hipStream_t s, double *d, double *v, const double *aOld,
+                            const double *aNew, double h, double beta, double gamma, const uint8_t *instActive);
+void launch_be_correct(const DevMesh &m, hipStream_t s, double *v, const double *dv, const uint8_t *instActive);
+void launch_axpy_rows(const DevMesh &m, hipStream_t s, double *y, double a, const double *x, const uint8_t *instActive);
+void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDone, const uint8_t *instActive);
+
+}  // namespace od

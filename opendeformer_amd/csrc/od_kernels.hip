@@ -1,0 +1,977 @@
+// HIP kernels for gfx950 (CDNA4, wave64) — the implicit-step hot path.
+//
+//  * element:   one thread per tet, FP64.  Writes the tet's 10 node-pair 3x3
+//               blocks into a contribution array ordered by destination block,
+//               and its 4 nodal 3-vectors into one ordered by node.
+//  * gather:    one thread per stored value; sums the contributions of its block
+//               in a fixed order (deterministic, atomic-free assembly) and fuses
+//               the matrix combine A = cK*K + cM*M (+ an FP32 copy).
+//  * spmv:      8 lanes per block row over the canonical upper-BSR values; the
+//               row's own upper blocks and its transposed lower blocks
+//               (through the transposed index) are one work list.
+//  * PCG:       chunked vector kernels with per-instance FP64 reductions; the
+//               last workgroup of every reduction kernel finalises the scalars
+//               (alpha, beta, convergence) on the device.
+//  * IC(0):     level-scheduled block factorisation and sweeps, one thread per row.
+#include <hip/hip_runtime.h>
+
+#include "od_elem.h"
+#include "od_internal.h"
+
+namespace od {
+
+using namespace odk;
+
+// =====================================================================================
+// element kernel
+// =====================================================================================
+__device__ __forceinline__ void store_block(double *dst, const double *m, int transpose) {
+    if (transpose) {
+        dst[0] = m[0]; dst[1] = m[3]; dst[2] = m[6];
+        dst[3] = m[1]; dst[4] = m[4]; dst[5] = m[7];
+        dst[6] = m[2]; dst[7] = m[5]; dst[8] = m[8];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) dst[i] = m[i];
+    }
+}
+
+__global__ __launch_bounds__(128) void k_element(DevMesh m, const double *__restrict__ d, const double *__restrict__ v,
+                                                 const double *__restrict__ a, ElemParams P, double *__restrict__ blkC,
+                                                 double *__restrict__ nodeC, const uint8_t *__restrict__ instActive) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.nTets) return;
+    if (instActive && !instActive[m.tetInst[e]]) return;
+    int tn[4], row[4];
+    v3 X[4], x[4], y[4], z[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        tn[k] = m.tets[4 * e + k];
+        row[k] = m.nodeRow[tn[k]];
+        X[k] = mk(m.X[3 * tn[k]], m.X[3 * tn[k] + 1], m.X[3 * tn[k] + 2]);
+        v3 u = mk(0, 0, 0), vv = mk(0, 0, 0), aa = mk(0, 0, 0);
+        if (row[k] >= 0) {
+            int o = 3 * row[k];
+            if (d) u = mk(d[o], d[o + 1], d[o + 2]);
+            if (v) vv = mk(v[o], v[o + 1], v[o + 2]);
+            if (a) aa = mk(a[o], a[o + 1], a[o + 2]);
+        }
+        x[k] = X[k] + u;
+        y[k] = vv * P.cyv + aa * P.cya;
+        z[k] = vv * P.czv + aa * P.cza;
+    }
+    const double V0 = tet_volume(X[0], X[1], X[2], X[3]);
+    const double rho = m.rho ? m.rho[e] : 0.0;
+    v3 g[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) g[k] = mk(0, 0, 0);
+
+    if (P.mode == 1) {  // consistent mass blocks rho*V/10 (diagonal pair) and rho*V/20 (tetelement.cpp:51-66)
+        int p = 0;
+        for (int aI = 0; aI < 4; aI++)
+            for (int bI = aI; bI < 4; bI++, p++) {
+                int slot = m.pairSlot[10 * e + p];
+                if (slot < 0) continue;
+                double mv = rho * (aI == bI ? V0 * 0.1 : V0 * 0.05);
+                double *dst = blkC + (size_t)slot * 9;
+                dst[0] = mv; dst[1] = 0; dst[2] = 0; dst[3] = 0; dst[4] = mv; dst[5] = 0; dst[6] = 0; dst[7] = 0; dst[8] = mv;
+            }
+        return;
+    }
+
+    v3 dN[4];
+    tet_dn(X[0], X[1], X[2], X[3], dN);
+    const double factor = 1.0 / (36.0 * V0);
+    // rest shape matrix inverse and deformation gradient F = sum_i (x_i - x_0) (x) DmInv row (i-1)
+    double Dm[9] = {X[1].x - X[0].x, X[2].x - X[0].x, X[3].x - X[0].x,
+                    X[1].y - X[0].y, X[2].y - X[0].y, X[3].y - X[0].y,
+                    X[1].z - X[0].z, X[2].z - X[0].z, X[3].z - X[0].z};
+    double Di[9];
+    inv3(Dm, Di);
+    double F[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) F[i] = 0.0;
+#pragma unroll
+    for (int i = 1; i < 4; i++) {
+        v3 ax = x[i] - x[0];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            F[0 + k] += ax.x * Di[(i - 1) * 3 + k];
+            F[3 + k] += ax.y * Di[(i - 1) * 3 + k];
+            F[6 + k] += ax.z * Di[(i - 1) * 3 + k];
+        }
+    }
+    const double En = m.E[e], nu = m.nu[e];
+    const double lambda = nu * En / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double mu = En / (2.0 * (1.0 + nu));
+
+    if (m.kind[e] == 0) {
+        // ---- corotational: R from polar (or QR when compressed), blocks R K0_ab R^T
+        double R[9];
+        const double Vc = tet_volume(x[0], x[1], x[2], x[3]);
+        if (Vc >= m.thrCorot * V0) polar3(F, R);
+        else qr3(F, R);
+        const double D0 = lambda + 2.0 * mu, D1 = lambda, D2 = mu;
+        int p = 0;
+        for (int aI = 0; aI < 4; aI++) {
+            for (int bI = aI; bI < 4; bI++, p++) {
+                const v3 Ba = dN[aI], Bb = dN[bI];
+                double k[9];
+                k[0] = (D0 * Ba.x * Bb.x + D2 * (Ba.y * Bb.y + Ba.z * Bb.z)) * factor;
+                k[1] = (D1 * Ba.x * Bb.y + D2 * Ba.y * Bb.x) * factor;
+                k[2] = (D1 * Ba.x * Bb.z + D2 * Ba.z * Bb.x) * factor;
+                k[3] = (D1 * Ba.y * Bb.x + D2 * Ba.x * Bb.y) * factor;
+                k[4] = (D0 * Ba.y * Bb.y + D2 * (Ba.x * Bb.x + Ba.z * Bb.z)) * factor;
+                k[5] = (D1 * Ba.y * Bb.z + D2 * Ba.z * Bb.y) * factor;
+                k[6] = (D1 * Ba.z * Bb.x + D2 * Ba.x * Bb.z) * factor;
+                k[7] = (D1 * Ba.z * Bb.y + D2 * Ba.y * Bb.z) * factor;
+                k[8] = (D0 * Ba.z * Bb.z + D2 * (Ba.x * Bb.x + Ba.y * Bb.y)) * factor;
+                double QK[9], KK[9];
+                mul3(R, k, QK);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        KK[i * 3 + j] = QK[i * 3 + 0] * R[j * 3 + 0] + QK[i * 3 + 1] * R[j * 3 + 1] + QK[i * 3 + 2] * R[j * 3 + 2];
+                int slot = m.pairSlot[10 * e + p];
+                if (slot >= 0) store_block(blkC + (size_t)slot * 9, KK, m.pairT[10 * e + p]);
+                if (!P.nodeTerms) continue;
+                // corotational force (tetelement.cpp:622-647) plus the K_e y term
+                v3 xb = x[bI] + y[bI];
+                g[aI] = g[aI] + mulv(KK, xb) - mulv(QK, X[bI]);
+                if (bI != aI) {
+                    // QKT = R k^T
+                    double QKT[9];
+#pragma unroll
+                    for (int i = 0; i < 3; i++)
+#pragma unroll
+                        for (int j = 0; j < 3; j++)
+                            QKT[i * 3 + j] = R[i * 3 + 0] * k[j * 3 + 0] + R[i * 3 + 1] * k[j * 3 + 1] + R[i * 3 + 2] * k[j * 3 + 2];
+                    v3 xa = x[aI] + y[aI];
+                    v3 kt = mk(KK[0] * xa.x + KK[3] * xa.y + KK[6] * xa.z, KK[1] * xa.x + KK[4] * xa.y + KK[7] * xa.z,
+                               KK[2] * xa.x + KK[5] * xa.y + KK[8] * xa.z);
+                    g[bI] = g[bI] + kt - mulv(QKT, X[aI]);
+                }
+            }
+        }
+    } else {
+        // ---- invertible neo-Hookean: P and the PSD-projected dP/dF in the diagonal frame
+        double U[9], VT[9], Ps[9], tm[6], al[3], be[3];
+        neo_stress(F, lambda, mu, m.thrNeo, U, VT, Ps, tm, al, be);
+        v3 dr[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) dr[k] = mulv(VT, dN[k]);
+        int p = 0;
+        for (int aI = 0; aI < 4; aI++) {
+            for (int bI = aI; bI < 4; bI++, p++) {
+                const v3 na = dr[aI], nb = dr[bI];
+                double S[9];
+                S[0] = tm[0] * na.x * nb.x + al[0] * na.y * nb.y + al[1] * na.z * nb.z;
+                S[4] = tm[3] * na.y * nb.y + al[0] * na.x * nb.x + al[2] * na.z * nb.z;
+                S[8] = tm[5] * na.z * nb.z + al[1] * na.x * nb.x + al[2] * na.y * nb.y;
+                S[1] = tm[1] * na.x * nb.y + be[0] * na.y * nb.x;
+                S[3] = tm[1] * na.y * nb.x + be[0] * na.x * nb.y;
+                S[2] = tm[2] * na.x * nb.z + be[1] * na.z * nb.x;
+                S[6] = tm[2] * na.z * nb.x + be[1] * na.x * nb.z;
+                S[5] = tm[4] * na.y * nb.z + be[2] * na.z * nb.y;
+                S[7] = tm[4] * na.z * nb.y + be[2] * na.y * nb.z;
+                double US[9], KK[9];
+                mul3(U, S, US);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        KK[i * 3 + j] = (US[i * 3 + 0] * U[j * 3 + 0] + US[i * 3 + 1] * U[j * 3 + 1] + US[i * 3 + 2] * U[j * 3 + 2]) * factor;
+                int slot = m.pairSlot[10 * e + p];
+                if (slot >= 0) store_block(blkC + (size_t)slot * 9, KK, m.pairT[10 * e + p]);
+                if (!P.nodeTerms) continue;
+                g[aI] = g[aI] + mulv(KK, y[bI]);
+                if (bI != aI) {
+                    v3 ya = y[aI];
+                    g[bI] = g[bI] + mk(KK[0] * ya.x + KK[3] * ya.y + KK[6] * ya.z, KK[1] * ya.x + KK[4] * ya.y + KK[7] * ya.z,
+                                       KK[2] * ya.x + KK[5] * ya.y + KK[8] * ya.z);
+                }
+            }
+        }
+        if (P.nodeTerms) {
+            // nodal forces f_i = P dN_i / 6, f_3 = -(f_0 + f_1 + f_2)  (tetelement.cpp:344-355)
+            const double f6 = 1.0 / 6.0;
+            v3 f[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) f[i] = mulv(Ps, dN[i]) * f6;
+            g[0] = g[0] + f[0];
+            g[1] = g[1] + f[1];
+            g[2] = g[2] + f[2];
+            g[3] = g[3] - (f[0] + f[1] + f[2]);
+        }
+    }
+    if (!P.nodeTerms) return;
+    if (P.cm != 0.0) {
+        const double md = rho * V0 * 0.1, mo = rho * V0 * 0.05;
+        v3 zs = z[0] + z[1] + z[2] + z[3];
+#pragma unroll
+        for (int k = 0; k < 4; k++) g[k] = g[k] + (z[k] * (md - mo) + zs * mo) * P.cm;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int slot = m.nodeSlot[4 * e + k];
+        if (slot < 0) continue;
+        double *dst = nodeC + (size_t)slot * 3;
+        dst[0] = g[k].x; dst[1] = g[k].y; dst[2] = g[k].z;
+    }
+}
+
+void launch_element(const DevMesh &m, hipStream_t s, const double *d, const double *v, const double *a,
+                    const ElemParams &p, double *blkC, double *nodeC, const uint8_t *instActive) {
+    int grid = (m.nTets + 127) / 128;
+    hipLaunchKernelGGL(k_element, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
+    OD_HIP(hipGetLastError());
+}
+
+// =====================================================================================
+// gathers
+// =====================================================================================
+__constant__ int c_diagSrc[6] = {0, 1, 2, 4, 5, 8};
+
+__global__ void k_gather_blocks(DevMesh m, const double *__restrict__ C, double cK, const double *__restrict__ M,
+                                double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
+                                const uint8_t *__restrict__ instActive) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= m.nBlocks * 9) return;
+    const int b = (int)(gid / 9), c = (int)(gid % 9);
+    const int r = m.blkRow[b];
+    if (instActive && !instActive[m.rowInst[r]]) return;
+    const bool diag = m.colIdx[b] == r;
+    if (diag && c >= 6) return;
+    const int src = diag ? c_diagSrc[c] : c;
+    const int64_t vi = (int64_t)m.blkOff[b] + c;
+    double s = 0.0;
+    for (int k = m.blkCStart[b]; k < m.blkCStart[b + 1]; k++) s += C[(size_t)k * 9 + src];
+    double o = cK * s;
+    if (M) o += cM * M[vi];
+    if (accumulate) o += out[vi];
+    out[vi] = o;
+    if (out32) out32[vi] = (float)o;
+}
+
+void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
+                          double *out, int accumulate, float *out32, const uint8_t *instActive) {
+    int64_t n = m.nBlocks * 9;
+    int grid = (int)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_gather_blocks, dim3(grid), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out, accumulate, out32, instActive);
+    OD_HIP(hipGetLastError());
+}
+
+// mode 0: out = cf*(fext + imp - sum);  mode 1: out += sum (internal force accumulate)
+__global__ void k_gather_rhs(DevMesh m, const double *__restrict__ C, const double *__restrict__ fext,
+                             const double *__restrict__ imp, double cf, double *__restrict__ out, int mode,
+                             const uint8_t *__restrict__ instActive) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * m.nRows) return;
+    const int r = i / 3, ax = i % 3;
+    if (instActive && !instActive[m.rowInst[r]]) return;
+    double s = 0.0;
+    for (int k = m.rowCStart[r]; k < m.rowCStart[r + 1]; k++) s += C[(size_t)k * 3 + ax];
+    if (mode == 1) { out[i] += s; return; }
+    double f = fext ? fext[i] : 0.0;
+    if (imp) f += imp[i];
+    out[i] = cf * (f - s);
+}
+
+void launch_gather_rhs(const DevMesh &m, hipStream_t s, const double *nodeC, const double *fext, const double *imp,
+                       double cf, double *out, int mode, const uint8_t *instActive) {
+    int n = 3 * m.nRows;
+    hipLaunchKernelGGL(k_gather_rhs, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m, nodeC, fext, imp, cf, out,
+                       mode, instActive);
+    OD_HIP(hipGetLastError());
+}
+
+// body force V/4 * b per node (tetelement.cpp:68-77), written per node slot
+__global__ void k_body(DevMesh m, const double *__restrict__ body, double *__restrict__ nodeC) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.nTets) return;
+    v3 X[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int n = m.tets[4 * e + k];
+        X[k] = mk(m.X[3 * n], m.X[3 * n + 1], m.X[3 * n + 2]);
+    }
+    double f = tet_volume(X[0], X[1], X[2], X[3]) / 4.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int slot = m.nodeSlot[4 * e + k];
+        if (slot < 0) continue;
+        nodeC[(size_t)slot * 3 + 0] = f * body[3 * e + 0];
+        nodeC[(size_t)slot * 3 + 1] = f * body[3 * e + 1];
+        nodeC[(size_t)slot * 3 + 2] = f * body[3 * e + 2];
+    }
+}
+
+void launch_body_force(const DevMesh &m, hipStream_t s, const double *bodyPerTet, double *nodeC, double *f) {
+    hipLaunchKernelGGL(k_body, dim3((m.nTets + 127) / 128), dim3(128), 0, s, m, bodyPerTet, nodeC);
+    OD_HIP(hipGetLastError());
+    OD_HIP(hipMemsetAsync(f, 0, sizeof(double) * 3 * (size_t)m.nRows, s));
+    launch_gather_rhs(m, s, nodeC, nullptr, nullptr, 1.0, f, 1, nullptr);
+}
+
+// Expands the block-level map into the reference's 78-entry per-tet map.
+__global__ void k_expand_map(DevMesh m, int *__restrict__ map78) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.nTets) return;
+    int *out = map78 + (size_t)78 * e;
+    // packed order: sub-column c = 3a+i, sub-row r >= c (element.h:109-113)
+    for (int c = 0; c < 12; c++) {
+        for (int r = c; r < 12; r++) {
+            int li = c * 12 - ((c - 1) * c) / 2 + (r - c);
+            int aN = c / 3, i = c % 3, bN = r / 3, j = r % 3;   // aN <= bN
+            int p = aN * 4 - (aN * (aN - 1)) / 2 + (bN - aN);  // pair index of (aN, bN), aN <= bN
+            int b = m.pairBlk[10 * e + p];
+            if (b < 0) { out[li] = -1; continue; }
+            int base = m.blkOff[b];
+            if (aN == bN) {
+                // packed upper row-major of the diagonal block, entry (i, j), i <= j
+                out[li] = base + (i == 0 ? j : (i == 1 ? 2 + j : 5));
+            } else {
+                // element entry (row a-DOF i, col b-DOF j); stored block is (lowRow, highRow) row-major
+                int t = m.pairT[10 * e + p];
+                out[li] = base + (t ? (j * 3 + i) : (i * 3 + j));
+            }
+        }
+    }
+}
+
+void launch_expand_map(const DevMesh &m, hipStream_t s, int *map78) {
+    hipLaunchKernelGGL(k_expand_map, dim3((m.nTets + 127) / 128), dim3(128), 0, s, m, map78);
+    OD_HIP(hipGetLastError());
+}
+
+// =====================================================================================
+// reductions: block partials + last-workgroup finalisation per instance
+// =====================================================================================
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Reduces NQ per-thread values over the workgroup; thread 0 gets the sums.
+template <int NQ>
+__device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        double s = wave_sum(v[q]);
+        if (lane == 0) lds[w * NQ + q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            double s = 0;
+            for (int k = 0; k < nw; k++) s += lds[k * NQ + q];
+            v[q] = s;
+        }
+    }
+}
+
+enum Phase { PH_INIT_RES = 0, PH_INIT_RZ = 1, PH_ALPHA = 2, PH_XR = 3, PH_XR_RZ = 4, PH_RZ = 5, PH_OUTER_RES = 6 };
+
+struct FinalArgs {
+    double tol;
+    int maxIt;
+    const uint8_t *preDone;
+};
+
+// Per-instance scalar update once all chunk partials of a phase are written.
+__device__ void finalize_phase(const DevMesh &m, int phase, const double *partials, InstScalars *sc, int *notDone,
+                               FinalArgs fa) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __shared__ int s_count[16];
+    int myCount = 0;
+    for (int inst = w; inst < m.nInst; inst += nw) {
+        const int c0 = m.instChunkStart[inst], c1 = m.instChunkStart[inst + 1];
+        double t0 = 0, t1 = 0;
+        for (int c = c0 + lane; c < c1; c += 64) { t0 += partials[4 * c + 0]; t1 += partials[4 * c + 1]; }
+        t0 = wave_sum(t0);
+        t1 = wave_sum(t1);
+        if (lane == 0) {
+            InstScalars S = sc[inst];
+            switch (phase) {
+            case PH_INIT_RES:
+                S.bb = t0; S.rr = t1;
+                break;
+            case PH_INIT_RZ:
+                S.delta = t0; S.it = 0; S.maxIt = fa.maxIt;
+                S.eps = fa.tol * fa.tol * S.bb;
+                S.done = (fa.preDone && fa.preDone[inst]) || !(S.rr > S.eps) || fa.maxIt <= 0;
+                break;
+            case PH_ALPHA:
+                if (!S.done) { S.pq = t0; S.alpha = S.delta / t0; }
+                break;
+            case PH_XR:
+                if (!S.done) S.rr = t0;
+                break;
+            case PH_XR_RZ:
+            case PH_RZ:
+                if (!S.done) {
+                    if (phase == PH_XR_RZ) { S.rr = t0; S.rz = t1; }
+                    else S.rz = t0;
+                    S.beta = S.rz / S.delta;
+                    S.delta = S.rz;
+                    S.it += 1;
+                    S.done = !(S.rr > S.eps) || S.it >= S.maxIt;
+                }
+                break;
+            case PH_OUTER_RES:
+                // outer refinement: t0 = r.r of the FP64 residual, t1 = b.b
+                S.rr = t0;
+                if (fa.maxIt < 0) S.bb = t1;                 // first residual of the solve
+                S.eps = fa.tol * fa.tol * S.bb;
+                S.done = (fa.preDone && fa.preDone[inst]) || !(S.rr > S.eps);
+                break;
+            }
+            sc[inst] = S;
+            myCount += S.done ? 0 : 1;
+        }
+    }
+    if (lane == 0) s_count[w] = myCount;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < nw; k++) t += s_count[k];
+        *notDone = t;
+    }
+}
+
+// Writes this workgroup's partials; the last workgroup to arrive finalises.
+template <int NQ>
+__device__ void commit_partials(const DevMesh &m, double (&v)[NQ], double *partials, int chunk, int *counter,
+                                int phase, InstScalars *sc, int *notDone, FinalArgs fa) {
+    __shared__ double lds[16 * 4];
+    __shared__ int s_last;
+    block_sum<NQ>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) partials[4 * chunk + q] = v[q];
+        for (int q = NQ; q < 2; q++) partials[4 * chunk + q] = 0.0;
+        __threadfence();
+        int prev = atomicAdd(counter, 1);
+        s_last = (prev == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        finalize_phase(m, phase, partials, sc, notDone, fa);
+        if (threadIdx.x == 0) atomicExch(counter, 0);
+    }
+}
+
+// =====================================================================================
+// SpMV over the canonical upper-BSR layout, 8 lanes per block row
+// =====================================================================================
+template <typename T>
+__device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x, int r,
+                                         int lane8, T &o0, T &o1, T &o2) {
+    const int b0 = m.rowPtr[r], nu = m.rowPtr[r + 1] - b0;
+    const int t0 = m.tPtr[r], nt = m.tPtr[r + 1] - t0;
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int it = lane8; it < nu + nt; it += 8) {
+        if (it < nu) {
+            const int b = b0 + it;
+            const T *A = vals + m.blkOff[b];
+            const int c = m.colIdx[b];
+            const T x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
+            if (it == 0) {  // packed diagonal (symmetric)
+                a0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
+                a1 += A[1] * x0 + A[3] * x1 + A[4] * x2;
+                a2 += A[2] * x0 + A[4] * x1 + A[5] * x2;
+            } else {
+                a0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
+                a1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
+                a2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
+            }
+        } else {
+            const int k = t0 + it - nu;
+            const int K = m.tRow[k];
+            const T *A = vals + m.blkOff[m.tBlk[k]];
+            const T x0 = x[3 * K], x1 = x[3 * K + 1], x2 = x[3 * K + 2];
+            a0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
+            a1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
+            a2 += A[2] * x0 + A[5] * x1 + A[8] * x2;
+        }
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 8);
+        a1 += __shfl_xor(a1, o, 8);
+        a2 += __shfl_xor(a2, o, 8);
+    }
+    o0 = a0; o1 = a1; o2 = a2;
+}
+
+// One workgroup per chunk; optional dot partial (x.y) with PH_ALPHA finalisation.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
+                                              T *__restrict__ y, int accumulate, double *partials, int *counter,
+                                              InstScalars *sc, int *notDone, int withDot) {
+    const int chunk = blockIdx.x;
+    const int r0 = m.chunkRow0[chunk], r1 = m.chunkRow1[chunk];
+    const int inst = m.chunkInst[chunk];
+    const bool skip = sc && sc[inst].done;
+    const int group = threadIdx.x >> 3, lane8 = threadIdx.x & 7;
+    double dsum = 0.0;
+    if (!skip) {
+        for (int r = r0 + group; r < r1; r += (int)(blockDim.x >> 3)) {
+            T o0, o1, o2;
+            row_spmv<T>(m, vals, x, r, lane8, o0, o1, o2);
+            if (lane8 == 0) {
+                if (accumulate) { o0 += y[3 * r]; o1 += y[3 * r + 1]; o2 += y[3 * r + 2]; }
+                y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
+                if (withDot) dsum += (double)x[3 * r] * (double)o0 + (double)x[3 * r + 1] * (double)o1 + (double)x[3 * r + 2] * (double)o2;
+            }
+        }
+    }
+    if (withDot) {
+        double v[1] = {dsum};
+        commit_partials<1>(m, v, partials, chunk, counter, PH_ALPHA, sc, notDone, FinalArgs{0, 0, nullptr});
+    }
+}
+
+template <typename T>
+void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate, double *partials,
+                 const InstScalars *sc) {
+    hipLaunchKernelGGL(k_spmv<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, vals, x, y, accumulate, partials, nullptr,
+                       const_cast<InstScalars *>(sc), nullptr, 0);
+    OD_HIP(hipGetLastError());
+}
+template void launch_spmv<double>(const DevMesh &, hipStream_t, const double *, const double *, double *, int, double *,
+                                  const InstScalars *);
+template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, const float *, float *, int, double *,
+                                 const InstScalars *);
+
+// =====================================================================================
+// PCG vector kernels (one workgroup per chunk, 3 DOFs per block row)
+// =====================================================================================
+#define CHUNK_PROLOGUE                                                         \
+    const int chunk = blockIdx.x;                                              \
+    const int i0 = 3 * m.chunkRow0[chunk], i1 = 3 * m.chunkRow1[chunk];        \
+    const int inst = m.chunkInst[chunk];
+
+// r = b - q ; partials b.b, r.r
+template <typename T>
+__global__ void k_init_res(DevMesh m, const T *b, const T *q, T *r, double *partials, int *counter, InstScalars *sc,
+                           int *notDone) {
+    CHUNK_PROLOGUE
+    double bb = 0, rr = 0;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        T ri = b[i] - q[i];
+        r[i] = ri;
+        bb += (double)b[i] * (double)b[i];
+        rr += (double)ri * (double)ri;
+    }
+    (void)inst;
+    double v[2] = {bb, rr};
+    commit_partials<2>(m, v, partials, chunk, counter, PH_INIT_RES, sc, notDone, FinalArgs{0, 0, nullptr});
+}
+
+template <typename T>
+void pcg_init_residual(const DevMesh &m, hipStream_t s, const T *b, const T *q, T *r, PcgWork &w) {
+    hipLaunchKernelGGL(k_init_res<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, b, q, r, w.partials, w.counter, w.sc, w.notDone);
+    OD_HIP(hipGetLastError());
+}
+
+// z = invDiag * r (no reduction)
+template <typename T>
+__global__ void k_jacobi(DevMesh m, const T *invDiag, const T *r, T *z, InstScalars *sc) {
+    CHUNK_PROLOGUE
+    if (sc && sc[inst].done) return;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) z[i] = invDiag ? invDiag[i] * r[i] : r[i];
+}
+
+template <typename T>
+void pcg_jacobi(const DevMesh &m, hipStream_t s, const T *invDiag, const T *r, T *z, PcgWork &w) {
+    hipLaunchKernelGGL(k_jacobi<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, invDiag, r, z, (InstScalars *)nullptr);
+    OD_HIP(hipGetLastError());
+}
+
+// p = z ; partial r.z ; finalise delta, eps, done
+template <typename T>
+__global__ void k_init_rz(DevMesh m, const T *r, const T *z, T *p, double *partials, int *counter, InstScalars *sc,
+                          int *notDone, FinalArgs fa) {
+    CHUNK_PROLOGUE
+    (void)inst;
+    double rz = 0;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        p[i] = z[i];
+        rz += (double)r[i] * (double)z[i];
+    }
+    double v[1] = {rz};
+    commit_partials<1>(m, v, partials, chunk, counter, PH_INIT_RZ, sc, notDone, fa);
+}
+
+template <typename T>
+void pcg_dot_rz_init(const DevMesh &m, hipStream_t s, const T *r, const T *z, T *p, PcgWork &w, double tol, int maxIt,
+                     const uint8_t *preDone) {
+    hipLaunchKernelGGL(k_init_rz<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, r, z, p, w.partials, w.counter, w.sc,
+                       w.notDone, FinalArgs{tol, maxIt, preDone});
+    OD_HIP(hipGetLastError());
+}
+
+template <typename T>
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w) {
+    hipLaunchKernelGGL(k_spmv<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, vals, p, q, 0, w.partials, w.counter, w.sc,
+                       w.notDone, 1);
+    OD_HIP(hipGetLastError());
+}
+
+// x += alpha p ; r -= alpha q ; (Jacobi: z = D^-1 r) ; partials r.r (and r.z)
+template <typename T>
+__global__ void k_update_xr(DevMesh m, const T *p, const T *q, T *x, T *r, const T *invDiag, T *z, int fuseJacobi,
+                            double *partials, int *counter, InstScalars *sc, int *notDone) {
+    CHUNK_PROLOGUE
+    double rr = 0, rz = 0;
+    if (!sc[inst].done) {
+        const double al = sc[inst].alpha;
+        const T a = (T)al;
+        for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            T xi = x[i] + a * p[i];
+            T ri = r[i] - a * q[i];
+            x[i] = xi;
+            r[i] = ri;
+            rr += (double)ri * (double)ri;
+            if (fuseJacobi) {
+                T zi = invDiag ? invDiag[i] * ri : ri;
+                z[i] = zi;
+                rz += (double)ri * (double)zi;
+            }
+        }
+    }
+    if (fuseJacobi) {
+        double v[2] = {rr, rz};
+        commit_partials<2>(m, v, partials, chunk, counter, PH_XR_RZ, sc, notDone, FinalArgs{0, 0, nullptr});
+    } else {
+        double v[1] = {rr};
+        commit_partials<1>(m, v, partials, chunk, counter, PH_XR, sc, notDone, FinalArgs{0, 0, nullptr});
+    }
+}
+
+template <typename T>
+void pcg_update_xr(const DevMesh &m, hipStream_t s, const T *p, const T *q, T *x, T *r, const T *invDiag, T *z,
+                   PcgWork &w) {
+    // z == nullptr -> the preconditioner runs separately (IC sweeps)
+    hipLaunchKernelGGL(k_update_xr<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, p, q, x, r, invDiag, z, z ? 1 : 0,
+                       w.partials, w.counter, w.sc, w.notDone);
+    OD_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ void k_dot_rz(DevMesh m, const T *r, const T *z, double *partials, int *counter, InstScalars *sc, int *notDone) {
+    CHUNK_PROLOGUE
+    double rz = 0;
+    if (!sc[inst].done)
+        for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) rz += (double)r[i] * (double)z[i];
+    double v[1] = {rz};
+    commit_partials<1>(m, v, partials, chunk, counter, PH_RZ, sc, notDone, FinalArgs{0, 0, nullptr});
+}
+
+template <typename T>
+void pcg_dot_rz(const DevMesh &m, hipStream_t s, const T *r, const T *z, PcgWork &w) {
+    hipLaunchKernelGGL(k_dot_rz<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, r, z, w.partials, w.counter, w.sc, w.notDone);
+    OD_HIP(hipGetLastError());
+}
+
+// p = z + beta p  (skipped for instances that finished this iteration)
+template <typename T>
+__global__ void k_update_p(DevMesh m, const T *z, T *p, InstScalars *sc) {
+    CHUNK_PROLOGUE
+    if (sc[inst].done) return;
+    const T be = (T)sc[inst].beta;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = z[i] + be * p[i];
+}
+
+template <typename T>
+void pcg_update_p(const DevMesh &m, hipStream_t s, const T *z, T *p, PcgWork &w) {
+    hipLaunchKernelGGL(k_update_p<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, z, p, w.sc);
+    OD_HIP(hipGetLastError());
+}
+
+// inverse diagonal, 1 where the diagonal is <= 0 (jacobiPreconditioner.cpp:18-21)
+template <typename T>
+__global__ void k_jacobi_diag(DevMesh m, const double *vals, T *inv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * m.nRows) return;
+    const int r = i / 3, ax = i % 3;
+    const int off = m.blkOff[m.rowPtr[r]] + (ax == 0 ? 0 : (ax == 1 ? 3 : 5));
+    double dg = vals[off];
+    inv[i] = (T)(dg > 0.0 ? 1.0 / dg : 1.0);
+}
+
+template <typename T>
+void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *invDiag) {
+    int n = 3 * m.nRows;
+    hipLaunchKernelGGL(k_jacobi_diag<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m, vals, invDiag);
+    OD_HIP(hipGetLastError());
+}
+
+// =====================================================================================
+// IC(0), block form: U upper (canonical layout), U^T U = A on the block pattern
+// =====================================================================================
+__device__ __forceinline__ double pivot_fix(double d, double eps2) { return d <= 0.0 ? eps2 * d * d : d; }
+
+template <typename T>
+__global__ void k_ic0_factor_level(DevMesh m, const int *__restrict__ rows, int count, const double *__restrict__ A,
+                                   double *__restrict__ U, T *__restrict__ U32, double eps2,
+                                   const uint8_t *__restrict__ skip) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int I = rows[t];
+    if (skip && skip[m.rowInst[I]]) return;
+    const int b0 = m.rowPtr[I], b1 = m.rowPtr[I + 1];
+    const int v0 = m.blkOff[b0], v1 = (b1 < m.nBlocks) ? m.blkOff[b1] : (int)m.nValues;
+    // row I of U starts as row I of A
+    for (int k = v0; k < v1; k++) U[k] = A[k];
+    double s00 = U[v0], s01 = U[v0 + 1], s02 = U[v0 + 2], s11 = U[v0 + 3], s12 = U[v0 + 4], s22 = U[v0 + 5];
+    // subtract sum over K < I of U_KI^T U_KJ (J in row I's pattern, J >= I)
+    for (int k = m.tPtr[I]; k < m.tPtr[I + 1]; k++) {
+        const int K = m.tRow[k];
+        const double *W = U + m.blkOff[m.tBlk[k]];  // U_KI, row-major
+        double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5], w6 = W[6], w7 = W[7], w8 = W[8];
+        // diagonal: S_II -= W^T W
+        s00 -= w0 * w0 + w3 * w3 + w6 * w6;
+        s01 -= w0 * w1 + w3 * w4 + w6 * w7;
+        s02 -= w0 * w2 + w3 * w5 + w6 * w8;
+        s11 -= w1 * w1 + w4 * w4 + w7 * w7;
+        s12 -= w1 * w2 + w4 * w5 + w7 * w8;
+        s22 -= w2 * w2 + w5 * w5 + w8 * w8;
+        // off-diagonal: merge row K's columns (> I) with row I's columns
+        int p = m.tBlk[k] + 1, pe = m.rowPtr[K + 1];
+        int q = b0 + 1;
+        while (p < pe && q < b1) {
+            const int cp = m.colIdx[p], cq = m.colIdx[q];
+            if (cp < cq) { p++; continue; }
+            if (cq < cp) { q++; continue; }
+            const double *Z = U + m.blkOff[p];
+            double *S = U + m.blkOff[q];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) S[i * 3 + j] -= W[0 * 3 + i] * Z[0 * 3 + j] + W[1 * 3 + i] * Z[1 * 3 + j] + W[2 * 3 + i] * Z[2 * 3 + j];
+            p++; q++;
+        }
+    }
+    // dense Cholesky of the 3x3 diagonal block, pivot rule d <= 0 -> eps^2 d^2
+    double u00 = sqrt(pivot_fix(s00, eps2));
+    double u01 = s01 / u00, u02 = s02 / u00;
+    double u11 = sqrt(pivot_fix(s11 - u01 * u01, eps2));
+    double u12 = (s12 - u01 * u02) / u11;
+    double u22 = sqrt(pivot_fix(s22 - u02 * u02 - u12 * u12, eps2));
+    U[v0] = u00; U[v0 + 1] = u01; U[v0 + 2] = u02; U[v0 + 3] = u11; U[v0 + 4] = u12; U[v0 + 5] = u22;
+    // U_IJ = U_II^-T S_IJ (forward substitution with the lower factor U_II^T)
+    for (int b = b0 + 1; b < b1; b++) {
+        double *S = U + m.blkOff[b];
+        double x0[3], x1[3], x2[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            x0[j] = S[j] / u00;
+            x1[j] = (S[3 + j] - u01 * x0[j]) / u11;
+            x2[j] = (S[6 + j] - u02 * x0[j] - u12 * x1[j]) / u22;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) { S[j] = x0[j]; S[3 + j] = x1[j]; S[6 + j] = x2[j]; }
+    }
+    if (U32)
+        for (int k = v0; k < v1; k++) U32[k] = (T)U[k];
+}
+
+template <typename T>
+void launch_ic0_factor(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const double *A, double *U,
+                       T *U32, double pivotEps, const uint8_t *skip) {
+    const double eps2 = pivotEps * pivotEps;
+    for (size_t l = 0; l + 1 < levelPtr.size(); l++) {
+        int cnt = levelPtr[l + 1] - levelPtr[l];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_ic0_factor_level<T>, dim3((cnt + 127) / 128), dim3(128), 0, s, m, m.levelRows + levelPtr[l],
+                           cnt, A, U, U32, eps2, skip);
+    }
+    OD_HIP(hipGetLastError());
+}
+
+// forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_K U_KI^T y_K)
+template <typename T>
+__global__ void k_ic0_fwd(DevMesh m, const int *__restrict__ rows, int count, const T *__restrict__ U,
+                          const T *__restrict__ r, T *__restrict__ y, const InstScalars *__restrict__ sc,
+                          const uint8_t *__restrict__ skip) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int I = rows[t];
+    if (sc && sc[m.rowInst[I]].done) return;
+    if (skip && skip[m.rowInst[I]]) return;
+    T a0 = r[3 * I], a1 = r[3 * I + 1], a2 = r[3 * I + 2];
+    for (int k = m.tPtr[I]; k < m.tPtr[I + 1]; k++) {
+        const int K = m.tRow[k];
+        const T *W = U + m.blkOff[m.tBlk[k]];
+        const T y0 = y[3 * K], y1 = y[3 * K + 1], y2 = y[3 * K + 2];
+        a0 -= W[0] * y0 + W[3] * y1 + W[6] * y2;
+        a1 -= W[1] * y0 + W[4] * y1 + W[7] * y2;
+        a2 -= W[2] * y0 + W[5] * y1 + W[8] * y2;
+    }
+    const T *D = U + m.blkOff[m.rowPtr[I]];
+    T z0 = a0 / D[0];
+    T z1 = (a1 - D[1] * z0) / D[3];
+    T z2 = (a2 - D[2] * z0 - D[4] * z1) / D[5];
+    y[3 * I] = z0; y[3 * I + 1] = z1; y[3 * I + 2] = z2;
+}
+
+// backward: U z = y  ->  z_I = U_II^-1 (y_I - sum_{J>I} U_IJ z_J)
+template <typename T>
+__global__ void k_ic0_bwd(DevMesh m, const int *__restrict__ rows, int count, const T *__restrict__ U, T *__restrict__ z,
+                          const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int I = rows[t];
+    if (sc && sc[m.rowInst[I]].done) return;
+    if (skip && skip[m.rowInst[I]]) return;
+    T a0 = z[3 * I], a1 = z[3 * I + 1], a2 = z[3 * I + 2];
+    const int b0 = m.rowPtr[I];
+    for (int b = b0 + 1; b < m.rowPtr[I + 1]; b++) {
+        const int J = m.colIdx[b];
+        const T *W = U + m.blkOff[b];
+        const T z0 = z[3 * J], z1 = z[3 * J + 1], z2 = z[3 * J + 2];
+        a0 -= W[0] * z0 + W[1] * z1 + W[2] * z2;
+        a1 -= W[3] * z0 + W[4] * z1 + W[5] * z2;
+        a2 -= W[6] * z0 + W[7] * z1 + W[8] * z2;
+    }
+    const T *D = U + m.blkOff[b0];
+    T x2 = a2 / D[5];
+    T x1 = (a1 - D[4] * x2) / D[3];
+    T x0 = (a0 - D[1] * x1 - D[2] * x2) / D[0];
+    z[3 * I] = x0; z[3 * I + 1] = x1; z[3 * I + 2] = x2;
+}
+
+template <typename T>
+void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *U, const T *r, T *z,
+                      const InstScalars *sc, const uint8_t *skip) {
+    const int nl = (int)levelPtr.size() - 1;
+    for (int l = 0; l < nl; l++) {
+        int cnt = levelPtr[l + 1] - levelPtr[l];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_ic0_fwd<T>, dim3((cnt + 127) / 128), dim3(128), 0, s, m, m.levelRows + levelPtr[l], cnt, U, r, z, sc, skip);
+    }
+    for (int l = nl - 1; l >= 0; l--) {
+        int cnt = levelPtr[l + 1] - levelPtr[l];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(k_ic0_bwd<T>, dim3((cnt + 127) / 128), dim3(128), 0, s, m, m.levelRows + levelPtr[l], cnt, U, z, sc, skip);
+    }
+    OD_HIP(hipGetLastError());
+}
+
+// =====================================================================================
+// mixed precision and integrator updates
+// =====================================================================================
+__global__ void k_cast(const double *a, float *b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+void launch_cast_f64_f32(hipStream_t s, const double *a, float *b, int64_t n) {
+    hipLaunchKernelGGL(k_cast, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a, b, n);
+    OD_HIP(hipGetLastError());
+}
+
+// r = b - Ax (FP64), r32 = (float) r ; partials r.r, b.b ; outer done flags
+__global__ void k_outer_res(DevMesh m, const double *b, const double *Ax, double *r, float *r32, double *partials,
+                            int *counter, InstScalars *sc, int *notDone, FinalArgs fa) {
+    CHUNK_PROLOGUE
+    (void)inst;
+    double rr = 0, bb = 0;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        double ri = b[i] - Ax[i];
+        r[i] = ri;
+        r32[i] = (float)ri;
+        rr += ri * ri;
+        bb += b[i] * b[i];
+    }
+    double v[2] = {rr, bb};
+    commit_partials<2>(m, v, partials, chunk, counter, PH_OUTER_RES, sc, notDone, fa);
+}
+
+void launch_residual_f64(const DevMesh &m, hipStream_t s, const double *b, const double *Ax, double *r, float *r32,
+                         PcgWork &w, double tol, const uint8_t *preDone, int first) {
+    hipLaunchKernelGGL(k_outer_res, dim3(m.nChunks), dim3(kBlock), 0, s, m, b, Ax, r, r32, w.partials, w.counter, w.sc,
+                       w.notDone, FinalArgs{tol, first ? -1 : 0, preDone});
+    OD_HIP(hipGetLastError());
+}
+
+__global__ void k_add_corr(DevMesh m, const float *dx, double *x, const InstScalars *outer) {
+    CHUNK_PROLOGUE
+    if (outer[inst].done) return;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) x[i] += (double)dx[i];
+}
+void launch_add_correction(const DevMesh &m, hipStream_t s, const float *dx, double *x, const InstScalars *outer) {
+    hipLaunchKernelGGL(k_add_corr, dim3(m.nChunks), dim3(kBlock), 0, s, m, dx, x, outer);
+    OD_HIP(hipGetLastError());
+}
+
+// Newmark corrector (linearNewmark.cpp:68-69, 76-77), predictor recomputed from the old state
+__global__ void k_newmark_correct(DevMesh m, double *d, double *v, const double *aOld, const double *aNew, double h,
+                                  double beta, double gamma, const uint8_t *instActive) {
+    CHUNK_PROLOGUE
+    if (instActive && !instActive[inst]) return;
+    const double bh2 = beta * h * h, gh = gamma * h, mbh2 = h * h * 0.5 - bh2, mgh = h - gh;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        double ao = aOld[i], an = aNew[i];
+        double dp = d[i] + h * v[i] + mbh2 * ao;
+        double vp = v[i] + mgh * ao;
+        v[i] = vp + gh * an;
+        d[i] = dp + bh2 * an;
+    }
+}
+void launch_newmark_correct(const DevMesh &m, hipStream_t s, double *d, double *v, const double *aOld,
+                            const double *aNew, double h, double beta, double gamma, const uint8_t *instActive) {
+    hipLaunchKernelGGL(k_newmark_correct, dim3(m.nChunks), dim3(kBlock), 0, s, m, d, v, aOld, aNew, h, beta, gamma, instActive);
+    OD_HIP(hipGetLastError());
+}
+
+__global__ void k_axpy_rows(DevMesh m, double *y, double a, const double *x, const uint8_t *instActive) {
+    CHUNK_PROLOGUE
+    if (instActive && !instActive[inst]) return;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) y[i] += a * x[i];
+}
+void launch_axpy_rows(const DevMesh &m, hipStream_t s, double *y, double a, const double *x, const uint8_t *instActive) {
+    hipLaunchKernelGGL(k_axpy_rows, dim3(m.nChunks), dim3(kBlock), 0, s, m, y, a, x, instActive);
+    OD_HIP(hipGetLastError());
+}
+void launch_be_correct(const DevMesh &m, hipStream_t s, double *v, const double *dv, const uint8_t *instActive) {
+    launch_axpy_rows(m, s, v, 1.0, dv, instActive);
+}
+
+__global__ void k_done_from_active(int nInst, uint8_t *preDone, const uint8_t *active) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nInst) preDone[i] = active ? (active[i] ? 0 : 1) : 0;
+}
+void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDone, const uint8_t *instActive) {
+    hipLaunchKernelGGL(k_done_from_active, dim3((m.nInst + 255) / 256), dim3(256), 0, s, m.nInst, preDone, instActive);
+    OD_HIP(hipGetLastError());
+}
+
+// explicit instantiations
+#define INST_T(T)                                                                                                     \
+    template void pcg_init_residual<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &);           \
+    template void pcg_jacobi<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &);                  \
+    template void pcg_dot_rz_init<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &, double, int, \
+                                     const uint8_t *);                                                                \
+    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &);                   \
+    template void pcg_update_xr<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, T *, const T *, T *,      \
+                                   PcgWork &);                                                                        \
+    template void pcg_dot_rz<T>(const DevMesh &, hipStream_t, const T *, const T *, PcgWork &);                       \
+    template void pcg_update_p<T>(const DevMesh &, hipStream_t, const T *, T *, PcgWork &);                           \
+    template void launch_jacobi_diag<T>(const DevMesh &, hipStream_t, const double *, T *);                           \
+    template void launch_ic0_apply<T>(const DevMesh &, hipStream_t, const std::vector<int> &, const T *, const T *,   \
+                                      T *, const InstScalars *, const uint8_t *);
+INST_T(double)
+INST_T(float)
+template void launch_ic0_factor<double>(const DevMesh &, hipStream_t, const std::vector<int> &, const double *, double *,
+                                        double *, double, const uint8_t *);
+template void launch_ic0_factor<float>(const DevMesh &, hipStream_t, const std::vector<int> &, const double *, double *,
+                                       float *, double, const uint8_t *);
+
+}  // namespace od

@@ -1,0 +1,68 @@
+// Host-side set-up of the device-resident mesh: DOF indexing, the canonical
+// blocked-symmetric pattern, the per-element 78-entry map, the block-level
+// assembly map, the transposed block index used by the SpMV and the IC(0)
+// sweeps, the IC(0) level schedule and the per-instance row chunks.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace od {
+
+struct Topology {
+    // ---- inputs -------------------------------------------------------------
+    int nNodes = 0, nTets = 0;
+    std::vector<double> X;              // 3 per node
+    std::vector<int> tets;              // 4 per tet
+    std::vector<int> constrained;       // sorted constrained DOF ids (3*node+axis)
+    std::vector<int> instNodeStart;     // instance node ranges (size nInst+1)
+    std::vector<int> instTetStart;      // instance tet ranges (size nInst+1)
+
+    // ---- DOF indexing (core/nodeIndexer.cpp:12-25 semantics) ----------------
+    int ndof = 0;                       // free DOFs
+    int nRows = 0;                      // block rows = free nodes (whole-node constraints)
+    std::vector<int> nodeRow;           // node -> block row, or -1 when constrained
+    std::vector<int> rowNode;           // block row -> node
+
+    // ---- canonical pattern (core/sparseMatrix.h:186-274 layout) -------------
+    std::vector<int> rowPtr;            // blockPcol: size nRows+1
+    std::vector<int> colIdx;            // block column of each stored block (blockRows/3)
+    std::vector<int> valOff;            // blockColumnOris: size nRows+1
+    std::vector<int> blkOff;            // first value index of each block
+    int64_t nBlocks = 0, nValues = 0;
+
+    // ---- transposed index: for row I, blocks (K, I) with K < I -------------
+    std::vector<int> tPtr, tRow, tBlk;
+
+    // ---- element maps ---------------------------------------------------------
+    std::vector<int> elemMap78;         // 78 per tet (fullOrderNonlinear.h:73-110)
+    std::vector<int> pairBlk;           // 10 per tet: destination block or -1
+    std::vector<uint8_t> pairT;         // 10 per tet: 1 when the element block lands transposed
+    std::vector<int> pairSlot;          // 10 per tet: contribution slot (destination-sorted) or -1
+    std::vector<int> blkCStart;         // size nBlocks+1: contribution range of each block
+    std::vector<int> nodeSlot;          // 4 per tet: node-force contribution slot or -1
+    std::vector<int> rowCStart;         // size nRows+1: node-force contribution range per row
+    int64_t nPairContrib = 0, nNodeContrib = 0;
+
+    // ---- IC(0) level schedule -------------------------------------------------
+    std::vector<int> level;             // per row
+    std::vector<int> levelPtr;          // size nLevels+1
+    std::vector<int> levelRows;         // rows sorted by level (stable)
+    int nLevels = 0;
+
+    // ---- instances and row chunks ---------------------------------------------
+    int nInst = 1;
+    std::vector<int> instRowStart;      // size nInst+1
+    std::vector<int> instChunkStart;    // size nInst+1
+    std::vector<int> chunkRow0, chunkRow1, chunkInst;
+    std::vector<int> rowInst;           // per row
+    std::vector<int> tetInst;           // per tet
+};
+
+// Builds every derived structure.  Returns an empty string or an error text.
+std::string buildTopology(Topology &t, int chunkRows);
+
+// Global DOF of (node, axis): -1 when constrained (nodeIndexer.cpp:12-25).
+int globalIndex(const Topology &t, int node, int axis);
+
+}  // namespace od
