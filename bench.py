@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Implicit Newmark step throughput at 1M DOF on one MI355X.
+
+Workload (default): 200x40x40 Kuhn box, 5x1x1 m, 1,920,000 tets, 1,008,600
+free DOF, corotational material with per-tet E log-uniform in [1e4, 1e6],
+nu = 0.45, rho = 1000, x = 0 face fixed, gravity, Newmark beta = 1/4,
+gamma = 1/2, dt = 1/60, PCG to 1e-6 relative residual.  A "step" is one full
+Newmark step on the device: element pass, assembly, matrix combine, IC(0)
+factorisation, PCG to tolerance, corrector.
+
+Prints one JSON line (rank 0).  Extra fields: SpMV roofline from HIP events
+on the library stream over the timed steps, and the CPU FP64 oracle timed on
+a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Newmark steps/s + SpMV GB/s at 1M DOF, rel.res 1e-6; p50/p99 served step latency"
+
+
+def spmv_bytes(mesh, value_bytes=8, vec_bytes=8):
+    """Algorithmic bytes of one SpMV over the upper-symmetric canonical layout:
+    values + block column indices + block row pointers + x read + y write."""
+    return value_bytes * mesh.n_values + 4 * mesh.n_blocks + 4 * (mesh.n_rows + 1) + 2 * vec_bytes * mesh.ndof
+
+
+def cpu_baseline(args, gpu_iters_per_step, full_ndof):
+    """Times the FP64 oracle on a slab of the same workload (single thread)."""
+    import oracle as O
+    from opendeformer_amd import workloads as W
+    # slab: the first 20 of the 200 cells along x, same cross-section, seed and ordering
+    from opendeformer_amd import mesh as M
+    nx = 20
+    m = M.box_mesh(nx, 40, 40, 5.0 * nx / 200, 1.0, 1.0, seed=3)
+    if args.ordering == "color8":
+        m = M.relabel(m, M.multicolor_permutation(m))
+    E = M.log_uniform(3, M.STREAM_YOUNG, np.arange(m.n_tets, dtype=np.uint64), 1e4, 1e6)
+    fixed = M.fixed_dofs_from_nodes(M.plane_fixed_nodes(m))
+    t0 = time.perf_counter()
+    s = O.System(m.X, m.tets, fixed)
+    t_setup = time.perf_counter() - t0
+    body = np.tile([0.0, 0.0, -9.81 * 1000.0], (m.n_tets, 1))
+    pre = {"jacobi": O.PRE_JACOBI, "ic0": O.PRE_IC0}[args.precond]
+    nm = O.Newmark(s, 0, E, 0.45, 1000.0, 1.0 / 60, pre=pre, p1=1e-3)
+    nm.set_external(s.body_force(body))
+    nm.step(1)                                           # warm-up (includes the initial-acceleration solve)
+    t0 = time.perf_counter()
+    its, _, _ = nm.step(2)
+    t_step = (time.perf_counter() - t0) / 2
+    # split: element pass + assembly vs. PCG iterations
+    t0 = time.perf_counter()
+    s.assemble(0, E, 0.45)
+    t_asm = time.perf_counter() - t0
+    it_s = float(np.mean(its))
+    t_iter = max(t_step - t_asm, 0.0) / max(it_s, 1.0)
+    ratio = full_ndof / s.ndof
+    t_full = ratio * (t_asm + t_iter * gpu_iters_per_step)
+    return {"value": 1.0 / t_full, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": (f"FP64 oracle, 1 thread, on a {nx}x40x40 slab of the same workload ({s.ndof} DOF, seed 3, "
+                       f"{args.ordering} ordering, {args.precond}); 2 timed steps after 1 warm-up: "
+                       f"{t_step:.2f} s/step ({it_s:.0f} PCG iterations), element+assembly {t_asm:.2f} s; scaled to "
+                       f"{full_ndof} DOF linearly in DOF and to the GPU's {gpu_iters_per_step:.0f} iterations/step; "
+                       f"oracle set-up {t_setup:.1f} s not counted; host {platform.processor() or platform.machine()}")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precond", choices=["jacobi", "ic0"], default="ic0")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--ordering", choices=["natural", "color8"], default="color8")
+    ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = 200x40x40)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(every box here has one GPU; see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import opendeformer_amd as od
+    from opendeformer_amd import capi
+    from opendeformer_amd import workloads as W
+
+    w = W.box_1m_dof(args.ordering, scale=args.scale)
+    ctx = od.Context(0)
+    mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
+    mesh.set_material(w.kind, w.E, w.nu, w.rho)
+    f = mesh.body_force(w.body)
+    pre = {"jacobi": capi.OD_PRE_JACOBI, "ic0": capi.OD_PRE_IC0}[args.precond]
+    prec = {"fp64": capi.OD_FP64, "mixed": capi.OD_MIXED}[args.precision]
+    nm = od.NewmarkIntegrator(mesh, w.dt, beta=0.25, gamma=0.5, precond=pre, precision=prec, tol=1e-6)
+    nm.set_external_virtual_work(f)
+    nm.run_time_steps(args.warmup)
+    ctx.sync()
+
+    ctx.profile(True)
+    iters, relres, outer = [], [], []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nm.run_one_time_step()
+        it, rr, ou, _ = nm.stats()
+        iters.append(int(it[0])); relres.append(float(rr[0])); outer.append(ou)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    spmv_ms, spmv_n = ctx.profile_read()
+    ctx.profile(False)
+
+    ms_step = elapsed / args.steps * 1e3
+    steps_s = args.steps / elapsed
+    vb = 4 if args.precision == "mixed" else 8
+    by = spmv_bytes(mesh, vb, vb)
+    avg_spmv_s = (spmv_ms / max(spmv_n, 1)) / 1e3
+    achieved = by / avg_spmv_s / 1e9 if spmv_n else 0.0
+    line = {
+        "metric": METRIC, "value": round(steps_s, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32-inner/f64-refinement",
+        "data": "synthetic (seeded box mesh, seed 3)",
+        "config": {"workload": f"{w.name}: {mesh.n_tets} tets, {mesh.ndof} DOF, Newmark beta=1/4 gamma=1/2 dt=1/60, "
+                               f"PCG tol 1e-6 relative residual",
+                   "precond": args.precond, "precision": args.precision, "ordering": args.ordering,
+                   "ic0_levels": mesh.n_levels},
+        "pcg_iters_per_step": round(float(np.mean(iters)), 2),
+        "relres_max": float(np.max(relres)),
+        "spmv": {"kernel": "k_spmv_dot" if vb == 8 else "k_spmv_dot<float>", "launches": spmv_n,
+                 "avg_us": round(avg_spmv_s * 1e6, 2), "algorithmic_MB": round(by / 1e6, 2),
+                 "GBps": round(achieved, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(args, float(np.mean(iters)), mesh.ndof)
+        except Exception as e:  # the GPU line stays valid without the CPU leg
+            line["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

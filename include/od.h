@@ -50,6 +50,11 @@ int od_version(void);
 int od_ctx_create(int device, od_ctx **out);
 int od_ctx_destroy(od_ctx *ctx);
 int od_ctx_sync(od_ctx *ctx);
+/* measurement mode (no reference counterpart): when enabled, every PCG SpMV
+ * launch (q = A p) is bracketed by HIP events on the context stream;
+ * od_ctx_profile_read returns their summed duration and count since enabling */
+int od_ctx_profile(od_ctx *ctx, int enable);
+int od_ctx_profile_read(od_ctx *ctx, double *totalMs_host, long long *launches_host);
 int od_dev_alloc(od_ctx *ctx, long long bytes, void **dev);
 int od_dev_free(od_ctx *ctx, void *dev);
 int od_copy_h2d(od_ctx *ctx, void *dev, const void *host, long long bytes);
@@ -121,6 +126,11 @@ int od_spmv(od_matrix *A, const double *x_dev, double *y_dev, int accumulate);
 int od_matrix_sync_f32(od_matrix *A);
 int od_spmv_f32(od_matrix *A, const float *x_dev, float *y_dev, int accumulate);
 
+/* Measurement helper (no reference counterpart): launches the SpMV `iters`
+ * times between two HIP events on the context stream; *avg_ms = mean
+ * duration per launch.  fp32 selects the FP32 copy (x, y float). */
+int od_spmv_timed(od_matrix *A, const void *x_dev, void *y_dev, int iters, int fp32, double *avg_ms);
+
 /* ---- PCG ---------------------------------------------------------------------
  * Replaces CGSolver::solveLinearSystem (solver/cgSolver.cpp:13-57) with its
  * preconditioner (JacobiPreconditioner, solver/jacobiPreconditioner.cpp:6-26;
@@ -159,6 +169,9 @@ int od_integrator_set_active(od_integrator *it, const unsigned char *active_host
 int od_integrator_step(od_integrator *it, int nSteps);
 int od_integrator_get_state(od_integrator *it, double *d_host, double *v_host, double *a_host);
 int od_integrator_set_state(od_integrator *it, const double *d_host, const double *v_host, const double *a_host);
+/* the integrator's effective matrix A = M + gamma h C + beta h^2 K of the last
+ * step (owned by the integrator; not to be destroyed by the caller) */
+int od_integrator_matrix(od_integrator *it, od_matrix **A);
 /* device pointers of the state vectors (d, v, a) for device-side consumers */
 int od_integrator_state_dev(od_integrator *it, double **d_dev, double **v_dev, double **a_dev);
 /* last step: PCG iterations and relative residual per instance, outer

@@ -29,7 +29,7 @@ EXPORTS = [
     "od_newmark_create", "od_backward_euler_create", "od_integrator_destroy", "od_integrator_set_external",
     "od_integrator_set_impulse", "od_integrator_set_active", "od_integrator_step", "od_integrator_get_state",
     "od_integrator_set_state", "od_integrator_state_dev", "od_integrator_stats", "od_backward_euler_update_mesh",
-    "od_integrator_positions",
+    "od_integrator_positions", "od_spmv_timed", "od_integrator_matrix", "od_ctx_profile", "od_ctx_profile_read",
 ]
 
 
@@ -69,6 +69,8 @@ def lib():
             "od_integrator_get_state": [vp, dp, dp, dp], "od_integrator_set_state": [vp, dp, dp, dp],
             "od_integrator_state_dev": [vp, pvp, pvp, pvp], "od_integrator_stats": [vp, ip, dp, ip, ip],
             "od_backward_euler_update_mesh": [vp], "od_integrator_positions": [vp, C.POINTER(C.c_float)],
+            "od_spmv_timed": [vp, vp, vp, i, i, dp], "od_integrator_matrix": [vp, pvp],
+            "od_ctx_profile": [vp, i], "od_ctx_profile_read": [vp, dp, C.POINTER(ll)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

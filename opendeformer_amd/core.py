@@ -28,6 +28,15 @@ class Context:
     def sync(self):
         check(lib().od_ctx_sync(self.h))
 
+    def profile(self, enable: bool):
+        """HIP-event timing of every PCG SpMV launch on this context's stream."""
+        check(lib().od_ctx_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        ms = C.c_double(0); n = C.c_longlong(0)
+        check(lib().od_ctx_profile_read(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def close(self):
         if self.h:
             lib().od_ctx_destroy(self.h)
@@ -268,6 +277,11 @@ class _Integrator:
         outer = C.c_int(0); init = C.c_int(0)
         check(lib().od_integrator_stats(self.h, capi.iptr(it), dptr(rr), C.byref(outer), C.byref(init)))
         return it, rr, outer.value, init.value
+
+    def matrix_handle(self):
+        A = C.c_void_p()
+        check(lib().od_integrator_matrix(self.h, C.byref(A)))
+        return A
 
     def vertex_positions(self):
         pos = np.zeros(3 * self.mesh.n_nodes, np.float32)

@@ -43,7 +43,32 @@ struct od_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int *pinned = nullptr;   // small host-pinned mailbox for convergence polling
+    // optional HIP-event timing of the PCG SpMV launches (measurement mode)
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    size_t evUsed = 0;
+    double profMs = 0.0;
+    long long profCount = 0;
 };
+
+// Accumulates the elapsed time of completed event pairs (caller has synchronised the stream).
+static void prof_flush(od_ctx *c) {
+    for (size_t k = 0; k + 1 < c->evUsed; k += 2) {
+        float ms = 0;
+        OD_HIP(hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]));
+        c->profMs += ms;
+        c->profCount++;
+    }
+    c->evUsed = 0;
+}
+static void prof_mark(od_ctx *c) {
+    if (!c->prof) return;
+    if (c->evUsed == c->ev.size()) {
+        OD_HIP(hipEventSynchronize(c->ev.back()));
+        prof_flush(c);
+    }
+    OD_HIP(hipEventRecord(c->ev[c->evUsed++], c->stream));
+}
 
 struct od_mesh {
     od_ctx *ctx = nullptr;
@@ -200,6 +225,7 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
 static int read_flag(od_ctx *ctx, const int *dev) {
     OD_HIP(hipMemcpyAsync(ctx->pinned, dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     OD_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_flush(ctx);
     return *ctx->pinned;
 }
 
@@ -238,7 +264,9 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *invDiag, 
         if (read_flag(mesh->ctx, notDone) == 0) break;
         const int k = std::min(batch, maxIt - launched);
         for (int j = 0; j < k; j++) {
+            prof_mark(mesh->ctx);
             pcg_alpha<T>(m, st, vals, p, q, w);
+            prof_mark(mesh->ctx);
             if (ic) {
                 pcg_update_xr<T>(m, st, p, q, x, r, (const T *)nullptr, (T *)nullptr, w);
                 launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, r, z, sc, preDone);
@@ -339,6 +367,7 @@ int od_ctx_destroy(od_ctx *ctx) {
     API_BEGIN
     if (!ctx) return OD_OK;
     (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipHostFree(ctx->pinned);
     delete ctx;
@@ -349,6 +378,31 @@ int od_ctx_sync(od_ctx *ctx) {
     API_BEGIN
     require(ctx != nullptr, "ctx is NULL");
     OD_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_flush(ctx);
+    API_END
+}
+
+int od_ctx_profile(od_ctx *ctx, int enable) {
+    API_BEGIN
+    require(ctx != nullptr, "ctx is NULL");
+    OD_HIP(hipStreamSynchronize(ctx->stream));
+    if (enable && ctx->ev.empty()) {
+        ctx->ev.resize(512);
+        for (auto &e : ctx->ev) OD_HIP(hipEventCreate(&e));
+    }
+    if (ctx->prof) prof_flush(ctx);
+    ctx->prof = enable != 0;
+    if (enable) { ctx->profMs = 0; ctx->profCount = 0; ctx->evUsed = 0; }
+    API_END
+}
+
+int od_ctx_profile_read(od_ctx *ctx, double *totalMs, long long *launches) {
+    API_BEGIN
+    require(ctx && totalMs && launches, "NULL argument");
+    OD_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_flush(ctx);
+    *totalMs = ctx->profMs;
+    *launches = ctx->profCount;
     API_END
 }
 
@@ -655,6 +709,36 @@ int od_spmv_f32(od_matrix *A, const float *x, float *y, int accumulate) {
     API_END
 }
 
+int od_spmv_timed(od_matrix *A, const void *x, void *y, int iters, int fp32, double *avg_ms) {
+    API_BEGIN
+    require(A && x && y && avg_ms && iters > 0 && x != y, "bad arguments");
+    require(!fp32 || A->vals32, "call od_matrix_sync_f32 first", OD_ERR_STATE);
+    hipStream_t st = A->mesh->ctx->stream;
+    hipEvent_t e0, e1;
+    OD_HIP(hipEventCreate(&e0));
+    OD_HIP(hipEventCreate(&e1));
+    OD_HIP(hipEventRecord(e0, st));
+    for (int k = 0; k < iters; k++) {
+        if (fp32) launch_spmv<float>(A->mesh->dm, st, A->vals32, (const float *)x, (float *)y, 0, nullptr, nullptr);
+        else launch_spmv<double>(A->mesh->dm, st, A->vals, (const double *)x, (double *)y, 0, nullptr, nullptr);
+    }
+    OD_HIP(hipEventRecord(e1, st));
+    OD_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    OD_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / iters;
+    API_END
+}
+
+int od_integrator_matrix(od_integrator *it, od_matrix **A) {
+    API_BEGIN
+    require(it && A, "NULL argument");
+    *A = &it->A;
+    API_END
+}
+
 int od_solver_create(od_mesh *mesh, int precond, double param, int precision, od_solver **out) {
     API_BEGIN
     require(mesh && out, "NULL argument");
@@ -860,6 +944,7 @@ int od_integrator_step(od_integrator *it, int nSteps) {
     require(it != nullptr && nSteps >= 0, "bad arguments");
     for (int k = 0; k < nSteps; k++) integrator_step(it);
     OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    if (it->mesh->ctx->prof) prof_flush(it->mesh->ctx);
     API_END
 }
 

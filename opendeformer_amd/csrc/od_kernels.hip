@@ -537,6 +537,31 @@ __global__ __launch_bounds__(256) void k_spmv(DevMesh m, const T *__restrict__ v
     }
 }
 
+// Same kernel under its own name for the PCG (q = A p with the p.q partial), so profiles separate the two.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spmv_dot(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
+                                                  T *__restrict__ y, double *partials, int *counter, InstScalars *sc,
+                                                  int *notDone) {
+    const int chunk = blockIdx.x;
+    const int r0 = m.chunkRow0[chunk], r1 = m.chunkRow1[chunk];
+    const int inst = m.chunkInst[chunk];
+    const bool skip = sc[inst].done;
+    const int group = threadIdx.x >> 3, lane8 = threadIdx.x & 7;
+    double dsum = 0.0;
+    if (!skip) {
+        for (int r = r0 + group; r < r1; r += (int)(blockDim.x >> 3)) {
+            T o0, o1, o2;
+            row_spmv<T>(m, vals, x, r, lane8, o0, o1, o2);
+            if (lane8 == 0) {
+                y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
+                dsum += (double)x[3 * r] * (double)o0 + (double)x[3 * r + 1] * (double)o1 + (double)x[3 * r + 2] * (double)o2;
+            }
+        }
+    }
+    double v[1] = {dsum};
+    commit_partials<1>(m, v, partials, chunk, counter, PH_ALPHA, sc, notDone, FinalArgs{0, 0, nullptr});
+}
+
 template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate, double *partials,
                  const InstScalars *sc) {
@@ -619,8 +644,8 @@ void pcg_dot_rz_init(const DevMesh &m, hipStream_t s, const T *r, const T *z, T 
 
 template <typename T>
 void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w) {
-    hipLaunchKernelGGL(k_spmv<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, vals, p, q, 0, w.partials, w.counter, w.sc,
-                       w.notDone, 1);
+    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, vals, p, q, w.partials, w.counter, w.sc,
+                       w.notDone);
     OD_HIP(hipGetLastError());
 }
 
