@@ -42,7 +42,8 @@ __global__ void k_done_to_mask(int n, const InstScalars *sc, const uint8_t *extr
 struct od_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    int *pinned = nullptr;   // small host-pinned mailbox for convergence polling
+    int *pinned = nullptr;   // host-pinned mailbox for convergence polling
+    int pinnedCount = 0;
     // optional HIP-event timing of the PCG SpMV launches (measurement mode)
     bool prof = false;
     std::vector<hipEvent_t> ev;
@@ -127,7 +128,7 @@ struct od_solver {
     float *invDiag32 = nullptr, *U32 = nullptr;
     double *Ax = nullptr;
     double *partials = nullptr;
-    int *counter = nullptr, *notDoneIn = nullptr, *notDoneOut = nullptr;
+    int *doneIn = nullptr, *doneOut = nullptr;   // per-instance done flags (device)
     InstScalars *scIn = nullptr, *scOut = nullptr;
     uint8_t *mask = nullptr, *zeroMask = nullptr;
     std::vector<int> iters;
@@ -202,10 +203,10 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
             if (precond == OD_PRE_IC0) s->U32 = (float *)s->alloc(4 * nv);
         }
         s->partials = (double *)s->alloc(sizeof(double) * 4 * (size_t)t.chunkRow0.size());
-        s->counter = (int *)s->alloc(sizeof(int) * 4);
-        OD_HIP(hipMemset(s->counter, 0, sizeof(int) * 4));
-        s->notDoneIn = s->counter + 1;
-        s->notDoneOut = s->counter + 2;
+        s->doneIn = (int *)s->alloc(sizeof(int) * t.nInst);
+        s->doneOut = (int *)s->alloc(sizeof(int) * t.nInst);
+        OD_HIP(hipMemset(s->doneIn, 0, sizeof(int) * t.nInst));
+        OD_HIP(hipMemset(s->doneOut, 0, sizeof(int) * t.nInst));
         s->scIn = (InstScalars *)s->alloc(sizeof(InstScalars) * t.nInst);
         s->scOut = (InstScalars *)s->alloc(sizeof(InstScalars) * t.nInst);
         OD_HIP(hipMemset(s->scIn, 0, sizeof(InstScalars) * t.nInst));
@@ -222,11 +223,19 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
     return s;
 }
 
-static int read_flag(od_ctx *ctx, const int *dev) {
-    OD_HIP(hipMemcpyAsync(ctx->pinned, dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+// Number of instances whose done flag is still 0 (synchronises the stream).
+static int count_not_done(od_ctx *ctx, const int *doneDev, int nInst) {
+    if (ctx->pinnedCount < nInst) {
+        if (ctx->pinned) OD_HIP(hipHostFree(ctx->pinned));
+        OD_HIP(hipHostMalloc((void **)&ctx->pinned, sizeof(int) * (size_t)nInst, hipHostMallocDefault));
+        ctx->pinnedCount = nInst;
+    }
+    OD_HIP(hipMemcpyAsync(ctx->pinned, doneDev, sizeof(int) * (size_t)nInst, hipMemcpyDeviceToHost, ctx->stream));
     OD_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->prof) prof_flush(ctx);
-    return *ctx->pinned;
+    int n = 0;
+    for (int i = 0; i < nInst; i++) n += ctx->pinned[i] ? 0 : 1;
+    return n;
 }
 
 // Builds the preconditioner from A (Preconditioner::Preprocess, cgSolver.cpp:23).
@@ -246,11 +255,11 @@ static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip)
 // PCG on the chunked instances: x in/out, b fixed.  preDone marks instances to skip.
 template <typename T>
 static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *invDiag, const T *b, T *x, T *r, T *z, T *p, T *q,
-                    double tol, int maxIt, const uint8_t *preDone, InstScalars *sc, int *notDone, bool xZero) {
+                    double tol, int maxIt, const uint8_t *preDone, InstScalars *sc, int *doneFlags, bool xZero) {
     od_mesh *mesh = s->mesh;
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
-    PcgWork w{s->partials, sc, s->counter, notDone};
+    PcgWork w{s->partials, sc, doneFlags};
     const size_t n = (size_t)mesh->topo.ndof;
     if (xZero) OD_HIP(hipMemsetAsync(q, 0, sizeof(T) * n, st));
     else launch_spmv<T>(m, st, vals, x, q, 0, nullptr, nullptr);
@@ -261,7 +270,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *invDiag, 
     pcg_dot_rz_init<T>(m, st, r, z, p, w, tol, maxIt, preDone);
     int launched = 0, batch = 4;
     while (launched < maxIt) {
-        if (read_flag(mesh->ctx, notDone) == 0) break;
+        if (count_not_done(mesh->ctx, doneFlags, m.nInst) == 0) break;
         const int k = std::min(batch, maxIt - launched);
         for (int j = 0; j < k; j++) {
             prof_mark(mesh->ctx);
@@ -310,7 +319,7 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     if (s->precision == OD_FP64) {
         build_precond(s, A, pre);
         run_pcg<double>(s, A->vals, s->U, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre, s->scIn,
-                        s->notDoneIn, false);
+                        s->doneIn, false);
         collect_stats(s, s->scIn, false);
         collect_relres(s, s->scIn);
         s->outer = 0;
@@ -320,17 +329,17 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     require(A->vals32 != nullptr, "mixed precision needs the FP32 copy of A (od_matrix_sync_f32)", OD_ERR_STATE);
     build_precond(s, A, pre);
     std::fill(s->iters.begin(), s->iters.end(), 0);
-    PcgWork wo{s->partials, s->scOut, s->counter, s->notDoneOut};
+    PcgWork wo{s->partials, s->scOut, s->doneOut};
     launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr, nullptr);
     launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 1);
     int outer = 0;
     for (; outer < s->maxOuter; outer++) {
-        if (read_flag(mesh->ctx, s->notDoneOut) == 0) break;
+        if (count_not_done(mesh->ctx, s->doneOut, t.nInst) == 0) break;
         hipLaunchKernelGGL(k_done_to_mask, dim3((t.nInst + 255) / 256), dim3(256), 0, st, t.nInst, s->scOut, pre, s->mask);
         OD_HIP(hipGetLastError());
         OD_HIP(hipMemsetAsync(s->x32, 0, sizeof(float) * t.ndof, st));
         run_pcg<float>(s, A->vals32, s->U32, s->invDiag32, s->r32, s->x32, s->ri32, s->z32, s->p32, s->q32, s->innerTol,
-                       maxIt, s->mask, s->scIn, s->notDoneIn, true);
+                       maxIt, s->mask, s->scIn, s->doneIn, true);
         collect_stats(s, s->scIn, true);
         launch_add_correction(m, st, s->x32, x, s->scOut);
         launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr, nullptr);
@@ -358,7 +367,8 @@ int od_ctx_create(int device, od_ctx **out) {
     od_ctx *c = new od_ctx();
     c->device = device;
     OD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    OD_HIP(hipHostMalloc((void **)&c->pinned, 64, hipHostMallocDefault));
+    OD_HIP(hipHostMalloc((void **)&c->pinned, 64 * sizeof(int), hipHostMallocDefault));
+    c->pinnedCount = 64;
     *out = c;
     API_END
 }
@@ -479,6 +489,10 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         for (int r = 0; r < t.nRows; r++)
             for (int b = t.rowPtr[r]; b < t.rowPtr[r + 1]; b++) blkRow[b] = r;
         m.blkRow = mesh->upload(blkRow);
+        m.valOff = mesh->upload(t.valOff);
+        std::vector<int> tOff(t.tBlk.size());
+        for (size_t k = 0; k < t.tBlk.size(); k++) tOff[k] = t.blkOff[t.tBlk[k]];
+        m.tOff = mesh->upload(tOff);
         m.tPtr = mesh->upload(t.tPtr);
         m.tRow = mesh->upload(t.tRow);
         m.tBlk = mesh->upload(t.tBlk);
@@ -774,7 +788,7 @@ int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
     require(s && A && r && z && A->mesh == s->mesh, "bad arguments");
     od_mesh *mesh = s->mesh;
     build_precond(s, A, s->zeroMask);
-    PcgWork w{s->partials, s->scIn, s->counter, s->notDoneIn};
+    PcgWork w{s->partials, s->scIn, s->doneIn};
     if (s->precond == OD_PRE_IC0) launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U, r, z, nullptr, nullptr);
     else pcg_jacobi<double>(mesh->dm, mesh->ctx->stream, s->precond == OD_PRE_JACOBI ? s->invDiag : nullptr, r, z, w);
     OD_HIP(hipStreamSynchronize(mesh->ctx->stream));

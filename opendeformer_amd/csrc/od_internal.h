@@ -22,7 +22,7 @@ struct HipError : std::runtime_error {
             throw ::od::HipError(std::string(#call) + " failed: " + hipGetErrorString(e_));            \
     } while (0)
 
-constexpr int kChunkRows = 256;     // block rows per workgroup chunk (never straddles an instance)
+constexpr int kChunkRows = 64;      // block rows per workgroup chunk (never straddles an instance)
 constexpr int kBlock = 256;         // threads per workgroup
 
 // Device copy of the topology and material.
@@ -33,7 +33,8 @@ struct DevMesh {
     int *tets;            // 4*nTets
     int *nodeRow;         // nNodes
     int *rowPtr, *colIdx, *blkOff, *blkRow;     // pattern
-    int *tPtr, *tRow, *tBlk;                    // transposed index
+    int *valOff;                                 // first value of each block row (nRows+1)
+    int *tPtr, *tRow, *tBlk, *tOff;             // transposed index (+ value offset of each block)
     int *pairBlk, *pairSlot; uint8_t *pairT;    // 10 per tet
     int *blkCStart;                              // nBlocks+1
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
@@ -75,10 +76,9 @@ void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *
 
 // PCG building blocks (see od_kernels.hip for the exact phase semantics)
 struct PcgWork {
-    double *partials;     // nChunks * 4
+    double *partials;     // nChunks * 4 (slots: r.z or p.q, r.r, b.b)
     InstScalars *sc;      // nInst
-    int *counter;         // last-workgroup counter
-    int *notDone;         // number of instances not done (device)
+    int *doneFlags;       // nInst, mirrors sc[i].done for host polling
 };
 template <typename T>
 void pcg_init_residual(const DevMesh &m, hipStream_t s, const T *b, const T *q, T *r, PcgWork &w);

@@ -346,7 +346,7 @@ void launch_expand_map(const DevMesh &m, hipStream_t s, int *map78) {
 }
 
 // =====================================================================================
-// reductions: block partials + last-workgroup finalisation per instance
+// reductions: per-chunk partials, then one finalisation workgroup per instance
 // =====================================================================================
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -354,27 +354,29 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Reduces NQ per-thread values over the workgroup; thread 0 gets the sums.
+// Sums NQ per-thread values over the workgroup in a fixed order; thread 0 gets the result.
 template <int NQ>
-__device__ __forceinline__ void block_sum(double (&v)[NQ], double *lds) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+__device__ __forceinline__ void block_sum(double (&v)[NQ]) {
+    __shared__ double lds[16 * 3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         double s = wave_sum(v[q]);
-        if (lane == 0) lds[w * NQ + q] = s;
+        if (lane == 0) lds[w * 3 + q] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             double s = 0;
-            for (int k = 0; k < nw; k++) s += lds[k * NQ + q];
+            for (int k = 0; k < nw; k++) s += lds[k * 3 + q];
             v[q] = s;
         }
     }
 }
 
-enum Phase { PH_INIT_RES = 0, PH_INIT_RZ = 1, PH_ALPHA = 2, PH_XR = 3, PH_XR_RZ = 4, PH_RZ = 5, PH_OUTER_RES = 6 };
+// partial slots per chunk: [0] r.z or p.q, [1] r.r, [2] b.b
+enum Phase { PH_INIT = 0, PH_ALPHA = 1, PH_ITER = 2, PH_OUTER_FIRST = 3, PH_OUTER = 4 };
 
 struct FinalArgs {
     double tol;
@@ -382,118 +384,88 @@ struct FinalArgs {
     const uint8_t *preDone;
 };
 
-// Per-instance scalar update once all chunk partials of a phase are written.
-__device__ void finalize_phase(const DevMesh &m, int phase, const double *partials, InstScalars *sc, int *notDone,
-                               FinalArgs fa) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    __shared__ int s_count[16];
-    int myCount = 0;
-    for (int inst = w; inst < m.nInst; inst += nw) {
-        const int c0 = m.instChunkStart[inst], c1 = m.instChunkStart[inst + 1];
-        double t0 = 0, t1 = 0;
-        for (int c = c0 + lane; c < c1; c += 64) { t0 += partials[4 * c + 0]; t1 += partials[4 * c + 1]; }
-        t0 = wave_sum(t0);
-        t1 = wave_sum(t1);
-        if (lane == 0) {
-            InstScalars S = sc[inst];
-            switch (phase) {
-            case PH_INIT_RES:
-                S.bb = t0; S.rr = t1;
-                break;
-            case PH_INIT_RZ:
-                S.delta = t0; S.it = 0; S.maxIt = fa.maxIt;
-                S.eps = fa.tol * fa.tol * S.bb;
-                S.done = (fa.preDone && fa.preDone[inst]) || !(S.rr > S.eps) || fa.maxIt <= 0;
-                break;
-            case PH_ALPHA:
-                if (!S.done) { S.pq = t0; S.alpha = S.delta / t0; }
-                break;
-            case PH_XR:
-                if (!S.done) S.rr = t0;
-                break;
-            case PH_XR_RZ:
-            case PH_RZ:
-                if (!S.done) {
-                    if (phase == PH_XR_RZ) { S.rr = t0; S.rz = t1; }
-                    else S.rz = t0;
-                    S.beta = S.rz / S.delta;
-                    S.delta = S.rz;
-                    S.it += 1;
-                    S.done = !(S.rr > S.eps) || S.it >= S.maxIt;
-                }
-                break;
-            case PH_OUTER_RES:
-                // outer refinement: t0 = r.r of the FP64 residual, t1 = b.b
-                S.rr = t0;
-                if (fa.maxIt < 0) S.bb = t1;                 // first residual of the solve
-                S.eps = fa.tol * fa.tol * S.bb;
-                S.done = (fa.preDone && fa.preDone[inst]) || !(S.rr > S.eps);
-                break;
-            }
-            sc[inst] = S;
-            myCount += S.done ? 0 : 1;
+// One workgroup per instance: reduce its chunk partials (fixed order) and update its scalars.
+__global__ __launch_bounds__(256) void k_finalize(DevMesh m, int phase, const double *__restrict__ partials,
+                                                  InstScalars *__restrict__ sc, int *__restrict__ doneFlags,
+                                                  FinalArgs fa) {
+    const int inst = blockIdx.x;
+    const int c0 = m.instChunkStart[inst], c1 = m.instChunkStart[inst + 1];
+    double t[3] = {0, 0, 0};
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
+        t[0] += partials[4 * c + 0];
+        t[1] += partials[4 * c + 1];
+        t[2] += partials[4 * c + 2];
+    }
+    block_sum<3>(t);
+    if (threadIdx.x != 0) return;
+    InstScalars S = sc[inst];
+    switch (phase) {
+    case PH_INIT:
+        S.delta = t[0]; S.rr = t[1]; S.bb = t[2];
+        S.it = 0; S.maxIt = fa.maxIt;
+        S.eps = fa.tol * fa.tol * S.bb;
+        S.done = (fa.preDone && fa.preDone[inst]) || !(S.rr > S.eps) || fa.maxIt <= 0;
+        break;
+    case PH_ALPHA:
+        if (!S.done) { S.pq = t[0]; S.alpha = S.delta / t[0]; }
+        break;
+    case PH_ITER:
+        if (!S.done) {
+            S.rz = t[0]; S.rr = t[1];
+            S.beta = S.rz / S.delta;
+            S.delta = S.rz;
+            S.it += 1;
+            S.done = !(S.rr > S.eps) || S.it >= S.maxIt;
         }
+        break;
+    case PH_OUTER_FIRST:
+    case PH_OUTER:
+        S.rr = t[1];
+        if (phase == PH_OUTER_FIRST) S.bb = t[2];
+        S.eps = fa.tol * fa.tol * S.bb;
+        S.done = (fa.preDone && fa.preDone[inst]) || !(S.rr > S.eps);
+        break;
     }
-    if (lane == 0) s_count[w] = myCount;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int k = 0; k < nw; k++) t += s_count[k];
-        *notDone = t;
-    }
+    sc[inst] = S;
+    doneFlags[inst] = S.done;
 }
 
-// Writes this workgroup's partials; the last workgroup to arrive finalises.
-template <int NQ>
-__device__ void commit_partials(const DevMesh &m, double (&v)[NQ], double *partials, int chunk, int *counter,
-                                int phase, InstScalars *sc, int *notDone, FinalArgs fa) {
-    __shared__ double lds[16 * 4];
-    __shared__ int s_last;
-    block_sum<NQ>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++) partials[4 * chunk + q] = v[q];
-        for (int q = NQ; q < 2; q++) partials[4 * chunk + q] = 0.0;
-        __threadfence();
-        int prev = atomicAdd(counter, 1);
-        s_last = (prev == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        finalize_phase(m, phase, partials, sc, notDone, fa);
-        if (threadIdx.x == 0) atomicExch(counter, 0);
-    }
+void launch_finalize(const DevMesh &m, hipStream_t s, int phase, PcgWork &w, double tol, int maxIt,
+                     const uint8_t *preDone) {
+    hipLaunchKernelGGL(k_finalize, dim3(m.nInst), dim3(256), 0, s, m, phase, w.partials, w.sc, w.doneFlags,
+                       FinalArgs{tol, maxIt, preDone});
+    OD_HIP(hipGetLastError());
 }
 
 // =====================================================================================
-// SpMV over the canonical upper-BSR layout, 8 lanes per block row
+// SpMV over the canonical upper-BSR layout: one workgroup = one 64-row chunk, 8 lanes per row
 // =====================================================================================
 template <typename T>
 __device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x, int r,
                                          int lane8, T &o0, T &o1, T &o2) {
-    const int b0 = m.rowPtr[r], nu = m.rowPtr[r + 1] - b0;
-    const int t0 = m.tPtr[r], nt = m.tPtr[r + 1] - t0;
+    const int v0 = m.valOff[r], b0 = m.rowPtr[r];
+    const int nOff = m.rowPtr[r + 1] - b0 - 1;          // upper off-diagonal blocks
+    const int t0 = m.tPtr[r], nt = m.tPtr[r + 1] - t0;  // transposed (lower) blocks
     T a0 = 0, a1 = 0, a2 = 0;
-    for (int it = lane8; it < nu + nt; it += 8) {
-        if (it < nu) {
-            const int b = b0 + it;
-            const T *A = vals + m.blkOff[b];
-            const int c = m.colIdx[b];
+    if (lane8 == 0) {  // packed symmetric diagonal block
+        const T *D = vals + v0;
+        const T x0 = x[3 * r], x1 = x[3 * r + 1], x2 = x[3 * r + 2];
+        a0 = D[0] * x0 + D[1] * x1 + D[2] * x2;
+        a1 = D[1] * x0 + D[3] * x1 + D[4] * x2;
+        a2 = D[2] * x0 + D[4] * x1 + D[5] * x2;
+    }
+    for (int it = lane8; it < nOff + nt; it += 8) {
+        if (it < nOff) {
+            const T *A = vals + v0 + 6 + 9 * it;
+            const int c = m.colIdx[b0 + 1 + it];
             const T x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
-            if (it == 0) {  // packed diagonal (symmetric)
-                a0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
-                a1 += A[1] * x0 + A[3] * x1 + A[4] * x2;
-                a2 += A[2] * x0 + A[4] * x1 + A[5] * x2;
-            } else {
-                a0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
-                a1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
-                a2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
-            }
+            a0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
+            a1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
+            a2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
         } else {
-            const int k = t0 + it - nu;
+            const int k = t0 + it - nOff;
+            const T *A = vals + m.tOff[k];
             const int K = m.tRow[k];
-            const T *A = vals + m.blkOff[m.tBlk[k]];
             const T x0 = x[3 * K], x1 = x[3 * K + 1], x2 = x[3 * K + 2];
             a0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
             a1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
@@ -509,64 +481,50 @@ __device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__
     o0 = a0; o1 = a1; o2 = a2;
 }
 
-// One workgroup per chunk; optional dot partial (x.y) with PH_ALPHA finalisation.
+constexpr int kSpmvThreads = 8 * kChunkRows;
+
+// y = A x (or y += A x); optional chunk partial x.y into slot 0.
 template <typename T>
-__global__ __launch_bounds__(256) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
-                                              T *__restrict__ y, int accumulate, double *partials, int *counter,
-                                              InstScalars *sc, int *notDone, int withDot) {
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
+                                                       T *__restrict__ y, int accumulate, const InstScalars *sc) {
     const int chunk = blockIdx.x;
-    const int r0 = m.chunkRow0[chunk], r1 = m.chunkRow1[chunk];
-    const int inst = m.chunkInst[chunk];
-    const bool skip = sc && sc[inst].done;
-    const int group = threadIdx.x >> 3, lane8 = threadIdx.x & 7;
-    double dsum = 0.0;
-    if (!skip) {
-        for (int r = r0 + group; r < r1; r += (int)(blockDim.x >> 3)) {
-            T o0, o1, o2;
-            row_spmv<T>(m, vals, x, r, lane8, o0, o1, o2);
-            if (lane8 == 0) {
-                if (accumulate) { o0 += y[3 * r]; o1 += y[3 * r + 1]; o2 += y[3 * r + 2]; }
-                y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
-                if (withDot) dsum += (double)x[3 * r] * (double)o0 + (double)x[3 * r + 1] * (double)o1 + (double)x[3 * r + 2] * (double)o2;
-            }
-        }
-    }
-    if (withDot) {
-        double v[1] = {dsum};
-        commit_partials<1>(m, v, partials, chunk, counter, PH_ALPHA, sc, notDone, FinalArgs{0, 0, nullptr});
+    if (sc && sc[m.chunkInst[chunk]].done) return;
+    const int r = m.chunkRow0[chunk] + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
+    if (r >= m.chunkRow1[chunk]) return;
+    T o0, o1, o2;
+    row_spmv<T>(m, vals, x, r, lane8, o0, o1, o2);
+    if (lane8 == 0) {
+        if (accumulate) { o0 += y[3 * r]; o1 += y[3 * r + 1]; o2 += y[3 * r + 2]; }
+        y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
     }
 }
 
-// Same kernel under its own name for the PCG (q = A p with the p.q partial), so profiles separate the two.
+// The PCG product q = A p with the chunk partial p.q (slot 0).
 template <typename T>
-__global__ __launch_bounds__(256) void k_spmv_dot(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
-                                                  T *__restrict__ y, double *partials, int *counter, InstScalars *sc,
-                                                  int *notDone) {
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv_dot(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
+                                                           T *__restrict__ y, double *__restrict__ partials,
+                                                           const InstScalars *__restrict__ sc) {
     const int chunk = blockIdx.x;
-    const int r0 = m.chunkRow0[chunk], r1 = m.chunkRow1[chunk];
-    const int inst = m.chunkInst[chunk];
-    const bool skip = sc[inst].done;
-    const int group = threadIdx.x >> 3, lane8 = threadIdx.x & 7;
-    double dsum = 0.0;
-    if (!skip) {
-        for (int r = r0 + group; r < r1; r += (int)(blockDim.x >> 3)) {
-            T o0, o1, o2;
-            row_spmv<T>(m, vals, x, r, lane8, o0, o1, o2);
-            if (lane8 == 0) {
-                y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
-                dsum += (double)x[3 * r] * (double)o0 + (double)x[3 * r + 1] * (double)o1 + (double)x[3 * r + 2] * (double)o2;
-            }
+    const bool skip = sc[m.chunkInst[chunk]].done;
+    const int r = m.chunkRow0[chunk] + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
+    double v[1] = {0.0};
+    if (!skip && r < m.chunkRow1[chunk]) {
+        T o0, o1, o2;
+        row_spmv<T>(m, vals, x, r, lane8, o0, o1, o2);
+        if (lane8 == 0) {
+            y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
+            v[0] = (double)x[3 * r] * (double)o0 + (double)x[3 * r + 1] * (double)o1 + (double)x[3 * r + 2] * (double)o2;
         }
     }
-    double v[1] = {dsum};
-    commit_partials<1>(m, v, partials, chunk, counter, PH_ALPHA, sc, notDone, FinalArgs{0, 0, nullptr});
+    block_sum<1>(v);
+    if (threadIdx.x == 0) partials[4 * chunk] = v[0];
 }
 
 template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate, double *partials,
                  const InstScalars *sc) {
-    hipLaunchKernelGGL(k_spmv<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, vals, x, y, accumulate, partials, nullptr,
-                       const_cast<InstScalars *>(sc), nullptr, 0);
+    (void)partials;
+    hipLaunchKernelGGL(k_spmv<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, x, y, accumulate, sc);
     OD_HIP(hipGetLastError());
 }
 template void launch_spmv<double>(const DevMesh &, hipStream_t, const double *, const double *, double *, int, double *,
@@ -574,149 +532,146 @@ template void launch_spmv<double>(const DevMesh &, hipStream_t, const double *, 
 template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, const float *, float *, int, double *,
                                  const InstScalars *);
 
-// =====================================================================================
-// PCG vector kernels (one workgroup per chunk, 3 DOFs per block row)
-// =====================================================================================
-#define CHUNK_PROLOGUE                                                         \
-    const int chunk = blockIdx.x;                                              \
-    const int i0 = 3 * m.chunkRow0[chunk], i1 = 3 * m.chunkRow1[chunk];        \
-    const int inst = m.chunkInst[chunk];
-
-// r = b - q ; partials b.b, r.r
 template <typename T>
-__global__ void k_init_res(DevMesh m, const T *b, const T *q, T *r, double *partials, int *counter, InstScalars *sc,
-                           int *notDone) {
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w) {
+    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, p, q, w.partials, w.sc);
+    OD_HIP(hipGetLastError());
+    launch_finalize(m, s, PH_ALPHA, w, 0.0, 0, nullptr);
+}
+
+// =====================================================================================
+// PCG vector kernels: one workgroup per chunk, one DOF per thread
+// =====================================================================================
+constexpr int kVecThreads = 3 * kChunkRows;
+
+#define CHUNK_PROLOGUE                                            \
+    const int chunk = blockIdx.x;                                 \
+    const int inst = m.chunkInst[chunk];                          \
+    const int i = 3 * m.chunkRow0[chunk] + (int)threadIdx.x;      \
+    const bool in = i < 3 * m.chunkRow1[chunk];                   \
+    (void)inst;
+
+// r = b - q ; slots [1] r.r, [2] b.b
+template <typename T>
+__global__ __launch_bounds__(kVecThreads) void k_init_res(DevMesh m, const T *b, const T *q, T *r, double *partials) {
     CHUNK_PROLOGUE
-    double bb = 0, rr = 0;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double v[2] = {0, 0};
+    if (in) {
         T ri = b[i] - q[i];
         r[i] = ri;
-        bb += (double)b[i] * (double)b[i];
-        rr += (double)ri * (double)ri;
+        v[0] = (double)ri * (double)ri;
+        v[1] = (double)b[i] * (double)b[i];
     }
-    (void)inst;
-    double v[2] = {bb, rr};
-    commit_partials<2>(m, v, partials, chunk, counter, PH_INIT_RES, sc, notDone, FinalArgs{0, 0, nullptr});
+    block_sum<2>(v);
+    if (threadIdx.x == 0) { partials[4 * chunk + 1] = v[0]; partials[4 * chunk + 2] = v[1]; }
 }
 
 template <typename T>
 void pcg_init_residual(const DevMesh &m, hipStream_t s, const T *b, const T *q, T *r, PcgWork &w) {
-    hipLaunchKernelGGL(k_init_res<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, b, q, r, w.partials, w.counter, w.sc, w.notDone);
+    hipLaunchKernelGGL(k_init_res<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, b, q, r, w.partials);
     OD_HIP(hipGetLastError());
 }
 
-// z = invDiag * r (no reduction)
+// z = invDiag * r (no reduction); invDiag == nullptr: identity
 template <typename T>
-__global__ void k_jacobi(DevMesh m, const T *invDiag, const T *r, T *z, InstScalars *sc) {
+__global__ __launch_bounds__(kVecThreads) void k_jacobi(DevMesh m, const T *invDiag, const T *r, T *z) {
     CHUNK_PROLOGUE
-    if (sc && sc[inst].done) return;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) z[i] = invDiag ? invDiag[i] * r[i] : r[i];
+    if (in) z[i] = invDiag ? invDiag[i] * r[i] : r[i];
 }
 
 template <typename T>
 void pcg_jacobi(const DevMesh &m, hipStream_t s, const T *invDiag, const T *r, T *z, PcgWork &w) {
-    hipLaunchKernelGGL(k_jacobi<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, invDiag, r, z, (InstScalars *)nullptr);
+    (void)w;
+    hipLaunchKernelGGL(k_jacobi<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, invDiag, r, z);
     OD_HIP(hipGetLastError());
 }
 
-// p = z ; partial r.z ; finalise delta, eps, done
+// p = z ; slot [0] r.z ; then finalise: delta, eps, done
 template <typename T>
-__global__ void k_init_rz(DevMesh m, const T *r, const T *z, T *p, double *partials, int *counter, InstScalars *sc,
-                          int *notDone, FinalArgs fa) {
+__global__ __launch_bounds__(kVecThreads) void k_init_rz(DevMesh m, const T *r, const T *z, T *p, double *partials) {
     CHUNK_PROLOGUE
-    (void)inst;
-    double rz = 0;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double v[1] = {0};
+    if (in) {
         p[i] = z[i];
-        rz += (double)r[i] * (double)z[i];
+        v[0] = (double)r[i] * (double)z[i];
     }
-    double v[1] = {rz};
-    commit_partials<1>(m, v, partials, chunk, counter, PH_INIT_RZ, sc, notDone, fa);
+    block_sum<1>(v);
+    if (threadIdx.x == 0) partials[4 * chunk] = v[0];
 }
 
 template <typename T>
 void pcg_dot_rz_init(const DevMesh &m, hipStream_t s, const T *r, const T *z, T *p, PcgWork &w, double tol, int maxIt,
                      const uint8_t *preDone) {
-    hipLaunchKernelGGL(k_init_rz<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, r, z, p, w.partials, w.counter, w.sc,
-                       w.notDone, FinalArgs{tol, maxIt, preDone});
+    hipLaunchKernelGGL(k_init_rz<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, r, z, p, w.partials);
     OD_HIP(hipGetLastError());
+    launch_finalize(m, s, PH_INIT, w, tol, maxIt, preDone);
 }
 
+// x += alpha p ; r -= alpha q ; slot [1] r.r ; fused Jacobi: z = D^-1 r and slot [0] r.z (+ finalise)
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w) {
-    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, vals, p, q, w.partials, w.counter, w.sc,
-                       w.notDone);
-    OD_HIP(hipGetLastError());
-}
-
-// x += alpha p ; r -= alpha q ; (Jacobi: z = D^-1 r) ; partials r.r (and r.z)
-template <typename T>
-__global__ void k_update_xr(DevMesh m, const T *p, const T *q, T *x, T *r, const T *invDiag, T *z, int fuseJacobi,
-                            double *partials, int *counter, InstScalars *sc, int *notDone) {
+__global__ __launch_bounds__(kVecThreads) void k_update_xr(DevMesh m, const T *p, const T *q, T *x, T *r, const T *invDiag,
+                                                           T *z, int fuseJacobi, double *partials,
+                                                           const InstScalars *sc) {
     CHUNK_PROLOGUE
-    double rr = 0, rz = 0;
-    if (!sc[inst].done) {
-        const double al = sc[inst].alpha;
-        const T a = (T)al;
-        for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            T xi = x[i] + a * p[i];
-            T ri = r[i] - a * q[i];
-            x[i] = xi;
-            r[i] = ri;
-            rr += (double)ri * (double)ri;
-            if (fuseJacobi) {
-                T zi = invDiag ? invDiag[i] * ri : ri;
-                z[i] = zi;
-                rz += (double)ri * (double)zi;
-            }
+    double v[2] = {0, 0};
+    if (in && !sc[inst].done) {
+        const T a = (T)sc[inst].alpha;
+        T xi = x[i] + a * p[i];
+        T ri = r[i] - a * q[i];
+        x[i] = xi;
+        r[i] = ri;
+        v[1] = (double)ri * (double)ri;
+        if (fuseJacobi) {
+            T zi = invDiag ? invDiag[i] * ri : ri;
+            z[i] = zi;
+            v[0] = (double)ri * (double)zi;
         }
     }
-    if (fuseJacobi) {
-        double v[2] = {rr, rz};
-        commit_partials<2>(m, v, partials, chunk, counter, PH_XR_RZ, sc, notDone, FinalArgs{0, 0, nullptr});
-    } else {
-        double v[1] = {rr};
-        commit_partials<1>(m, v, partials, chunk, counter, PH_XR, sc, notDone, FinalArgs{0, 0, nullptr});
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        partials[4 * chunk + 1] = v[1];
+        if (fuseJacobi) partials[4 * chunk] = v[0];
     }
 }
 
 template <typename T>
 void pcg_update_xr(const DevMesh &m, hipStream_t s, const T *p, const T *q, T *x, T *r, const T *invDiag, T *z,
                    PcgWork &w) {
-    // z == nullptr -> the preconditioner runs separately (IC sweeps)
-    hipLaunchKernelGGL(k_update_xr<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, p, q, x, r, invDiag, z, z ? 1 : 0,
-                       w.partials, w.counter, w.sc, w.notDone);
+    hipLaunchKernelGGL(k_update_xr<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, p, q, x, r, invDiag, z, z ? 1 : 0,
+                       w.partials, w.sc);
     OD_HIP(hipGetLastError());
+    if (z) launch_finalize(m, s, PH_ITER, w, 0.0, 0, nullptr);
 }
 
+// slot [0] r.z ; finalise beta, delta, iteration count, done
 template <typename T>
-__global__ void k_dot_rz(DevMesh m, const T *r, const T *z, double *partials, int *counter, InstScalars *sc, int *notDone) {
+__global__ __launch_bounds__(kVecThreads) void k_dot_rz(DevMesh m, const T *r, const T *z, double *partials,
+                                                        const InstScalars *sc) {
     CHUNK_PROLOGUE
-    double rz = 0;
-    if (!sc[inst].done)
-        for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) rz += (double)r[i] * (double)z[i];
-    double v[1] = {rz};
-    commit_partials<1>(m, v, partials, chunk, counter, PH_RZ, sc, notDone, FinalArgs{0, 0, nullptr});
+    double v[1] = {0};
+    if (in && !sc[inst].done) v[0] = (double)r[i] * (double)z[i];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) partials[4 * chunk] = v[0];
 }
 
 template <typename T>
 void pcg_dot_rz(const DevMesh &m, hipStream_t s, const T *r, const T *z, PcgWork &w) {
-    hipLaunchKernelGGL(k_dot_rz<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, r, z, w.partials, w.counter, w.sc, w.notDone);
+    hipLaunchKernelGGL(k_dot_rz<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, r, z, w.partials, w.sc);
     OD_HIP(hipGetLastError());
+    launch_finalize(m, s, PH_ITER, w, 0.0, 0, nullptr);
 }
 
-// p = z + beta p  (skipped for instances that finished this iteration)
+// p = z + beta p  (skipped for instances that are done)
 template <typename T>
-__global__ void k_update_p(DevMesh m, const T *z, T *p, InstScalars *sc) {
+__global__ __launch_bounds__(kVecThreads) void k_update_p(DevMesh m, const T *z, T *p, const InstScalars *sc) {
     CHUNK_PROLOGUE
-    if (sc[inst].done) return;
-    const T be = (T)sc[inst].beta;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = z[i] + be * p[i];
+    if (!in || sc[inst].done) return;
+    p[i] = z[i] + (T)sc[inst].beta * p[i];
 }
 
 template <typename T>
 void pcg_update_p(const DevMesh &m, hipStream_t s, const T *z, T *p, PcgWork &w) {
-    hipLaunchKernelGGL(k_update_p<T>, dim3(m.nChunks), dim3(kBlock), 0, s, m, z, p, w.sc);
+    hipLaunchKernelGGL(k_update_p<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, z, p, w.sc);
     OD_HIP(hipGetLastError());
 }
 
@@ -726,7 +681,7 @@ __global__ void k_jacobi_diag(DevMesh m, const double *vals, T *inv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 3 * m.nRows) return;
     const int r = i / 3, ax = i % 3;
-    const int off = m.blkOff[m.rowPtr[r]] + (ax == 0 ? 0 : (ax == 1 ? 3 : 5));
+    const int off = m.valOff[r] + (ax == 0 ? 0 : (ax == 1 ? 3 : 5));
     double dg = vals[off];
     inv[i] = (T)(dg > 0.0 ? 1.0 / dg : 1.0);
 }
@@ -752,14 +707,14 @@ __global__ void k_ic0_factor_level(DevMesh m, const int *__restrict__ rows, int 
     const int I = rows[t];
     if (skip && skip[m.rowInst[I]]) return;
     const int b0 = m.rowPtr[I], b1 = m.rowPtr[I + 1];
-    const int v0 = m.blkOff[b0], v1 = (b1 < m.nBlocks) ? m.blkOff[b1] : (int)m.nValues;
+    const int v0 = m.valOff[I], v1 = m.valOff[I + 1];
     // row I of U starts as row I of A
     for (int k = v0; k < v1; k++) U[k] = A[k];
     double s00 = U[v0], s01 = U[v0 + 1], s02 = U[v0 + 2], s11 = U[v0 + 3], s12 = U[v0 + 4], s22 = U[v0 + 5];
     // subtract sum over K < I of U_KI^T U_KJ (J in row I's pattern, J >= I)
     for (int k = m.tPtr[I]; k < m.tPtr[I + 1]; k++) {
         const int K = m.tRow[k];
-        const double *W = U + m.blkOff[m.tBlk[k]];  // U_KI, row-major
+        const double *W = U + m.tOff[k];  // U_KI, row-major
         double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5], w6 = W[6], w7 = W[7], w8 = W[8];
         // diagonal: S_II -= W^T W
         s00 -= w0 * w0 + w3 * w3 + w6 * w6;
@@ -821,55 +776,71 @@ void launch_ic0_factor(const DevMesh &m, hipStream_t s, const std::vector<int> &
     OD_HIP(hipGetLastError());
 }
 
-// forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_K U_KI^T y_K)
+// IC(0) sweeps, 8 lanes per row, one launch per level (rows of a level are independent).
+// forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K)
 template <typename T>
-__global__ void k_ic0_fwd(DevMesh m, const int *__restrict__ rows, int count, const T *__restrict__ U,
-                          const T *__restrict__ r, T *__restrict__ y, const InstScalars *__restrict__ sc,
-                          const uint8_t *__restrict__ skip) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_ic0_fwd(DevMesh m, const int *__restrict__ rows, int count,
+                                                 const T *__restrict__ U, const T *__restrict__ r, T *__restrict__ y,
+                                                 const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip) {
+    const int t = blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
     if (t >= count) return;
     const int I = rows[t];
-    if (sc && sc[m.rowInst[I]].done) return;
-    if (skip && skip[m.rowInst[I]]) return;
-    T a0 = r[3 * I], a1 = r[3 * I + 1], a2 = r[3 * I + 2];
-    for (int k = m.tPtr[I]; k < m.tPtr[I + 1]; k++) {
+    const int inst = m.rowInst[I];
+    if ((sc && sc[inst].done) || (skip && skip[inst])) return;
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int k = m.tPtr[I] + lane8; k < m.tPtr[I + 1]; k += 8) {
         const int K = m.tRow[k];
-        const T *W = U + m.blkOff[m.tBlk[k]];
+        const T *W = U + m.tOff[k];
         const T y0 = y[3 * K], y1 = y[3 * K + 1], y2 = y[3 * K + 2];
-        a0 -= W[0] * y0 + W[3] * y1 + W[6] * y2;
-        a1 -= W[1] * y0 + W[4] * y1 + W[7] * y2;
-        a2 -= W[2] * y0 + W[5] * y1 + W[8] * y2;
+        a0 += W[0] * y0 + W[3] * y1 + W[6] * y2;
+        a1 += W[1] * y0 + W[4] * y1 + W[7] * y2;
+        a2 += W[2] * y0 + W[5] * y1 + W[8] * y2;
     }
-    const T *D = U + m.blkOff[m.rowPtr[I]];
-    T z0 = a0 / D[0];
-    T z1 = (a1 - D[1] * z0) / D[3];
-    T z2 = (a2 - D[2] * z0 - D[4] * z1) / D[5];
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 8);
+        a1 += __shfl_xor(a1, o, 8);
+        a2 += __shfl_xor(a2, o, 8);
+    }
+    if (lane8 != 0) return;
+    const T *D = U + m.valOff[I];
+    const T z0 = (r[3 * I] - a0) / D[0];
+    const T z1 = (r[3 * I + 1] - a1 - D[1] * z0) / D[3];
+    const T z2 = (r[3 * I + 2] - a2 - D[2] * z0 - D[4] * z1) / D[5];
     y[3 * I] = z0; y[3 * I + 1] = z1; y[3 * I + 2] = z2;
 }
 
 // backward: U z = y  ->  z_I = U_II^-1 (y_I - sum_{J>I} U_IJ z_J)
 template <typename T>
-__global__ void k_ic0_bwd(DevMesh m, const int *__restrict__ rows, int count, const T *__restrict__ U, T *__restrict__ z,
-                          const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_ic0_bwd(DevMesh m, const int *__restrict__ rows, int count,
+                                                 const T *__restrict__ U, T *__restrict__ z,
+                                                 const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip) {
+    const int t = blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
     if (t >= count) return;
     const int I = rows[t];
-    if (sc && sc[m.rowInst[I]].done) return;
-    if (skip && skip[m.rowInst[I]]) return;
-    T a0 = z[3 * I], a1 = z[3 * I + 1], a2 = z[3 * I + 2];
-    const int b0 = m.rowPtr[I];
-    for (int b = b0 + 1; b < m.rowPtr[I + 1]; b++) {
-        const int J = m.colIdx[b];
-        const T *W = U + m.blkOff[b];
+    const int inst = m.rowInst[I];
+    if ((sc && sc[inst].done) || (skip && skip[inst])) return;
+    const int v0 = m.valOff[I], b0 = m.rowPtr[I], nOff = m.rowPtr[I + 1] - b0 - 1;
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int k = lane8; k < nOff; k += 8) {
+        const int J = m.colIdx[b0 + 1 + k];
+        const T *W = U + v0 + 6 + 9 * k;
         const T z0 = z[3 * J], z1 = z[3 * J + 1], z2 = z[3 * J + 2];
-        a0 -= W[0] * z0 + W[1] * z1 + W[2] * z2;
-        a1 -= W[3] * z0 + W[4] * z1 + W[5] * z2;
-        a2 -= W[6] * z0 + W[7] * z1 + W[8] * z2;
+        a0 += W[0] * z0 + W[1] * z1 + W[2] * z2;
+        a1 += W[3] * z0 + W[4] * z1 + W[5] * z2;
+        a2 += W[6] * z0 + W[7] * z1 + W[8] * z2;
     }
-    const T *D = U + m.blkOff[b0];
-    T x2 = a2 / D[5];
-    T x1 = (a1 - D[4] * x2) / D[3];
-    T x0 = (a0 - D[1] * x1 - D[2] * x2) / D[0];
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 8);
+        a1 += __shfl_xor(a1, o, 8);
+        a2 += __shfl_xor(a2, o, 8);
+    }
+    if (lane8 != 0) return;
+    const T *D = U + v0;
+    const T x2 = (z[3 * I + 2] - a2) / D[5];
+    const T x1 = (z[3 * I + 1] - a1 - D[4] * x2) / D[3];
+    const T x0 = (z[3 * I] - a0 - D[1] * x1 - D[2] * x2) / D[0];
     z[3 * I] = x0; z[3 * I + 1] = x1; z[3 * I + 2] = x2;
 }
 
@@ -880,12 +851,14 @@ void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &l
     for (int l = 0; l < nl; l++) {
         int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(k_ic0_fwd<T>, dim3((cnt + 127) / 128), dim3(128), 0, s, m, m.levelRows + levelPtr[l], cnt, U, r, z, sc, skip);
+        hipLaunchKernelGGL(k_ic0_fwd<T>, dim3((cnt + 31) / 32), dim3(256), 0, s, m, m.levelRows + levelPtr[l], cnt, U, r, z,
+                           sc, skip);
     }
     for (int l = nl - 1; l >= 0; l--) {
         int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(k_ic0_bwd<T>, dim3((cnt + 127) / 128), dim3(128), 0, s, m, m.levelRows + levelPtr[l], cnt, U, z, sc, skip);
+        hipLaunchKernelGGL(k_ic0_bwd<T>, dim3((cnt + 31) / 32), dim3(256), 0, s, m, m.levelRows + levelPtr[l], cnt, U, z, sc,
+                           skip);
     }
     OD_HIP(hipGetLastError());
 }
@@ -902,67 +875,67 @@ void launch_cast_f64_f32(hipStream_t s, const double *a, float *b, int64_t n) {
     OD_HIP(hipGetLastError());
 }
 
-// r = b - Ax (FP64), r32 = (float) r ; partials r.r, b.b ; outer done flags
-__global__ void k_outer_res(DevMesh m, const double *b, const double *Ax, double *r, float *r32, double *partials,
-                            int *counter, InstScalars *sc, int *notDone, FinalArgs fa) {
+// r = b - Ax (FP64), r32 = (float) r ; slots [1] r.r, [2] b.b ; then the outer done flags
+__global__ __launch_bounds__(kVecThreads) void k_outer_res(DevMesh m, const double *b, const double *Ax, double *r,
+                                                           float *r32, double *partials) {
     CHUNK_PROLOGUE
-    (void)inst;
-    double rr = 0, bb = 0;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double v[2] = {0, 0};
+    if (in) {
         double ri = b[i] - Ax[i];
         r[i] = ri;
         r32[i] = (float)ri;
-        rr += ri * ri;
-        bb += b[i] * b[i];
+        v[0] = ri * ri;
+        v[1] = b[i] * b[i];
     }
-    double v[2] = {rr, bb};
-    commit_partials<2>(m, v, partials, chunk, counter, PH_OUTER_RES, sc, notDone, fa);
+    block_sum<2>(v);
+    if (threadIdx.x == 0) { partials[4 * chunk + 1] = v[0]; partials[4 * chunk + 2] = v[1]; }
 }
 
 void launch_residual_f64(const DevMesh &m, hipStream_t s, const double *b, const double *Ax, double *r, float *r32,
                          PcgWork &w, double tol, const uint8_t *preDone, int first) {
-    hipLaunchKernelGGL(k_outer_res, dim3(m.nChunks), dim3(kBlock), 0, s, m, b, Ax, r, r32, w.partials, w.counter, w.sc,
-                       w.notDone, FinalArgs{tol, first ? -1 : 0, preDone});
+    hipLaunchKernelGGL(k_outer_res, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, b, Ax, r, r32, w.partials);
     OD_HIP(hipGetLastError());
+    launch_finalize(m, s, first ? PH_OUTER_FIRST : PH_OUTER, w, tol, 0, preDone);
 }
 
-__global__ void k_add_corr(DevMesh m, const float *dx, double *x, const InstScalars *outer) {
+__global__ __launch_bounds__(kVecThreads) void k_add_corr(DevMesh m, const float *dx, double *x, const InstScalars *outer) {
     CHUNK_PROLOGUE
-    if (outer[inst].done) return;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) x[i] += (double)dx[i];
+    if (!in || outer[inst].done) return;
+    x[i] += (double)dx[i];
 }
 void launch_add_correction(const DevMesh &m, hipStream_t s, const float *dx, double *x, const InstScalars *outer) {
-    hipLaunchKernelGGL(k_add_corr, dim3(m.nChunks), dim3(kBlock), 0, s, m, dx, x, outer);
+    hipLaunchKernelGGL(k_add_corr, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, dx, x, outer);
     OD_HIP(hipGetLastError());
 }
 
 // Newmark corrector (linearNewmark.cpp:68-69, 76-77), predictor recomputed from the old state
-__global__ void k_newmark_correct(DevMesh m, double *d, double *v, const double *aOld, const double *aNew, double h,
-                                  double beta, double gamma, const uint8_t *instActive) {
+__global__ __launch_bounds__(kVecThreads) void k_newmark_correct(DevMesh m, double *d, double *v, const double *aOld,
+                                                                 const double *aNew, double h, double beta,
+                                                                 double gamma, const uint8_t *instActive) {
     CHUNK_PROLOGUE
-    if (instActive && !instActive[inst]) return;
+    if (!in || (instActive && !instActive[inst])) return;
     const double bh2 = beta * h * h, gh = gamma * h, mbh2 = h * h * 0.5 - bh2, mgh = h - gh;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        double ao = aOld[i], an = aNew[i];
-        double dp = d[i] + h * v[i] + mbh2 * ao;
-        double vp = v[i] + mgh * ao;
-        v[i] = vp + gh * an;
-        d[i] = dp + bh2 * an;
-    }
+    double ao = aOld[i], an = aNew[i];
+    double dp = d[i] + h * v[i] + mbh2 * ao;
+    double vp = v[i] + mgh * ao;
+    v[i] = vp + gh * an;
+    d[i] = dp + bh2 * an;
 }
 void launch_newmark_correct(const DevMesh &m, hipStream_t s, double *d, double *v, const double *aOld,
                             const double *aNew, double h, double beta, double gamma, const uint8_t *instActive) {
-    hipLaunchKernelGGL(k_newmark_correct, dim3(m.nChunks), dim3(kBlock), 0, s, m, d, v, aOld, aNew, h, beta, gamma, instActive);
+    hipLaunchKernelGGL(k_newmark_correct, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, d, v, aOld, aNew, h, beta, gamma,
+                       instActive);
     OD_HIP(hipGetLastError());
 }
 
-__global__ void k_axpy_rows(DevMesh m, double *y, double a, const double *x, const uint8_t *instActive) {
+__global__ __launch_bounds__(kVecThreads) void k_axpy_rows(DevMesh m, double *y, double a, const double *x,
+                                                           const uint8_t *instActive) {
     CHUNK_PROLOGUE
-    if (instActive && !instActive[inst]) return;
-    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) y[i] += a * x[i];
+    if (!in || (instActive && !instActive[inst])) return;
+    y[i] += a * x[i];
 }
 void launch_axpy_rows(const DevMesh &m, hipStream_t s, double *y, double a, const double *x, const uint8_t *instActive) {
-    hipLaunchKernelGGL(k_axpy_rows, dim3(m.nChunks), dim3(kBlock), 0, s, m, y, a, x, instActive);
+    hipLaunchKernelGGL(k_axpy_rows, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, y, a, x, instActive);
     OD_HIP(hipGetLastError());
 }
 void launch_be_correct(const DevMesh &m, hipStream_t s, double *v, const double *dv, const uint8_t *instActive) {

@@ -1,0 +1,238 @@
+"""GPU parity: the HIP path through the C-ABI against the FP64 oracle on the same inputs.
+
+Bars: integer layout bit-exact; FP64 values within the stated relative
+tolerances (rounding-order differences only); per-step Newmark displacements
+within 1e-6 relative of the oracle (plain FP64 path).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from opendeformer_amd import capi
+from opendeformer_amd import mesh as M
+from opendeformer_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FP64_VALUE_RTOL = 1e-12      # element/assembly/SpMV values: summation order only
+NEWMARK_RTOL = 1e-6          # per-step displacement vs the oracle (plain FP64 path)
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def make(ctx, w):
+    import opendeformer_amd as od
+    mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs, w.inst_node_start, w.inst_tet_start)
+    mesh.set_material(w.kind, w.E, w.nu, w.rho)
+    s = O.System(w.X, w.tets, w.fixed_dofs)
+    return mesh, s
+
+
+@pytest.mark.parametrize("name", ["box_small", "box_small_color8", "partial_box"])
+def test_pattern_and_element_map_bit_exact(ctx, name):
+    import opendeformer_amd as od
+    if name == "partial_box":
+        m = M.box_mesh(5, 3, 2, 1.0, 0.6, 0.4, seed=9)
+        w = W.Workload("p", m.X, m.tets, m.facets, np.array([0, 7, 13], np.int32), np.zeros(m.n_tets, np.int32),
+                       np.full(m.n_tets, 1e5), np.full(m.n_tets, 0.4), np.full(m.n_tets, 1000.0), 0.01, 1)
+    else:
+        w = W.box_small("color8" if name.endswith("color8") else "natural")
+    mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
+    s = O.System(w.X, w.tets, w.fixed_dofs)
+    for a, b in zip(mesh.pattern(), s.pattern()):
+        assert np.array_equal(a, b)
+
+
+def test_partial_node_constraint_is_rejected(ctx):
+    import opendeformer_amd as od
+    w = W.box_small()
+    with pytest.raises(capi.ODError, match="partially constrained"):
+        od.Mesh(ctx, w.X, w.tets, np.array([0, 1], np.int32))
+
+
+def test_spmv_fp64_fp32_and_accumulate(ctx):
+    import opendeformer_amd as od
+    w = W.box_small("color8")
+    mesh, s = make(ctx, w)
+    rng = np.random.default_rng(0)
+    vals = rng.standard_normal(mesh.n_values)
+    x = rng.standard_normal(mesh.ndof)
+    y0 = rng.standard_normal(mesh.ndof)
+    A = od.Matrix(mesh); A.upload(vals)
+    dx = od.DeviceArray.from_host(ctx, x); dy = od.DeviceArray.from_host(ctx, y0)
+    od.spmdv(A, dx, dy, accumulate=True)
+    ref = s.spmdv(vals, x, y0.copy())
+    assert relerr(dy.download(), ref) < FP64_VALUE_RTOL
+    od.spmdv(A, dx, dy, accumulate=False)
+    assert relerr(dy.download(), ref - y0) < FP64_VALUE_RTOL
+    A.sync_f32()
+    dx32 = od.DeviceArray.from_host(ctx, x, np.float32); dy32 = od.DeviceArray(ctx, mesh.ndof, np.float32)
+    od.spmdv_f32(A, dx32, dy32, accumulate=False)
+    assert relerr(dy32.download().astype(np.float64), ref - y0) < 1e-5
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_assembly_matches_oracle(ctx, kind):
+    import opendeformer_amd as od
+    w = W.box_small()
+    w.kind[:] = kind
+    mesh, s = make(ctx, w)
+    rng = np.random.default_rng(1 + kind)
+    u = rng.standard_normal(mesh.ndof) * 0.02
+    K = od.Matrix(mesh); f = od.DeviceArray(ctx, mesh.ndof); f.zero()
+    od.generate_matrix_and_virtual_works(mesh, u, K, f)
+    s.set_free_displacements(u)
+    Kc, fc = s.assemble(w.kind, w.E, w.nu)
+    # corotational: polar + R K0 R^T, rounding order only.  neo-Hookean: the closed-form
+    # eigenvectors of F^T F are ill-conditioned when singular values nearly coincide (F near
+    # identity here), which amplifies FMA/order differences to ~1e-10 in K.
+    tol = 1e-11 if kind == 0 else 1e-8
+    assert relerr(K.download(), Kc) < tol
+    assert relerr(f.download(), fc) < tol * 10
+    # accumulate semantics: a second call adds the same values again
+    od.generate_matrix_and_virtual_works(mesh, u, K, f)
+    assert relerr(K.download(), 2 * Kc) < tol
+
+
+def test_mass_and_body_force(ctx):
+    import opendeformer_amd as od
+    w = W.box_small()
+    mesh, s = make(ctx, w)
+    Mm = od.Matrix(mesh); Mm.add_mass()
+    assert relerr(Mm.download(), s.mass(w.rho)) < FP64_VALUE_RTOL
+    assert relerr(mesh.body_force(w.body), s.body_force(w.body)) < FP64_VALUE_RTOL
+
+
+def newmark_system(s, w, kind=None):
+    Mv = s.mass(w.rho)
+    u = np.random.default_rng(4).standard_normal(s.ndof) * 0.01
+    s.set_free_displacements(u)
+    K, _ = s.assemble(w.kind if kind is None else kind, w.E, w.nu)
+    return Mv + 0.25 * w.dt ** 2 * K
+
+
+@pytest.mark.parametrize("ordering", ["natural", "color8"])
+def test_ic0_factor_matches_oracle(ctx, ordering):
+    import opendeformer_amd as od
+    w = W.box_small(ordering)
+    mesh, s = make(ctx, w)
+    A = newmark_system(s, w)
+    Ad = od.Matrix(mesh); Ad.upload(A)
+    solver = od.CGSolver(mesh, precond=capi.OD_PRE_IC0, precond_param=1e-3)
+    assert relerr(solver.ic0_factor(Ad), s.ic0(A, 1e-3)) < 1e-11
+    r = np.random.default_rng(3).standard_normal(mesh.ndof)
+    z = od.DeviceArray(ctx, mesh.ndof)
+    solver.precond_apply(Ad, od.DeviceArray.from_host(ctx, r), z)
+    assert relerr(z.download(), s.precond_apply(A, r, O.PRE_IC0, 1e-3)) < 1e-10
+
+
+@pytest.mark.parametrize("pre,opre", [(capi.OD_PRE_JACOBI, O.PRE_JACOBI), (capi.OD_PRE_IC0, O.PRE_IC0),
+                                      (capi.OD_PRE_NONE, O.PRE_NONE)])
+def test_pcg_matches_oracle(ctx, pre, opre):
+    import opendeformer_amd as od
+    w = W.box_small("color8")
+    mesh, s = make(ctx, w)
+    A = newmark_system(s, w)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(mesh.ndof); x0 = rng.standard_normal(mesh.ndof) * 1e-3
+    Ad = od.Matrix(mesh); Ad.upload(A)
+    solver = od.CGSolver(mesh, tol=1e-6, max_iter=5000, precond=pre, precond_param=1e-3)
+    dx = od.DeviceArray.from_host(ctx, x0)
+    it, rr = solver.solve(Ad, od.DeviceArray.from_host(ctx, b), dx)
+    xo, ito, rro = s.cg(A, b, x0, tol=1e-6, max_iter=5000, pre=opre, p1=1e-3)
+    assert abs(int(it[0]) - ito) <= 1
+    assert rr[0] <= 1e-6 and rro <= 1e-6
+    assert relerr(dx.download(), xo) < 1e-5
+    # maxIter stops silently, like cgSolver.cpp:32
+    dx.upload(x0)
+    solver2 = od.CGSolver(mesh, tol=1e-12, max_iter=3, precond=pre, precond_param=1e-3)
+    it, rr = solver2.solve(Ad, od.DeviceArray.from_host(ctx, b), dx)
+    assert it[0] == 3 and rr[0] > 1e-12
+
+
+def run_pair(ctx, w, steps, pre=capi.OD_PRE_IC0, opre=O.PRE_IC0, precision=capi.OD_FP64, impulse=None):
+    import opendeformer_amd as od
+    mesh, s = make(ctx, w)
+    f = s.body_force(w.body)
+    nm = od.NewmarkIntegrator(mesh, w.dt, precond=pre, precision=precision, precond_param=1e-3)
+    nm.set_external_virtual_work(f)
+    ref = O.Newmark(s, w.kind, w.E, w.nu, w.rho, w.dt, pre=opre, p1=1e-3)
+    ref.set_external(f)
+    if impulse is not None:
+        nm.set_impulse(impulse); ref.set_impulse(impulse)
+    errs, its = [], []
+    for _ in range(steps):
+        nm.run_one_time_step()
+        oi, _, _ = ref.step(1)
+        d, v, a = nm.state()
+        do, vo, ao = ref.state()
+        errs.append(relerr(d, do))
+        its.append((int(nm.stats()[0][0]), int(oi[0])))
+    return np.array(errs), its
+
+
+@pytest.mark.parametrize("ordering,pre,opre", [("natural", capi.OD_PRE_IC0, O.PRE_IC0),
+                                                ("color8", capi.OD_PRE_IC0, O.PRE_IC0),
+                                                ("natural", capi.OD_PRE_JACOBI, O.PRE_JACOBI)])
+def test_newmark_box_small_100_steps(ctx, ordering, pre, opre):
+    w = W.box_small(ordering)
+    errs, its = run_pair(ctx, w, w.steps, pre, opre)
+    assert errs.max() < NEWMARK_RTOL, errs.max()
+    assert sum(abs(a - b) for a, b in its) <= 2
+
+
+def test_newmark_neohookean_with_impulse(ctx):
+    m = M.box_mesh(24, 5, 5, 0.6, 0.125, 0.125, seed=2)
+    w = W._single("neo_small", m, 1, 5e4, 0.40, 1.0 / 120, 30, "natural", impulse=W.tip_impulse(m, 2))
+    errs, its = run_pair(ctx, w, w.steps, impulse=w.impulse)
+    assert errs.max() < NEWMARK_RTOL, errs.max()
+
+
+def test_mixed_precision_error_vs_fp64_oracle(ctx):
+    """FP32 inner PCG + FP64 refinement: stated bound 1e-5 relative per step on this workload."""
+    w = W.box_small("color8")
+    errs, _ = run_pair(ctx, w, 40, precision=capi.OD_MIXED)
+    assert errs.max() < 1e-5, errs.max()
+
+
+def test_backward_euler_matches_oracle(ctx):
+    import opendeformer_amd as od
+    w = W.box_small()
+    mesh, s = make(ctx, w)
+    f = s.body_force(w.body)
+    be = od.BackwardEulerIntegrator(mesh, w.dt, precond=capi.OD_PRE_JACOBI)
+    be.set_external_virtual_work(f)
+    ref = O.BackwardEuler(s, w.kind, w.E, w.nu, w.rho, w.dt, pre=O.PRE_JACOBI)
+    ref.set_external(f)
+    for _ in range(20):
+        be.run_one_time_step(); be.update_mesh_vertices_displacements()
+        ref.step(); ref.update_mesh()
+        _, v, _ = be.state()
+        assert relerr(v, ref.velocity()) < 1e-6
+    pos = be.vertex_positions()
+    np.testing.assert_allclose(pos, (w.X + s.node_displacements()).astype(np.float32), rtol=1e-6, atol=1e-6)
+
+
+def test_batch_equals_alone_bitwise(ctx):
+    """Instances in one concatenated batch give the same bits as each instance alone."""
+    import opendeformer_amd as od
+    wb = W.batch(n_inst=6, seed=11, tet_lo=300, tet_hi=3000, steps=5)
+    mesh = od.Mesh(ctx, wb.X, wb.tets, wb.fixed_dofs, wb.inst_node_start, wb.inst_tet_start)
+    mesh.set_material(wb.kind, wb.E, wb.nu, wb.rho)
+    f = mesh.body_force(wb.body)
+    nm = od.NewmarkIntegrator(mesh, wb.dt, precond=capi.OD_PRE_IC0)
+    nm.set_external_virtual_work(f)
+    nm.run_time_steps(wb.steps)
+    db, _, _ = nm.state()
+    irs, _ = mesh.rows()
+    for i in range(6):
+        wi = W.subset(wb, i)
+        mi = od.Mesh(ctx, wi.X, wi.tets, wi.fixed_dofs)
+        mi.set_material(wi.kind, wi.E, wi.nu, wi.rho)
+        ni = od.NewmarkIntegrator(mi, wi.dt, precond=capi.OD_PRE_IC0)
+        ni.set_external_virtual_work(mi.body_force(wi.body))
+        ni.run_time_steps(wb.steps)
+        di, _, _ = ni.state()
+        assert np.array_equal(di, db[3 * irs[i]:3 * irs[i + 1]])
