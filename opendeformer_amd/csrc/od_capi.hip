@@ -3,8 +3,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -109,6 +111,10 @@ struct od_matrix {
     od_mesh *mesh = nullptr;
     double *vals = nullptr;
     float *vals32 = nullptr;
+    // optional lower copy: the off-diagonal blocks again, contiguous per block row of the
+    // transposed list (written by the assembly gather; read by the SpMV)
+    double *low = nullptr;
+    float *low32 = nullptr;
     ~od_matrix() {
         if (vals) (void)hipFree(vals);
         if (vals32) (void)hipFree(vals32);
@@ -123,9 +129,9 @@ struct od_solver {
     double innerTol = 1e-3;     // mixed: inner FP32 relative tolerance
     int maxOuter = 12;
     std::vector<void *> allocs;
-    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *invDiag = nullptr, *U = nullptr;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *invDiag = nullptr, *U = nullptr, *Ul = nullptr;
     float *r32 = nullptr, *ri32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
-    float *invDiag32 = nullptr, *U32 = nullptr;
+    float *invDiag32 = nullptr, *U32 = nullptr, *Ul32 = nullptr;
     double *Ax = nullptr;
     double *partials = nullptr;
     int *doneIn = nullptr, *doneOut = nullptr;   // per-instance done flags (device)
@@ -134,6 +140,7 @@ struct od_solver {
     std::vector<int> iters;
     std::vector<double> relres;
     int outer = 0;
+    int lastMaxIters = 0;       // iterations of the previous solve (sizes the first launch batch)
     void *alloc(size_t b) {
         void *ptr = nullptr;
         OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
@@ -169,7 +176,7 @@ struct od_integrator {
         return ptr;
     }
     ~od_integrator() {
-        M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr;   // owned by allocs
+        M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr; A.low = nullptr; A.low32 = nullptr;   // owned by allocs
         for (void *ptr : allocs) (void)hipFree(ptr);
         delete solver;
     }
@@ -194,15 +201,22 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
         s->p = (double *)s->alloc(8 * n); s->q = (double *)s->alloc(8 * n);
         s->Ax = (double *)s->alloc(8 * n);
         if (precond == OD_PRE_JACOBI) s->invDiag = (double *)s->alloc(8 * n);
-        if (precond == OD_PRE_IC0) s->U = (double *)s->alloc(8 * nv);
+        const size_t nl = 9 * (size_t)t.tRow.size() + kValuePad;
+        if (precond == OD_PRE_IC0) {
+            s->U = (double *)s->alloc(8 * (nv + kValuePad));
+            s->Ul = (double *)s->alloc(8 * nl);
+        }
         if (precision == OD_MIXED) {
             s->r32 = (float *)s->alloc(4 * n); s->ri32 = (float *)s->alloc(4 * n); s->z32 = (float *)s->alloc(4 * n);
             s->p32 = (float *)s->alloc(4 * n); s->q32 = (float *)s->alloc(4 * n);
             s->x32 = (float *)s->alloc(4 * n);
             if (precond == OD_PRE_JACOBI) s->invDiag32 = (float *)s->alloc(4 * n);
-            if (precond == OD_PRE_IC0) s->U32 = (float *)s->alloc(4 * nv);
+            if (precond == OD_PRE_IC0) {
+                s->U32 = (float *)s->alloc(4 * (nv + kValuePad));
+                s->Ul32 = (float *)s->alloc(4 * nl);
+            }
         }
-        s->partials = (double *)s->alloc(sizeof(double) * 4 * (size_t)t.chunkRow0.size());
+        s->partials = (double *)s->alloc(sizeof(double) * 3 * (size_t)t.chunkRow0.size());
         s->doneIn = (int *)s->alloc(sizeof(int) * t.nInst);
         s->doneOut = (int *)s->alloc(sizeof(int) * t.nInst);
         OD_HIP(hipMemset(s->doneIn, 0, sizeof(int) * t.nInst));
@@ -247,38 +261,45 @@ static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip)
         launch_jacobi_diag<double>(m, st, A->vals, s->invDiag);
         if (s->invDiag32) launch_jacobi_diag<float>(m, st, A->vals, s->invDiag32);
     } else if (s->precond == OD_PRE_IC0) {
-        if (s->U32) launch_ic0_factor<float>(m, st, mesh->topo.levelPtr, A->vals, s->U, s->U32, s->param, skip);
-        else launch_ic0_factor<double>(m, st, mesh->topo.levelPtr, A->vals, s->U, (double *)nullptr, s->param, skip);
+        if (s->U32) launch_ic0_factor<float>(m, st, mesh->topo.levelPtr, A->vals, s->U, s->U32, s->Ul, s->Ul32, s->param, skip);
+        else launch_ic0_factor<double>(m, st, mesh->topo.levelPtr, A->vals, s->U, (double *)nullptr, s->Ul, (double *)nullptr,
+                                       s->param, skip);
     }
 }
 
 // PCG on the chunked instances: x in/out, b fixed.  preDone marks instances to skip.
 template <typename T>
-static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *invDiag, const T *b, T *x, T *r, T *z, T *p, T *q,
-                    double tol, int maxIt, const uint8_t *preDone, InstScalars *sc, int *doneFlags, bool xZero) {
+static void run_pcg(od_solver *s, const T *vals, const T *low, const T *Uf, const T *Ul, const T *invDiag, const T *b,
+                    T *x, T *r, T *z, T *p, T *q, double tol, int maxIt, const uint8_t *preDone, InstScalars *sc,
+                    int *doneFlags, bool xZero) {
     od_mesh *mesh = s->mesh;
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
     PcgWork w{s->partials, sc, doneFlags};
     const size_t n = (size_t)mesh->topo.ndof;
     if (xZero) OD_HIP(hipMemsetAsync(q, 0, sizeof(T) * n, st));
-    else launch_spmv<T>(m, st, vals, x, q, 0, nullptr, nullptr);
+    else launch_spmv<T>(m, st, vals, low, x, q, 0, nullptr);
     pcg_init_residual<T>(m, st, b, q, r, w);
     const bool ic = s->precond == OD_PRE_IC0;
-    if (ic) launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, r, z, nullptr, preDone);
+    if (ic) launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, Ul, r, z, nullptr, preDone);
     else pcg_jacobi<T>(m, st, invDiag, r, z, w);
     pcg_dot_rz_init<T>(m, st, r, z, p, w, tol, maxIt, preDone);
-    int launched = 0, batch = 4;
+    // Iterations are launched in batches between convergence polls (a poll synchronises the
+    // stream).  Warm-started solves of consecutive steps need nearly the same count, so the
+    // first batch is sized from the previous solve; afterwards the batches stay small.
+    int launched = 0;
+    int batch = s->lastMaxIters > 8 ? s->lastMaxIters - 2 : 4;
     while (launched < maxIt) {
         if (count_not_done(mesh->ctx, doneFlags, m.nInst) == 0) break;
         const int k = std::min(batch, maxIt - launched);
         for (int j = 0; j < k; j++) {
             prof_mark(mesh->ctx);
-            pcg_alpha<T>(m, st, vals, p, q, w);
+            pcg_alpha<T>(m, st, vals, low, p, q, w);
             prof_mark(mesh->ctx);
+            pcg_alpha_finalize(m, st, w);
             if (ic) {
                 pcg_update_xr<T>(m, st, p, q, x, r, (const T *)nullptr, (T *)nullptr, w);
-                launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, r, z, sc, preDone);
+                launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, Ul, r, z, sc, preDone);
                 pcg_dot_rz<T>(m, st, r, z, w);
             } else {
                 pcg_update_xr<T>(m, st, p, q, x, r, invDiag, z, w);
@@ -286,7 +307,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *invDiag, 
             pcg_update_p<T>(m, st, z, p, w);
         }
         launched += k;
-        batch = std::min(batch * 2, 64);
+        batch = s->lastMaxIters > 8 ? 3 : std::min(batch * 2, 32);
     }
 }
 
@@ -318,11 +339,12 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     const uint8_t *pre = skip ? skip : s->zeroMask;
     if (s->precision == OD_FP64) {
         build_precond(s, A, pre);
-        run_pcg<double>(s, A->vals, s->U, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre, s->scIn,
-                        s->doneIn, false);
+        run_pcg<double>(s, A->vals, A->low, s->U, s->Ul, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre,
+                        s->scIn, s->doneIn, false);
         collect_stats(s, s->scIn, false);
         collect_relres(s, s->scIn);
         s->outer = 0;
+        s->lastMaxIters = *std::max_element(s->iters.begin(), s->iters.end());
         return;
     }
     // ---- mixed: FP32 inner PCG, FP64 residual and correction (iterative refinement)
@@ -330,7 +352,7 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     build_precond(s, A, pre);
     std::fill(s->iters.begin(), s->iters.end(), 0);
     PcgWork wo{s->partials, s->scOut, s->doneOut};
-    launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr, nullptr);
+    launch_spmv<double>(m, st, A->vals, A->low, x, s->Ax, 0, nullptr);
     launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 1);
     int outer = 0;
     for (; outer < s->maxOuter; outer++) {
@@ -338,11 +360,16 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
         hipLaunchKernelGGL(k_done_to_mask, dim3((t.nInst + 255) / 256), dim3(256), 0, st, t.nInst, s->scOut, pre, s->mask);
         OD_HIP(hipGetLastError());
         OD_HIP(hipMemsetAsync(s->x32, 0, sizeof(float) * t.ndof, st));
-        run_pcg<float>(s, A->vals32, s->U32, s->invDiag32, s->r32, s->x32, s->ri32, s->z32, s->p32, s->q32, s->innerTol,
-                       maxIt, s->mask, s->scIn, s->doneIn, true);
-        collect_stats(s, s->scIn, true);
+        run_pcg<float>(s, A->vals32, A->low32, s->U32, s->Ul32, s->invDiag32, s->r32, s->x32, s->ri32, s->z32, s->p32,
+                       s->q32, s->innerTol, maxIt, s->mask, s->scIn, s->doneIn, true);
+        {
+            const int before = *std::max_element(s->iters.begin(), s->iters.end());
+            collect_stats(s, s->scIn, true);
+            const int after = *std::max_element(s->iters.begin(), s->iters.end());
+            s->lastMaxIters = after - before;
+        }
         launch_add_correction(m, st, s->x32, x, s->scOut);
-        launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr, nullptr);
+        launch_spmv<double>(m, st, A->vals, A->low, x, s->Ax, 0, nullptr);
         launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 0);
     }
     s->outer = outer;
@@ -493,6 +520,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         std::vector<int> tOff(t.tBlk.size());
         for (size_t k = 0; k < t.tBlk.size(); k++) tOff[k] = t.blkOff[t.tBlk[k]];
         m.tOff = mesh->upload(tOff);
+        m.tPos = mesh->upload(t.tPos);
         m.tPtr = mesh->upload(t.tPtr);
         m.tRow = mesh->upload(t.tRow);
         m.tBlk = mesh->upload(t.tBlk);
@@ -506,6 +534,8 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.chunkRow0 = mesh->upload(t.chunkRow0);
         m.chunkRow1 = mesh->upload(t.chunkRow1);
         m.chunkInst = mesh->upload(t.chunkInst);
+        if (std::getenv("OD_CHUNK_ORDER_IDENTITY")) std::iota(t.chunkOrder.begin(), t.chunkOrder.end(), 0);
+        m.chunkOrder = mesh->upload(t.chunkOrder);
         m.instChunkStart = mesh->upload(t.instChunkStart);
         m.instRowStart = mesh->upload(t.instRowStart);
         m.tetInst = mesh->upload(t.tetInst);
@@ -622,7 +652,7 @@ int od_matrix_create(od_mesh *mesh, od_matrix **out) {
     require(mesh && out, "NULL argument");
     od_matrix *m = new od_matrix();
     m->mesh = mesh;
-    size_t b = sizeof(double) * (size_t)mesh->topo.nValues;
+    size_t b = sizeof(double) * ((size_t)mesh->topo.nValues + kValuePad);
     if (hipMalloc(&m->vals, b) != hipSuccess) { delete m; throw ApiError(OD_ERR_HIP, "hipMalloc failed for matrix values"); }
     OD_HIP(hipMemsetAsync(m->vals, 0, b, mesh->ctx->stream));
     *out = m;
@@ -702,7 +732,7 @@ int od_spmv(od_matrix *A, const double *x, double *y, int accumulate) {
     API_BEGIN
     require(A && x && y, "NULL argument");
     require(x != y, "x and y must differ (SpMDV asserts src != dest)");
-    launch_spmv<double>(A->mesh->dm, A->mesh->ctx->stream, A->vals, x, y, accumulate, nullptr, nullptr);
+    launch_spmv<double>(A->mesh->dm, A->mesh->ctx->stream, A->vals, A->low, x, y, accumulate, nullptr);
     API_END
 }
 
@@ -710,7 +740,10 @@ int od_matrix_sync_f32(od_matrix *A) {
     API_BEGIN
     require(A != nullptr, "NULL argument");
     int64_t n = A->mesh->topo.nValues;
-    if (!A->vals32) OD_HIP(hipMalloc(&A->vals32, sizeof(float) * (size_t)n));
+    if (!A->vals32) {
+        OD_HIP(hipMalloc(&A->vals32, sizeof(float) * ((size_t)n + kValuePad)));
+        OD_HIP(hipMemsetAsync(A->vals32, 0, sizeof(float) * ((size_t)n + kValuePad), A->mesh->ctx->stream));
+    }
     launch_cast_f64_f32(A->mesh->ctx->stream, A->vals, A->vals32, n);
     API_END
 }
@@ -719,7 +752,7 @@ int od_spmv_f32(od_matrix *A, const float *x, float *y, int accumulate) {
     API_BEGIN
     require(A && x && y && x != y, "bad arguments");
     require(A->vals32 != nullptr, "call od_matrix_sync_f32 first", OD_ERR_STATE);
-    launch_spmv<float>(A->mesh->dm, A->mesh->ctx->stream, A->vals32, x, y, accumulate, nullptr, nullptr);
+    launch_spmv<float>(A->mesh->dm, A->mesh->ctx->stream, A->vals32, A->low32, x, y, accumulate, nullptr);
     API_END
 }
 
@@ -733,8 +766,8 @@ int od_spmv_timed(od_matrix *A, const void *x, void *y, int iters, int fp32, dou
     OD_HIP(hipEventCreate(&e1));
     OD_HIP(hipEventRecord(e0, st));
     for (int k = 0; k < iters; k++) {
-        if (fp32) launch_spmv<float>(A->mesh->dm, st, A->vals32, (const float *)x, (float *)y, 0, nullptr, nullptr);
-        else launch_spmv<double>(A->mesh->dm, st, A->vals, (const double *)x, (double *)y, 0, nullptr, nullptr);
+        if (fp32) launch_spmv<float>(A->mesh->dm, st, A->vals32, A->low32, (const float *)x, (float *)y, 0, nullptr);
+        else launch_spmv<double>(A->mesh->dm, st, A->vals, A->low, (const double *)x, (double *)y, 0, nullptr);
     }
     OD_HIP(hipEventRecord(e1, st));
     OD_HIP(hipEventSynchronize(e1));
@@ -770,9 +803,12 @@ int od_pcg_solve(od_solver *s, od_matrix *A, const double *b, double *x, double 
     API_BEGIN
     require(s && A && b && x && A->mesh == s->mesh, "bad arguments");
     require(tol >= 0 && maxIter >= 0, "tol and maxIter must be non-negative");
-    if (s->precision == OD_MIXED && !A->vals32) {
+    if (s->precision == OD_MIXED) {
         int64_t n = A->mesh->topo.nValues;
-        OD_HIP(hipMalloc(&A->vals32, sizeof(float) * (size_t)n));
+        if (!A->vals32) {
+            OD_HIP(hipMalloc(&A->vals32, sizeof(float) * ((size_t)n + kValuePad)));
+            OD_HIP(hipMemsetAsync(A->vals32, 0, sizeof(float) * ((size_t)n + kValuePad), A->mesh->ctx->stream));
+        }
         launch_cast_f64_f32(A->mesh->ctx->stream, A->vals, A->vals32, n);
     }
     solve(s, A, b, x, tol, maxIter, nullptr);
@@ -789,7 +825,8 @@ int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
     od_mesh *mesh = s->mesh;
     build_precond(s, A, s->zeroMask);
     PcgWork w{s->partials, s->scIn, s->doneIn};
-    if (s->precond == OD_PRE_IC0) launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U, r, z, nullptr, nullptr);
+    if (s->precond == OD_PRE_IC0)
+        launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U, s->Ul, r, z, nullptr, nullptr);
     else pcg_jacobi<double>(mesh->dm, mesh->ctx->stream, s->precond == OD_PRE_JACOBI ? s->invDiag : nullptr, r, z, w);
     OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
     API_END
@@ -817,9 +854,11 @@ static od_integrator *make_integrator(od_mesh *mesh, int kind, int precond, doub
         size_t n = (size_t)mesh->topo.ndof, nv = (size_t)mesh->topo.nValues;
         it->M.mesh = mesh;
         it->A.mesh = mesh;
-        it->M.vals = (double *)it->alloc(8 * nv);
-        it->A.vals = (double *)it->alloc(8 * nv);
-        if (precision == OD_MIXED) it->A.vals32 = (float *)it->alloc(4 * nv);
+        it->M.vals = (double *)it->alloc(8 * (nv + kValuePad));
+        it->A.vals = (double *)it->alloc(8 * (nv + kValuePad));
+        if (precision == OD_MIXED) it->A.vals32 = (float *)it->alloc(4 * (nv + kValuePad));
+        // A keeps no lower copy: re-reading the shared upper storage through the transposed
+        // index hits L2/MALL, while a second 9-values-per-block array doubles the unique bytes
         it->d = (double *)it->alloc(8 * n); it->v = (double *)it->alloc(8 * n);
         it->a = (double *)it->alloc(8 * n); it->aNew = (double *)it->alloc(8 * n);
         it->fext = (double *)it->alloc(8 * n); it->imp = (double *)it->alloc(8 * n);
@@ -928,7 +967,7 @@ static void integrator_step(od_integrator *it) {
         ElemParams P{0, 1, h + aK, (0.5 - beta) * h * h + aK * (1.0 - gamma) * h, 1.0, (1.0 - gamma) * h, aM};
         launch_element(m, st, it->d, it->v, it->a, P, mesh->blkC, mesh->nodeC, act);
         const double cK = gamma * h * aK + beta * h * h, cM = 1.0 + gamma * h * aM;
-        launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act);
+        launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act, it->A.low, it->A.low32);
         launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, 1.0, it->rhs, 0, act);
         OD_HIP(hipMemcpyAsync(it->aNew, it->a, sizeof(double) * mesh->topo.ndof, hipMemcpyDeviceToDevice, st));
         solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);
@@ -940,7 +979,8 @@ static void integrator_step(od_integrator *it) {
         const double aM = it->alphaM, aK = it->alphaK;
         ElemParams P{0, 1, h + aK, 0.0, 1.0, 0.0, aM};
         launch_element(m, st, it->d, it->v, nullptr, P, mesh->blkC, mesh->nodeC, act);
-        launch_gather_blocks(m, st, mesh->blkC, h * (h + aK), it->M.vals, 1.0 + h * aM, it->A.vals, 0, nullptr, act);
+        launch_gather_blocks(m, st, mesh->blkC, h * (h + aK), it->M.vals, 1.0 + h * aM, it->A.vals, 0, nullptr, act, it->A.low,
+                             nullptr);
         launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, h, it->rhs, 0, act);
         // delta_v persists across steps as the warm start (aNew holds it)
         solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);

@@ -24,6 +24,7 @@ struct HipError : std::runtime_error {
 
 constexpr int kChunkRows = 64;      // block rows per workgroup chunk (never straddles an instance)
 constexpr int kBlock = 256;         // threads per workgroup
+constexpr int kValuePad = 16;       // trailing entries allocated after every value array
 
 // Device copy of the topology and material.
 struct DevMesh {
@@ -35,11 +36,13 @@ struct DevMesh {
     int *rowPtr, *colIdx, *blkOff, *blkRow;     // pattern
     int *valOff;                                 // first value of each block row (nRows+1)
     int *tPtr, *tRow, *tBlk, *tOff;             // transposed index (+ value offset of each block)
+    int *tPos;                                   // per block: its place in the transposed list (-1: diagonal)
     int *pairBlk, *pairSlot; uint8_t *pairT;    // 10 per tet
     int *blkCStart;                              // nBlocks+1
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
     int *levelRows;                              // nRows
     int *chunkRow0, *chunkRow1, *chunkInst;      // nChunks
+    int *chunkOrder;      // workgroup -> chunk: consecutive workgroups of one XCD get spatially adjacent chunks
     int *instChunkStart, *instRowStart;          // nInst+1
     int *tetInst, *rowInst;
     // material (per tet)
@@ -65,18 +68,19 @@ struct ElemParams {
 void launch_element(const DevMesh &m, hipStream_t s, const double *d, const double *v, const double *a,
                     const ElemParams &p, double *blkC, double *nodeC, const uint8_t *instActive);
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
-                          double *out, int accumulate, float *out32, const uint8_t *instActive);
+                          double *out, int accumulate, float *out32, const uint8_t *instActive, double *low = nullptr,
+                          float *low32 = nullptr);
 void launch_gather_rhs(const DevMesh &m, hipStream_t s, const double *nodeC, const double *fext, const double *imp,
                        double cf, double *out, int mode, const uint8_t *instActive);
 void launch_body_force(const DevMesh &m, hipStream_t s, const double *bodyPerTet, double *nodeC, double *f);
 void launch_expand_map(const DevMesh &m, hipStream_t s, int *map78);
 template <typename T>
-void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
-                 double *partials, const InstScalars *sc);
+void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *x, T *y, int accumulate,
+                 const InstScalars *sc);
 
 // PCG building blocks (see od_kernels.hip for the exact phase semantics)
 struct PcgWork {
-    double *partials;     // nChunks * 4 (slots: r.z or p.q, r.r, b.b)
+    double *partials;     // 3 * nChunks, slot q of chunk c at [q * nChunks + c] (slots: r.z or p.q, r.r, b.b)
     InstScalars *sc;      // nInst
     int *doneFlags;       // nInst, mirrors sc[i].done for host polling
 };
@@ -88,7 +92,8 @@ template <typename T>
 void pcg_dot_rz_init(const DevMesh &m, hipStream_t s, const T *r, const T *z, T *p, PcgWork &w, double tol, int maxIt,
                      const uint8_t *preDone);
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w);
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *p, T *q, PcgWork &w);
+void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w);
 template <typename T>
 void pcg_update_xr(const DevMesh &m, hipStream_t s, const T *p, const T *q, T *x, T *r, const T *invDiag, T *z,
                    PcgWork &w);
@@ -100,10 +105,10 @@ template <typename T>
 void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *invDiag);
 template <typename T>
 void launch_ic0_factor(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const double *A, double *U,
-                       T *U32, double pivotEps, const uint8_t *skip);
+                       T *U32, double *Ul, T *Ul32, double pivotEps, const uint8_t *skip);
 template <typename T>
-void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *U, const T *r, T *z,
-                      const InstScalars *sc, const uint8_t *skip);
+void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *U, const T *Ul,
+                      const T *r, T *z, const InstScalars *sc, const uint8_t *skip);
 void launch_cast_f64_f32(hipStream_t s, const double *a, float *b, int64_t n);
 void launch_residual_f64(const DevMesh &m, hipStream_t s, const double *b, const double *Ax, double *r, float *r32,
                          PcgWork &w, double tol, const uint8_t *preDone, int first);

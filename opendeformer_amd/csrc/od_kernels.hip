@@ -235,7 +235,8 @@ __constant__ int c_diagSrc[6] = {0, 1, 2, 4, 5, 8};
 
 __global__ void k_gather_blocks(DevMesh m, const double *__restrict__ C, double cK, const double *__restrict__ M,
                                 double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
-                                const uint8_t *__restrict__ instActive) {
+                                const uint8_t *__restrict__ instActive, double *__restrict__ low,
+                                float *__restrict__ low32) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= m.nBlocks * 9) return;
     const int b = (int)(gid / 9), c = (int)(gid % 9);
@@ -252,13 +253,20 @@ __global__ void k_gather_blocks(DevMesh m, const double *__restrict__ C, double 
     if (accumulate) o += out[vi];
     out[vi] = o;
     if (out32) out32[vi] = (float)o;
+    if (!diag) {   // lower copy: the block again at its place in the transposed list of its column row
+        const int64_t li = 9 * (int64_t)m.tPos[b] + c;
+        if (low) low[li] = o;
+        if (low32) low32[li] = (float)o;
+    }
 }
 
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
-                          double *out, int accumulate, float *out32, const uint8_t *instActive) {
+                          double *out, int accumulate, float *out32, const uint8_t *instActive, double *low,
+                          float *low32) {
     int64_t n = m.nBlocks * 9;
     int grid = (int)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_gather_blocks, dim3(grid), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out, accumulate, out32, instActive);
+    hipLaunchKernelGGL(k_gather_blocks, dim3(grid), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out, accumulate, out32, instActive,
+                       low, low32);
     OD_HIP(hipGetLastError());
 }
 
@@ -385,16 +393,16 @@ struct FinalArgs {
 };
 
 // One workgroup per instance: reduce its chunk partials (fixed order) and update its scalars.
-__global__ __launch_bounds__(256) void k_finalize(DevMesh m, int phase, const double *__restrict__ partials,
+__global__ __launch_bounds__(1024) void k_finalize(DevMesh m, int phase, const double *__restrict__ partials,
                                                   InstScalars *__restrict__ sc, int *__restrict__ doneFlags,
                                                   FinalArgs fa) {
     const int inst = blockIdx.x;
     const int c0 = m.instChunkStart[inst], c1 = m.instChunkStart[inst + 1];
     double t[3] = {0, 0, 0};
     for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
-        t[0] += partials[4 * c + 0];
-        t[1] += partials[4 * c + 1];
-        t[2] += partials[4 * c + 2];
+        t[0] += partials[c];
+        t[1] += partials[m.nChunks + c];
+        t[2] += partials[2 * m.nChunks + c];
     }
     block_sum<3>(t);
     if (threadIdx.x != 0) return;
@@ -432,7 +440,9 @@ __global__ __launch_bounds__(256) void k_finalize(DevMesh m, int phase, const do
 
 void launch_finalize(const DevMesh &m, hipStream_t s, int phase, PcgWork &w, double tol, int maxIt,
                      const uint8_t *preDone) {
-    hipLaunchKernelGGL(k_finalize, dim3(m.nInst), dim3(256), 0, s, m, phase, w.partials, w.sc, w.doneFlags,
+    // a wide workgroup for few large instances, a narrow one for many small instances
+    const int threads = m.nInst == 1 ? 1024 : 256;
+    hipLaunchKernelGGL(k_finalize, dim3(m.nInst), dim3(threads), 0, s, m, phase, w.partials, w.sc, w.doneFlags,
                        FinalArgs{tol, maxIt, preDone});
     OD_HIP(hipGetLastError());
 }
@@ -440,60 +450,60 @@ void launch_finalize(const DevMesh &m, hipStream_t s, int phase, PcgWork &w, dou
 // =====================================================================================
 // SpMV over the canonical upper-BSR layout: one workgroup = one 64-row chunk, 8 lanes per row
 // =====================================================================================
+// One block row with 16 lanes, one branch-free round for interior rows: item 0 is the packed
+// diagonal block, items 1..nOff the row's upper blocks (contiguous values), the remaining items
+// the transposed lower blocks reached through the transposed index.  Value arrays are padded
+// by kValuePad entries so the 9-value load of a trailing diagonal block stays in bounds.
 template <typename T>
-__device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x, int r,
-                                         int lane8, T &o0, T &o1, T &o2) {
+__device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ low,
+                                         const T *__restrict__ x, int r, int lane, T &o0, T &o1, T &o2) {
     const int v0 = m.valOff[r], b0 = m.rowPtr[r];
-    const int nOff = m.rowPtr[r + 1] - b0 - 1;          // upper off-diagonal blocks
-    const int t0 = m.tPtr[r], nt = m.tPtr[r + 1] - t0;  // transposed (lower) blocks
+    const int nOff = m.rowPtr[r + 1] - b0 - 1;
+    const int t0 = m.tPtr[r], nItems = 1 + nOff + (m.tPtr[r + 1] - t0);
     T a0 = 0, a1 = 0, a2 = 0;
-    if (lane8 == 0) {  // packed symmetric diagonal block
-        const T *D = vals + v0;
-        const T x0 = x[3 * r], x1 = x[3 * r + 1], x2 = x[3 * r + 2];
-        a0 = D[0] * x0 + D[1] * x1 + D[2] * x2;
-        a1 = D[1] * x0 + D[3] * x1 + D[4] * x2;
-        a2 = D[2] * x0 + D[4] * x1 + D[5] * x2;
-    }
-    for (int it = lane8; it < nOff + nt; it += 8) {
-        if (it < nOff) {
-            const T *A = vals + v0 + 6 + 9 * it;
-            const int c = m.colIdx[b0 + 1 + it];
-            const T x0 = x[3 * c], x1 = x[3 * c + 1], x2 = x[3 * c + 2];
-            a0 += A[0] * x0 + A[1] * x1 + A[2] * x2;
-            a1 += A[3] * x0 + A[4] * x1 + A[5] * x2;
-            a2 += A[6] * x0 + A[7] * x1 + A[8] * x2;
-        } else {
-            const int k = t0 + it - nOff;
-            const T *A = vals + m.tOff[k];
-            const int K = m.tRow[k];
-            const T x0 = x[3 * K], x1 = x[3 * K + 1], x2 = x[3 * K + 2];
-            a0 += A[0] * x0 + A[3] * x1 + A[6] * x2;
-            a1 += A[1] * x0 + A[4] * x1 + A[7] * x2;
-            a2 += A[2] * x0 + A[5] * x1 + A[8] * x2;
+    for (int it = lane; it < nItems; it += 16) {
+        const bool up = it <= nOff;                       // diagonal or upper block
+        const int k = t0 + (up ? 0 : it - nOff - 1);
+        const int col = up ? m.colIdx[b0 + it] : m.tRow[k];
+        // transposed blocks: contiguous per row in the lower copy when the matrix keeps one
+        const T *A = up ? vals + (it == 0 ? v0 : v0 + 6 + 9 * (it - 1)) : (low ? low + 9 * k : vals + m.tOff[k]);
+        T e0 = A[0], e1 = A[1], e2 = A[2], e3 = A[3], e4 = A[4], e5 = A[5], e6 = A[6], e7 = A[7], e8 = A[8];
+        const T x0 = x[3 * col], x1 = x[3 * col + 1], x2 = x[3 * col + 2];
+        if (it == 0) {            // packed symmetric diagonal [e0 e1 e2; e1 e3 e4; e2 e4 e5]
+            e8 = e5; e7 = e4; e6 = e2; e5 = e4; e4 = e3; e3 = e1;
+        } else if (!up) {         // transposed block
+            T t;
+            t = e1; e1 = e3; e3 = t;
+            t = e2; e2 = e6; e6 = t;
+            t = e5; e5 = e7; e7 = t;
         }
+        a0 += e0 * x0 + e1 * x1 + e2 * x2;
+        a1 += e3 * x0 + e4 * x1 + e5 * x2;
+        a2 += e6 * x0 + e7 * x1 + e8 * x2;
     }
 #pragma unroll
-    for (int o = 4; o > 0; o >>= 1) {
-        a0 += __shfl_xor(a0, o, 8);
-        a1 += __shfl_xor(a1, o, 8);
-        a2 += __shfl_xor(a2, o, 8);
+    for (int o = 8; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 16);
+        a1 += __shfl_xor(a1, o, 16);
+        a2 += __shfl_xor(a2, o, 16);
     }
     o0 = a0; o1 = a1; o2 = a2;
 }
 
-constexpr int kSpmvThreads = 8 * kChunkRows;
+constexpr int kSpmvThreads = 16 * kChunkRows;
 
 // y = A x (or y += A x); optional chunk partial x.y into slot 0.
 template <typename T>
-__global__ __launch_bounds__(kSpmvThreads) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
-                                                       T *__restrict__ y, int accumulate, const InstScalars *sc) {
-    const int chunk = blockIdx.x;
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ low,
+                                                       const T *__restrict__ x, T *__restrict__ y, int accumulate,
+                                                       const InstScalars *sc) {
+    const int chunk = m.chunkOrder[blockIdx.x];
     if (sc && sc[m.chunkInst[chunk]].done) return;
-    const int r = m.chunkRow0[chunk] + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
+    const int r = m.chunkRow0[chunk] + (threadIdx.x >> 4), lane = threadIdx.x & 15;
     if (r >= m.chunkRow1[chunk]) return;
     T o0, o1, o2;
-    row_spmv<T>(m, vals, x, r, lane8, o0, o1, o2);
-    if (lane8 == 0) {
+    row_spmv<T>(m, vals, low, x, r, lane, o0, o1, o2);
+    if (lane == 0) {
         if (accumulate) { o0 += y[3 * r]; o1 += y[3 * r + 1]; o2 += y[3 * r + 2]; }
         y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
     }
@@ -501,41 +511,44 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(DevMesh m, const T *__res
 
 // The PCG product q = A p with the chunk partial p.q (slot 0).
 template <typename T>
-__global__ __launch_bounds__(kSpmvThreads) void k_spmv_dot(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv_dot(DevMesh m, const T *__restrict__ vals,
+                                                           const T *__restrict__ low, const T *__restrict__ x,
                                                            T *__restrict__ y, double *__restrict__ partials,
                                                            const InstScalars *__restrict__ sc) {
-    const int chunk = blockIdx.x;
+    const int chunk = m.chunkOrder[blockIdx.x];
     const bool skip = sc[m.chunkInst[chunk]].done;
-    const int r = m.chunkRow0[chunk] + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
+    const int r = m.chunkRow0[chunk] + (threadIdx.x >> 4), lane = threadIdx.x & 15;
     double v[1] = {0.0};
     if (!skip && r < m.chunkRow1[chunk]) {
         T o0, o1, o2;
-        row_spmv<T>(m, vals, x, r, lane8, o0, o1, o2);
-        if (lane8 == 0) {
+        row_spmv<T>(m, vals, low, x, r, lane, o0, o1, o2);
+        if (lane == 0) {
             y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
             v[0] = (double)x[3 * r] * (double)o0 + (double)x[3 * r + 1] * (double)o1 + (double)x[3 * r + 2] * (double)o2;
         }
     }
     block_sum<1>(v);
-    if (threadIdx.x == 0) partials[4 * chunk] = v[0];
+    if (threadIdx.x == 0) partials[chunk] = v[0];
 }
 
 template <typename T>
-void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate, double *partials,
+void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *x, T *y, int accumulate,
                  const InstScalars *sc) {
-    (void)partials;
-    hipLaunchKernelGGL(k_spmv<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, x, y, accumulate, sc);
+    hipLaunchKernelGGL(k_spmv<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, low, x, y, accumulate, sc);
     OD_HIP(hipGetLastError());
 }
-template void launch_spmv<double>(const DevMesh &, hipStream_t, const double *, const double *, double *, int, double *,
-                                  const InstScalars *);
-template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, const float *, float *, int, double *,
+template void launch_spmv<double>(const DevMesh &, hipStream_t, const double *, const double *, const double *, double *,
+                                  int, const InstScalars *);
+template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, const float *, const float *, float *, int,
                                  const InstScalars *);
 
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w) {
-    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, p, q, w.partials, w.sc);
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *p, T *q, PcgWork &w) {
+    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, low, p, q, w.partials, w.sc);
     OD_HIP(hipGetLastError());
+}
+
+void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w) {
     launch_finalize(m, s, PH_ALPHA, w, 0.0, 0, nullptr);
 }
 
@@ -563,7 +576,7 @@ __global__ __launch_bounds__(kVecThreads) void k_init_res(DevMesh m, const T *b,
         v[1] = (double)b[i] * (double)b[i];
     }
     block_sum<2>(v);
-    if (threadIdx.x == 0) { partials[4 * chunk + 1] = v[0]; partials[4 * chunk + 2] = v[1]; }
+    if (threadIdx.x == 0) { partials[m.nChunks + chunk] = v[0]; partials[2 * m.nChunks + chunk] = v[1]; }
 }
 
 template <typename T>
@@ -596,7 +609,7 @@ __global__ __launch_bounds__(kVecThreads) void k_init_rz(DevMesh m, const T *r, 
         v[0] = (double)r[i] * (double)z[i];
     }
     block_sum<1>(v);
-    if (threadIdx.x == 0) partials[4 * chunk] = v[0];
+    if (threadIdx.x == 0) partials[chunk] = v[0];
 }
 
 template <typename T>
@@ -629,8 +642,8 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xr(DevMesh m, const T *p
     }
     block_sum<2>(v);
     if (threadIdx.x == 0) {
-        partials[4 * chunk + 1] = v[1];
-        if (fuseJacobi) partials[4 * chunk] = v[0];
+        partials[m.nChunks + chunk] = v[1];
+        if (fuseJacobi) partials[chunk] = v[0];
     }
 }
 
@@ -651,7 +664,7 @@ __global__ __launch_bounds__(kVecThreads) void k_dot_rz(DevMesh m, const T *r, c
     double v[1] = {0};
     if (in && !sc[inst].done) v[0] = (double)r[i] * (double)z[i];
     block_sum<1>(v);
-    if (threadIdx.x == 0) partials[4 * chunk] = v[0];
+    if (threadIdx.x == 0) partials[chunk] = v[0];
 }
 
 template <typename T>
@@ -698,91 +711,115 @@ void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *
 // =====================================================================================
 __device__ __forceinline__ double pivot_fix(double d, double eps2) { return d <= 0.0 ? eps2 * d * d : d; }
 
+// One block row of U per 8-lane group: lane 0 forms and factors the diagonal block,
+// lanes 1..7 the row's upper off-diagonal blocks U_IJ = U_II^-T (A_IJ - sum_K U_KI^T U_KJ).
 template <typename T>
-__global__ void k_ic0_factor_level(DevMesh m, const int *__restrict__ rows, int count, const double *__restrict__ A,
-                                   double *__restrict__ U, T *__restrict__ U32, double eps2,
-                                   const uint8_t *__restrict__ skip) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *__restrict__ rows, int count,
+                                                          const double *__restrict__ A, double *__restrict__ U,
+                                                          T *__restrict__ U32, double *__restrict__ Ul,
+                                                          T *__restrict__ Ul32, double eps2,
+                                                          const uint8_t *__restrict__ skip) {
+    const int wg = ((count + 31) >> 5) > 0 ? ((int)(blockIdx.x & 7) * ((((count + 31) >> 5) + 7) >> 3) + (int)(blockIdx.x >> 3)) : 0;
+    const int t = wg * 32 + (threadIdx.x >> 3), lane = threadIdx.x & 7;
     if (t >= count) return;
     const int I = rows[t];
     if (skip && skip[m.rowInst[I]]) return;
-    const int b0 = m.rowPtr[I], b1 = m.rowPtr[I + 1];
-    const int v0 = m.valOff[I], v1 = m.valOff[I + 1];
-    // row I of U starts as row I of A
-    for (int k = v0; k < v1; k++) U[k] = A[k];
-    double s00 = U[v0], s01 = U[v0 + 1], s02 = U[v0 + 2], s11 = U[v0 + 3], s12 = U[v0 + 4], s22 = U[v0 + 5];
-    // subtract sum over K < I of U_KI^T U_KJ (J in row I's pattern, J >= I)
-    for (int k = m.tPtr[I]; k < m.tPtr[I + 1]; k++) {
-        const int K = m.tRow[k];
-        const double *W = U + m.tOff[k];  // U_KI, row-major
-        double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5], w6 = W[6], w7 = W[7], w8 = W[8];
-        // diagonal: S_II -= W^T W
-        s00 -= w0 * w0 + w3 * w3 + w6 * w6;
-        s01 -= w0 * w1 + w3 * w4 + w6 * w7;
-        s02 -= w0 * w2 + w3 * w5 + w6 * w8;
-        s11 -= w1 * w1 + w4 * w4 + w7 * w7;
-        s12 -= w1 * w2 + w4 * w5 + w7 * w8;
-        s22 -= w2 * w2 + w5 * w5 + w8 * w8;
-        // off-diagonal: merge row K's columns (> I) with row I's columns
-        int p = m.tBlk[k] + 1, pe = m.rowPtr[K + 1];
-        int q = b0 + 1;
-        while (p < pe && q < b1) {
-            const int cp = m.colIdx[p], cq = m.colIdx[q];
-            if (cp < cq) { p++; continue; }
-            if (cq < cp) { q++; continue; }
-            const double *Z = U + m.blkOff[p];
-            double *S = U + m.blkOff[q];
+    const int b0 = m.rowPtr[I], nOff = m.rowPtr[I + 1] - b0 - 1, v0 = m.valOff[I];
+    const int k0 = m.tPtr[I], k1 = m.tPtr[I + 1];
+    double u[6] = {0, 0, 0, 0, 0, 0};
+    if (lane == 0) {
+        double s00 = A[v0], s01 = A[v0 + 1], s02 = A[v0 + 2], s11 = A[v0 + 3], s12 = A[v0 + 4], s22 = A[v0 + 5];
+        for (int k = k0; k < k1; k++) {
+            const double *W = Ul + 9 * k;
+            const double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5], w6 = W[6], w7 = W[7], w8 = W[8];
+            s00 -= w0 * w0 + w3 * w3 + w6 * w6;
+            s01 -= w0 * w1 + w3 * w4 + w6 * w7;
+            s02 -= w0 * w2 + w3 * w5 + w6 * w8;
+            s11 -= w1 * w1 + w4 * w4 + w7 * w7;
+            s12 -= w1 * w2 + w4 * w5 + w7 * w8;
+            s22 -= w2 * w2 + w5 * w5 + w8 * w8;
+        }
+        // dense Cholesky of the 3x3 diagonal block, pivot rule d <= 0 -> eps^2 d^2
+        u[0] = sqrt(pivot_fix(s00, eps2));
+        u[1] = s01 / u[0];
+        u[2] = s02 / u[0];
+        u[3] = sqrt(pivot_fix(s11 - u[1] * u[1], eps2));
+        u[4] = (s12 - u[1] * u[2]) / u[3];
+        u[5] = sqrt(pivot_fix(s22 - u[2] * u[2] - u[4] * u[4], eps2));
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            U[v0 + j] = u[j];
+            if (U32) U32[v0 + j] = (T)u[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = __shfl(u[j], 0, 8);
+    if (lane == 0) return;
+    for (int q = lane - 1; q < nOff; q += 7) {
+        const int J = m.colIdx[b0 + 1 + q];
+        const int vq = v0 + 6 + 9 * q;
+        double S[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) S[j] = A[vq + j];
+        for (int k = k0; k < k1; k++) {
+            // U_KJ: row K's upper blocks after (K, I) are sorted; find column J
+            const int K = m.tRow[k];
+            const int pe = m.rowPtr[K + 1];
+            int p = m.tBlk[k] + 1;
+            while (p < pe && m.colIdx[p] < J) p++;
+            if (p >= pe || m.colIdx[p] != J) continue;
+            const double *W = Ul + 9 * k;
+            const double *Z = U + m.valOff[K] + 6 + 9 * (p - m.rowPtr[K] - 1);
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int j = 0; j < 3; j++) S[i * 3 + j] -= W[0 * 3 + i] * Z[0 * 3 + j] + W[1 * 3 + i] * Z[1 * 3 + j] + W[2 * 3 + i] * Z[2 * 3 + j];
-            p++; q++;
         }
-    }
-    // dense Cholesky of the 3x3 diagonal block, pivot rule d <= 0 -> eps^2 d^2
-    double u00 = sqrt(pivot_fix(s00, eps2));
-    double u01 = s01 / u00, u02 = s02 / u00;
-    double u11 = sqrt(pivot_fix(s11 - u01 * u01, eps2));
-    double u12 = (s12 - u01 * u02) / u11;
-    double u22 = sqrt(pivot_fix(s22 - u02 * u02 - u12 * u12, eps2));
-    U[v0] = u00; U[v0 + 1] = u01; U[v0 + 2] = u02; U[v0 + 3] = u11; U[v0 + 4] = u12; U[v0 + 5] = u22;
-    // U_IJ = U_II^-T S_IJ (forward substitution with the lower factor U_II^T)
-    for (int b = b0 + 1; b < b1; b++) {
-        double *S = U + m.blkOff[b];
-        double x0[3], x1[3], x2[3];
+        // U_IJ = U_II^-T S (forward substitution with the lower factor U_II^T)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            x0[j] = S[j] / u00;
-            x1[j] = (S[3 + j] - u01 * x0[j]) / u11;
-            x2[j] = (S[6 + j] - u02 * x0[j] - u12 * x1[j]) / u22;
+            const double x0 = S[j] / u[0];
+            const double x1 = (S[3 + j] - u[1] * x0) / u[3];
+            const double x2 = (S[6 + j] - u[2] * x0 - u[4] * x1) / u[5];
+            U[vq + j] = x0; U[vq + 3 + j] = x1; U[vq + 6 + j] = x2;
+            if (U32) { U32[vq + j] = (T)x0; U32[vq + 3 + j] = (T)x1; U32[vq + 6 + j] = (T)x2; }
+            const int64_t lq = 9 * (int64_t)m.tPos[b0 + 1 + q];
+            Ul[lq + j] = x0; Ul[lq + 3 + j] = x1; Ul[lq + 6 + j] = x2;
+            if (Ul32) { Ul32[lq + j] = (T)x0; Ul32[lq + 3 + j] = (T)x1; Ul32[lq + 6 + j] = (T)x2; }
         }
-#pragma unroll
-        for (int j = 0; j < 3; j++) { S[j] = x0[j]; S[3 + j] = x1[j]; S[6 + j] = x2[j]; }
     }
-    if (U32)
-        for (int k = v0; k < v1; k++) U32[k] = (T)U[k];
 }
 
 template <typename T>
 void launch_ic0_factor(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const double *A, double *U,
-                       T *U32, double pivotEps, const uint8_t *skip) {
+                       T *U32, double *Ul, T *Ul32, double pivotEps, const uint8_t *skip) {
     const double eps2 = pivotEps * pivotEps;
     for (size_t l = 0; l + 1 < levelPtr.size(); l++) {
         int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(k_ic0_factor_level<T>, dim3((cnt + 127) / 128), dim3(128), 0, s, m, m.levelRows + levelPtr[l],
-                           cnt, A, U, U32, eps2, skip);
+        hipLaunchKernelGGL(k_ic0_factor_level<T>, dim3(8 * ((((cnt + 31) / 32) + 7) / 8)), dim3(256), 0, s, m,
+                           m.levelRows + levelPtr[l], cnt, A, U, U32, Ul, Ul32, eps2, skip);
     }
     OD_HIP(hipGetLastError());
 }
+
+// XCD-aware workgroup index for level kernels: the launch has 8*ceil(nwg/8) workgroups;
+// XCD x (= blockIdx mod 8) runs the contiguous run [x*per, (x+1)*per) of the level's rows.
+__device__ __forceinline__ int xcd_wg(int nwg) {
+    const int per = (nwg + 7) >> 3;
+    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+static inline int xcd_grid(int nwg) { return 8 * ((nwg + 7) / 8); }
 
 // IC(0) sweeps, 8 lanes per row, one launch per level (rows of a level are independent).
 // forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K)
 template <typename T>
 __global__ __launch_bounds__(256) void k_ic0_fwd(DevMesh m, const int *__restrict__ rows, int count,
-                                                 const T *__restrict__ U, const T *__restrict__ r, T *__restrict__ y,
+                                                 const T *__restrict__ U, const T *__restrict__ Ul,
+                                                 const T *__restrict__ r, T *__restrict__ y,
                                                  const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip) {
-    const int t = blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
+    const int wg = xcd_wg((count + 31) >> 5);
+    const int t = wg * (blockDim.x >> 3) + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
     if (t >= count) return;
     const int I = rows[t];
     const int inst = m.rowInst[I];
@@ -790,7 +827,7 @@ __global__ __launch_bounds__(256) void k_ic0_fwd(DevMesh m, const int *__restric
     T a0 = 0, a1 = 0, a2 = 0;
     for (int k = m.tPtr[I] + lane8; k < m.tPtr[I + 1]; k += 8) {
         const int K = m.tRow[k];
-        const T *W = U + m.tOff[k];
+        const T *W = Ul + 9 * k;
         const T y0 = y[3 * K], y1 = y[3 * K + 1], y2 = y[3 * K + 2];
         a0 += W[0] * y0 + W[3] * y1 + W[6] * y2;
         a1 += W[1] * y0 + W[4] * y1 + W[7] * y2;
@@ -815,7 +852,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_ic0_bwd(DevMesh m, const int *__restrict__ rows, int count,
                                                  const T *__restrict__ U, T *__restrict__ z,
                                                  const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip) {
-    const int t = blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
+    const int wg = xcd_wg((count + 31) >> 5);
+    const int t = wg * (blockDim.x >> 3) + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
     if (t >= count) return;
     const int I = rows[t];
     const int inst = m.rowInst[I];
@@ -845,20 +883,20 @@ __global__ __launch_bounds__(256) void k_ic0_bwd(DevMesh m, const int *__restric
 }
 
 template <typename T>
-void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *U, const T *r, T *z,
-                      const InstScalars *sc, const uint8_t *skip) {
+void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *U, const T *Ul,
+                      const T *r, T *z, const InstScalars *sc, const uint8_t *skip) {
     const int nl = (int)levelPtr.size() - 1;
     for (int l = 0; l < nl; l++) {
         int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(k_ic0_fwd<T>, dim3((cnt + 31) / 32), dim3(256), 0, s, m, m.levelRows + levelPtr[l], cnt, U, r, z,
-                           sc, skip);
+        hipLaunchKernelGGL(k_ic0_fwd<T>, dim3(xcd_grid((cnt + 31) / 32)), dim3(256), 0, s, m, m.levelRows + levelPtr[l], cnt,
+                           U, Ul, r, z, sc, skip);
     }
     for (int l = nl - 1; l >= 0; l--) {
         int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        hipLaunchKernelGGL(k_ic0_bwd<T>, dim3((cnt + 31) / 32), dim3(256), 0, s, m, m.levelRows + levelPtr[l], cnt, U, z, sc,
-                           skip);
+        hipLaunchKernelGGL(k_ic0_bwd<T>, dim3(xcd_grid((cnt + 31) / 32)), dim3(256), 0, s, m, m.levelRows + levelPtr[l], cnt,
+                           U, z, sc, skip);
     }
     OD_HIP(hipGetLastError());
 }
@@ -888,7 +926,7 @@ __global__ __launch_bounds__(kVecThreads) void k_outer_res(DevMesh m, const doub
         v[1] = b[i] * b[i];
     }
     block_sum<2>(v);
-    if (threadIdx.x == 0) { partials[4 * chunk + 1] = v[0]; partials[4 * chunk + 2] = v[1]; }
+    if (threadIdx.x == 0) { partials[m.nChunks + chunk] = v[0]; partials[2 * m.nChunks + chunk] = v[1]; }
 }
 
 void launch_residual_f64(const DevMesh &m, hipStream_t s, const double *b, const double *Ax, double *r, float *r32,
@@ -957,19 +995,19 @@ void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDo
     template void pcg_jacobi<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &);                  \
     template void pcg_dot_rz_init<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &, double, int, \
                                      const uint8_t *);                                                                \
-    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &);                   \
+    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, const T *, T *, PcgWork &);        \
     template void pcg_update_xr<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, T *, const T *, T *,      \
                                    PcgWork &);                                                                        \
     template void pcg_dot_rz<T>(const DevMesh &, hipStream_t, const T *, const T *, PcgWork &);                       \
     template void pcg_update_p<T>(const DevMesh &, hipStream_t, const T *, T *, PcgWork &);                           \
     template void launch_jacobi_diag<T>(const DevMesh &, hipStream_t, const double *, T *);                           \
     template void launch_ic0_apply<T>(const DevMesh &, hipStream_t, const std::vector<int> &, const T *, const T *,   \
-                                      T *, const InstScalars *, const uint8_t *);
+                                      const T *, T *, const InstScalars *, const uint8_t *);
 INST_T(double)
 INST_T(float)
 template void launch_ic0_factor<double>(const DevMesh &, hipStream_t, const std::vector<int> &, const double *, double *,
-                                        double *, double, const uint8_t *);
+                                        double *, double *, double *, double, const uint8_t *);
 template void launch_ic0_factor<float>(const DevMesh &, hipStream_t, const std::vector<int> &, const double *, double *,
-                                       float *, double, const uint8_t *);
+                                       float *, double *, float *, double, const uint8_t *);
 
 }  // namespace od

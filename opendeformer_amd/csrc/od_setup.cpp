@@ -39,7 +39,7 @@ static int findBlock(const Topology &t, int J, int I) {
     return int(p - t.colIdx.data());
 }
 
-std::string buildTopology(Topology &t, int chunkRows) {
+std::string buildTopology(Topology &t, int chunkRows, int nXcd) {
     char buf[256];
     if (t.nNodes <= 0 || t.nTets <= 0) return "mesh has no nodes or no tets";
     for (size_t i = 0; i < t.tets.size(); i++)
@@ -127,6 +127,8 @@ std::string buildTopology(Topology &t, int chunkRows) {
                 t.tBlk[cur[I]] = b;
                 cur[I]++;
             }
+        t.tPos.assign(t.nBlocks, -1);
+        for (size_t k = 0; k < t.tBlk.size(); k++) t.tPos[t.tBlk[k]] = (int)k;
     }
 
     // ---- element maps -----------------------------------------------------------
@@ -250,6 +252,47 @@ std::string buildTopology(Topology &t, int chunkRows) {
         for (int r = t.instRowStart[i]; r < t.instRowStart[i + 1]; r++) t.rowInst[r] = i;
     }
     t.instChunkStart[t.nInst] = (int)t.chunkRow0.size();
+
+    // ---- XCD-aware launch order --------------------------------------------------
+    // Workgroup w is dispatched to XCD (w mod nXcd) and each XCD runs its workgroups in
+    // increasing w.  Chunks are sorted by (instance, centroid along the instance's longest
+    // axis) and cut into nXcd contiguous runs; workgroup nXcd*j + x gets element j of run x,
+    // so every XCD sweeps one spatial slab and neighbouring rows (which share x entries and
+    // transposed blocks) meet in the same L2.
+    {
+        const int nc = (int)t.chunkRow0.size();
+        std::vector<double> key(nc, 0.0);
+        std::vector<int> axis(t.nInst, 0);
+        for (int i = 0; i < t.nInst; i++) {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int n = t.instNodeStart[i]; n < t.instNodeStart[i + 1]; n++)
+                for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], t.X[3 * n + a]); hi[a] = std::max(hi[a], t.X[3 * n + a]); }
+            int best = 0;
+            for (int a = 1; a < 3; a++) if (hi[a] - lo[a] > hi[best] - lo[best]) best = a;
+            axis[i] = best;
+        }
+        for (int c = 0; c < nc; c++) {
+            double sum = 0;
+            for (int r = t.chunkRow0[c]; r < t.chunkRow1[c]; r++) sum += t.X[3 * t.rowNode[r] + axis[t.chunkInst[c]]];
+            key[c] = sum / std::max(1, t.chunkRow1[c] - t.chunkRow0[c]);
+        }
+        std::vector<int> sorted(nc);
+        std::iota(sorted.begin(), sorted.end(), 0);
+        std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) {
+            if (t.chunkInst[a] != t.chunkInst[b]) return t.chunkInst[a] < t.chunkInst[b];
+            return key[a] < key[b];
+        });
+        const int per = (nc + nXcd - 1) / nXcd;
+        std::vector<int> order;
+        order.reserve(nc);
+        // run x holds sorted[x*per .. (x+1)*per); workgroup nXcd*j + x -> run x, element j
+        for (int j = 0; j < per; j++)
+            for (int x = 0; x < nXcd; x++) {
+                int idx = x * per + j;
+                if (idx < nc) order.push_back(sorted[idx]);
+            }
+        t.chunkOrder = order;
+    }
     return "";
 }
 

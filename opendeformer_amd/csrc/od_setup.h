@@ -33,6 +33,7 @@ struct Topology {
 
     // ---- transposed index: for row I, blocks (K, I) with K < I -------------
     std::vector<int> tPtr, tRow, tBlk;
+    std::vector<int> tPos;              // per block: index in the transposed list, -1 for diagonal blocks
 
     // ---- element maps ---------------------------------------------------------
     std::vector<int> elemMap78;         // 78 per tet (fullOrderNonlinear.h:73-110)
@@ -55,12 +56,13 @@ struct Topology {
     std::vector<int> instRowStart;      // size nInst+1
     std::vector<int> instChunkStart;    // size nInst+1
     std::vector<int> chunkRow0, chunkRow1, chunkInst;
+    std::vector<int> chunkOrder;        // launch order: workgroup w -> chunk (see buildTopology)
     std::vector<int> rowInst;           // per row
     std::vector<int> tetInst;           // per tet
 };
 
 // Builds every derived structure.  Returns an empty string or an error text.
-std::string buildTopology(Topology &t, int chunkRows);
+std::string buildTopology(Topology &t, int chunkRows, int nXcd = 8);
 
 // Global DOF of (node, axis): -1 when constrained (nodeIndexer.cpp:12-25).
 int globalIndex(const Topology &t, int node, int axis);
