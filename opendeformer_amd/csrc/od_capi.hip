@@ -64,13 +64,16 @@ static void prof_flush(od_ctx *c) {
     }
     c->evUsed = 0;
 }
-static void prof_mark(od_ctx *c) {
+// Next (start, stop) event pair for a timed launch, or nulls outside measurement mode.
+static void prof_pair(od_ctx *c, hipEvent_t *e0, hipEvent_t *e1) {
+    *e0 = *e1 = nullptr;
     if (!c->prof) return;
-    if (c->evUsed == c->ev.size()) {
-        OD_HIP(hipEventSynchronize(c->ev.back()));
+    if (c->evUsed + 2 > c->ev.size()) {
+        OD_HIP(hipEventSynchronize(c->ev[c->evUsed - 1]));
         prof_flush(c);
     }
-    OD_HIP(hipEventRecord(c->ev[c->evUsed++], c->stream));
+    *e0 = c->ev[c->evUsed++];
+    *e1 = c->ev[c->evUsed++];
 }
 
 struct od_mesh {
@@ -293,9 +296,9 @@ static void run_pcg(od_solver *s, const T *vals, const T *low, const T *Uf, cons
         if (count_not_done(mesh->ctx, doneFlags, m.nInst) == 0) break;
         const int k = std::min(batch, maxIt - launched);
         for (int j = 0; j < k; j++) {
-            prof_mark(mesh->ctx);
-            pcg_alpha<T>(m, st, vals, low, p, q, w);
-            prof_mark(mesh->ctx);
+            hipEvent_t e0, e1;
+            prof_pair(mesh->ctx, &e0, &e1);
+            pcg_alpha<T>(m, st, vals, low, p, q, w, e0, e1);
             pcg_alpha_finalize(m, st, w);
             if (ic) {
                 pcg_update_xr<T>(m, st, p, q, x, r, (const T *)nullptr, (T *)nullptr, w);

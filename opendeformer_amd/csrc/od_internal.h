@@ -92,7 +92,8 @@ template <typename T>
 void pcg_dot_rz_init(const DevMesh &m, hipStream_t s, const T *r, const T *z, T *p, PcgWork &w, double tol, int maxIt,
                      const uint8_t *preDone);
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *p, T *q, PcgWork &w);
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *p, T *q, PcgWork &w,
+               hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w);
 template <typename T>
 void pcg_update_xr(const DevMesh &m, hipStream_t s, const T *p, const T *q, T *x, T *r, const T *invDiag, T *z,

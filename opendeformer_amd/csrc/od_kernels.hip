@@ -13,6 +13,7 @@
 //               last workgroup of every reduction kernel finalises the scalars
 //               (alpha, beta, convergence) on the device.
 //  * IC(0):     level-scheduled block factorisation and sweeps, one thread per row.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "od_elem.h"
@@ -543,8 +544,15 @@ template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, co
                                  const InstScalars *);
 
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *p, T *q, PcgWork &w) {
-    hipLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, low, p, q, w.partials, w.sc);
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *p, T *q, PcgWork &w,
+               hipEvent_t start, hipEvent_t stop) {
+    // With events, the launch records them from the dispatch's own begin/end timestamps,
+    // so their elapsed time is the kernel's execution alone (measurement mode).
+    if (start)
+        hipExtLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, start, stop, 0, m, vals, low, p,
+                              q, w.partials, w.sc);
+    else
+        hipLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, low, p, q, w.partials, w.sc);
     OD_HIP(hipGetLastError());
 }
 
@@ -995,7 +1003,8 @@ void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDo
     template void pcg_jacobi<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &);                  \
     template void pcg_dot_rz_init<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &, double, int, \
                                      const uint8_t *);                                                                \
-    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, const T *, T *, PcgWork &);        \
+    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, const T *, T *, PcgWork &,         \
+                               hipEvent_t, hipEvent_t);                                                           \
     template void pcg_update_xr<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, T *, const T *, T *,      \
                                    PcgWork &);                                                                        \
     template void pcg_dot_rz<T>(const DevMesh &, hipStream_t, const T *, const T *, PcgWork &);                       \

@@ -1,0 +1,79 @@
+"""The C++ host classes (include/od_gpu.hpp), used as reference-style host code
+uses CGSolver, SpMDV, the materials and the Intergrator, against the oracle.
+
+tests/cpp/host_api_check is built by the package build; this test writes its
+input file, runs it and compares every output it writes.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from opendeformer_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "host_api_check")
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_host_classes_match_oracle(tmp_path):
+    assert os.path.exists(BIN), "host_api_check is not built (run the package build)"
+    w = W.box_small("natural")
+    s = O.System(w.X, w.tets, w.fixed_dofs)
+    n, nn = s.ndof, w.X.shape[0]
+    free = np.setdiff1d(np.arange(3 * nn), w.fixed_dofs)
+    rng = np.random.default_rng(21)
+    u_free = rng.standard_normal(n) * 0.01
+    u_node = np.zeros(3 * nn); u_node[free] = u_free
+    x = rng.standard_normal(n); y0 = rng.standard_normal(n); b = rng.standard_normal(n)
+    g = s.body_force(w.body)
+    steps = 12
+    rho, E, nu = float(w.rho[0]), float(w.E[0]), float(w.nu[0])
+    inp = tmp_path / "in.bin"; out = tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        np.array([nn, w.tets.shape[0], len(w.fixed_dofs), n, steps], np.int32).tofile(f)
+        w.X.astype(np.float64).ravel().tofile(f)
+        w.tets.astype(np.int32).ravel().tofile(f)
+        np.asarray(w.fixed_dofs, np.int32).tofile(f)
+        np.array([rho, E, nu, w.dt], np.float64).tofile(f)
+        for v in (u_node, x, y0, b, g):
+            np.asarray(v, np.float64).tofile(f)
+    print("host_api_check on box_small", flush=True)
+    r = subprocess.run([BIN, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = open(out, "rb").read()
+    nv = int(np.frombuffer(raw[:8], np.int64)[0])
+    off = 8
+
+    def take(count, dt=np.float64):
+        nonlocal off
+        a = np.frombuffer(raw, dt, count, off); off += a.nbytes
+        return a
+
+    K = take(nv); vws = take(n); y = take(n); sol = take(n); iters = int(take(1, np.int32)[0])
+    d_node = take(3 * nn); v_node = take(3 * nn)
+
+    s.set_free_displacements(u_free)
+    Kc, fc = s.assemble(w.kind, w.E, w.nu)
+    assert relerr(K, Kc) < 1e-11
+    assert relerr(vws, fc) < 1e-10
+    assert relerr(y, s.spmdv(Kc, x, y0.copy())) < 1e-12
+    A = s.mass(w.rho) + 0.25 * w.dt ** 2 * Kc
+    xs = np.linalg.solve(s.dense(A), b)
+    assert 0 < iters < 5000
+    assert relerr(sol, xs) < 1e-8
+
+    ref = O.Newmark(s, w.kind, w.E, w.nu, w.rho, w.dt, pre=O.PRE_IC0, p1=1e-3)
+    ref.set_external(g)
+    ref.step(steps)
+    do, vo, _ = ref.state()
+    assert relerr(d_node[free], do) < 1e-6
+    assert relerr(v_node[free], vo) < 1e-6
+    assert np.all(d_node[np.asarray(w.fixed_dofs)] == 0)
