@@ -8,9 +8,14 @@ gamma = 1/2, dt = 1/60, PCG to 1e-6 relative residual.  A "step" is one full
 Newmark step on the device: element pass, assembly, matrix combine, IC(0)
 factorisation, PCG to tolerance, corrector.
 
-Prints one JSON line (rank 0).  Extra fields: SpMV roofline from HIP events
-on the library stream over the timed steps, and the CPU FP64 oracle timed on
-a bounded sample of the same workload.
+The default form is the precision form: FP32 inner PCG (FP32 matrix, factor and
+vectors, FP64 reductions) with FP64 residual corrections to the same 1e-6
+relative residual.  The plain FP64 path runs in the same process on the same
+mesh for the speed-up and the displacement error between the two.
+
+Prints one JSON line.  Extra fields: SpMV roofline from HIP events on the
+library stream over the timed steps, and the CPU FP64 oracle timed on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -109,10 +114,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precond", choices=["jacobi", "ic0"], default="ic0")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--ordering", choices=["natural", "color8"], default="color8")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = 200x40x40)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="with --precision mixed: skip the plain FP64 path run (speed-up and error report)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the run, launch a streaming update of known size three times (the reference "
                          "that scales HBM byte counters)")
@@ -132,56 +139,81 @@ def main():
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
     f = mesh.body_force(w.body)
     pre = {"jacobi": capi.OD_PRE_JACOBI, "ic0": capi.OD_PRE_IC0}[args.precond]
-    prec = {"fp64": capi.OD_FP64, "mixed": capi.OD_MIXED}[args.precision]
-    nm = od.NewmarkIntegrator(mesh, w.dt, beta=0.25, gamma=0.5, precond=pre, precision=prec, tol=1e-6)
-    nm.set_external_virtual_work(f)
-    nm.run_time_steps(args.warmup)
-    ctx.sync()
+    tune_table, r64, r32 = od.mesh_autotune(mesh)
 
-    ctx.profile(True)
-    iters, relres, outer = [], [], []
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        nm.run_one_time_step()
-        it, rr, ou, _ = nm.stats()
-        iters.append(int(it[0])); relres.append(float(rr[0])); outer.append(ou)
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    spmv_ms, spmv_n = ctx.profile_read()
-    ctx.profile(False)
+    def run_form(precision):
+        """Warm-up, then the timed steps; returns the measurements and the final displacement."""
+        prec = {"fp64": capi.OD_FP64, "mixed": capi.OD_MIXED}[precision]
+        nm = od.NewmarkIntegrator(mesh, w.dt, beta=0.25, gamma=0.5, precond=pre, precision=prec, tol=1e-6)
+        nm.set_external_virtual_work(f)
+        nm.run_time_steps(args.warmup)
+        ctx.sync()
+        ctx.profile(True)
+        iters, relres, outer = [], [], []
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            nm.run_one_time_step()
+            it, rr, ou, _ = nm.stats()
+            iters.append(int(it[0])); relres.append(float(rr[0])); outer.append(ou)
+        ctx.sync()
+        elapsed = time.perf_counter() - t0
+        spmv_ms, spmv_n = ctx.profile_read()
+        ctx.profile(False)
+        vb = 4 if precision == "mixed" else 8
+        by = spmv_bytes(mesh, vb, vb)
+        # Launches of a batch that finds its instance converged exit at once; their (small) time
+        # stays in the bracketed total, but only the iterations that ran count as SpMV work.
+        real_spmv = int(np.sum(iters))
+        avg_spmv_s = (spmv_ms / max(real_spmv, 1)) / 1e3
+        achieved = by / avg_spmv_s / 1e9 if real_spmv else 0.0
+        d = nm.state()[0]
+        del nm
+        return {"steps_s": args.steps / elapsed, "ms_step": elapsed / args.steps * 1e3,
+                "iters": float(np.mean(iters)), "relres_max": float(np.max(relres)),
+                "outer": float(np.mean(outer)),
+                "spmv": {"kernel": "k_spmv_dot<double>" if vb == 8 else "k_spmv_dot<float>", "launches": spmv_n,
+                         "iterations": real_spmv, "avg_us": round(avg_spmv_s * 1e6, 2),
+                         "algorithmic_MB": round(by / 1e6, 2), "GBps": round(achieved, 1)},
+                "achieved": achieved}, d
 
-    ms_step = elapsed / args.steps * 1e3
-    steps_s = args.steps / elapsed
-    vb = 4 if args.precision == "mixed" else 8
-    by = spmv_bytes(mesh, vb, vb)
-    # Launches of a batch that finds its instance converged exit at once; their (small) time
-    # stays in the bracketed total, but only the iterations that ran count as SpMV work.
-    real_spmv = int(np.sum(iters))
-    avg_spmv_s = (spmv_ms / max(real_spmv, 1)) / 1e3
-    achieved = by / avg_spmv_s / 1e9 if real_spmv else 0.0
+    main_r, d_main = run_form(args.precision)
+    fp64_r = None
+    if args.precision != "fp64" and not args.no_fp64:
+        fp64_r, d64 = run_form("fp64")
+        err = float(np.linalg.norm(d_main - d64) / np.linalg.norm(d64))
+    achieved = main_r["achieved"]
     traffic = traffic_from_profile(w.name, args)
     line = {
-        "metric": METRIC, "value": round(steps_s, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32-inner/f64-refinement",
+        "metric": METRIC, "value": round(main_r["steps_s"], 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(main_r["ms_step"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-inner/f64-refinement",
         "data": "synthetic (seeded box mesh, seed 3)",
         "config": {"workload": f"{w.name}: {mesh.n_tets} tets, {mesh.ndof} DOF, Newmark beta=1/4 gamma=1/2 dt=1/60, "
                                f"PCG tol 1e-6 relative residual",
                    "precond": args.precond, "precision": args.precision, "ordering": args.ordering,
                    "ic0_levels": mesh.n_levels},
-        "pcg_iters_per_step": round(float(np.mean(iters)), 2),
-        "relres_max": float(np.max(relres)),
-        "spmv": {"kernel": "k_spmv_dot" if vb == 8 else "k_spmv_dot<float>", "launches": spmv_n,
-                 "iterations": real_spmv,
-                 "avg_us": round(avg_spmv_s * 1e6, 2), "algorithmic_MB": round(by / 1e6, 2),
-                 "GBps": round(achieved, 1)},
+        "pcg_iters_per_step": round(main_r["iters"], 2),
+        "relres_max": main_r["relres_max"],
+        "spmv": main_r["spmv"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
+    line["spmv_autotune"] = {"chunks_per_workgroup": {"fp64": r64, "fp32": r32},
+                             "cold_us": {f"{p}_R{r}": round(v * 1e3, 2) for (p, r), v in tune_table.items()}}
+    if args.precision == "mixed":
+        line["refinement_sweeps_per_step"] = round(main_r["outer"], 2)
+    if fp64_r is not None:
+        line["fp64_path"] = {"value": round(fp64_r["steps_s"], 4), "ms_per_step": round(fp64_r["ms_step"], 3),
+                             "pcg_iters_per_step": round(fp64_r["iters"], 2), "relres_max": fp64_r["relres_max"],
+                             "spmv": fp64_r["spmv"]}
+        line["speedup_vs_fp64"] = round(main_r["steps_s"] / fp64_r["steps_s"], 3)
+        line["rel_err_vs_fp64_path"] = {"value": err, "at": f"displacement after step {args.warmup + args.steps}"}
     if not args.no_cpu:
         try:
-            line["cpu_baseline"] = cpu_baseline(args, float(np.mean(iters)), mesh.ndof)
+            it_ref = fp64_r["iters"] if fp64_r is not None else main_r["iters"]
+            line["cpu_baseline"] = cpu_baseline(args, it_ref, mesh.ndof)
         except Exception as e:  # the GPU line stays valid without the CPU leg
             line["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(line), flush=True)

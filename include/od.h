@@ -51,7 +51,7 @@ int od_ctx_create(int device, od_ctx **out);
 int od_ctx_destroy(od_ctx *ctx);
 int od_ctx_sync(od_ctx *ctx);
 /* measurement mode (no reference counterpart): when enabled, every PCG SpMV
- * launch (q = A p) is bracketed by HIP events on the context stream;
+ * launch (q = A p) carries start/stop HIP events taken from its own dispatch;
  * od_ctx_profile_read returns their summed duration and count since enabling */
 int od_ctx_profile(od_ctx *ctx, int enable);
 int od_ctx_profile_read(od_ctx *ctx, double *totalMs_host, long long *launches_host);
@@ -86,6 +86,15 @@ int od_mesh_info(od_mesh *mesh, long long *info_host);
  * block-level map the assembly kernels use.  elemMap78_host may be NULL. */
 int od_pattern_export(od_mesh *mesh, int *blockPcol_host, int *blockColumnOris_host, int *blockRows_host,
                       int *elemMap78_host);
+/* SpMV launch shape (no reference counterpart).  The PCG SpMV runs R = 1, 2 or
+ * 4 chunks of 64 block rows per workgroup (more rows' loads in flight per
+ * thread).  The first solver created on a mesh runs the autotuner, which times
+ * each R for FP64 and FP32 values with the matrix out of cache and keeps the
+ * fastest; od_mesh_autotune reruns it (ms_host: 6 median times [FP64 R=1,2,4,
+ * FP32 R=1,2,4], chosen_host: R for FP64 and FP32; either may be NULL) and
+ * od_mesh_spmv_variant sets R directly (no tuning). */
+int od_mesh_autotune(od_mesh *mesh, double *ms_host, int *chosen_host);
+int od_mesh_spmv_variant(od_mesh *mesh, int r64, int r32);
 /* Per-instance row range and IC(0) level structure for tests:
  * instRowStart_host (nInstances+1), level_host (nBlockRows). */
 int od_mesh_rows(od_mesh *mesh, int *instRowStart_host, int *level_host);
@@ -140,6 +149,12 @@ int od_spmv_timed(od_matrix *A, const void *x_dev, void *y_dev, int iters, int f
  * iters_host / relres_host (nInstances entries each) may be NULL. */
 int od_solver_create(od_mesh *mesh, int precond, double precondParam, int precision, od_solver **out);
 int od_solver_destroy(od_solver *s);
+/* Mixed precision (OD_MIXED) only, no reference counterpart: each FP32 inner
+ * solve stops at innerTol relative to its own right-hand side (default 1e-4)
+ * or once its residual is below half the outer tolerance tol*|b|, whichever
+ * comes first; at most maxOuter inner solves with FP64 residual corrections
+ * are made (default 12). */
+int od_solver_set_refinement(od_solver *s, double innerTol, int maxOuter);
 int od_pcg_solve(od_solver *s, od_matrix *A, const double *b_dev, double *x_dev, double tol, int maxIter,
                  int *iters_host, double *relres_host);
 /* Preconditioner::Preprocess + solvePreconditionerSystem once: z = P(A)^-1 r */
@@ -159,6 +174,8 @@ int od_newmark_create(od_mesh *mesh, double beta, double gamma, double dt, doubl
 int od_backward_euler_create(od_mesh *mesh, double dt, double alphaM, double alphaK, int precond,
                              double precondParam, double tol, int maxIter, od_integrator **out);
 int od_integrator_destroy(od_integrator *it);
+/* od_solver_set_refinement for the integrator's solver */
+int od_integrator_set_refinement(od_integrator *it, double innerTol, int maxOuter);
 /* Intergrator::setExternalVirtualWork (core/intergrator.h:18) */
 int od_integrator_set_external(od_integrator *it, const double *f_host);
 /* one-step load added to the next step's external force only */

@@ -30,6 +30,7 @@ EXPORTS = [
     "od_integrator_set_impulse", "od_integrator_set_active", "od_integrator_step", "od_integrator_get_state",
     "od_integrator_set_state", "od_integrator_state_dev", "od_integrator_stats", "od_backward_euler_update_mesh",
     "od_integrator_positions", "od_spmv_timed", "od_integrator_matrix", "od_ctx_profile", "od_ctx_profile_read",
+    "od_solver_set_refinement", "od_integrator_set_refinement", "od_mesh_autotune", "od_mesh_spmv_variant",
 ]
 
 
@@ -60,6 +61,8 @@ def lib():
             "od_matrix_axpby": [vp, d, d, vp], "od_mass_matrix": [vp, vp], "od_assemble": [vp, vp, vp, vp],
             "od_spmv": [vp, vp, vp, i], "od_matrix_sync_f32": [vp], "od_spmv_f32": [vp, vp, vp, i],
             "od_solver_create": [vp, i, d, i, pvp], "od_solver_destroy": [vp],
+            "od_solver_set_refinement": [vp, d, i], "od_integrator_set_refinement": [vp, d, i],
+            "od_mesh_autotune": [vp, dp, ip], "od_mesh_spmv_variant": [vp, i, i],
             "od_pcg_solve": [vp, vp, vp, vp, d, i, ip, dp], "od_precond_apply": [vp, vp, vp, vp],
             "od_ic0_factor": [vp, vp, dp],
             "od_newmark_create": [vp, d, d, d, d, d, i, d, d, i, i, i, pvp],

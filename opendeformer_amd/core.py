@@ -146,6 +146,18 @@ class Mesh:
         return f.download()
 
 
+def mesh_autotune(mesh: "Mesh"):
+    """Reruns the SpMV autotuner: ({(precision, R): ms}, chosen R for FP64, chosen R for FP32)."""
+    ms = np.zeros(6); ch = np.zeros(2, np.int32)
+    check(lib().od_mesh_autotune(mesh.h, dptr(ms), capi.iptr(ch)))
+    table = {(p, r): float(ms[3 * i + j]) for i, p in enumerate(("fp64", "fp32")) for j, r in enumerate((1, 2, 4))}
+    return table, int(ch[0]), int(ch[1])
+
+
+def mesh_spmv_variant(mesh: "Mesh", r64, r32):
+    check(lib().od_mesh_spmv_variant(mesh.h, r64, r32))
+
+
 class Matrix:
     """Values of a BlockedSymSparseMatrix<3,3> in the canonical layout, on the device."""
 
@@ -228,6 +240,9 @@ class CGSolver:
     def precond_apply(self, A: Matrix, r: DeviceArray, z: DeviceArray):
         check(lib().od_precond_apply(self.h, A.h, r.ptr, z.ptr))
 
+    def set_refinement(self, inner_tol, max_outer=12):
+        check(lib().od_solver_set_refinement(self.h, inner_tol, max_outer))
+
     def ic0_factor(self, A: Matrix):
         L = np.zeros(self.mesh.n_values)
         check(lib().od_ic0_factor(self.h, A.h, dptr(L)))
@@ -271,6 +286,10 @@ class _Integrator:
     def set_state(self, d=None, v=None, a=None):
         args = [dptr(f64(x)) if x is not None else None for x in (d, v, a)]
         check(lib().od_integrator_set_state(self.h, *args))
+
+    def set_refinement(self, inner_tol, max_outer=12):
+        """Mixed precision: FP32 inner tolerance and the cap on FP64 corrections."""
+        check(lib().od_integrator_set_refinement(self.h, inner_tol, max_outer))
 
     def stats(self):
         it = np.zeros(self.mesh.n_inst, np.int32); rr = np.zeros(self.mesh.n_inst)

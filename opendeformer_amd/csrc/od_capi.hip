@@ -83,6 +83,8 @@ struct od_mesh {
     std::vector<void *> allocs;
     size_t bytes = 0;
     bool hasMaterial = false;
+    bool spmvTuned = false;                      // SpMV variant chosen (autotuner or caller)
+    double tuneMs[2][3] = {{0, 0, 0}, {0, 0, 0}};   // [FP64, FP32][R = 1, 2, 4]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
     template <typename T>
     T *upload(const std::vector<T> &v) {
@@ -114,10 +116,6 @@ struct od_matrix {
     od_mesh *mesh = nullptr;
     double *vals = nullptr;
     float *vals32 = nullptr;
-    // optional lower copy: the off-diagonal blocks again, contiguous per block row of the
-    // transposed list (written by the assembly gather; read by the SpMV)
-    double *low = nullptr;
-    float *low32 = nullptr;
     ~od_matrix() {
         if (vals) (void)hipFree(vals);
         if (vals32) (void)hipFree(vals32);
@@ -129,7 +127,7 @@ struct od_solver {
     int precond = OD_PRE_JACOBI;
     double param = 1e-3;        // IC(0) pivot epsilon
     int precision = OD_FP64;
-    double innerTol = 1e-3;     // mixed: inner FP32 relative tolerance
+    double innerTol = 1e-4;     // mixed: inner FP32 relative tolerance (floor; see kInnerOuterScale2)
     int maxOuter = 12;
     std::vector<void *> allocs;
     double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *invDiag = nullptr, *U = nullptr, *Ul = nullptr;
@@ -144,6 +142,7 @@ struct od_solver {
     std::vector<double> relres;
     int outer = 0;
     int lastMaxIters = 0;       // iterations of the previous solve (sizes the first launch batch)
+    std::vector<int> sweepIters; // mixed form: iterations of each inner sweep of the previous solve
     void *alloc(size_t b) {
         void *ptr = nullptr;
         OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
@@ -179,7 +178,7 @@ struct od_integrator {
         return ptr;
     }
     ~od_integrator() {
-        M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr; A.low = nullptr; A.low32 = nullptr;   // owned by allocs
+        M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr;   // owned by allocs
         for (void *ptr : allocs) (void)hipFree(ptr);
         delete solver;
     }
@@ -188,16 +187,94 @@ struct od_integrator {
 // =====================================================================================
 // solver internals
 // =====================================================================================
+// Reads a large buffer once (one partial per workgroup): evicts the memory-side cache without
+// leaving dirty lines behind, as the read-mostly kernels before a PCG SpMV do.
+__global__ static void k_stream_read(const float4 *p, int64_t n, float *out) {
+    float s = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = p[i];
+        s += v.x + v.y + v.z + v.w;
+    }
+    if (s == 12345.f) out[blockIdx.x] = s;   // keeps the loads; the buffer never holds this sum
+}
+
+// Autotuner of the PCG SpMV: times each chunks-per-workgroup variant (1, 2, 4) for FP64 and
+// FP32 values on this mesh's pattern, with the matrix out of every cache as it is inside a
+// PCG iteration (a 512 MB streaming read before each launch evicts the 256 MB memory-side
+// cache), and keeps the fastest median of three.  Values are zero: the time depends only on
+// the pattern.
+static void autotune_spmv(od_mesh *mesh) {
+    if (mesh->spmvTuned) return;
+    DevMesh &m = mesh->dm;
+    hipStream_t st = mesh->ctx->stream;
+    const Topology &t = mesh->topo;
+    const size_t nv = (size_t)t.nValues + kValuePad, n = (size_t)t.ndof;
+    const int64_t flushN = (int64_t)128 << 20;    // floats: 512 MB
+    std::vector<void *> bufs;
+    auto get = [&](size_t bytes) {
+        void *p = nullptr;
+        OD_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+        bufs.push_back(p);
+        OD_HIP(hipMemsetAsync(p, 0, std::max<size_t>(bytes, 16), st));
+        return p;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+        void *vals = get(8 * nv), *x = get(8 * n), *y = get(8 * n);
+        double *partials = (double *)get(8 * (size_t)std::max(1, m.nChunks));
+        InstScalars *sc = (InstScalars *)get(sizeof(InstScalars) * t.nInst);   // zero: no instance done
+        float *flush = (float *)get(4 * (size_t)flushN);
+        OD_HIP(hipEventCreate(&e0));
+        OD_HIP(hipEventCreate(&e1));
+        for (int prec = 0; prec < 2; prec++)
+            for (int v = 0; v < 3; v++) {
+                const int R = kSpmvVariants[v];
+                float ms[3];
+                for (int rep = 0; rep < 3; rep++) {
+                    hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, st, (const float4 *)flush, flushN / 4,
+                                       flush);
+                    if (prec == 0)
+                        launch_spmv_dot<double>(m, st, R, (const double *)vals, (const double *)x, (double *)y, partials,
+                                                sc, e0, e1);
+                    else
+                        launch_spmv_dot<float>(m, st, R, (const float *)vals, (const float *)x, (float *)y, partials,
+                                               sc, e0, e1);
+                    OD_HIP(hipEventSynchronize(e1));
+                    OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
+                }
+                std::sort(ms, ms + 3);
+                mesh->tuneMs[prec][v] = ms[1];
+            }
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (void *p : bufs) (void)hipFree(p);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (void *p : bufs) (void)hipFree(p);
+    int best[2] = {0, 0};
+    for (int prec = 0; prec < 2; prec++)
+        for (int v = 1; v < 3; v++)
+            if (mesh->tuneMs[prec][v] < mesh->tuneMs[prec][best[prec]]) best[prec] = v;
+    m.spmvR64 = kSpmvVariants[best[0]];
+    m.spmvR32 = kSpmvVariants[best[1]];
+    mesh->spmvTuned = true;
+}
+
 static od_solver *make_solver(od_mesh *mesh, int precond, double param, int precision) {
     require(precond == OD_PRE_NONE || precond == OD_PRE_JACOBI || precond == OD_PRE_IC0, "unknown preconditioner",
             OD_ERR_UNSUPPORTED);
     require(precision == OD_FP64 || precision == OD_MIXED, "unknown precision", OD_ERR_UNSUPPORTED);
+    autotune_spmv(mesh);
     od_solver *s = new od_solver();
     try {
         s->mesh = mesh;
         s->precond = precond;
         s->param = param > 0 ? param : 1e-3;
         s->precision = precision;
+        s->sweepIters.assign(s->maxOuter, 0);
         const Topology &t = mesh->topo;
         size_t n = (size_t)t.ndof, nv = (size_t)t.nValues;
         s->r = (double *)s->alloc(8 * n); s->z = (double *)s->alloc(8 * n);
@@ -272,16 +349,16 @@ static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip)
 
 // PCG on the chunked instances: x in/out, b fixed.  preDone marks instances to skip.
 template <typename T>
-static void run_pcg(od_solver *s, const T *vals, const T *low, const T *Uf, const T *Ul, const T *invDiag, const T *b,
+static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const T *invDiag, const T *b,
                     T *x, T *r, T *z, T *p, T *q, double tol, int maxIt, const uint8_t *preDone, InstScalars *sc,
-                    int *doneFlags, bool xZero) {
+                    int *doneFlags, bool xZero, int expect, const InstScalars *outer = nullptr) {
     od_mesh *mesh = s->mesh;
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
-    PcgWork w{s->partials, sc, doneFlags};
+    PcgWork w{s->partials, sc, doneFlags, outer};
     const size_t n = (size_t)mesh->topo.ndof;
     if (xZero) OD_HIP(hipMemsetAsync(q, 0, sizeof(T) * n, st));
-    else launch_spmv<T>(m, st, vals, low, x, q, 0, nullptr);
+    else launch_spmv<T>(m, st, vals, x, q, 0, nullptr);
     pcg_init_residual<T>(m, st, b, q, r, w);
     const bool ic = s->precond == OD_PRE_IC0;
     if (ic) launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, Ul, r, z, nullptr, preDone);
@@ -289,16 +366,17 @@ static void run_pcg(od_solver *s, const T *vals, const T *low, const T *Uf, cons
     pcg_dot_rz_init<T>(m, st, r, z, p, w, tol, maxIt, preDone);
     // Iterations are launched in batches between convergence polls (a poll synchronises the
     // stream).  Warm-started solves of consecutive steps need nearly the same count, so the
-    // first batch is sized from the previous solve; afterwards the batches stay small.
+    // first batch is sized from the same solve of the previous step (`expect`); afterwards
+    // the batches stay small.
     int launched = 0;
-    int batch = s->lastMaxIters > 8 ? s->lastMaxIters - 2 : 4;
+    int batch = expect > 8 ? expect - 2 : 4;
     while (launched < maxIt) {
         if (count_not_done(mesh->ctx, doneFlags, m.nInst) == 0) break;
         const int k = std::min(batch, maxIt - launched);
         for (int j = 0; j < k; j++) {
             hipEvent_t e0, e1;
             prof_pair(mesh->ctx, &e0, &e1);
-            pcg_alpha<T>(m, st, vals, low, p, q, w, e0, e1);
+            pcg_alpha<T>(m, st, vals, p, q, w, e0, e1);
             pcg_alpha_finalize(m, st, w);
             if (ic) {
                 pcg_update_xr<T>(m, st, p, q, x, r, (const T *)nullptr, (T *)nullptr, w);
@@ -310,7 +388,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *low, const T *Uf, cons
             pcg_update_p<T>(m, st, z, p, w);
         }
         launched += k;
-        batch = s->lastMaxIters > 8 ? 3 : std::min(batch * 2, 32);
+        batch = expect > 8 ? 3 : std::min(batch * 2, 32);
     }
 }
 
@@ -342,8 +420,8 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     const uint8_t *pre = skip ? skip : s->zeroMask;
     if (s->precision == OD_FP64) {
         build_precond(s, A, pre);
-        run_pcg<double>(s, A->vals, A->low, s->U, s->Ul, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre,
-                        s->scIn, s->doneIn, false);
+        run_pcg<double>(s, A->vals, s->U, s->Ul, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre,
+                        s->scIn, s->doneIn, false, s->lastMaxIters);
         collect_stats(s, s->scIn, false);
         collect_relres(s, s->scIn);
         s->outer = 0;
@@ -355,7 +433,7 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     build_precond(s, A, pre);
     std::fill(s->iters.begin(), s->iters.end(), 0);
     PcgWork wo{s->partials, s->scOut, s->doneOut};
-    launch_spmv<double>(m, st, A->vals, A->low, x, s->Ax, 0, nullptr);
+    launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr);
     launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 1);
     int outer = 0;
     for (; outer < s->maxOuter; outer++) {
@@ -363,16 +441,16 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
         hipLaunchKernelGGL(k_done_to_mask, dim3((t.nInst + 255) / 256), dim3(256), 0, st, t.nInst, s->scOut, pre, s->mask);
         OD_HIP(hipGetLastError());
         OD_HIP(hipMemsetAsync(s->x32, 0, sizeof(float) * t.ndof, st));
-        run_pcg<float>(s, A->vals32, A->low32, s->U32, s->Ul32, s->invDiag32, s->r32, s->x32, s->ri32, s->z32, s->p32,
-                       s->q32, s->innerTol, maxIt, s->mask, s->scIn, s->doneIn, true);
+        run_pcg<float>(s, A->vals32, s->U32, s->Ul32, s->invDiag32, s->r32, s->x32, s->ri32, s->z32, s->p32,
+                       s->q32, s->innerTol, maxIt, s->mask, s->scIn, s->doneIn, true, s->sweepIters[outer], s->scOut);
         {
             const int before = *std::max_element(s->iters.begin(), s->iters.end());
             collect_stats(s, s->scIn, true);
             const int after = *std::max_element(s->iters.begin(), s->iters.end());
-            s->lastMaxIters = after - before;
+            s->sweepIters[outer] = after - before;
         }
         launch_add_correction(m, st, s->x32, x, s->scOut);
-        launch_spmv<double>(m, st, A->vals, A->low, x, s->Ax, 0, nullptr);
+        launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr);
         launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 0);
     }
     s->outer = outer;
@@ -537,8 +615,32 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.chunkRow0 = mesh->upload(t.chunkRow0);
         m.chunkRow1 = mesh->upload(t.chunkRow1);
         m.chunkInst = mesh->upload(t.chunkInst);
-        if (std::getenv("OD_CHUNK_ORDER_IDENTITY")) std::iota(t.chunkOrder.begin(), t.chunkOrder.end(), 0);
-        m.chunkOrder = mesh->upload(t.chunkOrder);
+        m.chunkSorted = mesh->upload(t.chunkSorted);
+        m.perRun = t.perRun;
+        m.spmvR64 = m.spmvR32 = 1;
+        {
+            // SpMV slots: per chunk, kSpmvLanes per row, rows in order (row-loop path for longer rows)
+            const int nc = m.nChunks;
+            std::vector<int> sOff((size_t)nc * kSpmvThreads, 0), sCol((size_t)nc * kSpmvThreads, -1);
+            std::vector<uint8_t> cLong(nc, 0);
+            require(t.nRows <= kSlotColMask, "too many block rows for the SpMV slot encoding", OD_ERR_UNSUPPORTED);
+            for (int c = 0; c < nc; c++)
+                for (int r = t.chunkRow0[c]; r < t.chunkRow1[c]; r++) {
+                    const size_t base = ((size_t)c * kChunkRows + (r - t.chunkRow0[c])) * kSpmvLanes;
+                    int n = 0;
+                    auto put = [&](int off, int col) {
+                        if (n < kSpmvLanes) { sOff[base + n] = off; sCol[base + n] = col; }
+                        n++;
+                    };
+                    put(t.valOff[r], r | kSlotDiag);
+                    for (int b = t.rowPtr[r] + 1; b < t.rowPtr[r + 1]; b++) put(t.blkOff[b], t.colIdx[b]);
+                    for (int k = t.tPtr[r]; k < t.tPtr[r + 1]; k++) put(tOff[k], t.tRow[k] | kSlotTrans);
+                    if (n > kSpmvLanes) cLong[c] = 1;
+                }
+            m.slotOff = mesh->upload(sOff);
+            m.slotCol = mesh->upload(sCol);
+            m.chunkLong = mesh->upload(cLong);
+        }
         m.instChunkStart = mesh->upload(t.instChunkStart);
         m.instRowStart = mesh->upload(t.instRowStart);
         m.tetInst = mesh->upload(t.tetInst);
@@ -603,6 +705,29 @@ int od_pattern_export(od_mesh *mesh, int *blockPcol, int *blockColumnOris, int *
         }
         OD_HIP(hipFree(dmap));
     }
+    API_END
+}
+
+int od_mesh_spmv_variant(od_mesh *mesh, int r64, int r32) {
+    API_BEGIN
+    require(mesh != nullptr, "NULL argument");
+    auto ok = [](int r) { return r == 1 || r == 2 || r == 4; };
+    require(ok(r64) && ok(r32), "chunks per SpMV workgroup must be 1, 2 or 4");
+    mesh->dm.spmvR64 = r64;
+    mesh->dm.spmvR32 = r32;
+    mesh->spmvTuned = true;
+    API_END
+}
+
+int od_mesh_autotune(od_mesh *mesh, double *ms, int *chosen) {
+    API_BEGIN
+    require(mesh != nullptr, "NULL argument");
+    mesh->spmvTuned = false;
+    autotune_spmv(mesh);
+    if (ms)
+        for (int p = 0; p < 2; p++)
+            for (int v = 0; v < 3; v++) ms[3 * p + v] = mesh->tuneMs[p][v];
+    if (chosen) { chosen[0] = mesh->dm.spmvR64; chosen[1] = mesh->dm.spmvR32; }
     API_END
 }
 
@@ -735,7 +860,7 @@ int od_spmv(od_matrix *A, const double *x, double *y, int accumulate) {
     API_BEGIN
     require(A && x && y, "NULL argument");
     require(x != y, "x and y must differ (SpMDV asserts src != dest)");
-    launch_spmv<double>(A->mesh->dm, A->mesh->ctx->stream, A->vals, A->low, x, y, accumulate, nullptr);
+    launch_spmv<double>(A->mesh->dm, A->mesh->ctx->stream, A->vals, x, y, accumulate, nullptr);
     API_END
 }
 
@@ -755,7 +880,7 @@ int od_spmv_f32(od_matrix *A, const float *x, float *y, int accumulate) {
     API_BEGIN
     require(A && x && y && x != y, "bad arguments");
     require(A->vals32 != nullptr, "call od_matrix_sync_f32 first", OD_ERR_STATE);
-    launch_spmv<float>(A->mesh->dm, A->mesh->ctx->stream, A->vals32, A->low32, x, y, accumulate, nullptr);
+    launch_spmv<float>(A->mesh->dm, A->mesh->ctx->stream, A->vals32, x, y, accumulate, nullptr);
     API_END
 }
 
@@ -769,8 +894,8 @@ int od_spmv_timed(od_matrix *A, const void *x, void *y, int iters, int fp32, dou
     OD_HIP(hipEventCreate(&e1));
     OD_HIP(hipEventRecord(e0, st));
     for (int k = 0; k < iters; k++) {
-        if (fp32) launch_spmv<float>(A->mesh->dm, st, A->vals32, A->low32, (const float *)x, (float *)y, 0, nullptr);
-        else launch_spmv<double>(A->mesh->dm, st, A->vals, A->low, (const double *)x, (double *)y, 0, nullptr);
+        if (fp32) launch_spmv<float>(A->mesh->dm, st, A->vals32, (const float *)x, (float *)y, 0, nullptr);
+        else launch_spmv<double>(A->mesh->dm, st, A->vals, (const double *)x, (double *)y, 0, nullptr);
     }
     OD_HIP(hipEventRecord(e1, st));
     OD_HIP(hipEventSynchronize(e1));
@@ -795,6 +920,16 @@ int od_solver_create(od_mesh *mesh, int precond, double param, int precision, od
     *out = make_solver(mesh, precond, param, precision);
     API_END
 }
+int od_solver_set_refinement(od_solver *s, double innerTol, int maxOuter) {
+    API_BEGIN
+    require(s != nullptr, "NULL argument");
+    require(innerTol > 0 && innerTol < 1 && maxOuter >= 1, "innerTol must be in (0, 1) and maxOuter >= 1");
+    s->innerTol = innerTol;
+    s->maxOuter = maxOuter;
+    s->sweepIters.assign(maxOuter, 0);
+    API_END
+}
+
 int od_solver_destroy(od_solver *s) {
     API_BEGIN
     if (s) { (void)hipStreamSynchronize(s->mesh->ctx->stream); delete s; }
@@ -860,8 +995,6 @@ static od_integrator *make_integrator(od_mesh *mesh, int kind, int precond, doub
         it->M.vals = (double *)it->alloc(8 * (nv + kValuePad));
         it->A.vals = (double *)it->alloc(8 * (nv + kValuePad));
         if (precision == OD_MIXED) it->A.vals32 = (float *)it->alloc(4 * (nv + kValuePad));
-        // A keeps no lower copy: re-reading the shared upper storage through the transposed
-        // index hits L2/MALL, while a second 9-values-per-block array doubles the unique bytes
         it->d = (double *)it->alloc(8 * n); it->v = (double *)it->alloc(8 * n);
         it->a = (double *)it->alloc(8 * n); it->aNew = (double *)it->alloc(8 * n);
         it->fext = (double *)it->alloc(8 * n); it->imp = (double *)it->alloc(8 * n);
@@ -901,6 +1034,13 @@ int od_backward_euler_create(od_mesh *mesh, double dt, double alphaM, double alp
     od_integrator *it = make_integrator(mesh, 1, precond, param, OD_FP64);
     it->h = dt; it->alphaM = alphaM; it->alphaK = alphaK; it->tol = tol; it->maxIter = maxIter;
     *out = it;
+    API_END
+}
+
+int od_integrator_set_refinement(od_integrator *it, double innerTol, int maxOuter) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    return od_solver_set_refinement(it->solver, innerTol, maxOuter);
     API_END
 }
 
@@ -970,7 +1110,7 @@ static void integrator_step(od_integrator *it) {
         ElemParams P{0, 1, h + aK, (0.5 - beta) * h * h + aK * (1.0 - gamma) * h, 1.0, (1.0 - gamma) * h, aM};
         launch_element(m, st, it->d, it->v, it->a, P, mesh->blkC, mesh->nodeC, act);
         const double cK = gamma * h * aK + beta * h * h, cM = 1.0 + gamma * h * aM;
-        launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act, it->A.low, it->A.low32);
+        launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act);
         launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, 1.0, it->rhs, 0, act);
         OD_HIP(hipMemcpyAsync(it->aNew, it->a, sizeof(double) * mesh->topo.ndof, hipMemcpyDeviceToDevice, st));
         solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);
@@ -982,8 +1122,7 @@ static void integrator_step(od_integrator *it) {
         const double aM = it->alphaM, aK = it->alphaK;
         ElemParams P{0, 1, h + aK, 0.0, 1.0, 0.0, aM};
         launch_element(m, st, it->d, it->v, nullptr, P, mesh->blkC, mesh->nodeC, act);
-        launch_gather_blocks(m, st, mesh->blkC, h * (h + aK), it->M.vals, 1.0 + h * aM, it->A.vals, 0, nullptr, act, it->A.low,
-                             nullptr);
+        launch_gather_blocks(m, st, mesh->blkC, h * (h + aK), it->M.vals, 1.0 + h * aM, it->A.vals, 0, nullptr, act);
         launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, h, it->rhs, 0, act);
         // delta_v persists across steps as the warm start (aNew holds it)
         solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);

@@ -25,6 +25,11 @@ struct HipError : std::runtime_error {
 constexpr int kChunkRows = 64;      // block rows per workgroup chunk (never straddles an instance)
 constexpr int kBlock = 256;         // threads per workgroup
 constexpr int kValuePad = 16;       // trailing entries allocated after every value array
+// SpMV slots: 16 lanes per block row; slot k of row r holds the value offset and the column
+// of the row's k-th block (diagonal first, then the upper blocks, then the transposed ones)
+constexpr int kSpmvLanes = 16;
+constexpr int kSpmvThreads = kSpmvLanes * kChunkRows;
+constexpr int kSlotDiag = 1 << 29, kSlotTrans = 1 << 30, kSlotColMask = (1 << 29) - 1;
 
 // Device copy of the topology and material.
 struct DevMesh {
@@ -42,7 +47,11 @@ struct DevMesh {
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
     int *levelRows;                              // nRows
     int *chunkRow0, *chunkRow1, *chunkInst;      // nChunks
-    int *chunkOrder;      // workgroup -> chunk: consecutive workgroups of one XCD get spatially adjacent chunks
+    int *chunkSorted;     // chunks in spatial order, cut into 8 runs of perRun (one run per XCD)
+    int perRun;
+    int spmvR64, spmvR32; // chunks per SpMV workgroup (FP64 / FP32 values), set by the autotuner
+    int *slotOff, *slotCol;                      // kSpmvThreads per chunk (padding: slotCol = -1)
+    uint8_t *chunkLong;   // 1 when a row of the chunk has more than kSpmvLanes blocks (row-loop path)
     int *instChunkStart, *instRowStart;          // nInst+1
     int *tetInst, *rowInst;
     // material (per tet)
@@ -68,21 +77,28 @@ struct ElemParams {
 void launch_element(const DevMesh &m, hipStream_t s, const double *d, const double *v, const double *a,
                     const ElemParams &p, double *blkC, double *nodeC, const uint8_t *instActive);
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
-                          double *out, int accumulate, float *out32, const uint8_t *instActive, double *low = nullptr,
-                          float *low32 = nullptr);
+                          double *out, int accumulate, float *out32, const uint8_t *instActive);
 void launch_gather_rhs(const DevMesh &m, hipStream_t s, const double *nodeC, const double *fext, const double *imp,
                        double cf, double *out, int mode, const uint8_t *instActive);
 void launch_body_force(const DevMesh &m, hipStream_t s, const double *bodyPerTet, double *nodeC, double *f);
 void launch_expand_map(const DevMesh &m, hipStream_t s, int *map78);
 template <typename T>
-void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *x, T *y, int accumulate,
+void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
                  const InstScalars *sc);
+// PCG SpMV with R chunks per workgroup (1, 2 or 4), timed by optional dispatch events
+template <typename T>
+void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, const T *vals, const T *x, T *y, double *partials,
+                     const InstScalars *sc, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+constexpr int kSpmvVariants[3] = {1, 2, 4};
 
 // PCG building blocks (see od_kernels.hip for the exact phase semantics)
 struct PcgWork {
     double *partials;     // 3 * nChunks, slot q of chunk c at [q * nChunks + c] (slots: r.z or p.q, r.r, b.b)
     InstScalars *sc;      // nInst
     int *doneFlags;       // nInst, mirrors sc[i].done for host polling
+    // inner solves of the mixed form: the outer scalars, whose eps (tol^2 |b|^2 of the
+    // FP64 system) bounds the inner target from below (see k_finalize, PH_INIT)
+    const InstScalars *outer = nullptr;
 };
 template <typename T>
 void pcg_init_residual(const DevMesh &m, hipStream_t s, const T *b, const T *q, T *r, PcgWork &w);
@@ -92,7 +108,7 @@ template <typename T>
 void pcg_dot_rz_init(const DevMesh &m, hipStream_t s, const T *r, const T *z, T *p, PcgWork &w, double tol, int maxIt,
                      const uint8_t *preDone);
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *p, T *q, PcgWork &w,
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w,
                hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w);
 template <typename T>

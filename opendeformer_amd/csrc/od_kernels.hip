@@ -236,8 +236,7 @@ __constant__ int c_diagSrc[6] = {0, 1, 2, 4, 5, 8};
 
 __global__ void k_gather_blocks(DevMesh m, const double *__restrict__ C, double cK, const double *__restrict__ M,
                                 double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
-                                const uint8_t *__restrict__ instActive, double *__restrict__ low,
-                                float *__restrict__ low32) {
+                                const uint8_t *__restrict__ instActive) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= m.nBlocks * 9) return;
     const int b = (int)(gid / 9), c = (int)(gid % 9);
@@ -254,20 +253,14 @@ __global__ void k_gather_blocks(DevMesh m, const double *__restrict__ C, double 
     if (accumulate) o += out[vi];
     out[vi] = o;
     if (out32) out32[vi] = (float)o;
-    if (!diag) {   // lower copy: the block again at its place in the transposed list of its column row
-        const int64_t li = 9 * (int64_t)m.tPos[b] + c;
-        if (low) low[li] = o;
-        if (low32) low32[li] = (float)o;
-    }
 }
 
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
-                          double *out, int accumulate, float *out32, const uint8_t *instActive, double *low,
-                          float *low32) {
+                          double *out, int accumulate, float *out32, const uint8_t *instActive) {
     int64_t n = m.nBlocks * 9;
     int grid = (int)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_gather_blocks, dim3(grid), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out, accumulate, out32, instActive,
-                       low, low32);
+    hipLaunchKernelGGL(k_gather_blocks, dim3(grid), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out, accumulate, out32,
+                       instActive);
     OD_HIP(hipGetLastError());
 }
 
@@ -366,19 +359,20 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Sums NQ per-thread values over the workgroup in a fixed order; thread 0 gets the result.
 template <int NQ>
 __device__ __forceinline__ void block_sum(double (&v)[NQ]) {
-    __shared__ double lds[16 * 3];
+    static_assert(NQ >= 1 && NQ <= 4, "block_sum holds up to 4 values per wave");
+    __shared__ double lds[16 * 4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         double s = wave_sum(v[q]);
-        if (lane == 0) lds[w * 3 + q] = s;
+        if (lane == 0) lds[w * NQ + q] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             double s = 0;
-            for (int k = 0; k < nw; k++) s += lds[k * 3 + q];
+            for (int k = 0; k < nw; k++) s += lds[k * NQ + q];
             v[q] = s;
         }
     }
@@ -391,7 +385,13 @@ struct FinalArgs {
     double tol;
     int maxIt;
     const uint8_t *preDone;
+    const InstScalars *outer;
 };
+
+// An inner (FP32) solve of the mixed form stops at innerTol relative to its own right-hand
+// side, or as soon as its residual is below half the outer FP64 tolerance, whichever is
+// looser: the correction then meets the outer test without a second sweep.
+constexpr double kInnerOuterScale2 = 0.25;
 
 // One workgroup per instance: reduce its chunk partials (fixed order) and update its scalars.
 __global__ __launch_bounds__(1024) void k_finalize(DevMesh m, int phase, const double *__restrict__ partials,
@@ -413,6 +413,7 @@ __global__ __launch_bounds__(1024) void k_finalize(DevMesh m, int phase, const d
         S.delta = t[0]; S.rr = t[1]; S.bb = t[2];
         S.it = 0; S.maxIt = fa.maxIt;
         S.eps = fa.tol * fa.tol * S.bb;
+        if (fa.outer) S.eps = fmax(S.eps, kInnerOuterScale2 * fa.outer[inst].eps);
         S.done = (fa.preDone && fa.preDone[inst]) || !(S.rr > S.eps) || fa.maxIt <= 0;
         break;
     case PH_ALPHA:
@@ -444,116 +445,207 @@ void launch_finalize(const DevMesh &m, hipStream_t s, int phase, PcgWork &w, dou
     // a wide workgroup for few large instances, a narrow one for many small instances
     const int threads = m.nInst == 1 ? 1024 : 256;
     hipLaunchKernelGGL(k_finalize, dim3(m.nInst), dim3(threads), 0, s, m, phase, w.partials, w.sc, w.doneFlags,
-                       FinalArgs{tol, maxIt, preDone});
+                       FinalArgs{tol, maxIt, preDone, phase == PH_INIT ? w.outer : nullptr});
     OD_HIP(hipGetLastError());
 }
 
 // =====================================================================================
 // SpMV over the canonical upper-BSR layout: one workgroup = one 64-row chunk, 8 lanes per row
 // =====================================================================================
-// One block row with 16 lanes, one branch-free round for interior rows: item 0 is the packed
-// diagonal block, items 1..nOff the row's upper blocks (contiguous values), the remaining items
-// the transposed lower blocks reached through the transposed index.  Value arrays are padded
+// One stored 3x3 block and the three entries of x it multiplies.  Value arrays are padded
 // by kValuePad entries so the 9-value load of a trailing diagonal block stays in bounds.
 template <typename T>
-__device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ low,
-                                         const T *__restrict__ x, int r, int lane, T &o0, T &o1, T &o2) {
-    const int v0 = m.valOff[r], b0 = m.rowPtr[r];
-    const int nOff = m.rowPtr[r + 1] - b0 - 1;
-    const int t0 = m.tPtr[r], nItems = 1 + nOff + (m.tPtr[r + 1] - t0);
-    T a0 = 0, a1 = 0, a2 = 0;
-    for (int it = lane; it < nItems; it += 16) {
-        const bool up = it <= nOff;                       // diagonal or upper block
-        const int k = t0 + (up ? 0 : it - nOff - 1);
-        const int col = up ? m.colIdx[b0 + it] : m.tRow[k];
-        // transposed blocks: contiguous per row in the lower copy when the matrix keeps one
-        const T *A = up ? vals + (it == 0 ? v0 : v0 + 6 + 9 * (it - 1)) : (low ? low + 9 * k : vals + m.tOff[k]);
-        T e0 = A[0], e1 = A[1], e2 = A[2], e3 = A[3], e4 = A[4], e5 = A[5], e6 = A[6], e7 = A[7], e8 = A[8];
-        const T x0 = x[3 * col], x1 = x[3 * col + 1], x2 = x[3 * col + 2];
-        if (it == 0) {            // packed symmetric diagonal [e0 e1 e2; e1 e3 e4; e2 e4 e5]
-            e8 = e5; e7 = e4; e6 = e2; e5 = e4; e4 = e3; e3 = e1;
-        } else if (!up) {         // transposed block
-            T t;
-            t = e1; e1 = e3; e3 = t;
-            t = e2; e2 = e6; e6 = t;
-            t = e5; e5 = e7; e7 = t;
-        }
-        a0 += e0 * x0 + e1 * x1 + e2 * x2;
-        a1 += e3 * x0 + e4 * x1 + e5 * x2;
-        a2 += e6 * x0 + e7 * x1 + e8 * x2;
+struct BlockX {
+    T e[9], x[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_block(const T *__restrict__ A, const T *__restrict__ xc, BlockX<T> &b) {
+#pragma unroll
+    for (int q = 0; q < 9; q++) b.e[q] = A[q];
+    b.x[0] = xc[0]; b.x[1] = xc[1]; b.x[2] = xc[2];
+}
+
+// a += B x for the packed symmetric diagonal [e0 e1 e2; e1 e3 e4; e2 e4 e5], a row-major
+// upper block, or the transpose of one.
+template <typename T>
+__device__ __forceinline__ void block_mv(const BlockX<T> &b, bool diag, bool trans, T &a0, T &a1, T &a2) {
+    T e0 = b.e[0], e1 = b.e[1], e2 = b.e[2], e3 = b.e[3], e4 = b.e[4], e5 = b.e[5], e6 = b.e[6], e7 = b.e[7],
+      e8 = b.e[8];
+    if (diag) {
+        e8 = e5; e7 = e4; e6 = e2; e5 = e4; e4 = e3; e3 = e1;
+    } else if (trans) {
+        T t;
+        t = e1; e1 = e3; e3 = t;
+        t = e2; e2 = e6; e6 = t;
+        t = e5; e5 = e7; e7 = t;
     }
+    a0 += e0 * b.x[0] + e1 * b.x[1] + e2 * b.x[2];
+    a1 += e3 * b.x[0] + e4 * b.x[1] + e5 * b.x[2];
+    a2 += e6 * b.x[0] + e7 * b.x[1] + e8 * b.x[2];
+}
+
+template <typename T>
+__device__ __forceinline__ void sum16(T &a0, T &a1, T &a2) {
 #pragma unroll
     for (int o = 8; o > 0; o >>= 1) {
         a0 += __shfl_xor(a0, o, 16);
         a1 += __shfl_xor(a1, o, 16);
         a2 += __shfl_xor(a2, o, 16);
     }
+}
+
+// One block row with 16 lanes through the row pointers (rows with more than 16 blocks):
+// item 0 is the diagonal block, items 1..nOff the row's upper blocks (contiguous values),
+// the remaining items the transposed lower blocks reached through the transposed index.
+template <typename T>
+__device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x, int r,
+                                         int lane, T &o0, T &o1, T &o2) {
+    const int v0 = m.valOff[r], b0 = m.rowPtr[r];
+    const int nOff = m.rowPtr[r + 1] - b0 - 1;
+    const int t0 = m.tPtr[r], nItems = 1 + nOff + (m.tPtr[r + 1] - t0);
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int it = lane; it < nItems; it += kSpmvLanes) {
+        const bool up = it <= nOff;
+        const int k = t0 + (up ? 0 : it - nOff - 1);
+        const int col = up ? m.colIdx[b0 + it] : m.tRow[k];
+        const T *A = up ? vals + (it == 0 ? v0 : v0 + 6 + 9 * (it - 1)) : vals + m.tOff[k];
+        BlockX<T> b;
+        load_block<T>(A, x + 3 * col, b);
+        block_mv<T>(b, it == 0, !up, a0, a1, a2);
+    }
+    sum16(a0, a1, a2);
     o0 = a0; o1 = a1; o2 = a2;
 }
 
-constexpr int kSpmvThreads = 16 * kChunkRows;
-
-// y = A x (or y += A x); optional chunk partial x.y into slot 0.
-template <typename T>
-__global__ __launch_bounds__(kSpmvThreads) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ low,
-                                                       const T *__restrict__ x, T *__restrict__ y, int accumulate,
-                                                       const InstScalars *sc) {
-    const int chunk = m.chunkOrder[blockIdx.x];
-    if (sc && sc[m.chunkInst[chunk]].done) return;
-    const int r = m.chunkRow0[chunk] + (threadIdx.x >> 4), lane = threadIdx.x & 15;
-    if (r >= m.chunkRow1[chunk]) return;
-    T o0, o1, o2;
-    row_spmv<T>(m, vals, low, x, r, lane, o0, o1, o2);
-    if (lane == 0) {
-        if (accumulate) { o0 += y[3 * r]; o1 += y[3 * r + 1]; o2 += y[3 * r + 2]; }
-        y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
-    }
+// Chunk q (0-based) of the run of the XCD this workgroup runs on, or -1 past the run's end:
+// workgroup w runs on XCD (w mod 8) and is that XCD's (w div 8)-th.
+__device__ __forceinline__ int run_chunk(const DevMesh &m, int q) {
+    const int x = (int)(blockIdx.x & 7), idx = x * m.perRun + q;
+    return (q < m.perRun && idx < m.nChunks) ? m.chunkSorted[idx] : -1;
 }
 
-// The PCG product q = A p with the chunk partial p.q (slot 0).
-template <typename T>
-__global__ __launch_bounds__(kSpmvThreads) void k_spmv_dot(DevMesh m, const T *__restrict__ vals,
-                                                           const T *__restrict__ low, const T *__restrict__ x,
-                                                           T *__restrict__ y, double *__restrict__ partials,
-                                                           const InstScalars *__restrict__ sc) {
-    const int chunk = m.chunkOrder[blockIdx.x];
-    const bool skip = sc[m.chunkInst[chunk]].done;
-    const int r = m.chunkRow0[chunk] + (threadIdx.x >> 4), lane = threadIdx.x & 15;
-    double v[1] = {0.0};
-    if (!skip && r < m.chunkRow1[chunk]) {
-        T o0, o1, o2;
-        row_spmv<T>(m, vals, low, x, r, lane, o0, o1, o2);
-        if (lane == 0) {
-            y[3 * r] = o0; y[3 * r + 1] = o1; y[3 * r + 2] = o2;
-            v[0] = (double)x[3 * r] * (double)o0 + (double)x[3 * r + 1] * (double)o1 + (double)x[3 * r + 2] * (double)o2;
+// y = A x (or y += A x) for R chunks per workgroup.  Every load is unconditional (slots of
+// padding, inactive or long-row chunks point at valid memory and their results are dropped),
+// so the slot loads of all R chunks, then all their block and x loads, are in flight together.
+// The dot partial x.y of each chunk goes to partials[chunk] when partials is set.
+template <typename T, int R>
+__device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x,
+                                            T *__restrict__ y, int accumulate, double *__restrict__ partials,
+                                            const InstScalars *__restrict__ sc) {
+    const int j = (int)(blockIdx.x >> 3), lane = threadIdx.x & 15, lr = threadIdx.x >> 4;
+    int chunk[R], r[R], off[R], cf[R];
+    bool act[R], lng[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        chunk[i] = run_chunk(m, R * j + i);
+        const int c = chunk[i] >= 0 ? chunk[i] : 0;
+        act[i] = chunk[i] >= 0 && !(sc && sc[m.chunkInst[c]].done);
+        lng[i] = m.chunkLong[c] != 0;
+        r[i] = m.chunkRow0[c] + lr;
+        const size_t slot = (size_t)c * kSpmvThreads + threadIdx.x;
+        off[i] = m.slotOff[slot];
+        cf[i] = m.slotCol[slot];
+        if (!act[i] || lng[i]) cf[i] = -1;
+    }
+    BlockX<T> b[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const bool ok = cf[i] >= 0;
+        load_block<T>(vals + (ok ? off[i] : 0), x + 3 * (ok ? (cf[i] & kSlotColMask) : 0), b[i]);
+    }
+    T a[R][3];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        a[i][0] = a[i][1] = a[i][2] = 0;
+        block_mv<T>(b[i], (cf[i] & kSlotDiag) != 0, (cf[i] & kSlotTrans) != 0, a[i][0], a[i][1], a[i][2]);
+        if (cf[i] < 0) a[i][0] = a[i][1] = a[i][2] = 0;
+    }
+    double v[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        v[i] = 0.0;
+        if (!act[i]) continue;                      // uniform over the workgroup
+        const bool inRow = r[i] < m.chunkRow1[chunk[i]];
+        if (lng[i]) {
+            if (inRow) row_spmv<T>(m, vals, x, r[i], lane, a[i][0], a[i][1], a[i][2]);
+        } else {
+            sum16(a[i][0], a[i][1], a[i][2]);
+        }
+        if (inRow && lane == 0) {
+            T o0 = a[i][0], o1 = a[i][1], o2 = a[i][2];
+            const int rr = r[i];
+            if (accumulate) { o0 += y[3 * rr]; o1 += y[3 * rr + 1]; o2 += y[3 * rr + 2]; }
+            y[3 * rr] = o0; y[3 * rr + 1] = o1; y[3 * rr + 2] = o2;
+            if (partials)
+                v[i] = (double)x[3 * rr] * (double)o0 + (double)x[3 * rr + 1] * (double)o1 +
+                       (double)x[3 * rr + 2] * (double)o2;
         }
     }
-    block_sum<1>(v);
-    if (threadIdx.x == 0) partials[chunk] = v[0];
+    if (partials) {
+        block_sum<R>(v);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int i = 0; i < R; i++)
+                if (chunk[i] >= 0) partials[chunk[i]] = v[i];
+        }
+    }
 }
 
 template <typename T>
-void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *x, T *y, int accumulate,
-                 const InstScalars *sc) {
-    hipLaunchKernelGGL(k_spmv<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, low, x, y, accumulate, sc);
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
+                                                       T *__restrict__ y, int accumulate, const InstScalars *sc) {
+    spmv_chunks<T, 1>(m, vals, x, y, accumulate, nullptr, sc);
+}
+
+// The PCG product q = A p with the chunk partials p.q (slot 0), R chunks per workgroup.
+template <typename T, int R>
+__global__ __launch_bounds__(kSpmvThreads) void k_spmv_dot(DevMesh m, const T *__restrict__ vals,
+                                                           const T *__restrict__ x, T *__restrict__ y,
+                                                           double *__restrict__ partials,
+                                                           const InstScalars *__restrict__ sc) {
+    spmv_chunks<T, R>(m, vals, x, y, 0, partials, sc);
+}
+
+static inline int spmv_grid(const DevMesh &m, int R) { return 8 * ((m.perRun + R - 1) / R); }
+
+template <typename T>
+void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, const T *vals, const T *x, T *y, double *partials,
+                     const InstScalars *sc, hipEvent_t start, hipEvent_t stop) {
+    // With events, the launch records them from the dispatch's own begin/end timestamps,
+    // so their elapsed time is the kernel's execution alone (measurement mode).
+    auto go = [&](auto kernel) {
+        if (start)
+            hipExtLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(kSpmvThreads), 0, s, start, stop, 0, m, vals, x, y,
+                                  partials, sc);
+        else
+            hipLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(kSpmvThreads), 0, s, m, vals, x, y, partials, sc);
+    };
+    if (R == 4) go(k_spmv_dot<T, 4>);
+    else if (R == 2) go(k_spmv_dot<T, 2>);
+    else go(k_spmv_dot<T, 1>);
     OD_HIP(hipGetLastError());
 }
-template void launch_spmv<double>(const DevMesh &, hipStream_t, const double *, const double *, const double *, double *,
-                                  int, const InstScalars *);
-template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, const float *, const float *, float *, int,
+template void launch_spmv_dot<double>(const DevMesh &, hipStream_t, int, const double *, const double *, double *,
+                                      double *, const InstScalars *, hipEvent_t, hipEvent_t);
+template void launch_spmv_dot<float>(const DevMesh &, hipStream_t, int, const float *, const float *, float *, double *,
+                                     const InstScalars *, hipEvent_t, hipEvent_t);
+
+template <typename T>
+void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
+                 const InstScalars *sc) {
+    hipLaunchKernelGGL(k_spmv<T>, dim3(spmv_grid(m, 1)), dim3(kSpmvThreads), 0, s, m, vals, x, y, accumulate, sc);
+    OD_HIP(hipGetLastError());
+}
+template void launch_spmv<double>(const DevMesh &, hipStream_t, const double *, const double *, double *, int,
+                                  const InstScalars *);
+template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, const float *, float *, int,
                                  const InstScalars *);
 
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *low, const T *p, T *q, PcgWork &w,
-               hipEvent_t start, hipEvent_t stop) {
-    // With events, the launch records them from the dispatch's own begin/end timestamps,
-    // so their elapsed time is the kernel's execution alone (measurement mode).
-    if (start)
-        hipExtLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, start, stop, 0, m, vals, low, p,
-                              q, w.partials, w.sc);
-    else
-        hipLaunchKernelGGL(k_spmv_dot<T>, dim3(m.nChunks), dim3(kSpmvThreads), 0, s, m, vals, low, p, q, w.partials, w.sc);
-    OD_HIP(hipGetLastError());
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w, hipEvent_t start,
+               hipEvent_t stop) {
+    launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, vals, p, q, w.partials, w.sc, start, stop);
 }
 
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w) {
@@ -1003,8 +1095,8 @@ void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDo
     template void pcg_jacobi<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &);                  \
     template void pcg_dot_rz_init<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &, double, int, \
                                      const uint8_t *);                                                                \
-    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, const T *, T *, PcgWork &,         \
-                               hipEvent_t, hipEvent_t);                                                           \
+    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &, hipEvent_t,       \
+                               hipEvent_t);                                                                       \
     template void pcg_update_xr<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, T *, const T *, T *,      \
                                    PcgWork &);                                                                        \
     template void pcg_dot_rz<T>(const DevMesh &, hipStream_t, const T *, const T *, PcgWork &);                       \

@@ -256,9 +256,9 @@ std::string buildTopology(Topology &t, int chunkRows, int nXcd) {
     // ---- XCD-aware launch order --------------------------------------------------
     // Workgroup w is dispatched to XCD (w mod nXcd) and each XCD runs its workgroups in
     // increasing w.  Chunks are sorted by (instance, centroid along the instance's longest
-    // axis) and cut into nXcd contiguous runs; workgroup nXcd*j + x gets element j of run x,
-    // so every XCD sweeps one spatial slab and neighbouring rows (which share x entries and
-    // transposed blocks) meet in the same L2.
+    // axis) and cut into nXcd contiguous runs; the kernels give the j-th workgroup of XCD x
+    // the next chunks of run x, so every XCD sweeps one spatial slab and neighbouring rows
+    // (which share x entries and transposed blocks) meet in the same L2.
     {
         const int nc = (int)t.chunkRow0.size();
         std::vector<double> key(nc, 0.0);
@@ -282,16 +282,8 @@ std::string buildTopology(Topology &t, int chunkRows, int nXcd) {
             if (t.chunkInst[a] != t.chunkInst[b]) return t.chunkInst[a] < t.chunkInst[b];
             return key[a] < key[b];
         });
-        const int per = (nc + nXcd - 1) / nXcd;
-        std::vector<int> order;
-        order.reserve(nc);
-        // run x holds sorted[x*per .. (x+1)*per); workgroup nXcd*j + x -> run x, element j
-        for (int j = 0; j < per; j++)
-            for (int x = 0; x < nXcd; x++) {
-                int idx = x * per + j;
-                if (idx < nc) order.push_back(sorted[idx]);
-            }
-        t.chunkOrder = order;
+        t.chunkSorted = sorted;
+        t.perRun = (nc + nXcd - 1) / nXcd;
     }
     return "";
 }

@@ -56,7 +56,8 @@ struct Topology {
     std::vector<int> instRowStart;      // size nInst+1
     std::vector<int> instChunkStart;    // size nInst+1
     std::vector<int> chunkRow0, chunkRow1, chunkInst;
-    std::vector<int> chunkOrder;        // launch order: workgroup w -> chunk (see buildTopology)
+    std::vector<int> chunkSorted;       // chunks by (instance, centroid along the longest axis)
+    int perRun = 0;                     // chunkSorted is cut into nXcd runs of perRun chunks (last shorter)
     std::vector<int> rowInst;           // per row
     std::vector<int> tetInst;           // per tet
 };

@@ -236,3 +236,31 @@ def test_batch_equals_alone_bitwise(ctx):
         ni.run_time_steps(wb.steps)
         di, _, _ = ni.state()
         assert np.array_equal(di, db[3 * irs[i]:3 * irs[i + 1]])
+
+
+def test_spmv_variants_give_identical_pcg_bits(ctx):
+    """The autotuned SpMV launch shapes (1, 2 or 4 chunks per workgroup) compute every row
+    and every chunk partial in the same order: PCG results are bitwise equal."""
+    import opendeformer_amd as od
+    w = W.box_small("color8")
+    mesh, s = make(ctx, w)
+    table, r64, r32 = od.mesh_autotune(mesh)
+    assert r64 in (1, 2, 4) and r32 in (1, 2, 4) and all(v > 0 for v in table.values())
+    A = newmark_system(s, w)
+    Ad = od.Matrix(mesh); Ad.upload(A)
+    b = np.random.default_rng(17).standard_normal(mesh.ndof)
+    outs = []
+    for r in (1, 2, 4):
+        od.mesh_spmv_variant(mesh, r, r)
+        for prec in (capi.OD_FP64, capi.OD_MIXED):
+            if prec == capi.OD_MIXED:
+                Ad.sync_f32()
+            solver = od.CGSolver(mesh, tol=1e-8, max_iter=2000, precond=capi.OD_PRE_IC0, precond_param=1e-3,
+                                 precision=prec)
+            dx = od.DeviceArray(ctx, mesh.ndof); dx.zero()
+            it, rr = solver.solve(Ad, od.DeviceArray.from_host(ctx, b), dx)
+            outs.append((r, prec, int(it[0]), dx.download()))
+    for prec in (capi.OD_FP64, capi.OD_MIXED):
+        sel = [o for o in outs if o[1] == prec]
+        for o in sel[1:]:
+            assert o[2] == sel[0][2] and np.array_equal(o[3], sel[0][3]), (prec, o[0])
