@@ -77,8 +77,8 @@ def cpu_baseline(args, gpu_iters_per_step, full_ndof):
     from opendeformer_amd import mesh as M
     nx = 20
     m = M.box_mesh(nx, 40, 40, 5.0 * nx / 200, 1.0, 1.0, seed=3)
-    if args.ordering == "color8":
-        m = M.relabel(m, M.multicolor_permutation(m))
+    if args.ordering in ("color8", "color5"):
+        m = M.relabel(m, M.multicolor_permutation(m, 8 if args.ordering == "color8" else 5))
     E = M.log_uniform(3, M.STREAM_YOUNG, np.arange(m.n_tets, dtype=np.uint64), 1e4, 1e6)
     fixed = M.fixed_dofs_from_nodes(M.plane_fixed_nodes(m))
     t0 = time.perf_counter()
@@ -115,7 +115,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precond", choices=["jacobi", "ic0"], default="ic0")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
-    ap.add_argument("--ordering", choices=["natural", "color8"], default="color8")
+    ap.add_argument("--ordering", choices=["natural", "color8", "color5"], default="color5",
+                    help="node numbering: natural (x-slowest grid order) or grouped by an 8- or 5-colouring "
+                         "(5 colours: 5 IC(0) dependency levels)")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = 200x40x40)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true",

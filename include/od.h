@@ -155,6 +155,11 @@ int od_solver_destroy(od_solver *s);
  * comes first; at most maxOuter inner solves with FP64 residual corrections
  * are made (default 12). */
 int od_solver_set_refinement(od_solver *s, double innerTol, int maxOuter);
+/* Graph mode (no reference counterpart): PCG iterations are captured once as a
+ * HIP graph of 4 iterations and replayed (iterations after convergence exit
+ * at once on the device; the iteration cap is checked on the device).
+ * Ignored while od_ctx_profile is on. */
+int od_solver_set_graph(od_solver *s, int enable);
 int od_pcg_solve(od_solver *s, od_matrix *A, const double *b_dev, double *x_dev, double tol, int maxIter,
                  int *iters_host, double *relres_host);
 /* Preconditioner::Preprocess + solvePreconditionerSystem once: z = P(A)^-1 r */
@@ -176,6 +181,8 @@ int od_backward_euler_create(od_mesh *mesh, double dt, double alphaM, double alp
 int od_integrator_destroy(od_integrator *it);
 /* od_solver_set_refinement for the integrator's solver */
 int od_integrator_set_refinement(od_integrator *it, double innerTol, int maxOuter);
+/* od_solver_set_graph for the integrator's solver */
+int od_integrator_set_graph(od_integrator *it, int enable);
 /* Intergrator::setExternalVirtualWork (core/intergrator.h:18) */
 int od_integrator_set_external(od_integrator *it, const double *f_host);
 /* one-step load added to the next step's external force only */

@@ -31,6 +31,7 @@ EXPORTS = [
     "od_integrator_set_state", "od_integrator_state_dev", "od_integrator_stats", "od_backward_euler_update_mesh",
     "od_integrator_positions", "od_spmv_timed", "od_integrator_matrix", "od_ctx_profile", "od_ctx_profile_read",
     "od_solver_set_refinement", "od_integrator_set_refinement", "od_mesh_autotune", "od_mesh_spmv_variant",
+    "od_solver_set_graph", "od_integrator_set_graph",
 ]
 
 
@@ -63,6 +64,7 @@ def lib():
             "od_solver_create": [vp, i, d, i, pvp], "od_solver_destroy": [vp],
             "od_solver_set_refinement": [vp, d, i], "od_integrator_set_refinement": [vp, d, i],
             "od_mesh_autotune": [vp, dp, ip], "od_mesh_spmv_variant": [vp, i, i],
+            "od_solver_set_graph": [vp, i], "od_integrator_set_graph": [vp, i],
             "od_pcg_solve": [vp, vp, vp, vp, d, i, ip, dp], "od_precond_apply": [vp, vp, vp, vp],
             "od_ic0_factor": [vp, vp, dp],
             "od_newmark_create": [vp, d, d, d, d, d, i, d, d, i, i, i, pvp],

@@ -243,6 +243,9 @@ class CGSolver:
     def set_refinement(self, inner_tol, max_outer=12):
         check(lib().od_solver_set_refinement(self.h, inner_tol, max_outer))
 
+    def set_graph(self, enable=True):
+        check(lib().od_solver_set_graph(self.h, 1 if enable else 0))
+
     def ic0_factor(self, A: Matrix):
         L = np.zeros(self.mesh.n_values)
         check(lib().od_ic0_factor(self.h, A.h, dptr(L)))
@@ -290,6 +293,10 @@ class _Integrator:
     def set_refinement(self, inner_tol, max_outer=12):
         """Mixed precision: FP32 inner tolerance and the cap on FP64 corrections."""
         check(lib().od_integrator_set_refinement(self.h, inner_tol, max_outer))
+
+    def set_graph(self, enable=True):
+        """Replay PCG iterations from a captured HIP graph."""
+        check(lib().od_integrator_set_graph(self.h, 1 if enable else 0))
 
     def stats(self):
         it = np.zeros(self.mesh.n_inst, np.int32); rr = np.zeros(self.mesh.n_inst)

@@ -149,10 +149,20 @@ struct od_solver {
         allocs.push_back(ptr);
         return ptr;
     }
+    // graph mode: PCG iterations replayed from a captured block of kGraphIters iterations
+    bool useGraph = false;
+    struct IterGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<const void *> key;   // the pointers baked into the captured launches
+    } graph[2];                          // [FP64, FP32]
     ~od_solver() {
+        for (auto &g : graph)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (void *ptr : allocs) (void)hipFree(ptr);
     }
 };
+
+constexpr int kGraphIters = 4;
 
 struct od_integrator {
     od_mesh *mesh = nullptr;
@@ -368,26 +378,57 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     // stream).  Warm-started solves of consecutive steps need nearly the same count, so the
     // first batch is sized from the same solve of the previous step (`expect`); afterwards
     // the batches stay small.
+    auto iteration = [&]() {
+        hipEvent_t e0, e1;
+        prof_pair(mesh->ctx, &e0, &e1);
+        pcg_alpha<T>(m, st, vals, p, q, w, e0, e1);
+        pcg_alpha_finalize(m, st, w);
+        if (ic) {
+            pcg_update_xr<T>(m, st, p, q, x, r, (const T *)nullptr, (T *)nullptr, w);
+            launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, Ul, r, z, sc, preDone);
+            pcg_dot_rz<T>(m, st, r, z, w);
+        } else {
+            pcg_update_xr<T>(m, st, p, q, x, r, invDiag, z, w);
+        }
+        pcg_update_p<T>(m, st, z, p, w);
+    };
+    // Graph mode (not while timing launches): a block of kGraphIters iterations is captured
+    // once per set of buffers and replayed; iterations past convergence exit at once.
+    od_solver::IterGraph *g = nullptr;
+    if (s->useGraph && !mesh->ctx->prof) {
+        g = &s->graph[sizeof(T) == 8 ? 0 : 1];
+        const std::vector<const void *> key = {vals, Uf, Ul, invDiag, x, r, z, p, q, sc, doneFlags, preDone};
+        if (!g->exec || g->key != key) {
+            if (g->exec) { OD_HIP(hipGraphExecDestroy(g->exec)); g->exec = nullptr; }
+            hipGraph_t graph = nullptr;
+            OD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                for (int j = 0; j < kGraphIters; j++) iteration();
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            OD_HIP(hipStreamEndCapture(st, &graph));
+            const hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            OD_HIP(e);
+            g->key = key;
+        }
+    }
     int launched = 0;
     int batch = expect > 8 ? expect - 2 : 4;
     while (launched < maxIt) {
         if (count_not_done(mesh->ctx, doneFlags, m.nInst) == 0) break;
         const int k = std::min(batch, maxIt - launched);
-        for (int j = 0; j < k; j++) {
-            hipEvent_t e0, e1;
-            prof_pair(mesh->ctx, &e0, &e1);
-            pcg_alpha<T>(m, st, vals, p, q, w, e0, e1);
-            pcg_alpha_finalize(m, st, w);
-            if (ic) {
-                pcg_update_xr<T>(m, st, p, q, x, r, (const T *)nullptr, (T *)nullptr, w);
-                launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, Ul, r, z, sc, preDone);
-                pcg_dot_rz<T>(m, st, r, z, w);
-            } else {
-                pcg_update_xr<T>(m, st, p, q, x, r, invDiag, z, w);
-            }
-            pcg_update_p<T>(m, st, z, p, w);
+        if (g) {
+            // whole blocks; the device-side iteration cap (maxIt) still holds
+            for (int j = 0; j < k; j += kGraphIters) OD_HIP(hipGraphLaunch(g->exec, st));
+            launched += ((k + kGraphIters - 1) / kGraphIters) * kGraphIters;
+        } else {
+            for (int j = 0; j < k; j++) iteration();
+            launched += k;
         }
-        launched += k;
         batch = expect > 8 ? 3 : std::min(batch * 2, 32);
     }
 }
@@ -930,6 +971,13 @@ int od_solver_set_refinement(od_solver *s, double innerTol, int maxOuter) {
     API_END
 }
 
+int od_solver_set_graph(od_solver *s, int enable) {
+    API_BEGIN
+    require(s != nullptr, "NULL argument");
+    s->useGraph = enable != 0;
+    API_END
+}
+
 int od_solver_destroy(od_solver *s) {
     API_BEGIN
     if (s) { (void)hipStreamSynchronize(s->mesh->ctx->stream); delete s; }
@@ -1041,6 +1089,13 @@ int od_integrator_set_refinement(od_integrator *it, double innerTol, int maxOute
     API_BEGIN
     require(it != nullptr, "NULL argument");
     return od_solver_set_refinement(it->solver, innerTol, maxOuter);
+    API_END
+}
+
+int od_integrator_set_graph(od_integrator *it, int enable) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    return od_solver_set_graph(it->solver, enable);
     API_END
 }
 

@@ -156,20 +156,27 @@ def relabel(mesh: TetMesh, perm: np.ndarray) -> TetMesh:
                    shape=mesh.shape, size=mesh.size, extra=dict(mesh.extra, perm=perm))
 
 
-def multicolor_permutation(mesh: TetMesh) -> np.ndarray:
+def multicolor_permutation(mesh: TetMesh, colors: int = 8) -> np.ndarray:
     """Node relabelling that groups nodes by colour so no two adjacent nodes share one.
 
-    For Kuhn grids the colour (i + 2j + 4k) mod 8 is proper: the 7 edge
-    directions of the split are 0/1 vectors whose codes 1..7 are non-zero mod 8.
-    Within a colour the natural order is kept.  Returns perm with
-    new_id = perm[old_id].
+    The 7 edge directions of the Kuhn split are the non-zero 0/1 vectors
+    (a, b, c).  With 8 colours, (i + 2j + 4k) mod 8 is proper: the direction
+    codes a + 2b + 4c are 1..7.  With 5 colours, (i + j + 2k) mod 5 is proper:
+    the codes a + b + 2c are 1..4, never 0 mod 5.  Fewer colours give fewer
+    IC(0) dependency levels.  Within a colour the natural order is kept.
+    Returns perm with new_id = perm[old_id].
     """
     nx, ny, nz = mesh.shape
     n = np.arange(mesh.n_nodes)
     k = n % (nz + 1)
     j = (n // (nz + 1)) % (ny + 1)
     i = n // ((nz + 1) * (ny + 1))
-    color = (i + 2 * j + 4 * k) % 8
+    if colors == 8:
+        color = (i + 2 * j + 4 * k) % 8
+    elif colors == 5:
+        color = (i + j + 2 * k) % 5
+    else:
+        raise ValueError("multicolour orderings of Kuhn grids: 8 or 5 colours")
     order = np.lexsort((n, color))         # old ids sorted by (colour, old id)
     perm = np.empty_like(order)
     perm[order] = np.arange(mesh.n_nodes)

@@ -77,8 +77,8 @@ class Workload:
 
 
 def _single(name, m: M.TetMesh, kind, E, nu, dt, steps, ordering="natural", impulse=None, meta=None):
-    if ordering == "color8":
-        perm = M.multicolor_permutation(m)
+    if ordering in ("color8", "color5"):
+        perm = M.multicolor_permutation(m, 8 if ordering == "color8" else 5)
         m = M.relabel(m, perm)
         if impulse is not None:
             imp = np.zeros_like(impulse)
@@ -147,8 +147,8 @@ def batch(n_inst=256, seed=4, tet_lo=500, tet_hi=50000, cell=0.05, steps=120, or
         s = seed * 100003 + i
         nx, ny, nz, kind, E, nu = instance_shape(s, tet_lo, tet_hi)
         m = M.box_mesh(nx, ny, nz, nx * cell, ny * cell, nz * cell, seed=s)
-        if ordering == "color8":
-            m = M.relabel(m, M.multicolor_permutation(m))
+        if ordering in ("color8", "color5"):
+            m = M.relabel(m, M.multicolor_permutation(m, 8 if ordering == "color8" else 5))
         off = node_start[-1]
         Xs.append(m.X); Ts.append(m.tets + off); Fs.append(m.facets + off)
         fixed.append(M.plane_fixed_nodes(m, axis=0, value=0.0) + off)

@@ -30,7 +30,7 @@ def make(ctx, w):
     return mesh, s
 
 
-@pytest.mark.parametrize("name", ["box_small", "box_small_color8", "partial_box"])
+@pytest.mark.parametrize("name", ["box_small", "box_small_color8", "box_small_color5", "partial_box"])
 def test_pattern_and_element_map_bit_exact(ctx, name):
     import opendeformer_amd as od
     if name == "partial_box":
@@ -38,7 +38,7 @@ def test_pattern_and_element_map_bit_exact(ctx, name):
         w = W.Workload("p", m.X, m.tets, m.facets, np.array([0, 7, 13], np.int32), np.zeros(m.n_tets, np.int32),
                        np.full(m.n_tets, 1e5), np.full(m.n_tets, 0.4), np.full(m.n_tets, 1000.0), 0.01, 1)
     else:
-        w = W.box_small("color8" if name.endswith("color8") else "natural")
+        w = W.box_small(name[len("box_small_"):] if name.startswith("box_small_") else "natural")
     mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
     s = O.System(w.X, w.tets, w.fixed_dofs)
     for a, b in zip(mesh.pattern(), s.pattern()):
@@ -113,7 +113,7 @@ def newmark_system(s, w, kind=None):
     return Mv + 0.25 * w.dt ** 2 * K
 
 
-@pytest.mark.parametrize("ordering", ["natural", "color8"])
+@pytest.mark.parametrize("ordering", ["natural", "color8", "color5"])
 def test_ic0_factor_matches_oracle(ctx, ordering):
     import opendeformer_amd as od
     w = W.box_small(ordering)
@@ -175,6 +175,7 @@ def run_pair(ctx, w, steps, pre=capi.OD_PRE_IC0, opre=O.PRE_IC0, precision=capi.
 
 @pytest.mark.parametrize("ordering,pre,opre", [("natural", capi.OD_PRE_IC0, O.PRE_IC0),
                                                 ("color8", capi.OD_PRE_IC0, O.PRE_IC0),
+                                                ("color5", capi.OD_PRE_IC0, O.PRE_IC0),
                                                 ("natural", capi.OD_PRE_JACOBI, O.PRE_JACOBI)])
 def test_newmark_box_small_100_steps(ctx, ordering, pre, opre):
     w = W.box_small(ordering)
@@ -264,3 +265,24 @@ def test_spmv_variants_give_identical_pcg_bits(ctx):
         sel = [o for o in outs if o[1] == prec]
         for o in sel[1:]:
             assert o[2] == sel[0][2] and np.array_equal(o[3], sel[0][3]), (prec, o[0])
+
+
+@pytest.mark.parametrize("precision", [capi.OD_FP64, capi.OD_MIXED])
+def test_graph_replay_matches_eager_bitwise(ctx, precision):
+    """PCG iterations replayed from a captured HIP graph give the eager path's bits, step
+    after step (eager element, assembly and factor kernels run between the replays)."""
+    import opendeformer_amd as od
+    w = W.box_small("color8")
+    outs = []
+    for graph in (False, True):
+        mesh, s = make(ctx, w)
+        nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=precision)
+        nm.set_graph(graph)
+        nm.set_external_virtual_work(s.body_force(w.body))
+        states = []
+        for _ in range(6):
+            nm.run_one_time_step()
+            states.append((nm.state()[0].copy(), int(nm.stats()[0][0])))
+        outs.append(states)
+    for (de, ie), (dg, ig) in zip(*outs):
+        assert ie == ig and np.array_equal(de, dg)

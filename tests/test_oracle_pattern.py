@@ -84,7 +84,28 @@ CASES = {
     "partial": lambda: (M.box_mesh(3, 2, 2, 1, 1, 1, seed=5).X, M.box_mesh(3, 2, 2, 1, 1, 1, seed=5).tets,
                         np.array([0, 1, 2, 4, 7, 8, 13], np.int32)),
     "color8": lambda: (W.box_small("color8").X, W.box_small("color8").tets, W.box_small("color8").fixed_dofs),
+    "color5": lambda: (W.box_small("color5").X, W.box_small("color5").tets, W.box_small("color5").fixed_dofs),
 }
+
+
+@pytest.mark.parametrize("colors", [8, 5])
+def test_multicolour_orderings_are_proper(colors):
+    """No tet edge joins two nodes of one colour, so the IC(0) dependency levels of a
+    colour-grouped numbering number at most the colours."""
+    m = M.box_mesh(9, 6, 5, 1.0, 0.6, 0.5, seed=3)
+    perm = M.multicolor_permutation(m, colors)
+    n = np.arange(m.n_nodes)
+    k, j, i = n % 6, (n // 6) % 7, n // 42
+    color = (i + 2 * j + 4 * k) % 8 if colors == 8 else (i + j + 2 * k) % 5
+    assert len(np.unique(color)) == colors
+    for a in range(4):
+        for b in range(a + 1, 4):
+            assert (color[m.tets[:, a]] != color[m.tets[:, b]]).all()
+    # the relabelling groups colours: new ids of colour c precede those of colour c + 1
+    new_color = np.empty_like(color); new_color[perm] = color
+    assert (np.diff(new_color) >= 0).all()
+    with pytest.raises(ValueError):
+        M.multicolor_permutation(m, 4)
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
