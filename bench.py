@@ -69,6 +69,57 @@ def copy_calibration(mesh):
           f"{8 * mesh.n_values} bytes written per launch", flush=True)
 
 
+def batched_leg(args, n_inst=256, warmup=2, steps=20):
+    """256 uneven boxes (tets log-uniform [500, 5e4], aspect U(1, 6), corotational or
+    neo-Hookean, E log-uniform [1e4, 1e6], nu U(0.3, 0.48)) concatenated into one system and
+    stepped together: mixed-precision IC(0)-PCG to 1e-6 with per-instance convergence, PCG
+    iterations replayed from a captured HIP graph, SpMV launch shape from the autotuner."""
+    import opendeformer_amd as od
+    from opendeformer_amd import capi
+    from opendeformer_amd import workloads as W
+    w = W.batch(n_inst, seed=4, ordering="color5")
+    ctx = od.Context(0)
+    mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs, w.inst_node_start, w.inst_tet_start)
+    mesh.set_material(w.kind, w.E, w.nu, w.rho)
+    _, r64, r32 = od.mesh_autotune(mesh)
+    nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=capi.OD_MIXED, tol=1e-6)
+    nm.set_graph(True)
+    nm.set_external_virtual_work(mesh.body_force(w.body))
+    nm.run_time_steps(warmup)
+    ctx.sync()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nm.run_one_time_step()
+        its.append(nm.stats()[0].copy())
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / steps
+    its = np.array(its)
+    return {"workload": f"{n_inst} boxes, {mesh.n_tets} tets, {mesh.ndof} DOF in one concatenated system "
+                        f"(5-colour ordering per box), mixed precision, graph replay",
+            "ms_per_step": round(dt * 1e3, 3), "batch_steps_per_s": round(1.0 / dt, 3),
+            "instance_steps_per_s": round(n_inst / dt, 1), "steps": steps, "warmup": warmup,
+            "pcg_iters_per_instance_step": {"median": float(np.median(its)), "max": int(its.max()),
+                                            "min": int(its.min())},
+            "spmv_chunks_per_workgroup": {"fp64": r64, "fp32": r32}}
+
+
+def serving_leg(args, n_obj=64, tet_lo=1000, tet_hi=50000, rates=(50.0, 200.0), duration=4.0):
+    """Served stream on a pool scaled down from 1,024 boxes of [1k, 500k] tets (about 82M
+    tets, minutes of host-side mesh building) to n_obj boxes of [tet_lo, tet_hi] tets, with the
+    200 GB HBM budget scaled by the same tet ratio; Poisson arrivals for `duration` s per rate."""
+    from opendeformer_amd import serving as SV
+    from opendeformer_amd import workloads as W
+    pool = W.served_pool(n_obj, seed=5, tet_lo=tet_lo, tet_hi=tet_hi)
+    pool_tets = sum(w.n_tets for w in pool)
+    budget = int(200 * 2 ** 30 * pool_tets / 82.0e6)
+    srv = SV.build_server(pool, budget, slots=4)
+    legs = [SV.run_stream(srv, pool, r, duration, seed=6 + i) for i, r in enumerate(rates)]
+    return {"pool": {"objects": n_obj, "tets_range": [tet_lo, tet_hi], "tets": int(pool_tets),
+                     "hbm_budget_bytes": budget, "slots": 4, "precision": "mixed", "tol": 1e-6},
+            "streams": legs}
+
+
 def cpu_baseline(args, gpu_iters_per_step, full_ndof):
     """Times the FP64 oracle on a slab of the same workload (single thread)."""
     import oracle as O
@@ -122,6 +173,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true",
                     help="with --precision mixed: skip the plain FP64 path run (speed-up and error report)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the run, launch a streaming update of known size three times (the reference "
                          "that scales HBM byte counters)")
@@ -212,6 +265,10 @@ def main():
                              "spmv": fp64_r["spmv"]}
         line["speedup_vs_fp64"] = round(main_r["steps_s"] / fp64_r["steps_s"], 3)
         line["rel_err_vs_fp64_path"] = {"value": err, "at": f"displacement after step {args.warmup + args.steps}"}
+    if not args.no_batch:
+        line["batched"] = batched_leg(args)
+    if not args.no_serving:
+        line["serving"] = serving_leg(args)
     if not args.no_cpu:
         try:
             it_ref = fp64_r["iters"] if fp64_r is not None else main_r["iters"]

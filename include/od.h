@@ -95,6 +95,10 @@ int od_pattern_export(od_mesh *mesh, int *blockPcol_host, int *blockColumnOris_h
  * od_mesh_spmv_variant sets R directly (no tuning). */
 int od_mesh_autotune(od_mesh *mesh, double *ms_host, int *chosen_host);
 int od_mesh_spmv_variant(od_mesh *mesh, int r64, int r32);
+/* Rebinds a mesh (and the solvers and integrators built on it) to another
+ * context's stream on the same device, after the work pending on its current
+ * stream has finished (no reference counterpart; used by the serving loop). */
+int od_mesh_bind_ctx(od_mesh *mesh, od_ctx *ctx);
 /* Per-instance row range and IC(0) level structure for tests:
  * instRowStart_host (nInstances+1), level_host (nBlockRows). */
 int od_mesh_rows(od_mesh *mesh, int *instRowStart_host, int *level_host);
@@ -183,6 +187,8 @@ int od_integrator_destroy(od_integrator *it);
 int od_integrator_set_refinement(od_integrator *it, double innerTol, int maxOuter);
 /* od_solver_set_graph for the integrator's solver */
 int od_integrator_set_graph(od_integrator *it, int enable);
+/* device bytes held by the integrator, its solver and its mesh */
+int od_integrator_device_bytes(od_integrator *it, long long *bytes_host);
 /* Intergrator::setExternalVirtualWork (core/intergrator.h:18) */
 int od_integrator_set_external(od_integrator *it, const double *f_host);
 /* one-step load added to the next step's external force only */
@@ -206,6 +212,41 @@ int od_integrator_stats(od_integrator *it, int *iters_host, double *relres_host,
 int od_backward_euler_update_mesh(od_integrator *it);
 /* Simulator::getVertexPositions (core/simulator.cpp:24-33): float X + u per node */
 int od_integrator_positions(od_integrator *it, float *pos_host);
+
+/* ---- serving (no reference counterpart: the reference steps one mesh in one
+ * thread, core/simulator.cpp:24-38) ----------------------------------------------
+ * A pool of registered objects, each a mesh with its per-tet material, body
+ * force per tet and time step.  Objects become device-resident on first use
+ * and are evicted least-recently-used (state kept on the host, restored
+ * exactly) when a load needs room under hbmBudgetBytes.  nSlots (1..16)
+ * worker slots, each a host thread with its own HIP stream, serve requests:
+ * request k arrives at arrival_host[k] seconds after od_server_run starts and
+ * advances object obj_host[k] by nSteps_host[k] Newmark steps (beta 1/4,
+ * gamma 1/2, no damping, PCG to tol with the given preconditioner and
+ * precision); impulse forces on the nodes impNodes_host[impStart_host[k] ..
+ * impStart_host[k+1]) (3 doubles each in impForces_host) are added to its
+ * first step (impStart_host may be NULL).  A request starts once it has
+ * arrived, it is its object's oldest waiting request, a slot is free and its
+ * object is resident or fits; the slots' work overlaps on the device.  Each
+ * object's arithmetic is its own, so its results do not depend on the mix.
+ * Outputs per request: start and finish seconds and PCG iterations over its
+ * steps; stats_host[4]: peak resident bytes, evictions, loads, resident bytes
+ * at the end. */
+typedef struct od_server od_server;
+int od_server_create(int device, long long hbmBudgetBytes, int nSlots, int precision, int precond, double tol,
+                     od_server **out);
+int od_server_destroy(od_server *srv);
+int od_server_add_object(od_server *srv, int nNodes, const double *X_host, int nTets, const int *tets_host,
+                         int nConstrained, const int *constrained_host, const int *kind_host, const double *E_host,
+                         const double *nu_host, const double *rho_host, const double *bodyPerTet_host, double dt,
+                         int *objId);
+int od_server_run(od_server *srv, int nReq, const int *obj_host, const int *nSteps_host, const double *arrival_host,
+                  const int *impStart_host, const int *impNodes_host, const double *impForces_host,
+                  double *start_host, double *finish_host, int *iters_host, long long *stats_host);
+/* free-DOF state of an object (resident or not) */
+int od_server_object_state(od_server *srv, int objId, double *d_host, double *v_host, double *a_host);
+/* info[5]: ndof, resident (0/1), device bytes (estimate before its first load), steps taken, tets */
+int od_server_object_info(od_server *srv, int objId, long long *info_host);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,9 @@ EXPORTS = [
     "od_integrator_set_state", "od_integrator_state_dev", "od_integrator_stats", "od_backward_euler_update_mesh",
     "od_integrator_positions", "od_spmv_timed", "od_integrator_matrix", "od_ctx_profile", "od_ctx_profile_read",
     "od_solver_set_refinement", "od_integrator_set_refinement", "od_mesh_autotune", "od_mesh_spmv_variant",
-    "od_solver_set_graph", "od_integrator_set_graph",
+    "od_solver_set_graph", "od_integrator_set_graph", "od_mesh_bind_ctx", "od_integrator_device_bytes",
+    "od_server_create", "od_server_destroy", "od_server_add_object", "od_server_run", "od_server_object_state",
+    "od_server_object_info",
 ]
 
 
@@ -65,6 +67,11 @@ def lib():
             "od_solver_set_refinement": [vp, d, i], "od_integrator_set_refinement": [vp, d, i],
             "od_mesh_autotune": [vp, dp, ip], "od_mesh_spmv_variant": [vp, i, i],
             "od_solver_set_graph": [vp, i], "od_integrator_set_graph": [vp, i],
+            "od_mesh_bind_ctx": [vp, vp], "od_integrator_device_bytes": [vp, C.POINTER(ll)],
+            "od_server_create": [i, ll, i, i, i, d, pvp], "od_server_destroy": [vp],
+            "od_server_add_object": [vp, i, dp, i, ip, i, ip, ip, dp, dp, dp, dp, d, ip],
+            "od_server_run": [vp, i, ip, ip, dp, ip, ip, dp, dp, dp, ip, C.POINTER(ll)],
+            "od_server_object_state": [vp, i, dp, dp, dp], "od_server_object_info": [vp, i, C.POINTER(ll)],
             "od_pcg_solve": [vp, vp, vp, vp, d, i, ip, dp], "od_precond_apply": [vp, vp, vp, vp],
             "od_ic0_factor": [vp, vp, dp],
             "od_newmark_create": [vp, d, d, d, d, d, i, d, d, i, i, i, pvp],

@@ -338,3 +338,69 @@ class BackwardEulerIntegrator(_Integrator):
 
     def update_mesh_vertices_displacements(self):
         check(lib().od_backward_euler_update_mesh(self.h))
+
+
+class Server:
+    """Serving loop (od_server_*): a resident pool of objects, HBM-budget admission and
+    nSlots overlapping worker slots; requests advance one object by k Newmark steps."""
+
+    def __init__(self, budget_bytes, slots=4, precision=capi.OD_MIXED, precond=capi.OD_PRE_IC0, tol=1e-6, device=0):
+        self.h = C.c_void_p()
+        check(lib().od_server_create(device, int(budget_bytes), slots, precision, precond, tol, C.byref(self.h)))
+        self.ndof = []
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().od_server_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+    def add_object(self, X, tets, constrained_dofs, kind, E, nu, rho, body_per_tet, dt):
+        X = f64(X).reshape(-1, 3); tets = i32(tets).reshape(-1, 4); con = i32(constrained_dofs)
+        n = tets.shape[0]
+        kind = i32(np.broadcast_to(kind, (n,))); E = f64(np.broadcast_to(E, (n,)))
+        nu = f64(np.broadcast_to(nu, (n,))); rho = f64(np.broadcast_to(rho, (n,)))
+        body = f64(np.broadcast_to(body_per_tet, (n, 3)))
+        oid = C.c_int()
+        check(lib().od_server_add_object(self.h, X.shape[0], dptr(X), n, capi.iptr(tets), con.size,
+                                         capi.iptr(con) if con.size else None, capi.iptr(kind), dptr(E), dptr(nu),
+                                         dptr(rho), dptr(body), float(dt), C.byref(oid)))
+        self.ndof.append(3 * X.shape[0] - con.size)
+        return oid.value
+
+    def run(self, obj, n_steps, arrival, impulses=None):
+        """impulses: per request a list of (node, (fx, fy, fz)) or None.  Returns dict of
+        start/finish seconds, PCG iterations per request and run stats."""
+        obj = i32(obj); n_steps = i32(n_steps); arrival = f64(arrival)
+        k = obj.size
+        start = np.zeros(k); finish = np.zeros(k); iters = np.zeros(k, np.int32); stats = np.zeros(4, np.int64)
+        if impulses is not None:
+            starts = [0]; nodes = []; forces = []
+            for imp in impulses:
+                for node, fv in (imp or []):
+                    nodes.append(int(node)); forces.extend(float(x) for x in fv)
+                starts.append(len(nodes))
+            starts = i32(starts); nodes = i32(nodes if nodes else [0]); forces = f64(forces if forces else [0.0] * 3)
+            imp_args = (capi.iptr(starts), capi.iptr(nodes), dptr(forces))
+        else:
+            imp_args = (None, None, None)
+        check(lib().od_server_run(self.h, k, capi.iptr(obj), capi.iptr(n_steps), dptr(arrival), *imp_args,
+                                  dptr(start), dptr(finish), capi.iptr(iters),
+                                  stats.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return {"start": start, "finish": finish, "iters": iters,
+                "stats": {"peak_resident_bytes": int(stats[0]), "evictions": int(stats[1]), "loads": int(stats[2]),
+                          "resident_bytes": int(stats[3])}}
+
+    def object_state(self, oid):
+        n = self.ndof[oid]
+        d, v, a = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(lib().od_server_object_state(self.h, oid, dptr(d), dptr(v), dptr(a)))
+        return d, v, a
+
+    def object_info(self, oid):
+        info = np.zeros(5, np.int64)
+        check(lib().od_server_object_info(self.h, oid, info.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return {"ndof": int(info[0]), "resident": bool(info[1]), "bytes": int(info[2]), "steps": int(info[3]),
+                "tets": int(info[4])}

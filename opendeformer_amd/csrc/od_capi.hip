@@ -143,10 +143,12 @@ struct od_solver {
     int outer = 0;
     int lastMaxIters = 0;       // iterations of the previous solve (sizes the first launch batch)
     std::vector<int> sweepIters; // mixed form: iterations of each inner sweep of the previous solve
+    size_t bytes = 0;
     void *alloc(size_t b) {
         void *ptr = nullptr;
         OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
         allocs.push_back(ptr);
+        bytes += b;
         return ptr;
     }
     // graph mode: PCG iterations replayed from a captured block of kGraphIters iterations
@@ -180,11 +182,13 @@ struct od_integrator {
     std::vector<int> lastIters;
     std::vector<double> lastRelres;
     int lastOuter = 0, initIters = 0;
+    size_t bytes = 0;
     void *alloc(size_t b) {
         void *ptr = nullptr;
         OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
         OD_HIP(hipMemset(ptr, 0, std::max<size_t>(b, 16)));
         allocs.push_back(ptr);
+        bytes += b;
         return ptr;
     }
     ~od_integrator() {
@@ -498,6 +502,10 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     collect_relres(s, s->scOut);
 }
 
+namespace od {
+void set_last_error(const std::string &msg) { g_err = msg; }
+}
+
 // =====================================================================================
 // C-ABI
 // =====================================================================================
@@ -702,6 +710,17 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         throw;
     }
     *out = mesh;
+    API_END
+}
+
+int od_mesh_bind_ctx(od_mesh *mesh, od_ctx *ctx) {
+    API_BEGIN
+    require(mesh && ctx, "NULL argument");
+    require(ctx->device == mesh->ctx->device, "context of another device");
+    if (ctx != mesh->ctx) {
+        OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
+        mesh->ctx = ctx;
+    }
     API_END
 }
 
@@ -1089,6 +1108,13 @@ int od_integrator_set_refinement(od_integrator *it, double innerTol, int maxOute
     API_BEGIN
     require(it != nullptr, "NULL argument");
     return od_solver_set_refinement(it->solver, innerTol, maxOuter);
+    API_END
+}
+
+int od_integrator_device_bytes(od_integrator *it, long long *bytes) {
+    API_BEGIN
+    require(it && bytes, "NULL argument");
+    *bytes = (long long)(it->mesh->bytes + it->bytes + it->solver->bytes);
     API_END
 }
 

@@ -171,3 +171,56 @@ def subset(w: Workload, i: int) -> Workload:
     return Workload(name=f"{w.name}[{i}]", X=w.X[n0:n1], tets=(w.tets[t0:t1] - n0).astype(np.int32), facets=w.facets[:0],
                     fixed_nodes=fixed.astype(np.int32), kind=w.kind[t0:t1], E=w.E[t0:t1], nu=w.nu[t0:t1],
                     rho=w.rho[t0:t1], dt=w.dt, steps=w.steps, shapes=[w.shapes[i]], meta=dict(w.meta))
+
+
+def served_pool(n_obj=1024, seed=5, tet_lo=1000, tet_hi=500000, cell=0.05, ordering="color5"):
+    """Pool of independent boxes for the serving stream: tets log-uniform in [tet_lo, tet_hi],
+    aspect U(1, 6), corotational or neo-Hookean, E log-uniform [1e4, 1e6], nu U(0.3, 0.48),
+    the x = 0 face fixed, gravity.  Nodes are numbered by a 5-colouring by default (5 IC(0)
+    dependency levels per object)."""
+    objs = []
+    for i in range(n_obj):
+        s = seed * 100003 + i
+        nx, ny, nz, kind, E, nu = instance_shape(s, tet_lo, tet_hi)
+        m = M.box_mesh(nx, ny, nz, nx * cell, ny * cell, nz * cell, seed=s)
+        if ordering in ("color8", "color5"):
+            m = M.relabel(m, M.multicolor_permutation(m, 8 if ordering == "color8" else 5))
+        elif ordering != "natural":
+            raise ValueError(f"unknown ordering {ordering}")
+        fixed = M.plane_fixed_nodes(m, axis=0, value=0.0)
+        n = m.n_tets
+        objs.append(Workload(name=f"pool{seed}[{i}]", X=m.X, tets=m.tets, facets=m.facets, fixed_nodes=fixed,
+                             kind=np.full(n, kind, np.int32), E=np.full(n, E), nu=np.full(n, nu),
+                             rho=np.full(n, 1000.0), dt=1.0 / 60, steps=0, shapes=[m.shape]))
+    return objs
+
+
+def served_requests(pool, rate, duration, seed=6, max_steps=8, max_force=100.0, max_nodes=32):
+    """Poisson arrivals at `rate` requests/s over `duration` s.  Each request advances a
+    uniformly chosen object 1..max_steps steps (uniform) with an impulse of U(0, max_force) N
+    split over 1..max_nodes surface nodes (not fixed), direction uniform on the sphere.
+    Returns (obj, steps, arrival, impulses) with impulses[k] = [(node, (fx, fy, fz)), ...]."""
+    rng = np.random.default_rng(seed)
+    t, arrival = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        arrival.append(t)
+    k = len(arrival)
+    obj = rng.integers(0, len(pool), k).astype(np.int32)
+    steps = rng.integers(1, max_steps + 1, k).astype(np.int32)
+    surface = []
+    for w in pool:
+        s = np.setdiff1d(np.unique(w.facets.ravel()), w.fixed_nodes)
+        surface.append(s)
+    impulses = []
+    for q in range(k):
+        cand = surface[obj[q]]
+        nn = int(rng.integers(1, max_nodes + 1))
+        nodes = rng.choice(cand, size=min(nn, cand.size), replace=False)
+        mag = rng.uniform(0.0, max_force)
+        d = rng.standard_normal(3); d /= np.linalg.norm(d)
+        f = d * mag / nodes.size
+        impulses.append([(int(n), tuple(f)) for n in nodes])
+    return obj, steps, np.asarray(arrival), impulses
