@@ -218,7 +218,7 @@ def main():
         vb = 4 if precision == "mixed" else 8
         by = spmv_bytes(mesh, vb, vb)
         # Launches of a batch that finds its instance converged exit at once; their (small) time
-        # stays in the bracketed total, but only the iterations that ran count as SpMV work.
+        # stays in the timed total, but only the iterations that ran count as SpMV work.
         real_spmv = int(np.sum(iters))
         avg_spmv_s = (spmv_ms / max(real_spmv, 1)) / 1e3
         achieved = by / avg_spmv_s / 1e9 if real_spmv else 0.0
@@ -227,8 +227,10 @@ def main():
         return {"steps_s": args.steps / elapsed, "ms_step": elapsed / args.steps * 1e3,
                 "iters": float(np.mean(iters)), "relres_max": float(np.max(relres)),
                 "outer": float(np.mean(outer)),
-                "spmv": {"kernel": "k_spmv_dot<double>" if vb == 8 else "k_spmv_dot<float>", "launches": spmv_n,
-                         "iterations": real_spmv, "avg_us": round(avg_spmv_s * 1e6, 2),
+                "spmv": {"kernel": f"k_spmv_dot<{'double' if vb == 8 else 'float'}, {r64 if vb == 8 else r32}>",
+                         "launches": spmv_n, "iterations": real_spmv,
+                         "avg_us_per_launch": round(spmv_ms / max(spmv_n, 1) * 1e3, 2),
+                         "avg_us_per_iteration": round(avg_spmv_s * 1e6, 2),
                          "algorithmic_MB": round(by / 1e6, 2), "GBps": round(achieved, 1)},
                 "achieved": achieved}, d
 

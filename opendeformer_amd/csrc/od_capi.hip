@@ -388,8 +388,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
         pcg_alpha<T>(m, st, vals, p, q, w, e0, e1);
         pcg_alpha_finalize(m, st, w);
         if (ic) {
-            pcg_update_xr<T>(m, st, p, q, x, r, (const T *)nullptr, (T *)nullptr, w);
-            launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, Ul, r, z, sc, preDone);
+            pcg_update_xr_ic0<T>(m, st, mesh->topo.levelPtr, p, q, x, r, Uf, Ul, z, w, preDone);
             pcg_dot_rz<T>(m, st, r, z, w);
         } else {
             pcg_update_xr<T>(m, st, p, q, x, r, invDiag, z, w);
@@ -420,8 +419,10 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
             g->key = key;
         }
     }
+    // A skipped iteration (all kernels launched, each exiting at once) costs about as much as
+    // two polls, so after the first batch the batches stay at 2 iterations.
     int launched = 0;
-    int batch = expect > 8 ? expect - 2 : 4;
+    int batch = expect > 1 ? expect - 1 : 4;
     while (launched < maxIt) {
         if (count_not_done(mesh->ctx, doneFlags, m.nInst) == 0) break;
         const int k = std::min(batch, maxIt - launched);
@@ -433,7 +434,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
             for (int j = 0; j < k; j++) iteration();
             launched += k;
         }
-        batch = expect > 8 ? 3 : std::min(batch * 2, 32);
+        batch = expect > 1 ? 2 : std::min(batch * 2, 32);
     }
 }
 
@@ -661,6 +662,20 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.nodeSlot = mesh->upload(t.nodeSlot);
         m.rowCStart = mesh->upload(t.rowCStart);
         m.levelRows = mesh->upload(t.levelRows);
+        m.rowLevel = mesh->upload(t.level);
+        for (int l = 0; l < DevMesh::kMaxLevelFlags; l++) {
+            m.levelWideFwd[l] = m.levelWideBwd[l] = 0;
+            if (l >= t.nLevels) continue;
+            double nT = 0, nU = 0;
+            for (int q = t.levelPtr[l]; q < t.levelPtr[l + 1]; q++) {
+                const int I = t.levelRows[q];
+                nT += t.tPtr[I + 1] - t.tPtr[I];
+                nU += t.rowPtr[I + 1] - t.rowPtr[I] - 1;
+            }
+            const double cnt = std::max(1, t.levelPtr[l + 1] - t.levelPtr[l]);
+            m.levelWideFwd[l] = nT / cnt > 8.0;
+            m.levelWideBwd[l] = nU / cnt > 8.0;
+        }
         m.chunkRow0 = mesh->upload(t.chunkRow0);
         m.chunkRow1 = mesh->upload(t.chunkRow1);
         m.chunkInst = mesh->upload(t.chunkInst);

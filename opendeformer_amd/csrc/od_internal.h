@@ -46,6 +46,11 @@ struct DevMesh {
     int *blkCStart;                              // nBlocks+1
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
     int *levelRows;                              // nRows
+    int *rowLevel;                               // IC(0) dependency level of each row
+    // per level (host-side flags read at launch): rows average more than 8 blocks on the side
+    // the forward / backward sweep reads (run 16 lanes per row instead of 8)
+    static constexpr int kMaxLevelFlags = 64;
+    uint8_t levelWideFwd[kMaxLevelFlags], levelWideBwd[kMaxLevelFlags];
     int *chunkRow0, *chunkRow1, *chunkInst;      // nChunks
     int *chunkSorted;     // chunks in spatial order, cut into 8 runs of perRun (one run per XCD)
     int perRun;
@@ -116,6 +121,12 @@ void pcg_update_xr(const DevMesh &m, hipStream_t s, const T *p, const T *q, T *x
                    PcgWork &w);
 template <typename T>
 void pcg_dot_rz(const DevMesh &m, hipStream_t s, const T *r, const T *z, PcgWork &w);
+// IC(0)-preconditioned iteration: x += alpha p, r -= alpha q with the forward solve of the
+// level-0 rows fused in, then the remaining sweep levels (the backward solve of the last
+// level fused into its forward launch).  Same result as pcg_update_xr + launch_ic0_apply.
+template <typename T>
+void pcg_update_xr_ic0(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *p, const T *q, T *x,
+                       T *r, const T *U, const T *Ul, T *z, PcgWork &w, const uint8_t *skip);
 template <typename T>
 void pcg_update_p(const DevMesh &m, hipStream_t s, const T *z, T *p, PcgWork &w);
 template <typename T>

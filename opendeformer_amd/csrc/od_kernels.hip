@@ -400,6 +400,7 @@ __global__ __launch_bounds__(1024) void k_finalize(DevMesh m, int phase, const d
     const int inst = blockIdx.x;
     const int c0 = m.instChunkStart[inst], c1 = m.instChunkStart[inst + 1];
     double t[3] = {0, 0, 0};
+#pragma unroll 4
     for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
         t[0] += partials[c];
         t[1] += partials[m.nChunks + c];
@@ -911,70 +912,86 @@ __device__ __forceinline__ int xcd_wg(int nwg) {
 }
 static inline int xcd_grid(int nwg) { return 8 * ((nwg + 7) / 8); }
 
-// IC(0) sweeps, 8 lanes per row, one launch per level (rows of a level are independent).
-// forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K)
+// IC(0) sweeps, one launch per level (rows of a level are independent), L lanes per row.
+// In a colour-grouped numbering a row of the first colour has all its neighbours above it
+// and a row of the last colour all below, so in those levels one side holds up to 14 blocks
+// of a Kuhn grid: levels whose rows average more than 8 blocks on the swept side run 16
+// lanes per row, the others 8.
+constexpr int kIcThreads = 256;
+
+template <int L, typename T>
+__device__ __forceinline__ void sum_lanes(T &a0, T &a1, T &a2) {
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, L);
+        a1 += __shfl_xor(a1, o, L);
+        a2 += __shfl_xor(a2, o, L);
+    }
+}
+
+// y <- U_II^-1 y for the packed upper diagonal block D = [d0 d1 d2; . d3 d4; . . d5]
 template <typename T>
-__global__ __launch_bounds__(256) void k_ic0_fwd(DevMesh m, const int *__restrict__ rows, int count,
-                                                 const T *__restrict__ U, const T *__restrict__ Ul,
-                                                 const T *__restrict__ r, T *__restrict__ y,
-                                                 const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip) {
-    const int wg = xcd_wg((count + 31) >> 5);
-    const int t = wg * (blockDim.x >> 3) + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
-    if (t >= count) return;
-    const int I = rows[t];
+__device__ __forceinline__ void diag_bwd(const T *D, T &y0, T &y1, T &y2) {
+    y2 = y2 / D[5];
+    y1 = (y1 - D[4] * y2) / D[3];
+    y0 = (y0 - D[1] * y1 - D[2] * y2) / D[0];
+}
+
+// forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the
+// lower copy (contiguous per row I)
+template <typename T, int L>
+__global__ __launch_bounds__(kIcThreads) void k_ic0_fwd(DevMesh m, int p0, int count, const T *__restrict__ U,
+                                                        const T *__restrict__ Ul, const T *__restrict__ r,
+                                                        T *__restrict__ y, const InstScalars *__restrict__ sc,
+                                                        const uint8_t *__restrict__ skip, int lastLevel) {
+    constexpr int kRows = kIcThreads / L;
+    const int wg = xcd_wg((count + kRows - 1) / kRows);
+    const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
+    if (pl >= count) return;
+    const int I = m.levelRows[p0 + pl];
     const int inst = m.rowInst[I];
     if ((sc && sc[inst].done) || (skip && skip[inst])) return;
     T a0 = 0, a1 = 0, a2 = 0;
-    for (int k = m.tPtr[I] + lane8; k < m.tPtr[I + 1]; k += 8) {
-        const int K = m.tRow[k];
-        const T *W = Ul + 9 * k;
-        const T y0 = y[3 * K], y1 = y[3 * K + 1], y2 = y[3 * K + 2];
-        a0 += W[0] * y0 + W[3] * y1 + W[6] * y2;
-        a1 += W[1] * y0 + W[4] * y1 + W[7] * y2;
-        a2 += W[2] * y0 + W[5] * y1 + W[8] * y2;
+    for (int k = m.tPtr[I] + lane; k < m.tPtr[I + 1]; k += L) {
+        BlockX<T> b;
+        load_block<T>(Ul + 9 * k, y + 3 * m.tRow[k], b);
+        a0 += b.e[0] * b.x[0] + b.e[3] * b.x[1] + b.e[6] * b.x[2];
+        a1 += b.e[1] * b.x[0] + b.e[4] * b.x[1] + b.e[7] * b.x[2];
+        a2 += b.e[2] * b.x[0] + b.e[5] * b.x[1] + b.e[8] * b.x[2];
     }
-#pragma unroll
-    for (int o = 4; o > 0; o >>= 1) {
-        a0 += __shfl_xor(a0, o, 8);
-        a1 += __shfl_xor(a1, o, 8);
-        a2 += __shfl_xor(a2, o, 8);
-    }
-    if (lane8 != 0) return;
+    sum_lanes<L>(a0, a1, a2);
+    if (lane != 0) return;
     const T *D = U + m.valOff[I];
-    const T z0 = (r[3 * I] - a0) / D[0];
-    const T z1 = (r[3 * I + 1] - a1 - D[1] * z0) / D[3];
-    const T z2 = (r[3 * I + 2] - a2 - D[2] * z0 - D[4] * z1) / D[5];
+    T z0 = (r[3 * I] - a0) / D[0];
+    T z1 = (r[3 * I + 1] - a1 - D[1] * z0) / D[3];
+    T z2 = (r[3 * I + 2] - a2 - D[2] * z0 - D[4] * z1) / D[5];
+    if (lastLevel) diag_bwd(D, z0, z1, z2);   // rows of the last level have no blocks above them
     y[3 * I] = z0; y[3 * I + 1] = z1; y[3 * I + 2] = z2;
 }
 
 // backward: U z = y  ->  z_I = U_II^-1 (y_I - sum_{J>I} U_IJ z_J)
-template <typename T>
-__global__ __launch_bounds__(256) void k_ic0_bwd(DevMesh m, const int *__restrict__ rows, int count,
-                                                 const T *__restrict__ U, T *__restrict__ z,
-                                                 const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip) {
-    const int wg = xcd_wg((count + 31) >> 5);
-    const int t = wg * (blockDim.x >> 3) + (threadIdx.x >> 3), lane8 = threadIdx.x & 7;
-    if (t >= count) return;
-    const int I = rows[t];
+template <typename T, int L>
+__global__ __launch_bounds__(kIcThreads) void k_ic0_bwd(DevMesh m, int p0, int count, const T *__restrict__ U,
+                                                        T *__restrict__ z, const InstScalars *__restrict__ sc,
+                                                        const uint8_t *__restrict__ skip) {
+    constexpr int kRows = kIcThreads / L;
+    const int wg = xcd_wg((count + kRows - 1) / kRows);
+    const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
+    if (pl >= count) return;
+    const int I = m.levelRows[p0 + pl];
     const int inst = m.rowInst[I];
     if ((sc && sc[inst].done) || (skip && skip[inst])) return;
     const int v0 = m.valOff[I], b0 = m.rowPtr[I], nOff = m.rowPtr[I + 1] - b0 - 1;
     T a0 = 0, a1 = 0, a2 = 0;
-    for (int k = lane8; k < nOff; k += 8) {
-        const int J = m.colIdx[b0 + 1 + k];
-        const T *W = U + v0 + 6 + 9 * k;
-        const T z0 = z[3 * J], z1 = z[3 * J + 1], z2 = z[3 * J + 2];
-        a0 += W[0] * z0 + W[1] * z1 + W[2] * z2;
-        a1 += W[3] * z0 + W[4] * z1 + W[5] * z2;
-        a2 += W[6] * z0 + W[7] * z1 + W[8] * z2;
+    for (int k = lane; k < nOff; k += L) {
+        BlockX<T> b;
+        load_block<T>(U + v0 + 6 + 9 * k, z + 3 * m.colIdx[b0 + 1 + k], b);
+        a0 += b.e[0] * b.x[0] + b.e[1] * b.x[1] + b.e[2] * b.x[2];
+        a1 += b.e[3] * b.x[0] + b.e[4] * b.x[1] + b.e[5] * b.x[2];
+        a2 += b.e[6] * b.x[0] + b.e[7] * b.x[1] + b.e[8] * b.x[2];
     }
-#pragma unroll
-    for (int o = 4; o > 0; o >>= 1) {
-        a0 += __shfl_xor(a0, o, 8);
-        a1 += __shfl_xor(a1, o, 8);
-        a2 += __shfl_xor(a2, o, 8);
-    }
-    if (lane8 != 0) return;
+    sum_lanes<L>(a0, a1, a2);
+    if (lane != 0) return;
     const T *D = U + v0;
     const T x2 = (z[3 * I + 2] - a2) / D[5];
     const T x1 = (z[3 * I + 1] - a1 - D[4] * x2) / D[3];
@@ -982,23 +999,83 @@ __global__ __launch_bounds__(256) void k_ic0_bwd(DevMesh m, const int *__restric
     z[3 * I] = x0; z[3 * I + 1] = x1; z[3 * I + 2] = x2;
 }
 
+// Sweep launches for forward levels [f0, nl) (the last one also solving its backward step
+// when fuseLast) and backward levels [0, b1).
+template <typename T>
+static void ic0_levels(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, int f0, bool fuseLast, int b1,
+                       const T *U, const T *Ul, const T *r, T *z, const InstScalars *sc, const uint8_t *skip) {
+    const int nl = (int)levelPtr.size() - 1;
+    for (int l = f0; l < nl; l++) {
+        int cnt = levelPtr[l + 1] - levelPtr[l];
+        if (cnt <= 0) continue;
+        const int last = (fuseLast && l == nl - 1) ? 1 : 0;
+        if (m.levelWideFwd[l])
+            hipLaunchKernelGGL((k_ic0_fwd<T, 16>), dim3(xcd_grid((cnt + 15) / 16)), dim3(kIcThreads), 0, s, m,
+                               levelPtr[l], cnt, U, Ul, r, z, sc, skip, last);
+        else
+            hipLaunchKernelGGL((k_ic0_fwd<T, 8>), dim3(xcd_grid((cnt + 31) / 32)), dim3(kIcThreads), 0, s, m,
+                               levelPtr[l], cnt, U, Ul, r, z, sc, skip, last);
+    }
+    for (int l = b1 - 1; l >= 0; l--) {
+        int cnt = levelPtr[l + 1] - levelPtr[l];
+        if (cnt <= 0) continue;
+        if (m.levelWideBwd[l])
+            hipLaunchKernelGGL((k_ic0_bwd<T, 16>), dim3(xcd_grid((cnt + 15) / 16)), dim3(kIcThreads), 0, s, m,
+                               levelPtr[l], cnt, U, z, sc, skip);
+        else
+            hipLaunchKernelGGL((k_ic0_bwd<T, 8>), dim3(xcd_grid((cnt + 31) / 32)), dim3(kIcThreads), 0, s, m,
+                               levelPtr[l], cnt, U, z, sc, skip);
+    }
+    OD_HIP(hipGetLastError());
+}
+
 template <typename T>
 void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *U, const T *Ul,
                       const T *r, T *z, const InstScalars *sc, const uint8_t *skip) {
     const int nl = (int)levelPtr.size() - 1;
-    for (int l = 0; l < nl; l++) {
-        int cnt = levelPtr[l + 1] - levelPtr[l];
-        if (cnt <= 0) continue;
-        hipLaunchKernelGGL(k_ic0_fwd<T>, dim3(xcd_grid((cnt + 31) / 32)), dim3(256), 0, s, m, m.levelRows + levelPtr[l], cnt,
-                           U, Ul, r, z, sc, skip);
+    ic0_levels<T>(m, s, levelPtr, 0, false, nl, U, Ul, r, z, sc, skip);
+}
+
+// x += alpha p ; r -= alpha q ; slot [1] r.r ; then, for the rows of level 0 (no blocks below
+// them), the forward solve y = U_II^-T r (and the backward one when there is a single level)
+template <typename T>
+__global__ __launch_bounds__(kVecThreads) void k_update_xr_ic0(DevMesh m, const T *p, const T *q, T *x, T *r,
+                                                               const T *__restrict__ U, T *z, int singleLevel,
+                                                               double *partials, const InstScalars *sc) {
+    CHUNK_PROLOGUE
+    __shared__ T rs[kVecThreads];
+    const bool act = !sc[inst].done;
+    double v[1] = {0};
+    T ri = 0;
+    if (in && act) {
+        const T a = (T)sc[inst].alpha;
+        x[i] = x[i] + a * p[i];
+        ri = r[i] - a * q[i];
+        r[i] = ri;
+        v[0] = (double)ri * (double)ri;
     }
-    for (int l = nl - 1; l >= 0; l--) {
-        int cnt = levelPtr[l + 1] - levelPtr[l];
-        if (cnt <= 0) continue;
-        hipLaunchKernelGGL(k_ic0_bwd<T>, dim3(xcd_grid((cnt + 31) / 32)), dim3(256), 0, s, m, m.levelRows + levelPtr[l], cnt,
-                           U, z, sc, skip);
-    }
+    rs[threadIdx.x] = ri;
+    block_sum<1>(v);                       // its barrier also publishes rs
+    if (threadIdx.x == 0) partials[m.nChunks + chunk] = v[0];
+    const int row = m.chunkRow0[chunk] + (int)threadIdx.x;
+    if (!act || threadIdx.x >= kChunkRows || row >= m.chunkRow1[chunk] || m.rowLevel[row] != 0) return;
+    const T *D = U + m.valOff[row];
+    const int b = 3 * (int)threadIdx.x;
+    T y0 = rs[b] / D[0];
+    T y1 = (rs[b + 1] - D[1] * y0) / D[3];
+    T y2 = (rs[b + 2] - D[2] * y0 - D[4] * y1) / D[5];
+    if (singleLevel) diag_bwd(D, y0, y1, y2);
+    z[3 * row] = y0; z[3 * row + 1] = y1; z[3 * row + 2] = y2;
+}
+
+template <typename T>
+void pcg_update_xr_ic0(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *p, const T *q, T *x,
+                       T *r, const T *U, const T *Ul, T *z, PcgWork &w, const uint8_t *skip) {
+    const int nl = (int)levelPtr.size() - 1;
+    hipLaunchKernelGGL(k_update_xr_ic0<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, p, q, x, r, U, z,
+                       nl == 1 ? 1 : 0, w.partials, w.sc);
     OD_HIP(hipGetLastError());
+    if (nl > 1) ic0_levels<T>(m, s, levelPtr, 1, true, nl - 1, U, Ul, r, z, w.sc, skip);
 }
 
 // =====================================================================================
@@ -1100,6 +1177,8 @@ void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDo
     template void pcg_update_xr<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, T *, const T *, T *,      \
                                    PcgWork &);                                                                        \
     template void pcg_dot_rz<T>(const DevMesh &, hipStream_t, const T *, const T *, PcgWork &);                       \
+    template void pcg_update_xr_ic0<T>(const DevMesh &, hipStream_t, const std::vector<int> &, const T *, const T *,   \
+                                       T *, T *, const T *, const T *, T *, PcgWork &, const uint8_t *);              \
     template void pcg_update_p<T>(const DevMesh &, hipStream_t, const T *, T *, PcgWork &);                           \
     template void launch_jacobi_diag<T>(const DevMesh &, hipStream_t, const double *, T *);                           \
     template void launch_ic0_apply<T>(const DevMesh &, hipStream_t, const std::vector<int> &, const T *, const T *,   \
