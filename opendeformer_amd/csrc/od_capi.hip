@@ -832,6 +832,8 @@ int od_mesh_set_material(od_mesh *mesh, const int *kind, const double *E, const 
     OD_HIP(hipStreamSynchronize(st));
     mesh->dm.thrCorot = corotThreshold;
     mesh->dm.thrNeo = neoThreshold;
+    mesh->dm.hasKind[0] = mesh->dm.hasKind[1] = 0;
+    for (int i = 0; i < n; i++) mesh->dm.hasKind[kind[i]] = 1;
     mesh->hasMaterial = true;
     API_END
 }

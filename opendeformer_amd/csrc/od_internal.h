@@ -61,6 +61,7 @@ struct DevMesh {
     int *tetInst, *rowInst;
     // material (per tet)
     int *kind; double *E, *nu, *rho;
+    uint8_t hasKind[2];   // host-side: some tet is corotational / neo-Hookean
     double thrCorot, thrNeo;
 };
 

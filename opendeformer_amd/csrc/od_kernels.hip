@@ -37,12 +37,17 @@ __device__ __forceinline__ void store_block(double *dst, const double *m, int tr
     }
 }
 
-__global__ __launch_bounds__(128) void k_element(DevMesh m, const double *__restrict__ d, const double *__restrict__ v,
-                                                 const double *__restrict__ a, ElemParams P, double *__restrict__ blkC,
-                                                 double *__restrict__ nodeC, const uint8_t *__restrict__ instActive) {
+// One kernel per material kind (KIND 0 corotational, 1 neo-Hookean) and one for the mass
+// blocks (KIND 2): each compiles to its own register budget, and the budget allows two waves
+// per SIMD.  A launch skips the tets of other kinds.
+template <int KIND>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_element(
+    DevMesh m, const double *__restrict__ d, const double *__restrict__ v, const double *__restrict__ a, ElemParams P,
+    double *__restrict__ blkC, double *__restrict__ nodeC, const uint8_t *__restrict__ instActive) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m.nTets) return;
     if (instActive && !instActive[m.tetInst[e]]) return;
+    if (KIND != 2 && m.kind[e] != KIND) return;
     int tn[4], row[4];
     v3 X[4], x[4], y[4], z[4];
 #pragma unroll
@@ -67,7 +72,7 @@ __global__ __launch_bounds__(128) void k_element(DevMesh m, const double *__rest
 #pragma unroll
     for (int k = 0; k < 4; k++) g[k] = mk(0, 0, 0);
 
-    if (P.mode == 1) {  // consistent mass blocks rho*V/10 (diagonal pair) and rho*V/20 (tetelement.cpp:51-66)
+    if (KIND == 2) {  // consistent mass blocks rho*V/10 (diagonal pair) and rho*V/20 (tetelement.cpp:51-66)
         int p = 0;
         for (int aI = 0; aI < 4; aI++)
             for (int bI = aI; bI < 4; bI++, p++) {
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(128) void k_element(DevMesh m, const double *__rest
     const double lambda = nu * En / ((1.0 + nu) * (1.0 - 2.0 * nu));
     const double mu = En / (2.0 * (1.0 + nu));
 
-    if (m.kind[e] == 0) {
+    if (KIND == 0) {
         // ---- corotational: R from polar (or QR when compressed), blocks R K0_ab R^T
         double R[9];
         const double Vc = tet_volume(x[0], x[1], x[2], x[3]);
@@ -225,7 +230,14 @@ __global__ __launch_bounds__(128) void k_element(DevMesh m, const double *__rest
 void launch_element(const DevMesh &m, hipStream_t s, const double *d, const double *v, const double *a,
                     const ElemParams &p, double *blkC, double *nodeC, const uint8_t *instActive) {
     int grid = (m.nTets + 127) / 128;
-    hipLaunchKernelGGL(k_element, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
+    if (p.mode == 1) {
+        hipLaunchKernelGGL(k_element<2>, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
+    } else {
+        if (m.hasKind[0])
+            hipLaunchKernelGGL(k_element<0>, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
+        if (m.hasKind[1])
+            hipLaunchKernelGGL(k_element<1>, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
+    }
     OD_HIP(hipGetLastError());
 }
 
