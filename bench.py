@@ -69,6 +69,45 @@ def copy_calibration(mesh):
           f"{8 * mesh.n_values} bytes written per launch", flush=True)
 
 
+def mid_neo_leg(args, steps=200):
+    """The mid-size precision-form workload: 80x16x16 box, 2.0x0.4x0.4 m, 122,880 tets,
+    compressible neo-Hookean E = 5e4, nu = 0.40, x = 0 face fixed, gravity plus a seeded tip
+    impulse of U(0, 50) N at step 0, dt = 1/120, 200 steps, IC(0)-PCG to 1e-6 in FP64 and in
+    the mixed form from the same start; the mixed form's per-step displacement error against
+    the FP64 path is reported."""
+    import opendeformer_amd as od
+    from opendeformer_amd import capi
+    from opendeformer_amd import workloads as W
+    w = W.box_mid_neo("color5", seed=2)
+    ctx = od.Context(0)
+    mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
+    mesh.set_material(w.kind, w.E, w.nu, w.rho)
+    f = mesh.body_force(w.body)
+    out, traj = {}, {}
+    for name, prec in (("fp64", capi.OD_FP64), ("mixed", capi.OD_MIXED)):
+        nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=prec, tol=1e-6)
+        nm.set_external_virtual_work(f)
+        nm.set_impulse(w.impulse)
+        ds, its = [], []
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            nm.run_one_time_step()
+            its.append(int(nm.stats()[0][0]))
+            if k % 20 == 19:
+                ds.append(nm.state()[0])
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        out[name] = {"steps_per_s": round(steps / dt, 2), "pcg_iters_per_step": round(float(np.mean(its)), 2)}
+        traj[name] = ds
+        del nm
+    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(traj["mixed"], traj["fp64"])]
+    return {"workload": f"{w.name}: {mesh.n_tets} tets, {mesh.ndof} DOF, tip impulse, dt=1/120, {steps} steps "
+                        f"(5-colour ordering; the timing includes a state read-back every 20 steps)",
+            "fp64": out["fp64"], "mixed": out["mixed"],
+            "mixed_rel_err_vs_fp64_max_over_sampled_steps": max(errs)}
+
+
 def batched_leg(args, n_inst=256, warmup=2, steps=20):
     """256 uneven boxes (tets log-uniform [500, 5e4], aspect U(1, 6), corotational or
     neo-Hookean, E log-uniform [1e4, 1e6], nu U(0.3, 0.48)) concatenated into one system and
@@ -95,12 +134,35 @@ def batched_leg(args, n_inst=256, warmup=2, steps=20):
     ctx.sync()
     dt = (time.perf_counter() - t0) / steps
     its = np.array(its)
+    del nm
+    # the plain FP64 drop-in path on the same instances, one at a time (every 16th instance,
+    # 10 steps each after 2 warm-up steps): instance-steps/s of stepping them alone
+    alone_t, alone_n = 0.0, 0
+    for i in range(0, n_inst, 16):
+        wi = W.subset(w, i)
+        mi = od.Mesh(ctx, wi.X, wi.tets, wi.fixed_dofs)
+        mi.set_material(wi.kind, wi.E, wi.nu, wi.rho)
+        od.mesh_spmv_variant(mi, r64, r32)
+        ni = od.NewmarkIntegrator(mi, wi.dt, precond=capi.OD_PRE_IC0, precision=capi.OD_FP64, tol=1e-6)
+        ni.set_external_virtual_work(mi.body_force(wi.body))
+        ni.run_time_steps(2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ni.run_time_steps(10)
+        ctx.sync()
+        alone_t += time.perf_counter() - t0
+        alone_n += 10
+        del ni, mi
+    alone_rate = alone_n / alone_t
     return {"workload": f"{n_inst} boxes, {mesh.n_tets} tets, {mesh.ndof} DOF in one concatenated system "
                         f"(5-colour ordering per box), mixed precision, graph replay",
             "ms_per_step": round(dt * 1e3, 3), "batch_steps_per_s": round(1.0 / dt, 3),
             "instance_steps_per_s": round(n_inst / dt, 1), "steps": steps, "warmup": warmup,
             "pcg_iters_per_instance_step": {"median": float(np.median(its)), "max": int(its.max()),
                                             "min": int(its.min())},
+            "plain_fp64_alone_instance_steps_per_s": round(alone_rate, 1),
+            "plain_fp64_alone_sample": f"instances 0, 16, ..., {n_inst - 16}: 10 steps each, one at a time",
+            "speedup_vs_plain_fp64_alone": round((n_inst / dt) / alone_rate, 2),
             "spmv_chunks_per_workgroup": {"fp64": r64, "fp32": r32}}
 
 
@@ -268,6 +330,7 @@ def main():
         line["speedup_vs_fp64"] = round(main_r["steps_s"] / fp64_r["steps_s"], 3)
         line["rel_err_vs_fp64_path"] = {"value": err, "at": f"displacement after step {args.warmup + args.steps}"}
     if not args.no_batch:
+        line["mid_neo_precision"] = mid_neo_leg(args)
         line["batched"] = batched_leg(args)
     if not args.no_serving:
         line["serving"] = serving_leg(args)

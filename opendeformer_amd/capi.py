@@ -17,6 +17,7 @@ _LIB = None
 OD_MAT_COROTATIONAL, OD_MAT_NEOHOOKEAN = 0, 1
 OD_PRE_NONE, OD_PRE_JACOBI, OD_PRE_IC0 = 0, 1, 3
 OD_FP64, OD_MIXED = 0, 1
+OD_OK, OD_ERR_ARG, OD_ERR_HIP, OD_ERR_STATE, OD_ERR_UNSUPPORTED = 0, -1, -2, -3, -4
 
 # every symbol include/od.h declares
 EXPORTS = [

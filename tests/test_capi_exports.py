@@ -47,3 +47,14 @@ def test_host_header_compiles_against_the_abi():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
                         "-I" + os.path.join(ROOT, "include"), src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_server_rejects_bad_parameters_before_touching_a_device():
+    import ctypes
+    lib = capi.lib()
+    h = ctypes.c_void_p()
+    # zero budget, too many slots, unknown precision: argument errors, no device call made
+    assert lib.od_server_create(0, 0, 4, capi.OD_MIXED, capi.OD_PRE_IC0, 1e-6, ctypes.byref(h)) == capi.OD_ERR_ARG
+    assert lib.od_server_create(0, 1 << 30, 64, capi.OD_MIXED, capi.OD_PRE_IC0, 1e-6, ctypes.byref(h)) == capi.OD_ERR_ARG
+    assert lib.od_server_create(0, 1 << 30, 4, 7, capi.OD_PRE_IC0, 1e-6, ctypes.byref(h)) == capi.OD_ERR_UNSUPPORTED
+    assert b"server" in lib.od_last_error() or b"precision" in lib.od_last_error()

@@ -191,6 +191,15 @@ def test_newmark_neohookean_with_impulse(ctx):
     assert errs.max() < NEWMARK_RTOL, errs.max()
 
 
+def test_mixed_precision_neohookean_with_impulse_vs_fp64_oracle(ctx):
+    """Precision form on the neo-Hookean workload shape (impulse at step 0): stated bound 1e-5
+    relative per-step displacement against the FP64 oracle."""
+    m = M.box_mesh(24, 5, 5, 0.6, 0.125, 0.125, seed=2)
+    w = W._single("neo_small", m, 1, 5e4, 0.40, 1.0 / 120, 30, "color5", impulse=W.tip_impulse(m, 2))
+    errs, _ = run_pair(ctx, w, w.steps, precision=capi.OD_MIXED, impulse=w.impulse)
+    assert errs.max() < 1e-5, errs.max()
+
+
 def test_mixed_precision_error_vs_fp64_oracle(ctx):
     """FP32 inner PCG + FP64 refinement: stated bound 1e-5 relative per step on this workload."""
     w = W.box_small("color8")
