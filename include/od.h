@@ -37,7 +37,7 @@ enum {
 };
 
 /* material kinds per tet */
-enum { OD_MAT_COROTATIONAL = 0, OD_MAT_NEOHOOKEAN = 1 };
+enum { OD_MAT_COROTATIONAL = 0, OD_MAT_NEOHOOKEAN = 1, OD_MAT_COROT_PLASTIC = 2 };
 /* preconditioners (core/preconditioner.h:12-22 family) */
 enum { OD_PRE_NONE = 0, OD_PRE_JACOBI = 1, OD_PRE_IC0 = 3 };
 /* arithmetic of the inner PCG iterations */
@@ -111,6 +111,18 @@ int od_mesh_rows(od_mesh *mesh, int *instRowStart_host, int *level_host);
  * neoThreshold the singular-value clamp (material/invertibleHyperelastic.cpp:239-240). */
 int od_mesh_set_material(od_mesh *mesh, const int *kind_host, const double *E_host, const double *nu_host,
                          const double *rho_host, double corotThreshold, double neoThreshold);
+
+/* Corotational plastic tets (OD_MAT_COROT_PLASTIC).  Replaces the constructor
+ * parameters of CorotationalPlasticMaterial (material/corotationalPlastic.cpp:8-21):
+ * per-tet yield criterion, creep ratio and maximum plastic strain; the plastic
+ * strain state (9 per tet, corotationalPlastic.cpp:45-46) restarts at zero.  Every
+ * stiffness and force evaluation (od_assemble, each integrator step) updates it,
+ * as generateMatrixAndVirtualWorks does (corotationalPlastic.cpp:57-114, 124-183).
+ * Plastic tets without parameters never yield. */
+int od_mesh_set_plastic(od_mesh *mesh, const double *yield_host, const double *creep_host,
+                        const double *maxStrain_host);
+/* reads (get_host) and/or writes (set_host) the plastic strain state, 9 per tet */
+int od_mesh_plastic_strain(od_mesh *mesh, double *get_host, const double *set_host);
 
 /* Free-DOF body-force virtual works (Forcer::addBodyForce + getVirtualWorks,
  * core/forcer.cpp:8-33, 80-96): f_dev = sum over tets of V/4 * body. */

@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libodhip.so")
 _LIB = None
 
-OD_MAT_COROTATIONAL, OD_MAT_NEOHOOKEAN = 0, 1
+OD_MAT_COROTATIONAL, OD_MAT_NEOHOOKEAN, OD_MAT_COROT_PLASTIC = 0, 1, 2
 OD_PRE_NONE, OD_PRE_JACOBI, OD_PRE_IC0 = 0, 1, 3
 OD_FP64, OD_MIXED = 0, 1
 OD_OK, OD_ERR_ARG, OD_ERR_HIP, OD_ERR_STATE, OD_ERR_UNSUPPORTED = 0, -1, -2, -3, -4
@@ -34,7 +34,7 @@ EXPORTS = [
     "od_solver_set_refinement", "od_integrator_set_refinement", "od_mesh_autotune", "od_mesh_spmv_variant",
     "od_solver_set_graph", "od_integrator_set_graph", "od_mesh_bind_ctx", "od_integrator_device_bytes",
     "od_server_create", "od_server_destroy", "od_server_add_object", "od_server_run", "od_server_object_state",
-    "od_server_object_info",
+    "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain",
 ]
 
 
@@ -73,6 +73,7 @@ def lib():
             "od_server_add_object": [vp, i, dp, i, ip, i, ip, ip, dp, dp, dp, dp, d, ip],
             "od_server_run": [vp, i, ip, ip, dp, ip, ip, dp, dp, dp, ip, C.POINTER(ll)],
             "od_server_object_state": [vp, i, dp, dp, dp], "od_server_object_info": [vp, i, C.POINTER(ll)],
+            "od_mesh_set_plastic": [vp, dp, dp, dp], "od_mesh_plastic_strain": [vp, dp, dp],
             "od_pcg_solve": [vp, vp, vp, vp, d, i, ip, dp], "od_precond_apply": [vp, vp, vp, vp],
             "od_ic0_factor": [vp, vp, dp],
             "od_newmark_create": [vp, d, d, d, d, d, i, d, d, i, i, i, pvp],

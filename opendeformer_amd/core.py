@@ -139,6 +139,18 @@ class Mesh:
         check(lib().od_mesh_set_material(self.h, capi.iptr(kind), dptr(E), dptr(nu), dptr(rho), corot_threshold,
                                          neo_threshold))
 
+    def set_plastic(self, yield_criterion, creep, max_strain):
+        """Corotational plastic parameters per tet (kind OD_MAT_COROT_PLASTIC); resets the state."""
+        n = self.n_tets
+        y = f64(np.broadcast_to(yield_criterion, (n,))); c = f64(np.broadcast_to(creep, (n,)))
+        mx = f64(np.broadcast_to(max_strain, (n,)))
+        check(lib().od_mesh_set_plastic(self.h, dptr(y), dptr(c), dptr(mx)))
+
+    def plastic_strain(self):
+        out = np.zeros(9 * self.n_tets)
+        check(lib().od_mesh_plastic_strain(self.h, dptr(out), None))
+        return out.reshape(-1, 3, 3)
+
     def body_force(self, body):
         body = f64(np.broadcast_to(np.asarray(body, np.float64).reshape(-1, 3), (self.n_tets, 3)))
         f = DeviceArray(self.ctx, self.ndof)

@@ -783,6 +783,40 @@ int od_pattern_export(od_mesh *mesh, int *blockPcol, int *blockColumnOris, int *
     API_END
 }
 
+int od_mesh_set_plastic(od_mesh *mesh, const double *yield, const double *creep, const double *maxStrain) {
+    API_BEGIN
+    require(mesh && yield && creep && maxStrain, "NULL argument");
+    const int n = mesh->topo.nTets;
+    for (int i = 0; i < n; i++)
+        require(yield[i] >= 0 && creep[i] >= 0 && maxStrain[i] >= 0, "plastic parameters must be non-negative");
+    hipStream_t st = mesh->ctx->stream;
+    DevMesh &m = mesh->dm;
+    if (!m.plStrain) {
+        m.plYield = (double *)mesh->alloc(sizeof(double) * n);
+        m.plCreep = (double *)mesh->alloc(sizeof(double) * n);
+        m.plMax = (double *)mesh->alloc(sizeof(double) * n);
+        m.plStrain = (double *)mesh->alloc(sizeof(double) * 9 * n);
+    }
+    OD_HIP(hipMemcpyAsync(m.plYield, yield, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    OD_HIP(hipMemcpyAsync(m.plCreep, creep, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    OD_HIP(hipMemcpyAsync(m.plMax, maxStrain, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    OD_HIP(hipMemsetAsync(m.plStrain, 0, sizeof(double) * 9 * n, st));
+    OD_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+int od_mesh_plastic_strain(od_mesh *mesh, double *get, const double *set) {
+    API_BEGIN
+    require(mesh != nullptr, "NULL argument");
+    require(mesh->dm.plStrain != nullptr, "the mesh has no plastic tets", OD_ERR_STATE);
+    const size_t b = sizeof(double) * 9 * (size_t)mesh->topo.nTets;
+    hipStream_t st = mesh->ctx->stream;
+    if (get) OD_HIP(hipMemcpyAsync(get, mesh->dm.plStrain, b, hipMemcpyDeviceToHost, st));
+    if (set) OD_HIP(hipMemcpyAsync(mesh->dm.plStrain, set, b, hipMemcpyHostToDevice, st));
+    OD_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
 int od_mesh_spmv_variant(od_mesh *mesh, int r64, int r32) {
     API_BEGIN
     require(mesh != nullptr, "NULL argument");
@@ -821,7 +855,8 @@ int od_mesh_set_material(od_mesh *mesh, const int *kind, const double *E, const 
     require(mesh && kind && E && nu && rho, "NULL argument");
     const int n = mesh->topo.nTets;
     for (int i = 0; i < n; i++) {
-        require(kind[i] == OD_MAT_COROTATIONAL || kind[i] == OD_MAT_NEOHOOKEAN, "unknown material kind", OD_ERR_UNSUPPORTED);
+        require(kind[i] == OD_MAT_COROTATIONAL || kind[i] == OD_MAT_NEOHOOKEAN || kind[i] == OD_MAT_COROT_PLASTIC,
+                "unknown material kind", OD_ERR_UNSUPPORTED);
         require(E[i] > 0 && nu[i] > -1.0 && nu[i] < 0.5 && rho[i] >= 0, "material parameters out of range");
     }
     hipStream_t st = mesh->ctx->stream;
@@ -832,8 +867,18 @@ int od_mesh_set_material(od_mesh *mesh, const int *kind, const double *E, const 
     OD_HIP(hipStreamSynchronize(st));
     mesh->dm.thrCorot = corotThreshold;
     mesh->dm.thrNeo = neoThreshold;
-    mesh->dm.hasKind[0] = mesh->dm.hasKind[1] = 0;
+    mesh->dm.hasKind[0] = mesh->dm.hasKind[1] = mesh->dm.hasKind[2] = 0;
     for (int i = 0; i < n; i++) mesh->dm.hasKind[kind[i]] = 1;
+    if (mesh->dm.hasKind[2] && !mesh->dm.plStrain) {
+        // plastic tets before od_mesh_set_plastic: never yielding (the corotational response)
+        std::vector<double> inf(n, HUGE_VAL), zero(n, 0.0);
+        mesh->dm.plYield = mesh->upload(inf);
+        mesh->dm.plCreep = mesh->upload(zero);
+        mesh->dm.plMax = mesh->upload(inf);
+        mesh->dm.plStrain = (double *)mesh->alloc(sizeof(double) * 9 * n);
+        OD_HIP(hipMemsetAsync(mesh->dm.plStrain, 0, sizeof(double) * 9 * n, st));
+        OD_HIP(hipStreamSynchronize(st));
+    }
     mesh->hasMaterial = true;
     API_END
 }

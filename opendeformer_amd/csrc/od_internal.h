@@ -61,7 +61,10 @@ struct DevMesh {
     int *tetInst, *rowInst;
     // material (per tet)
     int *kind; double *E, *nu, *rho;
-    uint8_t hasKind[2];   // host-side: some tet is corotational / neo-Hookean
+    uint8_t hasKind[3];   // host-side: some tet is corotational / neo-Hookean / corotational plastic
+    // corotational plastic: per-tet yield criterion, creep ratio, maximum plastic strain, and the
+    // plastic strain state (9 per tet), updated by every stiffness and force evaluation
+    double *plYield, *plCreep, *plMax, *plStrain;
     double thrCorot, thrNeo;
 };
 

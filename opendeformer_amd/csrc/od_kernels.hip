@@ -37,9 +37,51 @@ __device__ __forceinline__ void store_block(double *dst, const double *m, int tr
     }
 }
 
-// One kernel per material kind (KIND 0 corotational, 1 neo-Hookean) and one for the mass
-// blocks (KIND 2): each compiles to its own register budget, and the budget allows two waves
-// per SIMD.  A launch skips the tets of other kinds.
+// Corotational plastic stress of tet e (corotationalPlastic.cpp:124-183): elastic strain
+// sym(S) - I - plastic strain with S = R^T F, stress R (lambda tr(e) I + 2 mu e); then the
+// plastic strain creeps towards the elastic strain when its norm exceeds the yield criterion
+// and is capped at the maximum.  Both norms sum e[i]^2 over i, j < 3 with only i indexing the
+// flattened strain, 3 (e00^2 + e01^2 + e02^2), as the reference's loops (:159, :172).
+__device__ __forceinline__ void plastic_stress(const DevMesh &m, int e, const double *F, const double *R,
+                                               double lambda, double mu, double *P) {
+    double *ep = m.plStrain + 9 * (size_t)e;
+    double S[9], st[9], ee[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S[i * 3 + j] = R[0 * 3 + i] * F[0 * 3 + j] + R[1 * 3 + i] * F[1 * 3 + j] + R[2 * 3 + i] * F[2 * 3 + j];
+    double pl[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) pl[q] = ep[q];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) ee[i * 3 + j] = 0.5 * (S[i * 3 + j] + S[j * 3 + i]) - pl[i * 3 + j];
+    ee[0] -= 1.0; ee[4] -= 1.0; ee[8] -= 1.0;
+    const double mu2 = 2.0 * mu, dg = lambda * (ee[0] + ee[4] + ee[8]);
+#pragma unroll
+    for (int q = 0; q < 9; q++) st[q] = mu2 * ee[q];
+    st[0] += dg; st[4] += dg; st[8] += dg;
+    mul3(R, st, P);
+    const double en = sqrt(3.0 * (ee[0] * ee[0] + ee[1] * ee[1] + ee[2] * ee[2]));
+    if (en > m.plYield[e]) {
+        const double ratio = m.plCreep[e] * (en - m.plYield[e]) / en;
+#pragma unroll
+        for (int q = 0; q < 9; q++) pl[q] += ratio * ee[q];
+        const double pn = sqrt(3.0 * (pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]));
+        if (pn > m.plMax[e]) {
+            const double f = m.plMax[e] / pn;
+#pragma unroll
+            for (int q = 0; q < 9; q++) pl[q] *= f;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) ep[q] = pl[q];
+    }
+}
+
+// One kernel per material kind (KIND 0 corotational, 1 neo-Hookean, 2 corotational plastic)
+// and one for the mass blocks (KIND 3): each compiles to its own register budget, and the
+// budget allows two waves per SIMD.  A launch skips the tets of other kinds.
 template <int KIND>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k_element(
     DevMesh m, const double *__restrict__ d, const double *__restrict__ v, const double *__restrict__ a, ElemParams P,
@@ -47,7 +89,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m.nTets) return;
     if (instActive && !instActive[m.tetInst[e]]) return;
-    if (KIND != 2 && m.kind[e] != KIND) return;
+    if (KIND != 3 && m.kind[e] != KIND) return;
     int tn[4], row[4];
     v3 X[4], x[4], y[4], z[4];
 #pragma unroll
@@ -72,7 +114,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int k = 0; k < 4; k++) g[k] = mk(0, 0, 0);
 
-    if (KIND == 2) {  // consistent mass blocks rho*V/10 (diagonal pair) and rho*V/20 (tetelement.cpp:51-66)
+    if (KIND == 3) {  // consistent mass blocks rho*V/10 (diagonal pair) and rho*V/20 (tetelement.cpp:51-66)
         int p = 0;
         for (int aI = 0; aI < 4; aI++)
             for (int bI = aI; bI < 4; bI++, p++) {
@@ -111,13 +153,15 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
     const double lambda = nu * En / ((1.0 + nu) * (1.0 - 2.0 * nu));
     const double mu = En / (2.0 * (1.0 + nu));
 
-    if (KIND == 0) {
+    if (KIND == 0 || KIND == 2) {
         // ---- corotational: R from polar (or QR when compressed), blocks R K0_ab R^T
         double R[9];
         const double Vc = tet_volume(x[0], x[1], x[2], x[3]);
         if (Vc >= m.thrCorot * V0) polar3(F, R);
         else qr3(F, R);
         const double D0 = lambda + 2.0 * mu, D1 = lambda, D2 = mu;
+        double Pst[9];
+        if (KIND == 2) plastic_stress(m, e, F, R, lambda, mu, Pst);
         int p = 0;
         for (int aI = 0; aI < 4; aI++) {
             for (int bI = aI; bI < 4; bI++, p++) {
@@ -142,6 +186,16 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
                 int slot = m.pairSlot[10 * e + p];
                 if (slot >= 0) store_block(blkC + (size_t)slot * 9, KK, m.pairT[10 * e + p]);
                 if (!P.nodeTerms) continue;
+                if (KIND == 2) {   // the K_e y term; the forces come from the elastic stress below
+                    g[aI] = g[aI] + mulv(KK, y[bI]);
+                    if (bI != aI) {
+                        v3 ya = y[aI];
+                        g[bI] = g[bI] + mk(KK[0] * ya.x + KK[3] * ya.y + KK[6] * ya.z,
+                                           KK[1] * ya.x + KK[4] * ya.y + KK[7] * ya.z,
+                                           KK[2] * ya.x + KK[5] * ya.y + KK[8] * ya.z);
+                    }
+                    continue;
+                }
                 // corotational force (tetelement.cpp:622-647) plus the K_e y term
                 v3 xb = x[bI] + y[bI];
                 g[aI] = g[aI] + mulv(KK, xb) - mulv(QK, X[bI]);
@@ -159,6 +213,18 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
                     g[bI] = g[bI] + kt - mulv(QKT, X[aI]);
                 }
             }
+        }
+        if (KIND == 2 && P.nodeTerms) {
+            // nodal forces from the elastic stress, f_i = P dN_i / 6, f_3 = -(f_0 + f_1 + f_2)
+            // (corotationalPlastic.cpp:107-112, tetelement.cpp:715-729)
+            const double f6 = 1.0 / 6.0;
+            v3 f[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) f[i] = mulv(Pst, dN[i]) * f6;
+            g[0] = g[0] + f[0];
+            g[1] = g[1] + f[1];
+            g[2] = g[2] + f[2];
+            g[3] = g[3] - (f[0] + f[1] + f[2]);
         }
     } else {
         // ---- invertible neo-Hookean: P and the PSD-projected dP/dF in the diagonal frame
@@ -231,12 +297,14 @@ void launch_element(const DevMesh &m, hipStream_t s, const double *d, const doub
                     const ElemParams &p, double *blkC, double *nodeC, const uint8_t *instActive) {
     int grid = (m.nTets + 127) / 128;
     if (p.mode == 1) {
-        hipLaunchKernelGGL(k_element<2>, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
+        hipLaunchKernelGGL(k_element<3>, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
     } else {
         if (m.hasKind[0])
             hipLaunchKernelGGL(k_element<0>, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
         if (m.hasKind[1])
             hipLaunchKernelGGL(k_element<1>, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
+        if (m.hasKind[2])
+            hipLaunchKernelGGL(k_element<2>, dim3(grid), dim3(128), 0, s, m, d, v, a, p, blkC, nodeC, instActive);
     }
     OD_HIP(hipGetLastError());
 }

@@ -44,6 +44,8 @@ def lib():
         L.oro_body_force.argtypes = [vp, dp, dp]
         L.oro_nodal_force.argtypes = [vp, dp, i, ip, dp]
         L.oro_assemble.argtypes = [vp, ip, dp, dp, d, d, dp, dp]
+        L.oro_set_plastic.argtypes = [vp, dp, dp, dp]
+        L.oro_get_plastic_strain.argtypes = [vp, dp]
         L.oro_spmdv.argtypes = [vp, dp, dp, dp]
         L.oro_get_diagonal.argtypes = [vp, dp, dp]
         L.oro_cg.argtypes = [vp, dp, dp, dp, d, i, i, d, d, dp]
@@ -159,6 +161,18 @@ class System:
         u = np.zeros(3 * self.n_nodes)
         lib().oro_get_node_displacements(self.h, _d(u))
         return u.reshape(-1, 3)
+
+    def set_plastic(self, yield_criterion, creep, max_strain):
+        """Corotational plastic parameters per tet (kind 2); resets the plastic strain."""
+        n = self.n_tets
+        y = f64(np.broadcast_to(yield_criterion, (n,))); c = f64(np.broadcast_to(creep, (n,)))
+        mx = f64(np.broadcast_to(max_strain, (n,)))
+        lib().oro_set_plastic(self.h, _d(y), _d(c), _d(mx))
+
+    def plastic_strain(self):
+        out = np.zeros(9 * self.n_tets)
+        lib().oro_get_plastic_strain(self.h, _d(out))
+        return out.reshape(-1, 3, 3)
 
     def assemble(self, kind, E, nu, thr_corot=0.0, thr_neo=0.1):
         kind = i32(np.broadcast_to(kind, (self.n_tets,)))

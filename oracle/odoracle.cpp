@@ -11,6 +11,7 @@
 // form.  Functions no such check covers are marked "parity unpinned" below.
 #include "oro_numer.h"
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -391,6 +392,10 @@ struct System {
     int ndof = 0;
     Pattern pat;
     std::vector<int> elemMap;     // 78 per tet (sparseMatrix.cpp:97-118 storage)
+    // corotational plastic material (kind 2): per-tet yield criterion, creep ratio and maximum
+    // plastic strain, and the plastic strain state, 9 per tet, zero at the start
+    // (corotationalPlastic.cpp:8-21, 45-46)
+    std::vector<Scalar> plYield, plCreep, plMax, plStrain;
 };
 
 // element.h:109-113
@@ -826,7 +831,75 @@ static void neoElement(const Vec3 *ori, const Vec3 *cur, Scalar lambda, Scalar m
     for (int i = 0; i < 3; i++) nvw[9 + i] = -(nvw[0 + i] + nvw[3 + i] + nvw[6 + i]);
 }
 
-// Material description per tet: kind 0 = corotational, 1 = invertible neo-Hookean.
+// corotationalPlastic.cpp:124-183.  The elastic strain is sym(S) - I - plastic strain with S
+// the factored part of F (polar or QR); the stress returned is R (lambda tr(e) I + 2 mu e).
+// Then, when the norm of the elastic strain exceeds the yield criterion, the plastic strain
+// creeps towards it and is capped at the maximum norm.  Both norms sum strain[i]^2 over i, j
+// < 3 with only i indexing the strain (:159, :172): 3 (e00^2 + e01^2 + e02^2), as the reference.
+static void plasticElasticStress(Scalar lambda, Scalar mu, Scalar yield, Scalar creep, Scalar maxStrain,
+                                 const Scalar *ortho, const Scalar *factored, Scalar *plastic, Scalar *elasticStress) {
+    const Scalar mu2 = Scalar(2) * mu;
+    Scalar e[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            e[i * 3 + j] = Scalar(0.5) * (factored[i * 3 + j] + factored[j * 3 + i]) - plastic[i * 3 + j];
+    e[0] -= Scalar(1); e[4] -= Scalar(1); e[8] -= Scalar(1);
+    Scalar st[9];
+    for (int i = 0; i < 9; i++) st[i] = mu2 * e[i];
+    const Scalar dg = lambda * (e[0] + e[4] + e[8]);
+    st[0] += dg; st[4] += dg; st[8] += dg;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            elasticStress[i * 3 + j] = ortho[i * 3 + 0] * st[0 * 3 + j] + ortho[i * 3 + 1] * st[1 * 3 + j] +
+                                       ortho[i * 3 + 2] * st[2 * 3 + j];
+    Scalar en = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) en += e[i] * e[i];
+    en = std::sqrt(en);
+    if (en > yield) {
+        const Scalar ratio = creep * (en - yield) / en;
+        for (int i = 0; i < 9; i++) plastic[i] += ratio * e[i];
+        Scalar pn = 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) pn += plastic[i] * plastic[i];
+        pn = std::sqrt(pn);
+        if (pn > maxStrain) {
+            const Scalar f = maxStrain / pn;
+            for (int i = 0; i < 9; i++) plastic[i] *= f;
+        }
+    }
+}
+
+// corotationalPlastic.cpp:57-114 for one tet: stiffness as the corotational element
+// (tetelement.cpp:664-713), nodal virtual works from the elastic stress (tetelement.cpp:715-729).
+static void plasticElement(const Vec3 *ori, const Vec3 *cur, const Scalar *D, Scalar threshold, Scalar yield,
+                           Scalar creep, Scalar maxStrain, Scalar *plastic, Scalar *sub, Scalar *nvw) {
+    Scalar K0[90], DmInv[9], dr[12];
+    corotPrecompute(ori, D, K0, DmInv);
+    getTetShapeFunctionDerivatives(ori[0], ori[1], ori[2], ori[3], dr, dr + 3, dr + 6, dr + 9);
+    Scalar g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 1; i < 4; i++) {
+        Vec3 ax = cur[i] - cur[0];
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) g[j * 3 + k] += ax[j] * DmInv[(i - 1) * 3 + k];
+    }
+    const Scalar v0 = getTetVolume(ori[0], ori[1], ori[2], ori[3]), v1 = getTetVolume(cur[0], cur[1], cur[2], cur[3]);
+    Scalar R[9], S[9];
+    if (v1 >= threshold * v0) polarDecompostion3x3(g, R, S);
+    else properQRDecompostion3x3(g, R, S);
+    Scalar P[9];
+    plasticElasticStress(D[1], D[2], yield, creep, maxStrain, R, S, plastic, P);
+    corotSubStiffness(ori, cur, R, K0, sub, nvw);   // the 78 stiffness values (its forces are replaced)
+    const Scalar f6 = Scalar(1.0) / Scalar(6.0);
+    for (int i = 0; i < 3; i++) {
+        for (int r = 0; r < 3; r++)
+            nvw[i * 3 + r] = (P[r * 3 + 0] * dr[i * 3 + 0] + P[r * 3 + 1] * dr[i * 3 + 1] + P[r * 3 + 2] * dr[i * 3 + 2]) * f6;
+    }
+    for (int r = 0; r < 3; r++) nvw[9 + r] = -(nvw[0 + r] + nvw[3 + r] + nvw[6 + r]);
+}
+
+// Material description per tet: kind 0 = corotational, 1 = invertible neo-Hookean,
+// 2 = corotational plastic (parameters and state in the System).
 struct Materials {
     std::vector<int> kind;
     std::vector<Scalar> E, nu;
@@ -835,7 +908,7 @@ struct Materials {
 
 // corotationalHyperelastic.cpp:43-102 and invertibleHyperelastic.cpp:34-114:
 // element loop, K.addEntry(map index, value), vws[dof] += force.  Accumulates.
-static void assemble(const System &s, const Materials &m, Scalar *K, Scalar *vws) {
+static void assemble(System &s, const Materials &m, Scalar *K, Scalar *vws) {
     int gi[12];
     Scalar sub[78], nvw[12], K0[90], DmInv[9], R[9], D[3];
     for (int e = 0; e < s.nTets; e++) {
@@ -848,6 +921,10 @@ static void assemble(const System &s, const Materials &m, Scalar *K, Scalar *vws
             corotPrecompute(ori, D, K0, DmInv);
             corotRotation(ori, cur, DmInv, m.corotThreshold, R);
             corotSubStiffness(ori, cur, R, K0, sub, nvw);
+        } else if (m.kind[e] == 2) {
+            lameD(m.E[e], m.nu[e], D);
+            plasticElement(ori, cur, D, m.corotThreshold, s.plYield[e], s.plCreep[e], s.plMax[e], &s.plStrain[9 * (size_t)e],
+                           sub, nvw);
         } else {
             Scalar lambda = m.nu[e] * m.E[e] / ((Scalar(1.0) + m.nu[e]) * (Scalar(1.0) - Scalar(2.0) * m.nu[e]));
             Scalar mu = m.E[e] / (Scalar(2.0) * (Scalar(1.0) + m.nu[e]));
@@ -1512,6 +1589,9 @@ void *oro_system_create(int nNodes, const double *X, int nTets, const int *tets,
     s->tets.assign(tets, tets + 4 * (size_t)nTets);
     s->indexer.constrainIndices.assign(con, con + nCon);
     buildSystem(*s);
+    // plastic parameters until oro_set_plastic: never yielding (the corotational response)
+    s->plYield.assign(nTets, HUGE_VAL); s->plCreep.assign(nTets, 0.0); s->plMax.assign(nTets, HUGE_VAL);
+    s->plStrain.assign(9 * (size_t)nTets, 0.0);
     return s;
 }
 void oro_system_destroy(void *h) { delete (System *)h; }
@@ -1548,6 +1628,19 @@ void oro_nodal_force(void *h, const double *f, int count, const int *nodes, doub
                 int g = s->indexer.getGlobalIndex(nodes[i], j);
                 if (g >= 0) vws[g] += f[j];
             }
+}
+
+// corotational plastic parameters per tet (kind 2); resets the plastic strain state to zero
+void oro_set_plastic(void *h, const double *yield, const double *creep, const double *maxStrain) {
+    System *s = (System *)h;
+    s->plYield.assign(yield, yield + s->nTets);
+    s->plCreep.assign(creep, creep + s->nTets);
+    s->plMax.assign(maxStrain, maxStrain + s->nTets);
+    s->plStrain.assign(9 * (size_t)s->nTets, 0.0);
+}
+void oro_get_plastic_strain(void *h, double *out) {
+    System *s = (System *)h;
+    std::copy(s->plStrain.begin(), s->plStrain.end(), out);
 }
 
 void oro_assemble(void *h, const int *kind, const double *E, const double *nu, double thrC, double thrN, double *K, double *vws) {
