@@ -225,6 +225,33 @@ int od_backward_euler_update_mesh(od_integrator *it);
 /* Simulator::getVertexPositions (core/simulator.cpp:24-33): float X + u per node */
 int od_integrator_positions(od_integrator *it, float *pos_host);
 
+/* ---- plane collision damping ----------------------------------------------------
+ * MeshPlaneCollisionDampingForcer::setCollisionPair + addDampingMatrix per pair
+ * (collision/penaltyDampingForcer.h:25-38, penaltyDampingForcer.cpp:8-61, with the
+ * plane-tet intersection and barycentric coordinates of collision/intersection.cpp:
+ * 12-143): adds into D, in pair order, the damping matrix of nPairs (plane, tet)
+ * pairs.  planes_host: 4 doubles per pair, the plane's normal n and constant c
+ * (StaticPlaneShape, collisionShapes.h:17-20; below the plane: n.x + c < 0);
+ * tets_host: the pair's tet.  Positions are the rest positions plus disp_dev and
+ * the velocities vel_dev (free-DOF vectors; NULL: zero), constrained nodes at
+ * rest.  A pair adds coeff dir n̂^T scaled by the barycentric weights of the
+ * submerged part's centroid, only when the velocity there moves into the plane;
+ * the stored diagonal-block entry (i+j, i) receives t(i, j) as in the reference
+ * (penaltyDampingForcer.cpp:40-45).  The sums are in pair order, bit-reproducible. */
+int od_plane_damping(od_matrix *D, int nPairs, const double *planes_host, const int *tets_host, double coeff,
+                     const double *disp_dev, const double *vel_dev);
+/* NonlinearImplicitBackwardEuler::addExternalDampingForceMatrix with a
+ * MeshPlaneCollisionDampingForcer that reads the integrator's own displacement and
+ * velocity (nonlinearImplicitBackwardEuler.h:122-124); the damping matrix enters
+ * every later step's tangent (:89) until od_integrator_clear_damping (:126-128).
+ * Newmark integrators return OD_ERR_UNSUPPORTED, as the reference's base
+ * Intergrator reports the call unimplemented (core/intergrator.h:19-24). */
+int od_integrator_add_plane_damping(od_integrator *it, int nPairs, const double *planes_host, const int *tets_host,
+                                    double coeff);
+int od_integrator_clear_damping(od_integrator *it);
+/* the integrator's damping matrix values (canonical layout; zeros if none) */
+int od_integrator_damping(od_integrator *it, double *values_host);
+
 /* ---- serving (no reference counterpart: the reference steps one mesh in one
  * thread, core/simulator.cpp:24-38) ----------------------------------------------
  * A pool of registered objects, each a mesh with its per-tet material, body

@@ -450,6 +450,42 @@ private:
 // Intergrator (core/intergrator.h:13-35).  setExternalVirtualWork takes the
 // free-DOF virtual works that Forcer::getVirtualWorks produces
 // (core/forcer.cpp:80-96); velocities come out as 3 per node.
+// collision/collisionShapes.h:17-20: the plane n.x + c = 0 (below: n.x + c < 0)
+struct StaticPlaneShape {
+    Scalar planeNormal[3];
+    Scalar planeConstant;
+};
+
+// MeshPlaneCollisionDampingForcer (collision/penaltyDampingForcer.h:25-38): the (plane, tet)
+// pair whose damping matrix addExternalDampingForceMatrix adds.  The positions and velocities
+// it reads are the integrator's own (the reference passes the mesh and a velocities array
+// the caller fills from getMeshVerticesVelocities).  setCollisionPair replaces the pair as
+// the reference's does; addCollisionPair queues further pairs so that one
+// addExternalDampingForceMatrix call adds them all, in order.
+class MeshPlaneCollisionDampingForcer {
+public:
+    explicit MeshPlaneCollisionDampingForcer(Scalar dampingCoefficient) : coeff_(dampingCoefficient) {}
+    void setCollisionPair(const StaticPlaneShape &plane, int elementIndex) {
+        planes_.clear();
+        tets_.clear();
+        addCollisionPair(plane, elementIndex);
+    }
+    void addCollisionPair(const StaticPlaneShape &plane, int elementIndex) {
+        planes_.insert(planes_.end(), {plane.planeNormal[0], plane.planeNormal[1], plane.planeNormal[2],
+                                       plane.planeConstant});
+        tets_.push_back(elementIndex);
+    }
+    int pairCount() const { return (int)tets_.size(); }
+    const Scalar *planes() const { return planes_.data(); }
+    const int *elements() const { return tets_.data(); }
+    Scalar coefficient() const { return coeff_; }
+
+private:
+    Scalar coeff_;
+    std::vector<Scalar> planes_;
+    std::vector<int> tets_;
+};
+
 class Intergrator {
 public:
     Intergrator(int dofs, Scalar massDamp, Scalar stiffDamp, Scalar ts)
@@ -471,6 +507,15 @@ public:
         std::vector<Scalar> v((size_t)dofs);
         if (Severe(od_integrator_get_state(it_, nullptr, v.data(), nullptr), "od_integrator_get_state")) return;
         mesh.scatterFree(v.data(), velocities);
+    }
+    // core/intergrator.h:19-24: integrators without external damping report it unimplemented
+    virtual void addExternalDampingForceMatrix(const MeshPlaneCollisionDampingForcer &) {
+        std::fprintf(stderr, "Unimplemented features in Intergrator::addExternalDampingForceMatrix\n");
+        status_ = OD_ERR_UNSUPPORTED;
+    }
+    virtual void clearExternalDampingForceMatrix() {
+        std::fprintf(stderr, "Unimplemented features in Intergrator::clearExternalDampingForceMatrix\n");
+        status_ = OD_ERR_UNSUPPORTED;
     }
     // one-step nodal impulse (added to the next step's external force only)
     void setImpulse(const Scalar *vws) { status_ = Severe(od_integrator_set_impulse(it_, vws), "od_integrator_set_impulse"); }
@@ -526,6 +571,15 @@ public:
     void updateMeshVerticesDisplacements(TetMesh &mesh) const override {
         if (Severe(od_backward_euler_update_mesh(it_), "od_backward_euler_update_mesh")) return;
         Intergrator::updateMeshVerticesDisplacements(mesh);
+    }
+    // nonlinearImplicitBackwardEuler.h:122-128
+    void addExternalDampingForceMatrix(const MeshPlaneCollisionDampingForcer &forcer) override {
+        status_ = Severe(od_integrator_add_plane_damping(it_, forcer.pairCount(), forcer.planes(), forcer.elements(),
+                                                         forcer.coefficient()),
+                         "od_integrator_add_plane_damping");
+    }
+    void clearExternalDampingForceMatrix() override {
+        status_ = Severe(od_integrator_clear_damping(it_), "od_integrator_clear_damping");
     }
 };
 

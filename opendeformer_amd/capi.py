@@ -34,7 +34,8 @@ EXPORTS = [
     "od_solver_set_refinement", "od_integrator_set_refinement", "od_mesh_autotune", "od_mesh_spmv_variant",
     "od_solver_set_graph", "od_integrator_set_graph", "od_mesh_bind_ctx", "od_integrator_device_bytes",
     "od_server_create", "od_server_destroy", "od_server_add_object", "od_server_run", "od_server_object_state",
-    "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain",
+    "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain", "od_plane_damping",
+    "od_integrator_add_plane_damping", "od_integrator_clear_damping", "od_integrator_damping",
 ]
 
 
@@ -83,6 +84,8 @@ def lib():
             "od_integrator_get_state": [vp, dp, dp, dp], "od_integrator_set_state": [vp, dp, dp, dp],
             "od_integrator_state_dev": [vp, pvp, pvp, pvp], "od_integrator_stats": [vp, ip, dp, ip, ip],
             "od_backward_euler_update_mesh": [vp], "od_integrator_positions": [vp, C.POINTER(C.c_float)],
+            "od_plane_damping": [vp, i, dp, ip, d, vp, vp], "od_integrator_add_plane_damping": [vp, i, dp, ip, d],
+            "od_integrator_clear_damping": [vp], "od_integrator_damping": [vp, dp],
             "od_spmv_timed": [vp, vp, vp, i, i, dp], "od_integrator_matrix": [vp, pvp],
             "od_ctx_profile": [vp, i], "od_ctx_profile_read": [vp, dp, C.POINTER(ll)],
         }

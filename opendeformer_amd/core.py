@@ -199,6 +199,13 @@ class Matrix:
         check(lib().od_matrix_download(self.h, dptr(v)))
         return v
 
+    def add_plane_damping(self, planes, tets, coeff, disp_dev=None, vel_dev=None):
+        """MeshPlaneCollisionDampingForcer::addDampingMatrix for each (plane, tet) pair, in order
+        (collision/penaltyDampingForcer.cpp:8-61); planes: (k, 4) normal and constant."""
+        planes = f64(np.asarray(planes, np.float64).reshape(-1, 4)); tets = np.ascontiguousarray(tets, np.int32)
+        check(lib().od_plane_damping(self.h, int(tets.size), dptr(planes), capi.iptr(tets), float(coeff),
+                                     disp_dev, vel_dev))
+
     def set_zeros(self):
         check(lib().od_matrix_zero(self.h))
 
@@ -350,6 +357,22 @@ class BackwardEulerIntegrator(_Integrator):
 
     def update_mesh_vertices_displacements(self):
         check(lib().od_backward_euler_update_mesh(self.h))
+
+    def add_external_damping_force_matrix(self, planes, tets, coeff):
+        """addExternalDampingForceMatrix (nonlinearImplicitBackwardEuler.h:122-124) with a
+        MeshPlaneCollisionDampingForcer per (plane, tet) pair reading this integrator's
+        displacement and velocity."""
+        planes = f64(np.asarray(planes, np.float64).reshape(-1, 4)); tets = np.ascontiguousarray(tets, np.int32)
+        check(lib().od_integrator_add_plane_damping(self.h, int(tets.size), dptr(planes), capi.iptr(tets),
+                                                    float(coeff)))
+
+    def clear_external_damping_force_matrix(self):
+        check(lib().od_integrator_clear_damping(self.h))
+
+    def damping_matrix(self):
+        D = np.zeros(self.mesh.n_values)
+        check(lib().od_integrator_damping(self.h, dptr(D)))
+        return D
 
 
 class Server:

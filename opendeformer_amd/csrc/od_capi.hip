@@ -107,7 +107,42 @@ struct od_mesh {
         if (!blkC) blkC = (double *)alloc(sizeof(double) * 9 * std::max<int64_t>(1, topo.nPairContrib));
         if (!nodeC) nodeC = (double *)alloc(sizeof(double) * 3 * std::max<int64_t>(1, topo.nNodeContrib));
     }
+    // plane collision damping: per-block counters (zeroed once, left at zero by every call)
+    // and per-pair buffers grown to the largest call so far
+    DampScratch damp;
+    int dampCap = 0;
+    std::vector<void *> dampPairAllocs;
+    void ensureDamp(int nPairs) {
+        if (!damp.cnt) {
+            const size_t b = sizeof(int) * (size_t)std::max<int64_t>(1, topo.nBlocks);
+            damp.cnt = (int *)alloc(b);
+            damp.fill = (int *)alloc(b);
+            damp.start = (int *)alloc(b);
+            damp.nTouched = (int *)alloc(sizeof(int));
+            OD_HIP(hipMemsetAsync(damp.cnt, 0, b, ctx->stream));
+            OD_HIP(hipMemsetAsync(damp.fill, 0, b, ctx->stream));
+        }
+        if (nPairs <= dampCap) return;
+        OD_HIP(hipStreamSynchronize(ctx->stream));
+        for (void *p : dampPairAllocs) (void)hipFree(p);
+        dampPairAllocs.clear();
+        const int cap = std::max(nPairs, 2 * dampCap);
+        auto get = [&](size_t b) {
+            void *p = nullptr;
+            OD_HIP(hipMalloc(&p, b));
+            dampPairAllocs.push_back(p);
+            return p;
+        };
+        damp.planes = (double *)get(sizeof(double) * 4 * (size_t)cap);
+        damp.tets = (int *)get(sizeof(int) * (size_t)cap);
+        damp.pv = (double *)get(sizeof(double) * 90 * (size_t)cap);
+        damp.dst = (int *)get(sizeof(int) * 10 * (size_t)cap);
+        damp.list = (int *)get(sizeof(int) * 10 * (size_t)cap);
+        damp.touched = (int *)get(sizeof(int) * 10 * (size_t)cap);
+        dampCap = cap;
+    }
     ~od_mesh() {
+        for (void *p : dampPairAllocs) (void)hipFree(p);
         for (void *p : allocs) (void)hipFree(p);
     }
 };
@@ -178,6 +213,9 @@ struct od_integrator {
     double *d = nullptr, *v = nullptr, *a = nullptr, *aNew = nullptr, *fext = nullptr, *imp = nullptr, *rhs = nullptr;
     uint8_t *active = nullptr, *inactive = nullptr;
     bool hasImpulse = false, hasActive = false;
+    // external damping matrix (backward Euler; allocated on the first add) and D v
+    double *D = nullptr, *Dv = nullptr;
+    bool hasDamp = false;
     std::vector<void *> allocs;
     std::vector<int> lastIters;
     std::vector<double> lastRelres;
@@ -1267,6 +1305,15 @@ static void integrator_step(od_integrator *it) {
         launch_element(m, st, it->d, it->v, nullptr, P, mesh->blkC, mesh->nodeC, act);
         launch_gather_blocks(m, st, mesh->blkC, h * (h + aK), it->M.vals, 1.0 + h * aM, it->A.vals, 0, nullptr, act);
         launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, h, it->rhs, 0, act);
+        if (it->hasDamp) {
+            // the tangent (h + aK) K + aM M + D: A gains h D, the right-hand side -h D v
+            const int64_t nv = mesh->topo.nValues;
+            hipLaunchKernelGGL(k_axpby, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, it->A.vals, 1.0, h,
+                               (const double *)it->D, nv);
+            OD_HIP(hipGetLastError());
+            launch_spmv<double>(m, st, it->D, it->v, it->Dv, 0, nullptr);
+            launch_axpy_rows(m, st, it->rhs, -h, it->Dv, act);
+        }
         // delta_v persists across steps as the warm start (aNew holds it)
         solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);
         launch_be_correct(m, st, it->v, it->aNew, act);
@@ -1337,6 +1384,70 @@ int od_backward_euler_update_mesh(od_integrator *it) {
     require(it != nullptr && it->kind == 1, "not a backward-Euler integrator", OD_ERR_STATE);
     launch_axpy_rows(it->mesh->dm, it->mesh->ctx->stream, it->d, it->h, it->v, it->hasActive ? it->active : nullptr);
     OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    API_END
+}
+
+// ---- plane collision damping ---------------------------------------------------------
+static void plane_damping(od_mesh *mesh, double *D, int nPairs, const double *planes, const int *tets, double coeff,
+                          const double *disp, const double *vel) {
+    require(nPairs >= 0 && nPairs <= (1 << 27), "number of pairs out of range");
+    if (nPairs == 0) return;
+    require(planes && tets, "NULL argument");
+    for (int k = 0; k < nPairs; k++)
+        require(tets[k] >= 0 && tets[k] < mesh->topo.nTets, "collision pair names a tet outside the mesh");
+    mesh->ensureDamp(nPairs);
+    hipStream_t st = mesh->ctx->stream;
+    OD_HIP(hipMemcpyAsync(mesh->damp.planes, planes, sizeof(double) * 4 * (size_t)nPairs, hipMemcpyHostToDevice, st));
+    OD_HIP(hipMemcpyAsync(mesh->damp.tets, tets, sizeof(int) * (size_t)nPairs, hipMemcpyHostToDevice, st));
+    launch_plane_damping(mesh->dm, st, nPairs, mesh->damp, coeff, disp, vel, D);
+    OD_HIP(hipStreamSynchronize(st));
+}
+
+int od_plane_damping(od_matrix *D, int nPairs, const double *planes, const int *tets, double coeff, const double *disp,
+                     const double *vel) {
+    API_BEGIN
+    require(D != nullptr, "NULL argument");
+    plane_damping(D->mesh, D->vals, nPairs, planes, tets, coeff, disp, vel);
+    API_END
+}
+
+int od_integrator_add_plane_damping(od_integrator *it, int nPairs, const double *planes, const int *tets,
+                                    double coeff) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    require(it->kind == 1, "external damping matrices are a backward-Euler feature (Intergrator reports them unimplemented)",
+            OD_ERR_UNSUPPORTED);
+    if (!it->D) {
+        const size_t nv = (size_t)it->mesh->topo.nValues;
+        it->D = (double *)it->alloc(8 * (nv + kValuePad));
+        it->Dv = (double *)it->alloc(8 * (size_t)it->mesh->topo.ndof);
+    }
+    plane_damping(it->mesh, it->D, nPairs, planes, tets, coeff, it->d, it->v);
+    it->hasDamp = true;
+    API_END
+}
+
+int od_integrator_clear_damping(od_integrator *it) {
+    API_BEGIN
+    require(it != nullptr, "NULL argument");
+    if (it->D) {
+        OD_HIP(hipMemsetAsync(it->D, 0, 8 * (size_t)it->mesh->topo.nValues, it->mesh->ctx->stream));
+        OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    }
+    it->hasDamp = false;
+    API_END
+}
+
+int od_integrator_damping(od_integrator *it, double *values) {
+    API_BEGIN
+    require(it && values, "NULL argument");
+    const size_t b = 8 * (size_t)it->mesh->topo.nValues;
+    if (!it->D) {
+        std::memset(values, 0, b);
+    } else {
+        OD_HIP(hipMemcpyAsync(values, it->D, b, hipMemcpyDeviceToHost, it->mesh->ctx->stream));
+        OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+    }
     API_END
 }
 

@@ -149,6 +149,21 @@ void launch_newmark_correct(const DevMesh &m, hipStream_t s, double *d, double *
                             const double *aNew, double h, double beta, double gamma, const uint8_t *instActive);
 void launch_be_correct(const DevMesh &m, hipStream_t s, double *v, const double *dv, const uint8_t *instActive);
 void launch_axpy_rows(const DevMesh &m, hipStream_t s, double *y, double a, const double *x, const uint8_t *instActive);
+
+// ---- plane collision damping (od_damping.hip) --------------------------------------
+// Per-call buffers sized for nPairs, and per-block counters that are zero between calls.
+struct DampScratch {
+    int *cnt = nullptr, *fill = nullptr, *start = nullptr;   // nBlocks each
+    double *planes = nullptr;                                  // 4 per pair (normal, constant)
+    int *tets = nullptr;                                       // 1 per pair
+    double *pv = nullptr;                                      // 90 per pair
+    int *dst = nullptr, *list = nullptr, *touched = nullptr;   // 10 per pair each
+    int *nTouched = nullptr;
+};
+// D += the damping matrix of nPairs (plane, tet) pairs (w.planes, w.tets already on the
+// device); disp and vel are free-DOF vectors (NULL: zero)
+void launch_plane_damping(const DevMesh &m, hipStream_t s, int nPairs, const DampScratch &w, double coeff,
+                          const double *disp, const double *vel, double *D);
 void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDone, const uint8_t *instActive);
 
 }  // namespace od

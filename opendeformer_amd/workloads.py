@@ -224,3 +224,14 @@ def served_requests(pool, rate, duration, seed=6, max_steps=8, max_force=100.0, 
         f = d * mag / nodes.size
         impulses.append([(int(n), tuple(f)) for n in nodes])
     return obj, steps, np.asarray(arrival), impulses
+
+
+def plane_contacts(X, tets, normal, constant, eps=1e-6):
+    """(plane, tet) collision pairs of a static plane n.x + c = 0: every tet with a vertex at
+    n.x + c < eps, in tet order, as a MeshPlaneCollisionDampingForcer caller would set them
+    (collision/penaltyDampingForcer.h:25-38).  Returns (planes (k, 4), tets (k,))."""
+    n = np.asarray(normal, np.float64)
+    below = (np.asarray(X, np.float64) @ n + constant) < eps
+    hit = np.nonzero(below[np.asarray(tets)].any(axis=1))[0].astype(np.int32)
+    planes = np.tile(np.r_[n, constant], (hit.size, 1))
+    return planes, hit

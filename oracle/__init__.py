@@ -77,6 +77,13 @@ def lib():
         L.oro_be_update_mesh.argtypes = [vp]
         L.oro_be_get_velocity.argtypes = [vp, dp]
         L.oro_get_node_displacements.argtypes = [vp, dp]
+        L.oro_plane_damping.argtypes = [vp, i, dp, ip, d, dp, dp]
+        L.oro_be_add_plane_damping.argtypes = [vp, i, dp, ip, d]
+        L.oro_be_clear_damping.argtypes = [vp]
+        L.oro_be_damping.argtypes = [vp, dp]
+        L.oro_be_set_velocity.argtypes = [vp, dp]
+        L.oro_plane_tet_intersection.argtypes = [dp, dp, dp, dp]
+        L.oro_barycentric.argtypes = [dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -173,6 +180,15 @@ class System:
         out = np.zeros(9 * self.n_tets)
         lib().oro_get_plastic_strain(self.h, _d(out))
         return out.reshape(-1, 3, 3)
+
+    def plane_damping(self, planes, tets, coeff, vel_nodes, values=None):
+        """Adds MeshPlaneCollisionDampingForcer::addDampingMatrix of each (plane, tet) pair, in
+        order, into values (positions: rest + current node displacements)."""
+        values = np.zeros(self.n_values) if values is None else values
+        planes = f64(np.asarray(planes, np.float64).reshape(-1, 4)); tets = i32(tets)
+        lib().oro_plane_damping(self.h, len(tets), _d(planes), _i(tets), coeff, _d(f64(np.asarray(vel_nodes).reshape(-1))),
+                                _d(values))
+        return values
 
     def assemble(self, kind, E, nu, thr_corot=0.0, thr_neo=0.1):
         kind = i32(np.broadcast_to(kind, (self.n_tets,)))
@@ -297,3 +313,33 @@ class BackwardEuler:
         v = np.zeros(self.sys.ndof)
         lib().oro_be_get_velocity(self.h, _d(v))
         return v
+
+    def set_velocity(self, v):
+        lib().oro_be_set_velocity(self.h, _d(f64(v)))
+
+    def add_plane_damping(self, planes, tets, coeff):
+        """addExternalDampingForceMatrix of a MeshPlaneCollisionDampingForcer per (plane, tet)
+        pair, reading this integrator's velocities and the mesh displacements."""
+        planes = f64(np.asarray(planes, np.float64).reshape(-1, 4)); tets = i32(tets)
+        lib().oro_be_add_plane_damping(self.h, len(tets), _d(planes), _i(tets), coeff)
+
+    def clear_damping(self):
+        lib().oro_be_clear_damping(self.h)
+
+    def damping(self):
+        D = np.zeros(self.sys.n_values)
+        lib().oro_be_damping(self.h, _d(D))
+        return D
+
+
+def plane_tet_intersection(plane, pts):
+    """(centroid, dir) of the part of tet pts (4x3) below plane (n, c) (intersection.cpp:20-117)."""
+    c, d = np.zeros(3), np.zeros(3)
+    lib().oro_plane_tet_intersection(_d(f64(plane)), _d(f64(np.asarray(pts).reshape(12))), _d(c), _d(d))
+    return c, d
+
+
+def barycentric(pts, q):
+    out = np.zeros(4)
+    lib().oro_barycentric(_d(f64(np.asarray(pts).reshape(12))), _d(f64(q)), _d(out))
+    return out

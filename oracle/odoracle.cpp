@@ -1428,6 +1428,153 @@ struct CG {
 };
 
 // ---------------------------------------------------------------------------
+// Plane collision damping — collision/intersection.cpp:12-143,
+// collision/penaltyDampingForcer.cpp:8-61
+// ---------------------------------------------------------------------------
+struct Plane {
+    Vec3 n;        // planeNormal (collisionShapes.h:17-20); below the plane: n.x + c < 0
+    Scalar c;      // planeConstant
+};
+
+static Vec3 normalize(const Vec3 &v) { return v / v.length(); }   // latool.h:528-530
+
+// intersection.cpp:12-17: point where segment pa-pb crosses the plane, t clamped to [0, 1]
+static Vec3 segmentPlane(const Plane &pl, const Vec3 &pa, const Vec3 &pb) {
+    Vec3 ab = pb - pa;
+    Scalar t = (-pl.c - pl.n * pa) / (pl.n * ab);
+    t = t < 0 ? Scalar(0) : (t > 1 ? Scalar(1) : t);               // oder.h:121-125
+    return pa + t * ab;
+}
+
+// intersection.cpp:20-117: centroid of the part of the tet below the plane and the
+// direction vector dir (magnitude six times that part's volume, along the normal).
+// Points closer than 1e-6 above the plane count as below.
+static void planeTetIntersection(const Plane &pl, const Vec3 *p, Vec3 &centroid, Vec3 &dir) {
+    const Scalar eps = Scalar(1e-6);
+    Vec3 neg[4], pos[4];
+    int nn = 0, np = 0;
+    for (int i = 0; i < 4; i++) {
+        if (pl.n * p[i] + pl.c < eps) neg[nn++] = p[i];
+        else pos[np++] = p[i];
+    }
+    if (nn == 0) {
+        Scalar minDis = std::numeric_limits<Scalar>::max();
+        for (int i = 0; i < 4; i++) {
+            Scalar dis = pl.n * p[i] + pl.c;
+            if (dis < minDis) { minDis = dis; centroid = p[i]; }
+        }
+        dir = Vec3(0, 0, 0);
+    } else if (nn == 1) {
+        Vec3 q0 = segmentPlane(pl, neg[0], pos[0]), q1 = segmentPlane(pl, neg[0], pos[1]),
+             q2 = segmentPlane(pl, neg[0], pos[2]);
+        centroid = (neg[0] + q0 + q1 + q2) * Scalar(0.25);
+        Vec3 ab = q0 - neg[0], ac = q1 - neg[0], ad = q2 - neg[0];
+        Scalar svol = ab * (ac % ad);
+        dir = std::fabs(svol) * normalize((ab % ac) + (ac % ad) + (ad % ab));
+        if (svol < 0) dir = -dir;
+    } else if (nn == 2) {
+        Vec3 q0 = segmentPlane(pl, neg[0], pos[0]), q1 = segmentPlane(pl, neg[0], pos[1]);
+        Vec3 q2 = segmentPlane(pl, neg[1], pos[0]), q3 = segmentPlane(pl, neg[1], pos[1]);
+        Vec3 ab = neg[1] - neg[0];
+        // the wedge below the plane as three tets (n0 n1 q2 q3), (n0 q0 q1 q2), (n0 q1 q2 q3)
+        Vec3 c0 = neg[0] + neg[1] + q2 + q3;
+        Scalar v0 = std::fabs((neg[1] - neg[0]) * ((q2 - neg[0]) % (q3 - neg[0])));
+        Vec3 c1 = neg[0] + q0 + q1 + q2;
+        Scalar v1 = std::fabs((q0 - neg[0]) * ((q1 - neg[0]) % (q2 - neg[0])));
+        Vec3 c2 = neg[0] + q1 + q2 + q3;
+        Scalar v2 = std::fabs((q1 - neg[0]) * ((q2 - neg[0]) % (q3 - neg[0])));
+        centroid = (c0 * v0 + c1 * v1 + c2 * v2) / (v0 + v1 + v2) * Scalar(0.25);
+        dir = (q0 - neg[0]) % (q1 - neg[0]) + (q3 - neg[1]) % (q2 - neg[1]) + ab % (q2 - neg[0]) +
+              (q2 - neg[0]) % (q0 - neg[0]) + (q3 - neg[0]) % ab + (q1 - neg[0]) % (q3 - neg[0]);
+        dir = (v0 + v1 + v2) * normalize(dir);
+        if ((neg[1] - neg[0]) * ((pos[0] - neg[0]) % (pos[1] - neg[0])) < 0) dir = -dir;
+    } else if (nn == 3) {
+        Vec3 q0 = segmentPlane(pl, neg[0], pos[0]), q1 = segmentPlane(pl, neg[1], pos[0]),
+             q2 = segmentPlane(pl, neg[2], pos[0]);
+        Vec3 ab = q0 - pos[0], ac = q1 - pos[0], ad = q2 - pos[0];
+        Vec3 c = p[0] + p[1] + p[2] + p[3];
+        Scalar sv = (neg[0] - pos[0]) * ((neg[1] - pos[0]) % (neg[2] - pos[0]));
+        Scalar v = std::fabs(sv);
+        Vec3 c0 = pos[0] + q0 + q1 + q2;
+        Scalar v0 = std::fabs(ab * (ac % ad));
+        // the tet minus the small tet above the plane at pos[0]
+        centroid = (v * c - v0 * c0) / (v - v0) * Scalar(0.25);
+        dir = (v - v0) * normalize((ab % ac) + (ac % ad) + (ad % ab));
+        if (sv > 0) dir = -dir;
+    } else {
+        centroid = (p[0] + p[1] + p[2] + p[3]) * Scalar(0.25);
+        Scalar vol = std::fabs((p[1] - p[0]) * ((p[2] - p[0]) % (p[3] - p[0])));
+        dir = vol * pl.n;
+    }
+}
+
+// intersection.cpp:119-143: unsigned sub-volume ratios (1/4 each for a tet of 6V <= 1e-6)
+static void barycentric(const Vec3 *p, const Vec3 &q, Scalar *coords) {
+    const Scalar eps = Scalar(1e-6);
+    Vec3 ab = p[1] - p[0], ac = p[2] - p[0], ad = p[3] - p[0], bc = p[2] - p[1], bd = p[3] - p[1];
+    Vec3 ap = q - p[0], bp = q - p[1];
+    Scalar va = std::fabs(bp * (bc % bd)), vb = std::fabs(ap * (ac % ad)), vc = std::fabs(ap * (ab % ad)),
+           vd = std::fabs(ap * (ab % ac));
+    Scalar vol = std::fabs(ab * (ac % ad));
+    if (vol <= eps) { coords[0] = coords[1] = coords[2] = coords[3] = Scalar(0.25); return; }
+    Scalar inv = 1 / vol;
+    coords[0] = va * inv; coords[1] = vb * inv; coords[2] = vc * inv; coords[3] = vd * inv;
+}
+
+// penaltyDampingForcer.cpp:8-61: adds the damping matrix of one (plane, tet) pair into
+// values through the tet's 78-entry map.  vel: velocity per mesh node; positions are the
+// rest positions plus the System's displacements.  Nothing is added unless the barycentric
+// velocity at the centroid moves into the plane (vel . n̂ < 0).  The stored diagonal-block
+// entry (i + j, i) receives t(i, j), not t(i + j, i): penaltyDampingForcer.cpp:43 indexes
+// t with the packed offset j, and that is kept as is.
+static void planeDampingPair(const System &s, const Plane &pl, int e, Scalar coeff, const Vec3 *vel, Scalar *values) {
+    static const int diagIdx[12] = {0, 12, 23, 33, 42, 50, 57, 63, 68, 72, 75, 77};
+    const int *tn = &s.tets[4 * (size_t)e];
+    const int *local = &s.elemMap[78 * (size_t)e];
+    Vec3 p[4];
+    for (int i = 0; i < 4; i++) p[i] = s.X[tn[i]] + s.disp[tn[i]];
+    Vec3 centroid, dir;
+    planeTetIntersection(pl, p, centroid, dir);
+    Scalar coords[4];
+    barycentric(p, centroid, coords);
+    Vec3 nd;
+    if (dir.length2() != Scalar(0)) nd = normalize(dir);
+    Vec3 v;
+    for (int i = 0; i < 4; i++) v = v + coords[i] * vel[tn[i]];
+    if (!(v * nd < 0)) return;
+    Ten3 t;
+    Vec3 cd = coeff * dir;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t(i, j) = cd[i] * nd[j];            // latool.h:554-561
+    for (int a = 0; a < 4; a++) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3 - i; j++) {
+                int idx = diagIdx[a * 3 + i] + j;
+                if (local[idx] >= 0) values[local[idx]] += coords[a] * coords[a] * t(i, j);
+            }
+        for (int b = a + 1; b < 4; b++)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    int idx = diagIdx[a * 3 + i] + ((b - a) * 3 - i) + j;
+                    if (local[idx] >= 0) values[local[idx]] += coords[a] * coords[b] * t(i, j);
+                }
+    }
+}
+
+// Node velocities from free-DOF velocities, constrained DOFs zero
+// (nonlinearImplicitBackwardEuler.h:131-145).
+static std::vector<Vec3> nodeVelocities(const System &s, const Scalar *vFree) {
+    std::vector<Vec3> vel(s.nNodes);
+    size_t ci = 0; int dof = 0;
+    for (int n = 0; n < s.nNodes; n++)
+        for (int a = 0; a < 3; a++) {
+            if (ci < s.indexer.constrainIndices.size() && 3 * n + a == s.indexer.constrainIndices[ci]) ci++;
+            else vel[n][a] = vFree[dof++];
+        }
+    return vel;
+}
+
+// ---------------------------------------------------------------------------
 // Integrators
 // ---------------------------------------------------------------------------
 static void scaleValues(std::vector<Scalar> &v, Scalar a) { for (auto &x : v) x *= a; }                       // sparseMatrix.h:289-293
@@ -1504,11 +1651,12 @@ struct Newmark {
     }
 };
 
-// intergrator/nonlinearImplicitBackwardEuler.h:48-146 (damping matrix left zero)
+// intergrator/nonlinearImplicitBackwardEuler.h:48-146; D is the external damping matrix
+// (addExternalDampingForceMatrix / clearExternalDampingForceMatrix, :122-128)
 struct BackwardEuler {
     System *s; Materials mat; CG cg; Preconditioner *pre;
     Scalar h, alphaM, alphaK;
-    std::vector<Scalar> v, dv, fint, fext, rhs, K, M;
+    std::vector<Scalar> v, dv, fint, fext, rhs, K, M, D;
     void step() {
         int n = s->ndof;
         std::fill(K.begin(), K.end(), 0.0);
@@ -1517,6 +1665,7 @@ struct BackwardEuler {
         assemble(*s, mat, K.data(), fint.data());
         scaleValues(K, h + alphaK);
         addValues(K, alphaM, M);
+        addValues(K, Scalar(1), D);
         SpMDV(s->pat, K.data(), v.data(), rhs.data());
         for (int i = 0; i < n; i++) rhs[i] = h * (fext[i] - fint[i] - rhs[i]);
         scaleValues(K, h);
@@ -1808,7 +1957,7 @@ void *oro_be_create(void *sys, const int *kind, const double *E, const double *n
     int n = s->ndof;
     size_t nv = (size_t)s->pat.numEntries();
     for (auto *vec : {&be->v, &be->dv, &be->fint, &be->fext, &be->rhs}) vec->assign(n, 0.0);
-    be->K.assign(nv, 0.0); be->M.assign(nv, 0.0);
+    be->K.assign(nv, 0.0); be->M.assign(nv, 0.0); be->D.assign(nv, 0.0);
     massMatrix(*s, rhoPerTet, be->M.data());
     s->disp.assign(s->nNodes, Vec3());
     return be;
@@ -1818,6 +1967,44 @@ void oro_be_set_external(void *h, const double *f) { BackwardEuler *be = (Backwa
 int oro_be_step(void *h) { BackwardEuler *be = (BackwardEuler *)h; be->step(); return be->cg.lastIters; }
 void oro_be_update_mesh(void *h) { ((BackwardEuler *)h)->updateMesh(); }
 void oro_be_get_velocity(void *h, double *v) { BackwardEuler *be = (BackwardEuler *)h; std::memcpy(v, be->v.data(), be->v.size() * 8); }
+// MeshPlaneCollisionDampingForcer (penaltyDampingForcer.cpp:8-61) for nPairs (plane, tet)
+// pairs in order; planes: 4 per pair (normal, constant); velNodes: 3 per mesh node
+void oro_plane_damping(void *sys, int nPairs, const double *planes, const int *tets, double coeff,
+                       const double *velNodes, double *values) {
+    System *s = (System *)sys;
+    std::vector<Vec3> vel(s->nNodes);
+    for (int i = 0; i < s->nNodes; i++) vel[i] = Vec3(velNodes[3 * i], velNodes[3 * i + 1], velNodes[3 * i + 2]);
+    for (int k = 0; k < nPairs; k++) {
+        Plane pl{Vec3(planes[4 * k], planes[4 * k + 1], planes[4 * k + 2]), planes[4 * k + 3]};
+        planeDampingPair(*s, pl, tets[k], coeff, vel.data(), values);
+    }
+}
+// addExternalDampingForceMatrix with the forcer reading the integrator's own velocities
+// (getMeshVerticesVelocities) and the mesh displacements
+void oro_be_add_plane_damping(void *h, int nPairs, const double *planes, const int *tets, double coeff) {
+    BackwardEuler *be = (BackwardEuler *)h;
+    std::vector<Vec3> vel = nodeVelocities(*be->s, be->v.data());
+    for (int k = 0; k < nPairs; k++) {
+        Plane pl{Vec3(planes[4 * k], planes[4 * k + 1], planes[4 * k + 2]), planes[4 * k + 3]};
+        planeDampingPair(*be->s, pl, tets[k], coeff, vel.data(), be->D.data());
+    }
+}
+void oro_be_clear_damping(void *h) { BackwardEuler *be = (BackwardEuler *)h; std::fill(be->D.begin(), be->D.end(), 0.0); }
+void oro_be_damping(void *h, double *D) { BackwardEuler *be = (BackwardEuler *)h; std::memcpy(D, be->D.data(), be->D.size() * 8); }
+void oro_be_set_velocity(void *h, const double *v) { BackwardEuler *be = (BackwardEuler *)h; std::memcpy(be->v.data(), v, be->v.size() * 8); }
+void oro_plane_tet_intersection(const double *plane4, const double *pts12, double *centroid, double *dir) {
+    Plane pl{Vec3(plane4[0], plane4[1], plane4[2]), plane4[3]};
+    Vec3 p[4];
+    for (int i = 0; i < 4; i++) p[i] = Vec3(pts12[3 * i], pts12[3 * i + 1], pts12[3 * i + 2]);
+    Vec3 c, d;
+    planeTetIntersection(pl, p, c, d);
+    for (int i = 0; i < 3; i++) { centroid[i] = c[i]; dir[i] = d[i]; }
+}
+void oro_barycentric(const double *pts12, const double *q, double *coords) {
+    Vec3 p[4];
+    for (int i = 0; i < 4; i++) p[i] = Vec3(pts12[3 * i], pts12[3 * i + 1], pts12[3 * i + 2]);
+    barycentric(p, Vec3(q[0], q[1], q[2]), coords);
+}
 void oro_get_node_displacements(void *sys, double *u) {
     System *s = (System *)sys;
     for (int i = 0; i < s->nNodes; i++) { u[3 * i] = s->disp[i].x; u[3 * i + 1] = s->disp[i].y; u[3 * i + 2] = s->disp[i].z; }

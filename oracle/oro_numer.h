@@ -23,6 +23,7 @@ struct Vec3 {
     Scalar &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
     Vec3 operator+(const Vec3 &v) const { return Vec3(x + v.x, y + v.y, z + v.z); }
     Vec3 operator-(const Vec3 &v) const { return Vec3(x - v.x, y - v.y, z - v.z); }
+    Vec3 operator-() const { return Vec3(-x, -y, -z); }
     Vec3 operator*(Scalar f) const { return Vec3(x * f, y * f, z * f); }
     friend Vec3 operator*(Scalar f, const Vec3 &v) { return v * f; }
     Scalar operator*(const Vec3 &v) const { return x * v.x + y * v.y + z * v.z; }

@@ -1,7 +1,8 @@
 // Drives the C++ host classes of od_gpu.hpp the way host code written against
 // the reference's interfaces does: material -> generateMatrixAndVirtualWorks,
 // SpMDV, CGSolver::solveLinearSystem, Intergrator::runOneTimeStep and
-// updateMeshVerticesDisplacements.  Input and output are flat binary files
+// updateMeshVerticesDisplacements, and a backward-Euler run with a plane collision
+// damping forcer (addExternalDampingForceMatrix / clearExternalDampingForceMatrix).  Input and output are flat binary files
 // written and read by tests/test_gpu_host_api.py, which compares every output
 // with the oracle.
 //
@@ -30,6 +31,14 @@ int main(int argc, char **argv) {
     if (!rd(in, X.data(), X.size()) || !rd(in, tets.data(), tets.size()) || !rd(in, con.data(), con.size()) ||
         !rd(in, par.data(), 4) || !rd(in, uNode.data(), uNode.size()) || !rd(in, x.data(), ndof) ||
         !rd(in, y.data(), ndof) || !rd(in, b.data(), ndof) || !rd(in, g.data(), ndof))
+        return 3;
+    // plane collision pairs for the backward-Euler run: count, 4 doubles per plane, tets, coefficient
+    int nPairs = 0;
+    if (!rd(in, &nPairs, 1) || nPairs < 1) return 3;
+    std::vector<Scalar> planes(4 * (size_t)nPairs);
+    std::vector<int> pairTets(nPairs);
+    Scalar coeff = 0;
+    if (!rd(in, planes.data(), planes.size()) || !rd(in, pairTets.data(), pairTets.size()) || !rd(in, &coeff, 1))
         return 3;
     std::fclose(in);
     const Scalar rho = par[0], E = par[1], nu = par[2], dt = par[3];
@@ -75,6 +84,33 @@ int main(int argc, char **argv) {
     nm.updateMeshVerticesDisplacements(mesh);
     std::vector<Scalar> vel(3 * (size_t)nNodes);
     nm.getMeshVerticesVelocities(mesh, vel.data());
+    std::vector<Scalar> nmDisp(mesh.getDisplacements(), mesh.getDisplacements() + 3 * (size_t)nNodes);
+
+    // backward Euler with a plane collision damping forcer: every step clears the damping
+    // matrix, sets the pairs (setCollisionPair, then addCollisionPair) and adds it
+    auto pairPlane = [&](int k) {
+        odgpu::StaticPlaneShape pl;
+        for (int a = 0; a < 3; a++) pl.planeNormal[a] = planes[4 * (size_t)k + a];
+        pl.planeConstant = planes[4 * (size_t)k + 3];
+        return pl;
+    };
+    odgpu::MeshPlaneCollisionDampingForcer forcer(coeff);
+    nm.addExternalDampingForceMatrix(forcer);             // Newmark: reported unimplemented
+    if (nm.lastStatus() != OD_ERR_UNSUPPORTED) return 12;
+    odgpu::NonlinearImplicitBackwardEuler be(mesh, material, 0.0, 0.0, dt, 1e-6, 10000, ic0);
+    if (be.lastStatus()) return 13;
+    be.setExternalVirtualWork(g.data());
+    for (int s = 0; s < steps; s++) {
+        be.clearExternalDampingForceMatrix();
+        forcer.setCollisionPair(pairPlane(0), pairTets[0]);
+        for (int k = 1; k < nPairs; k++) forcer.addCollisionPair(pairPlane(k), pairTets[k]);
+        be.addExternalDampingForceMatrix(forcer);
+        be.runOneTimeStep();
+        be.updateMeshVerticesDisplacements(mesh);
+        if (be.lastStatus()) return 14;
+    }
+    std::vector<Scalar> beVel(3 * (size_t)nNodes);
+    be.getMeshVerticesVelocities(mesh, beVel.data());
 
     FILE *out = std::fopen(argv[2], "wb");
     if (!out) { std::perror("output"); return 2; }
@@ -85,8 +121,9 @@ int main(int argc, char **argv) {
     wr(out, y.data(), y.size());
     wr(out, sol.data(), sol.size());
     wr(out, &cgIters, 1);
-    wr(out, mesh.getDisplacements(), 3 * (size_t)nNodes);
+    wr(out, nmDisp.data(), nmDisp.size());
     wr(out, vel.data(), vel.size());
+    wr(out, beVel.data(), beVel.size());
     std::fclose(out);
     std::printf("host_api_check: ndof=%d values=%lld cg_iters=%d steps=%d\n", ndof, nv, cgIters, steps);
     return 0;

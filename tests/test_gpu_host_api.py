@@ -1,5 +1,6 @@
 """The C++ host classes (include/od_gpu.hpp), used as reference-style host code
-uses CGSolver, SpMDV, the materials and the Intergrator, against the oracle.
+uses CGSolver, SpMDV, the materials, the Intergrator and a plane collision damping
+forcer, against the oracle.
 
 tests/cpp/host_api_check is built by the package build; this test writes its
 input file, runs it and compares every output it writes.
@@ -36,6 +37,12 @@ def test_host_classes_match_oracle(tmp_path):
     g = s.body_force(w.body)
     steps = 12
     rho, E, nu = float(w.rho[0]), float(w.E[0]), float(w.nu[0])
+    # candidate pairs: the tets within 5 cm of a tilted floor 1 cm under the beam (the
+    # sagging beam reaches it after a few steps; pairs not yet touching add nothing)
+    nrm = np.array([0.02, 0.01, 1.0]); nrm /= np.linalg.norm(nrm)
+    planes, ptets = W.plane_contacts(w.X, w.tets, nrm, 0.01 - 0.05)
+    planes[:, 3] = 0.01
+    coeff = 2e6
     inp = tmp_path / "in.bin"; out = tmp_path / "out.bin"
     with open(inp, "wb") as f:
         np.array([nn, w.tets.shape[0], len(w.fixed_dofs), n, steps], np.int32).tofile(f)
@@ -45,6 +52,10 @@ def test_host_classes_match_oracle(tmp_path):
         np.array([rho, E, nu, w.dt], np.float64).tofile(f)
         for v in (u_node, x, y0, b, g):
             np.asarray(v, np.float64).tofile(f)
+        np.array([len(ptets)], np.int32).tofile(f)
+        planes.astype(np.float64).ravel().tofile(f)
+        ptets.astype(np.int32).tofile(f)
+        np.array([coeff], np.float64).tofile(f)
     print("host_api_check on box_small", flush=True)
     r = subprocess.run([BIN, str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -58,7 +69,7 @@ def test_host_classes_match_oracle(tmp_path):
         return a
 
     K = take(nv); vws = take(n); y = take(n); sol = take(n); iters = int(take(1, np.int32)[0])
-    d_node = take(3 * nn); v_node = take(3 * nn)
+    d_node = take(3 * nn); v_node = take(3 * nn); v_be = take(3 * nn)
 
     s.set_free_displacements(u_free)
     Kc, fc = s.assemble(w.kind, w.E, w.nu)
@@ -77,3 +88,16 @@ def test_host_classes_match_oracle(tmp_path):
     assert relerr(d_node[free], do) < 1e-6
     assert relerr(v_node[free], vo) < 1e-6
     assert np.all(d_node[np.asarray(w.fixed_dofs)] == 0)
+
+    s2 = O.System(w.X, w.tets, w.fixed_dofs)
+    be = O.BackwardEuler(s2, w.kind, w.E, w.nu, w.rho, w.dt, pre=O.PRE_IC0, p1=1e-3)
+    be.set_external(g)
+    touched = 0
+    for _ in range(steps):
+        be.clear_damping()
+        be.add_plane_damping(planes, ptets, coeff)
+        touched += int(be.damping().any())
+        be.step(); be.update_mesh()
+    assert touched >= 3
+    assert relerr(v_be[free], be.velocity()) < 1e-6
+    assert np.all(v_be[np.asarray(w.fixed_dofs)] == 0)
