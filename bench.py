@@ -182,6 +182,67 @@ def serving_leg(args, n_obj=64, tet_lo=1000, tet_hi=50000, rates=(50.0, 200.0), 
             "streams": legs}
 
 
+def damping_leg(ctx, mesh, w, calls=20):
+    """Plane collision damping on the main mesh: a floor tilted by (0.01, 0.005) under the box
+    cuts its bottom cells; every tet with a vertex below it is a (plane, tet) pair, and all
+    nodes move down at 1 m/s.  Times od_plane_damping (pair upload, the four damping kernels,
+    the synchronisation) into a fresh matrix, median of `calls` calls after one warm-up."""
+    import opendeformer_amd as od
+    from opendeformer_amd import workloads as W
+    n = np.array([0.01, 0.005, 1.0]); n /= np.linalg.norm(n)
+    planes, tets = W.plane_contacts(w.X, w.tets, n, -0.06)
+    vel = od.DeviceArray.from_host(ctx, np.tile([0.0, 0.0, -1.0], mesh.ndof // 3))
+    D = od.Matrix(mesh)
+    D.add_plane_damping(planes, tets, 1e5, None, vel.ptr)
+    ctx.sync()
+    ts = []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        D.add_plane_damping(planes, tets, 1e5, None, vel.ptr)
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    return {"workload": f"floor under {w.name}: {len(tets)} (plane, tet) pairs, nodes moving into it",
+            "pairs": int(len(tets)), "ms_per_call_median": round(med * 1e3, 3),
+            "pairs_per_s": round(len(tets) / med, 1), "calls": calls,
+            "note": "per call: upload of 36 B per pair, k_damp_eval, k_damp_offsets, k_damp_fill, "
+                    "k_damp_apply, stream synchronisation"}
+
+
+def ordering_leg(args, warmup=1, steps=3):
+    """The 1M-DOF workload under other node numberings, in the precision form: the natural
+    (x-slowest grid) order and the reference's GPS relabelling (od_relabel_gps, host code);
+    IC(0) dependency levels, PCG iterations and step time of each."""
+    import opendeformer_amd as od
+    from opendeformer_amd import capi
+    from opendeformer_amd import workloads as W
+    out = {}
+    for ordering in ("natural", "gps"):
+        t0 = time.perf_counter()
+        w = W.box_1m_dof(ordering)
+        t_host = time.perf_counter() - t0
+        ctx = od.Context(0)
+        mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
+        mesh.set_material(w.kind, w.E, w.nu, w.rho)
+        od.mesh_autotune(mesh)
+        nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=capi.OD_MIXED, tol=1e-6)
+        nm.set_external_virtual_work(mesh.body_force(w.body))
+        nm.run_time_steps(warmup)
+        ctx.sync()
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            nm.run_one_time_step()
+            its.append(int(nm.stats()[0][0]))
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / steps
+        out[ordering] = {"ic0_levels": mesh.n_levels, "pcg_iters_per_step": round(float(np.mean(its)), 2),
+                         "ms_per_step": round(dt * 1e3, 2), "steps_per_s": round(1.0 / dt, 3),
+                         "mesh_build_and_relabel_s": round(t_host, 2)}
+        del nm, mesh, ctx
+    return {"workload": "the 1M-DOF box, mixed precision, IC(0)-PCG to 1e-6", "warmup": warmup, "steps": steps,
+            **out}
+
+
 def cpu_baseline(args, gpu_iters_per_step, full_ndof):
     """Times the FP64 oracle on a slab of the same workload (single thread)."""
     import oracle as O
@@ -228,15 +289,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precond", choices=["jacobi", "ic0"], default="ic0")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
-    ap.add_argument("--ordering", choices=["natural", "color8", "color5"], default="color5",
-                    help="node numbering: natural (x-slowest grid order) or grouped by an 8- or 5-colouring "
-                         "(5 colours: 5 IC(0) dependency levels)")
+    ap.add_argument("--ordering", choices=["natural", "color8", "color5", "gps"], default="color5",
+                    help="node numbering: natural (x-slowest grid order), grouped by an 8- or 5-colouring "
+                         "(5 colours: 5 IC(0) dependency levels), or the GPS relabeller's")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = 200x40x40)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true",
                     help="with --precision mixed: skip the plain FP64 path run (speed-up and error report)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--no-damping", action="store_true", help="skip the plane collision damping leg")
+    ap.add_argument("--no-orderings", action="store_true",
+                    help="skip the natural and GPS node-numbering comparison leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the run, launch a streaming update of known size three times (the reference "
                          "that scales HBM byte counters)")
@@ -329,6 +393,13 @@ def main():
                              "spmv": fp64_r["spmv"]}
         line["speedup_vs_fp64"] = round(main_r["steps_s"] / fp64_r["steps_s"], 3)
         line["rel_err_vs_fp64_path"] = {"value": err, "at": f"displacement after step {args.warmup + args.steps}"}
+    if not args.no_damping:
+        line["plane_damping"] = damping_leg(ctx, mesh, w)
+    if not args.no_orderings:
+        line["orderings"] = ordering_leg(args)
+        line["orderings"]["color5"] = {"ic0_levels": mesh.n_levels, "pcg_iters_per_step": round(main_r["iters"], 2),
+                                       "ms_per_step": round(main_r["ms_step"], 2),
+                                       "steps_per_s": round(main_r["steps_s"], 3)}
     if not args.no_batch:
         line["mid_neo_precision"] = mid_neo_leg(args)
         line["batched"] = batched_leg(args)

@@ -61,6 +61,17 @@ int od_copy_h2d(od_ctx *ctx, void *dev, const void *host, long long bytes);
 int od_copy_d2h(od_ctx *ctx, void *host, const void *dev, long long bytes);
 int od_memset_d(od_ctx *ctx, void *dev, int byte, long long bytes);
 
+/* ---- node relabelling (host only: no context, no device) -------------------
+ * MeshRelabeler::getNewLables(elementCount, nodePerElementCount, newNodeLables,
+ * elements) (core/meshRelabeler.h:113, core/meshRelabler.cpp:57-498): the
+ * Gibbs-Poole-Stockmeyer labelling of the mesh graph.  newLabels_host[node] is
+ * the node's new label; elements_host (nElems x nodesPerElem) is rewritten with
+ * the new labels, sorted by the sum of each element's labels.  Orders the
+ * reference leaves to its standard library are fixed: edges in first-insertion
+ * order, stable sorts, equal degrees in push order.  A mesh whose graph is not
+ * connected is rejected with OD_ERR_ARG. */
+int od_relabel_gps(int nNodes, int nElems, int nodesPerElem, int *elements_host, int *newLabels_host);
+
 /* ---- mesh, DOF indexing and the canonical pattern --------------------------
  * Replaces: Mesh/TetMesh set-up (core/mesh.cpp:7-40), NodeIndexer
  * (core/nodeIndexer.cpp:8-74), FullOrderNonlinearMaterial::getMatrixStructure

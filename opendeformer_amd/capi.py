@@ -35,7 +35,7 @@ EXPORTS = [
     "od_solver_set_graph", "od_integrator_set_graph", "od_mesh_bind_ctx", "od_integrator_device_bytes",
     "od_server_create", "od_server_destroy", "od_server_add_object", "od_server_run", "od_server_object_state",
     "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain", "od_plane_damping",
-    "od_integrator_add_plane_damping", "od_integrator_clear_damping", "od_integrator_damping",
+    "od_integrator_add_plane_damping", "od_integrator_clear_damping", "od_integrator_damping", "od_relabel_gps",
 ]
 
 
@@ -86,6 +86,7 @@ def lib():
             "od_backward_euler_update_mesh": [vp], "od_integrator_positions": [vp, C.POINTER(C.c_float)],
             "od_plane_damping": [vp, i, dp, ip, d, vp, vp], "od_integrator_add_plane_damping": [vp, i, dp, ip, d],
             "od_integrator_clear_damping": [vp], "od_integrator_damping": [vp, dp],
+            "od_relabel_gps": [i, i, i, ip, ip],
             "od_spmv_timed": [vp, vp, vp, i, i, dp], "od_integrator_matrix": [vp, pvp],
             "od_ctx_profile": [vp, i], "od_ctx_profile_read": [vp, dp, C.POINTER(ll)],
         }

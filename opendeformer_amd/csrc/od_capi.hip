@@ -1387,6 +1387,14 @@ int od_backward_euler_update_mesh(od_integrator *it) {
     API_END
 }
 
+// ---- mesh relabelling (host only) ----------------------------------------------------
+int od_relabel_gps(int nNodes, int nElems, int nodesPerElem, int *elements, int *newLabels) {
+    API_BEGIN
+    const char *err = od::relabelGps(nNodes, nElems, nodesPerElem, elements, newLabels);
+    require(err == nullptr, err ? err : "");
+    API_END
+}
+
 // ---- plane collision damping ---------------------------------------------------------
 static void plane_damping(od_mesh *mesh, double *D, int nPairs, const double *planes, const int *tets, double coeff,
                           const double *disp, const double *vel) {

@@ -65,6 +65,10 @@ struct Topology {
 // Builds every derived structure.  Returns an empty string or an error text.
 std::string buildTopology(Topology &t, int chunkRows, int nXcd = 8);
 
+// GPS relabelling (od_relabel.cpp): new label per node, elements relabelled and sorted by
+// label sum in place; returns an error message or nullptr.
+const char *relabelGps(int nNodes, int nElems, int npe, int *elements, int *labels);
+
 // Global DOF of (node, axis): -1 when constrained (nodeIndexer.cpp:12-25).
 int globalIndex(const Topology &t, int node, int axis);
 

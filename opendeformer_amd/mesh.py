@@ -183,6 +183,25 @@ def multicolor_permutation(mesh: TetMesh, colors: int = 8) -> np.ndarray:
     return perm
 
 
+def gps_relabel(mesh: TetMesh) -> TetMesh:
+    """The mesh renumbered by the Gibbs-Poole-Stockmeyer relabeller (od_relabel_gps, host code
+    in the native library; MeshRelabeler::getNewLables, core/meshRelabler.cpp:468-498): new node
+    labels, and the tets in the relabeller's order (sorted by the sum of their labels)."""
+    import ctypes as C
+    from . import capi
+    tets = np.ascontiguousarray(mesh.tets, dtype=np.int32).copy()
+    lab = np.zeros(mesh.n_nodes, np.int32)
+    rc = capi.lib().od_relabel_gps(mesh.n_nodes, tets.shape[0], tets.shape[1], tets.ctypes.data_as(C.POINTER(C.c_int)),
+                                   lab.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != capi.OD_OK:
+        raise ValueError(capi.lib().od_last_error().decode())
+    perm = lab.astype(np.int64)
+    X = np.empty_like(mesh.X)
+    X[perm] = mesh.X
+    return TetMesh(X=X, tets=tets, facets=perm[mesh.facets].astype(np.int32), shape=mesh.shape, size=mesh.size,
+                   extra=dict(mesh.extra, perm=perm))
+
+
 def plane_fixed_nodes(mesh: TetMesh, axis: int = 0, value: float = 0.0, eps: float = 1e-4) -> np.ndarray:
     """Nodes of boundary facets on the plane X[axis] = value (constrainer.cpp:57-72 rule)."""
     v = np.unique(mesh.facets.ravel())

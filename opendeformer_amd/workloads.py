@@ -84,6 +84,12 @@ def _single(name, m: M.TetMesh, kind, E, nu, dt, steps, ordering="natural", impu
             imp = np.zeros_like(impulse)
             imp[perm] = impulse
             impulse = imp
+    elif ordering == "gps":
+        m = M.gps_relabel(m)
+        if impulse is not None:
+            imp = np.zeros_like(impulse)
+            imp[m.extra["perm"]] = impulse
+            impulse = imp
     elif ordering != "natural":
         raise ValueError(f"unknown ordering {ordering}")
     fixed = M.plane_fixed_nodes(m, axis=0, value=0.0)

@@ -84,6 +84,7 @@ def lib():
         L.oro_be_set_velocity.argtypes = [vp, dp]
         L.oro_plane_tet_intersection.argtypes = [dp, dp, dp, dp]
         L.oro_barycentric.argtypes = [dp, dp, dp]
+        L.oro_gps.argtypes = [i, i, i, ip, ip]
         _LIB = L
     return _LIB
 
@@ -343,3 +344,14 @@ def barycentric(pts, q):
     out = np.zeros(4)
     lib().oro_barycentric(_d(f64(np.asarray(pts).reshape(12))), _d(f64(q)), _d(out))
     return out
+
+
+def gps(n_nodes, elements):
+    """MeshRelabeler::getNewLables (core/meshRelabler.cpp:468-498): (new label per node,
+    elements relabelled and sorted by label sum)."""
+    el = i32(np.array(elements, np.int32, copy=True))
+    lab = np.zeros(n_nodes, np.int32)
+    rc = lib().oro_gps(n_nodes, el.shape[0], el.shape[1], _i(el), _i(lab))
+    if rc != 0:
+        raise ValueError("mesh graph is not connected")
+    return lab, el

@@ -14,7 +14,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <climits>
+#include <list>
 #include <map>
+#include <queue>
 #include <set>
 #include <unordered_map>
 #include <vector>
@@ -1693,6 +1696,232 @@ static Materials makeMaterials(int nTets, const int *kind, const Scalar *E, cons
     return m;
 }
 
+// ---------------------------------------------------------------------------
+// GPS node relabelling — core/meshRelabler.cpp:57-530, core/meshRelabeler.h:16-135
+//
+// The reference leaves four orders to its standard library; the restatement fixes them:
+//  (1) edges are visited in the order they are first met (element order, then node pairs
+//      j < k), where the reference iterates an unordered_set (meshRelabler.cpp:131-145);
+//      each vertex's arc list is built by prepending, as insertArc does (:10-16);
+//  (2) std::sort of a level by degree (meshRelabeler.h:85-89) keeps equal degrees in
+//      their order (stable);
+//  (3) the max-degree priority queue (DegreeComparer, meshRelabeler.h:59-63) pops equal
+//      degrees in push order;
+//  (4) the sorts of components by size (:334-336) and of elements by label sum (:496)
+//      are stable.
+// A disconnected mesh has no level structure over all nodes; it is rejected (the
+// reference leaves such nodes unnumbered).
+// ---------------------------------------------------------------------------
+struct GpsGraph {
+    std::vector<std::list<int>> adj;   // arc lists (front = last inserted)
+    int degree(int v) const { return (int)adj[v].size(); }
+};
+
+// RootedLevelStructure::generateLevels (meshRelabler.cpp:57-95): levels[0] = neighbours
+// of the root in arc order, then breadth-first in queue order
+struct GpsLevels {
+    int root = -1;
+    std::vector<std::vector<int>> levels;
+    int levelSize() const { return (int)levels.size() + 1; }
+    int width() const {
+        int w = INT_MIN;
+        for (auto &l : levels) w = std::max(w, (int)l.size());
+        return w;
+    }
+    void generate(const GpsGraph &g, int r, std::vector<char> &visited) {
+        root = r;
+        levels.clear();
+        visited[r] = 1;
+        std::queue<int> q;
+        levels.emplace_back();
+        for (int u : g.adj[r]) { levels[0].push_back(u); q.push(u); visited[u] = 1; }
+        int lv = 0;
+        while (!q.empty()) {
+            lv++;
+            levels.emplace_back();
+            int n = (int)q.size();
+            for (int i = 0; i < n; i++) {
+                int v = q.front(); q.pop();
+                for (int u : g.adj[v])
+                    if (!visited[u]) { levels[lv].push_back(u); q.push(u); visited[u] = 1; }
+            }
+        }
+        levels.pop_back();
+    }
+};
+
+// MeshRelabeler::processSigleVert (:148-172)
+static void gpsProcess(const GpsGraph &g, int v, std::list<int> &levelNodes, std::queue<int> &working,
+                       std::vector<int> &label, int &next) {
+    label[v] = next++;
+    working.push(v);
+    std::vector<int> toBe;                       // priority queue: max degree, ties in push order
+    for (int u : g.adj[v]) {
+        auto f = std::find(levelNodes.begin(), levelNodes.end(), u);
+        if (f != levelNodes.end()) { toBe.push_back(u); levelNodes.erase(f); }
+    }
+    while (!toBe.empty()) {
+        size_t best = 0;
+        for (size_t i = 1; i < toBe.size(); i++)
+            if (g.degree(toBe[i]) > g.degree(toBe[best])) best = i;
+        int u = toBe[best];
+        toBe.erase(toBe.begin() + best);
+        label[u] = next++;
+        working.push(u);
+    }
+}
+
+// MeshRelabeler::setNewNodeLables (:174-466); returns false for a disconnected graph
+static bool gpsLabels(GpsGraph &g, std::vector<int> &out) {
+    const int n = (int)g.adj.size();
+    std::vector<char> visited(n, 0);
+    int start = 0;
+    for (int v = 1; v < n; v++)
+        if (g.degree(v) < g.degree(start)) start = v;
+    GpsLevels startL, endL, mayL;
+    startL.generate(g, start, visited);
+    {
+        int reached = 1;
+        for (auto &l : startL.levels) reached += (int)l.size();
+        if (reached != n) return false;
+    }
+    bool found = false;
+    while (!found) {
+        found = true;
+        int endWidth = INT_MAX;
+        int last = startL.levelSize() - 1;
+        std::vector<int> &lastLevel = startL.levels[last - 1];
+        std::stable_sort(lastLevel.begin(), lastLevel.end(),
+                         [&](int a, int b) { return g.degree(a) < g.degree(b); });
+        for (size_t k = 0; k < lastLevel.size(); k++) {
+            std::fill(visited.begin(), visited.end(), 0);
+            mayL.generate(g, lastLevel[k], visited);
+            if (mayL.levelSize() > startL.levelSize()) {
+                found = false;
+                std::swap(startL, mayL);
+                break;
+            }
+            int w = mayL.width();
+            if (w < endWidth) { std::swap(endL, mayL); endWidth = w; }
+        }
+    }
+    // minimising level width (:224-373)
+    const int startSize = startL.levelSize(), endSize = endL.levelSize();
+    const int size = std::max(startSize, endSize);
+    const int startWidth = startL.width();
+    const bool startSmaller = endL.width() > startWidth;
+    std::vector<int> sLev(n, 0), eLev(n, INT_MIN);
+    sLev[startL.root] = 0;
+    for (int i = 1; i < startSize; i++)
+        for (int v : startL.levels[i - 1]) sLev[v] = i;
+    std::vector<int> deleted;
+    std::queue<int> remained;
+    std::vector<std::list<int>> newLevels(size);
+    auto place = [&](int v, int e) {
+        eLev[v] = e;
+        if (sLev[v] == e) { deleted.push_back(v); newLevels[e].push_back(v); }
+        else remained.push(v);
+    };
+    int e = endSize - 1;
+    place(endL.root, e);
+    for (int i = 1; i < endSize; i++) {
+        e--;
+        for (int v : endL.levels[i - 1]) place(v, e);
+    }
+    // deleteArc of every deleted vertex from its neighbours' lists (:285-293)
+    for (int v : deleted)
+        for (int u : g.adj[v]) {
+            auto f = std::find(g.adj[u].begin(), g.adj[u].end(), v);
+            if (f != g.adj[u].end()) g.adj[u].erase(f);
+        }
+    std::fill(visited.begin(), visited.end(), 0);
+    std::vector<std::vector<int>> comps;
+    std::queue<int> working;
+    while (!remained.empty()) {
+        int v = remained.front();
+        if (!visited[v]) {
+            comps.emplace_back();
+            visited[v] = 1;
+            comps.back().push_back(v);
+            for (int u : g.adj[v]) working.push(u);
+            while (!working.empty()) {
+                int a = working.front();
+                if (!visited[a]) {
+                    visited[a] = 1;
+                    comps.back().push_back(a);
+                    for (int u : g.adj[a]) working.push(u);
+                }
+                working.pop();
+            }
+        }
+        remained.pop();
+    }
+    std::stable_sort(comps.begin(), comps.end(),
+                     [](const std::vector<int> &a, const std::vector<int> &b) { return a.size() < b.size(); });
+    bool selectSecond = false, interchange = false;
+    std::vector<int> c1(size), c2(size);
+    for (size_t i = 0; i < comps.size(); i++) {
+        std::fill(c1.begin(), c1.end(), 0);
+        std::fill(c2.begin(), c2.end(), 0);
+        for (int v : comps[i]) { c1[sLev[v]]++; c2[eLev[v]]++; }
+        int m1 = *std::max_element(c1.begin(), c1.end()), m2 = *std::max_element(c2.begin(), c2.end());
+        if (i == 0 && (m1 > m2 || (m1 == m2 && !startSmaller))) selectSecond = true;
+        bool useStart = m1 < m2 ? true : (m1 > m2 ? false : startSmaller);
+        for (int v : comps[i]) newLevels[useStart ? sLev[v] : eLev[v]].push_back(v);
+    }
+    // restore the arcs: insertArc prepends (:375-383)
+    for (int v : deleted)
+        for (int u : g.adj[v]) g.adj[u].push_front(v);
+    for (auto &l : newLevels) {                  // std::list::sort is stable
+        std::vector<int> tmp(l.begin(), l.end());
+        std::stable_sort(tmp.begin(), tmp.end(), [&](int a, int b) { return g.degree(a) < g.degree(b); });
+        l.assign(tmp.begin(), tmp.end());
+    }
+    // numbering (:389-457)
+    int root = startL.root, init = 0, pace = 1;
+    if (g.degree(startL.root) > g.degree(endL.root)) { interchange = true; root = endL.root; init = size - 1; pace = -1; }
+    if (sLev[root] == eLev[root]) newLevels[init].remove(root);
+    else { newLevels[sLev[root]].remove(root); newLevels[eLev[root]].remove(root); }
+    std::vector<int> label(n, -1);
+    int next = 0;
+    std::queue<int> work;
+    int level = init;
+    gpsProcess(g, root, newLevels[level], work, label, next);
+    while (!newLevels[level].empty()) {
+        int v = newLevels[level].front();
+        newLevels[level].pop_front();
+        gpsProcess(g, v, newLevels[level], work, label, next);
+    }
+    for (level = init + pace; level >= 0 && level < size; level += pace) {
+        bool unfound = true;
+        int process = -1;
+        while (unfound && !work.empty()) {
+            int v = work.front();
+            for (int u : g.adj[v])
+                if (std::find(newLevels[level].begin(), newLevels[level].end(), u) != newLevels[level].end()) {
+                    unfound = false;
+                    if (process < 0 || g.degree(process) > g.degree(u)) process = u;
+                }
+            work.pop();
+        }
+        std::queue<int>().swap(work);
+        if (process < 0) {                       // a level no earlier vertex reaches (not in a connected mesh)
+            if (newLevels[level].empty()) continue;
+            process = newLevels[level].front();
+        }
+        newLevels[level].remove(process);
+        gpsProcess(g, process, newLevels[level], work, label, next);
+        while (!newLevels[level].empty()) {
+            int v = newLevels[level].front();
+            newLevels[level].pop_front();
+            gpsProcess(g, v, newLevels[level], work, label, next);
+        }
+    }
+    out.resize(n);
+    for (int v = 0; v < n; v++) out[v] = (interchange != selectSecond) ? label[v] : (n - 1) - label[v];
+    return true;
+}
+
 }  // namespace oro
 
 using namespace oro;
@@ -2005,6 +2234,43 @@ void oro_barycentric(const double *pts12, const double *q, double *coords) {
     for (int i = 0; i < 4; i++) p[i] = Vec3(pts12[3 * i], pts12[3 * i + 1], pts12[3 * i + 2]);
     barycentric(p, Vec3(q[0], q[1], q[2]), coords);
 }
+// MeshRelabeler::getNewLables(elementCount, nodePerElementCount, newNodeLables, elements)
+// (meshRelabler.cpp:468-498): new label per node, and the elements relabelled and sorted by
+// the sum of their labels.  Returns -1 for a disconnected mesh.
+int oro_gps(int nNodes, int nElems, int npe, int *elements, int *labels) {
+    GpsGraph g;
+    g.adj.resize(nNodes);
+    std::set<std::pair<int, int>> seen;
+    for (int e = 0; e < nElems; e++) {
+        const int *el = elements + (size_t)e * npe;
+        for (int j = 0; j < npe; j++)
+            for (int k = j + 1; k < npe; k++) {
+                int a = std::min(el[j], el[k]), b = std::max(el[j], el[k]);
+                if (a == b || !seen.insert({a, b}).second) continue;
+                g.adj[a].push_front(b);
+                g.adj[b].push_front(a);
+            }
+    }
+    std::vector<int> lab;
+    if (!gpsLabels(g, lab)) return -1;
+    std::copy(lab.begin(), lab.end(), labels);
+    std::vector<std::pair<long long, int>> order(nElems);
+    std::vector<int> relab((size_t)nElems * npe);
+    for (int e = 0; e < nElems; e++) {
+        long long score = 0;
+        for (int j = 0; j < npe; j++) {
+            relab[(size_t)e * npe + j] = lab[elements[(size_t)e * npe + j]];
+            score += relab[(size_t)e * npe + j];
+        }
+        order[e] = {score, e};
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<long long, int> &a, const std::pair<long long, int> &b) { return a.first < b.first; });
+    for (int i = 0; i < nElems; i++)
+        std::memcpy(elements + (size_t)i * npe, &relab[(size_t)order[i].second * npe], sizeof(int) * npe);
+    return 0;
+}
+
 void oro_get_node_displacements(void *sys, double *u) {
     System *s = (System *)sys;
     for (int i = 0; i < s->nNodes; i++) { u[3 * i] = s->disp[i].x; u[3 * i + 1] = s->disp[i].y; u[3 * i + 2] = s->disp[i].z; }
