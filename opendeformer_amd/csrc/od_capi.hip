@@ -86,6 +86,7 @@ struct od_mesh {
     bool spmvTuned = false;                      // SpMV variant chosen (autotuner or caller)
     double tuneMs[2][3] = {{0, 0, 0}, {0, 0, 0}};   // [FP64, FP32][R = 1, 2, 4]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
+    std::vector<uint8_t> levelWideFwd, levelWideBwd;   // DevMesh points at these (host memory)
     template <typename T>
     T *upload(const std::vector<T> &v) {
         if (v.empty()) return nullptr;
@@ -701,9 +702,9 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.rowCStart = mesh->upload(t.rowCStart);
         m.levelRows = mesh->upload(t.levelRows);
         m.rowLevel = mesh->upload(t.level);
-        for (int l = 0; l < DevMesh::kMaxLevelFlags; l++) {
-            m.levelWideFwd[l] = m.levelWideBwd[l] = 0;
-            if (l >= t.nLevels) continue;
+        mesh->levelWideFwd.assign(std::max(1, t.nLevels), 0);
+        mesh->levelWideBwd.assign(std::max(1, t.nLevels), 0);
+        for (int l = 0; l < t.nLevels; l++) {
             double nT = 0, nU = 0;
             for (int q = t.levelPtr[l]; q < t.levelPtr[l + 1]; q++) {
                 const int I = t.levelRows[q];
@@ -711,9 +712,11 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
                 nU += t.rowPtr[I + 1] - t.rowPtr[I] - 1;
             }
             const double cnt = std::max(1, t.levelPtr[l + 1] - t.levelPtr[l]);
-            m.levelWideFwd[l] = nT / cnt > 8.0;
-            m.levelWideBwd[l] = nU / cnt > 8.0;
+            mesh->levelWideFwd[l] = nT / cnt > 8.0;
+            mesh->levelWideBwd[l] = nU / cnt > 8.0;
         }
+        m.levelWideFwd = mesh->levelWideFwd.data();
+        m.levelWideBwd = mesh->levelWideBwd.data();
         m.chunkRow0 = mesh->upload(t.chunkRow0);
         m.chunkRow1 = mesh->upload(t.chunkRow1);
         m.chunkInst = mesh->upload(t.chunkInst);

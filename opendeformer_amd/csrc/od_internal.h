@@ -47,10 +47,9 @@ struct DevMesh {
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
     int *levelRows;                              // nRows
     int *rowLevel;                               // IC(0) dependency level of each row
-    // per level (host-side flags read at launch): rows average more than 8 blocks on the side
-    // the forward / backward sweep reads (run 16 lanes per row instead of 8)
-    static constexpr int kMaxLevelFlags = 64;
-    uint8_t levelWideFwd[kMaxLevelFlags], levelWideBwd[kMaxLevelFlags];
+    // per level, in host memory (read at launch, never on the device): rows average more than
+    // 8 blocks on the side the forward / backward sweep reads (run 16 lanes per row instead of 8)
+    const uint8_t *levelWideFwd, *levelWideBwd;
     int *chunkRow0, *chunkRow1, *chunkInst;      // nChunks
     int *chunkSorted;     // chunks in spatial order, cut into 8 runs of perRun (one run per XCD)
     int perRun;
