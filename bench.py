@@ -78,7 +78,7 @@ def mid_neo_leg(args, steps=200):
     import opendeformer_amd as od
     from opendeformer_amd import capi
     from opendeformer_amd import workloads as W
-    w = W.box_mid_neo("color5", seed=2)
+    w = W.box_mid_neo("color4", seed=2)
     ctx = od.Context(0)
     mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
@@ -103,7 +103,7 @@ def mid_neo_leg(args, steps=200):
         del nm
     errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(traj["mixed"], traj["fp64"])]
     return {"workload": f"{w.name}: {mesh.n_tets} tets, {mesh.ndof} DOF, tip impulse, dt=1/120, {steps} steps "
-                        f"(5-colour ordering; the timing includes a state read-back every 20 steps)",
+                        f"(4-colour ordering; the timing includes a state read-back every 20 steps)",
             "fp64": out["fp64"], "mixed": out["mixed"],
             "mixed_rel_err_vs_fp64_max_over_sampled_steps": max(errs)}
 
@@ -116,7 +116,7 @@ def batched_leg(args, n_inst=256, warmup=2, steps=20):
     import opendeformer_amd as od
     from opendeformer_amd import capi
     from opendeformer_amd import workloads as W
-    w = W.batch(n_inst, seed=4, ordering="color5")
+    w = W.batch(n_inst, seed=4, ordering="color4")
     ctx = od.Context(0)
     mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs, w.inst_node_start, w.inst_tet_start)
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
@@ -155,7 +155,7 @@ def batched_leg(args, n_inst=256, warmup=2, steps=20):
         del ni, mi
     alone_rate = alone_n / alone_t
     return {"workload": f"{n_inst} boxes, {mesh.n_tets} tets, {mesh.ndof} DOF in one concatenated system "
-                        f"(5-colour ordering per box), mixed precision, graph replay",
+                        f"(4-colour ordering per box), mixed precision, graph replay",
             "ms_per_step": round(dt * 1e3, 3), "batch_steps_per_s": round(1.0 / dt, 3),
             "instance_steps_per_s": round(n_inst / dt, 1), "steps": steps, "warmup": warmup,
             "pcg_iters_per_instance_step": {"median": float(np.median(its)), "max": int(its.max()),
@@ -214,9 +214,9 @@ def log(msg):
 
 
 def ordering_leg(args, scale=0.5, max_iter=2000,
-                 plan=(("natural", 1, 2), ("color5", 1, 2), ("gps", 0, 1))):
+                 plan=(("natural", 1, 2), ("color4", 1, 2), ("color5", 1, 2), ("gps", 0, 1))):
     """The box workload at half size (100x20x20 cells, 132,300 DOF) under three node numberings:
-    natural (x-slowest grid order), the 5-colouring, and the reference's GPS relabelling
+    natural (x-slowest grid order), the 4- and 5-colourings, and the reference's GPS relabelling
     (od_relabel_gps, host code).  FP64 IC(0)-PCG to 1e-6 with at most `max_iter` iterations;
     IC(0) dependency levels, PCG iterations and step time of each.  The GPS numbering gives
     about 1,400 dependency levels here and hundreds of iterations per step, so it is timed over
@@ -266,8 +266,10 @@ def cpu_baseline(args, gpu_iters_per_step, full_ndof):
     from opendeformer_amd import mesh as M
     nx = 20
     m = M.box_mesh(nx, 40, 40, 5.0 * nx / 200, 1.0, 1.0, seed=3)
-    if args.ordering in ("color8", "color5"):
-        m = M.relabel(m, M.multicolor_permutation(m, 8 if args.ordering == "color8" else 5))
+    if args.ordering in ("color8", "color5", "color4"):
+        m = M.relabel(m, M.multicolor_permutation(m, int(args.ordering[len("color"):])))
+    elif args.ordering == "gps":
+        m = M.gps_relabel(m)
     E = M.log_uniform(3, M.STREAM_YOUNG, np.arange(m.n_tets, dtype=np.uint64), 1e4, 1e6)
     fixed = M.fixed_dofs_from_nodes(M.plane_fixed_nodes(m))
     t0 = time.perf_counter()
@@ -304,9 +306,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precond", choices=["jacobi", "ic0"], default="ic0")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
-    ap.add_argument("--ordering", choices=["natural", "color8", "color5", "gps"], default="color5",
-                    help="node numbering: natural (x-slowest grid order), grouped by an 8- or 5-colouring "
-                         "(5 colours: 5 IC(0) dependency levels), or the GPS relabeller's")
+    ap.add_argument("--ordering", choices=["natural", "color8", "color5", "color4", "gps"], default="color4",
+                    help="node numbering: natural (x-slowest grid order), grouped by an 8-, 5- or 4-colouring "
+                         "(4 colours: 4 IC(0) dependency levels), or the GPS relabeller's")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = 200x40x40)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true",

@@ -162,8 +162,10 @@ def multicolor_permutation(mesh: TetMesh, colors: int = 8) -> np.ndarray:
     The 7 edge directions of the Kuhn split are the non-zero 0/1 vectors
     (a, b, c).  With 8 colours, (i + 2j + 4k) mod 8 is proper: the direction
     codes a + 2b + 4c are 1..7.  With 5 colours, (i + j + 2k) mod 5 is proper:
-    the codes a + b + 2c are 1..4, never 0 mod 5.  Fewer colours give fewer
-    IC(0) dependency levels.  Within a colour the natural order is kept.
+    the codes a + b + 2c are 1..4, never 0 mod 5.  With 4 colours, (i + j + k)
+    mod 4 is proper: the codes a + b + c are 1..3 (a tet is a 4-clique, so 4 is
+    the fewest possible).  Fewer colours give fewer IC(0) dependency levels.
+    Within a colour the natural order is kept.
     Returns perm with new_id = perm[old_id].
     """
     nx, ny, nz = mesh.shape
@@ -175,8 +177,10 @@ def multicolor_permutation(mesh: TetMesh, colors: int = 8) -> np.ndarray:
         color = (i + 2 * j + 4 * k) % 8
     elif colors == 5:
         color = (i + j + 2 * k) % 5
+    elif colors == 4:
+        color = (i + j + k) % 4
     else:
-        raise ValueError("multicolour orderings of Kuhn grids: 8 or 5 colours")
+        raise ValueError("multicolour orderings of Kuhn grids: 8, 5 or 4 colours")
     order = np.lexsort((n, color))         # old ids sorted by (colour, old id)
     perm = np.empty_like(order)
     perm[order] = np.arange(mesh.n_nodes)

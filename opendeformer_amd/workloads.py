@@ -77,8 +77,8 @@ class Workload:
 
 
 def _single(name, m: M.TetMesh, kind, E, nu, dt, steps, ordering="natural", impulse=None, meta=None):
-    if ordering in ("color8", "color5"):
-        perm = M.multicolor_permutation(m, 8 if ordering == "color8" else 5)
+    if ordering in ("color8", "color5", "color4"):
+        perm = M.multicolor_permutation(m, int(ordering[len("color"):]))
         m = M.relabel(m, perm)
         if impulse is not None:
             imp = np.zeros_like(impulse)
@@ -153,8 +153,8 @@ def batch(n_inst=256, seed=4, tet_lo=500, tet_hi=50000, cell=0.05, steps=120, or
         s = seed * 100003 + i
         nx, ny, nz, kind, E, nu = instance_shape(s, tet_lo, tet_hi)
         m = M.box_mesh(nx, ny, nz, nx * cell, ny * cell, nz * cell, seed=s)
-        if ordering in ("color8", "color5"):
-            m = M.relabel(m, M.multicolor_permutation(m, 8 if ordering == "color8" else 5))
+        if ordering in ("color8", "color5", "color4"):
+            m = M.relabel(m, M.multicolor_permutation(m, int(ordering[len("color"):])))
         off = node_start[-1]
         Xs.append(m.X); Ts.append(m.tets + off); Fs.append(m.facets + off)
         fixed.append(M.plane_fixed_nodes(m, axis=0, value=0.0) + off)
@@ -179,18 +179,18 @@ def subset(w: Workload, i: int) -> Workload:
                     rho=w.rho[t0:t1], dt=w.dt, steps=w.steps, shapes=[w.shapes[i]], meta=dict(w.meta))
 
 
-def served_pool(n_obj=1024, seed=5, tet_lo=1000, tet_hi=500000, cell=0.05, ordering="color5"):
+def served_pool(n_obj=1024, seed=5, tet_lo=1000, tet_hi=500000, cell=0.05, ordering="color4"):
     """Pool of independent boxes for the serving stream: tets log-uniform in [tet_lo, tet_hi],
     aspect U(1, 6), corotational or neo-Hookean, E log-uniform [1e4, 1e6], nu U(0.3, 0.48),
-    the x = 0 face fixed, gravity.  Nodes are numbered by a 5-colouring by default (5 IC(0)
+    the x = 0 face fixed, gravity.  Nodes are numbered by a 4-colouring by default (4 IC(0)
     dependency levels per object)."""
     objs = []
     for i in range(n_obj):
         s = seed * 100003 + i
         nx, ny, nz, kind, E, nu = instance_shape(s, tet_lo, tet_hi)
         m = M.box_mesh(nx, ny, nz, nx * cell, ny * cell, nz * cell, seed=s)
-        if ordering in ("color8", "color5"):
-            m = M.relabel(m, M.multicolor_permutation(m, 8 if ordering == "color8" else 5))
+        if ordering in ("color8", "color5", "color4"):
+            m = M.relabel(m, M.multicolor_permutation(m, int(ordering[len("color"):])))
         elif ordering != "natural":
             raise ValueError(f"unknown ordering {ordering}")
         fixed = M.plane_fixed_nodes(m, axis=0, value=0.0)

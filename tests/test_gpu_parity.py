@@ -30,7 +30,8 @@ def make(ctx, w):
     return mesh, s
 
 
-@pytest.mark.parametrize("name", ["box_small", "box_small_color8", "box_small_color5", "box_small_gps", "partial_box"])
+@pytest.mark.parametrize("name", ["box_small", "box_small_color8", "box_small_color5", "box_small_color4", "box_small_gps",
+                                  "partial_box"])
 def test_pattern_and_element_map_bit_exact(ctx, name):
     import opendeformer_amd as od
     if name == "partial_box":
@@ -113,7 +114,7 @@ def newmark_system(s, w, kind=None):
     return Mv + 0.25 * w.dt ** 2 * K
 
 
-@pytest.mark.parametrize("ordering", ["natural", "color8", "color5", "gps"])
+@pytest.mark.parametrize("ordering", ["natural", "color8", "color5", "color4", "gps"])
 def test_ic0_factor_matches_oracle(ctx, ordering):
     import opendeformer_amd as od
     w = W.box_small(ordering)
@@ -176,6 +177,7 @@ def run_pair(ctx, w, steps, pre=capi.OD_PRE_IC0, opre=O.PRE_IC0, precision=capi.
 @pytest.mark.parametrize("ordering,pre,opre", [("natural", capi.OD_PRE_IC0, O.PRE_IC0),
                                                 ("color8", capi.OD_PRE_IC0, O.PRE_IC0),
                                                 ("color5", capi.OD_PRE_IC0, O.PRE_IC0),
+                                                ("color4", capi.OD_PRE_IC0, O.PRE_IC0),
                                                 ("gps", capi.OD_PRE_IC0, O.PRE_IC0),
                                                 ("natural", capi.OD_PRE_JACOBI, O.PRE_JACOBI)])
 def test_newmark_box_small_100_steps(ctx, ordering, pre, opre):

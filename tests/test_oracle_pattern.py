@@ -88,7 +88,7 @@ CASES = {
 }
 
 
-@pytest.mark.parametrize("colors", [8, 5])
+@pytest.mark.parametrize("colors", [8, 5, 4])
 def test_multicolour_orderings_are_proper(colors):
     """No tet edge joins two nodes of one colour, so the IC(0) dependency levels of a
     colour-grouped numbering number at most the colours."""
@@ -96,7 +96,7 @@ def test_multicolour_orderings_are_proper(colors):
     perm = M.multicolor_permutation(m, colors)
     n = np.arange(m.n_nodes)
     k, j, i = n % 6, (n // 6) % 7, n // 42
-    color = (i + 2 * j + 4 * k) % 8 if colors == 8 else (i + j + 2 * k) % 5
+    color = {8: (i + 2 * j + 4 * k) % 8, 5: (i + j + 2 * k) % 5, 4: (i + j + k) % 4}[colors]
     assert len(np.unique(color)) == colors
     for a in range(4):
         for b in range(a + 1, 4):
@@ -104,8 +104,8 @@ def test_multicolour_orderings_are_proper(colors):
     # the relabelling groups colours: new ids of colour c precede those of colour c + 1
     new_color = np.empty_like(color); new_color[perm] = color
     assert (np.diff(new_color) >= 0).all()
-    with pytest.raises(ValueError):
-        M.multicolor_permutation(m, 4)
+    with pytest.raises(ValueError):          # a tet is a 4-clique: no 3-colouring exists
+        M.multicolor_permutation(m, 3)
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
