@@ -35,10 +35,12 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Newmark steps/s + SpMV GB/s at 1M DOF, rel.res 1e-6; p50/p99 served step latency"
 
 
-def spmv_bytes(mesh, value_bytes=8, vec_bytes=8):
+def spmv_bytes(mesh, value_bytes=8, vec_bytes=8, fused=False):
     """Algorithmic bytes of one SpMV over the upper-symmetric canonical layout:
-    values + block column indices + block row pointers + x read + y write."""
-    return value_bytes * mesh.n_values + 4 * mesh.n_blocks + 4 * (mesh.n_rows + 1) + 2 * vec_bytes * mesh.ndof
+    values + block column indices + block row pointers + x read + y write; the PCG product
+    (fused) also reads z and writes the new direction p = z + beta p_old."""
+    b = value_bytes * mesh.n_values + 4 * mesh.n_blocks + 4 * (mesh.n_rows + 1) + 2 * vec_bytes * mesh.ndof
+    return b + (2 * vec_bytes * mesh.ndof if fused else 0)
 
 
 def traffic_from_profile(workload, args):
@@ -359,7 +361,7 @@ def main():
         spmv_ms, spmv_n = ctx.profile_read()
         ctx.profile(False)
         vb = 4 if precision == "mixed" else 8
-        by = spmv_bytes(mesh, vb, vb)
+        by = spmv_bytes(mesh, vb, vb, fused=(vb == 4))
         # Launches of a batch that finds its instance converged exit at once; their (small) time
         # stays in the timed total, but only the iterations that ran count as SpMV work.
         real_spmv = int(np.sum(iters))
@@ -370,7 +372,7 @@ def main():
         return {"steps_s": args.steps / elapsed, "ms_step": elapsed / args.steps * 1e3,
                 "iters": float(np.mean(iters)), "relres_max": float(np.max(relres)),
                 "outer": float(np.mean(outer)),
-                "spmv": {"kernel": f"k_spmv_dot<{'double' if vb == 8 else 'float'}, {r64 if vb == 8 else r32}>",
+                "spmv": {"kernel": (f"k_spmv_dot<double, {r64}, false>" if vb == 8 else f"k_spmv_dot<float, {r32}, true>"),
                          "launches": spmv_n, "iterations": real_spmv,
                          "avg_us_per_launch": round(spmv_ms / max(spmv_n, 1) * 1e3, 2),
                          "avg_us_per_iteration": round(avg_spmv_s * 1e6, 2),

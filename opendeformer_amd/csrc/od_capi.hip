@@ -273,7 +273,8 @@ static void autotune_spmv(od_mesh *mesh) {
     };
     hipEvent_t e0 = nullptr, e1 = nullptr;
     try {
-        void *vals = get(8 * nv), *x = get(8 * n), *y = get(8 * n);
+        // the PCG products as the solver runs them: FP32 with the fused direction update (beta = 0)
+        void *vals = get(8 * nv), *x = get(8 * n), *y = get(8 * n), *z = get(8 * n), *pNew = get(8 * n);
         double *partials = (double *)get(8 * (size_t)std::max(1, m.nChunks));
         InstScalars *sc = (InstScalars *)get(sizeof(InstScalars) * t.nInst);   // zero: no instance done
         float *flush = (float *)get(4 * (size_t)flushN);
@@ -291,7 +292,7 @@ static void autotune_spmv(od_mesh *mesh) {
                                                 sc, e0, e1);
                     else
                         launch_spmv_dot<float>(m, st, R, (const float *)vals, (const float *)x, (float *)y, partials,
-                                               sc, e0, e1);
+                                               sc, e0, e1, (const float *)z, (float *)pNew);
                     OD_HIP(hipEventSynchronize(e1));
                     OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
                 }
@@ -341,7 +342,7 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
         }
         if (precision == OD_MIXED) {
             s->r32 = (float *)s->alloc(4 * n); s->ri32 = (float *)s->alloc(4 * n); s->z32 = (float *)s->alloc(4 * n);
-            s->p32 = (float *)s->alloc(4 * n); s->q32 = (float *)s->alloc(4 * n);
+            s->p32 = (float *)s->alloc(4 * 2 * n); s->q32 = (float *)s->alloc(4 * n);   // two direction buffers
             s->x32 = (float *)s->alloc(4 * n);
             if (precond == OD_PRE_JACOBI) s->invDiag32 = (float *)s->alloc(4 * n);
             if (precond == OD_PRE_IC0) {
@@ -401,6 +402,10 @@ static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip)
 }
 
 // PCG on the chunked instances: x in/out, b fixed.  preDone marks instances to skip.
+// FP32 solves fuse the direction update into the product: p holds two direction buffers of
+// ndof each, iteration k reads buffer k & 1 and its product writes the new direction
+// z + beta p into the other one.  FP64 solves update p in place after the dot product
+// (the fused FP64 product needs more registers than two workgroups per CU allow).
 template <typename T>
 static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const T *invDiag, const T *b,
                     T *x, T *r, T *z, T *p, T *q, double tol, int maxIt, const uint8_t *preDone, InstScalars *sc,
@@ -421,18 +426,21 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     // stream).  Warm-started solves of consecutive steps need nearly the same count, so the
     // first batch is sized from the same solve of the previous step (`expect`); afterwards
     // the batches stay small.
-    auto iteration = [&]() {
+    constexpr bool fuseDir = sizeof(T) == 4;
+    auto iteration = [&](int k) {
         hipEvent_t e0, e1;
         prof_pair(mesh->ctx, &e0, &e1);
-        pcg_alpha<T>(m, st, vals, p, q, w, e0, e1);
+        const T *pOld = fuseDir ? p + (size_t)(k & 1) * n : p;
+        T *pNew = fuseDir ? p + (size_t)((k + 1) & 1) * n : p;
+        pcg_alpha<T>(m, st, vals, pOld, fuseDir ? z : nullptr, pNew, q, w, e0, e1);
         pcg_alpha_finalize(m, st, w);
         if (ic) {
-            pcg_update_xr_ic0<T>(m, st, mesh->topo.levelPtr, p, q, x, r, Uf, Ul, z, w, preDone);
+            pcg_update_xr_ic0<T>(m, st, mesh->topo.levelPtr, pNew, q, x, r, Uf, Ul, z, w, preDone);
             pcg_dot_rz<T>(m, st, r, z, w);
         } else {
-            pcg_update_xr<T>(m, st, p, q, x, r, invDiag, z, w);
+            pcg_update_xr<T>(m, st, pNew, q, x, r, invDiag, z, w);
         }
-        pcg_update_p<T>(m, st, z, p, w);
+        if (!fuseDir) pcg_update_p<T>(m, st, z, p, w);
     };
     // Graph mode (not while timing launches): a block of kGraphIters iterations is captured
     // once per set of buffers and replayed; iterations past convergence exit at once.
@@ -445,7 +453,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
             hipGraph_t graph = nullptr;
             OD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                for (int j = 0; j < kGraphIters; j++) iteration();
+                for (int j = 0; j < kGraphIters; j++) iteration(j);
             } catch (...) {
                 (void)hipStreamEndCapture(st, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -470,7 +478,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
             for (int j = 0; j < k; j += kGraphIters) OD_HIP(hipGraphLaunch(g->exec, st));
             launched += ((k + kGraphIters - 1) / kGraphIters) * kGraphIters;
         } else {
-            for (int j = 0; j < k; j++) iteration();
+            for (int j = 0; j < k; j++) iteration(launched + j);
             launched += k;
         }
         batch = expect > 1 ? 2 : std::min(batch * 2, 32);

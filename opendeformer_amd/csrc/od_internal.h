@@ -94,9 +94,12 @@ template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
                  const InstScalars *sc);
 // PCG SpMV with R chunks per workgroup (1, 2 or 4), timed by optional dispatch events
+// With z set (the PCG product), x is the previous direction and the product runs on the new
+// direction p = z + beta x of each instance, written to pNew (see spmv_chunks).
 template <typename T>
 void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, const T *vals, const T *x, T *y, double *partials,
-                     const InstScalars *sc, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+                     const InstScalars *sc, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
+                     const T *z = nullptr, T *pNew = nullptr);
 constexpr int kSpmvVariants[3] = {1, 2, 4};
 
 // PCG building blocks (see od_kernels.hip for the exact phase semantics)
@@ -115,8 +118,13 @@ void pcg_jacobi(const DevMesh &m, hipStream_t s, const T *invDiag, const T *r, T
 template <typename T>
 void pcg_dot_rz_init(const DevMesh &m, hipStream_t s, const T *r, const T *z, T *p, PcgWork &w, double tol, int maxIt,
                      const uint8_t *preDone);
+// p = z + beta p (the direction update when the product does not fuse it)
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w,
+void pcg_update_p(const DevMesh &m, hipStream_t s, const T *z, T *p, PcgWork &w);
+// q = A p; with z set, p = z + beta pOld is computed on the fly and written to p (the
+// direction update fused into the product), else the product runs on p = pOld
+template <typename T>
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *pOld, const T *z, T *p, T *q, PcgWork &w,
                hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w);
 template <typename T>
@@ -130,8 +138,6 @@ void pcg_dot_rz(const DevMesh &m, hipStream_t s, const T *r, const T *z, PcgWork
 template <typename T>
 void pcg_update_xr_ic0(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *p, const T *q, T *x,
                        T *r, const T *U, const T *Ul, T *z, PcgWork &w, const uint8_t *skip);
-template <typename T>
-void pcg_update_p(const DevMesh &m, hipStream_t s, const T *z, T *p, PcgWork &w);
 template <typename T>
 void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *invDiag);
 template <typename T>

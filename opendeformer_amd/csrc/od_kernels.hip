@@ -473,25 +473,41 @@ struct FinalArgs {
 // looser: the correction then meets the outer test without a second sweep.
 constexpr double kInnerOuterScale2 = 0.25;
 
-// One workgroup per instance: reduce its chunk partials (fixed order) and update its scalars.
-__global__ __launch_bounds__(1024) void k_finalize(DevMesh m, int phase, const double *__restrict__ partials,
-                                                  InstScalars *__restrict__ sc, int *__restrict__ doneFlags,
-                                                  FinalArgs fa) {
-    const int inst = blockIdx.x;
+// Sums the chunk partials of slots [Q0, Q0 + NQ) over chunks [c0, c1), thread-strided then
+// over the workgroup (fixed order); thread 0 gets t.  Only the slots a phase uses are read.
+template <int Q0, int NQ>
+__device__ __forceinline__ void reduce_slots(const DevMesh &m, const double *__restrict__ partials, int c0, int c1,
+                                             double *t) {
+    double v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) v[q] = 0.0;
+#pragma unroll 8
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) v[q] += partials[(Q0 + q) * m.nChunks + c];
+    }
+    block_sum<NQ>(v);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) t[Q0 + q] = v[q];
+}
+
+// Reduces an instance's chunk partials (fixed order) and updates its scalars; called by every
+// thread of a workgroup (any size up to 1024).
+__device__ void finalize_inst(const DevMesh &m, int phase, int inst, const double *__restrict__ partials,
+                              InstScalars *__restrict__ sc, int *__restrict__ doneFlags, const FinalArgs &fa) {
     const int c0 = m.instChunkStart[inst], c1 = m.instChunkStart[inst + 1];
     double t[3] = {0, 0, 0};
-#pragma unroll 4
-    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
-        t[0] += partials[c];
-        t[1] += partials[m.nChunks + c];
-        t[2] += partials[2 * m.nChunks + c];
-    }
-    block_sum<3>(t);
+    if (phase == PH_ALPHA) reduce_slots<0, 1>(m, partials, c0, c1, t);
+    else if (phase == PH_ITER) reduce_slots<0, 2>(m, partials, c0, c1, t);
+    else if (phase == PH_INIT) reduce_slots<0, 3>(m, partials, c0, c1, t);
+    else if (phase == PH_OUTER_FIRST) reduce_slots<1, 2>(m, partials, c0, c1, t);
+    else reduce_slots<1, 1>(m, partials, c0, c1, t);
     if (threadIdx.x != 0) return;
     InstScalars S = sc[inst];
     switch (phase) {
     case PH_INIT:
         S.delta = t[0]; S.rr = t[1]; S.bb = t[2];
+        S.beta = 0.0;                       // the first product takes p = z
         S.it = 0; S.maxIt = fa.maxIt;
         S.eps = fa.tol * fa.tol * S.bb;
         if (fa.outer) S.eps = fmax(S.eps, kInnerOuterScale2 * fa.outer[inst].eps);
@@ -521,6 +537,13 @@ __global__ __launch_bounds__(1024) void k_finalize(DevMesh m, int phase, const d
     doneFlags[inst] = S.done;
 }
 
+// One workgroup per instance.
+__global__ __launch_bounds__(1024) void k_finalize(DevMesh m, int phase, const double *__restrict__ partials,
+                                                  InstScalars *__restrict__ sc, int *__restrict__ doneFlags,
+                                                  FinalArgs fa) {
+    finalize_inst(m, phase, blockIdx.x, partials, sc, doneFlags, fa);
+}
+
 void launch_finalize(const DevMesh &m, hipStream_t s, int phase, PcgWork &w, double tol, int maxIt,
                      const uint8_t *preDone) {
     // a wide workgroup for few large instances, a narrow one for many small instances
@@ -545,6 +568,17 @@ __device__ __forceinline__ void load_block(const T *__restrict__ A, const T *__r
 #pragma unroll
     for (int q = 0; q < 9; q++) b.e[q] = A[q];
     b.x[0] = xc[0]; b.x[1] = xc[1]; b.x[2] = xc[2];
+}
+
+// The PCG direction at one column, p = z + beta p_old (b.x holds p_old on entry): every
+// workgroup that gathers a column computes it with the same fused multiply-add, so all
+// copies of p_j agree to the bit.
+template <typename T>
+__device__ __forceinline__ void dir_update(const T *__restrict__ zc, T beta, BlockX<T> &b) {
+    const T z0 = zc[0], z1 = zc[1], z2 = zc[2];
+    b.x[0] = fma(beta, b.x[0], z0);
+    b.x[1] = fma(beta, b.x[1], z1);
+    b.x[2] = fma(beta, b.x[2], z2);
 }
 
 // a += B x for the packed symmetric diagonal [e0 e1 e2; e1 e3 e4; e2 e4 e5], a row-major
@@ -579,9 +613,11 @@ __device__ __forceinline__ void sum16(T &a0, T &a1, T &a2) {
 // One block row with 16 lanes through the row pointers (rows with more than 16 blocks):
 // item 0 is the diagonal block, items 1..nOff the row's upper blocks (contiguous values),
 // the remaining items the transposed lower blocks reached through the transposed index.
-template <typename T>
+// With FUSE, x is p_old and each gathered column is the direction z + beta p_old; the lane
+// of item 0 (lane 0) returns the row's own direction in px.
+template <typename T, bool FUSE>
 __device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x, int r,
-                                         int lane, T &o0, T &o1, T &o2) {
+                                         int lane, T &o0, T &o1, T &o2, const T *__restrict__ z, T beta, T *px) {
     const int v0 = m.valOff[r], b0 = m.rowPtr[r];
     const int nOff = m.rowPtr[r + 1] - b0 - 1;
     const int t0 = m.tPtr[r], nItems = 1 + nOff + (m.tPtr[r + 1] - t0);
@@ -593,6 +629,10 @@ __device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__
         const T *A = up ? vals + (it == 0 ? v0 : v0 + 6 + 9 * (it - 1)) : vals + m.tOff[k];
         BlockX<T> b;
         load_block<T>(A, x + 3 * col, b);
+        if (FUSE) {
+            dir_update<T>(z + 3 * col, beta, b);
+            if (it == 0) { px[0] = b.x[0]; px[1] = b.x[1]; px[2] = b.x[2]; }
+        }
         block_mv<T>(b, it == 0, !up, a0, a1, a2);
     }
     sum16(a0, a1, a2);
@@ -610,18 +650,25 @@ __device__ __forceinline__ int run_chunk(const DevMesh &m, int q) {
 // padding, inactive or long-row chunks point at valid memory and their results are dropped),
 // so the slot loads of all R chunks, then all their block and x loads, are in flight together.
 // The dot partial x.y of each chunk goes to partials[chunk] when partials is set.
-template <typename T, int R>
+//
+// With FUSE (the PCG product), x is the previous direction p_old: every gathered column is
+// the new direction p = z + beta p_old of its instance (dir_update), the row's own direction
+// (from its diagonal slot, lane 0) is written to pNew, and the dot partial is p.q.
+template <typename T, int R, bool FUSE>
 __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x,
                                             T *__restrict__ y, int accumulate, double *__restrict__ partials,
-                                            const InstScalars *__restrict__ sc) {
+                                            const InstScalars *__restrict__ sc, const T *__restrict__ z,
+                                            T *__restrict__ pNew) {
     const int j = (int)(blockIdx.x >> 3), lane = threadIdx.x & 15, lr = threadIdx.x >> 4;
     int chunk[R], r[R], off[R], cf[R];
     bool act[R], lng[R];
+    T beta[R];
 #pragma unroll
     for (int i = 0; i < R; i++) {
         chunk[i] = run_chunk(m, R * j + i);
         const int c = chunk[i] >= 0 ? chunk[i] : 0;
         act[i] = chunk[i] >= 0 && !(sc && sc[m.chunkInst[c]].done);
+        beta[i] = FUSE ? (T)sc[m.chunkInst[c]].beta : (T)0;
         lng[i] = m.chunkLong[c] != 0;
         r[i] = m.chunkRow0[c] + lr;
         const size_t slot = (size_t)c * kSpmvThreads + threadIdx.x;
@@ -633,7 +680,9 @@ __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restric
 #pragma unroll
     for (int i = 0; i < R; i++) {
         const bool ok = cf[i] >= 0;
-        load_block<T>(vals + (ok ? off[i] : 0), x + 3 * (ok ? (cf[i] & kSlotColMask) : 0), b[i]);
+        const int c3 = 3 * (ok ? (cf[i] & kSlotColMask) : 0);
+        load_block<T>(vals + (ok ? off[i] : 0), x + c3, b[i]);
+        if (FUSE) dir_update<T>(z + c3, beta[i], b[i]);
     }
     T a[R][3];
 #pragma unroll
@@ -648,19 +697,22 @@ __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restric
         v[i] = 0.0;
         if (!act[i]) continue;                      // uniform over the workgroup
         const bool inRow = r[i] < m.chunkRow1[chunk[i]];
+        // the row's own direction: its diagonal slot (lane 0) or item 0 of the row loop
+        T px[3] = {b[i].x[0], b[i].x[1], b[i].x[2]};
         if (lng[i]) {
-            if (inRow) row_spmv<T>(m, vals, x, r[i], lane, a[i][0], a[i][1], a[i][2]);
+            if (inRow) row_spmv<T, FUSE>(m, vals, x, r[i], lane, a[i][0], a[i][1], a[i][2], z, beta[i], px);
         } else {
             sum16(a[i][0], a[i][1], a[i][2]);
         }
         if (inRow && lane == 0) {
             T o0 = a[i][0], o1 = a[i][1], o2 = a[i][2];
             const int rr = r[i];
+            if (!FUSE) { px[0] = x[3 * rr]; px[1] = x[3 * rr + 1]; px[2] = x[3 * rr + 2]; }
+            if (FUSE) { pNew[3 * rr] = px[0]; pNew[3 * rr + 1] = px[1]; pNew[3 * rr + 2] = px[2]; }
             if (accumulate) { o0 += y[3 * rr]; o1 += y[3 * rr + 1]; o2 += y[3 * rr + 2]; }
             y[3 * rr] = o0; y[3 * rr + 1] = o1; y[3 * rr + 2] = o2;
             if (partials)
-                v[i] = (double)x[3 * rr] * (double)o0 + (double)x[3 * rr + 1] * (double)o1 +
-                       (double)x[3 * rr + 2] * (double)o2;
+                v[i] = (double)px[0] * (double)o0 + (double)px[1] * (double)o1 + (double)px[2] * (double)o2;
         }
     }
     if (partials) {
@@ -676,41 +728,56 @@ __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restric
 template <typename T>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
                                                        T *__restrict__ y, int accumulate, const InstScalars *sc) {
-    spmv_chunks<T, 1>(m, vals, x, y, accumulate, nullptr, sc);
+    spmv_chunks<T, 1, false>(m, vals, x, y, accumulate, nullptr, sc, nullptr, nullptr);
 }
 
-// The PCG product q = A p with the chunk partials p.q (slot 0), R chunks per workgroup.
-template <typename T, int R>
+// The PCG product q = A p with the chunk partials p.q (slot 0), R chunks per workgroup; with
+// FUSE the direction update p = z + beta p_old is done on the fly (x = p_old, p -> pNew).
+template <typename T, int R, bool FUSE>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv_dot(DevMesh m, const T *__restrict__ vals,
                                                            const T *__restrict__ x, T *__restrict__ y,
                                                            double *__restrict__ partials,
-                                                           const InstScalars *__restrict__ sc) {
-    spmv_chunks<T, R>(m, vals, x, y, 0, partials, sc);
+                                                           const InstScalars *__restrict__ sc, const T *__restrict__ z,
+                                                           T *__restrict__ pNew) {
+    spmv_chunks<T, R, FUSE>(m, vals, x, y, 0, partials, sc, z, pNew);
 }
 
 static inline int spmv_grid(const DevMesh &m, int R) { return 8 * ((m.perRun + R - 1) / R); }
 
 template <typename T>
 void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, const T *vals, const T *x, T *y, double *partials,
-                     const InstScalars *sc, hipEvent_t start, hipEvent_t stop) {
+                     const InstScalars *sc, hipEvent_t start, hipEvent_t stop, const T *z, T *pNew) {
     // With events, the launch records them from the dispatch's own begin/end timestamps,
     // so their elapsed time is the kernel's execution alone (measurement mode).
     auto go = [&](auto kernel) {
         if (start)
             hipExtLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(kSpmvThreads), 0, s, start, stop, 0, m, vals, x, y,
-                                  partials, sc);
+                                  partials, sc, z, pNew);
         else
-            hipLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(kSpmvThreads), 0, s, m, vals, x, y, partials, sc);
+            hipLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(kSpmvThreads), 0, s, m, vals, x, y, partials, sc, z,
+                               pNew);
     };
-    if (R == 4) go(k_spmv_dot<T, 4>);
-    else if (R == 2) go(k_spmv_dot<T, 2>);
-    else go(k_spmv_dot<T, 1>);
+    if constexpr (sizeof(T) == 4) {          // the fused direction update is an FP32 product
+        if (z) {
+            if (R == 4) go(k_spmv_dot<T, 4, true>);
+            else if (R == 2) go(k_spmv_dot<T, 2, true>);
+            else go(k_spmv_dot<T, 1, true>);
+            OD_HIP(hipGetLastError());
+            return;
+        }
+    }
+    if (z) throw HipError("the fused direction update is an FP32 product");
+    {
+        if (R == 4) go(k_spmv_dot<T, 4, false>);
+        else if (R == 2) go(k_spmv_dot<T, 2, false>);
+        else go(k_spmv_dot<T, 1, false>);
+    }
     OD_HIP(hipGetLastError());
 }
 template void launch_spmv_dot<double>(const DevMesh &, hipStream_t, int, const double *, const double *, double *,
-                                      double *, const InstScalars *, hipEvent_t, hipEvent_t);
+                                      double *, const InstScalars *, hipEvent_t, hipEvent_t, const double *, double *);
 template void launch_spmv_dot<float>(const DevMesh &, hipStream_t, int, const float *, const float *, float *, double *,
-                                     const InstScalars *, hipEvent_t, hipEvent_t);
+                                     const InstScalars *, hipEvent_t, hipEvent_t, const float *, float *);
 
 template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
@@ -724,9 +791,10 @@ template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, co
                                  const InstScalars *);
 
 template <typename T>
-void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *p, T *q, PcgWork &w, hipEvent_t start,
-               hipEvent_t stop) {
-    launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, vals, p, q, w.partials, w.sc, start, stop);
+void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *pOld, const T *z, T *p, T *q, PcgWork &w,
+               hipEvent_t start, hipEvent_t stop) {
+    launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, vals, z ? pOld : p, q, w.partials, w.sc, start,
+                       stop, z, z ? p : nullptr);
 }
 
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w) {
@@ -855,12 +923,14 @@ void pcg_dot_rz(const DevMesh &m, hipStream_t s, const T *r, const T *z, PcgWork
     launch_finalize(m, s, PH_ITER, w, 0.0, 0, nullptr);
 }
 
-// p = z + beta p  (skipped for instances that are done)
+// p = z + beta p (skipped for instances that are done): the direction update of the FP64
+// solve, whose product runs without the fused update (it would need more registers than two
+// workgroups per CU allow)
 template <typename T>
 __global__ __launch_bounds__(kVecThreads) void k_update_p(DevMesh m, const T *z, T *p, const InstScalars *sc) {
     CHUNK_PROLOGUE
     if (!in || sc[inst].done) return;
-    p[i] = z[i] + (T)sc[inst].beta * p[i];
+    p[i] = fma((T)sc[inst].beta, p[i], z[i]);
 }
 
 template <typename T>
@@ -1252,14 +1322,14 @@ void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDo
     template void pcg_jacobi<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &);                  \
     template void pcg_dot_rz_init<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &, double, int, \
                                      const uint8_t *);                                                                \
-    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, PcgWork &, hipEvent_t,       \
-                               hipEvent_t);                                                                       \
+    template void pcg_alpha<T>(const DevMesh &, hipStream_t, const T *, const T *, const T *, T *, T *, PcgWork &,    \
+                               hipEvent_t, hipEvent_t);                                                           \
     template void pcg_update_xr<T>(const DevMesh &, hipStream_t, const T *, const T *, T *, T *, const T *, T *,      \
                                    PcgWork &);                                                                        \
     template void pcg_dot_rz<T>(const DevMesh &, hipStream_t, const T *, const T *, PcgWork &);                       \
+    template void pcg_update_p<T>(const DevMesh &, hipStream_t, const T *, T *, PcgWork &);                           \
     template void pcg_update_xr_ic0<T>(const DevMesh &, hipStream_t, const std::vector<int> &, const T *, const T *,   \
                                        T *, T *, const T *, const T *, T *, PcgWork &, const uint8_t *);              \
-    template void pcg_update_p<T>(const DevMesh &, hipStream_t, const T *, T *, PcgWork &);                           \
     template void launch_jacobi_diag<T>(const DevMesh &, hipStream_t, const double *, T *);                           \
     template void launch_ic0_apply<T>(const DevMesh &, hipStream_t, const std::vector<int> &, const T *, const T *,   \
                                       const T *, T *, const InstScalars *, const uint8_t *);
