@@ -725,6 +725,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         }
         m.levelWideFwd = mesh->levelWideFwd.data();
         m.levelWideBwd = mesh->levelWideBwd.data();
+        m.levelsContiguous = t.levelsContiguous ? 1 : 0;
         m.chunkRow0 = mesh->upload(t.chunkRow0);
         m.chunkRow1 = mesh->upload(t.chunkRow1);
         m.chunkInst = mesh->upload(t.chunkInst);

@@ -46,6 +46,7 @@ struct DevMesh {
     int *blkCStart;                              // nBlocks+1
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
     int *levelRows;                              // nRows
+    int levelsContiguous;                        // levels are consecutive row ranges (levelRows is the identity)
     int *rowLevel;                               // IC(0) dependency level of each row
     // per level, in host memory (read at launch, never on the device): rows average more than
     // 8 blocks on the side the forward / backward sweep reads (run 16 lanes per row instead of 8)

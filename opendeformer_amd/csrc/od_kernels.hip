@@ -1098,7 +1098,7 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_fwd(DevMesh m, int p0, int c
     const int wg = xcd_wg((count + kRows - 1) / kRows);
     const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
     if (pl >= count) return;
-    const int I = m.levelRows[p0 + pl];
+    const int I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
     const int inst = m.rowInst[I];
     if ((sc && sc[inst].done) || (skip && skip[inst])) return;
     T a0 = 0, a1 = 0, a2 = 0;
@@ -1128,7 +1128,7 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_bwd(DevMesh m, int p0, int c
     const int wg = xcd_wg((count + kRows - 1) / kRows);
     const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
     if (pl >= count) return;
-    const int I = m.levelRows[p0 + pl];
+    const int I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
     const int inst = m.rowInst[I];
     if ((sc && sc[inst].done) || (skip && skip[inst])) return;
     const int v0 = m.valOff[I], b0 = m.rowPtr[I], nOff = m.rowPtr[I + 1] - b0 - 1;

@@ -212,6 +212,27 @@ std::string buildTopology(Topology &t, int chunkRows, int nXcd) {
         std::vector<int> cur(t.levelPtr.begin(), t.levelPtr.end() - 1);
         for (int I = 0; I < t.nRows; I++) t.levelRows[cur[t.level[I]]++] = I;
     }
+    // Contiguous schedule: the rows, in order, cut into ranges with no dependency inside a
+    // range (a colour-grouped numbering gives one range per colour).  When that needs no more
+    // steps than the dependency levels, the ranges are the schedule: a step's rows are then
+    // consecutive, and the sweeps index them without the level row list.  Every row still
+    // runs after all rows it depends on, so the results are the same bits.
+    {
+        std::vector<int> cuts = {0};
+        int start = 0;
+        for (int I = 0; I < t.nRows; I++)
+            for (int k = t.tPtr[I]; k < t.tPtr[I + 1]; k++)
+                if (t.tRow[k] >= start) { cuts.push_back(I); start = I; break; }
+        cuts.push_back(t.nRows);
+        const int nRanges = (int)cuts.size() - 1;
+        t.levelsContiguous = nRanges <= t.nLevels;
+        if (t.levelsContiguous) {
+            t.nLevels = nRanges;
+            t.levelPtr = cuts;
+            for (int l = 0; l < nRanges; l++)
+                for (int I = cuts[l]; I < cuts[l + 1]; I++) { t.level[I] = l; t.levelRows[I] = I; }
+        }
+    }
 
     // ---- instances and chunks -----------------------------------------------------
     if (t.instNodeStart.empty()) { t.instNodeStart = {0, t.nNodes}; t.instTetStart = {0, t.nTets}; }

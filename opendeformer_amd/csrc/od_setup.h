@@ -49,6 +49,7 @@ struct Topology {
     std::vector<int> level;             // per row
     std::vector<int> levelPtr;          // size nLevels+1
     std::vector<int> levelRows;         // rows sorted by level (stable)
+    bool levelsContiguous = false;      // each level is the consecutive rows [levelPtr[l], levelPtr[l+1])
     int nLevels = 0;
 
     // ---- instances and row chunks ---------------------------------------------
