@@ -407,6 +407,10 @@ def main():
     }
     line["spmv_autotune"] = {"chunks_per_workgroup": {"fp64": r64, "fp32": r32},
                              "cold_us": {f"{p}_R{r}": round(v * 1e3, 2) for (p, r), v in tune_table.items()}}
+    info = mesh.info_raw()
+    lanes = lambda word: [1 << ((int(word) >> (4 * l)) & 15) for l in range(min(8, mesh.n_levels))]
+    line["ic0_sweeps"] = {"contiguous_levels": bool(info[12]), "lanes_per_row_fwd": lanes(info[13]),
+                          "lanes_per_row_bwd": lanes(info[14])}
     if args.precision == "mixed":
         line["refinement_sweeps_per_step"] = round(main_r["outer"], 2)
     if fp64_r is not None:

@@ -88,7 +88,9 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X_host, int nTets, con
                    const int *instNodeStart_host, const int *instTetStart_host, od_mesh **out);
 int od_mesh_destroy(od_mesh *mesh);
 /* info[16]: ndof, nBlockRows, nBlocks, nValues, nLevels, nInstances, nTets,
- * nNodes, nChunks, nPairContrib, nNodeContrib, device bytes, 0... */
+ * nNodes, nChunks, nPairContrib, nNodeContrib, device bytes, 1 when the IC(0)
+ * levels are consecutive row ranges, then log2 of the sweep lanes per row of
+ * the first 8 levels (forward, backward; 4 bits per level), 0 */
 int od_mesh_info(od_mesh *mesh, long long *info_host);
 /* Canonical layout, identical to BlockedSymSparseMatrix<3,3>'s blockPcol,
  * blockColumnOris, blockRows (core/sparseMatrix.h:627-635), and the 78-entry

@@ -108,6 +108,12 @@ class Mesh:
         (self.ndof, self.n_rows, self.n_blocks, self.n_values, self.n_levels, self.n_inst, self.n_tets,
          self.n_nodes, self.n_chunks, self.n_pair_contrib, self.n_node_contrib, self.device_bytes) = (int(v) for v in info[:12])
 
+    def info_raw(self):
+        """The 16 values of od_mesh_info (read now: the sweep lanes change when a solver tunes them)."""
+        info = np.zeros(16, np.int64)
+        check(lib().od_mesh_info(self.h, info.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return info
+
     def close(self):
         if self.h:
             lib().od_mesh_destroy(self.h)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
@@ -44,6 +45,10 @@ __global__ void k_done_to_mask(int n, const InstScalars *sc, const uint8_t *extr
 struct od_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // meshes bound to this context; od_ctx_destroy with meshes still bound defers the release
+    // of the stream until the last of them is destroyed or rebound
+    std::atomic<int> meshes{0};
+    std::atomic<bool> closed{false};
     int *pinned = nullptr;   // host-pinned mailbox for convergence polling
     int pinnedCount = 0;
     // optional HIP-event timing of the PCG SpMV launches (measurement mode)
@@ -86,7 +91,7 @@ struct od_mesh {
     bool spmvTuned = false;                      // SpMV variant chosen (autotuner or caller)
     double tuneMs[2][3] = {{0, 0, 0}, {0, 0, 0}};   // [FP64, FP32][R = 1, 2, 4]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
-    std::vector<uint8_t> levelWideFwd, levelWideBwd;   // DevMesh points at these (host memory)
+    std::vector<uint8_t> levelLanesFwd, levelLanesBwd;   // DevMesh points at these (host memory)
     template <typename T>
     T *upload(const std::vector<T> &v) {
         if (v.empty()) return nullptr;
@@ -314,6 +319,11 @@ static void autotune_spmv(od_mesh *mesh) {
             if (mesh->tuneMs[prec][v] < mesh->tuneMs[prec][best[prec]]) best[prec] = v;
     m.spmvR64 = kSpmvVariants[best[0]];
     m.spmvR32 = kSpmvVariants[best[1]];
+    // lanes per row of the IC(0) sweeps, per level (meshes with few levels)
+    if (t.nLevels <= 64) {
+        tune_ic0_lanes(m, st, t.levelPtr, t.nValues, (int64_t)t.tRow.size(), mesh->levelLanesFwd.data(),
+                       mesh->levelLanesBwd.data());
+    }
     mesh->spmvTuned = true;
 }
 
@@ -578,14 +588,25 @@ int od_ctx_create(int device, od_ctx **out) {
     API_END
 }
 
-int od_ctx_destroy(od_ctx *ctx) {
-    API_BEGIN
-    if (!ctx) return OD_OK;
+static void ctx_free(od_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipHostFree(ctx->pinned);
     delete ctx;
+}
+
+// A mesh stops using ctx (destroyed or rebound): the last one frees a closed context.
+static void ctx_release(od_ctx *ctx) {
+    if (ctx->meshes.fetch_sub(1) == 1 && ctx->closed.load()) ctx_free(ctx);
+}
+
+int od_ctx_destroy(od_ctx *ctx) {
+    API_BEGIN
+    if (!ctx) return OD_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->closed.store(true);
+    if (ctx->meshes.load() == 0) ctx_free(ctx);   // else the last mesh frees it
     API_END
 }
 
@@ -710,8 +731,10 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.rowCStart = mesh->upload(t.rowCStart);
         m.levelRows = mesh->upload(t.levelRows);
         m.rowLevel = mesh->upload(t.level);
-        mesh->levelWideFwd.assign(std::max(1, t.nLevels), 0);
-        mesh->levelWideBwd.assign(std::max(1, t.nLevels), 0);
+        // lanes per row before tuning: 16 where a level's rows average more than 8 blocks on
+        // the swept side, else 8
+        mesh->levelLanesFwd.assign(std::max(1, t.nLevels), 8);
+        mesh->levelLanesBwd.assign(std::max(1, t.nLevels), 8);
         for (int l = 0; l < t.nLevels; l++) {
             double nT = 0, nU = 0;
             for (int q = t.levelPtr[l]; q < t.levelPtr[l + 1]; q++) {
@@ -720,11 +743,11 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
                 nU += t.rowPtr[I + 1] - t.rowPtr[I] - 1;
             }
             const double cnt = std::max(1, t.levelPtr[l + 1] - t.levelPtr[l]);
-            mesh->levelWideFwd[l] = nT / cnt > 8.0;
-            mesh->levelWideBwd[l] = nU / cnt > 8.0;
+            mesh->levelLanesFwd[l] = nT / cnt > 8.0 ? 16 : 8;
+            mesh->levelLanesBwd[l] = nU / cnt > 8.0 ? 16 : 8;
         }
-        m.levelWideFwd = mesh->levelWideFwd.data();
-        m.levelWideBwd = mesh->levelWideBwd.data();
+        m.levelLanesFwd = mesh->levelLanesFwd.data();
+        m.levelLanesBwd = mesh->levelLanesBwd.data();
         m.levelsContiguous = t.levelsContiguous ? 1 : 0;
         m.chunkRow0 = mesh->upload(t.chunkRow0);
         m.chunkRow1 = mesh->upload(t.chunkRow1);
@@ -774,6 +797,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         delete mesh;
         throw;
     }
+    ctx->meshes.fetch_add(1);
     *out = mesh;
     API_END
 }
@@ -784,7 +808,10 @@ int od_mesh_bind_ctx(od_mesh *mesh, od_ctx *ctx) {
     require(ctx->device == mesh->ctx->device, "context of another device");
     if (ctx != mesh->ctx) {
         OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
+        od_ctx *old = mesh->ctx;
+        ctx->meshes.fetch_add(1);
         mesh->ctx = ctx;
+        ctx_release(old);
     }
     API_END
 }
@@ -792,8 +819,10 @@ int od_mesh_bind_ctx(od_mesh *mesh, od_ctx *ctx) {
 int od_mesh_destroy(od_mesh *mesh) {
     API_BEGIN
     if (mesh) {
-        (void)hipStreamSynchronize(mesh->ctx->stream);
+        od_ctx *ctx = mesh->ctx;
+        (void)hipStreamSynchronize(ctx->stream);
         delete mesh;
+        ctx_release(ctx);
     }
     API_END
 }
@@ -806,6 +835,13 @@ int od_mesh_info(od_mesh *mesh, long long *info) {
     info[0] = t.ndof; info[1] = t.nRows; info[2] = t.nBlocks; info[3] = t.nValues; info[4] = t.nLevels;
     info[5] = t.nInst; info[6] = t.nTets; info[7] = t.nNodes; info[8] = (long long)t.chunkRow0.size();
     info[9] = t.nPairContrib; info[10] = t.nNodeContrib; info[11] = (long long)mesh->bytes;
+    info[12] = t.levelsContiguous ? 1 : 0;
+    // log2 of the lanes per row of the first 8 levels' forward and backward sweeps, 4 bits each
+    auto lg = [](int v) { return v == 16 ? 4 : (v == 4 ? 2 : 3); };
+    for (int l = 0; l < std::min(8, t.nLevels); l++) {
+        info[13] |= (long long)lg(mesh->levelLanesFwd[l]) << (4 * l);
+        info[14] |= (long long)lg(mesh->levelLanesBwd[l]) << (4 * l);
+    }
     API_END
 }
 

@@ -48,9 +48,9 @@ struct DevMesh {
     int *levelRows;                              // nRows
     int levelsContiguous;                        // levels are consecutive row ranges (levelRows is the identity)
     int *rowLevel;                               // IC(0) dependency level of each row
-    // per level, in host memory (read at launch, never on the device): rows average more than
-    // 8 blocks on the side the forward / backward sweep reads (run 16 lanes per row instead of 8)
-    const uint8_t *levelWideFwd, *levelWideBwd;
+    // per level, in host memory (read at launch, never on the device): lanes per row (4, 8 or
+    // 16) of the forward and backward sweeps
+    const uint8_t *levelLanesFwd, *levelLanesBwd;
     int *chunkRow0, *chunkRow1, *chunkInst;      // nChunks
     int *chunkSorted;     // chunks in spatial order, cut into 8 runs of perRun (one run per XCD)
     int perRun;
@@ -141,6 +141,8 @@ void pcg_update_xr_ic0(const DevMesh &m, hipStream_t s, const std::vector<int> &
                        T *r, const T *U, const T *Ul, T *z, PcgWork &w, const uint8_t *skip);
 template <typename T>
 void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *invDiag);
+void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, int64_t nValues, int64_t nT,
+                    uint8_t *fwd, uint8_t *bwd);
 template <typename T>
 void launch_ic0_factor(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const double *A, double *U,
                        T *U32, double *Ul, T *Ul32, double pivotEps, const uint8_t *skip);

@@ -1087,6 +1087,25 @@ __device__ __forceinline__ void diag_bwd(const T *D, T &y0, T &y1, T &y2) {
     y0 = (y0 - D[1] * y1 - D[2] * y2) / D[0];
 }
 
+// Lanes per row L (4, 8 or 16) is a launch choice only.  Each lane keeps the sums of 16 / L
+// virtual lanes (v = lane + g L, which take blocks v, v + 16, ... of the row) and combines
+// them in the order of a 16-lane shuffle tree (pairs at distance 8, then 4, 2, 1), so every
+// L gives the bits of 16 lanes.
+template <int L, typename T>
+__device__ __forceinline__ void combine16(const T (&acc)[16 / L][3], T &a0, T &a1, T &a2) {
+    if constexpr (L == 16) {
+        a0 = acc[0][0]; a1 = acc[0][1]; a2 = acc[0][2];
+    } else if constexpr (L == 8) {
+        a0 = acc[0][0] + acc[1][0]; a1 = acc[0][1] + acc[1][1]; a2 = acc[0][2] + acc[1][2];
+    } else {
+        static_assert(L == 4, "4, 8 or 16 lanes per row");
+        a0 = (acc[0][0] + acc[2][0]) + (acc[1][0] + acc[3][0]);
+        a1 = (acc[0][1] + acc[2][1]) + (acc[1][1] + acc[3][1]);
+        a2 = (acc[0][2] + acc[2][2]) + (acc[1][2] + acc[3][2]);
+    }
+    sum_lanes<L>(a0, a1, a2);
+}
+
 // forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the
 // lower copy (contiguous per row I)
 template <typename T, int L>
@@ -1094,22 +1113,32 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_fwd(DevMesh m, int p0, int c
                                                         const T *__restrict__ Ul, const T *__restrict__ r,
                                                         T *__restrict__ y, const InstScalars *__restrict__ sc,
                                                         const uint8_t *__restrict__ skip, int lastLevel) {
-    constexpr int kRows = kIcThreads / L;
+    constexpr int kRows = kIcThreads / L, G = 16 / L;
     const int wg = xcd_wg((count + kRows - 1) / kRows);
     const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
     if (pl >= count) return;
     const int I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
     const int inst = m.rowInst[I];
     if ((sc && sc[inst].done) || (skip && skip[inst])) return;
-    T a0 = 0, a1 = 0, a2 = 0;
-    for (int k = m.tPtr[I] + lane; k < m.tPtr[I + 1]; k += L) {
-        BlockX<T> b;
-        load_block<T>(Ul + 9 * k, y + 3 * m.tRow[k], b);
-        a0 += b.e[0] * b.x[0] + b.e[3] * b.x[1] + b.e[6] * b.x[2];
-        a1 += b.e[1] * b.x[0] + b.e[4] * b.x[1] + b.e[7] * b.x[2];
-        a2 += b.e[2] * b.x[0] + b.e[5] * b.x[1] + b.e[8] * b.x[2];
+    const int kb = m.tPtr[I], ke = m.tPtr[I + 1];
+    T acc[G][3];
+#pragma unroll
+    for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
+    for (int kk = kb + lane; kk < ke; kk += 16) {
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int k = kk + g * L;
+            if (k < ke) {
+                BlockX<T> b;
+                load_block<T>(Ul + 9 * k, y + 3 * m.tRow[k], b);
+                acc[g][0] += b.e[0] * b.x[0] + b.e[3] * b.x[1] + b.e[6] * b.x[2];
+                acc[g][1] += b.e[1] * b.x[0] + b.e[4] * b.x[1] + b.e[7] * b.x[2];
+                acc[g][2] += b.e[2] * b.x[0] + b.e[5] * b.x[1] + b.e[8] * b.x[2];
+            }
+        }
     }
-    sum_lanes<L>(a0, a1, a2);
+    T a0, a1, a2;
+    combine16<L>(acc, a0, a1, a2);
     if (lane != 0) return;
     const T *D = U + m.valOff[I];
     T z0 = (r[3 * I] - a0) / D[0];
@@ -1124,7 +1153,7 @@ template <typename T, int L>
 __global__ __launch_bounds__(kIcThreads) void k_ic0_bwd(DevMesh m, int p0, int count, const T *__restrict__ U,
                                                         T *__restrict__ z, const InstScalars *__restrict__ sc,
                                                         const uint8_t *__restrict__ skip) {
-    constexpr int kRows = kIcThreads / L;
+    constexpr int kRows = kIcThreads / L, G = 16 / L;
     const int wg = xcd_wg((count + kRows - 1) / kRows);
     const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
     if (pl >= count) return;
@@ -1132,15 +1161,24 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_bwd(DevMesh m, int p0, int c
     const int inst = m.rowInst[I];
     if ((sc && sc[inst].done) || (skip && skip[inst])) return;
     const int v0 = m.valOff[I], b0 = m.rowPtr[I], nOff = m.rowPtr[I + 1] - b0 - 1;
-    T a0 = 0, a1 = 0, a2 = 0;
-    for (int k = lane; k < nOff; k += L) {
-        BlockX<T> b;
-        load_block<T>(U + v0 + 6 + 9 * k, z + 3 * m.colIdx[b0 + 1 + k], b);
-        a0 += b.e[0] * b.x[0] + b.e[1] * b.x[1] + b.e[2] * b.x[2];
-        a1 += b.e[3] * b.x[0] + b.e[4] * b.x[1] + b.e[5] * b.x[2];
-        a2 += b.e[6] * b.x[0] + b.e[7] * b.x[1] + b.e[8] * b.x[2];
+    T acc[G][3];
+#pragma unroll
+    for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
+    for (int kk = lane; kk < nOff; kk += 16) {
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int k = kk + g * L;
+            if (k < nOff) {
+                BlockX<T> b;
+                load_block<T>(U + v0 + 6 + 9 * k, z + 3 * m.colIdx[b0 + 1 + k], b);
+                acc[g][0] += b.e[0] * b.x[0] + b.e[1] * b.x[1] + b.e[2] * b.x[2];
+                acc[g][1] += b.e[3] * b.x[0] + b.e[4] * b.x[1] + b.e[5] * b.x[2];
+                acc[g][2] += b.e[6] * b.x[0] + b.e[7] * b.x[1] + b.e[8] * b.x[2];
+            }
+        }
     }
-    sum_lanes<L>(a0, a1, a2);
+    T a0, a1, a2;
+    combine16<L>(acc, a0, a1, a2);
     if (lane != 0) return;
     const T *D = U + v0;
     const T x2 = (z[3 * I + 2] - a2) / D[5];
@@ -1149,34 +1187,100 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_bwd(DevMesh m, int p0, int c
     z[3 * I] = x0; z[3 * I + 1] = x1; z[3 * I + 2] = x2;
 }
 
+// One sweep launch of level l with L lanes per row (L = 4, 8 or 16); the optional events are
+// recorded from the dispatch's own begin/end timestamps (lane autotuner).
+template <typename T>
+static void launch_sweep(bool fwd, int L, const DevMesh &m, hipStream_t s, int p0, int cnt, const T *U, const T *Ul,
+                         const T *r, T *z, const InstScalars *sc, const uint8_t *skip, int last,
+                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    auto go = [&](auto kernel, int rowsPerWg, auto... args) {
+        const dim3 grid(xcd_grid((cnt + rowsPerWg - 1) / rowsPerWg)), block(kIcThreads);
+        if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, m, p0, cnt, args...);
+        else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, p0, cnt, args...);
+    };
+    if (fwd) {
+        if (L == 16) go(k_ic0_fwd<T, 16>, kIcThreads / 16, U, Ul, r, z, sc, skip, last);
+        else if (L == 4) go(k_ic0_fwd<T, 4>, kIcThreads / 4, U, Ul, r, z, sc, skip, last);
+        else go(k_ic0_fwd<T, 8>, kIcThreads / 8, U, Ul, r, z, sc, skip, last);
+    } else {
+        if (L == 16) go(k_ic0_bwd<T, 16>, kIcThreads / 16, U, z, sc, skip);
+        else if (L == 4) go(k_ic0_bwd<T, 4>, kIcThreads / 4, U, z, sc, skip);
+        else go(k_ic0_bwd<T, 8>, kIcThreads / 8, U, z, sc, skip);
+    }
+}
+
 // Sweep launches for forward levels [f0, nl) (the last one also solving its backward step
-// when fuseLast) and backward levels [0, b1).
+// when fuseLast) and backward levels [0, b1), with each level's lanes per row.
 template <typename T>
 static void ic0_levels(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, int f0, bool fuseLast, int b1,
                        const T *U, const T *Ul, const T *r, T *z, const InstScalars *sc, const uint8_t *skip) {
     const int nl = (int)levelPtr.size() - 1;
     for (int l = f0; l < nl; l++) {
-        int cnt = levelPtr[l + 1] - levelPtr[l];
+        const int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
         const int last = (fuseLast && l == nl - 1) ? 1 : 0;
-        if (m.levelWideFwd[l])
-            hipLaunchKernelGGL((k_ic0_fwd<T, 16>), dim3(xcd_grid((cnt + 15) / 16)), dim3(kIcThreads), 0, s, m,
-                               levelPtr[l], cnt, U, Ul, r, z, sc, skip, last);
-        else
-            hipLaunchKernelGGL((k_ic0_fwd<T, 8>), dim3(xcd_grid((cnt + 31) / 32)), dim3(kIcThreads), 0, s, m,
-                               levelPtr[l], cnt, U, Ul, r, z, sc, skip, last);
+        launch_sweep<T>(true, m.levelLanesFwd[l], m, s, levelPtr[l], cnt, U, Ul, r, z, sc, skip, last);
     }
     for (int l = b1 - 1; l >= 0; l--) {
-        int cnt = levelPtr[l + 1] - levelPtr[l];
+        const int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        if (m.levelWideBwd[l])
-            hipLaunchKernelGGL((k_ic0_bwd<T, 16>), dim3(xcd_grid((cnt + 15) / 16)), dim3(kIcThreads), 0, s, m,
-                               levelPtr[l], cnt, U, z, sc, skip);
-        else
-            hipLaunchKernelGGL((k_ic0_bwd<T, 8>), dim3(xcd_grid((cnt + 31) / 32)), dim3(kIcThreads), 0, s, m,
-                               levelPtr[l], cnt, U, z, sc, skip);
+        launch_sweep<T>(false, m.levelLanesBwd[l], m, s, levelPtr[l], cnt, U, Ul, r, z, sc, skip, 0);
     }
     OD_HIP(hipGetLastError());
+}
+
+// Lanes per row for each level's FP32 sweeps: times 4, 8 and 16 lanes (median of three
+// launches each, on zero-filled factors of this mesh's pattern) and keeps the fastest.
+// The last level's forward launch is timed with its fused backward step, as the solver runs it.
+void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, int64_t nValues, int64_t nT,
+                    uint8_t *fwd, uint8_t *bwd) {
+    const int nl = (int)levelPtr.size() - 1;
+    const size_t n = 3 * (size_t)m.nRows;
+    std::vector<void *> bufs;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto get = [&](size_t bytes) {
+        void *p = nullptr;
+        OD_HIP(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+        bufs.push_back(p);
+        OD_HIP(hipMemsetAsync(p, 0, std::max<size_t>(bytes, 16), s));
+        return p;
+    };
+    try {
+        float *U = (float *)get(4 * ((size_t)nValues + kValuePad)), *Ul = (float *)get(4 * (9 * (size_t)nT + kValuePad));
+        float *r = (float *)get(4 * n), *z = (float *)get(4 * n);
+        OD_HIP(hipEventCreate(&e0));
+        OD_HIP(hipEventCreate(&e1));
+        const int lanes[3] = {4, 8, 16};
+        for (int dir = 0; dir < 2; dir++)
+            for (int l = 0; l < nl; l++) {
+                const int cnt = levelPtr[l + 1] - levelPtr[l];
+                if (cnt <= 0) continue;
+                float best = 0.f;
+                for (int v = 0; v < 3; v++) {
+                    float ms[3];
+                    for (int rep = 0; rep < 3; rep++) {
+                        launch_sweep<float>(dir == 0, lanes[v], m, s, levelPtr[l], cnt, U, Ul, r, z, nullptr, nullptr,
+                                            dir == 0 && l == nl - 1 ? 1 : 0, e0, e1);
+                        OD_HIP(hipGetLastError());
+                        OD_HIP(hipEventSynchronize(e1));
+                        OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
+                    }
+                    std::sort(ms, ms + 3);
+                    if (v == 0 || ms[1] < best) {
+                        best = ms[1];
+                        (dir == 0 ? fwd : bwd)[l] = (uint8_t)lanes[v];
+                    }
+                }
+            }
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        for (void *p : bufs) (void)hipFree(p);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (void *p : bufs) (void)hipFree(p);
 }
 
 template <typename T>
