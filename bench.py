@@ -122,7 +122,7 @@ def batched_leg(args, n_inst=256, warmup=2, steps=20):
     ctx = od.Context(0)
     mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs, w.inst_node_start, w.inst_tet_start)
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
-    _, r64, r32 = od.mesh_autotune(mesh)
+    _, v64, v32 = od.mesh_autotune(mesh)
     nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=capi.OD_MIXED, tol=1e-6)
     nm.set_graph(True)
     nm.set_external_virtual_work(mesh.body_force(w.body))
@@ -144,7 +144,7 @@ def batched_leg(args, n_inst=256, warmup=2, steps=20):
         wi = W.subset(w, i)
         mi = od.Mesh(ctx, wi.X, wi.tets, wi.fixed_dofs)
         mi.set_material(wi.kind, wi.E, wi.nu, wi.rho)
-        od.mesh_spmv_variant(mi, r64, r32)
+        od.mesh_spmv_variant(mi, v64, v32)
         ni = od.NewmarkIntegrator(mi, wi.dt, precond=capi.OD_PRE_IC0, precision=capi.OD_FP64, tol=1e-6)
         ni.set_external_virtual_work(mi.body_force(wi.body))
         ni.run_time_steps(2)
@@ -165,7 +165,8 @@ def batched_leg(args, n_inst=256, warmup=2, steps=20):
             "plain_fp64_alone_instance_steps_per_s": round(alone_rate, 1),
             "plain_fp64_alone_sample": f"instances 0, 16, ..., {n_inst - 16}: 10 steps each, one at a time",
             "speedup_vs_plain_fp64_alone": round((n_inst / dt) / alone_rate, 2),
-            "spmv_chunks_per_workgroup": {"fp64": r64, "fp32": r32}}
+            "spmv_shape": {"fp64": {"chunks_per_workgroup": v64[0], "lanes_per_row": v64[1]},
+                           "fp32": {"chunks_per_workgroup": v32[0], "lanes_per_row": v32[1]}}}
 
 
 def serving_leg(args, n_obj=64, tet_lo=1000, tet_hi=50000, rates=(50.0, 200.0), duration=4.0):
@@ -339,7 +340,7 @@ def main():
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
     f = mesh.body_force(w.body)
     pre = {"jacobi": capi.OD_PRE_JACOBI, "ic0": capi.OD_PRE_IC0}[args.precond]
-    tune_table, r64, r32 = od.mesh_autotune(mesh)
+    tune_table, v64, v32 = od.mesh_autotune(mesh)
 
     def run_form(precision):
         """Warm-up, then the timed steps; returns the measurements and the final displacement."""
@@ -372,14 +373,15 @@ def main():
         return {"steps_s": args.steps / elapsed, "ms_step": elapsed / args.steps * 1e3,
                 "iters": float(np.mean(iters)), "relres_max": float(np.max(relres)),
                 "outer": float(np.mean(outer)),
-                "spmv": {"kernel": (f"k_spmv_dot<double, {r64}, false>" if vb == 8 else f"k_spmv_dot<float, {r32}, true>"),
+                "spmv": {"kernel": (f"k_spmv_dot<double, {v64[0]}, false, {v64[1]}>" if vb == 8
+                                    else f"k_spmv_dot<float, {v32[0]}, true, {v32[1]}>"),
                          "launches": spmv_n, "iterations": real_spmv,
                          "avg_us_per_launch": round(spmv_ms / max(spmv_n, 1) * 1e3, 2),
                          "avg_us_per_iteration": round(avg_spmv_s * 1e6, 2),
                          "algorithmic_MB": round(by / 1e6, 2), "GBps": round(achieved, 1)},
                 "achieved": achieved}, d
 
-    log(f"{w.name}: {mesh.n_tets} tets, {mesh.ndof} DOF; autotune {r64}/{r32}; timing the {args.precision} form")
+    log(f"{w.name}: {mesh.n_tets} tets, {mesh.ndof} DOF; SpMV shapes {v64}/{v32}; timing the {args.precision} form")
     main_r, d_main = run_form(args.precision)
     log(f"{args.precision}: {main_r['steps_s']:.2f} steps/s")
     fp64_r = None
@@ -405,8 +407,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
-    line["spmv_autotune"] = {"chunks_per_workgroup": {"fp64": r64, "fp32": r32},
-                             "cold_us": {f"{p}_R{r}": round(v * 1e3, 2) for (p, r), v in tune_table.items()}}
+    line["spmv_autotune"] = {"fp64": {"chunks_per_workgroup": v64[0], "lanes_per_row": v64[1]},
+                             "fp32": {"chunks_per_workgroup": v32[0], "lanes_per_row": v32[1]},
+                             "cold_us": {f"{p}_L{ls}_R{r}": round(v * 1e3, 2) for (p, ls, r), v in tune_table.items()}}
     info = mesh.info_raw()
     lanes = lambda word: [1 << ((int(word) >> (4 * l)) & 15) for l in range(min(8, mesh.n_levels))]
     line["ic0_sweeps"] = {"contiguous_levels": bool(info[12]), "lanes_per_row_fwd": lanes(info[13]),

@@ -67,7 +67,7 @@ def lib():
             "od_spmv": [vp, vp, vp, i], "od_matrix_sync_f32": [vp], "od_spmv_f32": [vp, vp, vp, i],
             "od_solver_create": [vp, i, d, i, pvp], "od_solver_destroy": [vp],
             "od_solver_set_refinement": [vp, d, i], "od_integrator_set_refinement": [vp, d, i],
-            "od_mesh_autotune": [vp, dp, ip], "od_mesh_spmv_variant": [vp, i, i],
+            "od_mesh_autotune": [vp, dp, ip], "od_mesh_spmv_variant": [vp, i, i, i, i],
             "od_solver_set_graph": [vp, i], "od_integrator_set_graph": [vp, i],
             "od_mesh_bind_ctx": [vp, vp], "od_integrator_device_bytes": [vp, C.POINTER(ll)],
             "od_server_create": [i, ll, i, i, i, d, pvp], "od_server_destroy": [vp],

@@ -165,15 +165,17 @@ class Mesh:
 
 
 def mesh_autotune(mesh: "Mesh"):
-    """Reruns the SpMV autotuner: ({(precision, R): ms}, chosen R for FP64, chosen R for FP32)."""
-    ms = np.zeros(6); ch = np.zeros(2, np.int32)
+    """Reruns the autotuner: ({(precision, lanes, R): ms}, (R, lanes) for FP64, (R, lanes) for FP32)."""
+    ms = np.zeros(12); ch = np.zeros(4, np.int32)
     check(lib().od_mesh_autotune(mesh.h, dptr(ms), capi.iptr(ch)))
-    table = {(p, r): float(ms[3 * i + j]) for i, p in enumerate(("fp64", "fp32")) for j, r in enumerate((1, 2, 4))}
-    return table, int(ch[0]), int(ch[1])
+    table = {(p, ls, r): float(ms[6 * i + 3 * k + j]) for i, p in enumerate(("fp64", "fp32"))
+             for k, ls in enumerate((16, 8)) for j, r in enumerate((1, 2, 4))}
+    return table, (int(ch[0]), int(ch[1])), (int(ch[2]), int(ch[3]))
 
 
-def mesh_spmv_variant(mesh: "Mesh", r64, r32):
-    check(lib().od_mesh_spmv_variant(mesh.h, r64, r32))
+def mesh_spmv_variant(mesh: "Mesh", v64, v32):
+    """Sets the SpMV launch shapes: v64, v32 = (chunks per workgroup R, lanes per row)."""
+    check(lib().od_mesh_spmv_variant(mesh.h, int(v64[0]), int(v64[1]), int(v32[0]), int(v32[1])))
 
 
 class Matrix:

@@ -89,7 +89,7 @@ struct od_mesh {
     size_t bytes = 0;
     bool hasMaterial = false;
     bool spmvTuned = false;                      // SpMV variant chosen (autotuner or caller)
-    double tuneMs[2][3] = {{0, 0, 0}, {0, 0, 0}};   // [FP64, FP32][R = 1, 2, 4]: cold launch time
+    double tuneMs[2][2][3] = {};   // [FP64, FP32][16, 8 lanes per row][R = 1, 2, 4]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
     std::vector<uint8_t> levelLanesFwd, levelLanesBwd;   // DevMesh points at these (host memory)
     template <typename T>
@@ -286,24 +286,25 @@ static void autotune_spmv(od_mesh *mesh) {
         OD_HIP(hipEventCreate(&e0));
         OD_HIP(hipEventCreate(&e1));
         for (int prec = 0; prec < 2; prec++)
-            for (int v = 0; v < 3; v++) {
-                const int R = kSpmvVariants[v];
-                float ms[3];
-                for (int rep = 0; rep < 3; rep++) {
-                    hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, st, (const float4 *)flush, flushN / 4,
-                                       flush);
-                    if (prec == 0)
-                        launch_spmv_dot<double>(m, st, R, (const double *)vals, (const double *)x, (double *)y, partials,
-                                                sc, e0, e1);
-                    else
-                        launch_spmv_dot<float>(m, st, R, (const float *)vals, (const float *)x, (float *)y, partials,
-                                               sc, e0, e1, (const float *)z, (float *)pNew);
-                    OD_HIP(hipEventSynchronize(e1));
-                    OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
+            for (int lv = 0; lv < 2; lv++)
+                for (int v = 0; v < 3; v++) {
+                    const int R = kSpmvVariants[v], LS = kSpmvLaneVariants[lv];
+                    float ms[3];
+                    for (int rep = 0; rep < 3; rep++) {
+                        hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, st, (const float4 *)flush,
+                                           flushN / 4, flush);
+                        if (prec == 0)
+                            launch_spmv_dot<double>(m, st, R, LS, (const double *)vals, (const double *)x, (double *)y,
+                                                    partials, sc, e0, e1);
+                        else
+                            launch_spmv_dot<float>(m, st, R, LS, (const float *)vals, (const float *)x, (float *)y,
+                                                   partials, sc, e0, e1, (const float *)z, (float *)pNew);
+                        OD_HIP(hipEventSynchronize(e1));
+                        OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
+                    }
+                    std::sort(ms, ms + 3);
+                    mesh->tuneMs[prec][lv][v] = ms[1];
                 }
-                std::sort(ms, ms + 3);
-                mesh->tuneMs[prec][v] = ms[1];
-            }
     } catch (...) {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
@@ -313,12 +314,18 @@ static void autotune_spmv(od_mesh *mesh) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     for (void *p : bufs) (void)hipFree(p);
-    int best[2] = {0, 0};
+    int best[2][2] = {{0, 0}, {0, 0}};   // [precision] -> (lane variant, R variant)
     for (int prec = 0; prec < 2; prec++)
-        for (int v = 1; v < 3; v++)
-            if (mesh->tuneMs[prec][v] < mesh->tuneMs[prec][best[prec]]) best[prec] = v;
-    m.spmvR64 = kSpmvVariants[best[0]];
-    m.spmvR32 = kSpmvVariants[best[1]];
+        for (int lv = 0; lv < 2; lv++)
+            for (int v = 0; v < 3; v++)
+                if (mesh->tuneMs[prec][lv][v] < mesh->tuneMs[prec][best[prec][0]][best[prec][1]]) {
+                    best[prec][0] = lv;
+                    best[prec][1] = v;
+                }
+    m.spmvR64 = kSpmvVariants[best[0][1]];
+    m.spmvL64 = kSpmvLaneVariants[best[0][0]];
+    m.spmvR32 = kSpmvVariants[best[1][1]];
+    m.spmvL32 = kSpmvLaneVariants[best[1][0]];
     // lanes per row of the IC(0) sweeps, per level (meshes with few levels)
     if (t.nLevels <= 64) {
         tune_ic0_lanes(m, st, t.levelPtr, t.nValues, (int64_t)t.tRow.size(), mesh->levelLanesFwd.data(),
@@ -755,6 +762,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.chunkSorted = mesh->upload(t.chunkSorted);
         m.perRun = t.perRun;
         m.spmvR64 = m.spmvR32 = 1;
+        m.spmvL64 = m.spmvL32 = kSpmvLanes;
         {
             // SpMV slots: per chunk, kSpmvLanes per row, rows in order (row-loop path for longer rows)
             const int nc = m.nChunks;
@@ -903,13 +911,17 @@ int od_mesh_plastic_strain(od_mesh *mesh, double *get, const double *set) {
     API_END
 }
 
-int od_mesh_spmv_variant(od_mesh *mesh, int r64, int r32) {
+int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes32) {
     API_BEGIN
     require(mesh != nullptr, "NULL argument");
-    auto ok = [](int r) { return r == 1 || r == 2 || r == 4; };
-    require(ok(r64) && ok(r32), "chunks per SpMV workgroup must be 1, 2 or 4");
+    auto okR = [](int r) { return r == 1 || r == 2 || r == 4; };
+    auto okL = [](int l) { return l == 8 || l == 16; };
+    require(okR(r64) && okR(r32), "chunks per SpMV workgroup must be 1, 2 or 4");
+    require(okL(lanes64) && okL(lanes32), "lanes per SpMV block row must be 8 or 16");
     mesh->dm.spmvR64 = r64;
+    mesh->dm.spmvL64 = lanes64;
     mesh->dm.spmvR32 = r32;
+    mesh->dm.spmvL32 = lanes32;
     mesh->spmvTuned = true;
     API_END
 }
@@ -921,8 +933,12 @@ int od_mesh_autotune(od_mesh *mesh, double *ms, int *chosen) {
     autotune_spmv(mesh);
     if (ms)
         for (int p = 0; p < 2; p++)
-            for (int v = 0; v < 3; v++) ms[3 * p + v] = mesh->tuneMs[p][v];
-    if (chosen) { chosen[0] = mesh->dm.spmvR64; chosen[1] = mesh->dm.spmvR32; }
+            for (int lv = 0; lv < 2; lv++)
+                for (int v = 0; v < 3; v++) ms[6 * p + 3 * lv + v] = mesh->tuneMs[p][lv][v];
+    if (chosen) {
+        chosen[0] = mesh->dm.spmvR64; chosen[1] = mesh->dm.spmvL64;
+        chosen[2] = mesh->dm.spmvR32; chosen[3] = mesh->dm.spmvL32;
+    }
     API_END
 }
 

@@ -55,6 +55,7 @@ struct DevMesh {
     int *chunkSorted;     // chunks in spatial order, cut into 8 runs of perRun (one run per XCD)
     int perRun;
     int spmvR64, spmvR32; // chunks per SpMV workgroup (FP64 / FP32 values), set by the autotuner
+    int spmvL64, spmvL32; // lanes per block row of the SpMV (16 or 8), set by the autotuner
     int *slotOff, *slotCol;                      // kSpmvThreads per chunk (padding: slotCol = -1)
     uint8_t *chunkLong;   // 1 when a row of the chunk has more than kSpmvLanes blocks (row-loop path)
     int *instChunkStart, *instRowStart;          // nInst+1
@@ -98,10 +99,11 @@ void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *
 // With z set (the PCG product), x is the previous direction and the product runs on the new
 // direction p = z + beta x of each instance, written to pNew (see spmv_chunks).
 template <typename T>
-void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, const T *vals, const T *x, T *y, double *partials,
+void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *vals, const T *x, T *y, double *partials,
                      const InstScalars *sc, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
                      const T *z = nullptr, T *pNew = nullptr);
-constexpr int kSpmvVariants[3] = {1, 2, 4};
+constexpr int kSpmvVariants[3] = {1, 2, 4};        // chunks per workgroup
+constexpr int kSpmvLaneVariants[2] = {16, 8};      // lanes per block row
 
 // PCG building blocks (see od_kernels.hip for the exact phase semantics)
 struct PcgWork {

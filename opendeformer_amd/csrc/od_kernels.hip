@@ -600,43 +600,70 @@ __device__ __forceinline__ void block_mv(const BlockX<T> &b, bool diag, bool tra
     a2 += e6 * b.x[0] + e7 * b.x[1] + e8 * b.x[2];
 }
 
-template <typename T>
-__device__ __forceinline__ void sum16(T &a0, T &a1, T &a2) {
+template <int L, typename T>
+__device__ __forceinline__ void sum_lanes(T &a0, T &a1, T &a2) {
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) {
-        a0 += __shfl_xor(a0, o, 16);
-        a1 += __shfl_xor(a1, o, 16);
-        a2 += __shfl_xor(a2, o, 16);
+    for (int o = L / 2; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, L);
+        a1 += __shfl_xor(a1, o, L);
+        a2 += __shfl_xor(a2, o, L);
     }
 }
 
-// One block row with 16 lanes through the row pointers (rows with more than 16 blocks):
-// item 0 is the diagonal block, items 1..nOff the row's upper blocks (contiguous values),
-// the remaining items the transposed lower blocks reached through the transposed index.
+// Lanes per row L (4, 8 or 16) is a launch choice only.  Each lane keeps the sums of 16 / L
+// virtual lanes (v = lane + g L, which take blocks v, v + 16, ... of the row) and combines
+// them in the order of a 16-lane shuffle tree (pairs at distance 8, then 4, 2, 1), so every
+// L gives the bits of 16 lanes.
+template <int L, typename T>
+__device__ __forceinline__ void combine16(const T (&acc)[16 / L][3], T &a0, T &a1, T &a2) {
+    if constexpr (L == 16) {
+        a0 = acc[0][0]; a1 = acc[0][1]; a2 = acc[0][2];
+    } else if constexpr (L == 8) {
+        a0 = acc[0][0] + acc[1][0]; a1 = acc[0][1] + acc[1][1]; a2 = acc[0][2] + acc[1][2];
+    } else {
+        static_assert(L == 4, "4, 8 or 16 lanes per row");
+        a0 = (acc[0][0] + acc[2][0]) + (acc[1][0] + acc[3][0]);
+        a1 = (acc[0][1] + acc[2][1]) + (acc[1][1] + acc[3][1]);
+        a2 = (acc[0][2] + acc[2][2]) + (acc[1][2] + acc[3][2]);
+    }
+    sum_lanes<L>(a0, a1, a2);
+}
+
+// One block row through the row pointers (rows with more than 16 blocks): item 0 is the
+// diagonal block, items 1..nOff the row's upper blocks (contiguous values), the remaining
+// items the transposed lower blocks reached through the transposed index.  LS lanes per row,
+// each with the sums of 16 / LS virtual lanes (combine16: the bits of 16 lanes).
 // With FUSE, x is p_old and each gathered column is the direction z + beta p_old; the lane
 // of item 0 (lane 0) returns the row's own direction in px.
-template <typename T, bool FUSE>
+template <typename T, bool FUSE, int LS>
 __device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x, int r,
                                          int lane, T &o0, T &o1, T &o2, const T *__restrict__ z, T beta, T *px) {
+    constexpr int G = kSpmvLanes / LS;
     const int v0 = m.valOff[r], b0 = m.rowPtr[r];
     const int nOff = m.rowPtr[r + 1] - b0 - 1;
     const int t0 = m.tPtr[r], nItems = 1 + nOff + (m.tPtr[r + 1] - t0);
-    T a0 = 0, a1 = 0, a2 = 0;
-    for (int it = lane; it < nItems; it += kSpmvLanes) {
-        const bool up = it <= nOff;
-        const int k = t0 + (up ? 0 : it - nOff - 1);
-        const int col = up ? m.colIdx[b0 + it] : m.tRow[k];
-        const T *A = up ? vals + (it == 0 ? v0 : v0 + 6 + 9 * (it - 1)) : vals + m.tOff[k];
-        BlockX<T> b;
-        load_block<T>(A, x + 3 * col, b);
-        if (FUSE) {
-            dir_update<T>(z + 3 * col, beta, b);
-            if (it == 0) { px[0] = b.x[0]; px[1] = b.x[1]; px[2] = b.x[2]; }
+    T acc[G][3];
+#pragma unroll
+    for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
+    for (int it0 = lane; it0 < nItems; it0 += kSpmvLanes) {
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int it = it0 + g * LS;
+            if (it >= nItems) continue;
+            const bool up = it <= nOff;
+            const int k = t0 + (up ? 0 : it - nOff - 1);
+            const int col = up ? m.colIdx[b0 + it] : m.tRow[k];
+            const T *A = up ? vals + (it == 0 ? v0 : v0 + 6 + 9 * (it - 1)) : vals + m.tOff[k];
+            BlockX<T> b;
+            load_block<T>(A, x + 3 * col, b);
+            if (FUSE) {
+                dir_update<T>(z + 3 * col, beta, b);
+                if (it == 0) { px[0] = b.x[0]; px[1] = b.x[1]; px[2] = b.x[2]; }
+            }
+            block_mv<T>(b, it == 0, !up, acc[g][0], acc[g][1], acc[g][2]);
         }
-        block_mv<T>(b, it == 0, !up, a0, a1, a2);
     }
-    sum16(a0, a1, a2);
-    o0 = a0; o1 = a1; o2 = a2;
+    combine16<LS>(acc, o0, o1, o2);
 }
 
 // Chunk q (0-based) of the run of the XCD this workgroup runs on, or -1 past the run's end:
@@ -646,21 +673,25 @@ __device__ __forceinline__ int run_chunk(const DevMesh &m, int q) {
     return (q < m.perRun && idx < m.nChunks) ? m.chunkSorted[idx] : -1;
 }
 
-// y = A x (or y += A x) for R chunks per workgroup.  Every load is unconditional (slots of
-// padding, inactive or long-row chunks point at valid memory and their results are dropped),
-// so the slot loads of all R chunks, then all their block and x loads, are in flight together.
-// The dot partial x.y of each chunk goes to partials[chunk] when partials is set.
+// y = A x (or y += A x) for R chunks per workgroup, LS lanes per block row (16 or 8; 64 LS
+// threads).  A row has 16 slots; lane l takes slots l, l + LS, ... and the row sum combines
+// them in the order of a 16-lane tree (combine16), so every launch shape gives the same bits.
+// Every load is unconditional (slots of padding, inactive or long-row chunks point at valid
+// memory and their results are dropped), so the slot loads of all R chunks, then all their
+// block and x loads, are in flight together.  The dot partial x.y of each chunk (sum over its
+// 64 rows by one wave, a fixed order) goes to partials[chunk] when partials is set.
 //
 // With FUSE (the PCG product), x is the previous direction p_old: every gathered column is
 // the new direction p = z + beta p_old of its instance (dir_update), the row's own direction
 // (from its diagonal slot, lane 0) is written to pNew, and the dot partial is p.q.
-template <typename T, int R, bool FUSE>
+template <typename T, int R, bool FUSE, int LS>
 __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x,
                                             T *__restrict__ y, int accumulate, double *__restrict__ partials,
                                             const InstScalars *__restrict__ sc, const T *__restrict__ z,
                                             T *__restrict__ pNew) {
-    const int j = (int)(blockIdx.x >> 3), lane = threadIdx.x & 15, lr = threadIdx.x >> 4;
-    int chunk[R], r[R], off[R], cf[R];
+    constexpr int G = kSpmvLanes / LS;
+    const int j = (int)(blockIdx.x >> 3), lane = (int)threadIdx.x % LS, lr = (int)threadIdx.x / LS;
+    int chunk[R], r[R], off[R][G], cf[R][G];
     bool act[R], lng[R];
     T beta[R];
 #pragma unroll
@@ -671,56 +702,69 @@ __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restric
         beta[i] = FUSE ? (T)sc[m.chunkInst[c]].beta : (T)0;
         lng[i] = m.chunkLong[c] != 0;
         r[i] = m.chunkRow0[c] + lr;
-        const size_t slot = (size_t)c * kSpmvThreads + threadIdx.x;
-        off[i] = m.slotOff[slot];
-        cf[i] = m.slotCol[slot];
-        if (!act[i] || lng[i]) cf[i] = -1;
-    }
-    BlockX<T> b[R];
 #pragma unroll
-    for (int i = 0; i < R; i++) {
-        const bool ok = cf[i] >= 0;
-        const int c3 = 3 * (ok ? (cf[i] & kSlotColMask) : 0);
-        load_block<T>(vals + (ok ? off[i] : 0), x + c3, b[i]);
-        if (FUSE) dir_update<T>(z + c3, beta[i], b[i]);
-    }
-    T a[R][3];
-#pragma unroll
-    for (int i = 0; i < R; i++) {
-        a[i][0] = a[i][1] = a[i][2] = 0;
-        block_mv<T>(b[i], (cf[i] & kSlotDiag) != 0, (cf[i] & kSlotTrans) != 0, a[i][0], a[i][1], a[i][2]);
-        if (cf[i] < 0) a[i][0] = a[i][1] = a[i][2] = 0;
-    }
-    double v[R];
-#pragma unroll
-    for (int i = 0; i < R; i++) {
-        v[i] = 0.0;
-        if (!act[i]) continue;                      // uniform over the workgroup
-        const bool inRow = r[i] < m.chunkRow1[chunk[i]];
-        // the row's own direction: its diagonal slot (lane 0) or item 0 of the row loop
-        T px[3] = {b[i].x[0], b[i].x[1], b[i].x[2]};
-        if (lng[i]) {
-            if (inRow) row_spmv<T, FUSE>(m, vals, x, r[i], lane, a[i][0], a[i][1], a[i][2], z, beta[i], px);
-        } else {
-            sum16(a[i][0], a[i][1], a[i][2]);
+        for (int g = 0; g < G; g++) {
+            const size_t slot = (size_t)c * kSpmvThreads + (size_t)lr * kSpmvLanes + lane + g * LS;
+            off[i][g] = m.slotOff[slot];
+            cf[i][g] = m.slotCol[slot];
+            if (!act[i] || lng[i]) cf[i][g] = -1;
         }
-        if (inRow && lane == 0) {
-            T o0 = a[i][0], o1 = a[i][1], o2 = a[i][2];
-            const int rr = r[i];
-            if (!FUSE) { px[0] = x[3 * rr]; px[1] = x[3 * rr + 1]; px[2] = x[3 * rr + 2]; }
-            if (FUSE) { pNew[3 * rr] = px[0]; pNew[3 * rr + 1] = px[1]; pNew[3 * rr + 2] = px[2]; }
-            if (accumulate) { o0 += y[3 * rr]; o1 += y[3 * rr + 1]; o2 += y[3 * rr + 2]; }
-            y[3 * rr] = o0; y[3 * rr + 1] = o1; y[3 * rr + 2] = o2;
-            if (partials)
-                v[i] = (double)px[0] * (double)o0 + (double)px[1] * (double)o1 + (double)px[2] * (double)o2;
+    }
+    BlockX<T> b[R][G];
+#pragma unroll
+    for (int i = 0; i < R; i++)
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const bool ok = cf[i][g] >= 0;
+            const int c3 = 3 * (ok ? (cf[i][g] & kSlotColMask) : 0);
+            load_block<T>(vals + (ok ? off[i][g] : 0), x + c3, b[i][g]);
+            if (FUSE) dir_update<T>(z + c3, beta[i], b[i][g]);
         }
+    T acc[R][G][3];
+#pragma unroll
+    for (int i = 0; i < R; i++)
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            acc[i][g][0] = acc[i][g][1] = acc[i][g][2] = 0;
+            block_mv<T>(b[i][g], (cf[i][g] & kSlotDiag) != 0, (cf[i][g] & kSlotTrans) != 0, acc[i][g][0],
+                        acc[i][g][1], acc[i][g][2]);
+            if (cf[i][g] < 0) acc[i][g][0] = acc[i][g][1] = acc[i][g][2] = 0;
+        }
+    __shared__ double rowDot[R][kChunkRows];
+    // row output (lane 0): y (or y +=), the new direction with FUSE, and the row's dot term
+    auto finish_row = [&](int i, T a0, T a1, T a2, const T *px) {
+        const int rr = r[i];
+        T d0 = px[0], d1 = px[1], d2 = px[2];
+        if (!FUSE) { d0 = x[3 * rr]; d1 = x[3 * rr + 1]; d2 = x[3 * rr + 2]; }
+        if (FUSE) { pNew[3 * rr] = d0; pNew[3 * rr + 1] = d1; pNew[3 * rr + 2] = d2; }
+        if (accumulate) { a0 += y[3 * rr]; a1 += y[3 * rr + 1]; a2 += y[3 * rr + 2]; }
+        y[3 * rr] = a0; y[3 * rr + 1] = a1; y[3 * rr + 2] = a2;
+        if (partials) rowDot[i][lr] = (double)d0 * (double)a0 + (double)d1 * (double)a1 + (double)d2 * (double)a2;
+    };
+    // short-row chunks first, from the slots loaded above; the row-loop chunks after, so their
+    // loop does not hold the slot blocks live (uniform branches over the workgroup)
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        if (partials && lane == 0) rowDot[i][lr] = 0.0;
+        if (!act[i] || lng[i] || r[i] >= m.chunkRow1[chunk[i]]) continue;
+        T a0, a1, a2;
+        combine16<LS>(acc[i], a0, a1, a2);
+        // the row's own direction: its diagonal, the first slot of lane 0
+        if (lane == 0) finish_row(i, a0, a1, a2, b[i][0].x);
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        if (!act[i] || !lng[i] || r[i] >= m.chunkRow1[chunk[i]]) continue;
+        T a0 = 0, a1 = 0, a2 = 0, px[3] = {0, 0, 0};
+        row_spmv<T, FUSE, LS>(m, vals, x, r[i], lane, a0, a1, a2, z, beta[i], px);   // item 0: lane 0's px
+        if (lane == 0) finish_row(i, a0, a1, a2, px);
     }
     if (partials) {
-        block_sum<R>(v);
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int i = 0; i < R; i++)
-                if (chunk[i] >= 0) partials[chunk[i]] = v[i];
+        __syncthreads();
+        const int w = (int)threadIdx.x >> 6;       // wave i sums chunk i's 64 rows
+        if (w < R) {
+            const double s = wave_sum(rowDot[w][threadIdx.x & 63]);
+            if ((threadIdx.x & 63) == 0 && chunk[w] >= 0) partials[chunk[w]] = s;
         }
     }
 }
@@ -728,56 +772,60 @@ __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restric
 template <typename T>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(DevMesh m, const T *__restrict__ vals, const T *__restrict__ x,
                                                        T *__restrict__ y, int accumulate, const InstScalars *sc) {
-    spmv_chunks<T, 1, false>(m, vals, x, y, accumulate, nullptr, sc, nullptr, nullptr);
+    spmv_chunks<T, 1, false, kSpmvLanes>(m, vals, x, y, accumulate, nullptr, sc, nullptr, nullptr);
 }
 
-// The PCG product q = A p with the chunk partials p.q (slot 0), R chunks per workgroup; with
-// FUSE the direction update p = z + beta p_old is done on the fly (x = p_old, p -> pNew).
-template <typename T, int R, bool FUSE>
-__global__ __launch_bounds__(kSpmvThreads) void k_spmv_dot(DevMesh m, const T *__restrict__ vals,
-                                                           const T *__restrict__ x, T *__restrict__ y,
-                                                           double *__restrict__ partials,
-                                                           const InstScalars *__restrict__ sc, const T *__restrict__ z,
-                                                           T *__restrict__ pNew) {
-    spmv_chunks<T, R, FUSE>(m, vals, x, y, 0, partials, sc, z, pNew);
+// The PCG product q = A p with the chunk partials p.q (slot 0), R chunks per workgroup and LS
+// lanes per row; with FUSE the direction update p = z + beta p_old is done on the fly (x =
+// p_old, p -> pNew).
+template <typename T, int R, bool FUSE, int LS>
+__global__ __launch_bounds__(kChunkRows * LS) void k_spmv_dot(DevMesh m, const T *__restrict__ vals,
+                                                              const T *__restrict__ x, T *__restrict__ y,
+                                                              double *__restrict__ partials,
+                                                              const InstScalars *__restrict__ sc,
+                                                              const T *__restrict__ z, T *__restrict__ pNew) {
+    spmv_chunks<T, R, FUSE, LS>(m, vals, x, y, 0, partials, sc, z, pNew);
 }
 
 static inline int spmv_grid(const DevMesh &m, int R) { return 8 * ((m.perRun + R - 1) / R); }
 
 template <typename T>
-void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, const T *vals, const T *x, T *y, double *partials,
+void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *vals, const T *x, T *y, double *partials,
                      const InstScalars *sc, hipEvent_t start, hipEvent_t stop, const T *z, T *pNew) {
     // With events, the launch records them from the dispatch's own begin/end timestamps,
     // so their elapsed time is the kernel's execution alone (measurement mode).
-    auto go = [&](auto kernel) {
+    auto go = [&](auto kernel, int threads) {
         if (start)
-            hipExtLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(kSpmvThreads), 0, s, start, stop, 0, m, vals, x, y,
+            hipExtLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(threads), 0, s, start, stop, 0, m, vals, x, y,
                                   partials, sc, z, pNew);
         else
-            hipLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(kSpmvThreads), 0, s, m, vals, x, y, partials, sc, z,
-                               pNew);
+            hipLaunchKernelGGL(kernel, dim3(spmv_grid(m, R)), dim3(threads), 0, s, m, vals, x, y, partials, sc, z, pNew);
+    };
+    auto shape = [&](auto fuse) {
+        constexpr bool F = decltype(fuse)::value;
+        if (LS == 8) {
+            if (R == 4) go(k_spmv_dot<T, 4, F, 8>, kChunkRows * 8);
+            else if (R == 2) go(k_spmv_dot<T, 2, F, 8>, kChunkRows * 8);
+            else go(k_spmv_dot<T, 1, F, 8>, kChunkRows * 8);
+        } else {
+            if (R == 4) go(k_spmv_dot<T, 4, F, 16>, kChunkRows * 16);
+            else if (R == 2) go(k_spmv_dot<T, 2, F, 16>, kChunkRows * 16);
+            else go(k_spmv_dot<T, 1, F, 16>, kChunkRows * 16);
+        }
     };
     if constexpr (sizeof(T) == 4) {          // the fused direction update is an FP32 product
-        if (z) {
-            if (R == 4) go(k_spmv_dot<T, 4, true>);
-            else if (R == 2) go(k_spmv_dot<T, 2, true>);
-            else go(k_spmv_dot<T, 1, true>);
-            OD_HIP(hipGetLastError());
-            return;
-        }
-    }
-    if (z) throw HipError("the fused direction update is an FP32 product");
-    {
-        if (R == 4) go(k_spmv_dot<T, 4, false>);
-        else if (R == 2) go(k_spmv_dot<T, 2, false>);
-        else go(k_spmv_dot<T, 1, false>);
+        if (!z) throw HipError("the FP32 product runs with the fused direction update");
+        shape(std::true_type{});
+    } else {
+        if (z) throw HipError("the fused direction update is an FP32 product");
+        shape(std::false_type{});
     }
     OD_HIP(hipGetLastError());
 }
-template void launch_spmv_dot<double>(const DevMesh &, hipStream_t, int, const double *, const double *, double *,
+template void launch_spmv_dot<double>(const DevMesh &, hipStream_t, int, int, const double *, const double *, double *,
                                       double *, const InstScalars *, hipEvent_t, hipEvent_t, const double *, double *);
-template void launch_spmv_dot<float>(const DevMesh &, hipStream_t, int, const float *, const float *, float *, double *,
-                                     const InstScalars *, hipEvent_t, hipEvent_t, const float *, float *);
+template void launch_spmv_dot<float>(const DevMesh &, hipStream_t, int, int, const float *, const float *, float *,
+                                     double *, const InstScalars *, hipEvent_t, hipEvent_t, const float *, float *);
 
 template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
@@ -793,8 +841,8 @@ template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, co
 template <typename T>
 void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *pOld, const T *z, T *p, T *q, PcgWork &w,
                hipEvent_t start, hipEvent_t stop) {
-    launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, vals, z ? pOld : p, q, w.partials, w.sc, start,
-                       stop, z, z ? p : nullptr);
+    launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, sizeof(T) == 8 ? m.spmvL64 : m.spmvL32, vals,
+                       z ? pOld : p, q, w.partials, w.sc, start, stop, z, z ? p : nullptr);
 }
 
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w) {
@@ -1069,41 +1117,12 @@ static inline int xcd_grid(int nwg) { return 8 * ((nwg + 7) / 8); }
 // lanes per row, the others 8.
 constexpr int kIcThreads = 256;
 
-template <int L, typename T>
-__device__ __forceinline__ void sum_lanes(T &a0, T &a1, T &a2) {
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) {
-        a0 += __shfl_xor(a0, o, L);
-        a1 += __shfl_xor(a1, o, L);
-        a2 += __shfl_xor(a2, o, L);
-    }
-}
-
 // y <- U_II^-1 y for the packed upper diagonal block D = [d0 d1 d2; . d3 d4; . . d5]
 template <typename T>
 __device__ __forceinline__ void diag_bwd(const T *D, T &y0, T &y1, T &y2) {
     y2 = y2 / D[5];
     y1 = (y1 - D[4] * y2) / D[3];
     y0 = (y0 - D[1] * y1 - D[2] * y2) / D[0];
-}
-
-// Lanes per row L (4, 8 or 16) is a launch choice only.  Each lane keeps the sums of 16 / L
-// virtual lanes (v = lane + g L, which take blocks v, v + 16, ... of the row) and combines
-// them in the order of a 16-lane shuffle tree (pairs at distance 8, then 4, 2, 1), so every
-// L gives the bits of 16 lanes.
-template <int L, typename T>
-__device__ __forceinline__ void combine16(const T (&acc)[16 / L][3], T &a0, T &a1, T &a2) {
-    if constexpr (L == 16) {
-        a0 = acc[0][0]; a1 = acc[0][1]; a2 = acc[0][2];
-    } else if constexpr (L == 8) {
-        a0 = acc[0][0] + acc[1][0]; a1 = acc[0][1] + acc[1][1]; a2 = acc[0][2] + acc[1][2];
-    } else {
-        static_assert(L == 4, "4, 8 or 16 lanes per row");
-        a0 = (acc[0][0] + acc[2][0]) + (acc[1][0] + acc[3][0]);
-        a1 = (acc[0][1] + acc[2][1]) + (acc[1][1] + acc[3][1]);
-        a2 = (acc[0][2] + acc[2][2]) + (acc[1][2] + acc[3][2]);
-    }
-    sum_lanes<L>(a0, a1, a2);
 }
 
 // forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the

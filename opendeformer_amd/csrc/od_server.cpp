@@ -76,7 +76,7 @@ struct od_server {
     int nSlots = 1;
     int precision = OD_MIXED, precond = OD_PRE_IC0;
     double tol = 1e-6;
-    int spmvR64 = 0, spmvR32 = 0;   // SpMV variant tuned on the first object, reused for the rest
+    int spmvR64 = 0, spmvL64 = 0, spmvR32 = 0, spmvL32 = 0;   // SpMV variant tuned on the first object, reused for the rest
     std::vector<od_ctx *> slotCtx;
     std::vector<Object> objs;
     // run state
@@ -130,14 +130,14 @@ void load(od_server *S, Object &o, od_ctx *ctx) {
         std::unique_lock<std::mutex> lk(S->mu);
         if (S->spmvR64 == 0) {
             lk.unlock();
-            int chosen[2];
+            int chosen[4];
             check(od_mesh_autotune(o.mesh, nullptr, chosen), "od_mesh_autotune");
             lk.lock();
-            S->spmvR64 = chosen[0];
-            S->spmvR32 = chosen[1];
+            S->spmvR64 = chosen[0]; S->spmvL64 = chosen[1];
+            S->spmvR32 = chosen[2]; S->spmvL32 = chosen[3];
         }
     }
-    check(od_mesh_spmv_variant(o.mesh, S->spmvR64, S->spmvR32), "od_mesh_spmv_variant");
+    check(od_mesh_spmv_variant(o.mesh, S->spmvR64, S->spmvL64, S->spmvR32, S->spmvL32), "od_mesh_spmv_variant");
     check(od_newmark_create(o.mesh, 0.25, 0.5, o.dt, 0.0, 0.0, S->precond, 1e-3, S->tol, 10000, S->precision, 1,
                             &o.it),
           "od_newmark_create");

@@ -252,19 +252,21 @@ def test_batch_equals_alone_bitwise(ctx):
 
 
 def test_spmv_variants_give_identical_pcg_bits(ctx):
-    """The autotuned SpMV launch shapes (1, 2 or 4 chunks per workgroup) compute every row
-    and every chunk partial in the same order: PCG results are bitwise equal."""
+    """The autotuned SpMV launch shapes (1, 2 or 4 chunks per workgroup, 16 or 8 lanes per
+    block row) compute every row and every chunk partial in the same order: PCG results are
+    bitwise equal."""
     import opendeformer_amd as od
     w = W.box_small("color8")
     mesh, s = make(ctx, w)
-    table, r64, r32 = od.mesh_autotune(mesh)
-    assert r64 in (1, 2, 4) and r32 in (1, 2, 4) and all(v > 0 for v in table.values())
+    table, v64, v32 = od.mesh_autotune(mesh)
+    assert v64[0] in (1, 2, 4) and v32[0] in (1, 2, 4) and v64[1] in (8, 16) and v32[1] in (8, 16)
+    assert len(table) == 12 and all(v > 0 for v in table.values())
     A = newmark_system(s, w)
     Ad = od.Matrix(mesh); Ad.upload(A)
     b = np.random.default_rng(17).standard_normal(mesh.ndof)
     outs = []
-    for r in (1, 2, 4):
-        od.mesh_spmv_variant(mesh, r, r)
+    for r, ls in ((1, 16), (2, 16), (4, 16), (1, 8), (2, 8), (4, 8)):
+        od.mesh_spmv_variant(mesh, (r, ls), (r, ls))
         for prec in (capi.OD_FP64, capi.OD_MIXED):
             if prec == capi.OD_MIXED:
                 Ad.sync_f32()
@@ -272,7 +274,7 @@ def test_spmv_variants_give_identical_pcg_bits(ctx):
                                  precision=prec)
             dx = od.DeviceArray(ctx, mesh.ndof); dx.zero()
             it, rr = solver.solve(Ad, od.DeviceArray.from_host(ctx, b), dx)
-            outs.append((r, prec, int(it[0]), dx.download()))
+            outs.append(((r, ls), prec, int(it[0]), dx.download()))
     for prec in (capi.OD_FP64, capi.OD_MIXED):
         sel = [o for o in outs if o[1] == prec]
         for o in sel[1:]:
