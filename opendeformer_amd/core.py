@@ -166,10 +166,10 @@ class Mesh:
 
 def mesh_autotune(mesh: "Mesh"):
     """Reruns the autotuner: ({(precision, lanes, R): ms}, (R, lanes) for FP64, (R, lanes) for FP32)."""
-    ms = np.zeros(12); ch = np.zeros(4, np.int32)
+    ms = np.zeros(18); ch = np.zeros(4, np.int32)
     check(lib().od_mesh_autotune(mesh.h, dptr(ms), capi.iptr(ch)))
-    table = {(p, ls, r): float(ms[6 * i + 3 * k + j]) for i, p in enumerate(("fp64", "fp32"))
-             for k, ls in enumerate((16, 8)) for j, r in enumerate((1, 2, 4))}
+    table = {(p, ls, r): float(ms[9 * i + 3 * k + j]) for i, p in enumerate(("fp64", "fp32"))
+             for k, ls in enumerate((16, 8, 4)) for j, r in enumerate((1, 2, 4))}
     return table, (int(ch[0]), int(ch[1])), (int(ch[2]), int(ch[3]))
 
 

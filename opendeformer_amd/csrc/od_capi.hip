@@ -89,7 +89,7 @@ struct od_mesh {
     size_t bytes = 0;
     bool hasMaterial = false;
     bool spmvTuned = false;                      // SpMV variant chosen (autotuner or caller)
-    double tuneMs[2][2][3] = {};   // [FP64, FP32][16, 8 lanes per row][R = 1, 2, 4]: cold launch time
+    double tuneMs[2][3][3] = {};   // [FP64, FP32][16, 8, 4 lanes per row][R = 1, 2, 4]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
     std::vector<uint8_t> levelLanesFwd, levelLanesBwd;   // DevMesh points at these (host memory)
     template <typename T>
@@ -286,7 +286,7 @@ static void autotune_spmv(od_mesh *mesh) {
         OD_HIP(hipEventCreate(&e0));
         OD_HIP(hipEventCreate(&e1));
         for (int prec = 0; prec < 2; prec++)
-            for (int lv = 0; lv < 2; lv++)
+            for (int lv = 0; lv < 3; lv++)
                 for (int v = 0; v < 3; v++) {
                     const int R = kSpmvVariants[v], LS = kSpmvLaneVariants[lv];
                     float ms[3];
@@ -316,7 +316,7 @@ static void autotune_spmv(od_mesh *mesh) {
     for (void *p : bufs) (void)hipFree(p);
     int best[2][2] = {{0, 0}, {0, 0}};   // [precision] -> (lane variant, R variant)
     for (int prec = 0; prec < 2; prec++)
-        for (int lv = 0; lv < 2; lv++)
+        for (int lv = 0; lv < 3; lv++)
             for (int v = 0; v < 3; v++)
                 if (mesh->tuneMs[prec][lv][v] < mesh->tuneMs[prec][best[prec][0]][best[prec][1]]) {
                     best[prec][0] = lv;
@@ -915,9 +915,9 @@ int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes
     API_BEGIN
     require(mesh != nullptr, "NULL argument");
     auto okR = [](int r) { return r == 1 || r == 2 || r == 4; };
-    auto okL = [](int l) { return l == 8 || l == 16; };
+    auto okL = [](int l) { return l == 4 || l == 8 || l == 16; };
     require(okR(r64) && okR(r32), "chunks per SpMV workgroup must be 1, 2 or 4");
-    require(okL(lanes64) && okL(lanes32), "lanes per SpMV block row must be 8 or 16");
+    require(okL(lanes64) && okL(lanes32), "lanes per SpMV block row must be 4, 8 or 16");
     mesh->dm.spmvR64 = r64;
     mesh->dm.spmvL64 = lanes64;
     mesh->dm.spmvR32 = r32;
@@ -933,8 +933,8 @@ int od_mesh_autotune(od_mesh *mesh, double *ms, int *chosen) {
     autotune_spmv(mesh);
     if (ms)
         for (int p = 0; p < 2; p++)
-            for (int lv = 0; lv < 2; lv++)
-                for (int v = 0; v < 3; v++) ms[6 * p + 3 * lv + v] = mesh->tuneMs[p][lv][v];
+            for (int lv = 0; lv < 3; lv++)
+                for (int v = 0; v < 3; v++) ms[9 * p + 3 * lv + v] = mesh->tuneMs[p][lv][v];
     if (chosen) {
         chosen[0] = mesh->dm.spmvR64; chosen[1] = mesh->dm.spmvL64;
         chosen[2] = mesh->dm.spmvR32; chosen[3] = mesh->dm.spmvL32;

@@ -55,7 +55,7 @@ struct DevMesh {
     int *chunkSorted;     // chunks in spatial order, cut into 8 runs of perRun (one run per XCD)
     int perRun;
     int spmvR64, spmvR32; // chunks per SpMV workgroup (FP64 / FP32 values), set by the autotuner
-    int spmvL64, spmvL32; // lanes per block row of the SpMV (16 or 8), set by the autotuner
+    int spmvL64, spmvL32; // lanes per block row of the SpMV (16, 8 or 4), set by the autotuner
     int *slotOff, *slotCol;                      // kSpmvThreads per chunk (padding: slotCol = -1)
     uint8_t *chunkLong;   // 1 when a row of the chunk has more than kSpmvLanes blocks (row-loop path)
     int *instChunkStart, *instRowStart;          // nInst+1
@@ -103,7 +103,7 @@ void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *va
                      const InstScalars *sc, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
                      const T *z = nullptr, T *pNew = nullptr);
 constexpr int kSpmvVariants[3] = {1, 2, 4};        // chunks per workgroup
-constexpr int kSpmvLaneVariants[2] = {16, 8};      // lanes per block row
+constexpr int kSpmvLaneVariants[3] = {16, 8, 4};   // lanes per block row
 
 // PCG building blocks (see od_kernels.hip for the exact phase semantics)
 struct PcgWork {

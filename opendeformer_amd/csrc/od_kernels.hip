@@ -673,7 +673,7 @@ __device__ __forceinline__ int run_chunk(const DevMesh &m, int q) {
     return (q < m.perRun && idx < m.nChunks) ? m.chunkSorted[idx] : -1;
 }
 
-// y = A x (or y += A x) for R chunks per workgroup, LS lanes per block row (16 or 8; 64 LS
+// y = A x (or y += A x) for R chunks per workgroup, LS lanes per block row (16, 8 or 4; 64 LS
 // threads).  A row has 16 slots; lane l takes slots l, l + LS, ... and the row sum combines
 // them in the order of a 16-lane tree (combine16), so every launch shape gives the same bits.
 // Every load is unconditional (slots of padding, inactive or long-row chunks point at valid
@@ -803,7 +803,11 @@ void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *va
     };
     auto shape = [&](auto fuse) {
         constexpr bool F = decltype(fuse)::value;
-        if (LS == 8) {
+        if (LS == 4) {
+            if (R == 4) go(k_spmv_dot<T, 4, F, 4>, kChunkRows * 4);
+            else if (R == 2) go(k_spmv_dot<T, 2, F, 4>, kChunkRows * 4);
+            else go(k_spmv_dot<T, 1, F, 4>, kChunkRows * 4);
+        } else if (LS == 8) {
             if (R == 4) go(k_spmv_dot<T, 4, F, 8>, kChunkRows * 8);
             else if (R == 2) go(k_spmv_dot<T, 2, F, 8>, kChunkRows * 8);
             else go(k_spmv_dot<T, 1, F, 8>, kChunkRows * 8);
