@@ -101,14 +101,19 @@ int od_pattern_export(od_mesh *mesh, int *blockPcol_host, int *blockColumnOris_h
                       int *elemMap78_host);
 /* Launch shapes (no reference counterpart).  The PCG SpMV runs R = 1, 2 or 4
  * chunks of 64 block rows per workgroup (more rows' loads in flight per thread)
- * with 16, 8 or 4 lanes per block row; every shape gives the same bits.  The
- * first solver created on a mesh runs the autotuner, which times each shape for
- * FP64 and FP32 values with the matrix out of cache and keeps the fastest, and
- * picks the lanes per row (4, 8 or 16) of each IC(0) sweep level the same way.
- * od_mesh_autotune reruns it (ms_host: 18 median times, [FP64, FP32] x [16, 8,
- * 4 lanes] x [R = 1, 2, 4]; chosen_host: R and lanes for FP64, then for FP32;
- * either may be NULL).  od_mesh_spmv_variant sets the SpMV shapes directly (no
- * tuning of the SpMV or the sweeps). */
+ * with 16, 8 or 4 lanes per block row.  The FP32 product also has a slot-value
+ * form (lanes 0): the solver keeps an FP32 copy of the matrix in the product's
+ * own slot order (36 B per slot, 16 slots per block row), refreshed at every
+ * mixed solve, so a wave's value loads read consecutive memory; R is then the
+ * slots per thread (1, 2 or 4).  Every shape gives the same bits.  The first
+ * solver created on a mesh runs the autotuner, which times each shape for FP64
+ * and FP32 values with the matrix out of cache and keeps the fastest, and picks
+ * the lanes per row (4, 8 or 16) of each IC(0) sweep level the same way.
+ * od_mesh_autotune reruns it (ms_host: 21 median times, [FP64, FP32] x [16, 8,
+ * 4 lanes] x [R = 1, 2, 4], then the FP32 slot-value form with 1, 2, 4 slots
+ * per thread; chosen_host: R and lanes for FP64, then for FP32; either may be
+ * NULL).  od_mesh_spmv_variant sets the SpMV shapes directly (no tuning of the
+ * SpMV or the sweeps). */
 int od_mesh_autotune(od_mesh *mesh, double *ms_host, int *chosen_host);
 int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes32);
 /* Rebinds a mesh (and the solvers and integrators built on it) to another

@@ -90,6 +90,7 @@ struct od_mesh {
     bool hasMaterial = false;
     bool spmvTuned = false;                      // SpMV variant chosen (autotuner or caller)
     double tuneMs[2][3][3] = {};   // [FP64, FP32][16, 8, 4 lanes per row][R = 1, 2, 4]: cold launch time
+    double tuneSvMs[3] = {};       // FP32 slot-value form, [1, 2, 4 slots per thread]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
     std::vector<uint8_t> levelLanesFwd, levelLanesBwd;   // DevMesh points at these (host memory)
     template <typename T>
@@ -174,6 +175,7 @@ struct od_solver {
     double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *invDiag = nullptr, *U = nullptr, *Ul = nullptr;
     float *r32 = nullptr, *ri32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
     float *invDiag32 = nullptr, *U32 = nullptr, *Ul32 = nullptr;
+    float *sv32 = nullptr;      // FP32 slot values of the matrix (slot-value product form)
     double *Ax = nullptr;
     double *partials = nullptr;
     int *doneIn = nullptr, *doneOut = nullptr;   // per-instance done flags (device)
@@ -283,8 +285,22 @@ static void autotune_spmv(od_mesh *mesh) {
         double *partials = (double *)get(8 * (size_t)std::max(1, m.nChunks));
         InstScalars *sc = (InstScalars *)get(sizeof(InstScalars) * t.nInst);   // zero: no instance done
         float *flush = (float *)get(4 * (size_t)flushN);
+        float *sv = (float *)get(sizeof(float) * kSlotValFloats * (size_t)std::max(1, m.nChunks));
         OD_HIP(hipEventCreate(&e0));
         OD_HIP(hipEventCreate(&e1));
+        for (int v = 0; v < 3; v++) {
+            float ms[3];
+            for (int rep = 0; rep < 3; rep++) {
+                hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, st, (const float4 *)flush, flushN / 4,
+                                   flush);
+                launch_spmv_dot<float>(m, st, kSpmvVariants[v], 0, (const float *)vals, (const float *)x, (float *)y,
+                                       partials, sc, e0, e1, (const float *)z, (float *)pNew, sv);
+                OD_HIP(hipEventSynchronize(e1));
+                OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
+            }
+            std::sort(ms, ms + 3);
+            mesh->tuneSvMs[v] = ms[1];
+        }
         for (int prec = 0; prec < 2; prec++)
             for (int lv = 0; lv < 3; lv++)
                 for (int v = 0; v < 3; v++) {
@@ -326,6 +342,14 @@ static void autotune_spmv(od_mesh *mesh) {
     m.spmvL64 = kSpmvLaneVariants[best[0][0]];
     m.spmvR32 = kSpmvVariants[best[1][1]];
     m.spmvL32 = kSpmvLaneVariants[best[1][0]];
+    // the FP32 slot-value form (lanes 0) when one of its shapes is faster still
+    double best32 = mesh->tuneMs[1][best[1][0]][best[1][1]];
+    for (int v = 0; v < 3; v++)
+        if (mesh->tuneSvMs[v] < best32) {
+            best32 = mesh->tuneSvMs[v];
+            m.spmvR32 = kSpmvVariants[v];
+            m.spmvL32 = 0;
+        }
     // lanes per row of the IC(0) sweeps, per level (meshes with few levels)
     if (t.nLevels <= 64) {
         tune_ic0_lanes(m, st, t.levelPtr, t.nValues, (int64_t)t.tRow.size(), mesh->levelLanesFwd.data(),
@@ -363,8 +387,8 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
             s->x32 = (float *)s->alloc(4 * n);
             if (precond == OD_PRE_JACOBI) s->invDiag32 = (float *)s->alloc(4 * n);
             if (precond == OD_PRE_IC0) {
-                s->U32 = (float *)s->alloc(4 * (nv + kValuePad));
-                s->Ul32 = (float *)s->alloc(4 * nl);
+                s->U32 = (float *)s->alloc(4 * (size_t)ic0_u32_floats(mesh->dm));
+                s->Ul32 = (float *)s->alloc(4 * (size_t)std::max<int64_t>(1, ic0_ul32_floats(mesh->dm)));
             }
         }
         s->partials = (double *)s->alloc(sizeof(double) * 3 * (size_t)t.chunkRow0.size());
@@ -431,6 +455,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
     PcgWork w{s->partials, sc, doneFlags, outer};
+    if (sizeof(T) == 4) w.slotVals = s->sv32;
     const size_t n = (size_t)mesh->topo.ndof;
     if (xZero) OD_HIP(hipMemsetAsync(q, 0, sizeof(T) * n, st));
     else launch_spmv<T>(m, st, vals, x, q, 0, nullptr);
@@ -464,7 +489,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     od_solver::IterGraph *g = nullptr;
     if (s->useGraph && !mesh->ctx->prof) {
         g = &s->graph[sizeof(T) == 8 ? 0 : 1];
-        const std::vector<const void *> key = {vals, Uf, Ul, invDiag, x, r, z, p, q, sc, doneFlags, preDone};
+        const std::vector<const void *> key = {vals, Uf, Ul, invDiag, x, r, z, p, q, sc, doneFlags, preDone, w.slotVals};
         if (!g->exec || g->key != key) {
             if (g->exec) { OD_HIP(hipGraphExecDestroy(g->exec)); g->exec = nullptr; }
             hipGraph_t graph = nullptr;
@@ -541,6 +566,10 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     // ---- mixed: FP32 inner PCG, FP64 residual and correction (iterative refinement)
     require(A->vals32 != nullptr, "mixed precision needs the FP32 copy of A (od_matrix_sync_f32)", OD_ERR_STATE);
     build_precond(s, A, pre);
+    if (m.spmvL32 == 0) {
+        if (!s->sv32) s->sv32 = (float *)s->alloc(sizeof(float) * kSlotValFloats * (size_t)std::max(1, m.nChunks));
+        launch_fill_slot_values(m, st, A->vals32, s->sv32);
+    }
     std::fill(s->iters.begin(), s->iters.end(), 0);
     PcgWork wo{s->partials, s->scOut, s->doneOut};
     launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr);
@@ -711,7 +740,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         DevMesh &m = mesh->dm;
         m.nNodes = t.nNodes; m.nTets = t.nTets; m.nRows = t.nRows; m.nInst = t.nInst;
         m.nChunks = (int)t.chunkRow0.size(); m.nLevels = t.nLevels;
-        m.nBlocks = t.nBlocks; m.nValues = t.nValues;
+        m.nBlocks = t.nBlocks; m.nValues = t.nValues; m.nT = (int64_t)t.tRow.size();
         m.X = mesh->upload(t.X);
         m.tets = mesh->upload(t.tets);
         m.nodeRow = mesh->upload(t.nodeRow);
@@ -738,6 +767,17 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.rowCStart = mesh->upload(t.rowCStart);
         m.levelRows = mesh->upload(t.levelRows);
         m.rowLevel = mesh->upload(t.level);
+        {
+            // per-row records of the sweeps, one 16-byte load each
+            std::vector<int> fw(4 * (size_t)t.nRows), bw(4 * (size_t)t.nRows);
+            for (int r = 0; r < t.nRows; r++) {
+                const size_t q = 4 * (size_t)r;
+                fw[q] = t.tPtr[r]; fw[q + 1] = t.tPtr[r + 1]; fw[q + 2] = t.valOff[r]; fw[q + 3] = t.rowInst[r];
+                bw[q] = t.rowPtr[r]; bw[q + 1] = t.rowPtr[r + 1]; bw[q + 2] = t.valOff[r]; bw[q + 3] = t.rowInst[r];
+            }
+            m.rowFwd = (const int4 *)mesh->upload(fw);
+            m.rowBwd = (const int4 *)mesh->upload(bw);
+        }
         // lanes per row before tuning: 16 where a level's rows average more than 8 blocks on
         // the swept side, else 8
         mesh->levelLanesFwd.assign(std::max(1, t.nLevels), 8);
@@ -759,19 +799,22 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.chunkRow0 = mesh->upload(t.chunkRow0);
         m.chunkRow1 = mesh->upload(t.chunkRow1);
         m.chunkInst = mesh->upload(t.chunkInst);
-        m.chunkSorted = mesh->upload(t.chunkSorted);
         m.perRun = t.perRun;
         m.spmvR64 = m.spmvR32 = 1;
         m.spmvL64 = m.spmvL32 = kSpmvLanes;
         {
-            // SpMV slots: per chunk, kSpmvLanes per row, rows in order (row-loop path for longer rows)
+            // SpMV slots in run order (position p = t.chunkSorted index): kSpmvLanes per row,
+            // rows in order (row-loop path for longer rows), and one record per position
             const int nc = m.nChunks;
             std::vector<int> sOff((size_t)nc * kSpmvThreads, 0), sCol((size_t)nc * kSpmvThreads, -1);
-            std::vector<uint8_t> cLong(nc, 0);
+            std::vector<int> info(4 * (size_t)nc);
             require(t.nRows <= kSlotColMask, "too many block rows for the SpMV slot encoding", OD_ERR_UNSUPPORTED);
-            for (int c = 0; c < nc; c++)
+            require(t.nInst <= kRunInstMask, "too many instances for the SpMV run records", OD_ERR_UNSUPPORTED);
+            for (int p = 0; p < nc; p++) {
+                const int c = t.chunkSorted[p];
+                bool lng = false;
                 for (int r = t.chunkRow0[c]; r < t.chunkRow1[c]; r++) {
-                    const size_t base = ((size_t)c * kChunkRows + (r - t.chunkRow0[c])) * kSpmvLanes;
+                    const size_t base = ((size_t)p * kChunkRows + (r - t.chunkRow0[c])) * kSpmvLanes;
                     int n = 0;
                     auto put = [&](int off, int col) {
                         if (n < kSpmvLanes) { sOff[base + n] = off; sCol[base + n] = col; }
@@ -780,11 +823,16 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
                     put(t.valOff[r], r | kSlotDiag);
                     for (int b = t.rowPtr[r] + 1; b < t.rowPtr[r + 1]; b++) put(t.blkOff[b], t.colIdx[b]);
                     for (int k = t.tPtr[r]; k < t.tPtr[r + 1]; k++) put(tOff[k], t.tRow[k] | kSlotTrans);
-                    if (n > kSpmvLanes) cLong[c] = 1;
+                    if (n > kSpmvLanes) lng = true;
                 }
+                info[4 * (size_t)p] = c;
+                info[4 * (size_t)p + 1] = t.chunkRow0[c];
+                info[4 * (size_t)p + 2] = t.chunkRow1[c];
+                info[4 * (size_t)p + 3] = t.chunkInst[c] | (lng ? kRunLong : 0);
+            }
             m.slotOff = mesh->upload(sOff);
             m.slotCol = mesh->upload(sCol);
-            m.chunkLong = mesh->upload(cLong);
+            m.runInfo = (const int4 *)mesh->upload(info);
         }
         m.instChunkStart = mesh->upload(t.instChunkStart);
         m.instRowStart = mesh->upload(t.instRowStart);
@@ -917,7 +965,8 @@ int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes
     auto okR = [](int r) { return r == 1 || r == 2 || r == 4; };
     auto okL = [](int l) { return l == 4 || l == 8 || l == 16; };
     require(okR(r64) && okR(r32), "chunks per SpMV workgroup must be 1, 2 or 4");
-    require(okL(lanes64) && okL(lanes32), "lanes per SpMV block row must be 4, 8 or 16");
+    require(okL(lanes64) && (okL(lanes32) || lanes32 == 0),
+            "lanes per SpMV block row must be 4, 8 or 16 (or 0 for the FP32 slot-value form)");
     mesh->dm.spmvR64 = r64;
     mesh->dm.spmvL64 = lanes64;
     mesh->dm.spmvR32 = r32;
@@ -935,6 +984,8 @@ int od_mesh_autotune(od_mesh *mesh, double *ms, int *chosen) {
         for (int p = 0; p < 2; p++)
             for (int lv = 0; lv < 3; lv++)
                 for (int v = 0; v < 3; v++) ms[9 * p + 3 * lv + v] = mesh->tuneMs[p][lv][v];
+    if (ms)
+        for (int v = 0; v < 3; v++) ms[18 + v] = mesh->tuneSvMs[v];
     if (chosen) {
         chosen[0] = mesh->dm.spmvR64; chosen[1] = mesh->dm.spmvL64;
         chosen[2] = mesh->dm.spmvR32; chosen[3] = mesh->dm.spmvL32;

@@ -30,11 +30,12 @@ constexpr int kValuePad = 16;       // trailing entries allocated after every va
 constexpr int kSpmvLanes = 16;
 constexpr int kSpmvThreads = kSpmvLanes * kChunkRows;
 constexpr int kSlotDiag = 1 << 29, kSlotTrans = 1 << 30, kSlotColMask = (1 << 29) - 1;
+constexpr int kRunLong = 1 << 30, kRunInstMask = kRunLong - 1;   // run record: a row of the chunk has more than kSpmvLanes blocks
 
 // Device copy of the topology and material.
 struct DevMesh {
     int nNodes, nTets, nRows, nInst, nChunks, nLevels;
-    int64_t nBlocks, nValues;
+    int64_t nBlocks, nValues, nT;   // nT: entries of the transposed index (lower blocks)
     double *X;            // 3*nNodes
     int *tets;            // 4*nTets
     int *nodeRow;         // nNodes
@@ -48,16 +49,20 @@ struct DevMesh {
     int *levelRows;                              // nRows
     int levelsContiguous;                        // levels are consecutive row ranges (levelRows is the identity)
     int *rowLevel;                               // IC(0) dependency level of each row
+    // per row, for the sweeps: {tPtr[r], tPtr[r+1], valOff[r], instance} and
+    // {rowPtr[r], rowPtr[r+1], valOff[r], instance}
+    const int4 *rowFwd, *rowBwd;
     // per level, in host memory (read at launch, never on the device): lanes per row (4, 8 or
     // 16) of the forward and backward sweeps
     const uint8_t *levelLanesFwd, *levelLanesBwd;
     int *chunkRow0, *chunkRow1, *chunkInst;      // nChunks
-    int *chunkSorted;     // chunks in spatial order, cut into 8 runs of perRun (one run per XCD)
+    // SpMV run order: the chunks in spatial order, cut into 8 runs of perRun positions (one
+    // run per XCD); per position {chunk, first row, end row, instance | kRunLong}
+    const int4 *runInfo;
     int perRun;
     int spmvR64, spmvR32; // chunks per SpMV workgroup (FP64 / FP32 values), set by the autotuner
     int spmvL64, spmvL32; // lanes per block row of the SpMV (16, 8 or 4), set by the autotuner
-    int *slotOff, *slotCol;                      // kSpmvThreads per chunk (padding: slotCol = -1)
-    uint8_t *chunkLong;   // 1 when a row of the chunk has more than kSpmvLanes blocks (row-loop path)
+    int *slotOff, *slotCol;                      // kSpmvThreads per run position (padding: slotCol = -1)
     int *instChunkStart, *instRowStart;          // nInst+1
     int *tetInst, *rowInst;
     // material (per tet)
@@ -98,12 +103,27 @@ void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *
 // PCG SpMV with R chunks per workgroup (1, 2 or 4), timed by optional dispatch events
 // With z set (the PCG product), x is the previous direction and the product runs on the new
 // direction p = z + beta x of each instance, written to pNew (see spmv_chunks).
+// LS = 0 selects the FP32 slot-value form (slotVals, filled by launch_fill_slot_values from
+// vals; R is then the slots per thread, 1, 2 or 4); vals still serves rows longer than
+// kSpmvLanes blocks.
 template <typename T>
 void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *vals, const T *x, T *y, double *partials,
                      const InstScalars *sc, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
-                     const T *z = nullptr, T *pNew = nullptr);
-constexpr int kSpmvVariants[3] = {1, 2, 4};        // chunks per workgroup
+                     const T *z = nullptr, T *pNew = nullptr, const T *slotVals = nullptr);
+constexpr int kSpmvVariants[3] = {1, 2, 4};        // chunks per workgroup (slot-value form: slots per thread)
 constexpr int kSpmvLaneVariants[3] = {16, 8, 4};   // lanes per block row
+// FP32 slot values: per run position, the kSpmvThreads slot blocks as the product multiplies
+// them (diagonal blocks full, lower blocks transposed, padding zero), part-major: entries
+// 0-3 of every slot (float4 each), then entries 4-7, then entry 8
+constexpr int kSlotValFloats = 9 * kSpmvThreads;
+void launch_fill_slot_values(const DevMesh &m, hipStream_t s, const float *vals, float *slotVals);
+// FP32 IC(0) factor copies, part-major: block b of an array of nb blocks has entries 0-3 at
+// [4 b, 4 b + 4), entries 4-7 at [4 nb + 4 b, ...) and entry 8 at [8 nb + b], so a wave's
+// loads of consecutive blocks read consecutive memory.  U32 holds the diagonal blocks first
+// (8 floats per row: the 6 packed entries, 2 unused), then the upper blocks part-major by
+// block index (nBlocks); Ul32 holds the lower copy part-major by transposed index (nT).
+inline int64_t ic0_u32_floats(const DevMesh &m) { return 8 * (int64_t)m.nRows + 9 * m.nBlocks; }
+inline int64_t ic0_ul32_floats(const DevMesh &m) { return 9 * m.nT; }
 
 // PCG building blocks (see od_kernels.hip for the exact phase semantics)
 struct PcgWork {
@@ -113,6 +133,7 @@ struct PcgWork {
     // inner solves of the mixed form: the outer scalars, whose eps (tol^2 |b|^2 of the
     // FP64 system) bounds the inner target from below (see k_finalize, PH_INIT)
     const InstScalars *outer = nullptr;
+    const void *slotVals = nullptr;   // FP32 slot values of the matrix (slot-value product form)
 };
 template <typename T>
 void pcg_init_residual(const DevMesh &m, hipStream_t s, const T *b, const T *q, T *r, PcgWork &w);

@@ -581,6 +581,21 @@ __device__ __forceinline__ void dir_update(const T *__restrict__ zc, T beta, Blo
     b.x[2] = fma(beta, b.x[2], z2);
 }
 
+// One block-row term and its accumulation, rounded the same way in every kernel: the row
+// dot product e0 x0 + e1 x1 + e2 x2 as explicit fused multiply-adds (x0 first), then one
+// plain add into the running sum.  Contraction is off here so the compiler cannot fold the
+// running sum into the products (a fold it makes or not depending on the loop's shape).
+template <typename T>
+__device__ __forceinline__ T row3(T e0, T e1, T e2, T x0, T x1, T x2) {
+#pragma clang fp contract(off)
+    return fma(e2, x2, fma(e1, x1, e0 * x0));
+}
+template <typename T>
+__device__ __forceinline__ void acc_add(T &a, T t) {
+#pragma clang fp contract(off)
+    a = a + t;
+}
+
 // a += B x for the packed symmetric diagonal [e0 e1 e2; e1 e3 e4; e2 e4 e5], a row-major
 // upper block, or the transpose of one.
 template <typename T>
@@ -595,9 +610,9 @@ __device__ __forceinline__ void block_mv(const BlockX<T> &b, bool diag, bool tra
         t = e2; e2 = e6; e6 = t;
         t = e5; e5 = e7; e7 = t;
     }
-    a0 += e0 * b.x[0] + e1 * b.x[1] + e2 * b.x[2];
-    a1 += e3 * b.x[0] + e4 * b.x[1] + e5 * b.x[2];
-    a2 += e6 * b.x[0] + e7 * b.x[1] + e8 * b.x[2];
+    acc_add(a0, row3(e0, e1, e2, b.x[0], b.x[1], b.x[2]));
+    acc_add(a1, row3(e3, e4, e5, b.x[0], b.x[1], b.x[2]));
+    acc_add(a2, row3(e6, e7, e8, b.x[0], b.x[1], b.x[2]));
 }
 
 template <int L, typename T>
@@ -666,11 +681,11 @@ __device__ __forceinline__ void row_spmv(const DevMesh &m, const T *__restrict__
     combine16<LS>(acc, o0, o1, o2);
 }
 
-// Chunk q (0-based) of the run of the XCD this workgroup runs on, or -1 past the run's end:
-// workgroup w runs on XCD (w mod 8) and is that XCD's (w div 8)-th.
-__device__ __forceinline__ int run_chunk(const DevMesh &m, int q) {
+// Run position of the q-th (0-based) chunk of the run of the XCD this workgroup runs on, or
+// -1 past the run's end: workgroup w runs on XCD (w mod 8) and is that XCD's (w div 8)-th.
+__device__ __forceinline__ int run_pos(const DevMesh &m, int q) {
     const int x = (int)(blockIdx.x & 7), idx = x * m.perRun + q;
-    return (q < m.perRun && idx < m.nChunks) ? m.chunkSorted[idx] : -1;
+    return (q < m.perRun && idx < m.nChunks) ? idx : -1;
 }
 
 // y = A x (or y += A x) for R chunks per workgroup, LS lanes per block row (16, 8 or 4; 64 LS
@@ -684,38 +699,56 @@ __device__ __forceinline__ int run_chunk(const DevMesh &m, int q) {
 // With FUSE (the PCG product), x is the previous direction p_old: every gathered column is
 // the new direction p = z + beta p_old of its instance (dir_update), the row's own direction
 // (from its diagonal slot, lane 0) is written to pNew, and the dot partial is p.q.
+//
+// Dependent loads: the run record and the slots need only the workgroup index, and the block
+// and x loads only the slots.  A mesh of one instance checks its converged flag first, from
+// a fixed address (the whole grid exits together past convergence); with several instances
+// a chunk of a converged instance loads nothing but its slots.
 template <typename T, int R, bool FUSE, int LS>
 __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restrict__ vals, const T *__restrict__ x,
                                             T *__restrict__ y, int accumulate, double *__restrict__ partials,
                                             const InstScalars *__restrict__ sc, const T *__restrict__ z,
                                             T *__restrict__ pNew) {
     constexpr int G = kSpmvLanes / LS;
+    const bool single = m.nInst == 1;
+    if (single && sc && sc[0].done) return;
     const int j = (int)(blockIdx.x >> 3), lane = (int)threadIdx.x % LS, lr = (int)threadIdx.x / LS;
-    int chunk[R], r[R], off[R][G], cf[R][G];
-    bool act[R], lng[R];
+    int chunk[R], r[R], r1[R], off[R][G], cf[R][G];
+    bool act[R], lng[R], use[R];
     T beta[R];
+    int4 info[R];
 #pragma unroll
     for (int i = 0; i < R; i++) {
-        chunk[i] = run_chunk(m, R * j + i);
-        const int c = chunk[i] >= 0 ? chunk[i] : 0;
-        act[i] = chunk[i] >= 0 && !(sc && sc[m.chunkInst[c]].done);
-        beta[i] = FUSE ? (T)sc[m.chunkInst[c]].beta : (T)0;
-        lng[i] = m.chunkLong[c] != 0;
-        r[i] = m.chunkRow0[c] + lr;
+        const int pos = run_pos(m, R * j + i), p = pos >= 0 ? pos : 0;
+        act[i] = pos >= 0;
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            const size_t slot = (size_t)c * kSpmvThreads + (size_t)lr * kSpmvLanes + lane + g * LS;
+            const size_t slot = (size_t)p * kSpmvThreads + (size_t)lr * kSpmvLanes + lane + g * LS;
             off[i][g] = m.slotOff[slot];
             cf[i][g] = m.slotCol[slot];
-            if (!act[i] || lng[i]) cf[i][g] = -1;
         }
+        info[i] = m.runInfo[p];
     }
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        chunk[i] = act[i] ? info[i].x : -1;
+        r[i] = info[i].y + lr;
+        r1[i] = info[i].z;
+        lng[i] = (info[i].w & kRunLong) != 0;
+        const int inst = info[i].w & kRunInstMask;
+        if (!single && sc && act[i]) act[i] = !sc[inst].done;
+        use[i] = act[i] && !lng[i];          // the slots hold the whole rows
+        beta[i] = FUSE ? (T)sc[single ? 0 : inst].beta : (T)0;
+    }
+    // slots not used (padding, converged instance, row-loop chunk) read x[0] and the first
+    // value and count as padding (cf = -1) from here on
     BlockX<T> b[R][G];
 #pragma unroll
     for (int i = 0; i < R; i++)
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            const bool ok = cf[i][g] >= 0;
+            const bool ok = use[i] && cf[i][g] >= 0;
+            if (!ok) cf[i][g] = -1;
             const int c3 = 3 * (ok ? (cf[i][g] & kSlotColMask) : 0);
             load_block<T>(vals + (ok ? off[i][g] : 0), x + c3, b[i][g]);
             if (FUSE) dir_update<T>(z + c3, beta[i], b[i][g]);
@@ -739,14 +772,14 @@ __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restric
         if (FUSE) { pNew[3 * rr] = d0; pNew[3 * rr + 1] = d1; pNew[3 * rr + 2] = d2; }
         if (accumulate) { a0 += y[3 * rr]; a1 += y[3 * rr + 1]; a2 += y[3 * rr + 2]; }
         y[3 * rr] = a0; y[3 * rr + 1] = a1; y[3 * rr + 2] = a2;
-        if (partials) rowDot[i][lr] = (double)d0 * (double)a0 + (double)d1 * (double)a1 + (double)d2 * (double)a2;
+        if (partials) rowDot[i][lr] = row3<double>(d0, d1, d2, a0, a1, a2);
     };
     // short-row chunks first, from the slots loaded above; the row-loop chunks after, so their
     // loop does not hold the slot blocks live (uniform branches over the workgroup)
 #pragma unroll
     for (int i = 0; i < R; i++) {
         if (partials && lane == 0) rowDot[i][lr] = 0.0;
-        if (!act[i] || lng[i] || r[i] >= m.chunkRow1[chunk[i]]) continue;
+        if (!act[i] || lng[i] || r[i] >= r1[i]) continue;
         T a0, a1, a2;
         combine16<LS>(acc[i], a0, a1, a2);
         // the row's own direction: its diagonal, the first slot of lane 0
@@ -754,7 +787,7 @@ __device__ __forceinline__ void spmv_chunks(const DevMesh &m, const T *__restric
     }
 #pragma unroll
     for (int i = 0; i < R; i++) {
-        if (!act[i] || !lng[i] || r[i] >= m.chunkRow1[chunk[i]]) continue;
+        if (!act[i] || !lng[i] || r[i] >= r1[i]) continue;
         T a0 = 0, a1 = 0, a2 = 0, px[3] = {0, 0, 0};
         row_spmv<T, FUSE, LS>(m, vals, x, r[i], lane, a0, a1, a2, z, beta[i], px);   // item 0: lane 0's px
         if (lane == 0) finish_row(i, a0, a1, a2, px);
@@ -789,9 +822,131 @@ __global__ __launch_bounds__(kChunkRows * LS) void k_spmv_dot(DevMesh m, const T
 
 static inline int spmv_grid(const DevMesh &m, int R) { return 8 * ((m.perRun + R - 1) / R); }
 
+// FP32 slot values from the FP32 upper values: one thread per slot.
+__global__ __launch_bounds__(256) void k_fill_slot_values(DevMesh m, const float *__restrict__ vals,
+                                                          float *__restrict__ sv) {
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (int64_t)m.nChunks * kSpmvThreads) return;
+    const int64_t p = g / kSpmvThreads;
+    const int s = (int)(g % kSpmvThreads), cf = m.slotCol[g];
+    float e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (cf >= 0) {
+        const float *A = vals + m.slotOff[g];
+        if (cf & kSlotDiag) {
+            const float d[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
+#pragma unroll
+            for (int q = 0; q < 9; q++) e[q] = d[q];
+        } else if (cf & kSlotTrans) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) e[3 * i + j] = A[3 * j + i];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; q++) e[q] = A[q];
+        }
+    }
+    float *b = sv + p * kSlotValFloats;
+    *(float4 *)(b + 4 * s) = make_float4(e[0], e[1], e[2], e[3]);
+    *(float4 *)(b + 4 * kSpmvThreads + 4 * s) = make_float4(e[4], e[5], e[6], e[7]);
+    b[8 * kSpmvThreads + s] = e[8];
+}
+
+void launch_fill_slot_values(const DevMesh &m, hipStream_t s, const float *vals, float *slotVals) {
+    const int64_t n = (int64_t)m.nChunks * kSpmvThreads;
+    hipLaunchKernelGGL(k_fill_slot_values, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, vals, slotVals);
+    OD_HIP(hipGetLastError());
+}
+
+// The FP32 PCG product on slot values (the fused direction update as in spmv_chunks): one
+// chunk per workgroup, K slots per thread; thread t takes slots t + k (1024 / K), so a wave's
+// value loads read whole consecutive slot runs (1 KB per float4 part) and its 16-lane groups
+// are the block rows.  Per slot the same block product, and per row the same 16-lane tree, as
+// spmv_chunks with 16 lanes: the same bits as every other product shape.
+template <int K>
+__global__ __launch_bounds__(kSpmvThreads / K) void k_spmv_sv(DevMesh m, const float *__restrict__ sv,
+                                                              const float *__restrict__ vals,
+                                                              const float *__restrict__ x, float *__restrict__ y,
+                                                              double *__restrict__ partials,
+                                                              const InstScalars *__restrict__ sc,
+                                                              const float *__restrict__ z, float *__restrict__ pNew) {
+    constexpr int T = kSpmvThreads / K;
+    const bool single = m.nInst == 1;
+    if (single && sc[0].done) return;
+    const int pos = run_pos(m, (int)(blockIdx.x >> 3)), p = pos >= 0 ? pos : 0;
+    const int t = (int)threadIdx.x, v = t & (kSpmvLanes - 1);
+    const int4 info = m.runInfo[p];
+    const int inst = info.w & kRunInstMask;
+    // several instances: a chunk of a converged instance loads nothing (its partial is zero)
+    if (!single && pos >= 0 && sc[inst].done) {
+        if (partials && t == 0) partials[info.x] = 0.0;
+        return;
+    }
+    const int *__restrict__ sCol = m.slotCol + (size_t)p * kSpmvThreads;
+    const float *__restrict__ vb = sv + (size_t)p * kSlotValFloats;
+    int cf[K];
+    BlockX<float> b[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int s = t + k * T;
+        cf[k] = sCol[s];
+        const float4 a0 = *(const float4 *)(vb + 4 * s), a1 = *(const float4 *)(vb + 4 * kSpmvThreads + 4 * s);
+        b[k].e[0] = a0.x; b[k].e[1] = a0.y; b[k].e[2] = a0.z; b[k].e[3] = a0.w;
+        b[k].e[4] = a1.x; b[k].e[5] = a1.y; b[k].e[6] = a1.z; b[k].e[7] = a1.w;
+        b[k].e[8] = vb[8 * kSpmvThreads + s];
+    }
+    const bool act = pos >= 0;
+    const bool lng = (info.w & kRunLong) != 0, use = act && !lng;
+    const float beta = (float)sc[single ? 0 : inst].beta;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const bool ok = use && cf[k] >= 0;
+        if (!ok) cf[k] = -1;
+        const int c3 = 3 * (ok ? (cf[k] & kSlotColMask) : 0);
+        b[k].x[0] = x[c3]; b[k].x[1] = x[c3 + 1]; b[k].x[2] = x[c3 + 2];
+        dir_update<float>(z + c3, beta, b[k]);
+    }
+    __shared__ double rowDot[kChunkRows];
+    auto finish_row = [&](int lr, float a0, float a1, float a2, const float *px) {
+        const int rr = info.y + lr;
+        pNew[3 * rr] = px[0]; pNew[3 * rr + 1] = px[1]; pNew[3 * rr + 2] = px[2];
+        y[3 * rr] = a0; y[3 * rr + 1] = a1; y[3 * rr + 2] = a2;
+        if (partials) rowDot[lr] = row3<double>(px[0], px[1], px[2], a0, a1, a2);
+    };
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        float a0 = 0, a1 = 0, a2 = 0;
+        block_mv<float>(b[k], false, false, a0, a1, a2);
+        if (cf[k] < 0) a0 = a1 = a2 = 0;
+        sum_lanes<kSpmvLanes>(a0, a1, a2);
+        const int lr = (t + k * T) / kSpmvLanes;
+        if (partials && v == 0) rowDot[lr] = 0.0;
+        // the row's own direction: its diagonal, slot 0 of the row
+        if (use && v == 0 && info.y + lr < info.z) finish_row(lr, a0, a1, a2, b[k].x);
+    }
+    if (act && lng) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int lr = (t + k * T) / kSpmvLanes;
+            if (info.y + lr >= info.z) continue;
+            float a0 = 0, a1 = 0, a2 = 0, px[3] = {0, 0, 0};
+            row_spmv<float, true, kSpmvLanes>(m, vals, x, info.y + lr, v, a0, a1, a2, z, beta, px);
+            if (v == 0) finish_row(lr, a0, a1, a2, px);
+        }
+    }
+    if (partials) {
+        __syncthreads();
+        if (t < 64) {
+            const double s = wave_sum(rowDot[t]);
+            if (t == 0 && pos >= 0) partials[info.x] = s;
+        }
+    }
+}
+
 template <typename T>
 void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *vals, const T *x, T *y, double *partials,
-                     const InstScalars *sc, hipEvent_t start, hipEvent_t stop, const T *z, T *pNew) {
+                     const InstScalars *sc, hipEvent_t start, hipEvent_t stop, const T *z, T *pNew,
+                     const T *slotVals) {
     // With events, the launch records them from the dispatch's own begin/end timestamps,
     // so their elapsed time is the kernel's execution alone (measurement mode).
     auto go = [&](auto kernel, int threads) {
@@ -819,17 +974,35 @@ void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *va
     };
     if constexpr (sizeof(T) == 4) {          // the fused direction update is an FP32 product
         if (!z) throw HipError("the FP32 product runs with the fused direction update");
-        shape(std::true_type{});
+        if (LS == 0) {
+            if (!slotVals || !sc) throw HipError("the slot-value product needs the slot values and scalars");
+            auto sv = [&](auto kernel, int threads) {
+                if (start)
+                    hipExtLaunchKernelGGL(kernel, dim3(spmv_grid(m, 1)), dim3(threads), 0, s, start, stop, 0, m,
+                                          slotVals, vals, x, y, partials, sc, z, pNew);
+                else
+                    hipLaunchKernelGGL(kernel, dim3(spmv_grid(m, 1)), dim3(threads), 0, s, m, slotVals, vals, x, y,
+                                       partials, sc, z, pNew);
+            };
+            if (R == 4) sv(k_spmv_sv<4>, kSpmvThreads / 4);
+            else if (R == 2) sv(k_spmv_sv<2>, kSpmvThreads / 2);
+            else sv(k_spmv_sv<1>, kSpmvThreads);
+        } else {
+            shape(std::true_type{});
+        }
     } else {
         if (z) throw HipError("the fused direction update is an FP32 product");
+        if (LS == 0) throw HipError("the slot-value product is an FP32 product");
         shape(std::false_type{});
     }
     OD_HIP(hipGetLastError());
 }
 template void launch_spmv_dot<double>(const DevMesh &, hipStream_t, int, int, const double *, const double *, double *,
-                                      double *, const InstScalars *, hipEvent_t, hipEvent_t, const double *, double *);
+                                      double *, const InstScalars *, hipEvent_t, hipEvent_t, const double *, double *,
+                                      const double *);
 template void launch_spmv_dot<float>(const DevMesh &, hipStream_t, int, int, const float *, const float *, float *,
-                                     double *, const InstScalars *, hipEvent_t, hipEvent_t, const float *, float *);
+                                     double *, const InstScalars *, hipEvent_t, hipEvent_t, const float *, float *,
+                                     const float *);
 
 template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
@@ -846,7 +1019,8 @@ template <typename T>
 void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *pOld, const T *z, T *p, T *q, PcgWork &w,
                hipEvent_t start, hipEvent_t stop) {
     launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, sizeof(T) == 8 ? m.spmvL64 : m.spmvL32, vals,
-                       z ? pOld : p, q, w.partials, w.sc, start, stop, z, z ? p : nullptr);
+                       z ? pOld : p, q, w.partials, w.sc, start, stop, z, z ? p : nullptr,
+                       sizeof(T) == 4 ? (const T *)w.slotVals : nullptr);
 }
 
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w) {
@@ -858,11 +1032,16 @@ void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w) {
 // =====================================================================================
 constexpr int kVecThreads = 3 * kChunkRows;
 
-#define CHUNK_PROLOGUE                                            \
-    const int chunk = blockIdx.x;                                 \
-    const int inst = m.chunkInst[chunk];                          \
-    const int i = 3 * m.chunkRow0[chunk] + (int)threadIdx.x;      \
-    const bool in = i < 3 * m.chunkRow1[chunk];                   \
+// A mesh of one instance has chunk c on rows [64 c, min(64 c + 64, nRows)): no chunk record
+// is loaded before the vectors.
+#define CHUNK_PROLOGUE                                                                        \
+    const int chunk = blockIdx.x;                                                             \
+    const bool singleInst = m.nInst == 1;                                                     \
+    const int inst = singleInst ? 0 : m.chunkInst[chunk];                                     \
+    const int row0 = singleInst ? chunk * kChunkRows : m.chunkRow0[chunk];                    \
+    const int row1 = singleInst ? min(row0 + kChunkRows, m.nRows) : m.chunkRow1[chunk];       \
+    const int i = 3 * row0 + (int)threadIdx.x;                                                \
+    const bool in = i < 3 * row1;                                                             \
     (void)inst;
 
 // r = b - q ; slots [1] r.r, [2] b.b
@@ -1014,6 +1193,19 @@ void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *
 // =====================================================================================
 __device__ __forceinline__ double pivot_fix(double d, double eps2) { return d <= 0.0 ? eps2 * d * d : d; }
 
+// Entry q (0-8) of block b in a part-major array of nb blocks (the FP32 factor copies).
+__device__ __forceinline__ int64_t pm_at(int64_t nb, int64_t b, int q) {
+    return q < 4 ? 4 * b + q : (q < 8 ? 4 * nb + 4 * b + (q - 4) : 8 * nb + b);
+}
+
+// The nine entries of block b of a part-major FP32 array: two float4 loads and one float.
+__device__ __forceinline__ void load_pm(const float *__restrict__ P, int64_t nb, int64_t b, float (&e)[9]) {
+    const float4 a = ((const float4 *)P)[b], c = ((const float4 *)(P + 4 * nb))[b];
+    e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+    e[4] = c.x; e[5] = c.y; e[6] = c.z; e[7] = c.w;
+    e[8] = P[8 * nb + b];
+}
+
 // One block row of U per 8-lane group: lane 0 forms and factors the diagonal block,
 // lanes 1..7 the row's upper off-diagonal blocks U_IJ = U_II^-T (A_IJ - sum_K U_KI^T U_KJ).
 template <typename T>
@@ -1052,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
 #pragma unroll
         for (int j = 0; j < 6; j++) {
             U[v0 + j] = u[j];
-            if (U32) U32[v0 + j] = (T)u[j];
+            if (U32) U32[8 * (int64_t)I + j] = (T)u[j];
         }
     }
 #pragma unroll
@@ -1085,10 +1277,18 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
             const double x1 = (S[3 + j] - u[1] * x0) / u[3];
             const double x2 = (S[6 + j] - u[2] * x0 - u[4] * x1) / u[5];
             U[vq + j] = x0; U[vq + 3 + j] = x1; U[vq + 6 + j] = x2;
-            if (U32) { U32[vq + j] = (T)x0; U32[vq + 3 + j] = (T)x1; U32[vq + 6 + j] = (T)x2; }
-            const int64_t lq = 9 * (int64_t)m.tPos[b0 + 1 + q];
+            const int64_t bq = b0 + 1 + q, tq = m.tPos[bq];
+            if (U32) {
+                T *Up = U32 + 8 * (int64_t)m.nRows;
+                Up[pm_at(m.nBlocks, bq, j)] = (T)x0; Up[pm_at(m.nBlocks, bq, 3 + j)] = (T)x1;
+                Up[pm_at(m.nBlocks, bq, 6 + j)] = (T)x2;
+            }
+            const int64_t lq = 9 * tq;
             Ul[lq + j] = x0; Ul[lq + 3 + j] = x1; Ul[lq + 6 + j] = x2;
-            if (Ul32) { Ul32[lq + j] = (T)x0; Ul32[lq + 3 + j] = (T)x1; Ul32[lq + 6 + j] = (T)x2; }
+            if (Ul32) {
+                Ul32[pm_at(m.nT, tq, j)] = (T)x0; Ul32[pm_at(m.nT, tq, 3 + j)] = (T)x1;
+                Ul32[pm_at(m.nT, tq, 6 + j)] = (T)x2;
+            }
         }
     }
 }
@@ -1129,6 +1329,43 @@ __device__ __forceinline__ void diag_bwd(const T *D, T &y0, T &y1, T &y2) {
     y0 = (y0 - D[1] * y1 - D[2] * y2) / D[0];
 }
 
+// Whether a sweep row still runs: instance inst neither converged (sc) nor skipped.  Called
+// twice per kernel: first (head) with instance 0 for a mesh of one instance, from fixed
+// addresses before any row data is loaded (the whole grid exits together past convergence),
+// then with the row's instance for a mesh of several.
+__device__ __forceinline__ bool sweep_row_live(const DevMesh &m, int inst, const InstScalars *__restrict__ sc,
+                                               const uint8_t *__restrict__ skip, bool head) {
+    if ((m.nInst == 1) != head) return true;
+    return !((sc && sc[inst].done) || (skip && skip[inst]));
+}
+
+// A factor block and the three entries of x it multiplies: the FP32 copies are part-major
+// (block b of nb), the FP64 factor is in the canonical layout (9 entries from offset aos).
+template <typename T>
+__device__ __forceinline__ void load_factor_block(const T *__restrict__ P, int64_t nb, int64_t b, int64_t aos,
+                                                  const T *__restrict__ xc, BlockX<T> &blk) {
+    if constexpr (sizeof(T) == 4) {
+        load_pm(P, nb, b, blk.e);
+        blk.x[0] = xc[0]; blk.x[1] = xc[1]; blk.x[2] = xc[2];
+    } else {
+        load_block<T>(P + aos, xc, blk);
+    }
+}
+
+// The packed diagonal block of row I: FP32 from the diagonal part of U32 (8 floats per row),
+// FP64 from the canonical layout at valOff[I].
+template <typename T>
+__device__ __forceinline__ void load_diag(const T *__restrict__ U, int I, int v0, T (&d)[6]) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 a = ((const float4 *)U)[2 * (int64_t)I];
+        const float2 c = ((const float2 *)U)[4 * (int64_t)I + 2];
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = c.x; d[5] = c.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 6; q++) d[q] = U[v0 + q];
+    }
+}
+
 // forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the
 // lower copy (contiguous per row I)
 template <typename T, int L>
@@ -1137,13 +1374,18 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_fwd(DevMesh m, int p0, int c
                                                         T *__restrict__ y, const InstScalars *__restrict__ sc,
                                                         const uint8_t *__restrict__ skip, int lastLevel) {
     constexpr int kRows = kIcThreads / L, G = 16 / L;
+    if (!sweep_row_live(m, 0, sc, skip, true)) return;
     const int wg = xcd_wg((count + kRows - 1) / kRows);
     const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
     if (pl >= count) return;
     const int I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
-    const int inst = m.rowInst[I];
-    if ((sc && sc[inst].done) || (skip && skip[inst])) return;
-    const int kb = m.tPtr[I], ke = m.tPtr[I + 1];
+    const int4 ri = m.rowFwd[I];
+    if (!sweep_row_live(m, ri.w, sc, skip, false)) return;
+    const int kb = ri.x, ke = ri.y;
+    // the diagonal block and the right-hand side, loaded with the blocks
+    T d[6], rhs[3];
+    load_diag<T>(U, I, ri.z, d);
+    rhs[0] = r[3 * I]; rhs[1] = r[3 * I + 1]; rhs[2] = r[3 * I + 2];
     T acc[G][3];
 #pragma unroll
     for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
@@ -1153,21 +1395,20 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_fwd(DevMesh m, int p0, int c
             const int k = kk + g * L;
             if (k < ke) {
                 BlockX<T> b;
-                load_block<T>(Ul + 9 * k, y + 3 * m.tRow[k], b);
-                acc[g][0] += b.e[0] * b.x[0] + b.e[3] * b.x[1] + b.e[6] * b.x[2];
-                acc[g][1] += b.e[1] * b.x[0] + b.e[4] * b.x[1] + b.e[7] * b.x[2];
-                acc[g][2] += b.e[2] * b.x[0] + b.e[5] * b.x[1] + b.e[8] * b.x[2];
+                load_factor_block<T>(Ul, m.nT, k, 9 * (int64_t)k, y + 3 * m.tRow[k], b);
+                acc_add(acc[g][0], row3(b.e[0], b.e[3], b.e[6], b.x[0], b.x[1], b.x[2]));
+                acc_add(acc[g][1], row3(b.e[1], b.e[4], b.e[7], b.x[0], b.x[1], b.x[2]));
+                acc_add(acc[g][2], row3(b.e[2], b.e[5], b.e[8], b.x[0], b.x[1], b.x[2]));
             }
         }
     }
     T a0, a1, a2;
     combine16<L>(acc, a0, a1, a2);
     if (lane != 0) return;
-    const T *D = U + m.valOff[I];
-    T z0 = (r[3 * I] - a0) / D[0];
-    T z1 = (r[3 * I + 1] - a1 - D[1] * z0) / D[3];
-    T z2 = (r[3 * I + 2] - a2 - D[2] * z0 - D[4] * z1) / D[5];
-    if (lastLevel) diag_bwd(D, z0, z1, z2);   // rows of the last level have no blocks above them
+    T z0 = (rhs[0] - a0) / d[0];
+    T z1 = (rhs[1] - a1 - d[1] * z0) / d[3];
+    T z2 = (rhs[2] - a2 - d[2] * z0 - d[4] * z1) / d[5];
+    if (lastLevel) diag_bwd(d, z0, z1, z2);   // rows of the last level have no blocks above them
     y[3 * I] = z0; y[3 * I + 1] = z1; y[3 * I + 2] = z2;
 }
 
@@ -1177,13 +1418,19 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_bwd(DevMesh m, int p0, int c
                                                         T *__restrict__ z, const InstScalars *__restrict__ sc,
                                                         const uint8_t *__restrict__ skip) {
     constexpr int kRows = kIcThreads / L, G = 16 / L;
+    if (!sweep_row_live(m, 0, sc, skip, true)) return;
     const int wg = xcd_wg((count + kRows - 1) / kRows);
     const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
     if (pl >= count) return;
     const int I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
-    const int inst = m.rowInst[I];
-    if ((sc && sc[inst].done) || (skip && skip[inst])) return;
-    const int v0 = m.valOff[I], b0 = m.rowPtr[I], nOff = m.rowPtr[I + 1] - b0 - 1;
+    const int4 ri = m.rowBwd[I];
+    if (!sweep_row_live(m, ri.w, sc, skip, false)) return;
+    const int v0 = ri.z, b0 = ri.x, nOff = ri.y - b0 - 1;
+    // the diagonal block and the forward result, loaded with the blocks
+    T d[6], yv[3];
+    load_diag<T>(U, I, v0, d);
+    yv[0] = z[3 * I]; yv[1] = z[3 * I + 1]; yv[2] = z[3 * I + 2];
+    const T *Uo = sizeof(T) == 4 ? U + 8 * (int64_t)m.nRows : U;   // FP32: the part-major upper blocks
     T acc[G][3];
 #pragma unroll
     for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
@@ -1193,20 +1440,19 @@ __global__ __launch_bounds__(kIcThreads) void k_ic0_bwd(DevMesh m, int p0, int c
             const int k = kk + g * L;
             if (k < nOff) {
                 BlockX<T> b;
-                load_block<T>(U + v0 + 6 + 9 * k, z + 3 * m.colIdx[b0 + 1 + k], b);
-                acc[g][0] += b.e[0] * b.x[0] + b.e[1] * b.x[1] + b.e[2] * b.x[2];
-                acc[g][1] += b.e[3] * b.x[0] + b.e[4] * b.x[1] + b.e[5] * b.x[2];
-                acc[g][2] += b.e[6] * b.x[0] + b.e[7] * b.x[1] + b.e[8] * b.x[2];
+                load_factor_block<T>(Uo, m.nBlocks, b0 + 1 + k, v0 + 6 + 9 * (int64_t)k, z + 3 * m.colIdx[b0 + 1 + k], b);
+                acc_add(acc[g][0], row3(b.e[0], b.e[1], b.e[2], b.x[0], b.x[1], b.x[2]));
+                acc_add(acc[g][1], row3(b.e[3], b.e[4], b.e[5], b.x[0], b.x[1], b.x[2]));
+                acc_add(acc[g][2], row3(b.e[6], b.e[7], b.e[8], b.x[0], b.x[1], b.x[2]));
             }
         }
     }
     T a0, a1, a2;
     combine16<L>(acc, a0, a1, a2);
     if (lane != 0) return;
-    const T *D = U + v0;
-    const T x2 = (z[3 * I + 2] - a2) / D[5];
-    const T x1 = (z[3 * I + 1] - a1 - D[4] * x2) / D[3];
-    const T x0 = (z[3 * I] - a0 - D[1] * x1 - D[2] * x2) / D[0];
+    const T x2 = (yv[2] - a2) / d[5];
+    const T x1 = (yv[1] - a1 - d[4] * x2) / d[3];
+    const T x0 = (yv[0] - a0 - d[1] * x1 - d[2] * x2) / d[0];
     z[3 * I] = x0; z[3 * I + 1] = x1; z[3 * I + 2] = x2;
 }
 
@@ -1269,7 +1515,8 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
         return p;
     };
     try {
-        float *U = (float *)get(4 * ((size_t)nValues + kValuePad)), *Ul = (float *)get(4 * (9 * (size_t)nT + kValuePad));
+        (void)nValues; (void)nT;
+        float *U = (float *)get(4 * (size_t)ic0_u32_floats(m)), *Ul = (float *)get(4 * (size_t)ic0_ul32_floats(m));
         float *r = (float *)get(4 * n), *z = (float *)get(4 * n);
         OD_HIP(hipEventCreate(&e0));
         OD_HIP(hipEventCreate(&e1));
@@ -1318,10 +1565,18 @@ void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &l
 template <typename T>
 __global__ __launch_bounds__(kVecThreads) void k_update_xr_ic0(DevMesh m, const T *p, const T *q, T *x, T *r,
                                                                const T *__restrict__ U, T *z, int singleLevel,
-                                                               double *partials, const InstScalars *sc) {
+                                                               int level0End, double *partials,
+                                                               const InstScalars *sc) {
     CHUNK_PROLOGUE
     __shared__ T rs[kVecThreads];
     const bool act = !sc[inst].done;
+    // rows of level 0 (rows below level0End when the levels are consecutive ranges): their
+    // diagonal blocks are loaded with the vectors
+    const int row = row0 + (int)threadIdx.x;
+    const bool lev0 = act && threadIdx.x < kChunkRows && row < row1 &&
+                      (level0End >= 0 ? row < level0End : m.rowLevel[row] == 0);
+    T d[6] = {1, 0, 0, 1, 0, 1};
+    if (lev0) load_diag<T>(U, row, sizeof(T) == 4 ? 0 : m.rowFwd[row].z, d);
     double v[1] = {0};
     T ri = 0;
     if (in && act) {
@@ -1334,14 +1589,12 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xr_ic0(DevMesh m, const 
     rs[threadIdx.x] = ri;
     block_sum<1>(v);                       // its barrier also publishes rs
     if (threadIdx.x == 0) partials[m.nChunks + chunk] = v[0];
-    const int row = m.chunkRow0[chunk] + (int)threadIdx.x;
-    if (!act || threadIdx.x >= kChunkRows || row >= m.chunkRow1[chunk] || m.rowLevel[row] != 0) return;
-    const T *D = U + m.valOff[row];
+    if (!lev0) return;
     const int b = 3 * (int)threadIdx.x;
-    T y0 = rs[b] / D[0];
-    T y1 = (rs[b + 1] - D[1] * y0) / D[3];
-    T y2 = (rs[b + 2] - D[2] * y0 - D[4] * y1) / D[5];
-    if (singleLevel) diag_bwd(D, y0, y1, y2);
+    T y0 = rs[b] / d[0];
+    T y1 = (rs[b + 1] - d[1] * y0) / d[3];
+    T y2 = (rs[b + 2] - d[2] * y0 - d[4] * y1) / d[5];
+    if (singleLevel) diag_bwd(d, y0, y1, y2);
     z[3 * row] = y0; z[3 * row + 1] = y1; z[3 * row + 2] = y2;
 }
 
@@ -1350,7 +1603,7 @@ void pcg_update_xr_ic0(const DevMesh &m, hipStream_t s, const std::vector<int> &
                        T *r, const T *U, const T *Ul, T *z, PcgWork &w, const uint8_t *skip) {
     const int nl = (int)levelPtr.size() - 1;
     hipLaunchKernelGGL(k_update_xr_ic0<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, p, q, x, r, U, z,
-                       nl == 1 ? 1 : 0, w.partials, w.sc);
+                       nl == 1 ? 1 : 0, m.levelsContiguous && nl >= 1 ? levelPtr[1] : -1, w.partials, w.sc);
     OD_HIP(hipGetLastError());
     if (nl > 1) ic0_levels<T>(m, s, levelPtr, 1, true, nl - 1, U, Ul, r, z, w.sc, skip);
 }
