@@ -427,8 +427,9 @@ static int count_not_done(od_ctx *ctx, const int *doneDev, int nInst) {
     return n;
 }
 
-// Builds the preconditioner from A (Preconditioner::Preprocess, cgSolver.cpp:23).
-static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip) {
+// Builds the preconditioner from A (Preconditioner::Preprocess, cgSolver.cpp:23).  The FP64
+// lower copy of the IC(0) factor is written when the FP64 sweeps will run (fp64Sweeps).
+static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip, bool fp64Sweeps = true) {
     od_mesh *mesh = s->mesh;
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
@@ -436,7 +437,9 @@ static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip)
         launch_jacobi_diag<double>(m, st, A->vals, s->invDiag);
         if (s->invDiag32) launch_jacobi_diag<float>(m, st, A->vals, s->invDiag32);
     } else if (s->precond == OD_PRE_IC0) {
-        if (s->U32) launch_ic0_factor<float>(m, st, mesh->topo.levelPtr, A->vals, s->U, s->U32, s->Ul, s->Ul32, s->param, skip);
+        if (s->U32)
+            launch_ic0_factor<float>(m, st, mesh->topo.levelPtr, A->vals, s->U, s->U32, fp64Sweeps ? s->Ul : nullptr,
+                                     s->Ul32, s->param, skip);
         else launch_ic0_factor<double>(m, st, mesh->topo.levelPtr, A->vals, s->U, (double *)nullptr, s->Ul, (double *)nullptr,
                                        s->param, skip);
     }
@@ -565,7 +568,7 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     }
     // ---- mixed: FP32 inner PCG, FP64 residual and correction (iterative refinement)
     require(A->vals32 != nullptr, "mixed precision needs the FP32 copy of A (od_matrix_sync_f32)", OD_ERR_STATE);
-    build_precond(s, A, pre);
+    build_precond(s, A, pre, false);
     if (m.spmvL32 == 0) {
         if (!s->sv32) s->sv32 = (float *)s->alloc(sizeof(float) * kSlotValFloats * (size_t)std::max(1, m.nChunks));
         launch_fill_slot_values(m, st, A->vals32, s->sv32);
@@ -767,6 +770,29 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.rowCStart = mesh->upload(t.rowCStart);
         m.levelRows = mesh->upload(t.levelRows);
         m.rowLevel = mesh->upload(t.level);
+        {
+            // IC(0) update pairs (the factor kernel then searches nothing)
+            std::vector<int> pp((size_t)t.nBlocks + 1, 0), pr;
+            pr.reserve(8 * (size_t)t.nBlocks);
+            require(t.nValues < (int64_t)INT32_MAX, "too many values for the IC(0) pair offsets", OD_ERR_UNSUPPORTED);
+            for (int I = 0; I < t.nRows; I++)
+                for (int b = t.rowPtr[I] + 1; b < t.rowPtr[I + 1]; b++) {
+                    const int J = t.colIdx[b];
+                    for (int k = t.tPtr[I]; k < t.tPtr[I + 1]; k++) {
+                        const int K = t.tRow[k];
+                        const int *c0 = t.colIdx.data() + t.tBlk[k] + 1, *c1 = t.colIdx.data() + t.rowPtr[K + 1];
+                        const int *f = std::lower_bound(c0, c1, J);
+                        if (f == c1 || *f != J) continue;
+                        pr.push_back(tOff[k]);
+                        pr.push_back(t.blkOff[f - t.colIdx.data()]);
+                    }
+                    pp[b + 1] = (int)(pr.size() / 2);
+                }
+            for (int64_t b = 1; b <= t.nBlocks; b++) pp[b] = std::max(pp[b], pp[b - 1]);   // diagonal blocks: empty
+            m.icPairPtr = mesh->upload(pp);
+            if (pr.empty()) pr.assign(2, 0);
+            m.icPairs = (const int2 *)mesh->upload(pr);
+        }
         {
             // per-row records of the sweeps, one 16-byte load each
             std::vector<int> fw(4 * (size_t)t.nRows), bw(4 * (size_t)t.nRows);

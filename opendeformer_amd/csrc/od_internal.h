@@ -49,6 +49,10 @@ struct DevMesh {
     int *levelRows;                              // nRows
     int levelsContiguous;                        // levels are consecutive row ranges (levelRows is the identity)
     int *rowLevel;                               // IC(0) dependency level of each row
+    // IC(0) update pairs of each upper block b = (I, J): the lower neighbours K of I (in
+    // transposed-list order) whose row also holds column J, as value offsets of U_KI and U_KJ
+    int *icPairPtr;                              // nBlocks+1
+    const int2 *icPairs;
     // per row, for the sweeps: {tPtr[r], tPtr[r+1], valOff[r], instance} and
     // {rowPtr[r], rowPtr[r+1], valOff[r], instance}
     const int4 *rowFwd, *rowBwd;

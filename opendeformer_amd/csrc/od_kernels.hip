@@ -1007,6 +1007,13 @@ template void launch_spmv_dot<float>(const DevMesh &, hipStream_t, int, int, con
 template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
                  const InstScalars *sc) {
+    // an FP64 product y = A x runs the autotuned FP64 shape (same bits as every shape)
+    if constexpr (sizeof(T) == 8) {
+        if (!accumulate) {
+            launch_spmv_dot<T>(m, s, m.spmvR64, m.spmvL64, vals, x, y, nullptr, sc);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_spmv<T>, dim3(spmv_grid(m, 1)), dim3(kSpmvThreads), 0, s, m, vals, x, y, accumulate, sc);
     OD_HIP(hipGetLastError());
 }
@@ -1207,7 +1214,11 @@ __device__ __forceinline__ void load_pm(const float *__restrict__ P, int64_t nb,
 }
 
 // One block row of U per 8-lane group: lane 0 forms and factors the diagonal block,
-// lanes 1..7 the row's upper off-diagonal blocks U_IJ = U_II^-T (A_IJ - sum_K U_KI^T U_KJ).
+// lanes 1..7 the row's upper off-diagonal blocks U_IJ = U_II^-T (A_IJ - sum_K U_KI^T U_KJ),
+// the terms K from the block's precomputed pair list (icPairs: no search).  Every U_KI is
+// read from U itself (transposed value offsets); the FP64 lower copy Ul is written only when
+// given (the FP64 sweeps read it).
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *__restrict__ rows, int count,
                                                           const double *__restrict__ A, double *__restrict__ U,
@@ -1225,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
     if (lane == 0) {
         double s00 = A[v0], s01 = A[v0 + 1], s02 = A[v0 + 2], s11 = A[v0 + 3], s12 = A[v0 + 4], s22 = A[v0 + 5];
         for (int k = k0; k < k1; k++) {
-            const double *W = Ul + 9 * k;
+            const double *W = U + m.tOff[k];
             const double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5], w6 = W[6], w7 = W[7], w8 = W[8];
             s00 -= w0 * w0 + w3 * w3 + w6 * w6;
             s01 -= w0 * w1 + w3 * w4 + w6 * w7;
@@ -1251,20 +1262,14 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
     for (int j = 0; j < 6; j++) u[j] = __shfl(u[j], 0, 8);
     if (lane == 0) return;
     for (int q = lane - 1; q < nOff; q += 7) {
-        const int J = m.colIdx[b0 + 1 + q];
         const int vq = v0 + 6 + 9 * q;
         double S[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) S[j] = A[vq + j];
-        for (int k = k0; k < k1; k++) {
-            // U_KJ: row K's upper blocks after (K, I) are sorted; find column J
-            const int K = m.tRow[k];
-            const int pe = m.rowPtr[K + 1];
-            int p = m.tBlk[k] + 1;
-            while (p < pe && m.colIdx[p] < J) p++;
-            if (p >= pe || m.colIdx[p] != J) continue;
-            const double *W = Ul + 9 * k;
-            const double *Z = U + m.valOff[K] + 6 + 9 * (p - m.rowPtr[K] - 1);
+        const int e1 = m.icPairPtr[b0 + 2 + q];
+        for (int e = m.icPairPtr[b0 + 1 + q]; e < e1; e++) {
+            const int2 pr = m.icPairs[e];
+            const double *W = U + pr.x, *Z = U + pr.y;
 #pragma unroll
             for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -1283,8 +1288,10 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
                 Up[pm_at(m.nBlocks, bq, j)] = (T)x0; Up[pm_at(m.nBlocks, bq, 3 + j)] = (T)x1;
                 Up[pm_at(m.nBlocks, bq, 6 + j)] = (T)x2;
             }
-            const int64_t lq = 9 * tq;
-            Ul[lq + j] = x0; Ul[lq + 3 + j] = x1; Ul[lq + 6 + j] = x2;
+            if (Ul) {
+                const int64_t lq = 9 * tq;
+                Ul[lq + j] = x0; Ul[lq + 3 + j] = x1; Ul[lq + 6 + j] = x2;
+            }
             if (Ul32) {
                 Ul32[pm_at(m.nT, tq, j)] = (T)x0; Ul32[pm_at(m.nT, tq, 3 + j)] = (T)x1;
                 Ul32[pm_at(m.nT, tq, 6 + j)] = (T)x2;
