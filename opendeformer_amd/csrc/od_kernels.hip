@@ -495,7 +495,11 @@ __device__ __forceinline__ void reduce_slots(const DevMesh &m, const double *__r
 // thread of a workgroup (any size up to 1024).
 __device__ void finalize_inst(const DevMesh &m, int phase, int inst, const double *__restrict__ partials,
                               InstScalars *__restrict__ sc, int *__restrict__ doneFlags, const FinalArgs &fa) {
-    const int c0 = m.instChunkStart[inst], c1 = m.instChunkStart[inst + 1];
+    // one instance: all chunks (no range load); the scalars are loaded with the partials
+    const bool single = m.nInst == 1;
+    const int c0 = single ? 0 : m.instChunkStart[inst], c1 = single ? m.nChunks : m.instChunkStart[inst + 1];
+    InstScalars S{};
+    if (threadIdx.x == 0) S = sc[inst];
     double t[3] = {0, 0, 0};
     if (phase == PH_ALPHA) reduce_slots<0, 1>(m, partials, c0, c1, t);
     else if (phase == PH_ITER) reduce_slots<0, 2>(m, partials, c0, c1, t);
@@ -503,7 +507,6 @@ __device__ void finalize_inst(const DevMesh &m, int phase, int inst, const doubl
     else if (phase == PH_OUTER_FIRST) reduce_slots<1, 2>(m, partials, c0, c1, t);
     else reduce_slots<1, 1>(m, partials, c0, c1, t);
     if (threadIdx.x != 0) return;
-    InstScalars S = sc[inst];
     switch (phase) {
     case PH_INIT:
         S.delta = t[0]; S.rr = t[1]; S.bb = t[2];
@@ -1276,26 +1279,29 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
                 for (int j = 0; j < 3; j++) S[i * 3 + j] -= W[0 * 3 + i] * Z[0 * 3 + j] + W[1 * 3 + i] * Z[1 * 3 + j] + W[2 * 3 + i] * Z[2 * 3 + j];
         }
         // U_IJ = U_II^-T S (forward substitution with the lower factor U_II^T)
+        double X[9];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const double x0 = S[j] / u[0];
-            const double x1 = (S[3 + j] - u[1] * x0) / u[3];
-            const double x2 = (S[6 + j] - u[2] * x0 - u[4] * x1) / u[5];
-            U[vq + j] = x0; U[vq + 3 + j] = x1; U[vq + 6 + j] = x2;
-            const int64_t bq = b0 + 1 + q, tq = m.tPos[bq];
-            if (U32) {
-                T *Up = U32 + 8 * (int64_t)m.nRows;
-                Up[pm_at(m.nBlocks, bq, j)] = (T)x0; Up[pm_at(m.nBlocks, bq, 3 + j)] = (T)x1;
-                Up[pm_at(m.nBlocks, bq, 6 + j)] = (T)x2;
-            }
-            if (Ul) {
-                const int64_t lq = 9 * tq;
-                Ul[lq + j] = x0; Ul[lq + 3 + j] = x1; Ul[lq + 6 + j] = x2;
-            }
-            if (Ul32) {
-                Ul32[pm_at(m.nT, tq, j)] = (T)x0; Ul32[pm_at(m.nT, tq, 3 + j)] = (T)x1;
-                Ul32[pm_at(m.nT, tq, 6 + j)] = (T)x2;
-            }
+            X[j] = S[j] / u[0];
+            X[3 + j] = (S[3 + j] - u[1] * X[j]) / u[3];
+            X[6 + j] = (S[6 + j] - u[2] * X[j] - u[4] * X[3 + j]) / u[5];
+        }
+        const int64_t bq = b0 + 1 + q, tq = m.tPos[bq];
+#pragma unroll
+        for (int j = 0; j < 9; j++) U[vq + j] = X[j];
+        if (Ul) {
+#pragma unroll
+            for (int j = 0; j < 9; j++) Ul[9 * tq + j] = X[j];
+        }
+        if constexpr (sizeof(T) == 4) {
+            // part-major FP32 copies: one float4, float4, float store each
+            auto put = [&](float *P, int64_t nb, int64_t b) {
+                ((float4 *)P)[b] = make_float4((float)X[0], (float)X[1], (float)X[2], (float)X[3]);
+                ((float4 *)(P + 4 * nb))[b] = make_float4((float)X[4], (float)X[5], (float)X[6], (float)X[7]);
+                P[8 * nb + b] = (float)X[8];
+            };
+            if (U32) put(U32 + 8 * (int64_t)m.nRows, m.nBlocks, bq);
+            if (Ul32) put(Ul32, m.nT, tq);
         }
     }
 }
