@@ -481,7 +481,8 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
         pcg_alpha_finalize(m, st, w);
         if (ic) {
             pcg_update_xr_ic0<T>(m, st, mesh->topo.levelPtr, pNew, q, x, r, Uf, Ul, z, w, preDone);
-            pcg_dot_rz<T>(m, st, r, z, w);
+            if (ic0_dot_fused(m, mesh->topo.levelPtr)) pcg_finalize_iter(m, st, w);
+            else pcg_dot_rz<T>(m, st, r, z, w);
         } else {
             pcg_update_xr<T>(m, st, pNew, q, x, r, invDiag, z, w);
         }

@@ -155,6 +155,11 @@ template <typename T>
 void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *pOld, const T *z, T *p, T *q, PcgWork &w,
                hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w);
+// The IC(0) iteration forms the r.z chunk partials inside its sweeps (one instance, levels as
+// consecutive row ranges, at least two levels): the iteration then ends with
+// pcg_finalize_iter instead of pcg_dot_rz.
+bool ic0_dot_fused(const DevMesh &m, const std::vector<int> &levelPtr);
+void pcg_finalize_iter(const DevMesh &m, hipStream_t s, PcgWork &w);
 template <typename T>
 void pcg_update_xr(const DevMesh &m, hipStream_t s, const T *p, const T *q, T *x, T *r, const T *invDiag, T *z,
                    PcgWork &w);

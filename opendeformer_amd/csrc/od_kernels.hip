@@ -1381,132 +1381,230 @@ __device__ __forceinline__ void load_diag(const T *__restrict__ U, int I, int v0
 
 // forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the
 // lower copy (contiguous per row I)
-template <typename T, int L>
-__global__ __launch_bounds__(kIcThreads) void k_ic0_fwd(DevMesh m, int p0, int count, const T *__restrict__ U,
-                                                        const T *__restrict__ Ul, const T *__restrict__ r,
-                                                        T *__restrict__ y, const InstScalars *__restrict__ sc,
-                                                        const uint8_t *__restrict__ skip, int lastLevel) {
-    constexpr int kRows = kIcThreads / L, G = 16 / L;
-    if (!sweep_row_live(m, 0, sc, skip, true)) return;
-    const int wg = xcd_wg((count + kRows - 1) / kRows);
-    const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
-    if (pl >= count) return;
-    const int I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
-    const int4 ri = m.rowFwd[I];
-    if (!sweep_row_live(m, ri.w, sc, skip, false)) return;
-    const int kb = ri.x, ke = ri.y;
-    // the diagonal block and the right-hand side, loaded with the blocks
-    T d[6], rhs[3];
-    load_diag<T>(U, I, ri.z, d);
-    rhs[0] = r[3 * I]; rhs[1] = r[3 * I + 1]; rhs[2] = r[3 * I + 2];
-    T acc[G][3];
-#pragma unroll
-    for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
-    for (int kk = kb + lane; kk < ke; kk += 16) {
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-            const int k = kk + g * L;
-            if (k < ke) {
-                BlockX<T> b;
-                load_factor_block<T>(Ul, m.nT, k, 9 * (int64_t)k, y + 3 * m.tRow[k], b);
-                acc_add(acc[g][0], row3(b.e[0], b.e[3], b.e[6], b.x[0], b.x[1], b.x[2]));
-                acc_add(acc[g][1], row3(b.e[1], b.e[4], b.e[7], b.x[0], b.x[1], b.x[2]));
-                acc_add(acc[g][2], row3(b.e[2], b.e[5], b.e[8], b.x[0], b.x[1], b.x[2]));
-            }
-        }
-    }
-    T a0, a1, a2;
-    combine16<L>(acc, a0, a1, a2);
-    if (lane != 0) return;
-    T z0 = (rhs[0] - a0) / d[0];
-    T z1 = (rhs[1] - a1 - d[1] * z0) / d[3];
-    T z2 = (rhs[2] - a2 - d[2] * z0 - d[4] * z1) / d[5];
-    if (lastLevel) diag_bwd(d, z0, z1, z2);   // rows of the last level have no blocks above them
-    y[3 * I] = z0; y[3 * I + 1] = z1; y[3 * I + 2] = z2;
+// Rows of a sweep workgroup.  Plain launches: kIcThreads / L consecutive rows of the level
+// (level-relative).  Launches that also form the r.z dot (DOT; one instance, levels as
+// consecutive row ranges): the rows of one 64-row chunk that lie in the level, so the
+// workgroup can reduce the chunk's r.z terms.  xcd_wg spreads either over the XCDs.
+__device__ __forceinline__ int sweep_chunks(int p0, int count) {
+    return (p0 + count - 1) / kChunkRows - p0 / kChunkRows + 1;
 }
 
-// backward: U z = y  ->  z_I = U_II^-1 (y_I - sum_{J>I} U_IJ z_J)
-template <typename T, int L>
-__global__ __launch_bounds__(kIcThreads) void k_ic0_bwd(DevMesh m, int p0, int count, const T *__restrict__ U,
-                                                        T *__restrict__ z, const InstScalars *__restrict__ sc,
-                                                        const uint8_t *__restrict__ skip) {
-    constexpr int kRows = kIcThreads / L, G = 16 / L;
-    if (!sweep_row_live(m, 0, sc, skip, true)) return;
-    const int wg = xcd_wg((count + kRows - 1) / kRows);
-    const int pl = wg * kRows + (int)(threadIdx.x / L), lane = threadIdx.x % L;
-    if (pl >= count) return;
-    const int I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
-    const int4 ri = m.rowBwd[I];
-    if (!sweep_row_live(m, ri.w, sc, skip, false)) return;
-    const int v0 = ri.z, b0 = ri.x, nOff = ri.y - b0 - 1;
-    // the diagonal block and the forward result, loaded with the blocks
-    T d[6], yv[3];
-    load_diag<T>(U, I, v0, d);
-    yv[0] = z[3 * I]; yv[1] = z[3 * I + 1]; yv[2] = z[3 * I + 2];
-    const T *Uo = sizeof(T) == 4 ? U + 8 * (int64_t)m.nRows : U;   // FP32: the part-major upper blocks
-    T acc[G][3];
-#pragma unroll
-    for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
-    for (int kk = lane; kk < nOff; kk += 16) {
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-            const int k = kk + g * L;
-            if (k < nOff) {
-                BlockX<T> b;
-                load_factor_block<T>(Uo, m.nBlocks, b0 + 1 + k, v0 + 6 + 9 * (int64_t)k, z + 3 * m.colIdx[b0 + 1 + k], b);
-                acc_add(acc[g][0], row3(b.e[0], b.e[1], b.e[2], b.x[0], b.x[1], b.x[2]));
-                acc_add(acc[g][1], row3(b.e[3], b.e[4], b.e[5], b.x[0], b.x[1], b.x[2]));
-                acc_add(acc[g][2], row3(b.e[6], b.e[7], b.e[8], b.x[0], b.x[1], b.x[2]));
-            }
-        }
+// The chunk partial of r.z from the final z of the rows of chunk c in this level: the 192
+// products r_i z_i of the chunk (zero outside the level) summed as k_dot_rz sums them (one
+// wave per 64 entries, then the three wave sums in order).  The chunk's first launch (the
+// highest level it touches: levels finish in decreasing order) stores the sum, later ones
+// add theirs to it.  Every thread of the workgroup calls this.
+__device__ __forceinline__ void sweep_dot(const DevMesh &m, int c, int p0, int count, int lr, bool writer,
+                                          const double (&v)[3], double *__restrict__ partials) {
+    __shared__ double rz[3 * kChunkRows];
+    __shared__ double ws[3];
+    if (writer) { rz[3 * lr] = v[0]; rz[3 * lr + 1] = v[1]; rz[3 * lr + 2] = v[2]; }
+    __syncthreads();
+    if (threadIdx.x < 3 * kChunkRows) {
+        const double s = wave_sum(rz[threadIdx.x]);
+        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
     }
-    T a0, a1, a2;
-    combine16<L>(acc, a0, a1, a2);
-    if (lane != 0) return;
-    const T x2 = (yv[2] - a2) / d[5];
-    const T x1 = (yv[1] - a1 - d[4] * x2) / d[3];
-    const T x0 = (yv[0] - a0 - d[1] * x1 - d[2] * x2) / d[0];
-    z[3 * I] = x0; z[3 * I + 1] = x1; z[3 * I + 2] = x2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int k = 0; k < 3; k++) s += ws[k];
+        const int cEnd = min((c + 1) * kChunkRows, m.nRows);
+        partials[c] = cEnd <= p0 + count ? s : partials[c] + s;
+    }
 }
 
-// One sweep launch of level l with L lanes per row (L = 4, 8 or 16); the optional events are
-// recorded from the dispatch's own begin/end timestamps (lane autotuner).
+// forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the
+// lower copy (contiguous per row I)
+template <typename T, int L, bool DOT>
+__global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
+    DevMesh m, int p0, int count, const T *__restrict__ U, const T *__restrict__ Ul, const T *__restrict__ r,
+    T *__restrict__ y, const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip, int lastLevel,
+    double *__restrict__ partials) {
+    constexpr int kRows = DOT ? kChunkRows : kIcThreads / L, G = 16 / L;
+    if (!sweep_row_live(m, 0, sc, skip, true)) return;
+    const int lane = threadIdx.x % L, lr = threadIdx.x / L;
+    int I, c = 0;
+    bool live;
+    if constexpr (DOT) {
+        const int nwg = sweep_chunks(p0, count), wg = xcd_wg(nwg);
+        if (wg >= nwg) return;                  // grid padding: the whole workgroup
+        c = p0 / kChunkRows + wg;
+        I = c * kChunkRows + lr;
+        live = I >= p0 && I < p0 + count;
+    } else {
+        const int pl = xcd_wg((count + kRows - 1) / kRows) * kRows + lr;
+        if (pl >= count) return;
+        I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
+        live = true;
+    }
+    int4 ri = make_int4(0, 0, 0, 0);
+    if (live) {
+        ri = m.rowFwd[I];
+        if (!sweep_row_live(m, ri.w, sc, skip, false)) {
+            if constexpr (!DOT) return;
+            live = false;
+        }
+    }
+    double v[3] = {0, 0, 0};
+    if (live) {
+        const int kb = ri.x, ke = ri.y;
+        // the diagonal block and the right-hand side, loaded with the blocks
+        T d[6], rhs[3];
+        load_diag<T>(U, I, ri.z, d);
+        rhs[0] = r[3 * I]; rhs[1] = r[3 * I + 1]; rhs[2] = r[3 * I + 2];
+        T acc[G][3];
+#pragma unroll
+        for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
+        for (int kk = kb + lane; kk < ke; kk += 16) {
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                const int k = kk + g * L;
+                if (k < ke) {
+                    BlockX<T> b;
+                    load_factor_block<T>(Ul, m.nT, k, 9 * (int64_t)k, y + 3 * m.tRow[k], b);
+                    acc_add(acc[g][0], row3(b.e[0], b.e[3], b.e[6], b.x[0], b.x[1], b.x[2]));
+                    acc_add(acc[g][1], row3(b.e[1], b.e[4], b.e[7], b.x[0], b.x[1], b.x[2]));
+                    acc_add(acc[g][2], row3(b.e[2], b.e[5], b.e[8], b.x[0], b.x[1], b.x[2]));
+                }
+            }
+        }
+        T a0, a1, a2;
+        combine16<L>(acc, a0, a1, a2);
+        if (lane == 0) {
+            T z0 = (rhs[0] - a0) / d[0];
+            T z1 = (rhs[1] - a1 - d[1] * z0) / d[3];
+            T z2 = (rhs[2] - a2 - d[2] * z0 - d[4] * z1) / d[5];
+            if (lastLevel) diag_bwd(d, z0, z1, z2);   // rows of the last level have no blocks above them
+            y[3 * I] = z0; y[3 * I + 1] = z1; y[3 * I + 2] = z2;
+            v[0] = (double)rhs[0] * (double)z0; v[1] = (double)rhs[1] * (double)z1; v[2] = (double)rhs[2] * (double)z2;
+        }
+    }
+    if constexpr (DOT) sweep_dot(m, c, p0, count, lr, lane == 0, v, partials);
+}
+
+// backward: U z = y  ->  z_I = U_II^-1 (y_I - sum_{J>I} U_IJ z_J); with DOT r is the residual
+// (the r.z terms of the final z)
+template <typename T, int L, bool DOT>
+__global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
+    DevMesh m, int p0, int count, const T *__restrict__ U, T *__restrict__ z, const InstScalars *__restrict__ sc,
+    const uint8_t *__restrict__ skip, const T *__restrict__ r, double *__restrict__ partials) {
+    constexpr int kRows = DOT ? kChunkRows : kIcThreads / L, G = 16 / L;
+    if (!sweep_row_live(m, 0, sc, skip, true)) return;
+    const int lane = threadIdx.x % L, lr = threadIdx.x / L;
+    int I, c = 0;
+    bool live;
+    if constexpr (DOT) {
+        const int nwg = sweep_chunks(p0, count), wg = xcd_wg(nwg);
+        if (wg >= nwg) return;                  // grid padding: the whole workgroup
+        c = p0 / kChunkRows + wg;
+        I = c * kChunkRows + lr;
+        live = I >= p0 && I < p0 + count;
+    } else {
+        const int pl = xcd_wg((count + kRows - 1) / kRows) * kRows + lr;
+        if (pl >= count) return;
+        I = m.levelsContiguous ? p0 + pl : m.levelRows[p0 + pl];
+        live = true;
+    }
+    int4 ri = make_int4(0, 0, 0, 0);
+    if (live) {
+        ri = m.rowBwd[I];
+        if (!sweep_row_live(m, ri.w, sc, skip, false)) {
+            if constexpr (!DOT) return;
+            live = false;
+        }
+    }
+    double v[3] = {0, 0, 0};
+    if (live) {
+        const int v0 = ri.z, b0 = ri.x, nOff = ri.y - b0 - 1;
+        // the diagonal block and the forward result, loaded with the blocks
+        T d[6], yv[3], rv[3] = {0, 0, 0};
+        load_diag<T>(U, I, v0, d);
+        yv[0] = z[3 * I]; yv[1] = z[3 * I + 1]; yv[2] = z[3 * I + 2];
+        if constexpr (DOT) { rv[0] = r[3 * I]; rv[1] = r[3 * I + 1]; rv[2] = r[3 * I + 2]; }
+        const T *Uo = sizeof(T) == 4 ? U + 8 * (int64_t)m.nRows : U;   // FP32: the part-major upper blocks
+        T acc[G][3];
+#pragma unroll
+        for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
+        for (int kk = lane; kk < nOff; kk += 16) {
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                const int k = kk + g * L;
+                if (k < nOff) {
+                    BlockX<T> b;
+                    load_factor_block<T>(Uo, m.nBlocks, b0 + 1 + k, v0 + 6 + 9 * (int64_t)k,
+                                         z + 3 * m.colIdx[b0 + 1 + k], b);
+                    acc_add(acc[g][0], row3(b.e[0], b.e[1], b.e[2], b.x[0], b.x[1], b.x[2]));
+                    acc_add(acc[g][1], row3(b.e[3], b.e[4], b.e[5], b.x[0], b.x[1], b.x[2]));
+                    acc_add(acc[g][2], row3(b.e[6], b.e[7], b.e[8], b.x[0], b.x[1], b.x[2]));
+                }
+            }
+        }
+        T a0, a1, a2;
+        combine16<L>(acc, a0, a1, a2);
+        if (lane == 0) {
+            const T x2 = (yv[2] - a2) / d[5];
+            const T x1 = (yv[1] - a1 - d[4] * x2) / d[3];
+            const T x0 = (yv[0] - a0 - d[1] * x1 - d[2] * x2) / d[0];
+            z[3 * I] = x0; z[3 * I + 1] = x1; z[3 * I + 2] = x2;
+            v[0] = (double)rv[0] * (double)x0; v[1] = (double)rv[1] * (double)x1; v[2] = (double)rv[2] * (double)x2;
+        }
+    }
+    if constexpr (DOT) sweep_dot(m, c, p0, count, lr, lane == 0, v, partials);
+}
+
+// One sweep launch of level l with L lanes per row (L = 4, 8 or 16); with partials set, the
+// launch also forms the r.z chunk partials of the rows whose z it finishes (one instance,
+// consecutive levels).  The optional events are recorded from the dispatch's own begin/end
+// timestamps (lane autotuner).
 template <typename T>
 static void launch_sweep(bool fwd, int L, const DevMesh &m, hipStream_t s, int p0, int cnt, const T *U, const T *Ul,
-                         const T *r, T *z, const InstScalars *sc, const uint8_t *skip, int last,
+                         const T *r, T *z, const InstScalars *sc, const uint8_t *skip, int last, double *partials,
                          hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    auto go = [&](auto kernel, int rowsPerWg, auto... args) {
-        const dim3 grid(xcd_grid((cnt + rowsPerWg - 1) / rowsPerWg)), block(kIcThreads);
+    const bool dot = partials != nullptr;
+    if (dot && (m.nInst != 1 || !m.levelsContiguous)) throw HipError("sweep dot needs one instance, consecutive levels");
+    auto go = [&](auto kernel, int lanes, auto... args) {
+        const int nwg = dot ? (p0 + cnt - 1) / kChunkRows - p0 / kChunkRows + 1
+                            : (cnt + kIcThreads / lanes - 1) / (kIcThreads / lanes);
+        const dim3 grid(xcd_grid(nwg)), block(dot ? kChunkRows * lanes : kIcThreads);
         if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, m, p0, cnt, args...);
         else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, p0, cnt, args...);
     };
-    if (fwd) {
-        if (L == 16) go(k_ic0_fwd<T, 16>, kIcThreads / 16, U, Ul, r, z, sc, skip, last);
-        else if (L == 4) go(k_ic0_fwd<T, 4>, kIcThreads / 4, U, Ul, r, z, sc, skip, last);
-        else go(k_ic0_fwd<T, 8>, kIcThreads / 8, U, Ul, r, z, sc, skip, last);
-    } else {
-        if (L == 16) go(k_ic0_bwd<T, 16>, kIcThreads / 16, U, z, sc, skip);
-        else if (L == 4) go(k_ic0_bwd<T, 4>, kIcThreads / 4, U, z, sc, skip);
-        else go(k_ic0_bwd<T, 8>, kIcThreads / 8, U, z, sc, skip);
-    }
+    auto pick = [&](auto dotTag) {
+        constexpr bool D = decltype(dotTag)::value;
+        if (fwd) {
+            if (L == 16) go(k_ic0_fwd<T, 16, D>, 16, U, Ul, r, z, sc, skip, last, partials);
+            else if (L == 4) go(k_ic0_fwd<T, 4, D>, 4, U, Ul, r, z, sc, skip, last, partials);
+            else go(k_ic0_fwd<T, 8, D>, 8, U, Ul, r, z, sc, skip, last, partials);
+        } else {
+            if (L == 16) go(k_ic0_bwd<T, 16, D>, 16, U, z, sc, skip, r, partials);
+            else if (L == 4) go(k_ic0_bwd<T, 4, D>, 4, U, z, sc, skip, r, partials);
+            else go(k_ic0_bwd<T, 8, D>, 8, U, z, sc, skip, r, partials);
+        }
+    };
+    if (dot) pick(std::true_type{});
+    else pick(std::false_type{});
 }
 
 // Sweep launches for forward levels [f0, nl) (the last one also solving its backward step
 // when fuseLast) and backward levels [0, b1), with each level's lanes per row.
+// With dotPartials (fuseLast and b1 = nl - 1: every level's z finishes in one of these
+// launches), the launches that finish z also form the r.z chunk partials.
 template <typename T>
 static void ic0_levels(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, int f0, bool fuseLast, int b1,
-                       const T *U, const T *Ul, const T *r, T *z, const InstScalars *sc, const uint8_t *skip) {
+                       const T *U, const T *Ul, const T *r, T *z, const InstScalars *sc, const uint8_t *skip,
+                       double *dotPartials = nullptr) {
     const int nl = (int)levelPtr.size() - 1;
     for (int l = f0; l < nl; l++) {
         const int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
         const int last = (fuseLast && l == nl - 1) ? 1 : 0;
-        launch_sweep<T>(true, m.levelLanesFwd[l], m, s, levelPtr[l], cnt, U, Ul, r, z, sc, skip, last);
+        launch_sweep<T>(true, m.levelLanesFwd[l], m, s, levelPtr[l], cnt, U, Ul, r, z, sc, skip, last,
+                        last ? dotPartials : nullptr);
     }
     for (int l = b1 - 1; l >= 0; l--) {
         const int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        launch_sweep<T>(false, m.levelLanesBwd[l], m, s, levelPtr[l], cnt, U, Ul, r, z, sc, skip, 0);
+        launch_sweep<T>(false, m.levelLanesBwd[l], m, s, levelPtr[l], cnt, U, Ul, r, z, sc, skip, 0, dotPartials);
     }
     OD_HIP(hipGetLastError());
 }
@@ -1531,6 +1629,8 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
         (void)nValues; (void)nT;
         float *U = (float *)get(4 * (size_t)ic0_u32_floats(m)), *Ul = (float *)get(4 * (size_t)ic0_ul32_floats(m));
         float *r = (float *)get(4 * n), *z = (float *)get(4 * n);
+        // the launches that finish z form the r.z partials when the solver fuses that dot
+        double *dots = ic0_dot_fused(m, levelPtr) ? (double *)get(8 * (size_t)std::max(1, m.nChunks)) : nullptr;
         OD_HIP(hipEventCreate(&e0));
         OD_HIP(hipEventCreate(&e1));
         const int lanes[3] = {4, 8, 16};
@@ -1542,8 +1642,9 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
                 for (int v = 0; v < 3; v++) {
                     float ms[3];
                     for (int rep = 0; rep < 3; rep++) {
+                        const bool last = dir == 0 && l == nl - 1, finishes = last || (dir == 1 && l < nl - 1);
                         launch_sweep<float>(dir == 0, lanes[v], m, s, levelPtr[l], cnt, U, Ul, r, z, nullptr, nullptr,
-                                            dir == 0 && l == nl - 1 ? 1 : 0, e0, e1);
+                                            last ? 1 : 0, finishes ? dots : nullptr, e0, e1);
                         OD_HIP(hipGetLastError());
                         OD_HIP(hipEventSynchronize(e1));
                         OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
@@ -1618,7 +1719,17 @@ void pcg_update_xr_ic0(const DevMesh &m, hipStream_t s, const std::vector<int> &
     hipLaunchKernelGGL(k_update_xr_ic0<T>, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, p, q, x, r, U, z,
                        nl == 1 ? 1 : 0, m.levelsContiguous && nl >= 1 ? levelPtr[1] : -1, w.partials, w.sc);
     OD_HIP(hipGetLastError());
-    if (nl > 1) ic0_levels<T>(m, s, levelPtr, 1, true, nl - 1, U, Ul, r, z, w.sc, skip);
+    if (nl > 1)
+        ic0_levels<T>(m, s, levelPtr, 1, true, nl - 1, U, Ul, r, z, w.sc, skip,
+                      ic0_dot_fused(m, levelPtr) ? w.partials : nullptr);
+}
+
+bool ic0_dot_fused(const DevMesh &m, const std::vector<int> &levelPtr) {
+    return m.nInst == 1 && m.levelsContiguous && levelPtr.size() >= 3;
+}
+
+void pcg_finalize_iter(const DevMesh &m, hipStream_t s, PcgWork &w) {
+    launch_finalize(m, s, PH_ITER, w, 0.0, 0, nullptr);
 }
 
 // =====================================================================================
