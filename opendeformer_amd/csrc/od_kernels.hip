@@ -320,14 +320,23 @@ __global__ void k_gather_blocks(DevMesh m, const double *__restrict__ C, double 
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= m.nBlocks * 9) return;
     const int b = (int)(gid / 9), c = (int)(gid % 9);
-    const int r = m.blkRow[b];
+    const int r = m.blkRow[b], col = m.colIdx[b], k0 = m.blkCStart[b], k1 = m.blkCStart[b + 1];
+    const int64_t bo = m.blkOff[b];
     if (instActive && !instActive[m.rowInst[r]]) return;
-    const bool diag = m.colIdx[b] == r;
+    const bool diag = col == r;
     if (diag && c >= 6) return;
     const int src = diag ? c_diagSrc[c] : c;
-    const int64_t vi = (int64_t)m.blkOff[b] + c;
+    const int64_t vi = bo + c;
+    // contributions in batches of 8 loads issued together; the sum keeps its order
     double s = 0.0;
-    for (int k = m.blkCStart[b]; k < m.blkCStart[b + 1]; k++) s += C[(size_t)k * 9 + src];
+    for (int k = k0; k < k1; k += 8) {
+        double t[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) t[j] = k + j < k1 ? C[(size_t)(k + j) * 9 + src] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (k + j < k1) s += t[j];
+    }
     double o = cK * s;
     if (M) o += cM * M[vi];
     if (accumulate) o += out[vi];
@@ -352,8 +361,16 @@ __global__ void k_gather_rhs(DevMesh m, const double *__restrict__ C, const doub
     if (i >= 3 * m.nRows) return;
     const int r = i / 3, ax = i % 3;
     if (instActive && !instActive[m.rowInst[r]]) return;
+    const int k0 = m.rowCStart[r], k1 = m.rowCStart[r + 1];
     double s = 0.0;
-    for (int k = m.rowCStart[r]; k < m.rowCStart[r + 1]; k++) s += C[(size_t)k * 3 + ax];
+    for (int k = k0; k < k1; k += 8) {
+        double t[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) t[j] = k + j < k1 ? C[(size_t)(k + j) * 3 + ax] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (k + j < k1) s += t[j];
+    }
     if (mode == 1) { out[i] += s; return; }
     double f = fext ? fext[i] : 0.0;
     if (imp) f += imp[i];
