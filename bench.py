@@ -59,7 +59,7 @@ def traffic_from_profile(workload, args):
     """HBM bytes per SpMV launch measured with FETCH_SIZE/WRITE_SIZE counters (scaled by a
     copy kernel of known size), read from the committed counter summary of this workload
     and configuration; None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"pmc_k_spmv_dot_{workload}_{args.precond}_{args.ordering}_"
+    path = os.path.join(ROOT, "profiles", f"pmc_spmv_{workload}_{args.precond}_{args.ordering}_"
                                           f"{args.precision}.md")
     if not os.path.exists(path):
         return None
