@@ -136,6 +136,24 @@ __device__ __forceinline__ void orthonormalize8(double *mat) {
 }
 
 // Rotation factor R of F = R S (R returned row-major; the reference's `ortho`).
+// Symmetric swap of positions K and J of a full 4x4 matrix (rows, then columns) and of the
+// index record ix, when c holds; K and J are compile-time, so every entry stays in registers.
+template <int K, int J>
+__device__ __forceinline__ void sym_swap4(double (&P)[16], int (&ix)[4], bool c) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const double a = P[K * 4 + t], b = P[J * 4 + t];
+        P[K * 4 + t] = c ? b : a; P[J * 4 + t] = c ? a : b;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const double a = P[t * 4 + K], b = P[t * 4 + J];
+        P[t * 4 + K] = c ? b : a; P[t * 4 + J] = c ? a : b;
+    }
+    const int a = ix[K], b = ix[J];
+    ix[K] = c ? b : a; ix[J] = c ? a : b;
+}
+
 __device__ void polar3(const double *mat, double *ortho) {
     double A[9], B[16];
     double fn = 0;
@@ -170,52 +188,60 @@ __device__ void polar3(const double *mat, double *ortho) {
     const double tau = 1e-4;
     const bool quick = detB <= 1.0 - tau;
     if (quick) {
-        // determinant by LU with partial pivoting
-        int rp0 = 1, rp1 = 2;
+        // determinant by LU with partial pivoting, on the rows as registers (the pivot row
+        // and the two others in the reference's order; the same operations on the same values)
+        const v3 R0 = mk(A[0], A[1], A[2]), R1 = mk(A[3], A[4], A[5]), R2 = mk(A[6], A[7], A[8]);
         int rowIndex = 0;
         double largest = fabs(A[0]);
         if (fabs(A[3]) > largest) { largest = fabs(A[3]); rowIndex = 1; }
         if (fabs(A[6]) > largest) rowIndex = 2;
-        if (rowIndex != 0) { detA = -detA; if (rowIndex == 1) rp0 = 0; else rp1 = 0; }
-        double divisor = A[rowIndex * 3];
+        const v3 Pr = rowIndex == 0 ? R0 : (rowIndex == 1 ? R1 : R2);
+        v3 Q0 = rowIndex == 1 ? R0 : R1;                 // row rp0
+        v3 Q1 = rowIndex == 2 ? R0 : R2;                 // row rp1
+        if (rowIndex != 0) detA = -detA;
+        const double divisor = Pr.x;
         detA *= divisor;
         if (detA != 0.0) {
-            double lo0 = A[rp0 * 3] / divisor, lo1 = A[rp1 * 3] / divisor;
-            A[rp0 * 3 + 1] -= lo0 * A[rowIndex * 3 + 1]; A[rp0 * 3 + 2] -= lo0 * A[rowIndex * 3 + 2];
-            A[rp1 * 3 + 1] -= lo1 * A[rowIndex * 3 + 1]; A[rp1 * 3 + 2] -= lo1 * A[rowIndex * 3 + 2];
-            int row, sub;
-            if (fabs(A[rp0 * 3 + 1]) > fabs(A[rp1 * 3 + 1])) { row = rp0; sub = rp1; }
-            else { detA = -detA; row = rp1; sub = rp0; }
-            double div2 = A[row * 3 + 1];
+            const double lo0 = Q0.x / divisor, lo1 = Q1.x / divisor;
+            Q0.y -= lo0 * Pr.y; Q0.z -= lo0 * Pr.z;
+            Q1.y -= lo1 * Pr.y; Q1.z -= lo1 * Pr.z;
+            v3 row, sub;
+            if (fabs(Q0.y) > fabs(Q1.y)) { row = Q0; sub = Q1; }
+            else { detA = -detA; row = Q1; sub = Q0; }
+            const double div2 = row.y;
             detA *= div2;
-            double lower = A[sub * 3 + 1] / div2;
-            detA *= (A[sub * 3 + 2] - lower * A[row * 3 + 2]);
+            const double lower = sub.y / div2;
+            detA *= (sub.z - lower * row.z);
         }
     } else {
-        // determinant by LU with complete pivoting; omega = second pivot magnitude
+        // determinant by LU with complete pivoting; omega = second pivot magnitude (on a local
+        // copy: its runtime-indexed entries live in private memory, touched only on this path)
+        double Aw[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) Aw[i] = A[i];
         int rp[3] = {0, 1, 2}, cp[3] = {0, 1, 2};
         for (int i = 0; i < 2; i++) {
             double largest = 0.0;
             int ri = i, ci = i;
             for (int j = i; j < 3; j++)
                 for (int k = i; k < 3; k++) {
-                    double en = fabs(A[rp[j] * 3 + cp[k]]);
+                    double en = fabs(Aw[rp[j] * 3 + cp[k]]);
                     if (en > largest) { largest = en; ri = j; ci = k; }
                 }
             if (rp[i] != rp[ri]) { int t = rp[i]; rp[i] = rp[ri]; rp[ri] = t; detA = -detA; }
             if (cp[i] != cp[ci]) { int t = cp[i]; cp[i] = cp[ci]; cp[ci] = t; detA = -detA; }
             int row = rp[i], col = cp[i];
-            double divisor = A[row * 3 + col];
+            double divisor = Aw[row * 3 + col];
             detA *= divisor;
             if (detA == 0.0) break;
             for (int j = i + 1; j < 3; j++) {
                 int sr = rp[j];
-                double lower = A[sr * 3 + col] / divisor;
-                for (int k = i + 1; k < 3; k++) A[sr * 3 + cp[k]] -= lower * A[row * 3 + cp[k]];
+                double lower = Aw[sr * 3 + col] / divisor;
+                for (int k = i + 1; k < 3; k++) Aw[sr * 3 + cp[k]] -= lower * Aw[row * 3 + cp[k]];
             }
         }
-        detA *= A[rp[2] * 3 + cp[2]];
-        omega = fabs(A[rp[1] * 3 + cp[1]]);
+        detA *= Aw[rp[2] * 3 + cp[2]];
+        omega = fabs(Aw[rp[1] * 3 + cp[1]]);
     }
     const bool notpd = detA < 0.0;
     if (notpd) {
@@ -250,76 +276,80 @@ __device__ void polar3(const double *mat, double *ortho) {
     }
     B[0] += dom; B[5] += dom; B[10] += dom; B[15] += dom;
 
-    // LDL^T with symmetric pivoting of the 4x4 B (first two columns)
+    // LDL^T with symmetric pivoting of the 4x4 B (first two columns).  The pivots permute B
+    // itself (rows and columns swapped, ix tracking which original index sits at each
+    // position), so every entry is a register: position (i, j) holds the reference's
+    // B[ix[i]][ix[j]] and sees the same operations in the same order.
     double L[6];
     int ix[4] = {0, 1, 2, 3};
     {
         int di = 0;
         double largest = B[0];
-        for (int j = 1; j < 4; j++) { double dg = B[ix[j] * 5]; if (dg > largest) { largest = dg; di = j; } }
-        if (di != 0) { int t = ix[0]; ix[0] = ix[di]; ix[di] = t; }
-        int td = ix[0];
-        L[0] = B[td * 4 + ix[1]] / largest;
-        L[1] = B[td * 4 + ix[2]] / largest;
-        L[2] = B[td * 4 + ix[3]] / largest;
-        int r = ix[1], c;
+        if (B[5] > largest) { largest = B[5]; di = 1; }
+        if (B[10] > largest) { largest = B[10]; di = 2; }
+        if (B[15] > largest) { largest = B[15]; di = 3; }
+        sym_swap4<0, 1>(B, ix, di == 1);
+        sym_swap4<0, 2>(B, ix, di == 2);
+        sym_swap4<0, 3>(B, ix, di == 3);
+        L[0] = B[1] / largest;
+        L[1] = B[2] / largest;
+        L[2] = B[3] / largest;
         double od;
-        B[r * 4 + r] -= L[0] * B[td * 4 + r];
-        c = ix[2]; od = B[r * 4 + c] - L[0] * B[td * 4 + c]; B[r * 4 + c] = od; B[c * 4 + r] = od;
-        c = ix[3]; od = B[r * 4 + c] - L[0] * B[td * 4 + c]; B[r * 4 + c] = od; B[c * 4 + r] = od;
-        r = ix[2];
-        B[r * 4 + r] -= L[1] * B[td * 4 + r];
-        c = ix[3]; od = B[r * 4 + c] - L[1] * B[td * 4 + c]; B[r * 4 + c] = od; B[c * 4 + r] = od;
-        r = ix[3];
-        B[r * 4 + r] -= L[2] * B[td * 4 + r];
+        B[5] -= L[0] * B[1];
+        od = B[6] - L[0] * B[2]; B[6] = od; B[9] = od;
+        od = B[7] - L[0] * B[3]; B[7] = od; B[13] = od;
+        B[10] -= L[1] * B[2];
+        od = B[11] - L[1] * B[3]; B[11] = od; B[14] = od;
+        B[15] -= L[2] * B[3];
     }
     {
-        int di = 1, td = ix[1];
-        double largest = B[td * 5];
-        for (int j = 2; j < 4; j++) { double dg = B[ix[j] * 5]; if (dg > largest) { largest = dg; di = j; } }
-        if (di != 1) {
-            int t = ix[1]; ix[1] = ix[di]; ix[di] = t;
-            double tl = L[0]; L[0] = L[di - 1]; L[di - 1] = tl;
-        }
-        td = ix[1];
-        L[3] = B[td * 4 + ix[2]] / largest;
-        L[4] = B[td * 4 + ix[3]] / largest;
-        int r = ix[2], c = ix[3];
-        B[r * 4 + r] -= L[3] * B[td * 4 + r];
-        double od = B[r * 4 + c] - L[3] * B[td * 4 + c]; B[r * 4 + c] = od; B[c * 4 + r] = od;
-        r = ix[3];
-        B[r * 4 + r] -= L[4] * B[td * 4 + r];
+        int di = 1;
+        double largest = B[5];
+        if (B[10] > largest) { largest = B[10]; di = 2; }
+        if (B[15] > largest) { largest = B[15]; di = 3; }
+        sym_swap4<1, 2>(B, ix, di == 2);
+        sym_swap4<1, 3>(B, ix, di == 3);
+        const double l0 = L[0], l1 = L[1], l2 = L[2];
+        L[0] = di == 2 ? l1 : (di == 3 ? l2 : l0);
+        L[1] = di == 2 ? l0 : l1;
+        L[2] = di == 3 ? l0 : l2;
+        L[3] = B[6] / largest;
+        L[4] = B[7] / largest;
+        B[10] -= L[3] * B[6];
+        const double od = B[11] - L[3] * B[7]; B[11] = od; B[14] = od;
+        B[15] -= L[4] * B[7];
     }
-    double v[4];
+    // the eigenvector by position (w[k] belongs at original index ix[k])
+    double w[4];
     if (quick) {
-        if (B[ix[2] * 5] < B[ix[3] * 5]) {
-            int t = ix[2]; ix[2] = ix[3]; ix[3] = t;
-            double tl = L[1]; L[1] = L[2]; L[2] = tl;
-            tl = L[3]; L[3] = L[4]; L[4] = tl;
-        }
-        L[5] = B[ix[2] * 4 + ix[3]] / B[ix[2] * 5];
-        v[ix[0]] = L[0] * L[4] + L[1] * L[5] - L[0] * L[5] * L[3] - L[2];
-        v[ix[1]] = L[5] * L[3] - L[4];
-        v[ix[2]] = -L[5];
-        v[ix[3]] = 1.0;
+        const bool sw = B[10] < B[15];
+        sym_swap4<2, 3>(B, ix, sw);
+        const double l1 = L[1], l2 = L[2], l3 = L[3], l4 = L[4];
+        L[1] = sw ? l2 : l1; L[2] = sw ? l1 : l2;
+        L[3] = sw ? l4 : l3; L[4] = sw ? l3 : l4;
+        L[5] = B[11] / B[10];
+        w[0] = L[0] * L[4] + L[1] * L[5] - L[0] * L[5] * L[3] - L[2];
+        w[1] = L[5] * L[3] - L[4];
+        w[2] = -L[5];
+        w[3] = 1.0;
     } else {
-        double b22 = B[ix[2] * 5], b33 = B[ix[3] * 5], b23 = B[ix[2] * 4 + ix[3]];
+        double b22 = B[10], b33 = B[15], b23 = B[11];
         double dd = b22 * b33 - b23 * b23;
         if (dd == 0.0) {
             if (b22 == 0.0 && b33 == 0.0 && b23 == 0.0) {
-                v[ix[0]] = L[0] * L[4] - L[2];
-                v[ix[1]] = -L[4];
-                v[ix[2]] = 0.0; v[ix[3]] = 1.0;
+                w[0] = L[0] * L[4] - L[2];
+                w[1] = -L[4];
+                w[2] = 0.0; w[3] = 1.0;
             } else {
                 double aa = -b23, bb = b22;
                 double nrm = sqrt(aa * aa + bb * bb);
                 aa /= nrm; bb /= nrm;
-                v[ix[3]] = bb; v[ix[2]] = aa;
-                v[ix[1]] = -L[4] * bb - L[3] * aa;
-                v[ix[0]] = -L[2] * bb - L[1] * aa - L[0] * v[ix[1]];
+                w[3] = bb; w[2] = aa;
+                w[1] = -L[4] * bb - L[3] * aa;
+                w[0] = -L[2] * bb - L[1] * aa - L[0] * w[1];
             }
         } else {
-            double ID0 = 1.0 / B[ix[0] * 5], ID1 = 1.0 / B[ix[1] * 5], ID2 = b33, ID3 = -b23, ID4 = b22;
+            double ID0 = 1.0 / B[0], ID1 = 1.0 / B[5], ID2 = b33, ID3 = -b23, ID4 = b22;
             double IL0 = -L[0], IL1 = L[0] * L[3] - L[1], IL2 = L[0] * L[4] - L[2], IL3 = -L[3], IL4 = -L[4];
             if (omega > 6.607e-8) {
                 double w0 = IL2, w1 = IL4, w2 = 0.0, w3 = 1.0;
@@ -338,7 +368,7 @@ __device__ void polar3(const double *mat, double *ortho) {
                     w0 = w0 + IL0 * w1 + IL1 * w2 + IL2 * w3;
                     w1 = w1 + IL3 * w2 + IL4 * w3;
                 }
-                v[ix[0]] = w0; v[ix[1]] = w1; v[ix[2]] = w2; v[ix[3]] = w3;
+                w[0] = w0; w[1] = w1; w[2] = w2; w[3] = w3;
             } else {
                 double vv[8] = {IL1, IL3, 1.0, 0.0, IL2, IL4, 0.0, 1.0};
                 orthonormalize8(vv);
@@ -373,7 +403,7 @@ __device__ void polar3(const double *mat, double *ortho) {
                 T0[4] = vv[4] + L[0] * vv[5] + L[1] * vv[6] + L[2] * vv[7];
                 T0[5] = vv[5] + L[3] * vv[6] + L[4] * vv[7];
                 T0[6] = vv[6]; T0[7] = vv[7];
-                double d00 = B[ix[0] * 5], d11 = B[ix[1] * 5];
+                double d00 = B[0], d11 = B[5];
                 T1[0] = T0[0] * d00; T1[1] = T0[1] * d11;
                 T1[2] = T0[2] * b22 + T0[3] * b23; T1[3] = T0[2] * b23 + T0[3] * b33;
                 T1[4] = T0[4] * d00; T1[5] = T0[5] * d11;
@@ -384,15 +414,18 @@ __device__ void polar3(const double *mat, double *ortho) {
                 if (fabs(BB1) > 1e-15) {
                     double rr = (BB0 - BB2) * 0.5;
                     double w0 = rr - sqrt(rr * rr + BB1 * BB1);
-                    for (int k = 0; k < 4; k++) v[ix[k]] = vv[k] * w0 + vv[4 + k] * BB1;
+                    for (int k = 0; k < 4; k++) w[k] = vv[k] * w0 + vv[4 + k] * BB1;
                 } else if (BB0 < BB2) {
-                    for (int k = 0; k < 4; k++) v[ix[k]] = vv[k];
+                    for (int k = 0; k < 4; k++) w[k] = vv[k];
                 } else {
-                    for (int k = 0; k < 4; k++) v[ix[k]] = vv[4 + k];
+                    for (int k = 0; k < 4; k++) w[k] = vv[4 + k];
                 }
             }
         }
     }
+    double v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = ix[0] == j ? w[0] : (ix[1] == j ? w[1] : (ix[2] == j ? w[2] : w[3]));
     double nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
     v[0] /= nrm; v[1] /= nrm; v[2] /= nrm; v[3] /= nrm;
     double v11 = v[1] * v[1], v22 = v[2] * v[2], v33 = v[3] * v[3];

@@ -162,9 +162,25 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
         const double D0 = lambda + 2.0 * mu, D1 = lambda, D2 = mu;
         double Pst[9];
         if (KIND == 2) plastic_stress(m, e, F, R, lambda, mu, Pst);
-        int p = 0;
+        // corotational force plus the K_e y term of pair (a, b) (tetelement.cpp:622-647):
+        // R k R^T (x_b + y_b) - R k X_b = R k w_b with w_b = R^T (x_b + y_b) - X_b, and for the
+        // transposed pair R k^T w_a; the k w sums are rotated once per node at the end
+        v3 w[4], h[4];
+        if (KIND == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const v3 xy = x[q] + y[q];
+                w[q] = mk(R[0] * xy.x + R[3] * xy.y + R[6] * xy.z, R[1] * xy.x + R[4] * xy.y + R[7] * xy.z,
+                          R[2] * xy.x + R[5] * xy.y + R[8] * xy.z) - X[q];
+                h[q] = mk(0, 0, 0);
+            }
+        }
+        // both pair loops unrolled: every per-node array index is a constant (registers)
+#pragma unroll
         for (int aI = 0; aI < 4; aI++) {
-            for (int bI = aI; bI < 4; bI++, p++) {
+#pragma unroll
+            for (int bI = aI; bI < 4; bI++) {
+                const int p = 4 * aI - aI * (aI - 1) / 2 + (bI - aI);   // pair index (aI, bI), aI <= bI
                 const v3 Ba = dN[aI], Bb = dN[bI];
                 double k[9];
                 k[0] = (D0 * Ba.x * Bb.x + D2 * (Ba.y * Bb.y + Ba.z * Bb.z)) * factor;
@@ -196,23 +212,17 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
                     }
                     continue;
                 }
-                // corotational force (tetelement.cpp:622-647) plus the K_e y term
-                v3 xb = x[bI] + y[bI];
-                g[aI] = g[aI] + mulv(KK, xb) - mulv(QK, X[bI]);
+                h[aI] = h[aI] + mulv(k, w[bI]);
                 if (bI != aI) {
-                    // QKT = R k^T
-                    double QKT[9];
-#pragma unroll
-                    for (int i = 0; i < 3; i++)
-#pragma unroll
-                        for (int j = 0; j < 3; j++)
-                            QKT[i * 3 + j] = R[i * 3 + 0] * k[j * 3 + 0] + R[i * 3 + 1] * k[j * 3 + 1] + R[i * 3 + 2] * k[j * 3 + 2];
-                    v3 xa = x[aI] + y[aI];
-                    v3 kt = mk(KK[0] * xa.x + KK[3] * xa.y + KK[6] * xa.z, KK[1] * xa.x + KK[4] * xa.y + KK[7] * xa.z,
-                               KK[2] * xa.x + KK[5] * xa.y + KK[8] * xa.z);
-                    g[bI] = g[bI] + kt - mulv(QKT, X[aI]);
+                    const v3 wa = w[aI];
+                    h[bI] = h[bI] + mk(k[0] * wa.x + k[3] * wa.y + k[6] * wa.z, k[1] * wa.x + k[4] * wa.y + k[7] * wa.z,
+                                       k[2] * wa.x + k[5] * wa.y + k[8] * wa.z);
                 }
             }
+        }
+        if (KIND == 0 && P.nodeTerms) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) g[q] = mulv(R, h[q]);
         }
         if (KIND == 2 && P.nodeTerms) {
             // nodal forces from the elastic stress, f_i = P dN_i / 6, f_3 = -(f_0 + f_1 + f_2)
