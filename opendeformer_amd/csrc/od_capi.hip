@@ -493,7 +493,10 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     od_solver::IterGraph *g = nullptr;
     if (s->useGraph && !mesh->ctx->prof) {
         g = &s->graph[sizeof(T) == 8 ? 0 : 1];
-        const std::vector<const void *> key = {vals, Uf, Ul, invDiag, x, r, z, p, q, sc, doneFlags, preDone, w.slotVals};
+        // the buffers and the launch shapes baked into the captured launches
+        const auto shape = [](int r, int l) { return (const void *)(intptr_t)(1 + 16 * (l + 1) + r); };
+        const std::vector<const void *> key = {vals, Uf, Ul, invDiag, x, r, z, p, q, sc, doneFlags, preDone, w.slotVals,
+                                               shape(m.spmvR64, m.spmvL64), shape(m.spmvR32, m.spmvL32)};
         if (!g->exec || g->key != key) {
             if (g->exec) { OD_HIP(hipGraphExecDestroy(g->exec)); g->exec = nullptr; }
             hipGraph_t graph = nullptr;

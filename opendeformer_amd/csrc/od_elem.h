@@ -528,9 +528,16 @@ __device__ void neo_diag(const double *F, double thr, double *sig, double *U, do
     case 0: {
         for (int i = 0; i < 3; i++) { double iv = 1.0 / ev[i]; VT[i * 3] *= iv; VT[i * 3 + 1] *= iv; VT[i * 3 + 2] *= iv; }
         if (det3(VT) < 0.0) {
-            int sm = ev[0] < ev[1] ? (ev[0] < ev[2] ? 0 : 2) : (ev[1] < ev[2] ? 1 : 2);
-            VT[sm * 3] = -VT[sm * 3]; VT[sm * 3 + 1] = -VT[sm * 3 + 1]; VT[sm * 3 + 2] = -VT[sm * 3 + 2];
-            ev[sm] = -ev[sm];
+            const int sm = ev[0] < ev[1] ? (ev[0] < ev[2] ? 0 : 2) : (ev[1] < ev[2] ? 1 : 2);
+            // row sm negated, by selects (a runtime index would put VT in private memory)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const bool ng = sm == r;
+                VT[r * 3] = ng ? -VT[r * 3] : VT[r * 3];
+                VT[r * 3 + 1] = ng ? -VT[r * 3 + 1] : VT[r * 3 + 1];
+                VT[r * 3 + 2] = ng ? -VT[r * 3 + 2] : VT[r * 3 + 2];
+                ev[r] = ng ? -ev[r] : ev[r];
+            }
         }
         break;
     }
