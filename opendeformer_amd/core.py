@@ -320,6 +320,12 @@ class _Integrator:
         args = [dptr(f64(x)) if x is not None else None for x in (d, v, a)]
         check(lib().od_integrator_set_state(self.h, *args))
 
+    def device_bytes(self):
+        """Device bytes of the mesh, the integrator and its solver."""
+        b = C.c_longlong(0)
+        check(lib().od_integrator_device_bytes(self.h, C.byref(b)))
+        return b.value
+
     def set_refinement(self, inner_tol, max_outer=12):
         """Mixed precision: FP32 inner tolerance and the cap on FP64 corrections."""
         check(lib().od_integrator_set_refinement(self.h, inner_tol, max_outer))

@@ -390,6 +390,11 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
                 s->U32 = (float *)s->alloc(4 * (size_t)ic0_u32_floats(mesh->dm));
                 s->Ul32 = (float *)s->alloc(4 * (size_t)std::max<int64_t>(1, ic0_ul32_floats(mesh->dm)));
             }
+            // the slot values of the FP32 product, when the mesh runs that form (allocated
+            // here so the solver's device bytes include them; a later change of form
+            // allocates them at the next solve)
+            if (mesh->dm.spmvL32 == 0)
+                s->sv32 = (float *)s->alloc(sizeof(float) * kSlotValFloats * (size_t)std::max(1, mesh->dm.nChunks));
         }
         s->partials = (double *)s->alloc(sizeof(double) * 3 * (size_t)t.chunkRow0.size());
         s->doneIn = (int *)s->alloc(sizeof(int) * t.nInst);
