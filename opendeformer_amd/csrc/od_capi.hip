@@ -867,6 +867,13 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
             }
             m.slotOff = mesh->upload(sOff);
             m.slotCol = mesh->upload(sCol);
+            // the columns again in the slot-value order (slot-major per run position)
+            std::vector<int> vCol(sCol.size());
+            for (size_t q = 0; q < (size_t)nc; q++)
+                for (int r = 0; r < kChunkRows; r++)
+                    for (int sl = 0; sl < kSpmvLanes; sl++)
+                        vCol[q * kSpmvThreads + (size_t)sl * kChunkRows + r] = sCol[q * kSpmvThreads + (size_t)r * kSpmvLanes + sl];
+            m.svCol = mesh->upload(vCol);
             m.runInfo = (const int4 *)mesh->upload(info);
         }
         m.instChunkStart = mesh->upload(t.instChunkStart);

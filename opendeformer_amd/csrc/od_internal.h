@@ -67,6 +67,7 @@ struct DevMesh {
     int spmvR64, spmvR32; // chunks per SpMV workgroup (FP64 / FP32 values), set by the autotuner
     int spmvL64, spmvL32; // lanes per block row of the SpMV (16, 8 or 4), set by the autotuner
     int *slotOff, *slotCol;                      // kSpmvThreads per run position (padding: slotCol = -1)
+    int *svCol;           // slotCol slot-major (position s * 64 + r: slot s of row r), the slot-value order
     int *instChunkStart, *instRowStart;          // nInst+1
     int *tetInst, *rowInst;
     // material (per tet)
@@ -117,8 +118,9 @@ void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *va
 constexpr int kSpmvVariants[3] = {1, 2, 4};        // chunks per workgroup (slot-value form: slots per thread)
 constexpr int kSpmvLaneVariants[3] = {16, 8, 4};   // lanes per block row
 // FP32 slot values: per run position, the kSpmvThreads slot blocks as the product multiplies
-// them (diagonal blocks full, lower blocks transposed, padding zero), part-major: entries
-// 0-3 of every slot (float4 each), then entries 4-7, then entry 8
+// them (diagonal blocks full, lower blocks transposed, padding zero), slot-major (position
+// s * 64 + r: slot s of row r) and part-major: entries 0-3 of every slot (float4 each), then
+// entries 4-7, then entry 8
 constexpr int kSlotValFloats = 9 * kSpmvThreads;
 void launch_fill_slot_values(const DevMesh &m, hipStream_t s, const float *vals, float *slotVals);
 // FP32 IC(0) factor copies, part-major: block b of an array of nb blocks has entries 0-3 at

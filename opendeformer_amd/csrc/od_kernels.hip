@@ -852,16 +852,20 @@ __global__ __launch_bounds__(kChunkRows * LS) void k_spmv_dot(DevMesh m, const T
 
 static inline int spmv_grid(const DevMesh &m, int R) { return 8 * ((m.perRun + R - 1) / R); }
 
-// FP32 slot values from the FP32 upper values: one thread per slot.
+// FP32 slot values from the FP32 upper values: one thread per slot position.  Within a run
+// position the slots are slot-major (position s * 64 + r holds slot s of row r), the order
+// the product's waves read them in.
 __global__ __launch_bounds__(256) void k_fill_slot_values(DevMesh m, const float *__restrict__ vals,
                                                           float *__restrict__ sv) {
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (g >= (int64_t)m.nChunks * kSpmvThreads) return;
     const int64_t p = g / kSpmvThreads;
-    const int s = (int)(g % kSpmvThreads), cf = m.slotCol[g];
+    const int pi = (int)(g % kSpmvThreads), sl = pi / kChunkRows, r = pi % kChunkRows;
+    const int64_t src = p * kSpmvThreads + r * kSpmvLanes + sl;   // the row-major slot record
+    const int cf = m.slotCol[src];
     float e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (cf >= 0) {
-        const float *A = vals + m.slotOff[g];
+        const float *A = vals + m.slotOff[src];
         if (cf & kSlotDiag) {
             const float d[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
 #pragma unroll
@@ -877,9 +881,9 @@ __global__ __launch_bounds__(256) void k_fill_slot_values(DevMesh m, const float
         }
     }
     float *b = sv + p * kSlotValFloats;
-    *(float4 *)(b + 4 * s) = make_float4(e[0], e[1], e[2], e[3]);
-    *(float4 *)(b + 4 * kSpmvThreads + 4 * s) = make_float4(e[4], e[5], e[6], e[7]);
-    b[8 * kSpmvThreads + s] = e[8];
+    *(float4 *)(b + 4 * pi) = make_float4(e[0], e[1], e[2], e[3]);
+    *(float4 *)(b + 4 * kSpmvThreads + 4 * pi) = make_float4(e[4], e[5], e[6], e[7]);
+    b[8 * kSpmvThreads + pi] = e[8];
 }
 
 void launch_fill_slot_values(const DevMesh &m, hipStream_t s, const float *vals, float *slotVals) {
@@ -889,10 +893,13 @@ void launch_fill_slot_values(const DevMesh &m, hipStream_t s, const float *vals,
 }
 
 // The FP32 PCG product on slot values (the fused direction update as in spmv_chunks): one
-// chunk per workgroup, K slots per thread; thread t takes slots t + k (1024 / K), so a wave's
-// value loads read whole consecutive slot runs (1 KB per float4 part) and its 16-lane groups
-// are the block rows.  Per slot the same block product, and per row the same 16-lane tree, as
-// spmv_chunks with 16 lanes: the same bits as every other product shape.
+// chunk per workgroup, K slots per thread.  Slots are slot-major, so the wave w of 64 threads
+// takes slot positions w + (16 / K) j (j < K) of the chunk's 64 rows, one row per lane: each
+// wave load reads consecutive memory (1 KB per float4 part), and each gather reads the
+// same-direction neighbours of 64 consecutive rows.  The row sum is the 16-lane tree of the
+// other product shapes: its first levels (pairs at distance 8, then 4) inside a thread where
+// its slots allow, the rest through LDS in the same pairing, so every shape gives the same
+// bits.
 template <int K>
 __global__ __launch_bounds__(kSpmvThreads / K) void k_spmv_sv(DevMesh m, const float *__restrict__ sv,
                                                               const float *__restrict__ vals,
@@ -900,11 +907,11 @@ __global__ __launch_bounds__(kSpmvThreads / K) void k_spmv_sv(DevMesh m, const f
                                                               double *__restrict__ partials,
                                                               const InstScalars *__restrict__ sc,
                                                               const float *__restrict__ z, float *__restrict__ pNew) {
-    constexpr int T = kSpmvThreads / K;
+    constexpr int T = kSpmvThreads / K, W = kSpmvLanes / K;   // threads, waves (slot groups)
     const bool single = m.nInst == 1;
     if (single && sc[0].done) return;
     const int pos = run_pos(m, (int)(blockIdx.x >> 3)), p = pos >= 0 ? pos : 0;
-    const int t = (int)threadIdx.x, v = t & (kSpmvLanes - 1);
+    const int t = (int)threadIdx.x, r = t % kChunkRows, w = t / kChunkRows;
     const int4 info = m.runInfo[p];
     const int inst = info.w & kRunInstMask;
     // several instances: a chunk of a converged instance loads nothing (its partial is zero)
@@ -912,13 +919,13 @@ __global__ __launch_bounds__(kSpmvThreads / K) void k_spmv_sv(DevMesh m, const f
         if (partials && t == 0) partials[info.x] = 0.0;
         return;
     }
-    const int *__restrict__ sCol = m.slotCol + (size_t)p * kSpmvThreads;
+    const int *__restrict__ sCol = m.svCol + (size_t)p * kSpmvThreads;
     const float *__restrict__ vb = sv + (size_t)p * kSlotValFloats;
     int cf[K];
     BlockX<float> b[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const int s = t + k * T;
+        const int s = t + k * T;                 // slot w + W k of row r
         cf[k] = sCol[s];
         const float4 a0 = *(const float4 *)(vb + 4 * s), a1 = *(const float4 *)(vb + 4 * kSpmvThreads + 4 * s);
         b[k].e[0] = a0.x; b[k].e[1] = a0.y; b[k].e[2] = a0.z; b[k].e[3] = a0.w;
@@ -937,31 +944,55 @@ __global__ __launch_bounds__(kSpmvThreads / K) void k_spmv_sv(DevMesh m, const f
         dir_update<float>(z + c3, beta, b[k]);
     }
     __shared__ double rowDot[kChunkRows];
+    __shared__ float red[W][kChunkRows][3];
+    if (partials && t < kChunkRows) rowDot[t] = 0.0;
     auto finish_row = [&](int lr, float a0, float a1, float a2, const float *px) {
         const int rr = info.y + lr;
         pNew[3 * rr] = px[0]; pNew[3 * rr + 1] = px[1]; pNew[3 * rr + 2] = px[2];
         y[3 * rr] = a0; y[3 * rr + 1] = a1; y[3 * rr + 2] = a2;
         if (partials) rowDot[lr] = row3<double>(px[0], px[1], px[2], a0, a1, a2);
     };
+    if (!lng) {
+        float c[K][3];
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        float a0 = 0, a1 = 0, a2 = 0;
-        block_mv<float>(b[k], false, false, a0, a1, a2);
-        if (cf[k] < 0) a0 = a1 = a2 = 0;
-        sum_lanes<kSpmvLanes>(a0, a1, a2);
-        const int lr = (t + k * T) / kSpmvLanes;
-        if (partials && v == 0) rowDot[lr] = 0.0;
-        // the row's own direction: its diagonal, slot 0 of the row
-        if (use && v == 0 && info.y + lr < info.z) finish_row(lr, a0, a1, a2, b[k].x);
-    }
-    if (act && lng) {
+        for (int k = 0; k < K; k++) {
+            c[k][0] = c[k][1] = c[k][2] = 0;
+            block_mv<float>(b[k], false, false, c[k][0], c[k][1], c[k][2]);
+            if (cf[k] < 0) c[k][0] = c[k][1] = c[k][2] = 0;
+        }
+        // the tree levels inside the thread: slots (w, w + 8) [, (w + 4, w + 12), then the pair]
+        float l[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if constexpr (K == 4) l[q] = (c[0][q] + c[2][q]) + (c[1][q] + c[3][q]);
+            else if constexpr (K == 2) l[q] = c[0][q] + c[1][q];
+            else l[q] = c[0][q];
+        }
+        red[w][r][0] = l[0]; red[w][r][1] = l[1]; red[w][r][2] = l[2];
+        __syncthreads();
+        if (w == 0) {
+            float v[W][3];
+#pragma unroll
+            for (int i = 0; i < W; i++) { v[i][0] = red[i][r][0]; v[i][1] = red[i][r][1]; v[i][2] = red[i][r][2]; }
+#pragma unroll
+            for (int n = W; n > 1; n /= 2)
+#pragma unroll
+                for (int i = 0; i < n / 2; i++) { v[i][0] += v[i + n / 2][0]; v[i][1] += v[i + n / 2][1]; v[i][2] += v[i + n / 2][2]; }
+            // the row's own direction: its diagonal, slot 0, this thread's first slot
+            if (use && info.y + r < info.z) finish_row(r, v[0][0], v[0][1], v[0][2], b[0].x);
+        }
+    } else if (act) {
+        // rows longer than 16 blocks: the row loop, 16 lanes per row (after the zeroing of
+        // rowDot, which other threads did)
+        __syncthreads();
+        const int v16 = t % kSpmvLanes;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int lr = (t + k * T) / kSpmvLanes;
             if (info.y + lr >= info.z) continue;
             float a0 = 0, a1 = 0, a2 = 0, px[3] = {0, 0, 0};
-            row_spmv<float, true, kSpmvLanes>(m, vals, x, info.y + lr, v, a0, a1, a2, z, beta, px);
-            if (v == 0) finish_row(lr, a0, a1, a2, px);
+            row_spmv<float, true, kSpmvLanes>(m, vals, x, info.y + lr, v16, a0, a1, a2, z, beta, px);
+            if (v16 == 0) finish_row(lr, a0, a1, a2, px);
         }
     }
     if (partials) {
