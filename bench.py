@@ -437,9 +437,12 @@ def main():
                              "cold_us": {(f"{p}_L{ls}_R{r}" if ls else f"{p}_slotvalues_K{r}"): round(v * 1e3, 2)
                                          for (p, ls, r), v in tune_table.items()}}
     info = mesh.info_raw()
-    lanes = lambda word: [1 << ((int(word) >> (4 * l)) & 15) for l in range(min(8, mesh.n_levels))]
-    line["ic0_sweeps"] = {"contiguous_levels": bool(info[12]), "lanes_per_row_fwd": lanes(info[13]),
-                          "lanes_per_row_bwd": lanes(info[14])}
+    def forms(word):
+        # per level: lanes per row of the lane-per-row kernel, or slots per thread of the slot kernel
+        codes = [(int(word) >> (4 * l)) & 15 for l in range(min(8, mesh.n_levels))]
+        return [f"slots{1 << (c - 8)}" if c >= 8 else f"lanes{1 << c}" for c in codes]
+    line["ic0_sweeps"] = {"contiguous_levels": bool(info[12]), "form_fwd": forms(info[13]),
+                          "form_bwd": forms(info[14])}
     if args.precision == "mixed":
         line["refinement_sweeps_per_step"] = round(main_r["outer"], 2)
     if fp64_r is not None:

@@ -89,8 +89,9 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X_host, int nTets, con
 int od_mesh_destroy(od_mesh *mesh);
 /* info[16]: ndof, nBlockRows, nBlocks, nValues, nLevels, nInstances, nTets,
  * nNodes, nChunks, nPairContrib, nNodeContrib, device bytes, 1 when the IC(0)
- * levels are consecutive row ranges, then log2 of the sweep lanes per row of
- * the first 8 levels (forward, backward; 4 bits per level), 0 */
+ * levels are consecutive row ranges, then the sweep form of the first 8 levels
+ * (forward, backward; 4 bits per level: log2 of the lanes per row, or 8 +
+ * log2(slots) for the FP32 slot kernel), 0 */
 int od_mesh_info(od_mesh *mesh, long long *info_host);
 /* Canonical layout, identical to BlockedSymSparseMatrix<3,3>'s blockPcol,
  * blockColumnOris, blockRows (core/sparseMatrix.h:627-635), and the 78-entry
@@ -108,7 +109,7 @@ int od_pattern_export(od_mesh *mesh, int *blockPcol_host, int *blockColumnOris_h
  * slots per thread (1, 2 or 4).  Every shape gives the same bits.  The first
  * solver created on a mesh runs the autotuner, which times each shape for FP64
  * and FP32 values with the matrix out of cache and keeps the fastest, and picks
- * the lanes per row (4, 8 or 16) of each IC(0) sweep level the same way.
+ * the form of each IC(0) sweep level (od_mesh_sweep_form) the same way.
  * od_mesh_autotune reruns it (ms_host: 21 median times, [FP64, FP32] x [16, 8,
  * 4 lanes] x [R = 1, 2, 4], then the FP32 slot-value form with 1, 2, 4 slots
  * per thread; chosen_host: R and lanes for FP64, then for FP32; either may be
@@ -116,6 +117,13 @@ int od_pattern_export(od_mesh *mesh, int *blockPcol_host, int *blockColumnOris_h
  * SpMV or the sweeps). */
 int od_mesh_autotune(od_mesh *mesh, double *ms_host, int *chosen_host);
 int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes32);
+/* IC(0) sweep form of every level (no reference counterpart): lanes per row (4, 8
+ * or 16) of the lane-per-row kernels, which the FP64 sweeps always run; slots > 0
+ * (2, 4, 8 or 16) runs the FP32 sweeps on the slot kernel instead, one 64-row
+ * chunk per workgroup with that many of a row's blocks per thread, over the FP32
+ * factor's slot-major copy.  Every form gives the same bits.  Like
+ * od_mesh_spmv_variant it marks the mesh tuned (no tuning on the next solver). */
+int od_mesh_sweep_form(od_mesh *mesh, int lanes, int slots);
 /* Rebinds a mesh (and the solvers and integrators built on it) to another
  * context's stream on the same device, after the work pending on its current
  * stream has finished (no reference counterpart; used by the serving loop). */

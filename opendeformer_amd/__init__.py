@@ -20,5 +20,5 @@ def build():
 from . import capi, mesh, workloads  # noqa: E402,F401
 from .core import (  # noqa: E402,F401
     BackwardEulerIntegrator, CGSolver, Context, DeviceArray, Matrix, Mesh, NewmarkIntegrator, Server,
-    generate_matrix_and_virtual_works, mesh_autotune, mesh_spmv_variant, spmdv, spmdv_f32,
+    generate_matrix_and_virtual_works, mesh_autotune, mesh_spmv_variant, mesh_sweep_form, spmdv, spmdv_f32,
 )

@@ -181,6 +181,12 @@ def mesh_spmv_variant(mesh: "Mesh", v64, v32):
     check(lib().od_mesh_spmv_variant(mesh.h, int(v64[0]), int(v64[1]), int(v32[0]), int(v32[1])))
 
 
+def mesh_sweep_form(mesh: "Mesh", lanes, slots=0):
+    """Sets every IC(0) sweep level's form: lanes per row (4, 8, 16) of the lane-per-row
+    kernels, or with slots (2, 4, 8, 16) the FP32 slot kernel."""
+    check(lib().od_mesh_sweep_form(mesh.h, int(lanes), int(slots)))
+
+
 class Matrix:
     """Values of a BlockedSymSparseMatrix<3,3> in the canonical layout, on the device."""
 

@@ -93,6 +93,7 @@ struct od_mesh {
     double tuneSvMs[3] = {};       // FP32 slot-value form, [1, 2, 4 slots per thread]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
     std::vector<uint8_t> levelLanesFwd, levelLanesBwd;   // DevMesh points at these (host memory)
+    std::vector<int> levelChunkPtr;                      // per level, its range in dm.levelChunks
     template <typename T>
     T *upload(const std::vector<T> &v) {
         if (v.empty()) return nullptr;
@@ -813,6 +814,59 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
             m.rowFwd = (const int4 *)mesh->upload(fw);
             m.rowBwd = (const int4 *)mesh->upload(bw);
         }
+        {
+            // FP32 sweep slots (see DevMesh) and the chunks of each level
+            const int nc = (int)t.chunkRow0.size();
+            std::vector<int> ch(4 * (size_t)std::max(1, nc), 0), baseF(t.nRows), baseB(t.nRows);
+            int64_t oF = 0, oB = 0;
+            for (int c = 0; c < nc; c++) {
+                int wF = 0, wB = 0;
+                for (int I = t.chunkRow0[c]; I < t.chunkRow1[c]; I++) {
+                    wF = std::max(wF, t.tPtr[I + 1] - t.tPtr[I]);
+                    wB = std::max(wB, t.rowPtr[I + 1] - t.rowPtr[I] - 1);
+                }
+                require(oF + (int64_t)kChunkRows * wF < INT32_MAX && oB + (int64_t)kChunkRows * wB < INT32_MAX,
+                        "too many sweep slots", OD_ERR_UNSUPPORTED);
+                ch[4 * (size_t)c] = (int)oF; ch[4 * (size_t)c + 1] = wF;
+                ch[4 * (size_t)c + 2] = (int)oB; ch[4 * (size_t)c + 3] = wB;
+                for (int I = t.chunkRow0[c]; I < t.chunkRow1[c]; I++) {
+                    baseF[I] = (int)oF + (I - t.chunkRow0[c]);
+                    baseB[I] = (int)oB + (I - t.chunkRow0[c]);
+                }
+                oF += (int64_t)kChunkRows * wF;
+                oB += (int64_t)kChunkRows * wB;
+            }
+            m.swNF = oF; m.swNB = oB;
+            std::vector<int> colF((size_t)std::max<int64_t>(1, oF), -1), colB((size_t)std::max<int64_t>(1, oB), -1);
+            for (int I = 0; I < t.nRows; I++) {
+                for (int k = 0; k < t.tPtr[I + 1] - t.tPtr[I]; k++)
+                    colF[(size_t)baseF[I] + (size_t)k * kChunkRows] = t.tRow[t.tPtr[I] + k];
+                for (int q = 0; q < t.rowPtr[I + 1] - t.rowPtr[I] - 1; q++)
+                    colB[(size_t)baseB[I] + (size_t)q * kChunkRows] = t.colIdx[t.rowPtr[I] + 1 + q];
+            }
+            m.swChunk = (const int4 *)mesh->upload(ch);
+            m.swColF = mesh->upload(colF);
+            m.swColB = mesh->upload(colB);
+            m.swBaseF = mesh->upload(baseF);
+            m.swBaseB = mesh->upload(baseB);
+            // per level, the chunks holding its rows, in chunk order
+            std::vector<std::pair<int, int>> lcPairs;   // (level, chunk)
+            for (int c = 0; c < nc; c++) {
+                const size_t first = lcPairs.size();
+                for (int I = t.chunkRow0[c]; I < t.chunkRow1[c]; I++) lcPairs.emplace_back(t.level[I], c);
+                std::sort(lcPairs.begin() + first, lcPairs.end());
+                lcPairs.erase(std::unique(lcPairs.begin() + first, lcPairs.end()), lcPairs.end());
+            }
+            std::stable_sort(lcPairs.begin(), lcPairs.end(),
+                             [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+            mesh->levelChunkPtr.assign(t.nLevels + 1, 0);
+            std::vector<int> lc;
+            for (const auto &pr : lcPairs) { lc.push_back(pr.second); mesh->levelChunkPtr[pr.first + 1]++; }
+            for (int l = 0; l < t.nLevels; l++) mesh->levelChunkPtr[l + 1] += mesh->levelChunkPtr[l];
+            if (lc.empty()) lc.push_back(0);
+            m.levelChunks = mesh->upload(lc);
+            m.levelChunkPtr = mesh->levelChunkPtr.data();
+        }
         // lanes per row before tuning: 16 where a level's rows average more than 8 blocks on
         // the swept side, else 8
         mesh->levelLanesFwd.assign(std::max(1, t.nLevels), 8);
@@ -934,8 +988,9 @@ int od_mesh_info(od_mesh *mesh, long long *info) {
     info[5] = t.nInst; info[6] = t.nTets; info[7] = t.nNodes; info[8] = (long long)t.chunkRow0.size();
     info[9] = t.nPairContrib; info[10] = t.nNodeContrib; info[11] = (long long)mesh->bytes;
     info[12] = t.levelsContiguous ? 1 : 0;
-    // log2 of the lanes per row of the first 8 levels' forward and backward sweeps, 4 bits each
-    auto lg = [](int v) { return v == 16 ? 4 : (v == 4 ? 2 : 3); };
+    // per level of the first 8, 4 bits each (forward, backward): log2 of the sweep lanes per
+    // row, or 8 + log2(K) when the FP32 sweep runs the slot kernel with K slots per thread
+    auto lg = [](int v) { return (v >> 5) ? 8 + (v >> 5) : ((v & 31) == 16 ? 4 : ((v & 31) == 4 ? 2 : 3)); };
     for (int l = 0; l < std::min(8, t.nLevels); l++) {
         info[13] |= (long long)lg(mesh->levelLanesFwd[l]) << (4 * l);
         info[14] |= (long long)lg(mesh->levelLanesBwd[l]) << (4 * l);
@@ -1013,6 +1068,20 @@ int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes
     mesh->dm.spmvL64 = lanes64;
     mesh->dm.spmvR32 = r32;
     mesh->dm.spmvL32 = lanes32;
+    mesh->spmvTuned = true;
+    API_END
+}
+
+int od_mesh_sweep_form(od_mesh *mesh, int lanes, int slots) {
+    API_BEGIN
+    require(mesh != nullptr, "NULL argument");
+    require(lanes == 4 || lanes == 8 || lanes == 16, "lanes per sweep row must be 4, 8 or 16");
+    require(slots == 0 || slots == 2 || slots == 4 || slots == 8 || slots == 16,
+            "slots per sweep thread must be 0 (lane-per-row kernel), 2, 4, 8 or 16");
+    const int sel = slots == 0 ? 0 : (slots == 2 ? 1 : (slots == 4 ? 2 : (slots == 8 ? 3 : 4)));
+    const uint8_t code = (uint8_t)(lanes | (sel << 5));
+    std::fill(mesh->levelLanesFwd.begin(), mesh->levelLanesFwd.end(), code);
+    std::fill(mesh->levelLanesBwd.begin(), mesh->levelLanesBwd.end(), code);
     mesh->spmvTuned = true;
     API_END
 }

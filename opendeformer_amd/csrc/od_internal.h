@@ -36,6 +36,15 @@ constexpr int kRunLong = 1 << 30, kRunInstMask = kRunLong - 1;   // run record: 
 struct DevMesh {
     int nNodes, nTets, nRows, nInst, nChunks, nLevels;
     int64_t nBlocks, nValues, nT;   // nT: entries of the transposed index (lower blocks)
+    // FP32 sweep slots: per chunk, the lower (forward) and the upper (backward) off-diagonal
+    // blocks of its rows slot-major (position k * 64 + r: block k of the chunk's row r), each
+    // chunk as wide as its longest row; swNF / swNB positions in all
+    int64_t swNF, swNB;
+    const int4 *swChunk;  // per chunk: {forward offset, forward width, backward offset, backward width}
+    int *swColF, *swColB; // column of each position (-1: padding)
+    int *swBaseF, *swBaseB;   // per row: its position of block 0 (offset + r)
+    int *levelChunks;     // per level, the chunks holding its rows, in chunk order
+    const int *levelChunkPtr;   // host memory: level l's chunks are levelChunks[ptr[l], ptr[l + 1])
     double *X;            // 3*nNodes
     int *tets;            // 4*nTets
     int *nodeRow;         // nNodes
@@ -123,13 +132,14 @@ constexpr int kSpmvLaneVariants[3] = {16, 8, 4};   // lanes per block row
 // entries 4-7, then entry 8
 constexpr int kSlotValFloats = 9 * kSpmvThreads;
 void launch_fill_slot_values(const DevMesh &m, hipStream_t s, const float *vals, float *slotVals);
-// FP32 IC(0) factor copies, part-major: block b of an array of nb blocks has entries 0-3 at
-// [4 b, 4 b + 4), entries 4-7 at [4 nb + 4 b, ...) and entry 8 at [8 nb + b], so a wave's
-// loads of consecutive blocks read consecutive memory.  U32 holds the diagonal blocks first
-// (8 floats per row: the 6 packed entries, 2 unused), then the upper blocks part-major by
-// block index (nBlocks); Ul32 holds the lower copy part-major by transposed index (nT).
-inline int64_t ic0_u32_floats(const DevMesh &m) { return 8 * (int64_t)m.nRows + 9 * m.nBlocks; }
-inline int64_t ic0_ul32_floats(const DevMesh &m) { return 9 * m.nT; }
+// FP32 IC(0) factor copies, part-major: position b of an array of nb positions has entries
+// 0-3 at [4 b, 4 b + 4), entries 4-7 at [4 nb + 4 b, ...) and entry 8 at [8 nb + b], so a
+// wave's loads of consecutive positions read consecutive memory.  U32 holds the diagonal
+// blocks first (8 floats per row: the 6 packed entries, 2 unused), then the upper blocks at
+// their backward sweep slots (swNB positions); Ul32 holds the lower blocks at their forward
+// sweep slots (swNF positions).  Padding positions stay zero.
+inline int64_t ic0_u32_floats(const DevMesh &m) { return 8 * (int64_t)m.nRows + 9 * m.swNB; }
+inline int64_t ic0_ul32_floats(const DevMesh &m) { return 9 * m.swNF; }
 
 // PCG building blocks (see od_kernels.hip for the exact phase semantics)
 struct PcgWork {

@@ -1358,8 +1358,12 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
                 ((float4 *)(P + 4 * nb))[b] = make_float4((float)X[4], (float)X[5], (float)X[6], (float)X[7]);
                 P[8 * nb + b] = (float)X[8];
             };
-            if (U32) put(U32 + 8 * (int64_t)m.nRows, m.nBlocks, bq);
-            if (Ul32) put(Ul32, m.nT, tq);
+            // U_IJ at slot q of row I's backward list, and as block tq - tPtr[J] of row J's forward list
+            if (U32) put(U32 + 8 * (int64_t)m.nRows, m.swNB, m.swBaseB[I] + (int64_t)q * kChunkRows);
+            if (Ul32) {
+                const int J = m.colIdx[bq];
+                put(Ul32, m.swNF, m.swBaseF[J] + (tq - m.tPtr[J]) * kChunkRows);
+            }
         }
     }
 }
@@ -1411,7 +1415,7 @@ __device__ __forceinline__ bool sweep_row_live(const DevMesh &m, int inst, const
 }
 
 // A factor block and the three entries of x it multiplies: the FP32 copies are part-major
-// (block b of nb), the FP64 factor is in the canonical layout (9 entries from offset aos).
+// (sweep slot b of nb), the FP64 factor is in the canonical layout (9 entries from offset aos).
 template <typename T>
 __device__ __forceinline__ void load_factor_block(const T *__restrict__ P, int64_t nb, int64_t b, int64_t aos,
                                                   const T *__restrict__ xc, BlockX<T> &blk) {
@@ -1458,9 +1462,9 @@ __device__ __forceinline__ void sweep_dot(const DevMesh &m, int c, int p0, int c
     __shared__ double ws[3];
     if (writer) { rz[3 * lr] = v[0]; rz[3 * lr + 1] = v[1]; rz[3 * lr + 2] = v[2]; }
     __syncthreads();
-    if (threadIdx.x < 3 * kChunkRows) {
-        const double s = wave_sum(rz[threadIdx.x]);
-        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    for (int q = threadIdx.x; q < 3 * kChunkRows; q += blockDim.x) {   // whole waves
+        const double s = wave_sum(rz[q]);
+        if ((q & 63) == 0) ws[q >> 6] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1496,8 +1500,10 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
         live = true;
     }
     int4 ri = make_int4(0, 0, 0, 0);
+    int sb = 0;                                 // FP32: the row's sweep slot of block 0
     if (live) {
         ri = m.rowFwd[I];
+        if constexpr (sizeof(T) == 4) sb = m.swBaseF[I];
         if (!sweep_row_live(m, ri.w, sc, skip, false)) {
             if constexpr (!DOT) return;
             live = false;
@@ -1519,7 +1525,8 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
                 const int k = kk + g * L;
                 if (k < ke) {
                     BlockX<T> b;
-                    load_factor_block<T>(Ul, m.nT, k, 9 * (int64_t)k, y + 3 * m.tRow[k], b);
+                    load_factor_block<T>(Ul, m.swNF, sb + (int64_t)(k - kb) * kChunkRows, 9 * (int64_t)k,
+                                         y + 3 * m.tRow[k], b);
                     acc_add(acc[g][0], row3(b.e[0], b.e[3], b.e[6], b.x[0], b.x[1], b.x[2]));
                     acc_add(acc[g][1], row3(b.e[1], b.e[4], b.e[7], b.x[0], b.x[1], b.x[2]));
                     acc_add(acc[g][2], row3(b.e[2], b.e[5], b.e[8], b.x[0], b.x[1], b.x[2]));
@@ -1564,8 +1571,10 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
         live = true;
     }
     int4 ri = make_int4(0, 0, 0, 0);
+    int sb = 0;                                 // FP32: the row's sweep slot of block 0
     if (live) {
         ri = m.rowBwd[I];
+        if constexpr (sizeof(T) == 4) sb = m.swBaseB[I];
         if (!sweep_row_live(m, ri.w, sc, skip, false)) {
             if constexpr (!DOT) return;
             live = false;
@@ -1589,7 +1598,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
                 const int k = kk + g * L;
                 if (k < nOff) {
                     BlockX<T> b;
-                    load_factor_block<T>(Uo, m.nBlocks, b0 + 1 + k, v0 + 6 + 9 * (int64_t)k,
+                    load_factor_block<T>(Uo, m.swNB, sb + (int64_t)k * kChunkRows, v0 + 6 + 9 * (int64_t)k,
                                          z + 3 * m.colIdx[b0 + 1 + k], b);
                     acc_add(acc[g][0], row3(b.e[0], b.e[1], b.e[2], b.x[0], b.x[1], b.x[2]));
                     acc_add(acc[g][1], row3(b.e[3], b.e[4], b.e[5], b.x[0], b.x[1], b.x[2]));
@@ -1610,16 +1619,158 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
     if constexpr (DOT) sweep_dot(m, c, p0, count, lr, lane == 0, v, partials);
 }
 
-// One sweep launch of level l with L lanes per row (L = 4, 8 or 16); with partials set, the
-// launch also forms the r.z chunk partials of the rows whose z it finishes (one instance,
+// FP32 sweeps over the sweep slots: one chunk of 64 rows per workgroup, 16 / K waves.  The
+// thread of row r in wave w keeps the virtual lanes v = w + (16 / K) i (i < K); virtual lane v
+// takes the row's blocks v, v + 16, ..., each wave load one slot position of all 64 rows
+// (1 KB of consecutive float4 per part).  The sums combine as the 16-lane tree combines
+// them (pairs at distance 8, 4, 2, 1: inside the thread first, then across the waves through
+// LDS), so every K gives the bits of the lane-per-row kernels above.  Rows of the chunk
+// outside the level load their slots and discard the sums.
+template <bool FWD, int K, bool DOT>
+__global__ __launch_bounds__(1024 / K) void k_ic0_slots(
+    DevMesh m, int p0, int count, int level, const int *__restrict__ chunks, const float *__restrict__ U,
+    const float *__restrict__ Ul, const float *__restrict__ r, float *__restrict__ y,
+    const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip, int lastLevel, int nwg,
+    double *__restrict__ partials) {
+    constexpr int W = 16 / K;
+    if (!sweep_row_live(m, 0, sc, skip, true)) return;
+    const int wg = xcd_wg(nwg);
+    if (wg >= nwg) return;                      // grid padding: the whole workgroup
+    const int c = chunks[wg], rr = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    const int4 ch = m.swChunk[c];
+    const int row0 = m.chunkRow0[c], I = row0 + rr;
+    if (!sweep_row_live(m, m.chunkInst[c], sc, skip, false)) return;   // the whole workgroup
+    const bool live = I < m.chunkRow1[c] && (m.levelsContiguous ? I >= p0 && I < p0 + count : m.rowLevel[I] == level);
+    const int off = FWD ? ch.x : ch.z, width = FWD ? ch.y : ch.w;
+    const int64_t nb = FWD ? m.swNF : m.swNB;
+    const float *P = FWD ? Ul : U + 8 * (int64_t)m.nRows;
+    const int *__restrict__ colA = FWD ? m.swColF : m.swColB;
+    // the diagonal block and the right-hand side (forward: r; backward: the forward result),
+    // loaded with the blocks
+    float d[6] = {1, 0, 0, 1, 0, 1}, rhs[3] = {0, 0, 0}, rv[3] = {0, 0, 0};
+    if (w == 0 && live) {
+        load_diag<float>(U, I, 0, d);
+        const float *src = FWD ? r : y;
+        rhs[0] = src[3 * I]; rhs[1] = src[3 * I + 1]; rhs[2] = src[3 * I + 2];
+        if constexpr (DOT && !FWD) { rv[0] = r[3 * I]; rv[1] = r[3 * I + 1]; rv[2] = r[3 * I + 2]; }
+    }
+    float acc[K][3];
+#pragma unroll
+    for (int i = 0; i < K; i++) acc[i][0] = acc[i][1] = acc[i][2] = 0.f;
+    for (int kb = 0; kb < width; kb += 16) {
+        float e[K][9], xv[K][3];
+        bool use[K];
+        // every load first: positions past the width repeat the last one (in the chunk's
+        // slots), padding columns read the first entries of y
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            const int k = kb + w + W * i;
+            const int64_t pos = off + (int64_t)min(k, width - 1) * kChunkRows + rr;
+            const int col = colA[pos];
+            use[i] = live && k < width && col >= 0;
+            load_pm(P, nb, pos, e[i]);
+            const float *xc = y + 3 * (int64_t)max(col, 0);
+            xv[i][0] = xc[0]; xv[i][1] = xc[1]; xv[i][2] = xc[2];
+        }
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+            // forward: U_KI^T y_K (columns of the stored block); backward: U_IJ z_J (rows)
+            const float t0 = FWD ? row3(e[i][0], e[i][3], e[i][6], xv[i][0], xv[i][1], xv[i][2])
+                                 : row3(e[i][0], e[i][1], e[i][2], xv[i][0], xv[i][1], xv[i][2]);
+            const float t1 = FWD ? row3(e[i][1], e[i][4], e[i][7], xv[i][0], xv[i][1], xv[i][2])
+                                 : row3(e[i][3], e[i][4], e[i][5], xv[i][0], xv[i][1], xv[i][2]);
+            const float t2 = FWD ? row3(e[i][2], e[i][5], e[i][8], xv[i][0], xv[i][1], xv[i][2])
+                                 : row3(e[i][6], e[i][7], e[i][8], xv[i][0], xv[i][1], xv[i][2]);
+            float a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];
+            acc_add(a0, t0); acc_add(a1, t1); acc_add(a2, t2);
+            if (use[i]) { acc[i][0] = a0; acc[i][1] = a1; acc[i][2] = a2; }
+        }
+    }
+    // the tree inside the thread: virtual distance dv is i-distance dv / W
+#pragma unroll
+    for (int dv = 8; dv >= W; dv >>= 1)
+#pragma unroll
+        for (int i = 0; i < dv / W; i++) {
+            acc_add(acc[i][0], acc[i + dv / W][0]);
+            acc_add(acc[i][1], acc[i + dv / W][1]);
+            acc_add(acc[i][2], acc[i + dv / W][2]);
+        }
+    float a0 = acc[0][0], a1 = acc[0][1], a2 = acc[0][2];
+    if constexpr (W > 1) {
+        __shared__ float red[W][kChunkRows][3];
+        red[w][rr][0] = a0; red[w][rr][1] = a1; red[w][rr][2] = a2;
+        __syncthreads();
+        if (w == 0) {
+            float t[W][3];
+#pragma unroll
+            for (int j = 0; j < W; j++) { t[j][0] = red[j][rr][0]; t[j][1] = red[j][rr][1]; t[j][2] = red[j][rr][2]; }
+#pragma unroll
+            for (int n = W; n > 1; n >>= 1)
+#pragma unroll
+                for (int j = 0; j < n / 2; j++) {
+                    acc_add(t[j][0], t[j + n / 2][0]);
+                    acc_add(t[j][1], t[j + n / 2][1]);
+                    acc_add(t[j][2], t[j + n / 2][2]);
+                }
+            a0 = t[0][0]; a1 = t[0][1]; a2 = t[0][2];
+        }
+    }
+    double v[3] = {0, 0, 0};
+    if (w == 0 && live) {
+        if constexpr (FWD) {
+            float z0 = (rhs[0] - a0) / d[0];
+            float z1 = (rhs[1] - a1 - d[1] * z0) / d[3];
+            float z2 = (rhs[2] - a2 - d[2] * z0 - d[4] * z1) / d[5];
+            if (lastLevel) diag_bwd(d, z0, z1, z2);   // rows of the last level have no blocks above them
+            y[3 * I] = z0; y[3 * I + 1] = z1; y[3 * I + 2] = z2;
+            v[0] = (double)rhs[0] * (double)z0; v[1] = (double)rhs[1] * (double)z1; v[2] = (double)rhs[2] * (double)z2;
+        } else {
+            const float x2 = (rhs[2] - a2) / d[5];
+            const float x1 = (rhs[1] - a1 - d[4] * x2) / d[3];
+            const float x0 = (rhs[0] - a0 - d[1] * x1 - d[2] * x2) / d[0];
+            y[3 * I] = x0; y[3 * I + 1] = x1; y[3 * I + 2] = x2;
+            v[0] = (double)rv[0] * (double)x0; v[1] = (double)rv[1] * (double)x1; v[2] = (double)rv[2] * (double)x2;
+        }
+    }
+    if constexpr (DOT) sweep_dot(m, c, p0, count, rr, w == 0, v, partials);
+}
+
+// One sweep launch of level l in the form `code` names: code & 31 is the lanes per row of
+// the lane-per-row kernels (4, 8 or 16; FP64 always runs these), code >> 5 (1-4) selects the
+// FP32 slot kernel with K = 2, 4, 8 or 16 slots per thread.  With partials set, the launch
+// also forms the r.z chunk partials of the rows whose z it finishes (one instance,
 // consecutive levels).  The optional events are recorded from the dispatch's own begin/end
 // timestamps (lane autotuner).
 template <typename T>
-static void launch_sweep(bool fwd, int L, const DevMesh &m, hipStream_t s, int p0, int cnt, const T *U, const T *Ul,
-                         const T *r, T *z, const InstScalars *sc, const uint8_t *skip, int last, double *partials,
-                         hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, int level, int p0, int cnt, const T *U,
+                         const T *Ul, const T *r, T *z, const InstScalars *sc, const uint8_t *skip, int last,
+                         double *partials, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const bool dot = partials != nullptr;
     if (dot && (m.nInst != 1 || !m.levelsContiguous)) throw HipError("sweep dot needs one instance, consecutive levels");
+    const int L = code & 31, sel = sizeof(T) == 4 ? code >> 5 : 0;
+    if (sel) {
+        const int c0 = m.levelChunkPtr[level], nwg = m.levelChunkPtr[level + 1] - c0;
+        if (nwg <= 0) return;
+        const int K = 1 << sel;
+        auto go = [&](auto kernel) {
+            const dim3 grid(xcd_grid(nwg)), block(1024 / K);
+            if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, m, p0, cnt, level, m.levelChunks + c0,
+                                          (const float *)U, (const float *)Ul, (const float *)r, (float *)z, sc, skip,
+                                          last, nwg, partials);
+            else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, p0, cnt, level, m.levelChunks + c0, (const float *)U,
+                                    (const float *)Ul, (const float *)r, (float *)z, sc, skip, last, nwg, partials);
+        };
+        auto pick = [&](auto fwdTag, auto dotTag) {
+            constexpr bool F = decltype(fwdTag)::value, D = decltype(dotTag)::value;
+            if (K == 2) go(k_ic0_slots<F, 2, D>);
+            else if (K == 4) go(k_ic0_slots<F, 4, D>);
+            else if (K == 8) go(k_ic0_slots<F, 8, D>);
+            else go(k_ic0_slots<F, 16, D>);
+        };
+        if (fwd) { if (dot) pick(std::true_type{}, std::true_type{}); else pick(std::true_type{}, std::false_type{}); }
+        else { if (dot) pick(std::false_type{}, std::true_type{}); else pick(std::false_type{}, std::false_type{}); }
+        return;
+    }
     auto go = [&](auto kernel, int lanes, auto... args) {
         const int nwg = dot ? (p0 + cnt - 1) / kChunkRows - p0 / kChunkRows + 1
                             : (cnt + kIcThreads / lanes - 1) / (kIcThreads / lanes);
@@ -1656,19 +1807,20 @@ static void ic0_levels(const DevMesh &m, hipStream_t s, const std::vector<int> &
         const int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
         const int last = (fuseLast && l == nl - 1) ? 1 : 0;
-        launch_sweep<T>(true, m.levelLanesFwd[l], m, s, levelPtr[l], cnt, U, Ul, r, z, sc, skip, last,
+        launch_sweep<T>(true, m.levelLanesFwd[l], m, s, l, levelPtr[l], cnt, U, Ul, r, z, sc, skip, last,
                         last ? dotPartials : nullptr);
     }
     for (int l = b1 - 1; l >= 0; l--) {
         const int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        launch_sweep<T>(false, m.levelLanesBwd[l], m, s, levelPtr[l], cnt, U, Ul, r, z, sc, skip, 0, dotPartials);
+        launch_sweep<T>(false, m.levelLanesBwd[l], m, s, l, levelPtr[l], cnt, U, Ul, r, z, sc, skip, 0, dotPartials);
     }
     OD_HIP(hipGetLastError());
 }
 
-// Lanes per row for each level's FP32 sweeps: times 4, 8 and 16 lanes (median of three
-// launches each, on zero-filled factors of this mesh's pattern) and keeps the fastest.
+// Launch form of each level's FP32 sweeps: times 4, 8 and 16 lanes per row and the slot
+// kernel with 2, 4, 8 and 16 slots per thread (median of three launches each, on zero-filled
+// factors of this mesh's pattern) and keeps the fastest (launch_sweep's code).
 // The last level's forward launch is timed with its fused backward step, as the solver runs it.
 void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, int64_t nValues, int64_t nT,
                     uint8_t *fwd, uint8_t *bwd) {
@@ -1691,17 +1843,21 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
         double *dots = ic0_dot_fused(m, levelPtr) ? (double *)get(8 * (size_t)std::max(1, m.nChunks)) : nullptr;
         OD_HIP(hipEventCreate(&e0));
         OD_HIP(hipEventCreate(&e1));
-        const int lanes[3] = {4, 8, 16};
+        // lane-per-row kernels (4, 8, 16 lanes), then the slot kernel (K = 2, 4, 8, 16): the
+        // code keeps the fastest lane count and, when a slot shape beats it, that shape
+        const int codes[7] = {4, 8, 16, 1 << 5, 2 << 5, 3 << 5, 4 << 5};
         for (int dir = 0; dir < 2; dir++)
             for (int l = 0; l < nl; l++) {
                 const int cnt = levelPtr[l + 1] - levelPtr[l];
                 if (cnt <= 0) continue;
                 float best = 0.f;
-                for (int v = 0; v < 3; v++) {
+                uint8_t &code = (dir == 0 ? fwd : bwd)[l];
+                for (int v = 0; v < 7; v++) {
+                    const int cv = v < 3 ? codes[v] : ((code & 31) | codes[v]);
                     float ms[3];
                     for (int rep = 0; rep < 3; rep++) {
                         const bool last = dir == 0 && l == nl - 1, finishes = last || (dir == 1 && l < nl - 1);
-                        launch_sweep<float>(dir == 0, lanes[v], m, s, levelPtr[l], cnt, U, Ul, r, z, nullptr, nullptr,
+                        launch_sweep<float>(dir == 0, cv, m, s, l, levelPtr[l], cnt, U, Ul, r, z, nullptr, nullptr,
                                             last ? 1 : 0, finishes ? dots : nullptr, e0, e1);
                         OD_HIP(hipGetLastError());
                         OD_HIP(hipEventSynchronize(e1));
@@ -1710,7 +1866,7 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
                     std::sort(ms, ms + 3);
                     if (v == 0 || ms[1] < best) {
                         best = ms[1];
-                        (dir == 0 ? fwd : bwd)[l] = (uint8_t)lanes[v];
+                        code = (uint8_t)cv;
                     }
                 }
             }
