@@ -500,22 +500,43 @@ struct FinalArgs {
 // looser: the correction then meets the outer test without a second sweep.
 constexpr double kInnerOuterScale2 = 0.25;
 
-// Sums the chunk partials of slots [Q0, Q0 + NQ) over chunks [c0, c1), thread-strided then
-// over the workgroup (fixed order); thread 0 gets t.  Only the slots a phase uses are read.
+// Sums the chunk partials of slots [Q0, Q0 + NQ) over chunks [c0, c1) in the order of 1024
+// threads whatever the workgroup size (a power of two from 64 to 1024): virtual thread v sums
+// chunks v, v + 1024, ... in order, each virtual wave of 64 is summed by wave_sum, and
+// thread 0 adds the 16 wave sums in order.  So an instance finalized by the 256-thread
+// workgroups of a batch gets the bits of the same instance alone (1024 threads), whatever
+// its size.  Only the slots a phase uses are read.
 template <int Q0, int NQ>
 __device__ __forceinline__ void reduce_slots(const DevMesh &m, const double *__restrict__ partials, int c0, int c1,
                                              double *t) {
-    double v[NQ];
+    constexpr int kV = 1024;
+    __shared__ double lds[(kV / 64) * NQ];
+    const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int j = 0; j < kV / (int)blockDim.x; j++) {
+        const int vt = (int)threadIdx.x + j * (int)blockDim.x;
+        double v[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) v[q] = 0.0;
+        for (int q = 0; q < NQ; q++) v[q] = 0.0;
 #pragma unroll 8
-    for (int c = c0 + (int)threadIdx.x; c < c1; c += blockDim.x) {
+        for (int c = c0 + vt; c < c1; c += kV) {
 #pragma unroll
-        for (int q = 0; q < NQ; q++) v[q] += partials[(Q0 + q) * m.nChunks + c];
+            for (int q = 0; q < NQ; q++) v[q] += partials[(Q0 + q) * m.nChunks + c];
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const double sw = wave_sum(v[q]);
+            if (lane == 0) lds[((int)(threadIdx.x >> 6) + j * nw) * NQ + q] = sw;
+        }
     }
-    block_sum<NQ>(v);
+    __syncthreads();
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < NQ; q++) t[Q0 + q] = v[q];
+        for (int q = 0; q < NQ; q++) {
+            double sum = 0;
+            for (int k = 0; k < kV / 64; k++) sum += lds[k * NQ + q];
+            t[Q0 + q] = sum;
+        }
+    }
 }
 
 // Reduces an instance's chunk partials (fixed order) and updates its scalars; called by every

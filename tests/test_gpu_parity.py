@@ -251,6 +251,31 @@ def test_batch_equals_alone_bitwise(ctx):
         assert np.array_equal(di, db[3 * irs[i]:3 * irs[i + 1]])
 
 
+@pytest.mark.parametrize("precision", [capi.OD_FP64, capi.OD_MIXED])
+def test_batch_equals_alone_bitwise_large_instances(ctx, precision):
+    """The same for instances of more than 256 chunks (more than 16,384 block rows), whose
+    partial sums a batch reduces with 256-thread workgroups and a lone instance with 1024."""
+    import opendeformer_amd as od
+    wb = W.batch(n_inst=2, seed=13, tet_lo=100000, tet_hi=120000, steps=2)
+    mesh = od.Mesh(ctx, wb.X, wb.tets, wb.fixed_dofs, wb.inst_node_start, wb.inst_tet_start)
+    mesh.set_material(wb.kind, wb.E, wb.nu, wb.rho)
+    irs, _ = mesh.rows()
+    assert min(np.diff(irs)) > 256 * 64, "every instance must span more than 256 chunks"
+    nm = od.NewmarkIntegrator(mesh, wb.dt, precond=capi.OD_PRE_IC0, precision=precision)
+    nm.set_external_virtual_work(mesh.body_force(wb.body))
+    nm.run_time_steps(wb.steps)
+    db, _, _ = nm.state()
+    for i in range(2):
+        wi = W.subset(wb, i)
+        mi = od.Mesh(ctx, wi.X, wi.tets, wi.fixed_dofs)
+        mi.set_material(wi.kind, wi.E, wi.nu, wi.rho)
+        ni = od.NewmarkIntegrator(mi, wi.dt, precond=capi.OD_PRE_IC0, precision=precision)
+        ni.set_external_virtual_work(mi.body_force(wi.body))
+        ni.run_time_steps(wb.steps)
+        di, _, _ = ni.state()
+        assert np.array_equal(di, db[3 * irs[i]:3 * irs[i + 1]]), i
+
+
 def test_spmv_variants_give_identical_pcg_bits(ctx):
     """The autotuned SpMV launch shapes (1, 2 or 4 chunks per workgroup, 16, 8 or 4 lanes per
     block row; for FP32 also the slot-value form with 1, 2 or 4 slots per thread) compute every
