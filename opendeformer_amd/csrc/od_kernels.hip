@@ -1429,10 +1429,17 @@ __device__ __forceinline__ void diag_bwd(const T *D, T &y0, T &y1, T &y2) {
 // twice per kernel: first (head) with instance 0 for a mesh of one instance, from fixed
 // addresses before any row data is loaded (the whole grid exits together past convergence),
 // then with the row's instance for a mesh of several.
+// Absent scalars or skip flags read as these (zero: live), so the two flags are loaded
+// together, without a branch on either pointer.
+__device__ InstScalars g_liveScalars;
+__device__ uint8_t g_liveSkip[1];
+
 __device__ __forceinline__ bool sweep_row_live(const DevMesh &m, int inst, const InstScalars *__restrict__ sc,
                                                const uint8_t *__restrict__ skip, bool head) {
     if ((m.nInst == 1) != head) return true;
-    return !((sc && sc[inst].done) || (skip && skip[inst]));
+    const int d = (sc ? sc : &g_liveScalars)[sc ? inst : 0].done;
+    const int k = (skip ? skip : g_liveSkip)[skip ? inst : 0];
+    return (d | k) == 0;
 }
 
 // A factor block and the three entries of x it multiplies: the FP32 copies are part-major
@@ -1654,15 +1661,21 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip, int lastLevel, int nwg,
     double *__restrict__ partials) {
     constexpr int W = 16 / K;
-    if (!sweep_row_live(m, 0, sc, skip, true)) return;
     const int wg = xcd_wg(nwg);
     if (wg >= nwg) return;                      // grid padding: the whole workgroup
-    const int c = chunks[wg], rr = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    // one instance: 64-row chunks from row 0, and with the level a row range its chunks are
+    // consecutive (no list load); the chunk's record is loaded with the flags
+    const bool one = m.nInst == 1;
+    const int c = one && m.levelsContiguous ? p0 / kChunkRows + wg : chunks[wg];
+    const int rr = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int4 ch = m.swChunk[c];
-    const int row0 = m.chunkRow0[c], I = row0 + rr;
-    if (!sweep_row_live(m, m.chunkInst[c], sc, skip, false)) return;   // the whole workgroup
-    const bool live = I < m.chunkRow1[c] && (m.levelsContiguous ? I >= p0 && I < p0 + count : m.rowLevel[I] == level);
+    const int row0 = one ? c * kChunkRows : m.chunkRow0[c], I = row0 + rr;
+    const int row1 = one ? min(row0 + kChunkRows, m.nRows) : m.chunkRow1[c];
     const int off = FWD ? ch.x : ch.z, width = FWD ? ch.y : ch.w;
+    // (offset and width, never negative, join the test so that their load is issued with the
+    // flags' instead of after the branch)
+    if (!sweep_row_live(m, one ? 0 : m.chunkInst[c], sc, skip, one) | ((off | width) < 0)) return;
+    const bool live = I < row1 && (m.levelsContiguous ? I >= p0 && I < p0 + count : m.rowLevel[I] == level);
     const int64_t nb = FWD ? m.swNF : m.swNB;
     const float *P = FWD ? Ul : U + 8 * (int64_t)m.nRows;
     const int *__restrict__ colA = FWD ? m.swColF : m.swColB;
