@@ -1511,7 +1511,6 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
     T *__restrict__ y, const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip, int lastLevel,
     double *__restrict__ partials) {
     constexpr int kRows = DOT ? kChunkRows : kIcThreads / L, G = 16 / L;
-    if (!sweep_row_live(m, 0, sc, skip, true)) return;
     const int lane = threadIdx.x % L, lr = threadIdx.x / L;
     int I, c = 0;
     bool live;
@@ -1532,6 +1531,11 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
     if (live) {
         ri = m.rowFwd[I];
         if constexpr (sizeof(T) == 4) sb = m.swBaseF[I];
+    }
+    // the flags of one instance, loaded with the row records (the first lane's record start,
+    // never negative, joins the test so that those loads are issued before the branch)
+    if (!sweep_row_live(m, 0, sc, skip, true) | (__builtin_amdgcn_readfirstlane(ri.x) < 0)) return;
+    if (live) {
         if (!sweep_row_live(m, ri.w, sc, skip, false)) {
             if constexpr (!DOT) return;
             live = false;
@@ -1582,7 +1586,6 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
     DevMesh m, int p0, int count, const T *__restrict__ U, T *__restrict__ z, const InstScalars *__restrict__ sc,
     const uint8_t *__restrict__ skip, const T *__restrict__ r, double *__restrict__ partials) {
     constexpr int kRows = DOT ? kChunkRows : kIcThreads / L, G = 16 / L;
-    if (!sweep_row_live(m, 0, sc, skip, true)) return;
     const int lane = threadIdx.x % L, lr = threadIdx.x / L;
     int I, c = 0;
     bool live;
@@ -1603,6 +1606,11 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
     if (live) {
         ri = m.rowBwd[I];
         if constexpr (sizeof(T) == 4) sb = m.swBaseB[I];
+    }
+    // the flags of one instance, loaded with the row records (the first lane's record start,
+    // never negative, joins the test so that those loads are issued before the branch)
+    if (!sweep_row_live(m, 0, sc, skip, true) | (__builtin_amdgcn_readfirstlane(ri.x) < 0)) return;
+    if (live) {
         if (!sweep_row_live(m, ri.w, sc, skip, false)) {
             if constexpr (!DOT) return;
             live = false;
