@@ -336,10 +336,13 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the run, launch a streaming update of known size three times (the reference "
                          "that scales HBM byte counters)")
+    ap.add_argument("--fp32-spmv", choices=["auto", "slots1", "slots2", "slots4"], default="auto",
+                    help="FP32 product shape: the autotuner's choice, or the slot-value form with 1, 2 or 4 slots "
+                         "per thread (for counter runs, whose profiling skews the autotuner's timings)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(it runs on one GPU; the multi-GPU design is documented, not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the sharded multi-GPU path is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import opendeformer_amd as od
@@ -353,6 +356,9 @@ def main():
     f = mesh.body_force(w.body)
     pre = {"jacobi": capi.OD_PRE_JACOBI, "ic0": capi.OD_PRE_IC0}[args.precond]
     tune_table, v64, v32 = od.mesh_autotune(mesh)
+    if args.fp32_spmv != "auto":
+        v32 = (int(args.fp32_spmv[len("slots"):]), 0)
+        od.mesh_spmv_variant(mesh, v64, v32)      # the sweep forms stay as tuned
 
     def run_form(precision):
         """Warm-up, then the timed steps; returns the measurements and the final displacement."""
