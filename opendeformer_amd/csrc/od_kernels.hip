@@ -873,43 +873,58 @@ __global__ __launch_bounds__(kChunkRows * LS) void k_spmv_dot(DevMesh m, const T
 
 static inline int spmv_grid(const DevMesh &m, int R) { return 8 * ((m.perRun + R - 1) / R); }
 
-// FP32 slot values from the FP32 upper values: one thread per slot position.  Within a run
-// position the slots are slot-major (position s * 64 + r holds slot s of row r), the order
-// the product's waves read them in.
+// FP32 slot values from the FP32 upper values.  Within a run position the slots are
+// slot-major (position s * 64 + r holds slot s of row r), the order the product's waves read
+// them in.
 __global__ __launch_bounds__(256) void k_fill_slot_values(DevMesh m, const float *__restrict__ vals,
                                                           float *__restrict__ sv) {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (g >= (int64_t)m.nChunks * kSpmvThreads) return;
-    const int64_t p = g / kSpmvThreads;
-    const int pi = (int)(g % kSpmvThreads), sl = pi / kChunkRows, r = pi % kChunkRows;
-    const int64_t src = p * kSpmvThreads + r * kSpmvLanes + sl;   // the row-major slot record
-    const int cf = m.slotCol[src];
-    float e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (cf >= 0) {
-        const float *A = vals + m.slotOff[src];
-        if (cf & kSlotDiag) {
-            const float d[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
+    // one run position per workgroup: the records are read in their row-major order (a row's
+    // 16 slots by 16 consecutive threads: coalesced records, contiguous upper blocks), the
+    // blocks go to LDS at their slot-major positions (row stride 65: no bank conflicts), and
+    // the slot-major parts are written in position order
+    constexpr int kStride = kChunkRows + 1;
+    __shared__ float lds[9][kSpmvLanes * kStride];
+    const int64_t p = blockIdx.x;
+    const int t = (int)threadIdx.x;
 #pragma unroll
-            for (int q = 0; q < 9; q++) e[q] = d[q];
-        } else if (cf & kSlotTrans) {
+    for (int j = 0; j < kSpmvThreads / 256; j++) {
+        const int rec = t + 256 * j, r = rec / kSpmvLanes, sl = rec % kSpmvLanes;
+        const int64_t src = p * kSpmvThreads + rec;
+        const int cf = m.slotCol[src];
+        float e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (cf >= 0) {
+            const float *A = vals + m.slotOff[src];
+            if (cf & kSlotDiag) {
+                const float d[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
 #pragma unroll
-            for (int i = 0; i < 3; i++)
+                for (int q = 0; q < 9; q++) e[q] = d[q];
+            } else if (cf & kSlotTrans) {
 #pragma unroll
-                for (int j = 0; j < 3; j++) e[3 * i + j] = A[3 * j + i];
-        } else {
+                for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int q = 0; q < 9; q++) e[q] = A[q];
+                    for (int k = 0; k < 3; k++) e[3 * i + k] = A[3 * k + i];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; q++) e[q] = A[q];
+            }
         }
+#pragma unroll
+        for (int q = 0; q < 9; q++) lds[q][sl * kStride + r] = e[q];
     }
+    __syncthreads();
     float *b = sv + p * kSlotValFloats;
-    *(float4 *)(b + 4 * pi) = make_float4(e[0], e[1], e[2], e[3]);
-    *(float4 *)(b + 4 * kSpmvThreads + 4 * pi) = make_float4(e[4], e[5], e[6], e[7]);
-    b[8 * kSpmvThreads + pi] = e[8];
+#pragma unroll
+    for (int j = 0; j < kSpmvThreads / 256; j++) {
+        const int pi = t + 256 * j, l = (pi / kChunkRows) * kStride + pi % kChunkRows;
+        *(float4 *)(b + 4 * pi) = make_float4(lds[0][l], lds[1][l], lds[2][l], lds[3][l]);
+        *(float4 *)(b + 4 * kSpmvThreads + 4 * pi) = make_float4(lds[4][l], lds[5][l], lds[6][l], lds[7][l]);
+        b[8 * kSpmvThreads + pi] = lds[8][l];
+    }
 }
 
 void launch_fill_slot_values(const DevMesh &m, hipStream_t s, const float *vals, float *slotVals) {
-    const int64_t n = (int64_t)m.nChunks * kSpmvThreads;
-    hipLaunchKernelGGL(k_fill_slot_values, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, vals, slotVals);
+    if (m.nChunks <= 0) return;
+    hipLaunchKernelGGL(k_fill_slot_values, dim3((unsigned)m.nChunks), dim3(256), 0, s, m, vals, slotVals);
     OD_HIP(hipGetLastError());
 }
 
