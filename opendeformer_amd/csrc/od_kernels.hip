@@ -1484,8 +1484,6 @@ __device__ __forceinline__ void load_diag(const T *__restrict__ U, int I, int v0
     }
 }
 
-// forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the
-// lower copy (contiguous per row I)
 // Rows of a sweep workgroup.  Plain launches: kIcThreads / L consecutive rows of the level
 // (level-relative).  Launches that also form the r.z dot (DOT; one instance, levels as
 // consecutive row ranges): the rows of one 64-row chunk that lie in the level, so the
@@ -1519,7 +1517,7 @@ __device__ __forceinline__ void sweep_dot(const DevMesh &m, int c, int p0, int c
 }
 
 // forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the
-// lower copy (contiguous per row I)
+// lower copy (FP64: contiguous per row I; FP32: row I's forward sweep slots)
 template <typename T, int L, bool DOT>
 __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
     DevMesh m, int p0, int count, const T *__restrict__ U, const T *__restrict__ Ul, const T *__restrict__ r,
