@@ -208,7 +208,10 @@ struct od_solver {
     }
 };
 
+// Even: the FP32 iterations alternate between two direction buffers by iteration parity,
+// and a replayed block must start on the parity it was captured with.
 constexpr int kGraphIters = 4;
+static_assert(kGraphIters % 2 == 0, "graph blocks hold an even number of iterations");
 
 struct od_integrator {
     od_mesh *mesh = nullptr;
