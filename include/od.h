@@ -198,13 +198,18 @@ int od_solver_destroy(od_solver *s);
  * solve stops at innerTol relative to its own right-hand side (default 1e-4)
  * or once its residual is below half the outer tolerance tol*|b|, whichever
  * comes first; at most maxOuter inner solves with FP64 residual corrections
- * are made (default 12). */
+ * are made (default 12).  The inner iterations of all sweeps together stop at
+ * the solve's maxIter, as the plain solve does. */
 int od_solver_set_refinement(od_solver *s, double innerTol, int maxOuter);
 /* Graph mode (no reference counterpart): PCG iterations are captured once as a
  * HIP graph of 4 iterations and replayed (iterations after convergence exit
  * at once on the device; the iteration cap is checked on the device).
  * Ignored while od_ctx_profile is on. */
 int od_solver_set_graph(od_solver *s, int enable);
+/* info[4] (no reference counterpart): graph captures so far (one per new set of buffers or
+ * launch forms), device bytes held by the solver, refinement sweeps and the largest
+ * per-instance iteration count of the last solve */
+int od_solver_info(od_solver *s, long long *info_host);
 int od_pcg_solve(od_solver *s, od_matrix *A, const double *b_dev, double *x_dev, double tol, int maxIter,
                  int *iters_host, double *relres_host);
 /* Preconditioner::Preprocess + solvePreconditionerSystem once: z = P(A)^-1 r */
@@ -228,6 +233,8 @@ int od_integrator_destroy(od_integrator *it);
 int od_integrator_set_refinement(od_integrator *it, double innerTol, int maxOuter);
 /* od_solver_set_graph for the integrator's solver */
 int od_integrator_set_graph(od_integrator *it, int enable);
+/* the integrator's solver (owned by the integrator; not to be destroyed by the caller) */
+int od_integrator_solver(od_integrator *it, od_solver **s);
 /* device bytes held by the integrator, its solver and its mesh */
 int od_integrator_device_bytes(od_integrator *it, long long *bytes_host);
 /* Intergrator::setExternalVirtualWork (core/intergrator.h:18) */

@@ -36,6 +36,7 @@ EXPORTS = [
     "od_server_create", "od_server_destroy", "od_server_add_object", "od_server_run", "od_server_object_state",
     "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain", "od_plane_damping",
     "od_integrator_add_plane_damping", "od_integrator_clear_damping", "od_integrator_damping", "od_relabel_gps",
+    "od_solver_info", "od_integrator_solver",
 ]
 
 
@@ -89,6 +90,7 @@ def lib():
             "od_relabel_gps": [i, i, i, ip, ip],
             "od_spmv_timed": [vp, vp, vp, i, i, dp], "od_integrator_matrix": [vp, pvp],
             "od_ctx_profile": [vp, i], "od_ctx_profile_read": [vp, dp, C.POINTER(ll)],
+            "od_solver_info": [vp, C.POINTER(ll)], "od_integrator_solver": [vp, pvp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

@@ -282,10 +282,22 @@ class CGSolver:
     def set_graph(self, enable=True):
         check(lib().od_solver_set_graph(self.h, 1 if enable else 0))
 
+    def info(self):
+        """Graph captures, device bytes, refinement sweeps and largest iteration count of the
+        last solve."""
+        return _solver_info(self.h)
+
     def ic0_factor(self, A: Matrix):
         L = np.zeros(self.mesh.n_values)
         check(lib().od_ic0_factor(self.h, A.h, dptr(L)))
         return L
+
+
+def _solver_info(h):
+    info = np.zeros(4, np.int64)
+    check(lib().od_solver_info(h, info.ctypes.data_as(C.POINTER(C.c_longlong))))
+    return {"graph_captures": int(info[0]), "device_bytes": int(info[1]), "outer": int(info[2]),
+            "iters_max": int(info[3])}
 
 
 class _Integrator:
@@ -339,6 +351,12 @@ class _Integrator:
     def set_graph(self, enable=True):
         """Replay PCG iterations from a captured HIP graph."""
         check(lib().od_integrator_set_graph(self.h, 1 if enable else 0))
+
+    def solver_info(self):
+        """od_solver_info of the integrator's solver (graph captures among others)."""
+        s = C.c_void_p()
+        check(lib().od_integrator_solver(self.h, C.byref(s)))
+        return _solver_info(s)
 
     def stats(self):
         it = np.zeros(self.mesh.n_inst, np.int32); rr = np.zeros(self.mesh.n_inst)

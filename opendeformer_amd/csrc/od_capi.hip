@@ -89,6 +89,9 @@ struct od_mesh {
     size_t bytes = 0;
     bool hasMaterial = false;
     bool spmvTuned = false;                      // SpMV variant chosen (autotuner or caller)
+    // generation of the launch forms (SpMV shapes, sweep form per level): bumped by every
+    // change, part of the key of the captured PCG graphs, which bake the forms in
+    uint64_t formGen = 0;
     double tuneMs[2][3][3] = {};   // [FP64, FP32][16, 8, 4 lanes per row][R = 1, 2, 4]: cold launch time
     double tuneSvMs[3] = {};       // FP32 slot-value form, [1, 2, 4 slots per thread]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
@@ -201,6 +204,7 @@ struct od_solver {
         hipGraphExec_t exec = nullptr;
         std::vector<const void *> key;   // the pointers baked into the captured launches
     } graph[2];                          // [FP64, FP32]
+    long long captures = 0;              // graph captures so far (each capture instantiates a graph)
     ~od_solver() {
         for (auto &g : graph)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -212,6 +216,12 @@ struct od_solver {
 // and a replayed block must start on the parity it was captured with.
 constexpr int kGraphIters = 4;
 static_assert(kGraphIters % 2 == 0, "graph blocks hold an even number of iterations");
+// Graph mode captures a block only while it holds at most this many kernel launches.  An
+// IC(0) iteration launches about 2 x (dependency levels) sweep kernels; a numbering with
+// thousands of levels (a bandwidth-reducing relabelling of a large mesh) would make a block
+// of tens of thousands of nodes, whose capture and instantiation cost more than the launch
+// overhead the replay saves on launches that each run a level's worth of rows.
+constexpr int kGraphMaxLaunches = 4096;
 
 struct od_integrator {
     od_mesh *mesh = nullptr;
@@ -360,6 +370,7 @@ static void autotune_spmv(od_mesh *mesh) {
                        mesh->levelLanesBwd.data());
     }
     mesh->spmvTuned = true;
+    mesh->formGen++;
 }
 
 static od_solver *make_solver(od_mesh *mesh, int precond, double param, int precision) {
@@ -383,7 +394,9 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
         const size_t nl = 9 * (size_t)t.tRow.size() + kValuePad;
         if (precond == OD_PRE_IC0) {
             s->U = (double *)s->alloc(8 * (nv + kValuePad));
-            s->Ul = (double *)s->alloc(8 * nl);
+            // the FP64 lower copy serves only the FP64 sweeps: mixed solvers allocate it on
+            // the first od_precond_apply
+            if (precision == OD_FP64) s->Ul = (double *)s->alloc(8 * nl);
         }
         if (precision == OD_MIXED) {
             s->r32 = (float *)s->alloc(4 * n); s->ri32 = (float *)s->alloc(4 * n); s->z32 = (float *)s->alloc(4 * n);
@@ -500,12 +513,13 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     // Graph mode (not while timing launches): a block of kGraphIters iterations is captured
     // once per set of buffers and replayed; iterations past convergence exit at once.
     od_solver::IterGraph *g = nullptr;
-    if (s->useGraph && !mesh->ctx->prof) {
+    const int launchesPerIter = ic ? 2 * m.nLevels + 4 : 5;
+    if (s->useGraph && !mesh->ctx->prof && kGraphIters * launchesPerIter <= kGraphMaxLaunches) {
         g = &s->graph[sizeof(T) == 8 ? 0 : 1];
-        // the buffers and the launch shapes baked into the captured launches
-        const auto shape = [](int r, int l) { return (const void *)(intptr_t)(1 + 16 * (l + 1) + r); };
+        // the buffers and the launch forms (SpMV shapes, sweep form per level) baked into the
+        // captured launches
         const std::vector<const void *> key = {vals, Uf, Ul, invDiag, x, r, z, p, q, sc, doneFlags, preDone, w.slotVals,
-                                               shape(m.spmvR64, m.spmvL64), shape(m.spmvR32, m.spmvL32)};
+                                               (const void *)(uintptr_t)mesh->formGen};
         if (!g->exec || g->key != key) {
             if (g->exec) { OD_HIP(hipGraphExecDestroy(g->exec)); g->exec = nullptr; }
             hipGraph_t graph = nullptr;
@@ -522,6 +536,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
             (void)hipGraphDestroy(graph);
             OD_HIP(e);
             g->key = key;
+            s->captures++;
         }
     }
     // A skipped iteration (all kernels launched, each exiting at once) costs about as much as
@@ -590,14 +605,20 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     PcgWork wo{s->partials, s->scOut, s->doneOut};
     launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr);
     launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 1);
+    // The inner iterations of all refinement sweeps count against maxIt together, as the
+    // reference's maxIteration caps the whole solve (cgSolver.cpp:31): a sweep gets the
+    // iterations the previous ones left.
     int outer = 0;
     for (; outer < s->maxOuter; outer++) {
+        const int used = *std::max_element(s->iters.begin(), s->iters.end());
+        if (used >= maxIt && outer > 0) break;
         if (count_not_done(mesh->ctx, s->doneOut, t.nInst) == 0) break;
         hipLaunchKernelGGL(k_done_to_mask, dim3((t.nInst + 255) / 256), dim3(256), 0, st, t.nInst, s->scOut, pre, s->mask);
         OD_HIP(hipGetLastError());
         OD_HIP(hipMemsetAsync(s->x32, 0, sizeof(float) * t.ndof, st));
         run_pcg<float>(s, A->vals32, s->U32, s->Ul32, s->invDiag32, s->r32, s->x32, s->ri32, s->z32, s->p32,
-                       s->q32, s->innerTol, maxIt, s->mask, s->scIn, s->doneIn, true, s->sweepIters[outer], s->scOut);
+                       s->q32, s->innerTol, maxIt - used, s->mask, s->scIn, s->doneIn, true, s->sweepIters[outer],
+                       s->scOut);
         {
             const int before = *std::max_element(s->iters.begin(), s->iters.end());
             collect_stats(s, s->scIn, true);
@@ -1072,6 +1093,7 @@ int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes
     mesh->dm.spmvR32 = r32;
     mesh->dm.spmvL32 = lanes32;
     mesh->spmvTuned = true;
+    mesh->formGen++;
     API_END
 }
 
@@ -1086,6 +1108,7 @@ int od_mesh_sweep_form(od_mesh *mesh, int lanes, int slots) {
     std::fill(mesh->levelLanesFwd.begin(), mesh->levelLanesFwd.end(), code);
     std::fill(mesh->levelLanesBwd.begin(), mesh->levelLanesBwd.end(), code);
     mesh->spmvTuned = true;
+    mesh->formGen++;
     API_END
 }
 
@@ -1326,6 +1349,16 @@ int od_solver_set_graph(od_solver *s, int enable) {
     API_END
 }
 
+int od_solver_info(od_solver *s, long long *info) {
+    API_BEGIN
+    require(s && info, "NULL argument");
+    info[0] = s->captures;
+    info[1] = (long long)s->bytes;
+    info[2] = s->outer;
+    info[3] = s->iters.empty() ? 0 : *std::max_element(s->iters.begin(), s->iters.end());
+    API_END
+}
+
 int od_solver_destroy(od_solver *s) {
     API_BEGIN
     if (s) { (void)hipStreamSynchronize(s->mesh->ctx->stream); delete s; }
@@ -1357,6 +1390,8 @@ int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
     API_BEGIN
     require(s && A && r && z && A->mesh == s->mesh, "bad arguments");
     od_mesh *mesh = s->mesh;
+    if (s->precond == OD_PRE_IC0 && !s->Ul)
+        s->Ul = (double *)s->alloc(8 * (9 * (size_t)mesh->topo.tRow.size() + kValuePad));
     build_precond(s, A, s->zeroMask);
     PcgWork w{s->partials, s->scIn, s->doneIn};
     if (s->precond == OD_PRE_IC0)
@@ -1447,6 +1482,13 @@ int od_integrator_device_bytes(od_integrator *it, long long *bytes) {
     API_END
 }
 
+int od_integrator_solver(od_integrator *it, od_solver **s) {
+    API_BEGIN
+    require(it && s, "NULL argument");
+    *s = it->solver;
+    API_END
+}
+
 int od_integrator_set_graph(od_integrator *it, int enable) {
     API_BEGIN
     require(it != nullptr, "NULL argument");
@@ -1522,11 +1564,14 @@ static void integrator_step(od_integrator *it) {
         const double cK = gamma * h * aK + beta * h * h, cM = 1.0 + gamma * h * aM;
         launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act);
         launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, 1.0, it->rhs, 0, act);
-        OD_HIP(hipMemcpyAsync(it->aNew, it->a, sizeof(double) * mesh->topo.ndof, hipMemcpyDeviceToDevice, st));
+        // the solve runs in place on aNew, warm-started from a; a and aNew keep their
+        // addresses, so the buffers baked into captured PCG graphs stay valid from step to step
+        const size_t vb = sizeof(double) * mesh->topo.ndof;
+        OD_HIP(hipMemcpyAsync(it->aNew, it->a, vb, hipMemcpyDeviceToDevice, st));
         solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);
         launch_newmark_correct(m, st, it->d, it->v, it->a, it->aNew, h, beta, gamma, act);
-        // inactive instances: aNew is the copy of a made before the solve, so the swap keeps their state
-        std::swap(it->a, it->aNew);
+        // inactive instances: their aNew still equals a, so the copy keeps their state
+        OD_HIP(hipMemcpyAsync(it->a, it->aNew, vb, hipMemcpyDeviceToDevice, st));
     } else {
         // backward Euler (nonlinearImplicitBackwardEuler.h:82-101); d is the mesh displacement state
         const double aM = it->alphaM, aK = it->alphaK;

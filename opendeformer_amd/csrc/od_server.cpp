@@ -85,7 +85,11 @@ struct od_server {
     std::vector<Request> reqs;      // arrival order
     size_t nextArrival = 0;         // first request not yet arrived
     std::vector<int> waiting;       // arrived, not started (arrival order)
-    long long resident = 0, peak = 0, evictions = 0, loads = 0;
+    // resident (the admission account) counts the bytes of resident objects, the
+    // reservations of objects being loaded and the objects being evicted until their unload
+    // has freed them; allocated counts only what is on the device (loaded objects, victims
+    // not yet freed), and its maximum is the reported peak
+    long long resident = 0, allocated = 0, peak = 0, evictions = 0, loads = 0;
     int nDone = 0;
     uint64_t clock = 0;
     std::chrono::steady_clock::time_point t0;
@@ -120,23 +124,21 @@ void unload(od_server *S, Object &o) {
     (void)S;
 }
 
+// Loads o (its bytes reserved in S->resident) with the launch shapes od_server_run tuned.
+// Its allocations start once they fit the budget beside what is on the device: victims of
+// other admissions may still be in flight, and their unloads (which never wait) free them.
 void load(od_server *S, Object &o, od_ctx *ctx) {
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv.wait(lk, [&] { return S->allocated + o.bytes <= S->budget || !S->failure.empty(); });
+        if (!S->failure.empty()) throw ServerError(OD_ERR_STATE, "server run failed elsewhere");
+        S->allocated += o.bytes;     // the reservation, replaced by the measured size below
+    }
     check(od_mesh_create(ctx, o.nNodes, o.X.data(), o.nTets, o.tets.data(), (int)o.constrained.size(),
                          o.constrained.empty() ? nullptr : o.constrained.data(), 1, nullptr, nullptr, &o.mesh),
           "od_mesh_create");
     check(od_mesh_set_material(o.mesh, o.kind.data(), o.E.data(), o.nu.data(), o.rho.data(), 0.0, 0.1),
           "od_mesh_set_material");
-    {
-        std::unique_lock<std::mutex> lk(S->mu);
-        if (S->spmvR64 == 0) {
-            lk.unlock();
-            int chosen[4];
-            check(od_mesh_autotune(o.mesh, nullptr, chosen), "od_mesh_autotune");
-            lk.lock();
-            S->spmvR64 = chosen[0]; S->spmvL64 = chosen[1];
-            S->spmvR32 = chosen[2]; S->spmvL32 = chosen[3];
-        }
-    }
     check(od_mesh_spmv_variant(o.mesh, S->spmvR64, S->spmvL64, S->spmvR32, S->spmvL32), "od_mesh_spmv_variant");
     check(od_newmark_create(o.mesh, 0.25, 0.5, o.dt, 0.0, 0.0, S->precond, 1e-3, S->tol, 10000, S->precision, 1,
                             &o.it),
@@ -157,7 +159,8 @@ void load(od_server *S, Object &o, od_ctx *ctx) {
     check(od_integrator_device_bytes(o.it, &bytes), "od_integrator_device_bytes");
     std::unique_lock<std::mutex> lk(S->mu);
     S->resident += bytes - o.bytes;   // the reservation becomes the measured size
-    S->peak = std::max(S->peak, S->resident);
+    S->allocated += bytes - o.bytes;
+    S->peak = std::max(S->peak, S->allocated);
     o.bytes = bytes;
     o.measured = true;
     S->loads++;
@@ -226,17 +229,16 @@ void worker(od_server *S, int slot) {
             r.started = true;
             r.start = S->now();
             o.busy = true;
-            // victims are unloaded by this worker before its load allocates: their bytes
-            // leave the account now, so the reservations of all workers stay within budget
-            for (int j : victims) {
-                S->objs[j].busy = true;
-                S->resident -= S->objs[j].bytes;
-            }
+            // The victims are unloaded by this worker before its load allocates.  Their bytes
+            // stay in the account until each unload has freed them, and the object's
+            // reservation enters it now, so no other worker can admit a load into memory
+            // that is still held: the account exceeds the budget only by reservations whose
+            // allocations wait for these unloads.
+            for (int j : victims) S->objs[j].busy = true;
             if (!o.mesh) {
                 const long long need = o.measured ? o.bytes : estimate_bytes(o);
                 o.bytes = need;              // reservation until the load measures it
                 S->resident += need;
-                S->peak = std::max(S->peak, S->resident);
             }
             lk.unlock();
 
@@ -245,8 +247,11 @@ void worker(od_server *S, int slot) {
                 check(od_mesh_bind_ctx(v.mesh, ctx), "od_mesh_bind_ctx");
                 unload(S, v);
                 std::lock_guard<std::mutex> g(S->mu);
+                S->resident -= v.bytes;
+                S->allocated -= v.bytes;
                 v.busy = false;
                 S->evictions++;
+                S->cv.notify_all();
             }
             if (!o.mesh) load(S, o, ctx);
             else check(od_mesh_bind_ctx(o.mesh, ctx), "od_mesh_bind_ctx");
@@ -401,8 +406,24 @@ int od_server_run(od_server *S, int nReq, const int *obj, const int *nSteps, con
     S->waiting.clear();
     for (Object &o : S->objs) { o.queue.clear(); o.queueHead = 0; }
     S->failure.clear();
+    if (S->spmvR64 == 0 && nReq > 0) {
+        // Launch shapes, tuned once per server on the first request's object before any request
+        // is admitted: the tuner's scratch (a 512 MB cache-evicting buffer and buffers of the
+        // object's size) is allocated and freed here, outside the residency the budget governs.
+        const Object &o = S->objs[S->reqs[0].obj];
+        od_mesh *tm = nullptr;
+        check(od_mesh_create(S->slotCtx[0], o.nNodes, o.X.data(), o.nTets, o.tets.data(), (int)o.constrained.size(),
+                             o.constrained.empty() ? nullptr : o.constrained.data(), 1, nullptr, nullptr, &tm),
+              "od_mesh_create");
+        int chosen[4];
+        const int st = od_mesh_autotune(tm, nullptr, chosen);
+        od_mesh_destroy(tm);
+        check(st, "od_mesh_autotune");
+        S->spmvR64 = chosen[0]; S->spmvL64 = chosen[1];
+        S->spmvR32 = chosen[2]; S->spmvL32 = chosen[3];
+    }
     S->evictions = S->loads = 0;
-    S->peak = S->resident;
+    S->peak = S->allocated;
     S->t0 = std::chrono::steady_clock::now();
     std::vector<std::thread> pool;
     for (int i = 0; i < S->nSlots; i++) pool.emplace_back(worker, S, i);

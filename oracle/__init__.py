@@ -34,6 +34,8 @@ def lib():
         L.oro_system_create.restype = vp
         L.oro_system_create.argtypes = [i, dp, i, ip, i, ip]
         L.oro_system_destroy.argtypes = [vp]
+        L.oro_set_threads.argtypes = [i]
+        L.oro_set_threads.restype = i
         L.oro_system_sizes.argtypes = [vp, C.POINTER(C.c_longlong)]
         L.oro_pattern.argtypes = [vp, ip, ip, ip, ip]
         L.oro_global_index.argtypes = [vp, i, i]
@@ -55,6 +57,8 @@ def lib():
         L.oro_ldlt_dense.argtypes = [vp, dp, d, d, dp, dp]
         L.oro_polar.argtypes = [dp, dp, dp]
         L.oro_qr.argtypes = [dp, dp, dp]
+        L.oro_psd3.argtypes = [dp]
+        L.oro_psd2.argtypes = [dp, dp]
         L.oro_eigsym.argtypes = [dp, dp, dp]
         L.oro_corot_element.argtypes = [dp, dp, d, d, d, dp, dp, dp]
         L.oro_corot_k0.argtypes = [dp, d, d, dp]
@@ -87,6 +91,12 @@ def lib():
         L.oro_gps.argtypes = [i, i, i, ip, ip]
         _LIB = L
     return _LIB
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's per-tet element evaluation (1 = the reference's single thread;
+    the scatter stays sequential in tet order, so results are the same bits)."""
+    return lib().oro_set_threads(int(n))
 
 
 def _d(a):

@@ -1,6 +1,8 @@
 // ORACLE — test infrastructure only (checker, never the thing measured or shipped).
 //
-// FP64 single-threaded CPU restatement of OpenDeformer's implicit-step path.
+// FP64 CPU restatement of OpenDeformer's implicit-step path, single-threaded like the
+// reference unless oro_set_threads asks for threads in the per-tet element evaluation (which
+// changes no bits: the scatter stays sequential in tet order).
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
 // this library.  Each function names the reference file:line it follows
 // (paths relative to OpenDeformer/ in the reference tree).
@@ -909,42 +911,62 @@ struct Materials {
     Scalar corotThreshold = 0, neoThreshold = 0.1;
 };
 
+// Threads of the element evaluation (oro_set_threads; default 1, as the reference, which has
+// no threads).  Each tet's element values depend only on that tet, and the scatter below runs
+// in tet order on one thread whatever the count, so the sums are the same bits.
+static int g_threads = 1;
+
+// One tet's element values (sub: 78 packed entries, nvw: 12 forces); updates the tet's own
+// plastic state.
+static void elementValues(System &s, const Materials &m, int e, Scalar *sub, Scalar *nvw) {
+    Scalar K0[90], DmInv[9], R[9], D[3];
+    const int *tn = &s.tets[4 * e];
+    Vec3 ori[4], cur[4];
+    for (int a = 0; a < 4; a++) { ori[a] = s.X[tn[a]]; cur[a] = ori[a] + s.disp[tn[a]]; }
+    if (m.kind[e] == 0) {
+        lameD(m.E[e], m.nu[e], D);
+        corotPrecompute(ori, D, K0, DmInv);
+        corotRotation(ori, cur, DmInv, m.corotThreshold, R);
+        corotSubStiffness(ori, cur, R, K0, sub, nvw);
+    } else if (m.kind[e] == 2) {
+        lameD(m.E[e], m.nu[e], D);
+        plasticElement(ori, cur, D, m.corotThreshold, s.plYield[e], s.plCreep[e], s.plMax[e], &s.plStrain[9 * (size_t)e],
+                       sub, nvw);
+    } else {
+        Scalar lambda = m.nu[e] * m.E[e] / ((Scalar(1.0) + m.nu[e]) * (Scalar(1.0) - Scalar(2.0) * m.nu[e]));
+        Scalar mu = m.E[e] / (Scalar(2.0) * (Scalar(1.0) + m.nu[e]));
+        for (int i = 0; i < 78; i++) sub[i] = 0;
+        neoElement(ori, cur, lambda, mu, m.neoThreshold, sub, nvw);
+    }
+}
+
 // corotationalHyperelastic.cpp:43-102 and invertibleHyperelastic.cpp:34-114:
-// element loop, K.addEntry(map index, value), vws[dof] += force.  Accumulates.
+// element loop, K.addEntry(map index, value), vws[dof] += force.  Accumulates.  The element
+// values of a block of tets are evaluated (on g_threads threads), then scattered in tet order.
 static void assemble(System &s, const Materials &m, Scalar *K, Scalar *vws) {
+    constexpr int kTetBlock = 16384;
+    std::vector<Scalar> subs((size_t)kTetBlock * 78), nvws((size_t)kTetBlock * 12);
     int gi[12];
-    Scalar sub[78], nvw[12], K0[90], DmInv[9], R[9], D[3];
-    for (int e = 0; e < s.nTets; e++) {
-        const int *tn = &s.tets[4 * e];
-        Vec3 ori[4], cur[4];
-        for (int a = 0; a < 4; a++) { ori[a] = s.X[tn[a]]; cur[a] = ori[a] + s.disp[tn[a]]; }
-        s.indexer.getElementNodesGlobalIndices(tn, 4, gi);
-        if (m.kind[e] == 0) {
-            lameD(m.E[e], m.nu[e], D);
-            corotPrecompute(ori, D, K0, DmInv);
-            corotRotation(ori, cur, DmInv, m.corotThreshold, R);
-            corotSubStiffness(ori, cur, R, K0, sub, nvw);
-        } else if (m.kind[e] == 2) {
-            lameD(m.E[e], m.nu[e], D);
-            plasticElement(ori, cur, D, m.corotThreshold, s.plYield[e], s.plCreep[e], s.plMax[e], &s.plStrain[9 * (size_t)e],
-                           sub, nvw);
-        } else {
-            Scalar lambda = m.nu[e] * m.E[e] / ((Scalar(1.0) + m.nu[e]) * (Scalar(1.0) - Scalar(2.0) * m.nu[e]));
-            Scalar mu = m.E[e] / (Scalar(2.0) * (Scalar(1.0) + m.nu[e]));
-            for (int i = 0; i < 78; i++) sub[i] = 0;
-            neoElement(ori, cur, lambda, mu, m.neoThreshold, sub, nvw);
+    for (int e0 = 0; e0 < s.nTets; e0 += kTetBlock) {
+        const int e1 = std::min(s.nTets, e0 + kTetBlock);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+        for (int e = e0; e < e1; e++)
+            elementValues(s, m, e, &subs[(size_t)(e - e0) * 78], &nvws[(size_t)(e - e0) * 12]);
+        for (int e = e0; e < e1; e++) {
+            const Scalar *sub = &subs[(size_t)(e - e0) * 78], *nvw = &nvws[(size_t)(e - e0) * 12];
+            s.indexer.getElementNodesGlobalIndices(&s.tets[4 * e], 4, gi);
+            int entryIndex = 0;
+            const int *local = &s.elemMap[(size_t)78 * e];
+            for (int subCol = 0; subCol < 12; subCol++) {
+                if (gi[subCol] >= 0) {
+                    for (int subRow = subCol; subRow < 12; subRow++) {
+                        if (gi[subRow] >= 0) K[local[entryIndex]] += sub[entryIndex];
+                        entryIndex += 1;
+                    }
+                } else entryIndex += 12 - subCol;
+            }
+            for (int l = 0; l < 12; l++) if (gi[l] >= 0) vws[gi[l]] += nvw[l];
         }
-        int entryIndex = 0;
-        const int *local = &s.elemMap[(size_t)78 * e];
-        for (int subCol = 0; subCol < 12; subCol++) {
-            if (gi[subCol] >= 0) {
-                for (int subRow = subCol; subRow < 12; subRow++) {
-                    if (gi[subRow] >= 0) K[local[entryIndex]] += sub[entryIndex];
-                    entryIndex += 1;
-                }
-            } else entryIndex += 12 - subCol;
-        }
-        for (int l = 0; l < 12; l++) if (gi[l] >= 0) vws[gi[l]] += nvw[l];
     }
 }
 
@@ -1973,6 +1995,8 @@ void *oro_system_create(int nNodes, const double *X, int nTets, const int *tets,
     return s;
 }
 void oro_system_destroy(void *h) { delete (System *)h; }
+// threads of the element evaluation (1 = the reference's single thread); returns the count
+int oro_set_threads(int n) { g_threads = n < 1 ? 1 : n; return g_threads; }
 
 // sizes: [ndof, numBlockColumn, numRemainedColumn, nBlockRowsEntries, nValues]
 void oro_system_sizes(void *h, long long *out) {
@@ -2084,6 +2108,9 @@ void oro_ldlt_dense(void *h, const double *values, double se, double le, double 
 
 // element-level entry points (unit tests)
 void oro_polar(const double *F, double *R, double *S) { polarDecompostion3x3(F, R, S); }
+// forceSemidefinite3x3 / forceSemidefinite2x2 (tetelement.cpp:446-499), in place
+void oro_psd3(double *packed6) { forceSemidefinite3x3(packed6); }
+void oro_psd2(double *diag, double *offdiag) { forceSemidefinite2x2(*diag, *offdiag); }
 void oro_qr(const double *F, double *Q, double *R) { properQRDecompostion3x3(F, Q, R); }
 void oro_eigsym(const double *tri, double *vals, double *vecs) { eigenSym3x3(tri, vals, vecs); }
 void oro_corot_element(const double *X12, const double *x12, double E, double nu, double threshold, double *sub78, double *f12,

@@ -369,3 +369,108 @@ def test_newmark_plastic_matches_oracle(ctx):
         assert relerr(nm.state()[0], ref.state()[0]) < NEWMARK_RTOL, k
     assert np.abs(s.plastic_strain()).max() > 0
     assert relerr(mesh.plastic_strain(), s.plastic_strain()) < 1e-6
+
+
+def signed_volumes(X, tets):
+    p = X[tets]
+    return np.einsum("ij,ij->i", np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), p[:, 3] - p[:, 0]) / 6.0
+
+
+def test_corotational_qr_branch_matches_oracle(ctx):
+    """Compression threshold 0.9: tets whose deformed volume falls below 0.9 of the rest volume
+    take the QR rotation (element/tetelement.cpp:571-572, core/numerMethod.h:431-507), the
+    others the polar one.  The right half of the bar is compressed to 0.6 of its length and
+    sheared, so both branches run; K and f match the oracle, and differ from the all-polar
+    evaluation (the QR branch changes the rotations)."""
+    import opendeformer_amd as od
+    w = W.box_small()
+    mesh, s = make(ctx, w)
+    thr = 0.9
+    mesh.set_material(w.kind, w.E, w.nu, w.rho, corot_threshold=thr)
+    X = w.X
+    un = np.zeros_like(X)
+    right = X[:, 0] > 0.5
+    un[right, 0] = -0.4 * (X[right, 0] - 0.5)
+    un[right, 1] = 0.15 * (X[right, 0] - 0.5)
+    un[:, 2] += 0.003 * np.random.default_rng(11).standard_normal(len(X)) * (X[:, 0] > 0)
+    ratio = signed_volumes(X + un, w.tets) / signed_volumes(X, w.tets)
+    n_qr = int((ratio < thr).sum())
+    assert 100 < n_qr < len(w.tets) - 100, n_qr
+    u = w.node_to_free(un)
+    K = od.Matrix(mesh); f = od.DeviceArray(ctx, mesh.ndof); f.zero()
+    od.generate_matrix_and_virtual_works(mesh, u, K, f)
+    s.set_free_displacements(u)
+    Kc, fc = s.assemble(w.kind, w.E, w.nu, thr_corot=thr)
+    Kp, _ = s.assemble(w.kind, w.E, w.nu, thr_corot=0.0)
+    assert relerr(Kc, Kp) > 1e-4
+    assert relerr(K.download(), Kc) < 1e-11
+    assert relerr(f.download(), fc) < 1e-10
+
+
+def test_newmark_with_qr_branch_matches_oracle(ctx):
+    """Newmark under gravity with compression threshold 0.99: the bending bar's compressed tets
+    switch to the QR rotation step by step; 40 steps against the oracle."""
+    import opendeformer_amd as od
+    w = W.box_small("color4")
+    mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
+    mesh.set_material(w.kind, w.E, w.nu, w.rho, corot_threshold=0.99)
+    s = O.System(w.X, w.tets, w.fixed_dofs)
+    f = s.body_force(w.body)
+    nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precond_param=1e-3)
+    nm.set_external_virtual_work(f)
+    ref = O.Newmark(s, w.kind, w.E, w.nu, w.rho, w.dt, pre=O.PRE_IC0, p1=1e-3, thr_corot=0.99)
+    ref.set_external(f)
+    compressed = 0
+    for k in range(40):
+        nm.run_one_time_step(); ref.step(1)
+        d = nm.state()[0]
+        assert relerr(d, ref.state()[0]) < NEWMARK_RTOL, k
+        ratio = signed_volumes(w.X + w.free_to_node(d), w.tets) / signed_volumes(w.X, w.tets)
+        compressed = max(compressed, int((ratio < 0.99).sum()))
+    assert compressed > 50, compressed
+
+
+def test_neohookean_inverted_tets_match_oracle(ctx):
+    """Inverted tets (det F < 0): eight interior nodes pushed through their neighbours invert
+    the tets around them, so the neo-Hookean element runs the reflection and the singular-value
+    clamp (material/invertibleHyperelastic.cpp:139-245) and the PSD projections
+    (element/tetelement.cpp:446-499) on them; K and f match the oracle."""
+    import opendeformer_amd as od
+    w = W.box_small()
+    w.kind[:] = capi.OD_MAT_NEOHOOKEAN
+    mesh, s = make(ctx, w)
+    X = w.X
+    un = 0.002 * np.random.default_rng(12).standard_normal(X.shape)
+    un[X[:, 0] == 0] = 0
+    interior = np.nonzero((X[:, 0] > 0.1) & (X[:, 0] < 0.9) & (X[:, 1] > 0.01) & (X[:, 1] < 0.19)
+                          & (X[:, 2] > 0.01) & (X[:, 2] < 0.19))[0]
+    for n in interior[::7][:8]:
+        un[n, 0] += 0.09     # almost two cells along x
+    vol = signed_volumes(X + un, w.tets)
+    n_inv = int((vol < 0).sum())
+    assert n_inv >= 8, n_inv
+    u = w.node_to_free(un)
+    K = od.Matrix(mesh); f = od.DeviceArray(ctx, mesh.ndof); f.zero()
+    od.generate_matrix_and_virtual_works(mesh, u, K, f)
+    s.set_free_displacements(u)
+    Kc, fc = s.assemble(w.kind, w.E, w.nu)
+    assert relerr(K.download(), Kc) < 1e-8
+    assert relerr(f.download(), fc) < 1e-8
+
+
+@pytest.mark.parametrize("precision", [capi.OD_FP64, capi.OD_MIXED])
+def test_graph_captured_once_per_buffer_set(ctx, precision):
+    """Graph mode captures the iteration block once for the integrator's buffers and replays it
+    on every later step (the solve's buffers keep their addresses); a change of the sweep form
+    makes exactly one new capture."""
+    import opendeformer_amd as od
+    w = W.box_small("color4")
+    mesh, s = make(ctx, w)
+    nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=precision)
+    nm.set_graph(True)
+    nm.set_external_virtual_work(s.body_force(w.body))
+    nm.run_time_steps(8)
+    assert nm.solver_info()["graph_captures"] == 1
+    od.mesh_sweep_form(mesh, 8, 4 if precision == capi.OD_MIXED else 0)
+    nm.run_time_steps(3)
+    assert nm.solver_info()["graph_captures"] == 2

@@ -184,3 +184,176 @@ def test_mass_and_body_force_totals(oracle):
     assert total < 40.0
     fb = s_free.body_force(w.body)
     np.testing.assert_allclose(fb.reshape(-1, 3).sum(0), [0, 0, -9.81 * 40.0], rtol=1e-12)
+
+
+# ---- neo-Hookean PSD projections and the inversion branch ------------------------------
+# forceSemidefinite3x3 / 2x2 (element/tetelement.cpp:446-499) against numpy's eigen-clamp, the
+# projected dP/dF against the block-wise clamp of a finite-difference derivative of P, and P on
+# inverted or flattened tets against the signed, clamped SVD (material/invertibleHyperelastic.cpp:
+# 139-245): the sign of det F goes onto the smallest singular value, which the threshold then
+# clamps from below.
+
+def eigen_clamp(A):
+    lam, Q = np.linalg.eigh(A)
+    return (Q * np.maximum(lam, 0.0)) @ Q.T
+
+
+def test_force_semidefinite_3x3_is_the_eigen_clamp(oracle):
+    rng = np.random.default_rng(21)
+    projected = 0
+    for trial in range(200):
+        B = rng.standard_normal((3, 3))
+        A = B + B.T if trial % 3 else B @ B.T              # indefinite, or PSD (left unchanged)
+        packed = np.array([A[0, 0], A[0, 1], A[0, 2], A[1, 1], A[1, 2], A[2, 2]])
+        out = packed.copy()
+        oracle.lib().oro_psd3(oracle._d(out))
+        if np.linalg.eigvalsh(A).min() >= 0:
+            assert np.array_equal(out, packed)
+            continue
+        C = eigen_clamp(A)
+        expect = np.array([C[0, 0], C[0, 1], C[0, 2], C[1, 1], C[1, 2], C[2, 2]])
+        np.testing.assert_allclose(out, expect, atol=1e-12 * np.abs(A).max())
+        projected += 1
+    assert projected > 100
+
+
+def test_force_semidefinite_2x2_is_the_eigen_clamp(oracle):
+    """The block [[diag, offdiag], [offdiag, diag]] (the (ij, ji) pair of dP/dF) has eigenvalues
+    diag +- offdiag; the projection clamps the negative ones to zero."""
+    rng = np.random.default_rng(22)
+    cases = [(d, o) for d, o in rng.standard_normal((300, 2)) * 3.0] + [(1.0, 2.0), (1.0, -2.0), (-1.0, 0.5),
+                                                                         (2.0, 0.5), (-3.0, -0.5)]
+    seen = set()
+    for d, o in cases:
+        dd, oo = C_double(d), C_double(o)
+        oracle.lib().oro_psd2(oracle._d(dd), oracle._d(oo))
+        A = np.array([[d, o], [o, d]])
+        C = eigen_clamp(A)
+        np.testing.assert_allclose([dd[0], oo[0]], [C[0, 0], C[0, 1]], atol=1e-14 * (abs(d) + abs(o)))
+        seen.add(((d + o) < 0, (d - o) < 0))
+    assert len(seen) == 4          # every clamp case taken
+
+
+def C_double(v):
+    return np.array([v], np.float64)
+
+
+def neo_stress(oracle, F, thr=0.1, with_dpdf=False):
+    P = np.zeros(9); T = np.zeros(81)
+    oracle.lib().oro_neo_stress(oracle._d(np.ascontiguousarray(F, np.float64).ravel().copy()), E, NU, thr,
+                                oracle._d(P), oracle._d(T) if with_dpdf else None)
+    return (P.reshape(3, 3), T.reshape(3, 3, 3, 3)) if with_dpdf else P.reshape(3, 3)
+
+
+@pytest.mark.parametrize("sigma", [(1.3, 1.1, 0.9), (0.6, 0.45, 0.3), (1.2, 0.5, 0.25), (0.95, 0.8, 0.7)])
+def test_neohookean_projected_dPdF_is_the_blockwise_clamp_of_dP_dF(oracle, sigma):
+    """At F = diag(sigma) (distinct, above the clamp) the oracle's projected dP/dF equals the
+    finite-difference derivative of its own P (itself pinned to the closed-form energy), with
+    the 3x3 block on (00, 11, 22) and the 2x2 blocks on (01, 10), (02, 20), (12, 21) each
+    replaced by its eigen-clamp.  Compressed states make the 2x2 blocks indefinite."""
+    F = np.diag(sigma).astype(float)
+    _, T = neo_stress(oracle, F, with_dpdf=True)
+    h = 1e-6
+    D = np.zeros((3, 3, 3, 3))
+    for k in range(3):
+        for l in range(3):
+            dF = np.zeros((3, 3)); dF[k, l] = h
+            D[:, :, k, l] = (neo_stress(oracle, F + dF) - neo_stress(oracle, F - dF)) / (2 * h)
+    expect = np.zeros((3, 3, 3, 3))
+    idx3 = [(0, 0), (1, 1), (2, 2)]
+    B3 = np.array([[D[i, j, k, l] for (k, l) in idx3] for (i, j) in idx3])
+    C3 = eigen_clamp(0.5 * (B3 + B3.T))
+    for p, (i, j) in enumerate(idx3):
+        for q, (k, l) in enumerate(idx3):
+            expect[i, j, k, l] = C3[p, q]
+    active = 0
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        pair = [(a, b), (b, a)]
+        B2 = np.array([[D[i, j, k, l] for (k, l) in pair] for (i, j) in pair])
+        active += int(np.linalg.eigvalsh(0.5 * (B2 + B2.T)).min() < 0)
+        C2 = eigen_clamp(0.5 * (B2 + B2.T))
+        for p, (i, j) in enumerate(pair):
+            for q, (k, l) in enumerate(pair):
+                expect[i, j, k, l] = C2[p, q]
+    np.testing.assert_allclose(T, expect, atol=1e-6 * np.abs(expect).max())
+    if min(sigma) < 0.5:
+        assert active > 0          # the compressed states exercise the 2x2 projection
+
+
+def signed_clamped_svd_stress(F, thr):
+    """U diag(p(s)) V^T from numpy's SVD with proper rotations U, V: when det F < 0 the smallest
+    singular value carries the sign; s = max(signed sigma, thr); p_i = mu s_i - (mu - lam ln J)/s_i
+    with J = s_0 s_1 s_2 (the derivative of the compressible neo-Hookean energy in the diagonal
+    space)."""
+    U, S, Vt = np.linalg.svd(F)
+    S = S.copy()
+    if np.linalg.det(U) < 0:
+        U[:, 2] *= -1; S[2] *= -1
+    if np.linalg.det(Vt) < 0:
+        Vt[2, :] *= -1; S[2] *= -1
+    s = np.maximum(S, thr)
+    J = np.prod(s)
+    p = MU * s - (MU - LAM * np.log(J)) / s
+    return (U * p) @ Vt, S
+
+
+def test_neohookean_inverted_and_flattened_stress_is_the_clamped_signed_svd(oracle):
+    """Inverted, compressed-below-the-clamp and nearly flat tets in general position (the
+    smallest singular value 1e-3 to 1e-2: the regular branch, then the clamp; below that the
+    reference's V^T row u^T F / sigma inherits the rounding of sigma^2 and drifts from unit
+    length, a conditioning property of its method, not a difference in P's formula)."""
+    rng = np.random.default_rng(23)
+    kinds = {"inverted": 0, "compressed": 0, "nearly flat": 0}
+    for trial in range(120):
+        Q1 = Rotation.from_rotvec(rng.standard_normal(3)).as_matrix()
+        Q2 = Rotation.from_rotvec(rng.standard_normal(3)).as_matrix()
+        sig = rng.uniform(0.3, 1.5, 3)
+        mode = trial % 3
+        if mode == 0:
+            sig[rng.integers(3)] *= -rng.uniform(0.1, 1.0)     # inverted (det F < 0)
+        elif mode == 1:
+            sig[rng.integers(3)] = rng.uniform(0.01, 0.09)      # compressed below the clamp
+        else:
+            sig[rng.integers(3)] = 10.0 ** rng.uniform(-3, -2)  # nearly flat
+        F = Q1 @ np.diag(sig) @ Q2.T
+        P = neo_stress(oracle, F, thr=0.1)
+        Pe, S = signed_clamped_svd_stress(F, 0.1)
+        np.testing.assert_allclose(P, Pe, atol=1e-9 * np.abs(Pe).max())
+        kinds[["inverted", "compressed", "nearly flat"][mode]] += 1
+    assert min(kinds.values()) == 40
+
+
+def test_neohookean_degenerate_branches_on_exactly_flat_tets(oracle):
+    """F = diag(sigma) with one exact zero: the reference's closed-form eigen-solver returns the
+    zero eigenvalue of F F^T as rounding noise of either sign (about 1e-16).  Below zero it is
+    taken as 0 and the degenerate branch (invertibleHyperelastic.cpp:156-231) completes V^T by a
+    cross product and clamps the zero singular value to the threshold: P = diag(p(s)).  Above
+    zero its root (about 1.5e-8) passes the 1e-8 test, the regular branch divides the zero row
+    u^T F by it, and the flat direction's term of P vanishes.  Both formulas are checked, and
+    both branches must occur over the cases."""
+    branches = set()
+    for sigma in [(1.2, 0.7, 0.0), (0.0, 1.1, 0.9), (0.8, 0.0, 1.3), (0.0, 1.21, 0.81), (0.5, 0.0, 0.6),
+                  (1.3, 0.9, 0.0), (0.0, 0.4, 0.3), (0.7, 1.7, 0.0), (2.0, 0.0, 0.05)]:
+        F = np.diag(sigma).astype(float)
+        P = neo_stress(oracle, F, thr=0.1)
+        s = np.maximum(np.array(sigma, float), 0.1)
+        J = np.prod(s)
+        p = MU * s - (MU - LAM * np.log(J)) / s
+        degenerate = np.diag(p)
+        regular = np.diag(np.where(np.array(sigma) == 0.0, 0.0, p))
+        tol = 1e-9 * np.abs(p).max()
+        if np.abs(P - degenerate).max() < tol:
+            branches.add("degenerate")
+        else:
+            np.testing.assert_allclose(P, regular, atol=tol)
+            branches.add("regular")
+    assert branches == {"degenerate", "regular"}
+
+
+def test_neohookean_inverted_diagonal_is_reflected_and_clamped(oracle):
+    """F = diag(-1.2, 0.7, 0.9): det F < 0, the smallest singular value in magnitude carries the
+    sign (invertibleHyperelastic.cpp:147-152), so 0.7 becomes -0.7 and clamps to the threshold."""
+    F = np.diag([-1.2, 0.7, 0.9])
+    P = neo_stress(oracle, F, thr=0.1)
+    Pe, S = signed_clamped_svd_stress(F, 0.1)
+    np.testing.assert_allclose(P, Pe, atol=1e-9 * np.abs(Pe).max())
