@@ -4,18 +4,20 @@
 Workload (default): 200x40x40 Kuhn box, 5x1x1 m, 1,920,000 tets, 1,008,600
 free DOF, corotational material with per-tet E log-uniform in [1e4, 1e6],
 nu = 0.45, rho = 1000, x = 0 face fixed, gravity, Newmark beta = 1/4,
-gamma = 1/2, dt = 1/60, PCG to 1e-6 relative residual.  A "step" is one full
-Newmark step on the device: element pass, assembly, matrix combine, IC(0)
-factorisation, PCG to tolerance, corrector.
+gamma = 1/2, dt = 1/60, IC(0)-PCG to 1e-6 relative residual, 4-colour node
+numbering.  A "step" is one full Newmark step on the device: element pass,
+assembly, matrix combine, IC(0) factorisation, PCG to tolerance, corrector.
 
-The default form is the precision form: FP32 inner PCG (FP32 matrix, factor and
-vectors, FP64 reductions) with FP64 residual corrections to the same 1e-6
-relative residual.  The plain FP64 path runs in the same process on the same
-mesh for the speed-up and the displacement error between the two.
+`value` is the plain FP64 path (the reference's arithmetic).  The precision
+form (FP32 inner PCG with FP64 residual corrections to the same 1e-6 relative
+residual) runs on the same mesh in the same process and is reported under
+`mixed_form`, with its speed-up and its displacement difference to the FP64
+path.
 
-Prints one JSON line.  Extra fields: SpMV roofline from HIP events on the
-library stream over the timed steps, and the CPU FP64 oracle timed on a bounded
-sample of the same workload.
+Prints one JSON line.  Extra fields: the SpMV roofline of each form from HIP
+events on the library stream, the step roofline of each form, the benchmark's
+other legs, and the CPU FP64 oracle timed on the full workload (single thread,
+on a host thread while the GPU legs run).
 """
 from __future__ import annotations
 
@@ -24,6 +26,7 @@ import json
 import os
 import platform
 import sys
+import threading
 import time
 
 import numpy as np
@@ -55,12 +58,29 @@ def spmv_bytes(mesh, value_bytes=8, vec_bytes=8, fused=False):
     return b + (2 * vec_bytes * mesh.ndof if fused else 0)
 
 
-def traffic_from_profile(workload, args):
+def step_bytes(mesh, precision, iters, outer=0.0):
+    """Algorithmic HBM bytes of one Newmark step (the survey's per-unit figures, from the mesh's
+    counts): fixed work = element pass (130.7 B per tet: connectivity, material, node positions
+    and state, K0 recomputed), the M-combine read (8 B per stored value), two right-hand-side
+    SpMVs and the IC(0) factorisation (read A, write the factor); per PCG iteration = one SpMV,
+    one IC(0) apply (two sweeps over a factor with A's pattern: twice the SpMV's bytes) and
+    eight vector passes.  The mixed form's iterations use FP32 values and vectors, and each
+    refinement sweep adds an FP64 residual product and four FP64 vector passes."""
+    n = mesh.ndof
+    spmv64 = spmv_bytes(mesh, 8, 8)
+    fixed = 130.7 * mesh.n_tets + 8 * mesh.n_values + 2 * spmv64 + 16 * mesh.n_values
+    if precision == "fp64":
+        return fixed + iters * (3 * spmv64 + 8 * 8 * n)
+    spmv32 = spmv_bytes(mesh, 4, 4)
+    return fixed + iters * (3 * spmv32 + 8 * 4 * n) + outer * (spmv64 + 4 * 8 * n)
+
+
+def traffic_from_profile(workload, args, precision=None):
     """HBM bytes per SpMV launch measured with FETCH_SIZE/WRITE_SIZE counters (scaled by a
     copy kernel of known size), read from the committed counter summary of this workload
     and configuration; None when there is none."""
     path = os.path.join(ROOT, "profiles", f"pmc_spmv_{workload}_{args.precond}_{args.ordering}_"
-                                          f"{args.precision}.md")
+                                          f"{precision or args.precision}.md")
     if not os.path.exists(path):
         return None
     for ln in open(path):
@@ -122,7 +142,7 @@ def mid_neo_leg(args, steps=200):
             "mixed_rel_err_vs_fp64_max_over_sampled_steps": max(errs)}
 
 
-def batched_leg(args, n_inst=256, warmup=2, steps=20):
+def batched_leg(args, fp64_dof_steps_per_s=None, n_inst=256, warmup=2, steps=120):
     """256 uneven boxes (tets log-uniform [500, 5e4], aspect U(1, 6), corotational or
     neo-Hookean, E log-uniform [1e4, 1e6], nu U(0.3, 0.48)) concatenated into one system and
     stepped together: mixed-precision IC(0)-PCG to 1e-6 with per-instance convergence, PCG
@@ -168,17 +188,23 @@ def batched_leg(args, n_inst=256, warmup=2, steps=20):
         alone_n += 10
         del ni, mi
     alone_rate = alone_n / alone_t
-    return {"workload": f"{n_inst} boxes, {mesh.n_tets} tets, {mesh.ndof} DOF in one concatenated system "
-                        f"(4-colour ordering per box), mixed precision, graph replay",
-            "ms_per_step": round(dt * 1e3, 3), "batch_steps_per_s": round(1.0 / dt, 3),
-            "instance_steps_per_s": round(n_inst / dt, 1), "steps": steps, "warmup": warmup,
-            "pcg_iters_per_instance_step": {"median": float(np.median(its)), "max": int(its.max()),
-                                            "min": int(its.min())},
-            "plain_fp64_alone_instance_steps_per_s": round(alone_rate, 1),
-            "plain_fp64_alone_sample": f"instances 0, 16, ..., {n_inst - 16}: 10 steps each, one at a time",
-            "speedup_vs_plain_fp64_alone": round((n_inst / dt) / alone_rate, 2),
-            "spmv_shape": {"fp64": {"chunks_per_workgroup": v64[0], "lanes_per_row": v64[1]},
-                           "fp32": shape_f32(v32)}}
+    out = {"workload": f"{n_inst} boxes, {mesh.n_tets} tets, {mesh.ndof} DOF in one concatenated system "
+                       f"(4-colour ordering per box), mixed precision, graph replay",
+           "ms_per_step": round(dt * 1e3, 3), "batch_steps_per_s": round(1.0 / dt, 3),
+           "instance_steps_per_s": round(n_inst / dt, 1), "dof_steps_per_s": round(mesh.ndof / dt, 1),
+           "steps": steps, "warmup": warmup,
+           "pcg_iters_per_instance_step": {"median": float(np.median(its)), "max": int(its.max()),
+                                           "min": int(its.min())},
+           "plain_fp64_alone_instance_steps_per_s": round(alone_rate, 1),
+           "plain_fp64_alone_sample": f"instances 0, 16, ..., {n_inst - 16}: 10 steps each, one at a time",
+           "speedup_vs_plain_fp64_alone": round((n_inst / dt) / alone_rate, 2),
+           "spmv_shape": {"fp64": {"chunks_per_workgroup": v64[0], "lanes_per_row": v64[1]},
+                          "fp32": shape_f32(v32)}}
+    if fp64_dof_steps_per_s:
+        # the like-for-like comparison: DOF advanced per second against the 1M-DOF box on the plain
+        # FP64 path (large single meshes are not launch-bound, small lone instances are)
+        out["speedup_vs_fp64_path_dof_steps"] = round(out["dof_steps_per_s"] / fp64_dof_steps_per_s, 3)
+    return out
 
 
 def serving_leg(args, n_obj=64, tet_lo=1000, tet_hi=50000, rates=(50.0, 200.0), duration=4.0):
@@ -273,45 +299,65 @@ def ordering_leg(args, scale=0.5, max_iter=2000,
                         f"FP64 IC(0)-PCG to 1e-6, at most {max_iter} iterations", **out}
 
 
-def cpu_baseline(args, gpu_iters_per_step, full_ndof):
-    """Times the FP64 oracle on a slab of the same workload (single thread)."""
-    import oracle as O
-    from opendeformer_amd import workloads as W
-    # slab: the first 20 of the 200 cells along x, same cross-section, seed and ordering
-    from opendeformer_amd import mesh as M
-    nx = 20
-    m = M.box_mesh(nx, 40, 40, 5.0 * nx / 200, 1.0, 1.0, seed=3)
-    if args.ordering in ("color8", "color5", "color4"):
-        m = M.relabel(m, M.multicolor_permutation(m, int(args.ordering[len("color"):])))
-    elif args.ordering == "gps":
-        m = M.gps_relabel(m)
-    E = M.log_uniform(3, M.STREAM_YOUNG, np.arange(m.n_tets, dtype=np.uint64), 1e4, 1e6)
-    fixed = M.fixed_dofs_from_nodes(M.plane_fixed_nodes(m))
-    t0 = time.perf_counter()
-    s = O.System(m.X, m.tets, fixed)
-    t_setup = time.perf_counter() - t0
-    body = np.tile([0.0, 0.0, -9.81 * 1000.0], (m.n_tets, 1))
-    pre = {"jacobi": O.PRE_JACOBI, "ic0": O.PRE_IC0}[args.precond]
-    nm = O.Newmark(s, 0, E, 0.45, 1000.0, 1.0 / 60, pre=pre, p1=1e-3)
-    nm.set_external(s.body_force(body))
-    nm.step(1)                                           # warm-up (includes the initial-acceleration solve)
-    t0 = time.perf_counter()
-    its, _, _ = nm.step(2)
-    t_step = (time.perf_counter() - t0) / 2
-    # split: element pass + assembly vs. PCG iterations
-    t0 = time.perf_counter()
-    s.assemble(0, E, 0.45)
-    t_asm = time.perf_counter() - t0
-    it_s = float(np.mean(its))
-    t_iter = max(t_step - t_asm, 0.0) / max(it_s, 1.0)
-    ratio = full_ndof / s.ndof
-    t_full = ratio * (t_asm + t_iter * gpu_iters_per_step)
-    return {"value": 1.0 / t_full, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": (f"FP64 oracle, 1 thread, on a {nx}x40x40 slab of the same workload ({s.ndof} DOF, seed 3, "
-                       f"{args.ordering} ordering, {args.precond}); 2 timed steps after 1 warm-up: "
-                       f"{t_step:.2f} s/step ({it_s:.0f} PCG iterations), element+assembly {t_asm:.2f} s; scaled to "
-                       f"{full_ndof} DOF linearly in DOF and to the GPU's {gpu_iters_per_step:.0f} iterations/step; "
-                       f"oracle set-up {t_setup:.1f} s not counted; host {platform.processor() or platform.machine()}")}
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+class CpuBaseline(threading.Thread):
+    """The FP64 oracle (single thread, as the reference) on the benchmark's own workload: same
+    mesh, seed, numbering and preconditioner; one warm-up step (it includes the initial
+    acceleration solve), then `steps` timed steps.  Runs on a host thread (the oracle's calls
+    release the GIL) while the GPU legs run."""
+
+    def __init__(self, w, args, steps=5):
+        super().__init__(daemon=True)
+        self.w, self.args, self.steps = w, args, steps
+        self.out, self.err = None, None
+
+    def run(self):
+        try:
+            import oracle as O
+            O.lib()
+            O.set_threads(1)
+            w = self.w
+            t0 = time.perf_counter()
+            s = O.System(w.X, w.tets, w.fixed_dofs)
+            pre = {"jacobi": O.PRE_JACOBI, "ic0": O.PRE_IC0}[self.args.precond]
+            nm = O.Newmark(s, w.kind, w.E, w.nu, w.rho, w.dt, pre=pre, p1=1e-3)
+            nm.set_external(s.body_force(w.body))
+            t_setup = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            nm.step(1)
+            t_warm = time.perf_counter() - t0
+            ts, its = [], []
+            for _ in range(self.steps):
+                t0 = time.perf_counter()
+                it, _, _ = nm.step(1)
+                ts.append(time.perf_counter() - t0)
+                its.append(int(it[0]))
+            med = float(np.median(ts))
+            self.out = {"value": 1.0 / med, "unit": "steps/s", "cores": 1, "kind": "port",
+                        "sample": (f"FP64 oracle (the reference's CPU algorithm restated), 1 thread, on the full "
+                                   f"workload ({w.name}, {s.ndof} DOF, {self.args.ordering} numbering, "
+                                   f"{self.args.precond}-PCG to 1e-6): set-up {t_setup:.1f} s, 1 warm-up step "
+                                   f"{t_warm:.1f} s (with the initial-acceleration solve), then {self.steps} timed "
+                                   f"steps, median {med:.2f} s/step (min {min(ts):.2f}, max {max(ts):.2f}), "
+                                   f"{np.mean(its):.1f} PCG iterations/step; host CPU {cpu_model()}, "
+                                   f"{os.cpu_count()} logical CPUs visible; run on a host thread beside the GPU legs")}
+        except BaseException as e:  # reported in the line, the GPU numbers stay valid
+            self.err = e
+
+    def result(self):
+        self.join()
+        if self.err is not None:
+            return {"value": None, "error": str(self.err)}
+        return self.out
 
 
 def main():
@@ -320,14 +366,16 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precond", choices=["jacobi", "ic0"], default="ic0")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="the form whose steps/s is `value` (default: the plain FP64 path); the other form "
+                         "runs beside it unless --no-second-form")
     ap.add_argument("--ordering", choices=["natural", "color8", "color5", "color4", "gps"], default="color4",
                     help="node numbering: natural (x-slowest grid order), grouped by an 8-, 5- or 4-colouring "
                          "(4 colours: 4 IC(0) dependency levels), or the GPS relabeller's")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = 200x40x40)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--no-fp64", action="store_true",
-                    help="with --precision mixed: skip the plain FP64 path run (speed-up and error report)")
+    ap.add_argument("--no-second-form", action="store_true",
+                    help="run only the --precision form (no speed-up or error report of the other one)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-instances leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--no-damping", action="store_true", help="skip the plane collision damping leg")
@@ -350,6 +398,10 @@ def main():
     from opendeformer_amd import workloads as W
 
     w = W.box_1m_dof(args.ordering, scale=args.scale)
+    cpu = None
+    if not args.no_cpu:
+        cpu = CpuBaseline(w, args)
+        cpu.start()
     ctx = od.Context(0)
     mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
@@ -379,8 +431,7 @@ def main():
             ctx.sync()
             return time.perf_counter() - t0, iters, relres, outer
 
-        # the headline: K steps as a user runs them (PCG iterations replayed from HIP graphs,
-        # no instrumentation)
+        # K steps as a user runs them (PCG iterations replayed from HIP graphs, no instrumentation)
         elapsed, iters, relres, outer = timed_steps()
         # the SpMV's launch durations: K more steps with a HIP event pair recorded from every
         # SpMV dispatch on the library's stream (eager launches: events cannot sit in a graph)
@@ -389,39 +440,51 @@ def main():
         spmv_ms, spmv_n = ctx.profile_read()
         ctx.profile(False)
         vb = 4 if precision == "mixed" else 8
-        by = spmv_bytes(mesh, vb, vb, fused=(vb == 4))
+        alg = spmv_bytes(mesh, vb, vb)                       # the survey's SpMV unit
+        fused = spmv_bytes(mesh, vb, vb, fused=(vb == 4))    # what the FP32 product also moves
         # Launches of a batch that finds its instance converged exit at once; their (small) time
         # stays in the timed total, but only the iterations that ran count as SpMV work.
         real_spmv = int(np.sum(iters_ev))
         avg_spmv_s = (spmv_ms / max(real_spmv, 1)) / 1e3
-        achieved = by / avg_spmv_s / 1e9 if real_spmv else 0.0
+        achieved = alg / avg_spmv_s / 1e9 if real_spmv else 0.0
         d = nm.state()[0]
         del nm
-        return {"steps_s": args.steps / elapsed, "ms_step": elapsed / args.steps * 1e3,
-                "steps_s_instrumented": args.steps / elapsed_ev,
+        t_step = elapsed / args.steps
+        sb = step_bytes(mesh, precision, float(np.mean(iters)), float(np.mean(outer)))
+        spmv = {"kernel": (f"k_spmv_dot<double, {v64[0]}, false, {v64[1]}>" if vb == 8 else spmv_kernel_f32(v32)),
+                "launches": spmv_n, "iterations": real_spmv,
+                "avg_us_per_launch": round(spmv_ms / max(spmv_n, 1) * 1e3, 2),
+                "avg_us_per_iteration": round(avg_spmv_s * 1e6, 2),
+                "algorithmic_MB": round(alg / 1e6, 2), "GBps": round(achieved, 1),
+                "timed_over": (f"{args.steps} further steps with a HIP event pair on every SpMV dispatch "
+                               f"(eager launches, {args.steps / elapsed_ev:.2f} steps/s); the headline "
+                               f"steps are not instrumented")}
+        if fused != alg:
+            spmv["fused_MB"] = round(fused / 1e6, 2)
+            spmv["fused_GBps"] = round(fused / avg_spmv_s / 1e9, 1) if real_spmv else 0.0
+            spmv["fused_note"] = ("the FP32 product also reads z and writes the new direction p = z + beta p_old "
+                                  "(the direction update fused into it)")
+        return {"steps_s": args.steps / elapsed, "ms_step": t_step * 1e3,
                 "iters": float(np.mean(iters)), "relres_max": float(np.max(relres)),
-                "outer": float(np.mean(outer)),
-                "spmv": {"kernel": (f"k_spmv_dot<double, {v64[0]}, false, {v64[1]}>" if vb == 8
-                                    else spmv_kernel_f32(v32)),
-                         "launches": spmv_n, "iterations": real_spmv,
-                         "avg_us_per_launch": round(spmv_ms / max(spmv_n, 1) * 1e3, 2),
-                         "avg_us_per_iteration": round(avg_spmv_s * 1e6, 2),
-                         "algorithmic_MB": round(by / 1e6, 2), "GBps": round(achieved, 1),
-                         "timed_over": (f"{args.steps} further steps with a HIP event pair on every SpMV dispatch "
-                                        f"(eager launches, {args.steps / elapsed_ev:.2f} steps/s); the headline "
-                                        f"steps are not instrumented")},
-                "achieved": achieved}, d
+                "outer": float(np.mean(outer)), "spmv": spmv, "achieved": achieved,
+                "step_roofline": {"bytes_per_step_GB": round(sb / 1e9, 3), "achieved_GBps": round(sb / t_step / 1e9, 1),
+                                  "frac": round(sb / t_step / 1e9 / HBM_PEAK_GBS, 4)}}, d
+
+    def roofline(r, precision):
+        return {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profile(w.name, args, precision)}
 
     log(f"{w.name}: {mesh.n_tets} tets, {mesh.ndof} DOF; SpMV shapes {v64}/{v32}; timing the {args.precision} form")
     main_r, d_main = run_form(args.precision)
     log(f"{args.precision}: {main_r['steps_s']:.2f} steps/s")
-    fp64_r = None
-    if args.precision != "fp64" and not args.no_fp64:
-        fp64_r, d64 = run_form("fp64")
-        log(f"fp64: {fp64_r['steps_s']:.2f} steps/s")
-        err = float(np.linalg.norm(d_main - d64) / np.linalg.norm(d64))
-    achieved = main_r["achieved"]
-    traffic = traffic_from_profile(w.name, args)
+    other = "mixed" if args.precision == "fp64" else "fp64"
+    other_r = None
+    if not args.no_second_form:
+        other_r, d_other = run_form(other)
+        log(f"{other}: {other_r['steps_s']:.2f} steps/s")
+        d64, d32 = (d_main, d_other) if args.precision == "fp64" else (d_other, d_main)
+        err = float(np.linalg.norm(d32 - d64) / np.linalg.norm(d64))
     line = {
         "metric": METRIC, "value": round(main_r["steps_s"], 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(main_r["ms_step"], 3), "higher_is_better": True,
@@ -435,9 +498,25 @@ def main():
         "pcg_iters_per_step": round(main_r["iters"], 2),
         "relres_max": main_r["relres_max"],
         "spmv": main_r["spmv"],
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline": roofline(main_r, args.precision),
+        "step_roofline": main_r["step_roofline"],
     }
+    if args.precision == "mixed":
+        line["refinement_sweeps_per_step"] = round(main_r["outer"], 2)
+    if other_r is not None:
+        key = "mixed_form" if other == "mixed" else "fp64_path"
+        line[key] = {"value": round(other_r["steps_s"], 4), "ms_per_step": round(other_r["ms_step"], 3),
+                     "dtype": "f32-inner/f64-refinement" if other == "mixed" else "f64",
+                     "pcg_iters_per_step": round(other_r["iters"], 2), "relres_max": other_r["relres_max"],
+                     "spmv": other_r["spmv"], "roofline": roofline(other_r, other),
+                     "step_roofline": other_r["step_roofline"]}
+        if other == "mixed":
+            line[key]["refinement_sweeps_per_step"] = round(other_r["outer"], 2)
+        mixed_r, fp64_r = (other_r, main_r) if other == "mixed" else (main_r, other_r)
+        line["mixed_speedup_vs_fp64"] = round(mixed_r["steps_s"] / fp64_r["steps_s"], 3)
+        line["mixed_rel_err_vs_fp64_path"] = {"value": err,
+                                              "at": f"displacement after step {args.warmup + 2 * args.steps} "
+                                                    f"of each form's run"}
     line["spmv_autotune"] = {"fp64": {"chunks_per_workgroup": v64[0], "lanes_per_row": v64[1]},
                              "fp32": shape_f32(v32),
                              "cold_us": {(f"{p}_L{ls}_R{r}" if ls else f"{p}_slotvalues_K{r}"): round(v * 1e3, 2)
@@ -449,14 +528,6 @@ def main():
         return [f"slots{1 << (c - 8)}" if c >= 8 else f"lanes{1 << c}" for c in codes]
     line["ic0_sweeps"] = {"contiguous_levels": bool(info[12]), "form_fwd": forms(info[13]),
                           "form_bwd": forms(info[14])}
-    if args.precision == "mixed":
-        line["refinement_sweeps_per_step"] = round(main_r["outer"], 2)
-    if fp64_r is not None:
-        line["fp64_path"] = {"value": round(fp64_r["steps_s"], 4), "ms_per_step": round(fp64_r["ms_step"], 3),
-                             "pcg_iters_per_step": round(fp64_r["iters"], 2), "relres_max": fp64_r["relres_max"],
-                             "spmv": fp64_r["spmv"]}
-        line["speedup_vs_fp64"] = round(main_r["steps_s"] / fp64_r["steps_s"], 3)
-        line["rel_err_vs_fp64_path"] = {"value": err, "at": f"displacement after step {args.warmup + 2 * args.steps}"}
     if not args.no_damping:
         line["plane_damping"] = damping_leg(ctx, mesh, w)
         log("plane damping leg done")
@@ -465,17 +536,15 @@ def main():
     if not args.no_batch:
         line["mid_neo_precision"] = mid_neo_leg(args)
         log("mid-size neo-Hookean leg done")
-        line["batched"] = batched_leg(args)
+        r64 = main_r if args.precision == "fp64" else other_r
+        line["batched"] = batched_leg(args, mesh.ndof * r64["steps_s"] if r64 else None)
         log("batched leg done")
     if not args.no_serving:
         line["serving"] = serving_leg(args)
         log("serving leg done")
-    if not args.no_cpu:
-        try:
-            it_ref = fp64_r["iters"] if fp64_r is not None else main_r["iters"]
-            line["cpu_baseline"] = cpu_baseline(args, it_ref, mesh.ndof)
-        except Exception as e:  # the GPU line stays valid without the CPU leg
-            line["cpu_baseline"] = {"value": None, "error": str(e)}
+    if cpu is not None:
+        log("waiting for the CPU baseline")
+        line["cpu_baseline"] = cpu.result()
     print(json.dumps(line), flush=True)
     if args.copy_calib:
         copy_calibration(mesh)
