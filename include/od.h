@@ -38,8 +38,17 @@ enum {
 
 /* material kinds per tet */
 enum { OD_MAT_COROTATIONAL = 0, OD_MAT_NEOHOOKEAN = 1, OD_MAT_COROT_PLASTIC = 2 };
-/* preconditioners (core/preconditioner.h:12-22 family) */
-enum { OD_PRE_NONE = 0, OD_PRE_JACOBI = 1, OD_PRE_IC0 = 3 };
+/* preconditioners (core/preconditioner.h:12-22 family):
+ *   OD_PRE_JACOBI  JacobiPreconditioner (solver/jacobiPreconditioner.cpp:6-26)
+ *   OD_PRE_ICT     InCholeskyPreconditioner, the reference's threshold IC
+ *                  (solver/inCholeskyPreconditioner.cpp:8-162): parameters epsilon and
+ *                  relaxedScale (default 1); factored on the host at every solve, applied by
+ *                  level-scheduled device sweeps; FP64 solvers only
+ *   OD_PRE_IC0     IC(0) on the block pattern of A, factored and applied on the device
+ *   OD_PRE_LDLT    InLDLTPreconditioner, SAINV incomplete LDL^T
+ *                  (solver/inLDLTPreconditioner.cpp:8-184): parameters sainvEpsilon and
+ *                  ldltEpsilon (default 0); host factor, device sweeps; FP64 solvers only */
+enum { OD_PRE_NONE = 0, OD_PRE_JACOBI = 1, OD_PRE_ICT = 2, OD_PRE_IC0 = 3, OD_PRE_LDLT = 4 };
 /* arithmetic of the inner PCG iterations */
 enum { OD_FP64 = 0, OD_MIXED = 1 };
 
@@ -206,6 +215,11 @@ int od_solver_set_refinement(od_solver *s, double innerTol, int maxOuter);
  * at once on the device; the iteration cap is checked on the device).
  * Ignored while od_ctx_profile is on. */
 int od_solver_set_graph(od_solver *s, int enable);
+/* Preconditioner parameters (the constructor arguments of InCholeskyPreconditioner(epsilon,
+ * relaxedScale), inCholeskyPreconditioner.h:14, and InLDLTPreconditioner(sainvEpsilon,
+ * ldltEpsilon), inLDLTPreconditioner.h:14-15; IC(0): the pivot epsilon, p2 unused).  The
+ * precondParam of od_solver_create / od_newmark_create sets p1 with p2 at its default. */
+int od_solver_set_precond_params(od_solver *s, double p1, double p2);
 /* info[4] (no reference counterpart): graph captures so far (one per new set of buffers or
  * launch forms), device bytes held by the solver, refinement sweeps and the largest
  * per-instance iteration count of the last solve */

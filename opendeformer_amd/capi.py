@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libodhip.so")
 _LIB = None
 
 OD_MAT_COROTATIONAL, OD_MAT_NEOHOOKEAN, OD_MAT_COROT_PLASTIC = 0, 1, 2
-OD_PRE_NONE, OD_PRE_JACOBI, OD_PRE_IC0 = 0, 1, 3
+OD_PRE_NONE, OD_PRE_JACOBI, OD_PRE_ICT, OD_PRE_IC0, OD_PRE_LDLT = 0, 1, 2, 3, 4
 OD_FP64, OD_MIXED = 0, 1
 OD_OK, OD_ERR_ARG, OD_ERR_HIP, OD_ERR_STATE, OD_ERR_UNSUPPORTED = 0, -1, -2, -3, -4
 
@@ -36,7 +36,7 @@ EXPORTS = [
     "od_server_create", "od_server_destroy", "od_server_add_object", "od_server_run", "od_server_object_state",
     "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain", "od_plane_damping",
     "od_integrator_add_plane_damping", "od_integrator_clear_damping", "od_integrator_damping", "od_relabel_gps",
-    "od_solver_info", "od_integrator_solver",
+    "od_solver_info", "od_integrator_solver", "od_solver_set_precond_params",
 ]
 
 
@@ -91,6 +91,7 @@ def lib():
             "od_spmv_timed": [vp, vp, vp, i, i, dp], "od_integrator_matrix": [vp, pvp],
             "od_ctx_profile": [vp, i], "od_ctx_profile_read": [vp, dp, C.POINTER(ll)],
             "od_solver_info": [vp, C.POINTER(ll)], "od_integrator_solver": [vp, pvp],
+            "od_solver_set_precond_params": [vp, d, d],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

@@ -282,6 +282,11 @@ class CGSolver:
     def set_graph(self, enable=True):
         check(lib().od_solver_set_graph(self.h, 1 if enable else 0))
 
+    def set_precond_params(self, p1, p2):
+        """InCholeskyPreconditioner(epsilon, relaxedScale) / InLDLTPreconditioner(sainvEpsilon,
+        ldltEpsilon) parameters; IC(0): the pivot epsilon."""
+        check(lib().od_solver_set_precond_params(self.h, float(p1), float(p2)))
+
     def info(self):
         """Graph captures, device bytes, refinement sweeps and largest iteration count of the
         last solve."""
@@ -351,6 +356,12 @@ class _Integrator:
     def set_graph(self, enable=True):
         """Replay PCG iterations from a captured HIP graph."""
         check(lib().od_integrator_set_graph(self.h, 1 if enable else 0))
+
+    def set_precond_params(self, p1, p2):
+        """od_solver_set_precond_params on the integrator's solver."""
+        sh = C.c_void_p()
+        check(lib().od_integrator_solver(self.h, C.byref(sh)))
+        check(lib().od_solver_set_precond_params(sh, float(p1), float(p2)))
 
     def solver_info(self):
         """od_solver_info of the integrator's solver (graph captures among others)."""

@@ -13,6 +13,7 @@
 
 #include "../../include/od.h"
 #include "od_internal.h"
+#include "od_precond.h"
 
 using namespace od;
 
@@ -168,10 +169,24 @@ struct od_matrix {
     }
 };
 
+// The reference's scalar preconditioners (threshold IC, SAINV LDL^T): factored on the host at
+// every solve from the downloaded values, swept on the device.
+struct ScalarPrecond {
+    ScalarFactor factor;
+    ScalarSweepsHost host;
+    ScalarSweepsDev dev;
+    std::vector<double> values;              // host copy of A's values
+    void *buf = nullptr;                     // device arrays of the sweeps, one allocation
+    size_t cap = 0;
+    ~ScalarPrecond() { if (buf) (void)hipFree(buf); }
+};
+
 struct od_solver {
     od_mesh *mesh = nullptr;
     int precond = OD_PRE_JACOBI;
-    double param = 1e-3;        // IC(0) pivot epsilon
+    double param = 1e-3;        // IC(0) pivot epsilon; threshold IC epsilon; SAINV epsilon
+    double param2 = 1.0;        // threshold IC relaxedScale; LDL^T entry epsilon
+    ScalarPrecond *scalar = nullptr;
     int precision = OD_FP64;
     double innerTol = 1e-4;     // mixed: inner FP32 relative tolerance (floor; see kInnerOuterScale2)
     int maxOuter = 12;
@@ -209,6 +224,7 @@ struct od_solver {
         for (auto &g : graph)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (void *ptr : allocs) (void)hipFree(ptr);
+        delete scalar;
     }
 };
 
@@ -373,17 +389,23 @@ static void autotune_spmv(od_mesh *mesh) {
     mesh->formGen++;
 }
 
+static bool scalar_precond(int precond) { return precond == OD_PRE_ICT || precond == OD_PRE_LDLT; }
+
 static od_solver *make_solver(od_mesh *mesh, int precond, double param, int precision) {
-    require(precond == OD_PRE_NONE || precond == OD_PRE_JACOBI || precond == OD_PRE_IC0, "unknown preconditioner",
-            OD_ERR_UNSUPPORTED);
+    require(precond == OD_PRE_NONE || precond == OD_PRE_JACOBI || precond == OD_PRE_IC0 || scalar_precond(precond),
+            "unknown preconditioner", OD_ERR_UNSUPPORTED);
     require(precision == OD_FP64 || precision == OD_MIXED, "unknown precision", OD_ERR_UNSUPPORTED);
+    require(!(scalar_precond(precond) && precision == OD_MIXED),
+            "the threshold IC and LDL^T preconditioners run in FP64 only", OD_ERR_UNSUPPORTED);
     autotune_spmv(mesh);
     od_solver *s = new od_solver();
     try {
         s->mesh = mesh;
         s->precond = precond;
-        s->param = param > 0 ? param : 1e-3;
+        s->param = scalar_precond(precond) ? param : (param > 0 ? param : 1e-3);
+        s->param2 = precond == OD_PRE_LDLT ? 0.0 : 1.0;
         s->precision = precision;
+        if (scalar_precond(precond)) s->scalar = new ScalarPrecond();
         s->sweepIters.assign(s->maxOuter, 0);
         const Topology &t = mesh->topo;
         size_t n = (size_t)t.ndof, nv = (size_t)t.nValues;
@@ -451,10 +473,73 @@ static int count_not_done(od_ctx *ctx, const int *doneDev, int nInst) {
 
 // Builds the preconditioner from A (Preconditioner::Preprocess, cgSolver.cpp:23).  The FP64
 // lower copy of the IC(0) factor is written when the FP64 sweeps will run (fp64Sweeps).
+// Threshold IC or SAINV LDL^T of A: the values come to the host, the factor and its sweep
+// schedule are built there, and the sweep arrays go back in one upload.
+static void build_scalar_precond(od_solver *s, const od_matrix *A) {
+    od_mesh *mesh = s->mesh;
+    hipStream_t st = mesh->ctx->stream;
+    ScalarPrecond &P = *s->scalar;
+    const Topology &t = mesh->topo;
+    P.values.resize((size_t)t.nValues);
+    OD_HIP(hipMemcpyAsync(P.values.data(), A->vals, sizeof(double) * (size_t)t.nValues, hipMemcpyDeviceToHost, st));
+    OD_HIP(hipStreamSynchronize(st));
+    if (s->precond == OD_PRE_ICT) thresholdIcFactor(t, P.values.data(), s->param, s->param2, P.factor);
+    else sainvLdltFactor(t, P.values.data(), s->param, s->param2, P.factor);
+    buildScalarSweeps(P.factor, s->precond == OD_PRE_LDLT, P.host);
+    const ScalarSweepsHost &h = P.host;
+    // one buffer: the int arrays, then (8-byte aligned) the double arrays
+    const std::vector<const std::vector<int> *> ints = {&h.fPtr, &h.fCol, &h.bPtr, &h.bRow, &h.fwd.levelPtr,
+                                                        &h.fwd.levelRows, &h.bwd.levelPtr, &h.bwd.levelRows};
+    const std::vector<const std::vector<double> *> dbls = {&h.fVal, &h.bVal, &h.diag};
+    size_t nInt = 0, nDbl = 0;
+    for (auto *v : ints) nInt += v->size();
+    for (auto *v : dbls) nDbl += v->size();
+    const size_t intBytes = (sizeof(int) * nInt + 255) & ~(size_t)255;
+    const size_t bytes = intBytes + sizeof(double) * nDbl + 256;
+    if (bytes > P.cap) {
+        OD_HIP(hipStreamSynchronize(st));
+        if (P.buf) OD_HIP(hipFree(P.buf));
+        P.buf = nullptr;
+        P.cap = 0;
+        OD_HIP(hipMalloc(&P.buf, bytes));
+        P.cap = bytes;
+    }
+    std::vector<char> staging(bytes, 0);
+    std::vector<const void *> devPtr;
+    size_t off = 0;
+    for (auto *v : ints) {
+        std::memcpy(staging.data() + off, v->data(), sizeof(int) * v->size());
+        devPtr.push_back((const char *)P.buf + off);
+        off += sizeof(int) * v->size();
+    }
+    off = intBytes;
+    for (auto *v : dbls) {
+        std::memcpy(staging.data() + off, v->data(), sizeof(double) * v->size());
+        devPtr.push_back((const char *)P.buf + off);
+        off += sizeof(double) * v->size();
+    }
+    OD_HIP(hipMemcpyAsync(P.buf, staging.data(), bytes, hipMemcpyHostToDevice, st));
+    OD_HIP(hipStreamSynchronize(st));   // the staging buffer is freed on return
+    ScalarSweepsDev &d = P.dev;
+    d.n = h.n;
+    d.unit = h.unit;
+    d.fPtr = (const int *)devPtr[0]; d.fCol = (const int *)devPtr[1];
+    d.bPtr = (const int *)devPtr[2]; d.bRow = (const int *)devPtr[3];
+    d.fLevelPtr = (const int *)devPtr[4]; d.fLevelRows = (const int *)devPtr[5];
+    d.bLevelPtr = (const int *)devPtr[6]; d.bLevelRows = (const int *)devPtr[7];
+    d.fVal = (const double *)devPtr[8]; d.bVal = (const double *)devPtr[9]; d.diag = (const double *)devPtr[10];
+    d.fPlan = &h.fwd;
+    d.bPlan = &h.bwd;
+}
+
 static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip, bool fp64Sweeps = true) {
     od_mesh *mesh = s->mesh;
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
+    if (s->scalar) {
+        build_scalar_precond(s, A);
+        return;
+    }
     if (s->precond == OD_PRE_JACOBI) {
         launch_jacobi_diag<double>(m, st, A->vals, s->invDiag);
         if (s->invDiag32) launch_jacobi_diag<float>(m, st, A->vals, s->invDiag32);
@@ -486,7 +571,12 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     else launch_spmv<T>(m, st, vals, x, q, 0, nullptr);
     pcg_init_residual<T>(m, st, b, q, r, w);
     const bool ic = s->precond == OD_PRE_IC0;
+    const bool sc64 = s->scalar != nullptr;   // threshold IC / LDL^T (FP64 solvers only)
+    auto scalar_apply = [&](const T *rr, T *zz) {
+        if constexpr (sizeof(T) == 8) launch_scalar_apply(s->scalar->dev, st, rr, zz);
+    };
     if (ic) launch_ic0_apply<T>(m, st, mesh->topo.levelPtr, Uf, Ul, r, z, nullptr, preDone);
+    else if (sc64) scalar_apply(r, z);
     else pcg_jacobi<T>(m, st, invDiag, r, z, w);
     pcg_dot_rz_init<T>(m, st, r, z, p, w, tol, maxIt, preDone);
     // Iterations are launched in batches between convergence polls (a poll synchronises the
@@ -505,6 +595,10 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
             pcg_update_xr_ic0<T>(m, st, mesh->topo.levelPtr, pNew, q, x, r, Uf, Ul, z, w, preDone);
             if (ic0_dot_fused(m, mesh->topo.levelPtr)) pcg_finalize_iter(m, st, w);
             else pcg_dot_rz<T>(m, st, r, z, w);
+        } else if (sc64) {
+            pcg_update_xr<T>(m, st, pNew, q, x, r, nullptr, nullptr, w);   // x, r and r.r only
+            scalar_apply(r, z);
+            pcg_dot_rz<T>(m, st, r, z, w);
         } else {
             pcg_update_xr<T>(m, st, pNew, q, x, r, invDiag, z, w);
         }
@@ -514,7 +608,8 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     // once per set of buffers and replayed; iterations past convergence exit at once.
     od_solver::IterGraph *g = nullptr;
     const int launchesPerIter = ic ? 2 * m.nLevels + 4 : 5;
-    if (s->useGraph && !mesh->ctx->prof && kGraphIters * launchesPerIter <= kGraphMaxLaunches) {
+    // the scalar preconditioners' sweep schedule changes with every factor: no graph
+    if (s->useGraph && !mesh->ctx->prof && !sc64 && kGraphIters * launchesPerIter <= kGraphMaxLaunches) {
         g = &s->graph[sizeof(T) == 8 ? 0 : 1];
         // the buffers and the launch forms (SpMV shapes, sweep form per level) baked into the
         // captured launches
@@ -1342,6 +1437,16 @@ int od_solver_set_refinement(od_solver *s, double innerTol, int maxOuter) {
     API_END
 }
 
+int od_solver_set_precond_params(od_solver *s, double p1, double p2) {
+    API_BEGIN
+    require(s != nullptr, "NULL argument");
+    require(p1 >= 0 && p2 >= 0, "preconditioner parameters must be non-negative");
+    if (s->precond == OD_PRE_IC0) require(p1 > 0, "the IC(0) pivot epsilon must be positive");
+    s->param = p1;
+    s->param2 = p2;
+    API_END
+}
+
 int od_solver_set_graph(od_solver *s, int enable) {
     API_BEGIN
     require(s != nullptr, "NULL argument");
@@ -1389,6 +1494,7 @@ int od_pcg_solve(od_solver *s, od_matrix *A, const double *b, double *x, double 
 int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
     API_BEGIN
     require(s && A && r && z && A->mesh == s->mesh, "bad arguments");
+    require(r != z, "r and z must differ");
     od_mesh *mesh = s->mesh;
     if (s->precond == OD_PRE_IC0 && !s->Ul)
         s->Ul = (double *)s->alloc(8 * (9 * (size_t)mesh->topo.tRow.size() + kValuePad));
@@ -1396,6 +1502,7 @@ int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
     PcgWork w{s->partials, s->scIn, s->doneIn};
     if (s->precond == OD_PRE_IC0)
         launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U, s->Ul, r, z, nullptr, nullptr);
+    else if (s->scalar) launch_scalar_apply(s->scalar->dev, mesh->ctx->stream, r, z);
     else pcg_jacobi<double>(mesh->dm, mesh->ctx->stream, s->precond == OD_PRE_JACOBI ? s->invDiag : nullptr, r, z, w);
     OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
     API_END

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "od_precond.h"
 #include "od_setup.h"
 
 namespace od {
@@ -217,5 +218,18 @@ struct DampScratch {
 void launch_plane_damping(const DevMesh &m, hipStream_t s, int nPairs, const DampScratch &w, double coeff,
                           const double *disp, const double *vel, double *D);
 void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDone, const uint8_t *instActive);
+
+// ---- scalar preconditioner sweeps (od_precond.hip) ----------------------------------
+// Device copy of a ScalarSweepsHost (arrays owned by the solver).
+struct ScalarSweepsDev {
+    int n = 0;
+    bool unit = false;
+    const int *fPtr = nullptr, *fCol = nullptr, *bPtr = nullptr, *bRow = nullptr;
+    const double *fVal = nullptr, *bVal = nullptr, *diag = nullptr;
+    const int *fLevelPtr = nullptr, *fLevelRows = nullptr, *bLevelPtr = nullptr, *bLevelRows = nullptr;
+    const ScalarSweepPlan *fPlan = nullptr, *bPlan = nullptr;   // host memory
+};
+// z = (L D L^T)^-1 r or (L L^T)^-1 r; z may not alias r.
+void launch_scalar_apply(const ScalarSweepsDev &s, hipStream_t st, const double *r, double *z);
 
 }  // namespace od
