@@ -226,6 +226,16 @@ int od_solver_set_precond_params(od_solver *s, double p1, double p2);
 int od_solver_info(od_solver *s, long long *info_host);
 int od_pcg_solve(od_solver *s, od_matrix *A, const double *b_dev, double *x_dev, double tol, int maxIter,
                  int *iters_host, double *relres_host);
+/* od_pcg_solve without any allocation (no reference counterpart: for callers that cannot
+ * allocate device memory, such as code under HIP graph capture or tight serving loops).  The
+ * solve's temporaries (five DOF vectors, chunk partials, per-instance scalars and flags) live
+ * in a fixed 64 MiB device array per GPU, held until the process ends and looked up when a
+ * context is created on that GPU.  Limits: FP64, OD_PRE_NONE or OD_PRE_JACOBI, at most 64
+ * instances, 5 x 8 x ndof bytes plus the partials within 64 MiB (about 1.6M DOF); larger
+ * systems are rejected with OD_ERR_ARG.  Same results as od_pcg_solve with the same
+ * preconditioner.  Calls that overlap on one GPU share the array and race. */
+int od_pcg_solve_noalloc(od_matrix *A, int precond, const double *b_dev, double *x_dev, double tol, int maxIter,
+                         int *iters_host, double *relres_host);
 /* Preconditioner::Preprocess + solvePreconditionerSystem once: z = P(A)^-1 r */
 int od_precond_apply(od_solver *s, od_matrix *A, const double *r_dev, double *z_dev);
 /* IC(0) factor of A in the canonical layout (U = L^T with U^T U ~ A). */

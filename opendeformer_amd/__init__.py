@@ -20,5 +20,6 @@ def build():
 from . import capi, mesh, workloads  # noqa: E402,F401
 from .core import (  # noqa: E402,F401
     BackwardEulerIntegrator, CGSolver, Context, DeviceArray, Matrix, Mesh, NewmarkIntegrator, Server,
-    generate_matrix_and_virtual_works, mesh_autotune, mesh_spmv_variant, mesh_sweep_form, spmdv, spmdv_f32,
+    generate_matrix_and_virtual_works, mesh_autotune, mesh_spmv_variant, mesh_sweep_form, pcg_solve_noalloc, spmdv,
+    spmdv_f32,
 )

@@ -36,7 +36,7 @@ EXPORTS = [
     "od_server_create", "od_server_destroy", "od_server_add_object", "od_server_run", "od_server_object_state",
     "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain", "od_plane_damping",
     "od_integrator_add_plane_damping", "od_integrator_clear_damping", "od_integrator_damping", "od_relabel_gps",
-    "od_solver_info", "od_integrator_solver", "od_solver_set_precond_params",
+    "od_solver_info", "od_integrator_solver", "od_solver_set_precond_params", "od_pcg_solve_noalloc",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
             "od_ctx_profile": [vp, i], "od_ctx_profile_read": [vp, dp, C.POINTER(ll)],
             "od_solver_info": [vp, C.POINTER(ll)], "od_integrator_solver": [vp, pvp],
             "od_solver_set_precond_params": [vp, d, d],
+            "od_pcg_solve_noalloc": [vp, i, vp, vp, d, i, ip, dp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

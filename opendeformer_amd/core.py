@@ -305,6 +305,16 @@ def _solver_info(h):
             "iters_max": int(info[3])}
 
 
+def pcg_solve_noalloc(A: Matrix, b: DeviceArray, x: DeviceArray, precond=capi.OD_PRE_JACOBI, tol=1e-6,
+                      max_iter=10000):
+    """od_pcg_solve_noalloc: the FP64 PCG solve (Jacobi or no preconditioner) with every
+    temporary in the library's fixed device scratch; x is the warm start and the result."""
+    n = A.mesh.n_inst
+    it = np.zeros(n, np.int32); rr = np.zeros(n)
+    check(lib().od_pcg_solve_noalloc(A.h, int(precond), b.ptr, x.ptr, tol, int(max_iter), capi.iptr(it), dptr(rr)))
+    return it, rr
+
+
 class _Integrator:
     def __del__(self):
         try:

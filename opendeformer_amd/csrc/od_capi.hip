@@ -43,6 +43,25 @@ __global__ void k_done_to_mask(int n, const InstScalars *sc, const uint8_t *extr
 }
 }  // namespace
 
+// ---- fixed scratch of od_pcg_solve_noalloc ---------------------------------------------
+// A file-scope device array (one copy per GPU, held until the process ends) that carries all
+// the temporaries of the allocation-free PCG solve.  Host code never passes the array's name
+// to a kernel (on the host it is only a stand-in symbol): the device address of each GPU's
+// copy is looked up once per device with hipGetSymbolAddress when a context is created on it,
+// and that address is what the solve hands to its kernels.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::atomic<unsigned char *> g_scratchDev[kMaxDevices];
+
+// Called with `device` current.
+static void scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices || g_scratchDev[device].load()) return;
+    void *p = nullptr;
+    OD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratchDev[device].store((unsigned char *)p);
+}
+
 struct od_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -747,6 +766,7 @@ int od_ctx_create(int device, od_ctx **out) {
     OD_HIP(hipGetDeviceCount(&count));
     require(device >= 0 && device < count, "device index out of range");
     OD_HIP(hipSetDevice(device));
+    scratch_setup(device);
     od_ctx *c = new od_ctx();
     c->device = device;
     OD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1488,6 +1508,67 @@ int od_pcg_solve(od_solver *s, od_matrix *A, const double *b, double *x, double 
     int nInst = s->mesh->topo.nInst;
     if (iters) std::memcpy(iters, s->iters.data(), sizeof(int) * nInst);
     if (relres) std::memcpy(relres, s->relres.data(), sizeof(double) * nInst);
+    API_END
+}
+
+__global__ static void k_fill_bytes(uint8_t *p, int n, uint8_t v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Instances of a system the allocation-free solve accepts: its per-instance scalars come back
+// through a fixed host array and the context's pinned mailbox (made for 64 at context creation).
+constexpr int kNoAllocMaxInst = 64;
+
+int od_pcg_solve_noalloc(od_matrix *A, int precond, const double *b, double *x, double tol, int maxIter, int *iters,
+                         double *relres) {
+    API_BEGIN
+    require(A && b && x, "NULL argument");
+    require(tol >= 0 && maxIter >= 0, "tol and maxIter must be non-negative");
+    require(precond == OD_PRE_NONE || precond == OD_PRE_JACOBI,
+            "the allocation-free solve takes OD_PRE_NONE or OD_PRE_JACOBI", OD_ERR_UNSUPPORTED);
+    od_mesh *mesh = A->mesh;
+    od_ctx *ctx = mesh->ctx;
+    const Topology &t = mesh->topo;
+    require(t.nInst <= kNoAllocMaxInst && t.nInst <= ctx->pinnedCount,
+            "the allocation-free solve takes at most 64 instances", OD_ERR_UNSUPPORTED);
+    unsigned char *base = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratchDev[ctx->device].load() : nullptr;
+    require(base != nullptr, "the device's scratch was not set up", OD_ERR_STATE);
+    // layout of the scratch: r, z, p, q, D^-1 (ndof doubles each), the chunk partials, the
+    // per-instance scalars, done flags and skip mask, each 256-byte aligned
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t vb = sizeof(double) * (size_t)t.ndof;
+    od_solver s;   // a view of the scratch: holds no device memory of its own
+    s.mesh = mesh;
+    s.precond = precond;
+    s.precision = OD_FP64;
+    s.r = (double *)take(vb); s.z = (double *)take(vb); s.p = (double *)take(vb); s.q = (double *)take(vb);
+    s.invDiag = (double *)take(vb);
+    s.partials = (double *)take(sizeof(double) * 3 * (size_t)t.chunkRow0.size());
+    s.scIn = (InstScalars *)take(sizeof(InstScalars) * (size_t)t.nInst);
+    s.doneIn = (int *)take(sizeof(int) * (size_t)t.nInst);
+    s.zeroMask = (uint8_t *)take((size_t)t.nInst);
+    require(off <= kNoAllocScratchBytes,
+            "the system is too large for the allocation-free solve's 64 MiB scratch (about 1.6M DOF at most)");
+    hipStream_t st = ctx->stream;
+    // the skip mask starts at zero (a fill kernel, so the call can be captured in a graph)
+    hipLaunchKernelGGL(k_fill_bytes, dim3((t.nInst + 255) / 256), dim3(256), 0, st, s.zeroMask, t.nInst, (uint8_t)0);
+    OD_HIP(hipGetLastError());
+    build_precond(&s, A, s.zeroMask);
+    run_pcg<double>(&s, A->vals, nullptr, nullptr, precond == OD_PRE_JACOBI ? s.invDiag : nullptr, b, x, s.r, s.z, s.p,
+                    s.q, tol, maxIter, s.zeroMask, s.scIn, s.doneIn, false, 0);
+    InstScalars h[kNoAllocMaxInst];
+    OD_HIP(hipMemcpyAsync(h, s.scIn, sizeof(InstScalars) * (size_t)t.nInst, hipMemcpyDeviceToHost, st));
+    OD_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < t.nInst; i++) {
+        if (iters) iters[i] = h[i].it;
+        if (relres) relres[i] = h[i].bb > 0 ? std::sqrt(h[i].rr / h[i].bb) : std::sqrt(h[i].rr);
+    }
     API_END
 }
 
