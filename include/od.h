@@ -236,6 +236,12 @@ int od_pcg_solve(od_solver *s, od_matrix *A, const double *b_dev, double *x_dev,
  * preconditioner.  Calls that overlap on one GPU share the array and race. */
 int od_pcg_solve_noalloc(od_matrix *A, int precond, const double *b_dev, double *x_dev, double tol, int maxIter,
                          int *iters_host, double *relres_host);
+/* Preconditioner::Preprocess(mat) (core/preconditioner.h:18): builds the factor of A (IC(0),
+ * threshold IC, SAINV LDL^T) or D^-1 (Jacobi); and solvePreconditionerSystem(width, rhs,
+ * result) (:14): z = P^-1 r with the factor of the last Preprocess.  r and z are device
+ * vectors of ndof; od_precond_solve synchronises. */
+int od_precond_preprocess(od_solver *s, od_matrix *A);
+int od_precond_solve(od_solver *s, const double *r_dev, double *z_dev);
 /* Preconditioner::Preprocess + solvePreconditionerSystem once: z = P(A)^-1 r */
 int od_precond_apply(od_solver *s, od_matrix *A, const double *r_dev, double *z_dev);
 /* IC(0) factor of A in the canonical layout (U = L^T with U^T U ~ A). */
@@ -309,6 +315,12 @@ int od_plane_damping(od_matrix *D, int nPairs, const double *planes_host, const 
 int od_integrator_add_plane_damping(od_integrator *it, int nPairs, const double *planes_host, const int *tets_host,
                                     double coeff);
 int od_integrator_clear_damping(od_integrator *it);
+/* The integrator's damping matrix as a matrix handle, for a StaDampingForcer's
+ * addDampingMatrix(matrixIndices, mat) (core/forcer.h:22-26), which the reference's
+ * addExternalDampingForceMatrix hands its damping matrix (nonlinearImplicitBackwardEuler.h:
+ * 122-124): allocated (zero) on the first call, owned by the integrator, in every later step's
+ * tangent until od_integrator_clear_damping.  Newmark: OD_ERR_UNSUPPORTED. */
+int od_integrator_damping_matrix(od_integrator *it, od_matrix **D);
 /* the integrator's damping matrix values (canonical layout; zeros if none) */
 int od_integrator_damping(od_integrator *it, double *values_host);
 

@@ -1,36 +1,54 @@
-// od_gpu.hpp — C++ host classes over libodhip.so that keep the reference's
-// solver, preconditioner, material and integrator interfaces, so that host
-// code written against them switches to the GPU path by changing types only.
+// od_gpu.hpp — the reference's host interfaces, re-declared with the reference's signatures,
+// and GPU implementations of them over libodhip.so (od.h).  Host code written against the
+// reference's solver, preconditioner, material, integrator and simulator interfaces switches to
+// the GPU path by changing the concrete types it constructs.
 //
-// Reference interfaces mirrored (paths relative to OpenDeformer/):
-//   LinearSolver / CGSolver ............ core/linearSolver.h:12-20, solver/cgSolver.h:14-30
-//   Preconditioner family .............. core/preconditioner.h:12-22,
-//                                        solver/jacobiPreconditioner.h:10-22,
-//                                        solver/inCholeskyPreconditioner.h:11-32
-//   BlockedSymSpMatrix + SpMDV ......... core/sparseMatrix.h:186-295, 627-639;
-//                                        core/numerMethod.h:134-136
-//   FullOrderNonlinearMaterial ......... material/fullOrderNonlinear.h:36-44,
-//                                        material/corotationalHyperelastic.h:11-17,
-//                                        material/invertibleNeoHookean.h:11-16
-//   Intergrator ........................ core/intergrator.h:13-35,
-//                                        intergrator/nonlinearImplicitBackwardEuler.h:16-47
-//   NodeIndexer ........................ core/nodeIndexer.cpp:8-74
+// Interfaces re-declared (paths relative to OpenDeformer/ in the reference tree):
+//   ReferenceCounted, Reference<T> ........ core/memory.h:18-76
+//   Mesh (accessors), TetMesh .............. core/mesh.h:15-64
+//   Constrainer, NodeIndexer ............... core/constrainer.h:16-36, core/nodeIndexer.h:12-29
+//   Forcer, StaDampingForcer ............... core/forcer.h:13-26
+//   BlockedSymSpMatrix (+ Assembler), SpMDV  core/sparseMatrix.h:186-295, 620-666;
+//                                            core/numerMethod.h:134-136
+//   SparseSymMatrixIndicesPerElementCache .. core/sparseMatrix.h:650-666
+//   LinearSolver<SpMatrix> ................. core/linearSolver.h:12-20
+//   Preconditioner ......................... core/preconditioner.h:12-22
+//   FullOrderNonlinearMaterialCache,
+//   FullOrderNonlinearMaterial<SpMatrix> ... material/fullOrderNonlinear.h:14-44
+//   MechMaterial::generateMassMatrix ....... core/mechMaterial.h:29-32
+//   Intergrator ............................ core/intergrator.h:13-35 (the damping hook takes a
+//                                            StaDampingForcer, as the override at
+//                                            nonlinearImplicitBackwardEuler.h:23 does)
+//   Simulator .............................. core/simulator.h:12-28, core/simulator.cpp:24-38
+//   StaticPlaneShape,
+//   MeshPlaneCollisionDampingForcer ........ collision/collisionShapes.h:17-20,
+//                                            collision/penaltyDampingForcer.h:25-38
 //
-// Semantics kept from the reference:
-//   - host pointers in and out, DOF vectors over free DOFs;
-//   - generateMatrixAndVirtualWorks and SpMDV accumulate into their outputs;
-//   - solveLinearSystem treats `result` as the initial guess and stops when
-//     ||r|| <= tol ||b|| or after maxIteration iterations;
-//   - errors are reported, not thrown: the reference's Severe prints and
-//     continues (core/oder.h:117-119); here Severe prints the library's error
-//     text to stderr and the object records the failure in lastStatus().
-// Differences: the matrix values live on the device and are mirrored to the
-// host on demand (hostValues()/markHostDirty()); materials may differ per tet.
+// GPU implementations: GpuCGSolver : LinearSolver<BlockedSymSpMatrix>; GpuJacobiPreconditioner,
+// GpuIC0Preconditioner, GpuInCholeskyPreconditioner, GpuInLDLTPreconditioner : Preconditioner;
+// GpuCorotationalMaterial, GpuNeoHookeanMaterial, GpuPerTetMaterial :
+// FullOrderNonlinearMaterial<BlockedSymSpMatrix>; GpuNewmark,
+// GpuBackwardEuler : Intergrator.  The reference's class names are aliases of them.
+//
+// Semantics kept from the reference: host pointers in and out, DOF vectors over free DOFs;
+// generateMatrixAndVirtualWorks, generateMassMatrix and SpMDV accumulate into their outputs;
+// solveLinearSystem treats `result` as the initial guess and stops when ||r|| <= tol ||b|| or
+// after maxIteration iterations; the preconditioner is rebuilt from the matrix at every solve;
+// errors are reported, not thrown (the reference's Severe prints and continues,
+// core/oder.h:117-119): here they are printed with the library's error text and recorded in
+// lastStatus().  Differences: the device runs one HIP stream per process-wide Device; matrix
+// values live on the device and are mirrored to the host on demand (hostValues /
+// markHostDirty); constraints must cover whole nodes; materials may differ per tet.
 #pragma once
+#include <algorithm>
+#include <atomic>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <set>
+#include <utility>
 #include <vector>
 
 #include "od.h"
@@ -44,150 +62,392 @@ inline int Severe(int status, const char *where) {
     return status;
 }
 
-// Owns an od_ctx (one HIP stream on one device).
-class Device {
+// ---- reference counting (core/memory.h:18-76) ------------------------------------------
+class ReferenceCounted {
+protected:
+    ReferenceCounted() : nReference(0) {}
+    std::atomic<int> nReference;
+    template <class T> friend class Reference;
+};
+
+template <class T> class Reference {
 public:
-    explicit Device(int device = 0) { status_ = Severe(od_ctx_create(device, &ctx_), "od_ctx_create"); }
-    ~Device() { if (ctx_) od_ctx_destroy(ctx_); }
-    Device(const Device &) = delete;
-    Device &operator=(const Device &) = delete;
-    od_ctx *get() const { return ctx_; }
-    int lastStatus() const { return status_; }
+    Reference(T *ptr = nullptr) noexcept : p(ptr) { if (p) p->nReference.fetch_add(1, std::memory_order_relaxed); }
+    Reference(const Reference &r) noexcept : p(r.p) { if (p) p->nReference.fetch_add(1, std::memory_order_relaxed); }
+    Reference(Reference &&r) noexcept : p(r.p) { r.p = nullptr; }
+    template <class U> Reference(const Reference<U> &r) noexcept : p(r.get()) {
+        if (p) p->nReference.fetch_add(1, std::memory_order_relaxed);
+    }
+    Reference &operator=(T *ptr) noexcept {
+        if (ptr) ptr->nReference.fetch_add(1, std::memory_order_relaxed);
+        release();
+        p = ptr;
+        return *this;
+    }
+    Reference &operator=(const Reference &r) noexcept { return *this = r.p; }
+    Reference &operator=(Reference &&r) noexcept {
+        std::swap(p, r.p);
+        return *this;
+    }
+    ~Reference() { release(); }
+    T &operator*() const noexcept { return *p; }
+    T *operator->() noexcept { return p; }
+    const T *operator->() const noexcept { return p; }
+    operator bool() const noexcept { return p != nullptr; }
+    T *get() const noexcept { return p; }
 
 private:
+    void release() noexcept {
+        if (p && p->nReference.fetch_sub(1, std::memory_order_acq_rel) == 1) delete p;
+    }
+    T *p;
+};
+
+struct Vector3 {
+    Scalar x = 0, y = 0, z = 0;
+    Vector3() = default;
+    Vector3(Scalar a, Scalar b, Scalar c) : x(a), y(b), z(c) {}
+    Scalar operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    Scalar &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    Vector3 operator+(const Vector3 &o) const { return Vector3(x + o.x, y + o.y, z + o.z); }
+};
+struct Vector3f {
+    float x = 0, y = 0, z = 0;
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+// ---- the device (one context, one HIP stream) ---------------------------------------------
+// The process-wide device the classes below run on (device 0 unless selectDevice is called
+// before first use).
+class Device {
+public:
+    static Device &instance() {
+        static Device d(selected());
+        return d;
+    }
+    static void selectDevice(int device) { selected() = device; }
+    od_ctx *get() const { return ctx_; }
+    int lastStatus() const { return status_; }
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+    ~Device() { if (ctx_) od_ctx_destroy(ctx_); }
+
+private:
+    explicit Device(int device) { status_ = Severe(od_ctx_create(device, &ctx_), "od_ctx_create"); }
+    static int &selected() {
+        static int d = 0;
+        return d;
+    }
     od_ctx *ctx_ = nullptr;
     int status_ = OD_OK;
 };
 
-// Device buffer of T, sized on first use.
+// Device buffer of T, grown on demand.
 template <typename T> class DeviceBuffer {
 public:
-    explicit DeviceBuffer(const Device &d) : dev_(d) {}
-    ~DeviceBuffer() { if (p_) od_dev_free(dev_.get(), p_); }
+    DeviceBuffer() = default;
+    ~DeviceBuffer() { if (p_) od_dev_free(Device::instance().get(), p_); }
     DeviceBuffer(const DeviceBuffer &) = delete;
     DeviceBuffer &operator=(const DeviceBuffer &) = delete;
     T *ensure(long long n) {
         if (n <= n_) return p_;
-        if (p_) od_dev_free(dev_.get(), p_);
+        od_ctx *c = Device::instance().get();
+        if (p_) od_dev_free(c, p_);
         p_ = nullptr;
         n_ = 0;
         void *q = nullptr;
-        if (Severe(od_dev_alloc(dev_.get(), n * (long long)sizeof(T), &q), "od_dev_alloc") == OD_OK) {
+        if (Severe(od_dev_alloc(c, n * (long long)sizeof(T), &q), "od_dev_alloc") == OD_OK) {
             p_ = (T *)q;
             n_ = n;
         }
         return p_;
     }
     int upload(const T *host, long long n) {
-        if (!ensure(n)) return OD_ERR_HIP;
-        return Severe(od_copy_h2d(dev_.get(), p_, host, n * (long long)sizeof(T)), "od_copy_h2d");
+        if (!ensure(std::max<long long>(n, 1))) return OD_ERR_HIP;
+        return Severe(od_copy_h2d(Device::instance().get(), p_, host, n * (long long)sizeof(T)), "od_copy_h2d");
     }
     int download(T *host, long long n) const {
-        return Severe(od_copy_d2h(dev_.get(), host, p_, n * (long long)sizeof(T)), "od_copy_d2h");
+        return Severe(od_copy_d2h(Device::instance().get(), host, p_, n * (long long)sizeof(T)), "od_copy_d2h");
     }
     T *get() const { return p_; }
 
 private:
-    const Device &dev_;
     T *p_ = nullptr;
     long long n_ = 0;
 };
 
-// Tet mesh, its DOF indexing and its canonical matrix structure.  Replaces the
-// set-up of Mesh + NodeIndexer + getMatrixStructure/getMatrixIndicesPerElement.
-// `constrains` is Constrainer::outputConstrains' set of DOF ids (3*node+axis).
-class TetMesh {
+// ---- mesh (core/mesh.h:15-64) ------------------------------------------------------------
+class Mesh : public ReferenceCounted {
 public:
-    TetMesh(const Device &d, int nNodes, const Scalar *X, int nTets, const int *tets, const std::set<int> &constrains)
-        : dev_(d), nNodes_(nNodes), nTets_(nTets), constrained_(constrains.begin(), constrains.end()),
-          u_(3 * (size_t)nNodes, 0.0) {
-        status_ = Severe(od_mesh_create(d.get(), nNodes, X, nTets, tets, (int)constrained_.size(),
-                                        constrained_.data(), 1, nullptr, nullptr, &mesh_),
-                         "od_mesh_create");
-        if (status_ == OD_OK) {
-            long long info[16] = {0};
-            od_mesh_info(mesh_, info);
-            order_ = (int)info[0];
-        }
+    Mesh(int numN, int numE, int numS, int numNPE, int numVPS = 3)
+        : numNodes(numN), numElements(numE), numSurfaces(numS), numNodesPerElement(numNPE), numVertPerSur(numVPS),
+          vertices((size_t)numN), elements((size_t)numE * numNPE), surfaces((size_t)numS * numVPS),
+          displacements((size_t)numN) {}
+    Mesh(const Mesh &) = delete;
+    Mesh &operator=(const Mesh &) = delete;
+    virtual ~Mesh() = default;
+    int getNodeCount() const { return numNodes; }
+    int getNodePerElementCount() const { return numNodesPerElement; }
+    int getElementCount() const { return numElements; }
+    int getVertPerFacetCount() const { return numVertPerSur; }
+    int getFacetCount() const { return numSurfaces; }
+    const Vector3 &getVertex(int vertIndex) const { return vertices[vertIndex]; }
+    Vector3 &getVertexDisplacement(int vertIndex) { return displacements[vertIndex]; }
+    const Vector3 &getVertexDisplacementConst(int vertIndex) const { return displacements[vertIndex]; }
+    void setVertex(int vertIndex, const Vector3 &v) { vertices[vertIndex] = v; }
+    int getElementNodeIndex(int elementIndex, int nodeIndex) { return elements[elementIndex * numNodesPerElement + nodeIndex]; }
+    const int *getElementNodeReference(int elementIndex) const { return &elements[(size_t)elementIndex * numNodesPerElement]; }
+    const int *getFacetVertReference(int facetIndex) const { return &surfaces[(size_t)facetIndex * numVertPerSur]; }
+    void setElement(int elementIndex, const int *nodeIndices) {
+        for (int i = 0; i < numNodesPerElement; i++) elements[(size_t)elementIndex * numNodesPerElement + i] = nodeIndices[i];
     }
-    ~TetMesh() { if (mesh_) od_mesh_destroy(mesh_); }
-    TetMesh(const TetMesh &) = delete;
-    TetMesh &operator=(const TetMesh &) = delete;
+    void setFacet(int facetIndex, const int *facetIndices) {
+        for (int i = 0; i < numVertPerSur; i++) surfaces[(size_t)facetIndex * numVertPerSur + i] = facetIndices[i];
+    }
 
-    // NodeIndexer::getMatrixOrder / getGlobalIndex (core/nodeIndexer.cpp:8-25)
-    int getMatrixOrder() const { return order_; }
-    int getGlobalIndex(int node, int axis) const {
-        const int dof = 3 * node + axis;
-        size_t lo = 0, hi = constrained_.size();
-        while (lo < hi) {
-            size_t mid = (lo + hi) / 2;
-            if (constrained_[mid] < dof) lo = mid + 1; else hi = mid;
+private:
+    int numNodes, numElements, numSurfaces;
+    const int numNodesPerElement, numVertPerSur;
+    std::vector<Vector3> vertices;
+    std::vector<int> elements, surfaces;
+    std::vector<Vector3> displacements;
+};
+
+// A tetrahedral mesh from arrays (the reference's TetMesh is read from files): 3 coordinates
+// per node, 4 nodes per tet, optionally 3 vertices per boundary facet.
+class TetMesh : public Mesh {
+public:
+    TetMesh(int nNodes, const Scalar *X, int nTets, const int *tets, int nFacets = 0, const int *facets = nullptr)
+        : Mesh(nNodes, nTets, nFacets, 4, 3) {
+        for (int n = 0; n < nNodes; n++) setVertex(n, Vector3(X[3 * n], X[3 * n + 1], X[3 * n + 2]));
+        for (int e = 0; e < nTets; e++) setElement(e, tets + 4 * (size_t)e);
+        for (int f = 0; f < nFacets; f++) setFacet(f, facets + 3 * (size_t)f);
+    }
+};
+
+// ---- constraints and DOF indexing (core/constrainer.h:16-36, core/nodeIndexer.h:12-29) ---------
+enum ConstrainType { Constrain_X = 1, Constrain_Y = 2, Constrain_Z = 4, Constrain_XY = 3, Constrain_YZ = 6,
+                     Constrain_XZ = 5, Constrain_XYZ = 7 };
+
+class Constrainer {
+public:
+    Constrainer(const Reference<Mesh> &m) : mesh(m) {}
+    void setConstrains(int nodeCounts, int *nodeIndices, ConstrainType *constrainTypes = nullptr) {
+        for (int i = 0; i < nodeCounts; i++) setConstrain(nodeIndices[i], constrainTypes ? constrainTypes[i] : Constrain_XYZ);
+    }
+    // every vertex of a facet with |(v - o) . n| < 1e-4 (core/constrainer.cpp:57-72)
+    void setConstrains(const Vector3 &origin, const Vector3 &normal, ConstrainType constrainType = Constrain_XYZ) {
+        const int vpf = mesh->getVertPerFacetCount();
+        for (int f = 0; f < mesh->getFacetCount(); f++) {
+            const int *v = mesh->getFacetVertReference(f);
+            bool on = true;
+            for (int k = 0; k < vpf && on; k++) {
+                const Vector3 &p = mesh->getVertex(v[k]);
+                const Scalar d = (p.x - origin.x) * normal.x + (p.y - origin.y) * normal.y + (p.z - origin.z) * normal.z;
+                on = std::fabs(d) < 1e-4;
+            }
+            if (on)
+                for (int k = 0; k < vpf; k++) setConstrain(v[k], constrainType);
         }
-        if (lo < constrained_.size() && constrained_[lo] == dof) return -1;
-        return dof - (int)lo;
     }
-    int getNodeCount() const { return nNodes_; }
-    int getTetCount() const { return nTets_; }
-    // Mesh displacement state (core/mesh.h:37-39): 3 per node.
-    Scalar *getDisplacements() { return u_.data(); }
-    const Scalar *getDisplacements() const { return u_.data(); }
-    // free-DOF compaction of the node displacements
-    void gatherFree(const Scalar *perNode, Scalar *freeDofs) const {
-        for (int n = 0; n < nNodes_; n++)
+    void outputConstrains(std::vector<int> &out) const { out.assign(constrains.begin(), constrains.end()); }
+    const Reference<Mesh> &getMesh() const { return mesh; }
+
+private:
+    void setConstrain(int nodeIndex, ConstrainType t) {
+        for (int a = 0; a < 3; a++)
+            if (t & (1 << a)) constrains.insert(3 * nodeIndex + a);
+    }
+    std::set<int> constrains;
+    Reference<Mesh> mesh;
+};
+
+class BlockedSymSpMatrix;
+
+class NodeIndexer : public ReferenceCounted {
+public:
+    typedef std::vector<int>::const_iterator ConstrainIter;
+    NodeIndexer(const Constrainer &constrainer) : mesh_(constrainer.getMesh()) { constrainer.outputConstrains(constrainIndices); }
+    ~NodeIndexer() { if (dev_) od_mesh_destroy(dev_); }
+    // core/nodeIndexer.cpp:12-25: -1 for a constrained DOF, else its compacted index
+    int getGlobalIndex(int nodeIndex, int axis) const {
+        const int dof = 3 * nodeIndex + axis;
+        auto p = std::lower_bound(constrainIndices.begin(), constrainIndices.end(), dof);
+        if (p != constrainIndices.end() && *p == dof) return -1;
+        return dof - (int)(p - constrainIndices.begin());
+    }
+    void getElementNodesGlobalIndices(const Reference<Mesh> &mesh, int elementIndex, int nodeCounts, int *nodeIndices) const {
+        const int *e = mesh->getElementNodeReference(elementIndex);
+        for (int i = 0; i < nodeCounts; i++)
+            for (int a = 0; a < 3; a++) nodeIndices[3 * i + a] = getGlobalIndex(e[i], a);
+    }
+    int getMatrixOrder(const Reference<Mesh> &mesh) const { return 3 * mesh->getNodeCount() - (int)constrainIndices.size(); }
+    ConstrainIter getConstrainIterBegin() const { return constrainIndices.begin(); }
+    ConstrainIter getConstrainIterEnd() const { return constrainIndices.end(); }
+
+    // The device copy of (mesh, constraints): pattern, maps, level schedule (od_mesh_create),
+    // built on first use and shared by every matrix, material and integrator of this indexer.
+    od_mesh *deviceMesh() const {
+        if (!dev_) {
+            const int nn = mesh_->getNodeCount(), ne = mesh_->getElementCount();
+            std::vector<Scalar> X(3 * (size_t)nn);
+            for (int n = 0; n < nn; n++)
+                for (int a = 0; a < 3; a++) X[3 * (size_t)n + a] = mesh_->getVertex(n)[a];
+            std::vector<int> tets(4 * (size_t)ne);
+            for (int e = 0; e < ne; e++) std::memcpy(&tets[4 * (size_t)e], mesh_->getElementNodeReference(e), 4 * sizeof(int));
+            status_ = Severe(od_mesh_create(Device::instance().get(), nn, X.data(), ne, tets.data(),
+                                            (int)constrainIndices.size(), constrainIndices.data(), 1, nullptr, nullptr,
+                                            &dev_),
+                             "od_mesh_create");
+        }
+        return dev_;
+    }
+    // free-DOF vector of a per-node quantity, and back (constrained DOFs read as 0)
+    void gatherFree(const Reference<Mesh> &mesh, const Vector3 *perNode, Scalar *freeDofs) const {
+        for (int n = 0; n < mesh->getNodeCount(); n++)
             for (int a = 0; a < 3; a++) {
-                int g = getGlobalIndex(n, a);
-                if (g >= 0) freeDofs[g] = perNode[3 * n + a];
+                const int g = getGlobalIndex(n, a);
+                if (g >= 0) freeDofs[g] = perNode[n][a];
             }
     }
-    void scatterFree(const Scalar *freeDofs, Scalar *perNode) const {
-        for (int n = 0; n < nNodes_; n++)
+    void scatterFree(const Reference<Mesh> &mesh, const Scalar *freeDofs, Vector3 *perNode) const {
+        for (int n = 0; n < mesh->getNodeCount(); n++)
             for (int a = 0; a < 3; a++) {
-                int g = getGlobalIndex(n, a);
-                perNode[3 * n + a] = g >= 0 ? freeDofs[g] : 0.0;
+                const int g = getGlobalIndex(n, a);
+                perNode[n][a] = g >= 0 ? freeDofs[g] : Scalar(0);
             }
     }
-    od_mesh *get() const { return mesh_; }
-    const Device &device() const { return dev_; }
+    std::vector<Scalar> freeDisplacements(const Reference<Mesh> &mesh) const {
+        std::vector<Scalar> u((size_t)getMatrixOrder(mesh), 0.0);
+        std::vector<Vector3> d((size_t)mesh->getNodeCount());
+        for (int n = 0; n < mesh->getNodeCount(); n++) d[n] = mesh->getVertexDisplacementConst(n);
+        gatherFree(mesh, d.data(), u.data());
+        return u;
+    }
     int lastStatus() const { return status_; }
 
 private:
-    const Device &dev_;
-    int nNodes_, nTets_;
-    std::vector<int> constrained_;
-    std::vector<Scalar> u_;
-    od_mesh *mesh_ = nullptr;
-    int order_ = 0;
-    int status_ = OD_OK;
+    Reference<Mesh> mesh_;
+    std::vector<int> constrainIndices;
+    mutable od_mesh *dev_ = nullptr;
+    mutable int status_ = OD_OK;
 };
 
-// BlockedSymSparseMatrix<3,3> over the canonical layout of its mesh.  Values
-// live on the device; hostValues() mirrors them, markHostDirty() pushes host
-// edits back before the next device use.
+// ---- loads (core/forcer.h:13-26, core/forcer.cpp:8-96) ---------------------------------------
+class Forcer {
+public:
+    // V/4 * bodyForce on each node of every tet (element/tetelement.cpp:68-77)
+    void addBodyForce(const Reference<Mesh> &mesh, Scalar bodyForce[3], const Reference<NodeIndexer> &indexer) {
+        int gi[12];
+        for (int e = 0; e < mesh->getElementCount(); e++) {
+            const int *t = mesh->getElementNodeReference(e);
+            const Vector3 &a = mesh->getVertex(t[0]), &b = mesh->getVertex(t[1]), &c = mesh->getVertex(t[2]),
+                          &d = mesh->getVertex(t[3]);
+            // getTetVolume (element/tetelement.cpp:6-12): |ab . (ac x ad)| / 6
+            const Scalar ux = b.x - a.x, uy = b.y - a.y, uz = b.z - a.z, vx = c.x - a.x, vy = c.y - a.y, vz = c.z - a.z;
+            const Scalar wx = d.x - a.x, wy = d.y - a.y, wz = d.z - a.z;
+            const Scalar cx = vy * wz - vz * wy, cy = vz * wx - vx * wz, cz = vx * wy - vy * wx;
+            const Scalar volume = std::fabs(ux * cx + uy * cy + uz * cz) / Scalar(6.0);
+            const Scalar factor = volume / Scalar(4.0);
+            indexer->getElementNodesGlobalIndices(mesh, e, 4, gi);
+            for (int j = 0; j < 4; j++)
+                for (int k = 0; k < 3; k++) {
+                    const Scalar r = factor * bodyForce[k];
+                    if (r != 0.0 && gi[3 * j + k] >= 0) virtualWorks[gi[3 * j + k]] += r;
+                }
+        }
+    }
+    void addSurfaceForceByNode(const Reference<Mesh> &mesh, Scalar nodeForce[3], int nodeCounts, int *nodeIndex,
+                               const Reference<NodeIndexer> &indexer) {
+        (void)mesh;
+        for (int i = 0; i < nodeCounts; i++)
+            for (int j = 0; j < 3; j++) {
+                const int g = nodeForce[j] != 0.0 ? indexer->getGlobalIndex(nodeIndex[i], j) : -1;
+                if (g >= 0) virtualWorks[g] += nodeForce[j];
+            }
+    }
+    // basises == NULL: the full-order virtual works (forcer.cpp:80-96)
+    void getVirtualWorks(int dofs, int totalDofs, const Scalar *basises, Scalar *vws) const {
+        std::memset(vws, 0, sizeof(Scalar) * (size_t)dofs);
+        if (basises) {
+            const Scalar *basis = basises;
+            for (int i = 0; i < dofs; i++, basis += totalDofs)
+                for (const auto &e : virtualWorks) vws[i] += e.second * basis[e.first];
+        } else {
+            for (const auto &e : virtualWorks) vws[e.first] = e.second;
+        }
+    }
+
+private:
+    std::map<int, Scalar> virtualWorks;
+};
+
+// ---- the blocked symmetric matrix (core/sparseMatrix.h:186-295) -----------------------------
+// Per-element 78-entry value indices (core/sparseMatrix.h:650-666), as getMatrixIndicesPerElement
+// returns them; the device keeps its own block-level form of the same map.
+class SparseSymMatrixIndicesPerElementCache {
+public:
+    SparseSymMatrixIndicesPerElementCache() = default;
+    SparseSymMatrixIndicesPerElementCache(int elementCount, int nodePerElementCount)
+        : count_(nodePerElementCount * 3 * (nodePerElementCount * 3 + 1) / 2),
+          indices_((size_t)elementCount * count_, -1) {}
+    SparseSymMatrixIndicesPerElementCache(SparseSymMatrixIndicesPerElementCache &&) = default;
+    SparseSymMatrixIndicesPerElementCache &operator=(SparseSymMatrixIndicesPerElementCache &&) = default;
+    const int *getElementMatIndices(int elementIndex) const { return indices_.data() + (size_t)elementIndex * count_; }
+    int *getElementMatIndices(int elementIndex) { return indices_.data() + (size_t)elementIndex * count_; }
+
+private:
+    int count_ = 0;
+    std::vector<int> indices_;
+};
+
+// Values live on the device in the reference's canonical layout; hostValues() mirrors them and
+// markHostDirty() pushes host edits back before the next device use.
 class BlockedSymSpMatrix {
 public:
-    explicit BlockedSymSpMatrix(const TetMesh &mesh) : mesh_(mesh) {
-        status_ = Severe(od_matrix_create(mesh.get(), &m_), "od_matrix_create");
-        long long info[16] = {0};
-        od_mesh_info(mesh.get(), info);
-        nValues_ = info[3];
+    // BlockedSymSparseMatrixAssembler: getMatrixStructure binds it to (mesh, indexer); the device
+    // derives the pattern itself (od_mesh_create), identical to the reference's.
+    struct Assembler {
+        explicit Assembler(int numColumns) : numColumn(numColumns) {}
+        void addEntry(int, int, Scalar) {}
+        int numColumn;
+        Reference<NodeIndexer> indexer;
+    };
+    explicit BlockedSymSpMatrix(const Assembler &assembler) : indexer_(assembler.indexer) {
+        if (!indexer_) { status_ = OD_ERR_STATE; std::fprintf(stderr, "odgpu: matrix without a structure\n"); return; }
+        status_ = Severe(od_matrix_create(indexer_->deviceMesh(), &m_), "od_matrix_create");
+        own_ = true;
+        describe();
     }
-    ~BlockedSymSpMatrix() { if (m_) od_matrix_destroy(m_); }
+    // a view of a matrix owned elsewhere (the integrator's damping matrix)
+    BlockedSymSpMatrix(const Reference<NodeIndexer> &indexer, od_matrix *m) : indexer_(indexer), m_(m) { describe(); }
+    ~BlockedSymSpMatrix() { if (own_ && m_) od_matrix_destroy(m_); }
     BlockedSymSpMatrix(const BlockedSymSpMatrix &) = delete;
     BlockedSymSpMatrix &operator=(const BlockedSymSpMatrix &) = delete;
 
-    int getNumColumns() const { return mesh_.getMatrixOrder(); }
-    int getNumRows() const { return mesh_.getMatrixOrder(); }
-    long long getValueCount() const { return nValues_; }
-    // sparseMatrix.h:277-295
+    int getNumColumns() const { return numColumns_; }
+    int getNumEntries() const { return (int)nValues_; }
     void setZeros() { host_.clear(); hostDirty_ = false; status_ = Severe(od_matrix_zero(m_), "od_matrix_zero"); }
-    void Scale(Scalar s) { deviceSync(); status_ = Severe(od_matrix_axpby(m_, s, 0.0, nullptr), "od_matrix_axpby"); host_.clear(); }
-    void Add(Scalar s, const BlockedSymSpMatrix &o) {
+    void Scale(Scalar alpha) {
         deviceSync();
-        o.deviceSync();
-        status_ = Severe(od_matrix_axpby(m_, 1.0, s, o.m_), "od_matrix_axpby");
+        status_ = Severe(od_matrix_axpby(m_, alpha, 0.0, nullptr), "od_matrix_axpby");
         host_.clear();
     }
-    // host mirror of `values` (core/sparseMatrix.h:627-635 layout)
+    void Add(Scalar alpha, const BlockedSymSpMatrix &mat) {
+        deviceSync();
+        mat.deviceSync();
+        status_ = Severe(od_matrix_axpby(m_, 1.0, alpha, mat.m_), "od_matrix_axpby");
+        host_.clear();
+    }
+    // BlockedSymSparseMatrix::getIndices as getMatrixIndicesPerElement uses it: the per-tet map
+    SparseSymMatrixIndicesPerElementCache elementIndices(int elementCount) const {
+        SparseSymMatrixIndicesPerElementCache c(elementCount, 4);
+        if (elementCount > 0)
+            Severe(od_pattern_export(indexer_->deviceMesh(), nullptr, nullptr, nullptr, c.getElementMatIndices(0)),
+                   "od_pattern_export");
+        return c;
+    }
     std::vector<Scalar> &hostValues() {
         if (host_.empty()) {
             host_.assign((size_t)nValues_, 0.0);
@@ -196,17 +456,6 @@ public:
         return host_;
     }
     void markHostDirty() { hostDirty_ = true; }
-    // the canonical index arrays blockPcol / blockColumnOris / blockRows
-    void getStructure(std::vector<int> &blockPcol, std::vector<int> &blockColumnOris, std::vector<int> &blockRows) const {
-        long long info[16] = {0};
-        od_mesh_info(mesh_.get(), info);
-        blockPcol.resize((size_t)info[1] + 1);
-        blockColumnOris.resize((size_t)info[1] + 1);
-        blockRows.resize((size_t)info[2]);
-        Severe(od_pattern_export(mesh_.get(), blockPcol.data(), blockColumnOris.data(), blockRows.data(), nullptr),
-               "od_pattern_export");
-    }
-    // pushes host edits; invalidates the mirror after device writes
     void deviceSync() const {
         if (hostDirty_ && !host_.empty()) {
             Severe(od_matrix_upload(m_, host_.data()), "od_matrix_upload");
@@ -215,12 +464,20 @@ public:
     }
     void deviceWritten() { host_.clear(); hostDirty_ = false; }
     od_matrix *get() const { return m_; }
-    const TetMesh &mesh() const { return mesh_; }
+    const Reference<NodeIndexer> &indexer() const { return indexer_; }
     int lastStatus() const { return status_; }
 
 private:
-    const TetMesh &mesh_;
+    void describe() {
+        long long info[16] = {0};
+        if (indexer_ && indexer_->deviceMesh()) od_mesh_info(indexer_->deviceMesh(), info);
+        numColumns_ = (int)info[0];
+        nValues_ = info[3];
+    }
+    Reference<NodeIndexer> indexer_;
     od_matrix *m_ = nullptr;
+    bool own_ = false;
+    int numColumns_ = 0;
     long long nValues_ = 0;
     mutable std::vector<Scalar> host_;
     mutable bool hostDirty_ = false;
@@ -230,8 +487,7 @@ private:
 // SpMDV (core/numerMethod.h:134-136): dest += mat * src, host vectors.
 inline int SpMDV(const BlockedSymSpMatrix &mat, const Scalar *src, Scalar *dest) {
     const int n = mat.getNumColumns();
-    const Device &d = mat.mesh().device();
-    DeviceBuffer<Scalar> x(d), y(d);
+    DeviceBuffer<Scalar> x, y;
     mat.deviceSync();
     int st = x.upload(src, n);
     if (st == OD_OK) st = y.upload(dest, n);
@@ -240,203 +496,291 @@ inline int SpMDV(const BlockedSymSpMatrix &mat, const Scalar *src, Scalar *dest)
     return st;
 }
 
-// Preconditioner family (core/preconditioner.h:12-22).  The identity base
-// matches the reference's default Preconditioner.
-class Preconditioner {
+// ---- linear solvers and preconditioners (core/linearSolver.h:12-20, core/preconditioner.h:12-22) --
+template <class SpMatrix> class LinearSolver : public ReferenceCounted {
 public:
-    explicit Preconditioner(int kind = OD_PRE_NONE, Scalar param = 0.0) : kind_(kind), param_(param) {}
-    virtual ~Preconditioner() { if (s_) od_solver_destroy(s_); }
-    Preconditioner(const Preconditioner &) = delete;
-    Preconditioner &operator=(const Preconditioner &) = delete;
-    // Preprocess / resetPreconditionerSystem: the factor is rebuilt from the
-    // bound matrix on every apply and inside every PCG solve, as the
-    // reference's CGSolver calls Preprocess per solve (solver/cgSolver.cpp:23).
-    virtual void resetPreconditionerSystem(const BlockedSymSpMatrix &mat) { mat_ = &mat; }
-    virtual void Preprocess(const BlockedSymSpMatrix &mat) { mat_ = &mat; }
+    LinearSolver(const SpMatrix *m = nullptr) : mat(m) {}
+    virtual void resetLinearSystem(const SpMatrix *m) { mat = m; }
+    virtual void solveLinearSystem(const Scalar *rhs, Scalar *result) = 0;
+    virtual ~LinearSolver() = default;
+
+protected:
+    const SpMatrix *mat;
+};
+
+class Preconditioner : public ReferenceCounted {
+public:
+    // the identity (the reference's default Preconditioner)
     virtual void solvePreconditionerSystem(int width, const Scalar *rhs, Scalar *result) const {
-        if (!mat_ || kind_ == OD_PRE_NONE) { std::memcpy(result, rhs, sizeof(Scalar) * (size_t)width); return; }
-        const Device &d = mat_->mesh().device();
-        if (!s_ && Severe(od_solver_create(mat_->mesh().get(), kind_, param_, OD_FP64, &s_), "od_solver_create"))
-            return;
-        DeviceBuffer<Scalar> r(d), z(d);
-        mat_->deviceSync();
+        std::memcpy(result, rhs, (size_t)width * sizeof(Scalar));
+    }
+    virtual void resetPreconditionerSystem(const BlockedSymSpMatrix &mat) { bound_ = &mat; }
+    // builds the factor (or D^-1) of mat on the device; solvePreconditionerSystem applies it
+    virtual void Preprocess(const BlockedSymSpMatrix &mat) {
+        bound_ = &mat;
+        if (deviceKind() == OD_PRE_NONE || !ensureSolver()) return;
+        mat.deviceSync();
+        Severe(od_precond_preprocess(s_, mat.get()), "od_precond_preprocess");
+    }
+    virtual ~Preconditioner() { if (s_) od_solver_destroy(s_); }
+    // the library's kind and parameters of this preconditioner (OD_PRE_NONE for the identity)
+    virtual int deviceKind() const { return OD_PRE_NONE; }
+    virtual Scalar deviceParam1() const { return 0.0; }
+    virtual Scalar deviceParam2() const { return 0.0; }
+
+protected:
+    bool ensureSolver() const {
+        if (s_) return true;
+        if (!bound_) { std::fprintf(stderr, "odgpu: preconditioner without a matrix (Preprocess first)\n"); return false; }
+        if (Severe(od_solver_create(bound_->indexer()->deviceMesh(), deviceKind(), deviceParam1(), OD_FP64, &s_),
+                   "od_solver_create"))
+            return false;
+        return Severe(od_solver_set_precond_params(s_, deviceParam1(), deviceParam2()),
+                      "od_solver_set_precond_params") == OD_OK;
+    }
+    // the device apply of the subclasses, with the factor of the last Preprocess
+    void deviceApply(int width, const Scalar *rhs, Scalar *result) const {
+        if (!ensureSolver()) return;
+        DeviceBuffer<Scalar> r, z;
         if (r.upload(rhs, width) || !z.ensure(width)) return;
-        if (Severe(od_precond_apply(s_, mat_->get(), r.get(), z.get()), "od_precond_apply")) return;
+        if (Severe(od_precond_solve(s_, r.get(), z.get()), "od_precond_solve")) return;
         z.download(result, width);
     }
-    int kind() const { return kind_; }
-    Scalar param() const { return param_; }
-
-private:
-    int kind_;
-    Scalar param_;
-    const BlockedSymSpMatrix *mat_ = nullptr;
+    const BlockedSymSpMatrix *bound_ = nullptr;
     mutable od_solver *s_ = nullptr;
 };
 
 // JacobiPreconditioner (solver/jacobiPreconditioner.h:10-22)
-class JacobiPreconditioner : public Preconditioner {
+class GpuJacobiPreconditioner : public Preconditioner {
 public:
-    JacobiPreconditioner() : Preconditioner(OD_PRE_JACOBI) {}
-};
-
-// IC(0) on the block pattern in the role of InCholeskyPreconditioner
-// (solver/inCholeskyPreconditioner.h:11-32); `epsilon` is the pivot guard
-// d <= 0 -> epsilon^2 d^2 of inCholeskyPreconditioner.cpp:52.
-class IC0Preconditioner : public Preconditioner {
-public:
-    explicit IC0Preconditioner(Scalar epsilon = 1e-8) : Preconditioner(OD_PRE_IC0, epsilon) {}
-};
-
-// CGSolver (solver/cgSolver.h:14-30): same constructor order (tol, maxIter,
-// preconditioner, matrix).  `precision` selects the FP32-inner form.
-class CGSolver {
-public:
-    CGSolver(Scalar tol, int maxIter, const Preconditioner &precondition = Preconditioner(),
-             const BlockedSymSpMatrix *m = nullptr, int precision = OD_FP64)
-        : tol_(tol), maxIt_(maxIter), pkind_(precondition.kind()), pparam_(precondition.param()),
-          precision_(precision), mat_(m) {}
-    ~CGSolver() {
-        b_.reset();
-        x_.reset();
-        if (s_) od_solver_destroy(s_);
+    void solvePreconditionerSystem(int width, const Scalar *rhs, Scalar *result) const override {
+        deviceApply(width, rhs, result);
     }
-    CGSolver(const CGSolver &) = delete;
-    CGSolver &operator=(const CGSolver &) = delete;
-    void resetLinearSystem(const BlockedSymSpMatrix *m) { mat_ = m; }
+    int deviceKind() const override { return OD_PRE_JACOBI; }
+};
+
+// IC(0) on the block pattern, the device-factored incomplete Cholesky; epsilon is the pivot
+// guard d <= 0 -> epsilon^2 d^2 (solver/inCholeskyPreconditioner.cpp:52).
+class GpuIC0Preconditioner : public Preconditioner {
+public:
+    explicit GpuIC0Preconditioner(Scalar epsilon = 1e-3) : eps_(epsilon) {}
+    void solvePreconditionerSystem(int width, const Scalar *rhs, Scalar *result) const override {
+        deviceApply(width, rhs, result);
+    }
+    int deviceKind() const override { return OD_PRE_IC0; }
+    Scalar deviceParam1() const override { return eps_; }
+
+private:
+    Scalar eps_;
+};
+
+// InCholeskyPreconditioner(epsilon, relaxedScale) (solver/inCholeskyPreconditioner.h:14)
+class GpuInCholeskyPreconditioner : public Preconditioner {
+public:
+    GpuInCholeskyPreconditioner(Scalar epsilon, Scalar relaxedScale) : eps_(epsilon), relax_(relaxedScale) {}
+    void solvePreconditionerSystem(int width, const Scalar *rhs, Scalar *result) const override {
+        deviceApply(width, rhs, result);
+    }
+    int deviceKind() const override { return OD_PRE_ICT; }
+    Scalar deviceParam1() const override { return eps_; }
+    Scalar deviceParam2() const override { return relax_; }
+
+private:
+    Scalar eps_, relax_;
+};
+
+// InLDLTPreconditioner(sainvEpsilon, ldltEpsilon) (solver/inLDLTPreconditioner.h:14-15)
+class GpuInLDLTPreconditioner : public Preconditioner {
+public:
+    GpuInLDLTPreconditioner(Scalar sainvEpsilon, Scalar ldltEpsilon) : se_(sainvEpsilon), le_(ldltEpsilon) {}
+    void solvePreconditionerSystem(int width, const Scalar *rhs, Scalar *result) const override {
+        deviceApply(width, rhs, result);
+    }
+    int deviceKind() const override { return OD_PRE_LDLT; }
+    Scalar deviceParam1() const override { return se_; }
+    Scalar deviceParam2() const override { return le_; }
+
+private:
+    Scalar se_, le_;
+};
+
+// CGSolver(tol, maxIter, preconditioner, matrix) (solver/cgSolver.h:14-30).  `precision`
+// (no reference counterpart) selects the FP32-inner form.
+class GpuCGSolver : public LinearSolver<BlockedSymSpMatrix> {
+public:
+    GpuCGSolver(Scalar tol, int maxIter, const Reference<Preconditioner> &precondition = new Preconditioner(),
+                const BlockedSymSpMatrix *m = nullptr, int precision = OD_FP64)
+        : LinearSolver(m), tolerant(tol), maxIteration(maxIter), preconditioner(precondition), precision_(precision) {}
+    GpuCGSolver(const GpuCGSolver &) = delete;
+    GpuCGSolver &operator=(const GpuCGSolver &) = delete;
+    ~GpuCGSolver() override { if (s_) od_solver_destroy(s_); }
+    void resetLinearSystem(const BlockedSymSpMatrix *m) override {
+        if (s_ && m && mat && m->indexer().get() != mat->indexer().get()) { od_solver_destroy(s_); s_ = nullptr; }
+        mat = m;
+    }
     // result: initial guess in, solution out (solver/cgSolver.cpp:13-57)
-    void solveLinearSystem(const Scalar *rhs, Scalar *result) {
-        if (!mat_) { status_ = OD_ERR_STATE; std::fprintf(stderr, "odgpu: CGSolver without a matrix\n"); return; }
-        const Device &d = mat_->mesh().device();
-        const int n = mat_->getNumColumns();
+    void solveLinearSystem(const Scalar *rhs, Scalar *result) override {
+        if (!mat) { status_ = OD_ERR_STATE; std::fprintf(stderr, "odgpu: CGSolver without a matrix\n"); return; }
+        const int n = mat->getNumColumns();
         if (!s_) {
-            status_ = Severe(od_solver_create(mat_->mesh().get(), pkind_, pparam_, precision_, &s_), "od_solver_create");
+            const Preconditioner &p = *preconditioner;
+            status_ = Severe(od_solver_create(mat->indexer()->deviceMesh(), p.deviceKind(), p.deviceParam1(), precision_,
+                                              &s_),
+                             "od_solver_create");
+            if (status_) return;
+            status_ = Severe(od_solver_set_precond_params(s_, p.deviceParam1(), p.deviceParam2()),
+                             "od_solver_set_precond_params");
             if (status_) return;
         }
-        if (!b_) { b_.reset(new DeviceBuffer<Scalar>(d)); x_.reset(new DeviceBuffer<Scalar>(d)); }
-        mat_->deviceSync();
-        if ((status_ = b_->upload(rhs, n)) || (status_ = x_->upload(result, n))) return;
-        status_ = Severe(od_pcg_solve(s_, mat_->get(), b_->get(), x_->get(), tol_, maxIt_, &iters_, &relres_),
+        mat->deviceSync();
+        if ((status_ = b_.upload(rhs, n)) || (status_ = x_.upload(result, n))) return;
+        status_ = Severe(od_pcg_solve(s_, mat->get(), b_.get(), x_.get(), tolerant, maxIteration, &iters_, &relres_),
                          "od_pcg_solve");
-        if (status_ == OD_OK) status_ = x_->download(result, n);
+        if (status_ == OD_OK) status_ = x_.download(result, n);
     }
+    Scalar tolerance() const { return tolerant; }
+    int maxIterations() const { return maxIteration; }
+    const Reference<Preconditioner> &getPreconditioner() const { return preconditioner; }
+    int precision() const { return precision_; }
     int lastIterations() const { return iters_; }
     Scalar lastRelativeResidual() const { return relres_; }
     int lastStatus() const { return status_; }
 
 private:
-    Scalar tol_;
-    int maxIt_, pkind_;
-    Scalar pparam_;
+    Scalar tolerant;
+    int maxIteration;
+    Reference<Preconditioner> preconditioner;
     int precision_;
-    const BlockedSymSpMatrix *mat_;
     od_solver *s_ = nullptr;
-    std::unique_ptr<DeviceBuffer<Scalar>> b_, x_;
+    DeviceBuffer<Scalar> b_, x_;
     int iters_ = 0;
     Scalar relres_ = 0;
     int status_ = OD_OK;
 };
 
-// FullOrderNonlinearMaterial family (material/fullOrderNonlinear.h:36-44).
-// The device stores (kind, E, nu, rho) per tet; a material object writes its
-// parameters for every tet of the mesh it is used with.
-class FullOrderNonlinearMaterial {
+// ---- materials (material/fullOrderNonlinear.h:14-44, core/mechMaterial.h:29-32) -------------
+// The reference's per-material precomputes (K0 and Dm^-1 per tet, material/corotationalHyperelastic.cpp:
+// 20-41) are recomputed on the device inside the element kernel, so the cache is empty.
+struct FullOrderNonlinearMaterialCache {
+    FullOrderNonlinearMaterialCache() = default;
+    FullOrderNonlinearMaterialCache(FullOrderNonlinearMaterialCache &&) = default;
+    FullOrderNonlinearMaterialCache &operator=(FullOrderNonlinearMaterialCache &&) = default;
+    FullOrderNonlinearMaterialCache(const FullOrderNonlinearMaterialCache &) = delete;
+    FullOrderNonlinearMaterialCache &operator=(const FullOrderNonlinearMaterialCache &) = delete;
+};
+
+template <class SpMatrix> class FullOrderNonlinearMaterial : public ReferenceCounted {
 public:
+    explicit FullOrderNonlinearMaterial(Scalar rho) : density(rho) {}
     virtual ~FullOrderNonlinearMaterial() = default;
-    // per-tet tables written to the mesh
+    virtual FullOrderNonlinearMaterialCache getPrecomputes(const Reference<Mesh> &mesh) const {
+        (void)mesh;
+        return FullOrderNonlinearMaterialCache();
+    }
+    // K += K(u), vws += f_int(u) at the mesh's current displacements (accumulating)
+    virtual void generateMatrixAndVirtualWorks(const Reference<Mesh> &mesh, const Reference<NodeIndexer> &indexer,
+                                               const FullOrderNonlinearMaterialCache &cache,
+                                               const SparseSymMatrixIndicesPerElementCache &matrixIndices,
+                                               SpMatrix &matrix, Scalar *vws) const = 0;
+    void getMatrixStructure(const Reference<Mesh> &mesh, const Reference<NodeIndexer> &indexer,
+                            typename SpMatrix::Assembler &assembler) const {
+        (void)mesh;
+        assembler.indexer = indexer;
+    }
+    SparseSymMatrixIndicesPerElementCache getMatrixIndicesPerElement(const Reference<Mesh> &mesh,
+                                                                     const Reference<NodeIndexer> &indexer,
+                                                                     const SpMatrix &mat) const {
+        (void)indexer;
+        return mat.elementIndices(mesh->getElementCount());
+    }
+    // MechMaterial::generateMassMatrix (core/mechMaterial.cpp:69-98): matrix += M
+    void generateMassMatrix(const Reference<Mesh> &mesh, const Reference<NodeIndexer> &indexer,
+                            const SparseSymMatrixIndicesPerElementCache &matrixIndices, SpMatrix &matrix) const {
+        (void)matrixIndices;
+        if (bindTo(mesh, indexer)) return;
+        matrix.deviceSync();
+        Severe(od_mass_matrix(indexer->deviceMesh(), matrix.get()), "od_mass_matrix");
+        matrix.deviceWritten();
+    }
+    Scalar getDensity() const { return density; }
+    // the per-tet device tables (kind, E, nu, rho) of this material on a mesh
     virtual void tables(int nTets, std::vector<int> &kind, std::vector<Scalar> &E, std::vector<Scalar> &nu,
                         std::vector<Scalar> &rho) const = 0;
     virtual Scalar corotationalThreshold() const { return 0.0; }
     virtual Scalar inversionThreshold() const { return 0.1; }
-    int bindTo(const TetMesh &mesh) const {
+    int bindTo(const Reference<Mesh> &mesh, const Reference<NodeIndexer> &indexer) const {
         std::vector<int> kind;
         std::vector<Scalar> E, nu, rho;
-        tables(mesh.getTetCount(), kind, E, nu, rho);
-        return Severe(od_mesh_set_material(mesh.get(), kind.data(), E.data(), nu.data(), rho.data(),
-                                           corotationalThreshold(), inversionThreshold()),
+        tables(mesh->getElementCount(), kind, E, nu, rho);
+        od_mesh *dm = indexer->deviceMesh();
+        if (!dm) return OD_ERR_STATE;
+        return Severe(od_mesh_set_material(dm, kind.data(), E.data(), nu.data(), rho.data(), corotationalThreshold(),
+                                           inversionThreshold()),
                       "od_mesh_set_material");
     }
-    // generateMatrixAndVirtualWorks (fullOrderNonlinear.h:40-41): matrix +=
-    // K(u) and vws += f_int(u) at the mesh's current node displacements.
-    int generateMatrixAndVirtualWorks(const TetMesh &mesh, BlockedSymSpMatrix &matrix, Scalar *vws) const {
-        int st = bindTo(mesh);
+
+protected:
+    Scalar density;
+    int assemble(const Reference<Mesh> &mesh, const Reference<NodeIndexer> &indexer, SpMatrix &matrix, Scalar *vws) const {
+        int st = bindTo(mesh, indexer);
         if (st) return st;
-        const int n = mesh.getMatrixOrder();
-        std::vector<Scalar> u((size_t)n, 0.0), f((size_t)n, 0.0);
-        mesh.gatherFree(mesh.getDisplacements(), u.data());
-        DeviceBuffer<Scalar> du(mesh.device()), df(mesh.device());
+        const int n = indexer->getMatrixOrder(mesh);
+        std::vector<Scalar> u = indexer->freeDisplacements(mesh), f((size_t)n, 0.0);
+        DeviceBuffer<Scalar> du, df;
         if ((st = du.upload(u.data(), n)) || (st = df.upload(f.data(), n))) return st;
         matrix.deviceSync();
-        st = Severe(od_assemble(mesh.get(), du.get(), matrix.get(), df.get()), "od_assemble");
+        st = Severe(od_assemble(indexer->deviceMesh(), du.get(), matrix.get(), df.get()), "od_assemble");
         matrix.deviceWritten();
         if (st == OD_OK) st = df.download(f.data(), n);
         if (st == OD_OK)
             for (int i = 0; i < n; i++) vws[i] += f[(size_t)i];
         return st;
     }
-    // MechMaterial::generateMassMatrix (core/mechMaterial.cpp:69-98): M += mass
-    int generateMassMatrix(const TetMesh &mesh, BlockedSymSpMatrix &M) const {
-        int st = bindTo(mesh);
-        if (st) return st;
-        M.deviceSync();
-        st = Severe(od_mass_matrix(mesh.get(), M.get()), "od_mass_matrix");
-        M.deviceWritten();
-        return st;
-    }
 };
 
+template <int KIND> class GpuUniformMaterial : public FullOrderNonlinearMaterial<BlockedSymSpMatrix> {
+public:
+    GpuUniformMaterial(Scalar rho, Scalar youngs, Scalar possion, Scalar threshold)
+        : FullOrderNonlinearMaterial(rho), E_(youngs), nu_(possion), thr_(threshold) {}
+    void generateMatrixAndVirtualWorks(const Reference<Mesh> &mesh, const Reference<NodeIndexer> &indexer,
+                                       const FullOrderNonlinearMaterialCache &, const SparseSymMatrixIndicesPerElementCache &,
+                                       BlockedSymSpMatrix &matrix, Scalar *vws) const override {
+        assemble(mesh, indexer, matrix, vws);
+    }
+    void tables(int n, std::vector<int> &kind, std::vector<Scalar> &E, std::vector<Scalar> &nu,
+                std::vector<Scalar> &rho) const override {
+        kind.assign((size_t)n, KIND);
+        E.assign((size_t)n, E_);
+        nu.assign((size_t)n, nu_);
+        rho.assign((size_t)n, density);
+    }
+    Scalar corotationalThreshold() const override { return KIND == OD_MAT_NEOHOOKEAN ? 0.0 : thr_; }
+    Scalar inversionThreshold() const override { return KIND == OD_MAT_NEOHOOKEAN ? thr_ : 0.1; }
+
+private:
+    Scalar E_, nu_, thr_;
+};
 // CorotationalHyperelasticMaterial(rho, youngs, possion, compressingTreshold)
 // (material/corotationalHyperelastic.h:13)
-class CorotationalHyperelasticMaterial : public FullOrderNonlinearMaterial {
+using GpuCorotationalMaterial = GpuUniformMaterial<OD_MAT_COROTATIONAL>;
+// InvertibleNeoHookeanMaterial(rho, youngs, possion, inversionTrashold) (material/invertibleNeoHookean.h:13)
+using GpuNeoHookeanMaterial = GpuUniformMaterial<OD_MAT_NEOHOOKEAN>;
+
+// Per-tet material tables (no reference counterpart: the reference binds one material per mesh).
+class GpuPerTetMaterial : public FullOrderNonlinearMaterial<BlockedSymSpMatrix> {
 public:
-    CorotationalHyperelasticMaterial(Scalar rho, Scalar youngs, Scalar possion, Scalar compressingThreshold)
-        : rho_(rho), E_(youngs), nu_(possion), thr_(compressingThreshold) {}
-    void tables(int n, std::vector<int> &kind, std::vector<Scalar> &E, std::vector<Scalar> &nu,
-                std::vector<Scalar> &rho) const override {
-        kind.assign((size_t)n, OD_MAT_COROTATIONAL);
-        E.assign((size_t)n, E_);
-        nu.assign((size_t)n, nu_);
-        rho.assign((size_t)n, rho_);
+    GpuPerTetMaterial(std::vector<int> kind, std::vector<Scalar> E, std::vector<Scalar> nu, std::vector<Scalar> rho,
+                      Scalar corotThreshold = 0.0, Scalar neoThreshold = 0.1)
+        : FullOrderNonlinearMaterial(rho.empty() ? 0.0 : rho[0]), kind_(std::move(kind)), E_(std::move(E)),
+          nu_(std::move(nu)), rho_(std::move(rho)), ct_(corotThreshold), nt_(neoThreshold) {}
+    void generateMatrixAndVirtualWorks(const Reference<Mesh> &mesh, const Reference<NodeIndexer> &indexer,
+                                       const FullOrderNonlinearMaterialCache &, const SparseSymMatrixIndicesPerElementCache &,
+                                       BlockedSymSpMatrix &matrix, Scalar *vws) const override {
+        assemble(mesh, indexer, matrix, vws);
     }
-    Scalar corotationalThreshold() const override { return thr_; }
-
-private:
-    Scalar rho_, E_, nu_, thr_;
-};
-
-// InvertibleNeoHookeanMaterial(rho, youngs, possion, inversionTrashold)
-// (material/invertibleNeoHookean.h:13)
-class InvertibleNeoHookeanMaterial : public FullOrderNonlinearMaterial {
-public:
-    InvertibleNeoHookeanMaterial(Scalar rho, Scalar youngs, Scalar possion, Scalar inversionThreshold)
-        : rho_(rho), E_(youngs), nu_(possion), thr_(inversionThreshold) {}
-    void tables(int n, std::vector<int> &kind, std::vector<Scalar> &E, std::vector<Scalar> &nu,
-                std::vector<Scalar> &rho) const override {
-        kind.assign((size_t)n, OD_MAT_NEOHOOKEAN);
-        E.assign((size_t)n, E_);
-        nu.assign((size_t)n, nu_);
-        rho.assign((size_t)n, rho_);
-    }
-    Scalar inversionThreshold() const override { return thr_; }
-
-private:
-    Scalar rho_, E_, nu_, thr_;
-};
-
-// Per-tet material table (no reference counterpart: the reference binds one
-// material per mesh).
-class PerTetMaterial : public FullOrderNonlinearMaterial {
-public:
-    PerTetMaterial(std::vector<int> kind, std::vector<Scalar> E, std::vector<Scalar> nu, std::vector<Scalar> rho,
-                   Scalar corotThreshold = 0.0, Scalar neoThreshold = 0.1)
-        : kind_(std::move(kind)), E_(std::move(E)), nu_(std::move(nu)), rho_(std::move(rho)), ct_(corotThreshold),
-          nt_(neoThreshold) {}
     void tables(int, std::vector<int> &kind, std::vector<Scalar> &E, std::vector<Scalar> &nu,
                 std::vector<Scalar> &rho) const override {
-        kind = kind_;
-        E = E_;
-        nu = nu_;
-        rho = rho_;
+        kind = kind_; E = E_; nu = nu_; rho = rho_;
     }
     Scalar corotationalThreshold() const override { return ct_; }
     Scalar inversionThreshold() const override { return nt_; }
@@ -447,78 +791,116 @@ private:
     Scalar ct_, nt_;
 };
 
-// Intergrator (core/intergrator.h:13-35).  setExternalVirtualWork takes the
-// free-DOF virtual works that Forcer::getVirtualWorks produces
-// (core/forcer.cpp:80-96); velocities come out as 3 per node.
-// collision/collisionShapes.h:17-20: the plane n.x + c = 0 (below: n.x + c < 0)
-struct StaticPlaneShape {
-    Scalar planeNormal[3];
-    Scalar planeConstant;
+// ---- collision damping (collision/collisionShapes.h:17-20, collision/penaltyDampingForcer.h:25-38) --
+class StaDampingForcer {
+public:
+    virtual void addDampingMatrix(const SparseSymMatrixIndicesPerElementCache &matrixIndices,
+                                  BlockedSymSpMatrix &mat) const = 0;
+    virtual ~StaDampingForcer() = default;
 };
 
-// MeshPlaneCollisionDampingForcer (collision/penaltyDampingForcer.h:25-38): the (plane, tet)
-// pair whose damping matrix addExternalDampingForceMatrix adds.  The positions and velocities
-// it reads are the integrator's own (the reference passes the mesh and a velocities array
-// the caller fills from getMeshVerticesVelocities).  setCollisionPair replaces the pair as
-// the reference's does; addCollisionPair queues further pairs so that one
-// addExternalDampingForceMatrix call adds them all, in order.
-class MeshPlaneCollisionDampingForcer {
+struct StaticPlaneShape {
+    Vector3 planeNormal;   // below the plane: n.x + c < 0
+    Scalar planeConstant = 0;
+};
+
+// The reference's forcer holds one (plane, tet) pair; addCollisionPair (no reference
+// counterpart) queues more, added in order by one addDampingMatrix call.  Positions are the
+// rest positions plus the mesh's displacements; velocities are per node, as the caller fills
+// them from Intergrator::getMeshVerticesVelocities.
+class MeshPlaneCollisionDampingForcer : public StaDampingForcer {
 public:
-    explicit MeshPlaneCollisionDampingForcer(Scalar dampingCoefficient) : coeff_(dampingCoefficient) {}
+    MeshPlaneCollisionDampingForcer(Scalar dampingCoefficient, const Reference<Mesh> &mesh,
+                                    const Reference<NodeIndexer> &indexer, const Vector3 *velocities)
+        : coeff_(dampingCoefficient), mesh_(mesh), indexer_(indexer), velocities_(velocities) {}
     void setCollisionPair(const StaticPlaneShape &plane, int elementIndex) {
         planes_.clear();
         tets_.clear();
         addCollisionPair(plane, elementIndex);
     }
     void addCollisionPair(const StaticPlaneShape &plane, int elementIndex) {
-        planes_.insert(planes_.end(), {plane.planeNormal[0], plane.planeNormal[1], plane.planeNormal[2],
-                                       plane.planeConstant});
+        planes_.insert(planes_.end(), {plane.planeNormal.x, plane.planeNormal.y, plane.planeNormal.z, plane.planeConstant});
         tets_.push_back(elementIndex);
     }
-    int pairCount() const { return (int)tets_.size(); }
-    const Scalar *planes() const { return planes_.data(); }
-    const int *elements() const { return tets_.data(); }
-    Scalar coefficient() const { return coeff_; }
+    void addDampingMatrix(const SparseSymMatrixIndicesPerElementCache &, BlockedSymSpMatrix &mat) const override {
+        const int n = indexer_->getMatrixOrder(mesh_);
+        std::vector<Scalar> u = indexer_->freeDisplacements(mesh_), v((size_t)n, 0.0);
+        if (velocities_) indexer_->gatherFree(mesh_, velocities_, v.data());
+        DeviceBuffer<Scalar> du, dv;
+        if (du.upload(u.data(), n) || dv.upload(v.data(), n)) return;
+        mat.deviceSync();
+        Severe(od_plane_damping(mat.get(), (int)tets_.size(), planes_.data(), tets_.data(), coeff_, du.get(), dv.get()),
+               "od_plane_damping");
+        mat.deviceWritten();
+    }
 
 private:
     Scalar coeff_;
+    Reference<Mesh> mesh_;
+    Reference<NodeIndexer> indexer_;
+    const Vector3 *velocities_;
     std::vector<Scalar> planes_;
     std::vector<int> tets_;
 };
 
+// ---- integrators (core/intergrator.h:13-35) ------------------------------------------------
 class Intergrator {
 public:
-    Intergrator(int dofs, Scalar massDamp, Scalar stiffDamp, Scalar ts)
-        : dofs(dofs), massDamping(massDamp), stiffnessDamping(stiffDamp), timeStep(ts) {}
-    virtual ~Intergrator() { if (it_) od_integrator_destroy(it_); }
-    Intergrator(const Intergrator &) = delete;
-    Intergrator &operator=(const Intergrator &) = delete;
-
-    virtual void setExternalVirtualWork(const Scalar *vws) {
-        status_ = Severe(od_integrator_set_external(it_, vws), "od_integrator_set_external");
-    }
-    virtual void runOneTimeStep() { status_ = Severe(od_integrator_step(it_, 1), "od_integrator_step"); }
-    virtual void updateMeshVerticesDisplacements(TetMesh &mesh) const {
-        std::vector<Scalar> d((size_t)dofs);
-        if (Severe(od_integrator_get_state(it_, d.data(), nullptr, nullptr), "od_integrator_get_state")) return;
-        mesh.scatterFree(d.data(), mesh.getDisplacements());
-    }
-    virtual void getMeshVerticesVelocities(const TetMesh &mesh, Scalar *velocities) const {
-        std::vector<Scalar> v((size_t)dofs);
-        if (Severe(od_integrator_get_state(it_, nullptr, v.data(), nullptr), "od_integrator_get_state")) return;
-        mesh.scatterFree(v.data(), velocities);
-    }
-    // core/intergrator.h:19-24: integrators without external damping report it unimplemented
-    virtual void addExternalDampingForceMatrix(const MeshPlaneCollisionDampingForcer &) {
-        std::fprintf(stderr, "Unimplemented features in Intergrator::addExternalDampingForceMatrix\n");
-        status_ = OD_ERR_UNSUPPORTED;
+    Intergrator(int DOFS, Scalar massDamp, Scalar stiffDamp, Scalar ts)
+        : dofs(DOFS), massDamping(massDamp), stiffnessDamping(stiffDamp), timeStep(ts) {}
+    virtual void setExternalVirtualWork(const Forcer &forcer) = 0;
+    virtual void addExternalDampingForceMatrix(const StaDampingForcer &forcer) {
+        (void)forcer;
+        std::fprintf(stderr, "Unimplemented features in Intergrator::setExternalDampingForceMatrix\n");
     }
     virtual void clearExternalDampingForceMatrix() {
         std::fprintf(stderr, "Unimplemented features in Intergrator::clearExternalDampingForceMatrix\n");
-        status_ = OD_ERR_UNSUPPORTED;
     }
-    // one-step nodal impulse (added to the next step's external force only)
-    void setImpulse(const Scalar *vws) { status_ = Severe(od_integrator_set_impulse(it_, vws), "od_integrator_set_impulse"); }
+    virtual void runOneTimeStep() = 0;
+    virtual void updateMeshVerticesDisplacements(const Reference<NodeIndexer> &indexer, Reference<Mesh> &mesh) const = 0;
+    virtual void getMeshVerticesVelocities(const Reference<NodeIndexer> &indexer, const Reference<Mesh> &mesh,
+                                           Vector3 *velocities) const = 0;
+    virtual ~Intergrator() = default;
+
+protected:
+    int dofs;
+    Scalar massDamping;
+    Scalar stiffnessDamping;
+    Scalar timeStep;
+};
+
+// The device-resident integrators: every step runs on the device (element pass, assembly,
+// factor, PCG, corrector); the linear solver given is a GpuCGSolver whose tolerance, iteration
+// cap, preconditioner and precision the step's solve uses.
+class GpuIntergrator : public Intergrator {
+public:
+    using Intergrator::Intergrator;
+    ~GpuIntergrator() override { if (it_) od_integrator_destroy(it_); }
+    void setExternalVirtualWork(const Forcer &forcer) override {
+        std::vector<Scalar> vws((size_t)dofs);
+        forcer.getVirtualWorks(dofs, dofs, nullptr, vws.data());
+        status_ = Severe(od_integrator_set_external(it_, vws.data()), "od_integrator_set_external");
+    }
+    void runOneTimeStep() override { status_ = Severe(od_integrator_step(it_, 1), "od_integrator_step"); }
+    void updateMeshVerticesDisplacements(const Reference<NodeIndexer> &indexer, Reference<Mesh> &mesh) const override {
+        std::vector<Scalar> d((size_t)dofs);
+        if (Severe(od_integrator_get_state(it_, d.data(), nullptr, nullptr), "od_integrator_get_state")) return;
+        std::vector<Vector3> u((size_t)mesh->getNodeCount());
+        indexer->scatterFree(mesh, d.data(), u.data());
+        for (int n = 0; n < mesh->getNodeCount(); n++) mesh->getVertexDisplacement(n) = u[n];
+    }
+    void getMeshVerticesVelocities(const Reference<NodeIndexer> &indexer, const Reference<Mesh> &mesh,
+                                   Vector3 *velocities) const override {
+        std::vector<Scalar> v((size_t)dofs);
+        if (Severe(od_integrator_get_state(it_, nullptr, v.data(), nullptr), "od_integrator_get_state")) return;
+        indexer->scatterFree(mesh, v.data(), velocities);
+    }
+    // one-step nodal impulse (no reference counterpart): added to the next step's load only
+    void setImpulse(const Forcer &forcer) {
+        std::vector<Scalar> vws((size_t)dofs);
+        forcer.getVirtualWorks(dofs, dofs, nullptr, vws.data());
+        status_ = Severe(od_integrator_set_impulse(it_, vws.data()), "od_integrator_set_impulse");
+    }
     int lastIterations() const {
         int it = 0;
         od_integrator_stats(it_, &it, nullptr, nullptr, nullptr);
@@ -528,59 +910,128 @@ public:
     int lastStatus() const { return status_; }
 
 protected:
-    int dofs;
-    Scalar massDamping, stiffnessDamping, timeStep;
+    int create(const Reference<Mesh> &mesh, const Reference<NodeIndexer> &indexer,
+               const Reference<FullOrderNonlinearMaterial<BlockedSymSpMatrix>> &material,
+               const Reference<LinearSolver<BlockedSymSpMatrix>> &linearSolver, bool newmark, Scalar beta, Scalar gamma) {
+        const GpuCGSolver *cg = dynamic_cast<const GpuCGSolver *>(linearSolver.get());
+        if (!cg) {
+            std::fprintf(stderr, "odgpu: the device integrators take a GpuCGSolver\n");
+            return OD_ERR_ARG;
+        }
+        mesh_ = mesh;
+        indexer_ = indexer;
+        int st = material->bindTo(mesh, indexer);
+        if (st) return st;
+        const Preconditioner &p = *cg->getPreconditioner();
+        st = newmark ? Severe(od_newmark_create(indexer->deviceMesh(), beta, gamma, timeStep, massDamping, stiffnessDamping,
+                                                p.deviceKind(), p.deviceParam1(), cg->tolerance(), cg->maxIterations(),
+                                                cg->precision(), 1, &it_),
+                              "od_newmark_create")
+                     : Severe(od_backward_euler_create(indexer->deviceMesh(), timeStep, massDamping, stiffnessDamping,
+                                                       p.deviceKind(), p.deviceParam1(), cg->tolerance(),
+                                                       cg->maxIterations(), &it_),
+                              "od_backward_euler_create");
+        if (st) return st;
+        od_solver *s = nullptr;
+        st = Severe(od_integrator_solver(it_, &s), "od_integrator_solver");
+        if (st == OD_OK) st = Severe(od_solver_set_precond_params(s, p.deviceParam1(), p.deviceParam2()),
+                                     "od_solver_set_precond_params");
+        return st;
+    }
+    Reference<Mesh> mesh_;
+    Reference<NodeIndexer> indexer_;
     od_integrator *it_ = nullptr;
     int status_ = OD_OK;
 };
 
-// Full-order linearised Newmark: the reference's Newmark update
-// (intergrator/linearNewmark.cpp:27-30, 57-79) at full order with the
-// assembly structure of NonlinearImplicitBackwardEuler.
-class NewmarkIntegrator : public Intergrator {
+// Full-order linearised Newmark (the reference's update, intergrator/linearNewmark.cpp:27-30,
+// 57-79, at full order with the assembly structure of NonlinearImplicitBackwardEuler), with the
+// constructor arguments of NonlinearImplicitBackwardEuler (nonlinearImplicitBackwardEuler.h:18-20)
+// plus beta and gamma.
+class GpuNewmark : public GpuIntergrator {
 public:
-    NewmarkIntegrator(const TetMesh &mesh, const FullOrderNonlinearMaterial &material, Scalar beta, Scalar gamma,
-                      Scalar ts, Scalar massDamp, Scalar stiffDamp, Scalar tol, int maxIter,
-                      const Preconditioner &precondition, int precision = OD_FP64, bool initialAcceleration = true)
-        : Intergrator(mesh.getMatrixOrder(), massDamp, stiffDamp, ts) {
-        status_ = material.bindTo(mesh);
-        if (status_ == OD_OK)
-            status_ = Severe(od_newmark_create(mesh.get(), beta, gamma, ts, massDamp, stiffDamp, precondition.kind(),
-                                               precondition.param(), tol, maxIter, precision,
-                                               initialAcceleration ? 1 : 0, &it_),
-                             "od_newmark_create");
+    GpuNewmark(int DOFS, Scalar beta, Scalar gamma, Scalar massDamp, Scalar stiffDamp, Scalar ts,
+               const Reference<Mesh> &m, const Reference<NodeIndexer> &nodeIndexer,
+               const Reference<FullOrderNonlinearMaterial<BlockedSymSpMatrix>> &mater,
+               const Reference<LinearSolver<BlockedSymSpMatrix>> &linearSolver)
+        : GpuIntergrator(DOFS, massDamp, stiffDamp, ts) {
+        status_ = create(m, nodeIndexer, mater, linearSolver, true, beta, gamma);
+    }
+    // core/intergrator.h:19-24: the base reports external damping unimplemented
+    void addExternalDampingForceMatrix(const StaDampingForcer &forcer) override {
+        Intergrator::addExternalDampingForceMatrix(forcer);
+        status_ = OD_ERR_UNSUPPORTED;
+    }
+    void clearExternalDampingForceMatrix() override {
+        Intergrator::clearExternalDampingForceMatrix();
+        status_ = OD_ERR_UNSUPPORTED;
     }
 };
 
-// NonlinearImplicitBackwardEuler (intergrator/nonlinearImplicitBackwardEuler.h:16-47):
-// same constructor parameters, with the linear solver given as (tol, maxIter,
-// preconditioner).  updateMeshVerticesDisplacements advances u += h v first,
-// as the reference's does (nonlinearImplicitBackwardEuler.h:103-116).
-class NonlinearImplicitBackwardEuler : public Intergrator {
+// NonlinearImplicitBackwardEuler(DOFS, massDamp, stiffDamp, ts, mesh, indexer, material,
+// linearSolver) (intergrator/nonlinearImplicitBackwardEuler.h:16-47).  updateMeshVerticesDisplacements
+// first advances u += h v (nonlinearImplicitBackwardEuler.h:103-116).
+class GpuBackwardEuler : public GpuIntergrator {
 public:
-    NonlinearImplicitBackwardEuler(const TetMesh &mesh, const FullOrderNonlinearMaterial &material, Scalar massDamp,
-                                   Scalar stiffDamp, Scalar ts, Scalar tol, int maxIter,
-                                   const Preconditioner &precondition)
-        : Intergrator(mesh.getMatrixOrder(), massDamp, stiffDamp, ts) {
-        status_ = material.bindTo(mesh);
-        if (status_ == OD_OK)
-            status_ = Severe(od_backward_euler_create(mesh.get(), ts, massDamp, stiffDamp, precondition.kind(),
-                                                      precondition.param(), tol, maxIter, &it_),
-                             "od_backward_euler_create");
+    GpuBackwardEuler(int DOFS, Scalar massDamp, Scalar stiffDamp, Scalar ts, const Reference<Mesh> &m,
+                     const Reference<NodeIndexer> &nodeIndexer,
+                     const Reference<FullOrderNonlinearMaterial<BlockedSymSpMatrix>> &mater,
+                     const Reference<LinearSolver<BlockedSymSpMatrix>> &linearSolver)
+        : GpuIntergrator(DOFS, massDamp, stiffDamp, ts) {
+        status_ = create(m, nodeIndexer, mater, linearSolver, false, 0.0, 0.0);
     }
-    void updateMeshVerticesDisplacements(TetMesh &mesh) const override {
+    void updateMeshVerticesDisplacements(const Reference<NodeIndexer> &indexer, Reference<Mesh> &mesh) const override {
         if (Severe(od_backward_euler_update_mesh(it_), "od_backward_euler_update_mesh")) return;
-        Intergrator::updateMeshVerticesDisplacements(mesh);
+        GpuIntergrator::updateMeshVerticesDisplacements(indexer, mesh);
     }
-    // nonlinearImplicitBackwardEuler.h:122-128
-    void addExternalDampingForceMatrix(const MeshPlaneCollisionDampingForcer &forcer) override {
-        status_ = Severe(od_integrator_add_plane_damping(it_, forcer.pairCount(), forcer.planes(), forcer.elements(),
-                                                         forcer.coefficient()),
-                         "od_integrator_add_plane_damping");
+    // nonlinearImplicitBackwardEuler.h:122-128: the forcer adds into the damping matrix
+    void addExternalDampingForceMatrix(const StaDampingForcer &forcer) override {
+        od_matrix *D = nullptr;
+        status_ = Severe(od_integrator_damping_matrix(it_, &D), "od_integrator_damping_matrix");
+        if (status_) return;
+        BlockedSymSpMatrix view(indexer_, D);
+        SparseSymMatrixIndicesPerElementCache none;
+        forcer.addDampingMatrix(none, view);
     }
     void clearExternalDampingForceMatrix() override {
         status_ = Severe(od_integrator_clear_damping(it_), "od_integrator_clear_damping");
     }
 };
+
+// ---- the driver (core/simulator.h:12-28, core/simulator.cpp:24-38) ---------------------------
+class Simulator {
+public:
+    Simulator() : intergrator(nullptr) {}
+    Simulator(const Reference<Mesh> m, const Reference<NodeIndexer> &nodeIndexer, Intergrator *intergrate)
+        : mesh(m), indexer(nodeIndexer), intergrator(intergrate) {}
+    Simulator(const Simulator &) = delete;
+    Simulator &operator=(const Simulator &) = delete;
+    void getVertexPositions(Vector3f *vertices) {
+        intergrator->updateMeshVerticesDisplacements(indexer, mesh);
+        for (int v = 0; v < mesh->getNodeCount(); v++) {
+            const Vector3 p = mesh->getVertex(v) + mesh->getVertexDisplacement(v);
+            for (int a = 0; a < 3; a++) vertices[v][a] = float(p[a]);
+        }
+    }
+    void runTimeSteps(int stepCount) {
+        for (int step = 0; step < stepCount; step++) intergrator->runOneTimeStep();
+    }
+    ~Simulator() { delete intergrator; }   // the simulator owns its integrator (core/simulator.cpp:41-43)
+
+private:
+    Reference<Mesh> mesh;
+    Reference<NodeIndexer> indexer;
+    Intergrator *intergrator;
+};
+
+// The reference's class names for the GPU implementations.
+using CGSolver = GpuCGSolver;
+using JacobiPreconditioner = GpuJacobiPreconditioner;
+using InCholeskyPreconditioner = GpuInCholeskyPreconditioner;
+using InLDLTPreconditioner = GpuInLDLTPreconditioner;
+using CorotationalHyperelasticMaterial = GpuCorotationalMaterial;
+using InvertibleNeoHookeanMaterial = GpuNeoHookeanMaterial;
+using NonlinearImplicitBackwardEuler = GpuBackwardEuler;
+using BlockedSymSpMatrixAssembler = BlockedSymSpMatrix::Assembler;
 
 }  // namespace odgpu

@@ -37,6 +37,7 @@ EXPORTS = [
     "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain", "od_plane_damping",
     "od_integrator_add_plane_damping", "od_integrator_clear_damping", "od_integrator_damping", "od_relabel_gps",
     "od_solver_info", "od_integrator_solver", "od_solver_set_precond_params", "od_pcg_solve_noalloc",
+    "od_integrator_damping_matrix", "od_precond_preprocess", "od_precond_solve",
 ]
 
 
@@ -92,7 +93,8 @@ def lib():
             "od_ctx_profile": [vp, i], "od_ctx_profile_read": [vp, dp, C.POINTER(ll)],
             "od_solver_info": [vp, C.POINTER(ll)], "od_integrator_solver": [vp, pvp],
             "od_solver_set_precond_params": [vp, d, d],
-            "od_pcg_solve_noalloc": [vp, i, vp, vp, d, i, ip, dp],
+            "od_pcg_solve_noalloc": [vp, i, vp, vp, d, i, ip, dp], "od_integrator_damping_matrix": [vp, pvp],
+            "od_precond_preprocess": [vp, vp], "od_precond_solve": [vp, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

@@ -206,6 +206,9 @@ struct od_solver {
     double param = 1e-3;        // IC(0) pivot epsilon; threshold IC epsilon; SAINV epsilon
     double param2 = 1.0;        // threshold IC relaxedScale; LDL^T entry epsilon
     ScalarPrecond *scalar = nullptr;
+    // the preconditioner holds a factor of a matrix (od_precond_preprocess / od_precond_apply;
+    // a PCG solve rebuilds it from its own matrix)
+    bool preprocessed = false;
     int precision = OD_FP64;
     double innerTol = 1e-4;     // mixed: inner FP32 relative tolerance (floor; see kInnerOuterScale2)
     int maxOuter = 12;
@@ -266,7 +269,7 @@ struct od_integrator {
     bool initAccel = true;
     int steps = 0;
     od_solver *solver = nullptr;
-    od_matrix M, A;
+    od_matrix M, A, Dm;   // Dm: the damping matrix D as a handle (backward Euler, od_integrator_damping_matrix)
     double *d = nullptr, *v = nullptr, *a = nullptr, *aNew = nullptr, *fext = nullptr, *imp = nullptr, *rhs = nullptr;
     uint8_t *active = nullptr, *inactive = nullptr;
     bool hasImpulse = false, hasActive = false;
@@ -287,7 +290,7 @@ struct od_integrator {
         return ptr;
     }
     ~od_integrator() {
-        M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr;   // owned by allocs
+        M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr; Dm.vals = nullptr;   // owned by allocs
         for (void *ptr : allocs) (void)hipFree(ptr);
         delete solver;
     }
@@ -1572,6 +1575,37 @@ int od_pcg_solve_noalloc(od_matrix *A, int precond, const double *b, double *x, 
     API_END
 }
 
+// Applies the preconditioner built last (IC(0): U and, FP64 sweeps, Ul; scalar kinds: the
+// uploaded sweeps; Jacobi: D^-1).
+static void precond_solve(od_solver *s, const double *r, double *z) {
+    od_mesh *mesh = s->mesh;
+    PcgWork w{s->partials, s->scIn, s->doneIn};
+    if (s->precond == OD_PRE_IC0)
+        launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U, s->Ul, r, z, nullptr, nullptr);
+    else if (s->scalar) launch_scalar_apply(s->scalar->dev, mesh->ctx->stream, r, z);
+    else pcg_jacobi<double>(mesh->dm, mesh->ctx->stream, s->precond == OD_PRE_JACOBI ? s->invDiag : nullptr, r, z, w);
+}
+
+int od_precond_preprocess(od_solver *s, od_matrix *A) {
+    API_BEGIN
+    require(s && A && A->mesh == s->mesh, "bad arguments");
+    if (s->precond == OD_PRE_IC0 && !s->Ul)
+        s->Ul = (double *)s->alloc(8 * (9 * (size_t)s->mesh->topo.tRow.size() + kValuePad));
+    build_precond(s, A, s->zeroMask);
+    s->preprocessed = true;
+    API_END
+}
+
+int od_precond_solve(od_solver *s, const double *r, double *z) {
+    API_BEGIN
+    require(s && r && z, "NULL argument");
+    require(r != z, "r and z must differ");
+    require(s->preprocessed || s->precond == OD_PRE_NONE, "od_precond_preprocess first", OD_ERR_STATE);
+    precond_solve(s, r, z);
+    OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
+    API_END
+}
+
 int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
     API_BEGIN
     require(s && A && r && z && A->mesh == s->mesh, "bad arguments");
@@ -1580,11 +1614,8 @@ int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
     if (s->precond == OD_PRE_IC0 && !s->Ul)
         s->Ul = (double *)s->alloc(8 * (9 * (size_t)mesh->topo.tRow.size() + kValuePad));
     build_precond(s, A, s->zeroMask);
-    PcgWork w{s->partials, s->scIn, s->doneIn};
-    if (s->precond == OD_PRE_IC0)
-        launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U, s->Ul, r, z, nullptr, nullptr);
-    else if (s->scalar) launch_scalar_apply(s->scalar->dev, mesh->ctx->stream, r, z);
-    else pcg_jacobi<double>(mesh->dm, mesh->ctx->stream, s->precond == OD_PRE_JACOBI ? s->invDiag : nullptr, r, z, w);
+    s->preprocessed = true;
+    precond_solve(s, r, z);
     OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
     API_END
 }
@@ -1894,6 +1925,23 @@ int od_integrator_add_plane_damping(od_integrator *it, int nPairs, const double 
     }
     plane_damping(it->mesh, it->D, nPairs, planes, tets, coeff, it->d, it->v);
     it->hasDamp = true;
+    API_END
+}
+
+int od_integrator_damping_matrix(od_integrator *it, od_matrix **D) {
+    API_BEGIN
+    require(it && D, "NULL argument");
+    require(it->kind == 1, "external damping matrices are a backward-Euler feature (Intergrator reports them unimplemented)",
+            OD_ERR_UNSUPPORTED);
+    if (!it->D) {
+        const size_t nv = (size_t)it->mesh->topo.nValues;
+        it->D = (double *)it->alloc(8 * (nv + kValuePad));
+        it->Dv = (double *)it->alloc(8 * (size_t)it->mesh->topo.ndof);
+    }
+    it->Dm.mesh = it->mesh;
+    it->Dm.vals = it->D;
+    it->hasDamp = true;
+    *D = &it->Dm;
     API_END
 }
 

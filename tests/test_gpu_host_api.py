@@ -1,6 +1,8 @@
-"""The C++ host classes (include/od_gpu.hpp), used as reference-style host code
-uses CGSolver, SpMDV, the materials, the Intergrator and a plane collision damping
-forcer, against the oracle.
+"""The C++ host classes (include/od_gpu.hpp), used the way host code written against the
+reference's interfaces uses them (TetMesh, Constrainer, NodeIndexer, a material's
+getMatrixStructure / getMatrixIndicesPerElement / generateMatrixAndVirtualWorks, SpMDV, a
+CGSolver with an IC(0) Preconditioner, a Forcer, a Simulator stepping a GpuNewmark, a
+GpuBackwardEuler with a plane collision damping forcer), against the oracle.
 
 tests/cpp/host_api_check is built by the package build; this test writes its
 input file, runs it and compares every output it writes.
@@ -50,7 +52,7 @@ def test_host_classes_match_oracle(tmp_path):
         w.tets.astype(np.int32).ravel().tofile(f)
         np.asarray(w.fixed_dofs, np.int32).tofile(f)
         np.array([rho, E, nu, w.dt], np.float64).tofile(f)
-        for v in (u_node, x, y0, b, g):
+        for v in (u_node, x, y0, b):
             np.asarray(v, np.float64).tofile(f)
         np.array([len(ptets)], np.int32).tofile(f)
         planes.astype(np.float64).ravel().tofile(f)
@@ -69,7 +71,10 @@ def test_host_classes_match_oracle(tmp_path):
         return a
 
     K = take(nv); vws = take(n); y = take(n); sol = take(n); iters = int(take(1, np.int32)[0])
-    d_node = take(3 * nn); v_node = take(3 * nn); v_be = take(3 * nn)
+    z_ic0 = take(n); z_ict = take(n)
+    g_host = take(n); d_node = take(3 * nn); v_node = take(3 * nn); pos = take(3 * nn, np.float32)
+    v_be = take(3 * nn)
+    assert relerr(g_host, g) < 1e-13        # Forcer::addBodyForce + getVirtualWorks
 
     s.set_free_displacements(u_free)
     Kc, fc = s.assemble(w.kind, w.E, w.nu)
@@ -80,6 +85,8 @@ def test_host_classes_match_oracle(tmp_path):
     xs = np.linalg.solve(s.dense(A), b)
     assert 0 < iters < 5000
     assert relerr(sol, xs) < 1e-8
+    assert relerr(z_ic0, s.precond_apply(A, b, O.PRE_IC0, 1e-3)) < 1e-10
+    assert relerr(z_ict, s.precond_apply(A, b, O.PRE_ICT, 1e-3, 1.0)) < 1e-13
 
     ref = O.Newmark(s, w.kind, w.E, w.nu, w.rho, w.dt, pre=O.PRE_IC0, p1=1e-3)
     ref.set_external(g)
@@ -88,6 +95,9 @@ def test_host_classes_match_oracle(tmp_path):
     assert relerr(d_node[free], do) < 1e-6
     assert relerr(v_node[free], vo) < 1e-6
     assert np.all(d_node[np.asarray(w.fixed_dofs)] == 0)
+    # Simulator::getVertexPositions: float X + u
+    d_full = np.zeros(3 * nn); d_full[free] = do
+    np.testing.assert_allclose(pos, (w.X.ravel() + d_full).astype(np.float32), rtol=1e-6, atol=1e-6)
 
     s2 = O.System(w.X, w.tets, w.fixed_dofs)
     be = O.BackwardEuler(s2, w.kind, w.E, w.nu, w.rho, w.dt, pre=O.PRE_IC0, p1=1e-3)
