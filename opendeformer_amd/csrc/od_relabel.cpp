@@ -12,6 +12,7 @@
 // membership tests and removals the reference does with std::find are O(1).
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <queue>
@@ -36,7 +37,7 @@ struct Graph {
     void eraseFirst(int v, int u) {      // first occurrence from the front (MeshGraphVertexNode::deleteArc)
         std::vector<int> &a = rev[v];
         for (size_t i = a.size(); i-- > 0;)
-            if (a[i] == u) { a.erase(a.begin() + (ptrdiff_t)i); return; }
+            if (a[i] == u) { a.erase(a.begin() + (std::ptrdiff_t)i); return; }
     }
 };
 

@@ -1,0 +1,163 @@
+// CPU driver for the sanitizer build (tests/test_sanitizers.py): runs the library's host code
+// (topology set-up, GPS relabelling, the scalar factorisations and their sweep schedules) and
+// the oracle (system set-up, assembly, CG, Newmark, GPS) on small meshes under AddressSanitizer
+// and UndefinedBehaviorSanitizer.  Exits non-zero on a wrong result; the sanitizers abort on
+// any memory or undefined-behaviour error.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../opendeformer_amd/csrc/od_precond.h"
+#include "../../opendeformer_amd/csrc/od_setup.h"
+
+extern "C" {
+void *oro_system_create(int nNodes, const double *X, int nTets, const int *tets, int nCon, const int *con);
+void oro_system_destroy(void *h);
+void oro_system_sizes(void *h, long long *out);
+void oro_mass(void *h, const double *rhoPerTet, double *M);
+void oro_assemble(void *h, const int *kind, const double *E, const double *nu, double thrC, double thrN, double *K,
+                  double *vws);
+int oro_cg(void *h, const double *values, const double *rhs, double *x, double tol, int maxIter, int pre, double p1,
+           double p2, double *relres);
+void *oro_newmark_create(void *sys, const int *kind, const double *E, const double *nu, double thrC, double thrN,
+                         const double *rho, double beta, double gamma, double dt, double aM, double aK, int pre,
+                         double p1, double p2, double tol, int maxIter, int initAccel);
+void oro_newmark_destroy(void *h);
+void oro_newmark_set_external(void *h, const double *f);
+int oro_newmark_step(void *h, int nSteps, int *iters, double *relres);
+int oro_gps(int nNodes, int nElems, int npe, int *elements, int *labels);
+}
+
+namespace {
+struct Box {
+    std::vector<double> X;
+    std::vector<int> tets, fixed;   // fixed: constrained DOFs (the x = 0 face)
+    int nNodes = 0;
+};
+// Kuhn-split box of nx x ny x nz unit cubes, nodes numbered x slowest
+Box box(int nx, int ny, int nz, int nodeOffset = 0) {
+    Box b;
+    auto id = [&](int i, int j, int k) { return k + (nz + 1) * (j + (ny + 1) * i); };
+    b.nNodes = (nx + 1) * (ny + 1) * (nz + 1);
+    for (int i = 0; i <= nx; i++)
+        for (int j = 0; j <= ny; j++)
+            for (int k = 0; k <= nz; k++) {
+                b.X.insert(b.X.end(), {0.1 * i + 0.013 * ((j * 7 + k * 3) % 5 - 2) * (i > 0 && i < nx),
+                                        0.1 * j, 0.1 * k});
+                if (i == 0)
+                    for (int a = 0; a < 3; a++) b.fixed.push_back(3 * (id(i, j, k) + nodeOffset) + a);
+            }
+    const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    const bool even[6] = {true, false, false, true, true, false};
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++)
+                for (int p = 0; p < 6; p++) {
+                    int a[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+                    a[perm[p][0]] = 1;
+                    c[perm[p][0]] = 1;
+                    c[perm[p][1]] = 1;
+                    const int v0 = id(i, j, k), va = id(i + a[0], j + a[1], k + a[2]), vb = id(i + c[0], j + c[1], k + c[2]),
+                              v1 = id(i + 1, j + 1, k + 1);
+                    const int t[4] = {v0, even[p] ? va : vb, even[p] ? vb : va, v1};
+                    for (int q = 0; q < 4; q++) b.tets.push_back(t[q] + nodeOffset);
+                }
+    return b;
+}
+int fail(const char *what) {
+    std::fprintf(stderr, "sanitize_check: %s\n", what);
+    return 1;
+}
+}  // namespace
+
+int main() {
+    // ---- library host code: topology of one box, of two instances, a partial constraint
+    Box b = box(6, 3, 3);
+    od::Topology t;
+    t.nNodes = b.nNodes;
+    t.nTets = (int)b.tets.size() / 4;
+    t.X = b.X;
+    t.tets = b.tets;
+    t.constrained = b.fixed;
+    if (!od::buildTopology(t, 64).empty()) return fail("buildTopology");
+    if (t.ndof != 3 * b.nNodes - (int)b.fixed.size()) return fail("ndof");
+    Box b2 = box(3, 2, 2, b.nNodes);
+    od::Topology t2;
+    t2.nNodes = b.nNodes + b2.nNodes;
+    t2.X = b.X;
+    t2.X.insert(t2.X.end(), b2.X.begin(), b2.X.end());
+    t2.tets = b.tets;
+    t2.tets.insert(t2.tets.end(), b2.tets.begin(), b2.tets.end());
+    t2.nTets = (int)t2.tets.size() / 4;
+    t2.constrained = b.fixed;
+    t2.constrained.insert(t2.constrained.end(), b2.fixed.begin(), b2.fixed.end());
+    t2.instNodeStart = {0, b.nNodes, t2.nNodes};
+    t2.instTetStart = {0, t.nTets, t2.nTets};
+    if (!od::buildTopology(t2, 64).empty() || t2.nInst != 2) return fail("two instances");
+    od::Topology t3 = od::Topology();
+    t3.nNodes = b.nNodes;
+    t3.nTets = t.nTets;
+    t3.X = b.X;
+    t3.tets = b.tets;
+    t3.constrained = {0, 1};   // a partially constrained node is rejected with a message
+    if (od::buildTopology(t3, 64).empty()) return fail("partial constraint accepted");
+
+    // ---- GPS relabelling (library and oracle)
+    std::vector<int> el = b.tets, el2 = b.tets, lab((size_t)b.nNodes), lab2((size_t)b.nNodes);
+    if (od::relabelGps(b.nNodes, t.nTets, 4, el.data(), lab.data()) != nullptr) return fail("relabelGps");
+    if (oro_gps(b.nNodes, t.nTets, 4, el2.data(), lab2.data()) != 0) return fail("oro_gps");
+    if (lab != lab2 || el != el2) return fail("GPS labels differ");
+
+    // ---- oracle: system, mass + stiffness, CG, Newmark
+    void *sys = oro_system_create(b.nNodes, b.X.data(), t.nTets, b.tets.data(), (int)b.fixed.size(), b.fixed.data());
+    long long sz[5];
+    oro_system_sizes(sys, sz);
+    const int n = (int)sz[0];
+    const long long nv = sz[4];
+    if (n != t.ndof || nv != t.nValues) return fail("oracle sizes");
+    std::vector<int> kind((size_t)t.nTets);
+    for (int e = 0; e < t.nTets; e++) kind[e] = e % 3 == 0 ? 1 : (e % 3 == 1 ? 0 : 2);
+    std::vector<double> E((size_t)t.nTets, 1e5), nu((size_t)t.nTets, 0.4), rho((size_t)t.nTets, 1000.0);
+    std::vector<double> M((size_t)nv, 0.0), K((size_t)nv, 0.0), f((size_t)n, 0.0);
+    oro_mass(sys, rho.data(), M.data());
+    oro_assemble(sys, kind.data(), E.data(), nu.data(), 0.0, 0.1, K.data(), f.data());
+    std::vector<double> A((size_t)nv);
+    for (long long i = 0; i < nv; i++) A[i] = M[i] + 1e-4 * K[i];
+    std::vector<double> rhs((size_t)n), x((size_t)n, 0.0);
+    for (int i = 0; i < n; i++) rhs[i] = std::sin(0.37 * i);
+    double rr = 0;
+    const int pres[4][3] = {{1, 0, 0}, {3, 0, 0}, {2, 1, 0}, {4, 1, 1}};
+    for (const auto &p : pres) {
+        std::fill(x.begin(), x.end(), 0.0);
+        const int it = oro_cg(sys, A.data(), rhs.data(), x.data(), 1e-8, 2000, p[0], p[1] ? 1e-3 : 1e-3,
+                              p[2] ? 1e-3 : 1.0, &rr);
+        if (it <= 0 || !(rr <= 1e-8)) return fail("oracle CG");
+    }
+    void *nm = oro_newmark_create(sys, kind.data(), E.data(), nu.data(), 0.0, 0.1, rho.data(), 0.25, 0.5, 1.0 / 60, 0.0,
+                                  0.0, 3, 1e-3, 0.0, 1e-6, 1000, 1);
+    std::vector<double> g((size_t)n, 0.0);
+    for (int i = 2; i < n; i += 3) g[i] = -0.01;
+    oro_newmark_set_external(nm, g.data());
+    int its[3];
+    double rrs[3];
+    oro_newmark_step(nm, 3, its, rrs);
+    oro_newmark_destroy(nm);
+    oro_system_destroy(sys);
+
+    // ---- library scalar factorisations and sweep schedules on A
+    od::ScalarFactor L;
+    od::ScalarSweepsHost s;
+    const double params[3][2] = {{0.0, 1.0}, {1e-3, 1.0}, {5e-2, 0.5}};
+    for (const auto &p : params) {
+        od::thresholdIcFactor(t, A.data(), p[0], p[1], L);
+        od::buildScalarSweeps(L, false, s);
+        if ((int)L.pcol.size() != n + 1 || s.fwd.levelRows.size() != (size_t)n) return fail("threshold IC");
+        od::sainvLdltFactor(t, A.data(), p[0] * 0.1, p[0], L);
+        od::buildScalarSweeps(L, true, s);
+        if ((int)L.pcol.size() != n + 1 || s.bwd.levelRows.size() != (size_t)n) return fail("SAINV LDL^T");
+    }
+    std::printf("sanitize_check: ok (ndof %d, %lld values)\n", n, nv);
+    return 0;
+}
