@@ -11,7 +11,13 @@
  *     that stream and handles must not be used from several threads at once;
  *   - DOF vectors are over free DOFs in the reference's compacted numbering
  *     (core/nodeIndexer.cpp:12-25): a constrained DOF is removed and later
- *     DOFs shift down.  The device layout needs whole-node constraints.
+ *     DOFs shift down; they hold ndof entries (od_mesh_info).  Nodes may be
+ *     partially constrained (Constrain_X .. Constrain_XZ, core/constrainer.h:
+ *     16-22): the kernels then block the free DOFs in triples, the last one
+ *     padded, and entry points that take device vectors run on padded copies
+ *     (one device-to-device copy in, one out); matrices still upload, download
+ *     and export in the reference's layout with its truncated last blocks and
+ *     scalar remained columns (core/sparseMatrix.h:90-274).
  *
  * Each entry point cites the reference interface it replaces
  * (paths relative to OpenDeformer/ in the reference tree).
@@ -91,7 +97,8 @@ int od_relabel_gps(int nNodes, int nElems, int nodesPerElem, int *elements_host,
  * constrained_host: sorted DOF ids 3*node+axis (Constrainer::outputConstrains,
  * core/constrainer.cpp:74-79).  nInstances > 1 concatenates independent
  * meshes: instNodeStart/instTetStart (nInstances+1 entries) give each one's
- * node and tet range (NULL when nInstances == 1). */
+ * node and tet range (NULL when nInstances == 1); a mesh with partially
+ * constrained nodes has one instance. */
 int od_mesh_create(od_ctx *ctx, int nNodes, const double *X_host, int nTets, const int *tets_host,
                    int nConstrained, const int *constrained_host, int nInstances,
                    const int *instNodeStart_host, const int *instTetStart_host, od_mesh **out);
@@ -100,10 +107,13 @@ int od_mesh_destroy(od_mesh *mesh);
  * nNodes, nChunks, nPairContrib, nNodeContrib, device bytes, 1 when the IC(0)
  * levels are consecutive row ranges, then the sweep form of the first 8 levels
  * (forward, backward; 4 bits per level: log2 of the lanes per row, or 8 +
- * log2(slots) for the FP32 slot kernel), 0 */
+ * log2(slots) for the FP32 slot kernel), 1 when some node is partially
+ * constrained.  ndof and nValues are the reference's (vector entries, canonical
+ * values); nBlockRows and nBlocks count the kernels' blocks. */
 int od_mesh_info(od_mesh *mesh, long long *info_host);
 /* Canonical layout, identical to BlockedSymSparseMatrix<3,3>'s blockPcol,
- * blockColumnOris, blockRows (core/sparseMatrix.h:627-635), and the 78-entry
+ * blockColumnOris, blockRows (core/sparseMatrix.h:627-635; ndof/3 + ndof%3 + 1
+ * column entries, blockPcol's last entry of rows), and the 78-entry
  * per-tet map (SparseSymMatrixIndicesPerElementCache,
  * core/sparseMatrix.cpp:97-118), the latter expanded on the device from the
  * block-level map the assembly kernels use.  elemMap78_host may be NULL. */
@@ -169,6 +179,9 @@ int od_body_force(od_mesh *mesh, const double *bodyPerTet_host /* 3*nTets */, do
 /* ---- matrices in the canonical layout ---------------------------------------- */
 int od_matrix_create(od_mesh *mesh, od_matrix **out);
 int od_matrix_destroy(od_matrix *m);
+/* The device values in the kernels' layout: the reference's canonical layout with
+ * whole-node constraints; with partially constrained nodes the padded blocks
+ * (od_matrix_upload / od_matrix_download convert). */
 int od_matrix_values(od_matrix *m, double **values_dev);
 int od_matrix_upload(od_matrix *m, const double *values_host);
 int od_matrix_download(od_matrix *m, double *values_host);
@@ -232,8 +245,9 @@ int od_pcg_solve(od_solver *s, od_matrix *A, const double *b_dev, double *x_dev,
  * in a fixed 64 MiB device array per GPU, held until the process ends and looked up when a
  * context is created on that GPU.  Limits: FP64, OD_PRE_NONE or OD_PRE_JACOBI, at most 64
  * instances, 5 x 8 x ndof bytes plus the partials within 64 MiB (about 1.6M DOF); larger
- * systems are rejected with OD_ERR_ARG.  Same results as od_pcg_solve with the same
- * preconditioner.  Calls that overlap on one GPU share the array and race. */
+ * systems are rejected with OD_ERR_ARG, meshes with partially constrained nodes (which
+ * would need padded copies) with OD_ERR_UNSUPPORTED.  Same results as od_pcg_solve with
+ * the same preconditioner.  Calls that overlap on one GPU share the array and race. */
 int od_pcg_solve_noalloc(od_matrix *A, int precond, const double *b_dev, double *x_dev, double tol, int maxIter,
                          int *iters_host, double *relres_host);
 /* Preconditioner::Preprocess(mat) (core/preconditioner.h:18): builds the factor of A (IC(0),
@@ -303,7 +317,8 @@ int od_integrator_positions(od_integrator *it, float *pos_host);
  * rest.  A pair adds coeff dir n̂^T scaled by the barycentric weights of the
  * submerged part's centroid, only when the velocity there moves into the plane;
  * the stored diagonal-block entry (i+j, i) receives t(i, j) as in the reference
- * (penaltyDampingForcer.cpp:40-45).  The sums are in pair order, bit-reproducible. */
+ * (penaltyDampingForcer.cpp:40-45).  The sums are in pair order, bit-reproducible.
+ * Meshes with partially constrained nodes are rejected (OD_ERR_UNSUPPORTED). */
 int od_plane_damping(od_matrix *D, int nPairs, const double *planes_host, const int *tets_host, double coeff,
                      const double *disp_dev, const double *vel_dev);
 /* NonlinearImplicitBackwardEuler::addExternalDampingForceMatrix with a

@@ -107,6 +107,9 @@ class Mesh:
         check(lib().od_mesh_info(self.h, info.ctypes.data_as(C.POINTER(C.c_longlong))))
         (self.ndof, self.n_rows, self.n_blocks, self.n_values, self.n_levels, self.n_inst, self.n_tets,
          self.n_nodes, self.n_chunks, self.n_pair_contrib, self.n_node_contrib, self.device_bytes) = (int(v) for v in info[:12])
+        # some node partially constrained: the block rows are DOF triples (n_rows, n_blocks count
+        # the kernels' padded blocks; ndof, n_values and pattern() are the reference's layout)
+        self.partial = bool(info[15])
 
     def info_raw(self):
         """The 16 values of od_mesh_info (read now: the sweep lanes change when a solver tunes them)."""
@@ -126,8 +129,11 @@ class Mesh:
             pass
 
     def pattern(self, with_map=True):
-        pcol = np.zeros(self.n_rows + 1, np.int32); oris = np.zeros(self.n_rows + 1, np.int32)
-        rows = np.zeros(self.n_blocks, np.int32)
+        ncols = self.ndof // 3 + self.ndof % 3 if self.partial else self.n_rows
+        pcol = np.zeros(ncols + 1, np.int32); oris = np.zeros(ncols + 1, np.int32)
+        if self.partial:
+            check(lib().od_pattern_export(self.h, capi.iptr(pcol), None, None, None))
+        rows = np.zeros(int(pcol[-1]) if self.partial else self.n_blocks, np.int32)
         emap = np.zeros(78 * self.n_tets, np.int32) if with_map else None
         check(lib().od_pattern_export(self.h, capi.iptr(pcol), capi.iptr(oris), capi.iptr(rows),
                                       capi.iptr(emap) if with_map else None))

@@ -115,6 +115,21 @@ struct od_mesh {
     double tuneMs[2][3][3] = {};   // [FP64, FP32][16, 8, 4 lanes per row][R = 1, 2, 4]: cold launch time
     double tuneSvMs[3] = {};       // FP32 slot-value form, [1, 2, 4 slots per thread]: cold launch time
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
+    // Partially constrained nodes: the caller's DOF vectors hold topo.ndofFree entries, the
+    // kernels' topo.ndof (the last block row padded).  Entry points that take device vectors
+    // run on padded copies kept here (zeroed once; the kernels leave the padding at zero), and
+    // upload/download map the reference's value layout (padDiag: the padding diagonal values).
+    void *padBuf[2][3] = {};                     // [double, float][slot]
+    std::vector<int64_t> padDiag;
+    template <typename T>
+    T *padded(int slot) {
+        void *&b = padBuf[sizeof(T) == 4][slot];
+        if (!b) {
+            b = alloc(sizeof(T) * (size_t)topo.ndof);
+            OD_HIP(hipMemsetAsync(b, 0, sizeof(T) * (size_t)topo.ndof, ctx->stream));
+        }
+        return (T *)b;
+    }
     std::vector<uint8_t> levelLanesFwd, levelLanesBwd;   // DevMesh points at these (host memory)
     std::vector<int> levelChunkPtr;                      // per level, its range in dm.levelChunks
     template <typename T>
@@ -177,6 +192,43 @@ struct od_mesh {
         for (void *p : allocs) (void)hipFree(p);
     }
 };
+
+// A caller's device DOF vector as the kernels see it: the vector itself, or (partially
+// constrained nodes) a padded copy, filled from it when `in` and copied back by done() when
+// `out`.  Calls on one mesh are ordered on its context stream, so the mesh's copies serve them
+// in turn.
+template <typename T>
+struct DofVec {
+    od_mesh *mesh;
+    T *user;
+    T *dev;
+    bool out;
+    DofVec(od_mesh *m, int slot, const T *u, bool in, bool outp) : mesh(m), user(const_cast<T *>(u)), dev(user), out(outp) {
+        if (!m->topo.partial || !u) return;
+        dev = m->padded<T>(slot);
+        if (in)
+            OD_HIP(hipMemcpyAsync(dev, u, sizeof(T) * (size_t)m->topo.ndofFree, hipMemcpyDeviceToDevice, m->ctx->stream));
+    }
+    void done() {
+        if (dev != user && out)
+            OD_HIP(hipMemcpyAsync(user, dev, sizeof(T) * (size_t)mesh->topo.ndofFree, hipMemcpyDeviceToDevice,
+                                  mesh->ctx->stream));
+    }
+};
+
+// reference value layout <-> the kernels' (identity unless partially constrained)
+static void values_to_kernel(const od_mesh *mesh, const double *ref, std::vector<double> &out) {
+    const Topology &t = mesh->topo;
+    out.assign((size_t)t.nValues, 0.0);
+    for (int64_t v = 0; v < t.nValues; v++)
+        if (t.refPos[v] >= 0) out[v] = ref[t.refPos[v]];
+    for (int64_t v : mesh->padDiag) out[v] = 1.0;
+}
+static void values_from_kernel(const od_mesh *mesh, const double *in, double *ref) {
+    const Topology &t = mesh->topo;
+    for (int64_t v = 0; v < t.nValues; v++)
+        if (t.refPos[v] >= 0) ref[t.refPos[v]] = in[v];
+}
 
 struct od_matrix {
     od_mesh *mesh = nullptr;
@@ -899,6 +951,16 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.X = mesh->upload(t.X);
         m.tets = mesh->upload(t.tets);
         m.nodeRow = mesh->upload(t.nodeRow);
+        if (t.partial) {
+            m.nodeDof = mesh->upload(t.nodeDof);
+            m.vPtr = mesh->upload(t.vPtr);
+            m.vSrc = mesh->upload(t.vSrc);
+            m.dPtr = mesh->upload(t.dPtr);
+            m.dSrc = mesh->upload(t.dSrc);
+            for (int64_t v = 0; v < t.nValues; v++)
+                for (int k = t.vPtr[v]; k < t.vPtr[v + 1]; k++)
+                    if (t.vSrc[k] < 0) mesh->padDiag.push_back(v);
+        }
         m.rowPtr = mesh->upload(t.rowPtr);
         m.colIdx = mesh->upload(t.colIdx);
         m.blkOff = mesh->upload(t.blkOff);
@@ -1083,10 +1145,13 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.thrCorot = 0.0;
         m.thrNeo = 0.1;
         OD_HIP(hipStreamSynchronize(ctx->stream));
-        // host copies no longer needed on the per-step path
-        t.elemMap78.clear(); t.elemMap78.shrink_to_fit();
+        // host copies no longer needed on the per-step path (a partially constrained mesh keeps
+        // its 78-entry map: the device expands only the block-level one)
+        if (!t.partial) { t.elemMap78.clear(); t.elemMap78.shrink_to_fit(); }
         t.pairSlot.clear(); t.pairSlot.shrink_to_fit();
         t.nodeSlot.clear(); t.nodeSlot.shrink_to_fit();
+        t.vPtr.clear(); t.vPtr.shrink_to_fit(); t.vSrc.clear(); t.vSrc.shrink_to_fit();
+        t.dPtr.clear(); t.dPtr.shrink_to_fit(); t.dSrc.clear(); t.dSrc.shrink_to_fit();
     } catch (...) {
         delete mesh;
         throw;
@@ -1126,7 +1191,7 @@ int od_mesh_info(od_mesh *mesh, long long *info) {
     require(mesh && info, "NULL argument");
     const Topology &t = mesh->topo;
     for (int i = 0; i < 16; i++) info[i] = 0;
-    info[0] = t.ndof; info[1] = t.nRows; info[2] = t.nBlocks; info[3] = t.nValues; info[4] = t.nLevels;
+    info[0] = t.ndofFree; info[1] = t.nRows; info[2] = t.nBlocks; info[3] = t.nValuesRef; info[4] = t.nLevels;
     info[5] = t.nInst; info[6] = t.nTets; info[7] = t.nNodes; info[8] = (long long)t.chunkRow0.size();
     info[9] = t.nPairContrib; info[10] = t.nNodeContrib; info[11] = (long long)mesh->bytes;
     info[12] = t.levelsContiguous ? 1 : 0;
@@ -1137,6 +1202,7 @@ int od_mesh_info(od_mesh *mesh, long long *info) {
         info[13] |= (long long)lg(mesh->levelLanesFwd[l]) << (4 * l);
         info[14] |= (long long)lg(mesh->levelLanesBwd[l]) << (4 * l);
     }
+    info[15] = t.partial ? 1 : 0;
     API_END
 }
 
@@ -1144,6 +1210,13 @@ int od_pattern_export(od_mesh *mesh, int *blockPcol, int *blockColumnOris, int *
     API_BEGIN
     require(mesh != nullptr, "mesh is NULL");
     const Topology &t = mesh->topo;
+    if (t.partial) {
+        if (blockPcol) std::memcpy(blockPcol, t.refPcol.data(), sizeof(int) * t.refPcol.size());
+        if (blockColumnOris) std::memcpy(blockColumnOris, t.refOris.data(), sizeof(int) * t.refOris.size());
+        if (blockRows) std::memcpy(blockRows, t.refRows.data(), sizeof(int) * t.refRows.size());
+        if (elemMap78) std::memcpy(elemMap78, t.elemMap78.data(), sizeof(int) * t.elemMap78.size());
+        return OD_OK;
+    }
     if (blockPcol) std::memcpy(blockPcol, t.rowPtr.data(), sizeof(int) * (t.nRows + 1));
     if (blockColumnOris) std::memcpy(blockColumnOris, t.valOff.data(), sizeof(int) * (t.nRows + 1));
     if (blockRows)
@@ -1299,7 +1372,9 @@ int od_body_force(od_mesh *mesh, const double *body, double *f_dev) {
     size_t bytes = sizeof(double) * 3 * (size_t)mesh->topo.nTets;
     OD_HIP(hipMalloc(&db, bytes));
     OD_HIP(hipMemcpyAsync(db, body, bytes, hipMemcpyHostToDevice, mesh->ctx->stream));
-    launch_body_force(mesh->dm, mesh->ctx->stream, db, mesh->nodeC, f_dev);
+    DofVec<double> f(mesh, 0, f_dev, false, true);
+    launch_body_force(mesh->dm, mesh->ctx->stream, db, mesh->nodeC, f.dev);
+    f.done();
     OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
     OD_HIP(hipFree(db));
     API_END
@@ -1330,6 +1405,13 @@ int od_matrix_values(od_matrix *m, double **values) {
 int od_matrix_upload(od_matrix *m, const double *host) {
     API_BEGIN
     require(m && host, "NULL argument");
+    if (m->mesh->topo.partial) {
+        std::vector<double> v;
+        values_to_kernel(m->mesh, host, v);
+        OD_HIP(hipMemcpyAsync(m->vals, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, m->mesh->ctx->stream));
+        OD_HIP(hipStreamSynchronize(m->mesh->ctx->stream));
+        return OD_OK;
+    }
     OD_HIP(hipMemcpyAsync(m->vals, host, sizeof(double) * m->mesh->topo.nValues, hipMemcpyHostToDevice, m->mesh->ctx->stream));
     OD_HIP(hipStreamSynchronize(m->mesh->ctx->stream));
     API_END
@@ -1337,6 +1419,13 @@ int od_matrix_upload(od_matrix *m, const double *host) {
 int od_matrix_download(od_matrix *m, double *host) {
     API_BEGIN
     require(m && host, "NULL argument");
+    if (m->mesh->topo.partial) {
+        std::vector<double> v((size_t)m->mesh->topo.nValues);
+        OD_HIP(hipMemcpyAsync(v.data(), m->vals, sizeof(double) * v.size(), hipMemcpyDeviceToHost, m->mesh->ctx->stream));
+        OD_HIP(hipStreamSynchronize(m->mesh->ctx->stream));
+        values_from_kernel(m->mesh, v.data(), host);
+        return OD_OK;
+    }
     OD_HIP(hipMemcpyAsync(host, m->vals, sizeof(double) * m->mesh->topo.nValues, hipMemcpyDeviceToHost, m->mesh->ctx->stream));
     OD_HIP(hipStreamSynchronize(m->mesh->ctx->stream));
     API_END
@@ -1380,9 +1469,11 @@ int od_assemble(od_mesh *mesh, const double *u_dev, od_matrix *K, double *fint_d
     require(mesh->hasMaterial, "od_mesh_set_material must be called first", OD_ERR_STATE);
     mesh->ensureContrib();
     ElemParams P{0, fint_dev ? 1 : 0, 0, 0, 0, 0, 0};
-    launch_element(mesh->dm, mesh->ctx->stream, u_dev, nullptr, nullptr, P, mesh->blkC, mesh->nodeC, nullptr);
+    DofVec<double> u(mesh, 0, u_dev, true, false), f(mesh, 1, fint_dev, true, true);
+    launch_element(mesh->dm, mesh->ctx->stream, u.dev, nullptr, nullptr, P, mesh->blkC, mesh->nodeC, nullptr);
     launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, K->vals, 1, nullptr, nullptr);
-    if (fint_dev) launch_gather_rhs(mesh->dm, mesh->ctx->stream, mesh->nodeC, nullptr, nullptr, 1.0, fint_dev, 1, nullptr);
+    if (fint_dev) launch_gather_rhs(mesh->dm, mesh->ctx->stream, mesh->nodeC, nullptr, nullptr, 1.0, f.dev, 1, nullptr);
+    f.done();
     API_END
 }
 
@@ -1390,7 +1481,9 @@ int od_spmv(od_matrix *A, const double *x, double *y, int accumulate) {
     API_BEGIN
     require(A && x && y, "NULL argument");
     require(x != y, "x and y must differ (SpMDV asserts src != dest)");
-    launch_spmv<double>(A->mesh->dm, A->mesh->ctx->stream, A->vals, x, y, accumulate, nullptr);
+    DofVec<double> xv(A->mesh, 0, x, true, false), yv(A->mesh, 1, y, accumulate != 0, true);
+    launch_spmv<double>(A->mesh->dm, A->mesh->ctx->stream, A->vals, xv.dev, yv.dev, accumulate, nullptr);
+    yv.done();
     API_END
 }
 
@@ -1410,7 +1503,9 @@ int od_spmv_f32(od_matrix *A, const float *x, float *y, int accumulate) {
     API_BEGIN
     require(A && x && y && x != y, "bad arguments");
     require(A->vals32 != nullptr, "call od_matrix_sync_f32 first", OD_ERR_STATE);
-    launch_spmv<float>(A->mesh->dm, A->mesh->ctx->stream, A->vals32, x, y, accumulate, nullptr);
+    DofVec<float> xv(A->mesh, 0, x, true, false), yv(A->mesh, 1, y, accumulate != 0, true);
+    launch_spmv<float>(A->mesh->dm, A->mesh->ctx->stream, A->vals32, xv.dev, yv.dev, accumulate, nullptr);
+    yv.done();
     API_END
 }
 
@@ -1419,15 +1514,21 @@ int od_spmv_timed(od_matrix *A, const void *x, void *y, int iters, int fp32, dou
     require(A && x && y && avg_ms && iters > 0 && x != y, "bad arguments");
     require(!fp32 || A->vals32, "call od_matrix_sync_f32 first", OD_ERR_STATE);
     hipStream_t st = A->mesh->ctx->stream;
+    // padded copies (partially constrained nodes) are made outside the timed launches
+    DofVec<float> x32(A->mesh, 0, fp32 ? (const float *)x : nullptr, true, false), y32(A->mesh, 1, fp32 ? (float *)y : nullptr, false, true);
+    DofVec<double> x64(A->mesh, 0, fp32 ? nullptr : (const double *)x, true, false),
+        y64(A->mesh, 1, fp32 ? nullptr : (double *)y, false, true);
     hipEvent_t e0, e1;
     OD_HIP(hipEventCreate(&e0));
     OD_HIP(hipEventCreate(&e1));
     OD_HIP(hipEventRecord(e0, st));
     for (int k = 0; k < iters; k++) {
-        if (fp32) launch_spmv<float>(A->mesh->dm, st, A->vals32, (const float *)x, (float *)y, 0, nullptr);
-        else launch_spmv<double>(A->mesh->dm, st, A->vals, (const double *)x, (double *)y, 0, nullptr);
+        if (fp32) launch_spmv<float>(A->mesh->dm, st, A->vals32, x32.dev, y32.dev, 0, nullptr);
+        else launch_spmv<double>(A->mesh->dm, st, A->vals, x64.dev, y64.dev, 0, nullptr);
     }
     OD_HIP(hipEventRecord(e1, st));
+    y32.done();
+    y64.done();
     OD_HIP(hipEventSynchronize(e1));
     float ms = 0;
     OD_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1506,7 +1607,9 @@ int od_pcg_solve(od_solver *s, od_matrix *A, const double *b, double *x, double 
         }
         launch_cast_f64_f32(A->mesh->ctx->stream, A->vals, A->vals32, n);
     }
-    solve(s, A, b, x, tol, maxIter, nullptr);
+    DofVec<double> bv(A->mesh, 0, b, true, false), xv(A->mesh, 1, x, true, true);
+    solve(s, A, bv.dev, xv.dev, tol, maxIter, nullptr);
+    xv.done();
     OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
     int nInst = s->mesh->topo.nInst;
     if (iters) std::memcpy(iters, s->iters.data(), sizeof(int) * nInst);
@@ -1527,6 +1630,8 @@ int od_pcg_solve_noalloc(od_matrix *A, int precond, const double *b, double *x, 
                          double *relres) {
     API_BEGIN
     require(A && b && x, "NULL argument");
+    require(!A->mesh->topo.partial, "the allocation-free solve needs whole-node constraints (no padded copies)",
+            OD_ERR_UNSUPPORTED);
     require(tol >= 0 && maxIter >= 0, "tol and maxIter must be non-negative");
     require(precond == OD_PRE_NONE || precond == OD_PRE_JACOBI,
             "the allocation-free solve takes OD_PRE_NONE or OD_PRE_JACOBI", OD_ERR_UNSUPPORTED);
@@ -1601,7 +1706,9 @@ int od_precond_solve(od_solver *s, const double *r, double *z) {
     require(s && r && z, "NULL argument");
     require(r != z, "r and z must differ");
     require(s->preprocessed || s->precond == OD_PRE_NONE, "od_precond_preprocess first", OD_ERR_STATE);
-    precond_solve(s, r, z);
+    DofVec<double> rv(s->mesh, 0, r, true, false), zv(s->mesh, 1, z, false, true);
+    precond_solve(s, rv.dev, zv.dev);
+    zv.done();
     OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
     API_END
 }
@@ -1615,7 +1722,9 @@ int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
         s->Ul = (double *)s->alloc(8 * (9 * (size_t)mesh->topo.tRow.size() + kValuePad));
     build_precond(s, A, s->zeroMask);
     s->preprocessed = true;
-    precond_solve(s, r, z);
+    DofVec<double> rv(mesh, 0, r, true, false), zv(mesh, 1, z, false, true);
+    precond_solve(s, rv.dev, zv.dev);
+    zv.done();
     OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
     API_END
 }
@@ -1625,6 +1734,13 @@ int od_ic0_factor(od_solver *s, od_matrix *A, double *L) {
     require(s && A && L && A->mesh == s->mesh, "bad arguments");
     require(s->precond == OD_PRE_IC0, "solver was not created with OD_PRE_IC0", OD_ERR_STATE);
     build_precond(s, A, s->zeroMask);
+    if (s->mesh->topo.partial) {
+        std::vector<double> v((size_t)s->mesh->topo.nValues);
+        OD_HIP(hipMemcpyAsync(v.data(), s->U, sizeof(double) * v.size(), hipMemcpyDeviceToHost, s->mesh->ctx->stream));
+        OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
+        values_from_kernel(s->mesh, v.data(), L);
+        return OD_OK;
+    }
     OD_HIP(hipMemcpyAsync(L, s->U, sizeof(double) * s->mesh->topo.nValues, hipMemcpyDeviceToHost, s->mesh->ctx->stream));
     OD_HIP(hipStreamSynchronize(s->mesh->ctx->stream));
     API_END
@@ -1727,14 +1843,14 @@ int od_integrator_destroy(od_integrator *it) {
 int od_integrator_set_external(od_integrator *it, const double *f) {
     API_BEGIN
     require(it && f, "NULL argument");
-    OD_HIP(hipMemcpyAsync(it->fext, f, sizeof(double) * it->mesh->topo.ndof, hipMemcpyHostToDevice, it->mesh->ctx->stream));
+    OD_HIP(hipMemcpyAsync(it->fext, f, sizeof(double) * it->mesh->topo.ndofFree, hipMemcpyHostToDevice, it->mesh->ctx->stream));
     OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
     API_END
 }
 int od_integrator_set_impulse(od_integrator *it, const double *f) {
     API_BEGIN
     require(it && f, "NULL argument");
-    OD_HIP(hipMemcpyAsync(it->imp, f, sizeof(double) * it->mesh->topo.ndof, hipMemcpyHostToDevice, it->mesh->ctx->stream));
+    OD_HIP(hipMemcpyAsync(it->imp, f, sizeof(double) * it->mesh->topo.ndofFree, hipMemcpyHostToDevice, it->mesh->ctx->stream));
     OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
     it->hasImpulse = true;
     API_END
@@ -1830,7 +1946,7 @@ int od_integrator_step(od_integrator *it, int nSteps) {
 int od_integrator_get_state(od_integrator *it, double *d, double *v, double *a) {
     API_BEGIN
     require(it != nullptr, "NULL argument");
-    size_t b = sizeof(double) * it->mesh->topo.ndof;
+    size_t b = sizeof(double) * it->mesh->topo.ndofFree;
     hipStream_t st = it->mesh->ctx->stream;
     if (d) OD_HIP(hipMemcpyAsync(d, it->d, b, hipMemcpyDeviceToHost, st));
     if (v) OD_HIP(hipMemcpyAsync(v, it->v, b, hipMemcpyDeviceToHost, st));
@@ -1842,7 +1958,7 @@ int od_integrator_get_state(od_integrator *it, double *d, double *v, double *a) 
 int od_integrator_set_state(od_integrator *it, const double *d, const double *v, const double *a) {
     API_BEGIN
     require(it != nullptr, "NULL argument");
-    size_t b = sizeof(double) * it->mesh->topo.ndof;
+    size_t b = sizeof(double) * it->mesh->topo.ndofFree;
     hipStream_t st = it->mesh->ctx->stream;
     if (d) OD_HIP(hipMemcpyAsync(it->d, d, b, hipMemcpyHostToDevice, st));
     if (v) OD_HIP(hipMemcpyAsync(it->v, v, b, hipMemcpyHostToDevice, st));
@@ -1892,6 +2008,8 @@ int od_relabel_gps(int nNodes, int nElems, int nodesPerElem, int *elements, int 
 static void plane_damping(od_mesh *mesh, double *D, int nPairs, const double *planes, const int *tets, double coeff,
                           const double *disp, const double *vel) {
     require(nPairs >= 0 && nPairs <= (1 << 27), "number of pairs out of range");
+    require(!mesh->topo.partial, "plane damping needs whole-node constraints (its pairs address node blocks)",
+            OD_ERR_UNSUPPORTED);
     if (nPairs == 0) return;
     require(planes && tets, "NULL argument");
     for (int k = 0; k < nPairs; k++)
@@ -1959,9 +2077,15 @@ int od_integrator_clear_damping(od_integrator *it) {
 int od_integrator_damping(od_integrator *it, double *values) {
     API_BEGIN
     require(it && values, "NULL argument");
-    const size_t b = 8 * (size_t)it->mesh->topo.nValues;
+    const Topology &t = it->mesh->topo;
+    const size_t b = 8 * (size_t)t.nValues;
     if (!it->D) {
-        std::memset(values, 0, b);
+        std::memset(values, 0, 8 * (size_t)t.nValuesRef);
+    } else if (t.partial) {
+        std::vector<double> v((size_t)t.nValues);
+        OD_HIP(hipMemcpyAsync(v.data(), it->D, b, hipMemcpyDeviceToHost, it->mesh->ctx->stream));
+        OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
+        values_from_kernel(it->mesh, v.data(), values);
     } else {
         OD_HIP(hipMemcpyAsync(values, it->D, b, hipMemcpyDeviceToHost, it->mesh->ctx->stream));
         OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
@@ -1978,7 +2102,10 @@ int od_integrator_positions(od_integrator *it, float *pos) {
     OD_HIP(hipStreamSynchronize(it->mesh->ctx->stream));
     for (int n = 0; n < t.nNodes; n++) {
         int r = t.nodeRow[n];
-        for (int a = 0; a < 3; a++) pos[3 * n + a] = float(t.X[3 * n + a] + (r >= 0 ? d[3 * r + a] : 0.0));
+        for (int a = 0; a < 3; a++) {
+            const int o = t.partial ? t.nodeDof[3 * (size_t)n + a] : (r >= 0 ? 3 * r + a : -1);
+            pos[3 * n + a] = float(t.X[3 * n + a] + (o >= 0 ? d[o] : 0.0));
+        }
     }
     API_END
 }

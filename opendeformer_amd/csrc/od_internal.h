@@ -49,6 +49,11 @@ struct DevMesh {
     double *X;            // 3*nNodes
     int *tets;            // 4*nTets
     int *nodeRow;         // nNodes
+    // partially constrained nodes (else NULL): free DOF of each node axis (-1: constrained),
+    // and the value-level / DOF-level gather lists that replace the block-level ones
+    int *nodeDof;         // 3*nNodes
+    int *vPtr, *vSrc;     // nValues+1 / entries: pair slot * 9 + entry, -1 for a padding diagonal (1.0)
+    int *dPtr, *dSrc;     // 3*nRows+1 / entries: node slot * 3 + axis
     int *rowPtr, *colIdx, *blkOff, *blkRow;     // pattern
     int *valOff;                                 // first value of each block row (nRows+1)
     int *tPtr, *tRow, *tBlk, *tOff;             // transposed index (+ value offset of each block)

@@ -98,7 +98,20 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
         row[k] = m.nodeRow[tn[k]];
         X[k] = mk(m.X[3 * tn[k]], m.X[3 * tn[k] + 1], m.X[3 * tn[k] + 2]);
         v3 u = mk(0, 0, 0), vv = mk(0, 0, 0), aa = mk(0, 0, 0);
-        if (row[k] >= 0) {
+        if (m.nodeDof) {   // partially constrained nodes: per-axis DOFs
+            double c[3][3] = {};
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+                const int o = m.nodeDof[3 * tn[k] + ax];
+                if (o < 0) continue;
+                if (d) c[0][ax] = d[o];
+                if (v) c[1][ax] = v[o];
+                if (a) c[2][ax] = a[o];
+            }
+            u = mk(c[0][0], c[0][1], c[0][2]);
+            vv = mk(c[1][0], c[1][1], c[1][2]);
+            aa = mk(c[2][0], c[2][1], c[2][2]);
+        } else if (row[k] >= 0) {
             int o = 3 * row[k];
             if (d) u = mk(d[o], d[o + 1], d[o + 2]);
             if (v) vv = mk(v[o], v[o + 1], v[o + 2]);
@@ -354,8 +367,36 @@ __global__ void k_gather_blocks(DevMesh m, const double *__restrict__ C, double 
     if (out32) out32[vi] = (float)o;
 }
 
+// Partially constrained nodes: one thread per stored value, summing its element entries in
+// tet order (the reference's scatter order); a padding diagonal sums 1.0, other padding
+// values nothing.
+__global__ void k_gather_values(DevMesh m, const double *__restrict__ C, double cK, const double *__restrict__ M,
+                                double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
+                                const uint8_t *__restrict__ instActive) {
+    const int64_t vi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (vi >= m.nValues) return;
+    if (instActive && !instActive[0]) return;   // one instance (partially constrained meshes)
+    const int k0 = m.vPtr[vi], k1 = m.vPtr[vi + 1];
+    double s = 0.0;
+    for (int k = k0; k < k1; k++) {
+        const int src = m.vSrc[k];
+        s += src < 0 ? 1.0 : C[src];
+    }
+    double o = cK * s;
+    if (M) o += cM * M[vi];
+    if (accumulate) o += out[vi];
+    out[vi] = o;
+    if (out32) out32[vi] = (float)o;
+}
+
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
                           double *out, int accumulate, float *out32, const uint8_t *instActive) {
+    if (m.vPtr) {
+        hipLaunchKernelGGL(k_gather_values, dim3((unsigned)((m.nValues + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m,
+                           blkC, cK, M, cM, out, accumulate, out32, instActive);
+        OD_HIP(hipGetLastError());
+        return;
+    }
     int64_t n = m.nBlocks * 9;
     int grid = (int)((n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_gather_blocks, dim3(grid), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out, accumulate, out32,
@@ -387,9 +428,31 @@ __global__ void k_gather_rhs(DevMesh m, const double *__restrict__ C, const doub
     out[i] = cf * (f - s);
 }
 
+// Partially constrained nodes: per DOF, its node-force entries in tet order (padding DOFs have
+// none and keep fext = imp = 0).
+__global__ void k_gather_rhs_dofs(DevMesh m, const double *__restrict__ C, const double *__restrict__ fext,
+                                  const double *__restrict__ imp, double cf, double *__restrict__ out, int mode,
+                                  const uint8_t *__restrict__ instActive) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * m.nRows) return;
+    if (instActive && !instActive[0]) return;   // one instance (partially constrained meshes)
+    double s = 0.0;
+    for (int k = m.dPtr[i]; k < m.dPtr[i + 1]; k++) s += C[m.dSrc[k]];
+    if (mode == 1) { out[i] += s; return; }
+    double f = fext ? fext[i] : 0.0;
+    if (imp) f += imp[i];
+    out[i] = cf * (f - s);
+}
+
 void launch_gather_rhs(const DevMesh &m, hipStream_t s, const double *nodeC, const double *fext, const double *imp,
                        double cf, double *out, int mode, const uint8_t *instActive) {
     int n = 3 * m.nRows;
+    if (m.dPtr) {
+        hipLaunchKernelGGL(k_gather_rhs_dofs, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m, nodeC, fext, imp,
+                           cf, out, mode, instActive);
+        OD_HIP(hipGetLastError());
+        return;
+    }
     hipLaunchKernelGGL(k_gather_rhs, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m, nodeC, fext, imp, cf, out,
                        mode, instActive);
     OD_HIP(hipGetLastError());

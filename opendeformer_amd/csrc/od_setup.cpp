@@ -39,6 +39,116 @@ static int findBlock(const Topology &t, int J, int I) {
     return int(p - t.colIdx.data());
 }
 
+// Partially constrained nodes: the reference's layout of the last DOF triple (truncated blocks
+// in the block columns, scalar "remained" columns after them; BlockedSymSparseMatrix's
+// constructor, core/sparseMatrix.h:186-274) next to the kernels' padded blocks, and the
+// value-level gather lists that replace the block-level map.
+template <typename ValueOf, typename TetDofs>
+static std::string buildPartialMaps(Topology &t, const ValueOf &valueOf, const TetDofs &tetDofs) {
+    const int nbc = t.ndofFree / 3, rem = t.ndofFree % 3, r0 = 3 * nbc;
+    // remained entries present: some tet holds both DOFs (row >= column)
+    bool remHas[3][3] = {};
+    for (int e = 0; e < t.nTets; e++) {
+        int g[12];
+        tetDofs(e, g);
+        for (int a = 0; a < 12; a++)
+            for (int b = 0; b < 12; b++)
+                if (g[a] >= r0 && g[b] >= g[a]) remHas[g[b] - r0][g[a] - r0] = true;
+    }
+    std::vector<int64_t> refBlk((size_t)t.nBlocks, -1);
+    t.refPcol.assign(1, 0);
+    t.refOris.assign(1, 0);
+    t.refRows.clear();
+    int64_t off = 0;
+    for (int J = 0; J < nbc; J++) {
+        for (int b = t.rowPtr[J]; b < t.rowPtr[J + 1]; b++) {
+            const int I = t.colIdx[b];
+            refBlk[b] = off;
+            t.refRows.push_back(3 * I);
+            off += I == J ? 6 : (I < nbc ? 9 : 3 * rem);
+        }
+        t.refPcol.push_back((int)t.refRows.size());
+        t.refOris.push_back((int)off);
+    }
+    int remPos[3][3];
+    for (int c = 0; c < rem; c++) {
+        for (int r = c; r < rem; r++) {
+            remPos[r][c] = -1;
+            if (!remHas[r][c]) continue;
+            remPos[r][c] = (int)off++;
+            t.refRows.push_back(r0 + r);
+        }
+        t.refPcol.push_back((int)t.refRows.size());
+        t.refOris.push_back((int)off);
+    }
+    if (off >= ((int64_t)1 << 31)) return "too many values for int32 indexing";
+    t.nValuesRef = off;
+    t.refPos.assign((size_t)t.nValues, -1);
+    for (int J = 0; J < t.nRows; J++)
+        for (int b = t.rowPtr[J]; b < t.rowPtr[J + 1]; b++) {
+            const int I = t.colIdx[b];
+            const int64_t base = t.blkOff[b];
+            if (J == nbc) {   // the padded last triple (J == I): its remained entries
+                for (int i = 0; i < rem; i++)
+                    for (int j = i; j < rem; j++)
+                        if (remPos[j][i] >= 0) t.refPos[base + kDiagPacked[i] + (j - i)] = remPos[j][i];
+            } else if (I == J) {
+                for (int k = 0; k < 6; k++) t.refPos[base + k] = (int)(refBlk[b] + k);
+            } else if (I < nbc) {
+                for (int k = 0; k < 9; k++) t.refPos[base + k] = (int)(refBlk[b] + k);
+            } else {          // truncated block: column-major with rem rows
+                for (int sc = 0; sc < 3; sc++)
+                    for (int sr = 0; sr < rem; sr++) t.refPos[base + sc * 3 + sr] = (int)(refBlk[b] + sc * rem + sr);
+            }
+        }
+    // value-level gather: each value sums its element entries in tet order, as the reference's
+    // scatter does; the entries of a pair (a, b), a <= b, are stored row-major with a's DOF as
+    // the row, so local DOFs m <= n give entry 3 (m % 3) + n % 3 of pair slot 10 e + p(a, b)
+    std::vector<std::pair<int64_t, int>> vc;
+    std::vector<std::pair<int, int>> dc;
+    vc.reserve((size_t)78 * t.nTets);
+    dc.reserve((size_t)12 * t.nTets);
+    for (int e = 0; e < t.nTets; e++) {
+        int g[12];
+        tetDofs(e, g);
+        for (int m = 0; m < 12; m++) {
+            if (g[m] < 0) continue;
+            dc.emplace_back(g[m], (4 * e + m / 3) * 3 + m % 3);
+            for (int n = m; n < 12; n++) {
+                if (g[n] < 0) continue;
+                const int a = m / 3, bN = n / 3;
+                const int p = a * 4 - (a * (a - 1)) / 2 + (bN - a);
+                const int64_t v = valueOf(std::max(g[m], g[n]), std::min(g[m], g[n]));
+                if (v < 0) return "internal error: element entry outside the pattern";
+                vc.emplace_back(v, (10 * e + p) * 9 + 3 * (m % 3) + n % 3);
+            }
+        }
+    }
+    if ((int64_t)10 * t.nTets * 9 >= ((int64_t)1 << 31)) return "too many tets for the value-level gather";
+    for (int d = t.ndofFree; d < t.ndof; d++) vc.emplace_back(valueOf(d, d), -1);
+    std::stable_sort(vc.begin(), vc.end(), [](const std::pair<int64_t, int> &x, const std::pair<int64_t, int> &y) {
+        return x.first < y.first;
+    });
+    std::stable_sort(dc.begin(), dc.end(), [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+        return x.first < y.first;
+    });
+    t.vPtr.assign((size_t)t.nValues + 1, 0);
+    t.vSrc.resize(vc.size());
+    for (size_t k = 0; k < vc.size(); k++) { t.vPtr[vc[k].first + 1]++; t.vSrc[k] = vc[k].second; }
+    for (int64_t v = 0; v < t.nValues; v++) t.vPtr[v + 1] += t.vPtr[v];
+    t.dPtr.assign((size_t)t.ndof + 1, 0);
+    t.dSrc.resize(dc.size());
+    for (size_t k = 0; k < dc.size(); k++) { t.dPtr[dc[k].first + 1]++; t.dSrc[k] = dc[k].second; }
+    for (int d = 0; d < t.ndof; d++) t.dPtr[d + 1] += t.dPtr[d];
+    for (int d = 0; d < t.ndofFree; d++)
+        if (t.dPtr[d + 1] == t.dPtr[d]) {
+            char buf[96];
+            std::snprintf(buf, sizeof(buf), "free DOF %d belongs to no tet (empty matrix column)", d);
+            return buf;
+        }
+    return "";
+}
+
 std::string buildTopology(Topology &t, int chunkRows, int nXcd) {
     char buf[256];
     if (t.nNodes <= 0 || t.nTets <= 0) return "mesh has no nodes or no tets";
@@ -49,37 +159,62 @@ std::string buildTopology(Topology &t, int chunkRows, int nXcd) {
     if (!t.constrained.empty() && (t.constrained.front() < 0 || t.constrained.back() >= 3 * t.nNodes))
         return "constrained DOF out of range";
 
-    // ---- node -> block row; whole-node constraints only -------------------
+    // ---- DOF indexing: block row R = free DOFs 3R .. 3R+2 --------------------------
     std::vector<int> nCon(t.nNodes, 0);
     for (int c : t.constrained) nCon[c / 3]++;
+    t.partial = false;
+    for (int n = 0; n < t.nNodes; n++)
+        if (nCon[n] == 1 || nCon[n] == 2) { t.partial = true; break; }
     t.nodeRow.assign(t.nNodes, -1);
     t.rowNode.clear();
-    for (int n = 0; n < t.nNodes; n++) {
-        if (nCon[n] != 0 && nCon[n] != 3) {
-            std::snprintf(buf, sizeof(buf), "node %d is partially constrained; the device layout needs whole-node constraints", n);
-            return buf;
+    t.nodeDof.clear();
+    if (!t.partial) {
+        for (int n = 0; n < t.nNodes; n++)
+            if (nCon[n] == 0) { t.nodeRow[n] = (int)t.rowNode.size(); t.rowNode.push_back(n); }
+        t.nRows = (int)t.rowNode.size();
+        t.ndofFree = 3 * t.nRows;
+    } else {
+        if (t.instNodeStart.size() > 2) return "partially constrained nodes need a single-instance mesh";
+        t.nodeDof.assign(3 * (size_t)t.nNodes, -1);
+        int d = 0;
+        size_t k = 0;
+        for (int i = 0; i < 3 * t.nNodes; i++) {
+            if (k < t.constrained.size() && t.constrained[k] == i) { k++; continue; }
+            if (d % 3 == 0) t.rowNode.push_back(i / 3);
+            if (t.nodeRow[i / 3] < 0) t.nodeRow[i / 3] = d / 3;
+            t.nodeDof[i] = d++;
         }
-        if (nCon[n] == 0) { t.nodeRow[n] = (int)t.rowNode.size(); t.rowNode.push_back(n); }
+        t.ndofFree = d;
+        t.nRows = (d + 2) / 3;
     }
-    t.nRows = (int)t.rowNode.size();
     t.ndof = 3 * t.nRows;
     if (t.nRows == 0) return "every node is constrained";
+    // free DOFs of a tet's 12 local DOFs (-1: constrained)
+    auto tetDofs = [&t](int e, int *g) {
+        for (int a = 0; a < 4; a++) {
+            const int n = t.tets[4 * (size_t)e + a];
+            for (int ax = 0; ax < 3; ax++)
+                g[3 * a + ax] = t.partial ? t.nodeDof[3 * (size_t)n + ax] : (t.nodeRow[n] < 0 ? -1 : 3 * t.nodeRow[n] + ax);
+        }
+    };
 
     // ---- symbolic pattern: union over tets of free node pairs --------------
     std::vector<uint64_t> keys;
     keys.reserve((size_t)t.nTets * 10);
     for (int e = 0; e < t.nTets; e++) {
-        const int *tn = &t.tets[4 * e];
-        for (int a = 0; a < 4; a++) {
-            int ra = t.nodeRow[tn[a]];
-            if (ra < 0) continue;
-            for (int b = a; b < 4; b++) {
-                int rb = t.nodeRow[tn[b]];
-                if (rb < 0) continue;
-                uint64_t J = (uint64_t)std::min(ra, rb), I = (uint64_t)std::max(ra, rb);
+        // the tet's block rows (a DOF triple per free node, or up to 12 triples when partial)
+        int g[12], rows[12], nr = 0;
+        tetDofs(e, g);
+        for (int q = 0; q < 12; q++) {
+            if (g[q] < 0) continue;
+            const int r = g[q] / 3;
+            if (std::find(rows, rows + nr, r) == rows + nr) rows[nr++] = r;
+        }
+        for (int a = 0; a < nr; a++)
+            for (int b = a; b < nr; b++) {
+                uint64_t J = (uint64_t)std::min(rows[a], rows[b]), I = (uint64_t)std::max(rows[a], rows[b]);
                 keys.push_back(J * (uint64_t)t.nRows + I);
             }
-        }
     }
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
@@ -132,28 +267,39 @@ std::string buildTopology(Topology &t, int chunkRows, int nXcd) {
     }
 
     // ---- element maps -----------------------------------------------------------
+    // value index of the lower-triangle entry (r, c), r >= c, of the kernels' layout
+    auto valueOf = [&t](int r, int c) -> int64_t {
+        const int J = c / 3, I = r / 3, sc = c % 3, sr = r % 3;
+        if (I == J) return t.valOff[J] + kDiagPacked[sc] + (sr - sc);
+        const int b = findBlock(t, J, I);
+        return b < 0 ? -1 : (int64_t)t.blkOff[b] + sc * 3 + sr;
+    };
+    if (t.partial) {
+        std::string err = buildPartialMaps(t, valueOf, tetDofs);
+        if (!err.empty()) return err;
+    } else {
+        t.refPcol.clear(); t.refOris.clear(); t.refRows.clear(); t.refPos.clear();
+        t.vPtr.clear(); t.vSrc.clear(); t.dPtr.clear(); t.dSrc.clear();
+        t.nValuesRef = t.nValues;
+    }
     t.elemMap78.assign((size_t)78 * t.nTets, -1);
     t.pairBlk.assign((size_t)10 * t.nTets, -1);
     t.pairT.assign((size_t)10 * t.nTets, 0);
     for (int e = 0; e < t.nTets; e++) {
-        const int *tn = &t.tets[4 * e];
         int gi[12];
-        for (int a = 0; a < 4; a++) {
-            int r = t.nodeRow[tn[a]];
-            for (int ax = 0; ax < 3; ax++) gi[3 * a + ax] = r < 0 ? -1 : 3 * r + ax;
-        }
+        tetDofs(e, gi);
         int *local = &t.elemMap78[(size_t)78 * e];
         for (int subCol = 0; subCol < 12; subCol++) {
             for (int subRow = subCol; subRow < 12; subRow++) {
                 int gc = gi[subCol], gr = gi[subRow];
                 int li = localMatrixIndex(subRow, subCol);
                 if (gc < 0 || gr < 0) { local[li] = -1; continue; }
-                int r = std::max(gc, gr), c = std::min(gc, gr);
-                int J = c / 3, I = r / 3, sc = c % 3, sr = r % 3;
-                if (I == J) local[li] = t.valOff[J] + kDiagPacked[sc] + (sr - sc);
-                else local[li] = t.blkOff[findBlock(t, J, I)] + sc * 3 + sr;
+                const int64_t v = valueOf(std::max(gc, gr), std::min(gc, gr));
+                local[li] = t.partial ? t.refPos[v] : (int)v;
             }
         }
+        if (t.partial) continue;   // the value-level gathers replace the block map
+        const int *tn = &t.tets[4 * e];
         int p = 0;
         for (int a = 0; a < 4; a++)
             for (int b = a; b < 4; b++, p++) {
@@ -166,32 +312,43 @@ std::string buildTopology(Topology &t, int chunkRows, int nXcd) {
 
     // ---- deterministic gather slots (ordered by destination, then tet, then pair)
     t.blkCStart.assign(t.nBlocks + 1, 0);
-    for (size_t k = 0; k < t.pairBlk.size(); k++)
-        if (t.pairBlk[k] >= 0) t.blkCStart[t.pairBlk[k] + 1]++;
-    for (int64_t b = 0; b < t.nBlocks; b++) t.blkCStart[b + 1] += t.blkCStart[b];
-    t.nPairContrib = t.blkCStart[t.nBlocks];
-    t.pairSlot.assign(t.pairBlk.size(), -1);
-    {
-        std::vector<int> cur(t.blkCStart.begin(), t.blkCStart.end() - 1);
-        for (size_t k = 0; k < t.pairBlk.size(); k++)
-            if (t.pairBlk[k] >= 0) t.pairSlot[k] = cur[t.pairBlk[k]]++;
-    }
     t.rowCStart.assign(t.nRows + 1, 0);
-    for (int e = 0; e < t.nTets; e++)
-        for (int a = 0; a < 4; a++) {
-            int r = t.nodeRow[t.tets[4 * e + a]];
-            if (r >= 0) t.rowCStart[r + 1]++;
+    if (t.partial) {
+        // every pair and node of every tet has its own slot, in tet order; the value-level
+        // lists read them
+        t.nPairContrib = (int64_t)10 * t.nTets;
+        t.nNodeContrib = (int64_t)4 * t.nTets;
+        t.pairSlot.resize((size_t)10 * t.nTets);
+        std::iota(t.pairSlot.begin(), t.pairSlot.end(), 0);
+        t.nodeSlot.resize((size_t)4 * t.nTets);
+        std::iota(t.nodeSlot.begin(), t.nodeSlot.end(), 0);
+    } else {
+        for (size_t k = 0; k < t.pairBlk.size(); k++)
+            if (t.pairBlk[k] >= 0) t.blkCStart[t.pairBlk[k] + 1]++;
+        for (int64_t b = 0; b < t.nBlocks; b++) t.blkCStart[b + 1] += t.blkCStart[b];
+        t.nPairContrib = t.blkCStart[t.nBlocks];
+        t.pairSlot.assign(t.pairBlk.size(), -1);
+        {
+            std::vector<int> cur(t.blkCStart.begin(), t.blkCStart.end() - 1);
+            for (size_t k = 0; k < t.pairBlk.size(); k++)
+                if (t.pairBlk[k] >= 0) t.pairSlot[k] = cur[t.pairBlk[k]]++;
         }
-    for (int r = 0; r < t.nRows; r++) t.rowCStart[r + 1] += t.rowCStart[r];
-    t.nNodeContrib = t.rowCStart[t.nRows];
-    t.nodeSlot.assign((size_t)4 * t.nTets, -1);
-    {
-        std::vector<int> cur(t.rowCStart.begin(), t.rowCStart.end() - 1);
         for (int e = 0; e < t.nTets; e++)
             for (int a = 0; a < 4; a++) {
                 int r = t.nodeRow[t.tets[4 * e + a]];
-                if (r >= 0) t.nodeSlot[(size_t)4 * e + a] = cur[r]++;
+                if (r >= 0) t.rowCStart[r + 1]++;
             }
+        for (int r = 0; r < t.nRows; r++) t.rowCStart[r + 1] += t.rowCStart[r];
+        t.nNodeContrib = t.rowCStart[t.nRows];
+        t.nodeSlot.assign((size_t)4 * t.nTets, -1);
+        {
+            std::vector<int> cur(t.rowCStart.begin(), t.rowCStart.end() - 1);
+            for (int e = 0; e < t.nTets; e++)
+                for (int a = 0; a < 4; a++) {
+                    int r = t.nodeRow[t.tets[4 * e + a]];
+                    if (r >= 0) t.nodeSlot[(size_t)4 * e + a] = cur[r]++;
+                }
+        }
     }
 
     // ---- IC(0) levels: level(I) = 1 + max level(K), K < I adjacent ---------------

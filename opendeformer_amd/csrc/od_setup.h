@@ -19,10 +19,19 @@ struct Topology {
     std::vector<int> instTetStart;      // instance tet ranges (size nInst+1)
 
     // ---- DOF indexing (core/nodeIndexer.cpp:12-25 semantics) ----------------
-    int ndof = 0;                       // free DOFs
-    int nRows = 0;                      // block rows = free nodes (whole-node constraints)
-    std::vector<int> nodeRow;           // node -> block row, or -1 when constrained
-    std::vector<int> rowNode;           // block row -> node
+    // Block row R holds the free DOFs 3R .. 3R+2 of the reference's numbering.  With whole-node
+    // constraints these are the three DOFs of one free node.  When some node is partially
+    // constrained (`partial`) a block row is a DOF triple that may span nodes, and the last
+    // row is padded to three DOFs when ndofFree is not a multiple of 3: the kernels' vectors
+    // then hold ndof = 3 * nRows entries, the padding entries stay zero, and the matrix holds
+    // 1.0 on their diagonal and zeros elsewhere, so the padded system decouples them.
+    int ndof = 0;                       // DOF entries of the kernels' vectors (3 * nRows)
+    int ndofFree = 0;                   // free DOFs (the reference's numColumns)
+    int nRows = 0;                      // block rows
+    bool partial = false;               // some node has one or two constrained DOFs
+    std::vector<int> nodeRow;           // node -> block row of its first free DOF, or -1
+    std::vector<int> rowNode;           // block row -> node of its first DOF
+    std::vector<int> nodeDof;           // partial only: 3 per node, free DOF index or -1
 
     // ---- canonical pattern (core/sparseMatrix.h:186-274 layout) -------------
     std::vector<int> rowPtr;            // blockPcol: size nRows+1
@@ -44,6 +53,15 @@ struct Topology {
     std::vector<int> nodeSlot;          // 4 per tet: node-force contribution slot or -1
     std::vector<int> rowCStart;         // size nRows+1: node-force contribution range per row
     int64_t nPairContrib = 0, nNodeContrib = 0;
+
+    // ---- partial only: value-level gathers and the reference's own layout -------
+    // The reference stores the last DOF triple as truncated blocks and scalar "remained"
+    // columns (core/sparseMatrix.h:90-274); the kernels keep full 3x3 blocks.
+    std::vector<int> vPtr, vSrc;        // per value: contribution entries, pair slot * 9 + entry (-1: padding diagonal, 1.0)
+    std::vector<int> dPtr, dSrc;        // per DOF: node-force entries, node slot * 3 + axis
+    std::vector<int> refPcol, refOris, refRows;   // the reference's blockPcol, blockColumnOris, blockRows
+    std::vector<int> refPos;            // per value: the reference's value index, or -1 (padding)
+    int64_t nValuesRef = 0;             // values of the reference's layout (nValues when not partial)
 
     // ---- IC(0) level schedule -------------------------------------------------
     std::vector<int> level;             // per row

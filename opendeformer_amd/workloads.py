@@ -38,6 +38,9 @@ class Workload:
     impulse_node_forces: np.ndarray | None = None   # (nNodes, 3) applied at step 0 only
     shapes: list = field(default_factory=list)
     meta: dict = field(default_factory=dict)
+    # axes held on the fixed nodes: all three, or one or two for the reference's partial
+    # constraint types (Constrain_X .. Constrain_XZ, core/constrainer.h:16-22)
+    constrain_axes: tuple = (0, 1, 2)
 
     @property
     def n_nodes(self):
@@ -49,7 +52,10 @@ class Workload:
 
     @property
     def fixed_dofs(self):
-        return M.fixed_dofs_from_nodes(self.fixed_nodes)
+        if tuple(self.constrain_axes) == (0, 1, 2):
+            return M.fixed_dofs_from_nodes(self.fixed_nodes)
+        n = np.asarray(self.fixed_nodes, np.int64)
+        return np.sort((3 * n[:, None] + np.asarray(self.constrain_axes)[None, :]).ravel()).astype(np.int32)
 
     @property
     def body(self):
@@ -101,9 +107,17 @@ def _single(name, m: M.TetMesh, kind, E, nu, dt, steps, ordering="natural", impu
                     impulse_node_forces=impulse, shapes=[m.shape], meta=dict(meta or {}, ordering=ordering))
 
 
-def box_small(ordering="natural", seed=1):
+CONSTRAIN_AXES = {"X": (0,), "Y": (1,), "Z": (2,), "XY": (0, 1), "YZ": (1, 2), "XZ": (0, 2), "XYZ": (0, 1, 2)}
+
+
+def box_small(ordering="natural", seed=1, constrain="XYZ"):
+    """constrain: the axes held on the x = 0 face, a ConstrainType name (X .. XYZ)."""
     m = M.box_mesh(20, 4, 4, 1.0, 0.2, 0.2, seed=seed)
-    return _single("box_small_20x4x4_corot", m, 0, 1e5, 0.45, 1.0 / 60, 100, ordering)
+    w = _single("box_small_20x4x4_corot", m, 0, 1e5, 0.45, 1.0 / 60, 100, ordering)
+    w.constrain_axes = CONSTRAIN_AXES[constrain]
+    if constrain != "XYZ":
+        w.name += f"_constrain{constrain}"
+    return w
 
 
 def tip_impulse(m: M.TetMesh, seed, max_newton=50.0):

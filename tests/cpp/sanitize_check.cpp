@@ -101,8 +101,12 @@ int main() {
     t3.nTets = t.nTets;
     t3.X = b.X;
     t3.tets = b.tets;
-    t3.constrained = {0, 1};   // a partially constrained node is rejected with a message
-    if (od::buildTopology(t3, 64).empty()) return fail("partial constraint accepted");
+    t3.constrained = {0, 1};   // a partially constrained node: DOF-triple rows, the last padded
+    if (!od::buildTopology(t3, 64).empty() || !t3.partial || t3.ndofFree != 3 * b.nNodes - 2 || t3.ndof % 3 != 0)
+        return fail("partial constraint");
+    od::Topology t4 = t2;
+    t4.constrained = {0, 1, 3 * b.nNodes + 2};   // ... and two instances with it are rejected
+    if (od::buildTopology(t4, 64).empty()) return fail("partial constraint over two instances accepted");
 
     // ---- GPS relabelling (library and oracle)
     std::vector<int> el = b.tets, el2 = b.tets, lab((size_t)b.nNodes), lab2((size_t)b.nNodes);

@@ -46,13 +46,6 @@ def test_pattern_and_element_map_bit_exact(ctx, name):
         assert np.array_equal(a, b)
 
 
-def test_partial_node_constraint_is_rejected(ctx):
-    import opendeformer_amd as od
-    w = W.box_small()
-    with pytest.raises(capi.ODError, match="partially constrained"):
-        od.Mesh(ctx, w.X, w.tets, np.array([0, 1], np.int32))
-
-
 def test_spmv_fp64_fp32_and_accumulate(ctx):
     import opendeformer_amd as od
     w = W.box_small("color8")
