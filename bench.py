@@ -387,6 +387,9 @@ def main():
     ap.add_argument("--fp32-spmv", choices=["auto", "slots1", "slots2", "slots4"], default="auto",
                     help="FP32 product shape: the autotuner's choice, or the slot-value form with 1, 2 or 4 slots "
                          "per thread (for counter runs, whose profiling skews the autotuner's timings)")
+    ap.add_argument("--fp64-spmv", default="auto",
+                    help="FP64 product shape 'R,L' (chunks per workgroup, lanes per row), or auto (the "
+                         "autotuner's choice); counter runs pin the shape the timed runs chose")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -408,8 +411,11 @@ def main():
     f = mesh.body_force(w.body)
     pre = {"jacobi": capi.OD_PRE_JACOBI, "ic0": capi.OD_PRE_IC0}[args.precond]
     tune_table, v64, v32 = od.mesh_autotune(mesh)
+    if args.fp64_spmv != "auto":
+        v64 = tuple(int(x) for x in args.fp64_spmv.split(","))
     if args.fp32_spmv != "auto":
         v32 = (int(args.fp32_spmv[len("slots"):]), 0)
+    if args.fp64_spmv != "auto" or args.fp32_spmv != "auto":
         od.mesh_spmv_variant(mesh, v64, v32)      # the sweep forms stay as tuned
 
     def run_form(precision):
