@@ -387,6 +387,9 @@ def main():
     ap.add_argument("--fp32-spmv", choices=["auto", "slots1", "slots2", "slots4"], default="auto",
                     help="FP32 product shape: the autotuner's choice, or the slot-value form with 1, 2 or 4 slots "
                          "per thread (for counter runs, whose profiling skews the autotuner's timings)")
+    ap.add_argument("--max-iter", type=int, default=10000,
+                    help="PCG iteration cap per solve (the reference's maxIteration; the mixed form's inner "
+                         "iterations count against it together)")
     ap.add_argument("--fp64-spmv", default="auto",
                     help="FP64 product shape 'R,L' (chunks per workgroup, lanes per row), or auto (the "
                          "autotuner's choice); counter runs pin the shape the timed runs chose")
@@ -421,7 +424,8 @@ def main():
     def run_form(precision):
         """Warm-up, then the timed steps; returns the measurements and the final displacement."""
         prec = {"fp64": capi.OD_FP64, "mixed": capi.OD_MIXED}[precision]
-        nm = od.NewmarkIntegrator(mesh, w.dt, beta=0.25, gamma=0.5, precond=pre, precision=prec, tol=1e-6)
+        nm = od.NewmarkIntegrator(mesh, w.dt, beta=0.25, gamma=0.5, precond=pre, precision=prec, tol=1e-6,
+                                  max_iter=args.max_iter)
         nm.set_graph(True)
         nm.set_external_virtual_work(f)
         nm.run_time_steps(args.warmup)
@@ -503,6 +507,7 @@ def main():
                    "ic0_levels": mesh.n_levels},
         "pcg_iters_per_step": round(main_r["iters"], 2),
         "relres_max": main_r["relres_max"],
+        "converged": bool(main_r["relres_max"] <= 1e-6),
         "spmv": main_r["spmv"],
         "roofline": roofline(main_r, args.precision),
         "step_roofline": main_r["step_roofline"],
