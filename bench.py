@@ -155,20 +155,27 @@ def batched_leg(args, fp64_dof_steps_per_s=None, n_inst=256, warmup=2, steps=120
     mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs, w.inst_node_start, w.inst_tet_start)
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
     _, v64, v32 = od.mesh_autotune(mesh)
-    nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=capi.OD_MIXED, tol=1e-6)
-    nm.set_graph(True)
-    nm.set_external_virtual_work(mesh.body_force(w.body))
-    nm.run_time_steps(warmup)
-    ctx.sync()
-    its = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        nm.run_one_time_step()
-        its.append(nm.stats()[0].copy())
-    ctx.sync()
-    dt = (time.perf_counter() - t0) / steps
-    its = np.array(its)
-    del nm
+    fb = mesh.body_force(w.body)
+
+    def timed(whole_step, n_steps):
+        nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=capi.OD_MIXED, tol=1e-6)
+        nm.set_graph(True, whole_step=whole_step)
+        nm.set_external_virtual_work(fb)
+        nm.run_time_steps(warmup)
+        ctx.sync()
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(n_steps):
+            nm.run_one_time_step()
+            its.append(nm.stats()[0].copy())
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / n_steps
+        return dt, np.array(its), nm.graph_info()
+
+    # PCG iteration graphs (the faster mode here), then whole-step graphs on the same steps
+    dt, its, _ = timed(False, steps)
+    dt_step, its_step, ginfo = timed(True, steps)
+    assert np.array_equal(its, its_step), "whole-step graphs must give the iteration counts of the eager path"
     # the plain FP64 drop-in path on the same instances, one at a time (every 16th instance,
     # 10 steps each after 2 warm-up steps): instance-steps/s of stepping them alone
     alone_t, alone_n = 0.0, 0
@@ -199,7 +206,12 @@ def batched_leg(args, fp64_dof_steps_per_s=None, n_inst=256, warmup=2, steps=120
            "plain_fp64_alone_sample": f"instances 0, 16, ..., {n_inst - 16}: 10 steps each, one at a time",
            "speedup_vs_plain_fp64_alone": round((n_inst / dt) / alone_rate, 2),
            "spmv_shape": {"fp64": {"chunks_per_workgroup": v64[0], "lanes_per_row": v64[1]},
-                          "fp32": shape_f32(v32)}}
+                          "fp32": shape_f32(v32)},
+           "whole_step_graph": {"ms_per_step": round(dt_step * 1e3, 3), "dof_steps_per_s": round(mesh.ndof / dt_step, 1),
+                                "vs_iteration_graphs": round(dt / dt_step, 3), **ginfo,
+                                "note": "each step one captured graph (element, gather, factor, the plan's PCG "
+                                        "iterations, corrector), plan = previous count + 4 rounded up to 4; spare "
+                                        "iterations exit at once but still launch over the whole batch"}}
     if fp64_dof_steps_per_s:
         # the like-for-like comparison: DOF advanced per second against the 1M-DOF box on the plain
         # FP64 path (large single meshes are not launch-bound, small lone instances are)

@@ -32,7 +32,7 @@ EXPORTS = [
     "od_integrator_set_state", "od_integrator_state_dev", "od_integrator_stats", "od_backward_euler_update_mesh",
     "od_integrator_positions", "od_spmv_timed", "od_integrator_matrix", "od_ctx_profile", "od_ctx_profile_read",
     "od_solver_set_refinement", "od_integrator_set_refinement", "od_mesh_autotune", "od_mesh_spmv_variant", "od_mesh_sweep_form",
-    "od_solver_set_graph", "od_integrator_set_graph", "od_mesh_bind_ctx", "od_integrator_device_bytes",
+    "od_solver_set_graph", "od_integrator_set_graph", "od_integrator_graph_info", "od_mesh_bind_ctx", "od_integrator_device_bytes",
     "od_server_create", "od_server_destroy", "od_server_add_object", "od_server_run", "od_server_object_state",
     "od_server_object_info", "od_mesh_set_plastic", "od_mesh_plastic_strain", "od_plane_damping",
     "od_integrator_add_plane_damping", "od_integrator_clear_damping", "od_integrator_damping", "od_relabel_gps",
@@ -71,6 +71,7 @@ def lib():
             "od_solver_set_refinement": [vp, d, i], "od_integrator_set_refinement": [vp, d, i],
             "od_mesh_autotune": [vp, dp, ip], "od_mesh_spmv_variant": [vp, i, i, i, i], "od_mesh_sweep_form": [vp, i, i],
             "od_solver_set_graph": [vp, i], "od_integrator_set_graph": [vp, i],
+            "od_integrator_graph_info": [vp, C.POINTER(ll)],
             "od_mesh_bind_ctx": [vp, vp], "od_integrator_device_bytes": [vp, C.POINTER(ll)],
             "od_server_create": [i, ll, i, i, i, d, pvp], "od_server_destroy": [vp],
             "od_server_add_object": [vp, i, dp, i, ip, i, ip, ip, dp, dp, dp, dp, d, ip],

@@ -369,9 +369,17 @@ class _Integrator:
         """Mixed precision: FP32 inner tolerance and the cap on FP64 corrections."""
         check(lib().od_integrator_set_refinement(self.h, inner_tol, max_outer))
 
-    def set_graph(self, enable=True):
-        """Replay PCG iterations from a captured HIP graph."""
-        check(lib().od_integrator_set_graph(self.h, 1 if enable else 0))
+    def set_graph(self, enable=True, whole_step=False):
+        """Graph mode: PCG iterations replayed from captured HIP graphs; with whole_step, Newmark
+        steps after the first run as one captured graph each."""
+        check(lib().od_integrator_set_graph(self.h, (2 if whole_step else 1) if enable else 0))
+
+    def graph_info(self):
+        """Step graphs captured, steps run as a graph, graph steps completed eagerly, cached graphs."""
+        info = np.zeros(4, np.int64)
+        check(lib().od_integrator_graph_info(self.h, info.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return {"captures": int(info[0]), "graph_steps": int(info[1]), "fallbacks": int(info[2]),
+                "cached": int(info[3])}
 
     def set_precond_params(self, p1, p2):
         """od_solver_set_precond_params on the integrator's solver."""

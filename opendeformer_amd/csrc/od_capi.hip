@@ -41,6 +41,42 @@ __global__ void k_done_to_mask(int n, const InstScalars *sc, const uint8_t *extr
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) mask[i] = (sc[i].done || (extra && extra[i])) ? 1 : 0;
 }
+
+// ---- step graph helpers (a captured graph holds no memset node: these kernels clear and copy)
+__global__ void k_zero_words(uint32_t *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+__global__ void k_copy_f64(double *dst, const double *src, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+// iterations of one refinement sweep, per instance
+__global__ void k_record_sweep(int n, const InstScalars *sc, int *rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rec[i] = sc[i].it;
+}
+// The step graph commits an instance when its solve is done as the eager step would leave it:
+// FP64, converged or at maxIter within the graph's iterations; mixed, the outer residual
+// converged with no refinement sweep stopped by the graph's iteration count.
+struct SweepPlan {
+    int n;
+    int k[16];
+};
+__global__ void k_commit_flags(int n, const InstScalars *sc, const int *rec, SweepPlan plan, const uint8_t *active,
+                               uint8_t *commit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool ok = sc[i].done && (!active || active[i]);
+    if (rec)
+        for (int q = 0; q < plan.n; q++) ok = ok && rec[q * n + i] < plan.k[q];
+    commit[i] = ok ? 1 : 0;
+}
+// active instances the graph did not commit
+__global__ void k_uncommitted(int n, const uint8_t *active, const uint8_t *commit, uint8_t *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = ((!active || active[i]) && !commit[i]) ? 1 : 0;
+}
 }  // namespace
 
 // ---- fixed scratch of od_pcg_solve_noalloc ---------------------------------------------
@@ -294,6 +330,7 @@ struct od_solver {
         std::vector<const void *> key;   // the pointers baked into the captured launches
     } graph[2];                          // [FP64, FP32]
     long long captures = 0;              // graph captures so far (each capture instantiates a graph)
+    int *sweepRec = nullptr;             // step graph, mixed form: iterations per (sweep, instance)
     ~od_solver() {
         for (auto &g : graph)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -333,6 +370,19 @@ struct od_integrator {
     std::vector<double> lastRelres;
     int lastOuter = 0, initIters = 0;
     size_t bytes = 0;
+    // Whole-step graphs (graph mode, Newmark): element -> gather -> factor -> a fixed number of
+    // PCG iterations (or refinement sweeps) with device-side exits -> commit, one launch per
+    // step, cached by iteration plan.
+    struct StepGraph {
+        std::vector<int> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t lastUse = 0;
+    };
+    std::vector<StepGraph> stepGraphs;
+    uint64_t stepClock = 0;
+    uint8_t *commit = nullptr, *redo = nullptr;   // per instance: committed by the graph; to complete eagerly
+    long long graphSteps = 0, graphFallbacks = 0, graphCaptures = 0;
+    bool stepGraph = false;   // graph mode 2: whole-step graphs (mode 1: PCG iteration graphs only)
     void *alloc(size_t b) {
         void *ptr = nullptr;
         OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
@@ -342,6 +392,8 @@ struct od_integrator {
         return ptr;
     }
     ~od_integrator() {
+        for (auto &g : stepGraphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr; Dm.vals = nullptr;   // owned by allocs
         for (void *ptr : allocs) (void)hipFree(ptr);
         delete solver;
@@ -634,15 +686,22 @@ static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip,
 template <typename T>
 static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const T *invDiag, const T *b,
                     T *x, T *r, T *z, T *p, T *q, double tol, int maxIt, const uint8_t *preDone, InstScalars *sc,
-                    int *doneFlags, bool xZero, int expect, const InstScalars *outer = nullptr) {
+                    int *doneFlags, bool xZero, int expect, const InstScalars *outer = nullptr, int fixedIters = 0) {
     od_mesh *mesh = s->mesh;
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
     PcgWork w{s->partials, sc, doneFlags, outer};
     if (sizeof(T) == 4) w.slotVals = s->sv32;
     const size_t n = (size_t)mesh->topo.ndof;
-    if (xZero) OD_HIP(hipMemsetAsync(q, 0, sizeof(T) * n, st));
-    else launch_spmv<T>(m, st, vals, x, q, 0, nullptr);
+    if (xZero && fixedIters) {   // under step-graph capture: a kernel, not a memset node
+        const size_t words = sizeof(T) * n / 4;
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, (uint32_t *)q, words);
+        OD_HIP(hipGetLastError());
+    } else if (xZero) {
+        OD_HIP(hipMemsetAsync(q, 0, sizeof(T) * n, st));
+    } else {
+        launch_spmv<T>(m, st, vals, x, q, 0, nullptr);
+    }
     pcg_init_residual<T>(m, st, b, q, r, w);
     const bool ic = s->precond == OD_PRE_IC0;
     const bool sc64 = s->scalar != nullptr;   // threshold IC / LDL^T (FP64 solvers only)
@@ -680,6 +739,11 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     };
     // Graph mode (not while timing launches): a block of kGraphIters iterations is captured
     // once per set of buffers and replayed; iterations past convergence exit at once.
+    if (fixedIters) {
+        // inside a step graph: exactly this many iterations, each exiting at once past convergence
+        for (int j = 0; j < fixedIters; j++) iteration(j);
+        return;
+    }
     od_solver::IterGraph *g = nullptr;
     const int launchesPerIter = ic ? 2 * m.nLevels + 4 : 5;
     // the scalar preconditioners' sweep schedule changes with every factor: no graph
@@ -747,7 +811,10 @@ static void collect_relres(od_solver *s, const InstScalars *scDev) {
 }
 
 // Full solve A x = b (x warm start).  skip: per-instance mask of instances left untouched.
-static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double tol, int maxIt, const uint8_t *skip) {
+// fixed (step graph capture): FP64, {iterations}; mixed, the iterations of each refinement
+// sweep.  The launches then need no host decision and the statistics stay on the device.
+static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double tol, int maxIt, const uint8_t *skip,
+                  const std::vector<int> *fixed = nullptr) {
     od_mesh *mesh = s->mesh;
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
@@ -756,7 +823,8 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     if (s->precision == OD_FP64) {
         build_precond(s, A, pre);
         run_pcg<double>(s, A->vals, s->U, s->Ul, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre,
-                        s->scIn, s->doneIn, false, s->lastMaxIters);
+                        s->scIn, s->doneIn, false, s->lastMaxIters, nullptr, fixed ? (*fixed)[0] : 0);
+        if (fixed) return;
         collect_stats(s, s->scIn, false);
         collect_relres(s, s->scIn);
         s->outer = 0;
@@ -774,6 +842,27 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     PcgWork wo{s->partials, s->scOut, s->doneOut};
     launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr);
     launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 1);
+    if (fixed) {
+        // the graph's sweeps: each masks converged instances and caps its inner iterations at
+        // the plan's count (the plan's total stays within maxIt)
+        const int nI = t.nInst;
+        const size_t words = t.ndof;
+        for (size_t q = 0; q < fixed->size(); q++) {
+            hipLaunchKernelGGL(k_done_to_mask, dim3((nI + 255) / 256), dim3(256), 0, st, nI, s->scOut, pre, s->mask);
+            hipLaunchKernelGGL(k_zero_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, (uint32_t *)s->x32,
+                               words);
+            OD_HIP(hipGetLastError());
+            run_pcg<float>(s, A->vals32, s->U32, s->Ul32, s->invDiag32, s->r32, s->x32, s->ri32, s->z32, s->p32,
+                           s->q32, s->innerTol, (*fixed)[q], s->mask, s->scIn, s->doneIn, true, 0, s->scOut, (*fixed)[q]);
+            hipLaunchKernelGGL(k_record_sweep, dim3((nI + 255) / 256), dim3(256), 0, st, nI, s->scIn,
+                               s->sweepRec + q * (size_t)nI);
+            OD_HIP(hipGetLastError());
+            launch_add_correction(m, st, s->x32, x, s->scOut);
+            launch_spmv<double>(m, st, A->vals, x, s->Ax, 0, nullptr);
+            launch_residual_f64(m, st, b, s->Ax, s->r, s->r32, wo, tol, pre, 0);
+        }
+        return;
+    }
     // The inner iterations of all refinement sweeps count against maxIt together, as the
     // reference's maxIteration caps the whole solve (cgSolver.cpp:31): a sweep gets the
     // iterations the previous ones left.
@@ -1827,6 +1916,8 @@ int od_integrator_solver(od_integrator *it, od_solver **s) {
 int od_integrator_set_graph(od_integrator *it, int enable) {
     API_BEGIN
     require(it != nullptr, "NULL argument");
+    require(enable >= 0 && enable <= 2, "enable must be 0, 1 or 2");
+    it->stepGraph = enable == 2;
     return od_solver_set_graph(it->solver, enable);
     API_END
 }
@@ -1865,11 +1956,13 @@ int od_integrator_set_active(od_integrator *it, const unsigned char *active) {
     API_END
 }
 
-static void integrator_step(od_integrator *it) {
+// One step with eager launches.  actOverride: instances to step (the step graph's remainder),
+// in place of the caller's active set.
+static void integrator_step_eager(od_integrator *it, const uint8_t *actOverride = nullptr) {
     od_mesh *mesh = it->mesh;
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
-    const uint8_t *act = it->hasActive ? it->active : nullptr;
+    const uint8_t *act = actOverride ? actOverride : (it->hasActive ? it->active : nullptr);
     od_solver *s = it->solver;
     const uint8_t *skipArg = nullptr;
     if (act) {
@@ -1927,11 +2020,194 @@ static void integrator_step(od_integrator *it) {
         solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg);
         launch_be_correct(m, st, it->v, it->aNew, act);
     }
+    if (actOverride) return;   // the step graph's caller merges the statistics
     it->hasImpulse = false;
     it->lastIters = s->iters;
     it->lastRelres = s->relres;
     it->lastOuter = s->outer;
     it->steps++;
+}
+
+
+// Graph mode (Newmark, after the first step, IC(0) or Jacobi, no impulse pending, not while
+// timing launches): one captured graph per step, replayed.  Iteration plan: the previous
+// step's count (per refinement sweep for the mixed form) plus a margin, rounded up to a
+// multiple of 4 so that nearby counts share a graph.  Instances the graph cannot finish as
+// the eager step would (more iterations needed) are stepped again eagerly from their
+// untouched state, the committed ones masked out: every instance gets the eager step's bits.
+constexpr int kStepGraphMaxLaunches = 16384;
+constexpr size_t kStepGraphCache = 8;
+// Margin 4 in steps of 4: on 256 batched boxes a margin of 2 in steps of 2 left instances in
+// 9 of 42 steps, whose eager completion cost more than the iterations the tighter plan saves.
+constexpr int kStepPlanMargin = 4, kStepPlanQuantum = 4;
+
+static bool step_plan(od_integrator *it, std::vector<int> &plan) {
+    od_solver *s = it->solver;
+    od_mesh *mesh = it->mesh;
+    if (!it->stepGraph || it->kind != 0 || it->steps < 1 || it->hasImpulse || !s->useGraph || mesh->ctx->prof ||
+        s->scalar)
+        return false;
+    if (s->precond != OD_PRE_IC0 && s->precond != OD_PRE_JACOBI && s->precond != OD_PRE_NONE) return false;
+    auto up4 = [](int v) { return ((v + kStepPlanMargin + kStepPlanQuantum - 1) / kStepPlanQuantum) * kStepPlanQuantum; };
+    plan.clear();
+    if (s->precision == OD_FP64) {
+        plan.push_back(up4(s->lastMaxIters));
+    } else {
+        const int sweeps = std::max(1, std::min(s->outer, (int)s->sweepIters.size()));
+        if (sweeps > 16) return false;
+        for (int q = 0; q < sweeps; q++) plan.push_back(up4(s->sweepIters[q]));
+    }
+    long long total = 0;
+    for (int k : plan) total += k;
+    const int perIter = s->precond == OD_PRE_IC0 ? 2 * mesh->dm.nLevels + 4 : 5;
+    return total <= it->maxIter && total * perIter <= kStepGraphMaxLaunches;
+}
+
+static void newmark_graph_body(od_integrator *it, const std::vector<int> &plan) {
+    od_mesh *mesh = it->mesh;
+    const DevMesh &m = mesh->dm;
+    hipStream_t st = mesh->ctx->stream;
+    od_solver *s = it->solver;
+    const uint8_t *act = it->hasActive ? it->active : nullptr;
+    const uint8_t *skipArg = nullptr;
+    if (act) {
+        launch_set_done_from_active(m, st, it->inactive, act);
+        skipArg = it->inactive;
+    }
+    const double h = it->h, beta = it->beta, gamma = it->gamma, aM = it->alphaM, aK = it->alphaK;
+    ElemParams P{0, 1, h + aK, (0.5 - beta) * h * h + aK * (1.0 - gamma) * h, 1.0, (1.0 - gamma) * h, aM};
+    launch_element(m, st, it->d, it->v, it->a, P, mesh->blkC, mesh->nodeC, act);
+    const double cK = gamma * h * aK + beta * h * h, cM = 1.0 + gamma * h * aM;
+    launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act);
+    launch_gather_rhs(m, st, mesh->nodeC, it->fext, nullptr, 1.0, it->rhs, 0, act);
+    const size_t n = (size_t)mesh->topo.ndof;
+    hipLaunchKernelGGL(k_copy_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, it->aNew, (const double *)it->a, n);
+    OD_HIP(hipGetLastError());
+    solve(s, &it->A, it->rhs, it->aNew, it->tol, it->maxIter, skipArg, &plan);
+    const int nI = mesh->topo.nInst;
+    const bool mixed = s->precision != OD_FP64;
+    SweepPlan sp{};
+    sp.n = mixed ? (int)plan.size() : 0;
+    for (int q = 0; q < sp.n; q++) sp.k[q] = plan[q];
+    hipLaunchKernelGGL(k_commit_flags, dim3((nI + 255) / 256), dim3(256), 0, st, nI, mixed ? s->scOut : s->scIn,
+                       mixed ? (const int *)s->sweepRec : nullptr, sp, act, it->commit);
+    hipLaunchKernelGGL(k_uncommitted, dim3((nI + 255) / 256), dim3(256), 0, st, nI, act, (const uint8_t *)it->commit,
+                       it->redo);
+    OD_HIP(hipGetLastError());
+    launch_newmark_commit(m, st, it->d, it->v, it->a, it->aNew, h, beta, gamma, it->commit);
+}
+
+// Returns false when the step is not graph-eligible (the caller steps eagerly).
+static bool integrator_step_graph(od_integrator *it) {
+    std::vector<int> plan;
+    if (!step_plan(it, plan)) return false;
+    od_mesh *mesh = it->mesh;
+    od_solver *s = it->solver;
+    hipStream_t st = mesh->ctx->stream;
+    const int nI = mesh->topo.nInst;
+    const bool mixed = s->precision != OD_FP64;
+    if (!it->commit) {
+        it->commit = (uint8_t *)it->alloc(nI);
+        it->redo = (uint8_t *)it->alloc(nI);
+    }
+    if (mixed && !s->sweepRec) s->sweepRec = (int *)s->alloc(sizeof(int) * 16 * (size_t)nI);
+    std::vector<int> key = plan;
+    key.push_back(mixed ? 1 : 0);
+    key.push_back(it->hasActive ? 1 : 0);
+    key.push_back((int)(mesh->formGen & 0x7fffffff));
+    od_integrator::StepGraph *g = nullptr;
+    for (auto &c : it->stepGraphs)
+        if (c.key == key) g = &c;
+    if (!g) {
+        if (it->stepGraphs.size() >= kStepGraphCache) {
+            auto lru = std::min_element(it->stepGraphs.begin(), it->stepGraphs.end(),
+                                        [](const od_integrator::StepGraph &a, const od_integrator::StepGraph &b) {
+                                            return a.lastUse < b.lastUse;
+                                        });
+            if (lru->exec) OD_HIP(hipGraphExecDestroy(lru->exec));
+            it->stepGraphs.erase(lru);
+        }
+        hipGraph_t graph = nullptr;
+        OD_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            newmark_graph_body(it, plan);
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        OD_HIP(hipStreamEndCapture(st, &graph));
+        od_integrator::StepGraph ng;
+        ng.key = key;
+        const hipError_t e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        OD_HIP(e);
+        it->stepGraphs.push_back(ng);
+        g = &it->stepGraphs.back();
+        it->graphCaptures++;
+    }
+    g->lastUse = ++it->stepClock;
+    OD_HIP(hipGraphLaunch(g->exec, st));
+    it->graphSteps++;
+    // statistics and commit flags, one synchronisation
+    std::vector<InstScalars> scIn(nI), scOut(nI);
+    std::vector<int> rec(mixed ? plan.size() * (size_t)nI : 0);
+    std::vector<uint8_t> redo(nI);
+    OD_HIP(hipMemcpyAsync(scIn.data(), s->scIn, sizeof(InstScalars) * nI, hipMemcpyDeviceToHost, st));
+    if (mixed) {
+        OD_HIP(hipMemcpyAsync(scOut.data(), s->scOut, sizeof(InstScalars) * nI, hipMemcpyDeviceToHost, st));
+        OD_HIP(hipMemcpyAsync(rec.data(), s->sweepRec, sizeof(int) * rec.size(), hipMemcpyDeviceToHost, st));
+    }
+    OD_HIP(hipMemcpyAsync(redo.data(), it->redo, nI, hipMemcpyDeviceToHost, st));
+    OD_HIP(hipStreamSynchronize(st));
+    std::vector<int> iters(nI, 0);
+    std::vector<double> relres(nI, 0.0);
+    std::vector<int> sweepMax(plan.size(), 0);
+    int outer = 0;
+    for (int i = 0; i < nI; i++) {
+        const InstScalars &f = mixed ? scOut[i] : scIn[i];
+        relres[i] = f.bb > 0 ? std::sqrt(f.rr / f.bb) : std::sqrt(f.rr);
+        if (!mixed) { iters[i] = scIn[i].it; continue; }
+        for (size_t q = 0; q < plan.size(); q++) {
+            const int c = rec[q * (size_t)nI + i];
+            iters[i] += c;
+            sweepMax[q] = std::max(sweepMax[q], c);
+            if (c > 0 && !redo[i]) outer = std::max(outer, (int)q + 1);   // redone instances count below
+        }
+    }
+    bool allCommitted = true;
+    for (int i = 0; i < nI; i++)
+        if (redo[i]) allCommitted = false;
+    if (!allCommitted) {
+        // the active instances the graph left, stepped again from their untouched state
+        integrator_step_eager(it, it->redo);
+        for (int i = 0; i < nI; i++)
+            if (redo[i]) {
+                iters[i] = s->iters[i];
+                relres[i] = s->relres[i];
+            }
+        it->graphFallbacks++;
+        outer = std::max(outer, s->outer);
+    }
+    s->iters = iters;
+    s->relres = relres;
+    s->lastMaxIters = *std::max_element(iters.begin(), iters.end());
+    if (mixed) {
+        s->outer = std::max(outer, 1);
+        for (size_t q = 0; q < plan.size() && q < s->sweepIters.size(); q++)
+            if (allCommitted) s->sweepIters[q] = sweepMax[q];
+    } else {
+        s->outer = 0;
+    }
+    it->lastIters = iters;
+    it->lastRelres = relres;
+    it->lastOuter = s->outer;
+    it->steps++;
+    return true;
+}
+
+static void integrator_step(od_integrator *it) {
+    if (!integrator_step_graph(it)) integrator_step_eager(it);
 }
 
 int od_integrator_step(od_integrator *it, int nSteps) {
@@ -1974,6 +2250,16 @@ int od_integrator_state_dev(od_integrator *it, double **d, double **v, double **
     if (d) *d = it->d;
     if (v) *v = it->v;
     if (a) *a = it->kind == 0 ? it->a : it->aNew;
+    API_END
+}
+
+int od_integrator_graph_info(od_integrator *it, long long *info) {
+    API_BEGIN
+    require(it && info, "NULL argument");
+    info[0] = it->graphCaptures;
+    info[1] = it->graphSteps;
+    info[2] = it->graphFallbacks;
+    info[3] = (long long)it->stepGraphs.size();
     API_END
 }
 

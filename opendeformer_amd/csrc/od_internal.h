@@ -205,6 +205,10 @@ void launch_residual_f64(const DevMesh &m, hipStream_t s, const double *b, const
 void launch_add_correction(const DevMesh &m, hipStream_t s, const float *dx, double *x, const InstScalars *outer);
 void launch_newmark_correct(const DevMesh &m, hipStream_t s, double *d, double *v, const double *aOld,
                             const double *aNew, double h, double beta, double gamma, const uint8_t *instActive);
+// The step graph's corrector: instances whose flag in `commit` is set get the Newmark
+// correction and a = aNew; the others keep their state (the eager step then completes them).
+void launch_newmark_commit(const DevMesh &m, hipStream_t s, double *d, double *v, double *a, const double *aNew,
+                           double h, double beta, double gamma, const uint8_t *commit);
 void launch_be_correct(const DevMesh &m, hipStream_t s, double *v, const double *dv, const uint8_t *instActive);
 void launch_axpy_rows(const DevMesh &m, hipStream_t s, double *y, double a, const double *x, const uint8_t *instActive);
 

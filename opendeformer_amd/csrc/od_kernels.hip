@@ -2129,6 +2129,27 @@ void launch_newmark_correct(const DevMesh &m, hipStream_t s, double *d, double *
     OD_HIP(hipGetLastError());
 }
 
+// k_newmark_correct and the copy a = aNew in one pass, for the instances `commit` marks
+__global__ __launch_bounds__(kVecThreads) void k_newmark_commit(DevMesh m, double *d, double *v, double *a,
+                                                                const double *aNew, double h, double beta,
+                                                                double gamma, const uint8_t *commit) {
+    CHUNK_PROLOGUE
+    if (!in || !commit[inst]) return;
+    const double bh2 = beta * h * h, gh = gamma * h, mbh2 = h * h * 0.5 - bh2, mgh = h - gh;
+    double ao = a[i], an = aNew[i];
+    double dp = d[i] + h * v[i] + mbh2 * ao;
+    double vp = v[i] + mgh * ao;
+    v[i] = vp + gh * an;
+    d[i] = dp + bh2 * an;
+    a[i] = an;
+}
+void launch_newmark_commit(const DevMesh &m, hipStream_t s, double *d, double *v, double *a, const double *aNew,
+                           double h, double beta, double gamma, const uint8_t *commit) {
+    hipLaunchKernelGGL(k_newmark_commit, dim3(m.nChunks), dim3(kVecThreads), 0, s, m, d, v, a, aNew, h, beta, gamma,
+                       commit);
+    OD_HIP(hipGetLastError());
+}
+
 __global__ __launch_bounds__(kVecThreads) void k_axpy_rows(DevMesh m, double *y, double a, const double *x,
                                                            const uint8_t *instActive) {
     CHUNK_PROLOGUE
