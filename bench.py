@@ -4,7 +4,7 @@
 Workload (default): 200x40x40 Kuhn box, 5x1x1 m, 1,920,000 tets, 1,008,600
 free DOF, corotational material with per-tet E log-uniform in [1e4, 1e6],
 nu = 0.45, rho = 1000, x = 0 face fixed, gravity, Newmark beta = 1/4,
-gamma = 1/2, dt = 1/60, IC(0)-PCG to 1e-6 relative residual, 4-colour node
+gamma = 1/2, dt = 1/60, IC(0)-PCG to 1e-6 relative residual, tiled 4-colour node
 numbering.  A "step" is one full Newmark step on the device: element pass,
 assembly, matrix combine, IC(0) factorisation, PCG to tolerance, corrector.
 
@@ -41,10 +41,18 @@ METRIC = "Newmark steps/s + SpMV GB/s at 1M DOF, rel.res 1e-6; p50/p99 served st
 def spmv_kernel_f32(v32):
     """Name of the FP32 PCG product kernel for launch shape v32 = (R, lanes); lanes 0 is the
     slot-value form, R its slots per thread."""
+    if v32[1] == -1:
+        return f"k_spmv_grp<float, true, {v32[0]}>"
     return f"k_spmv_sv<{v32[0]}>" if v32[1] == 0 else f"k_spmv_dot<float, {v32[0]}, true, {v32[1]}>"
 
 
+def spmv_kernel_f64(v64):
+    return f"k_spmv_grp<double, false, {v64[0]}>" if v64[1] == -1 else f"k_spmv_dot<double, {v64[0]}, false, {v64[1]}>"
+
+
 def shape_f32(v32):
+    if v32[1] == -1:
+        return {"form": "grouped", "slots_per_thread": v32[0]}
     if v32[1] == 0:
         return {"form": "slot values", "slots_per_thread": v32[0]}
     return {"chunks_per_workgroup": v32[0], "lanes_per_row": v32[1]}
@@ -112,7 +120,7 @@ def mid_neo_leg(args, steps=200):
     import opendeformer_amd as od
     from opendeformer_amd import capi
     from opendeformer_amd import workloads as W
-    w = W.box_mid_neo("color4", seed=2)
+    w = W.box_mid_neo("color4t", seed=2)
     ctx = od.Context(0)
     mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs)
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
@@ -137,7 +145,7 @@ def mid_neo_leg(args, steps=200):
         del nm
     errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(traj["mixed"], traj["fp64"])]
     return {"workload": f"{w.name}: {mesh.n_tets} tets, {mesh.ndof} DOF, tip impulse, dt=1/120, {steps} steps "
-                        f"(4-colour ordering; the timing includes a state read-back every 20 steps)",
+                        f"(tiled 4-colour ordering; the timing includes a state read-back every 20 steps)",
             "fp64": out["fp64"], "mixed": out["mixed"],
             "mixed_rel_err_vs_fp64_max_over_sampled_steps": max(errs)}
 
@@ -150,7 +158,7 @@ def batched_leg(args, fp64_dof_steps_per_s=None, n_inst=256, warmup=2, steps=120
     import opendeformer_amd as od
     from opendeformer_amd import capi
     from opendeformer_amd import workloads as W
-    w = W.batch(n_inst, seed=4, ordering="color4")
+    w = W.batch(n_inst, seed=4, ordering="color4t")
     ctx = od.Context(0)
     mesh = od.Mesh(ctx, w.X, w.tets, w.fixed_dofs, w.inst_node_start, w.inst_tet_start)
     mesh.set_material(w.kind, w.E, w.nu, w.rho)
@@ -196,7 +204,7 @@ def batched_leg(args, fp64_dof_steps_per_s=None, n_inst=256, warmup=2, steps=120
         del ni, mi
     alone_rate = alone_n / alone_t
     out = {"workload": f"{n_inst} boxes, {mesh.n_tets} tets, {mesh.ndof} DOF in one concatenated system "
-                       f"(4-colour ordering per box), mixed precision, graph replay",
+                       f"(tiled 4-colour ordering per box), mixed precision, graph replay",
            "ms_per_step": round(dt * 1e3, 3), "batch_steps_per_s": round(1.0 / dt, 3),
            "instance_steps_per_s": round(n_inst / dt, 1), "dof_steps_per_s": round(mesh.ndof / dt, 1),
            "steps": steps, "warmup": warmup,
@@ -267,7 +275,7 @@ def log(msg):
 
 
 def ordering_leg(args, scale=0.5, max_iter=2000,
-                 plan=(("natural", 1, 2), ("color4", 1, 2), ("color5", 1, 2), ("gps", 0, 1))):
+                 plan=(("natural", 1, 2), ("color4", 1, 2), ("color4t", 1, 2), ("color5", 1, 2), ("gps", 0, 1))):
     """The box workload at half size (100x20x20 cells, 132,300 DOF) under three node numberings:
     natural (x-slowest grid order), the 4- and 5-colourings, and the reference's GPS relabelling
     (od_relabel_gps, host code).  FP64 IC(0)-PCG to 1e-6 with at most `max_iter` iterations;
@@ -381,9 +389,11 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="the form whose steps/s is `value` (default: the plain FP64 path); the other form "
                          "runs beside it unless --no-second-form")
-    ap.add_argument("--ordering", choices=["natural", "color8", "color5", "color4", "gps"], default="color4",
+    ap.add_argument("--ordering", choices=["natural", "color8", "color5", "color4", "color4t", "gps"],
+                    default="color4t",
                     help="node numbering: natural (x-slowest grid order), grouped by an 8-, 5- or 4-colouring "
-                         "(4 colours: 4 IC(0) dependency levels), or the GPS relabeller's")
+                         "(4 colours: 4 IC(0) dependency levels; color4t: each colour's chunks are 4x8x8 tiles, "
+                         "the same tile in every colour, for the grouped SpMV), or the GPS relabeller's")
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1.0 = 200x40x40)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-second-form", action="store_true",
@@ -396,15 +406,17 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="after the run, launch a streaming update of known size three times (the reference "
                          "that scales HBM byte counters)")
-    ap.add_argument("--fp32-spmv", choices=["auto", "slots1", "slots2", "slots4"], default="auto",
-                    help="FP32 product shape: the autotuner's choice, or the slot-value form with 1, 2 or 4 slots "
-                         "per thread (for counter runs, whose profiling skews the autotuner's timings)")
+    ap.add_argument("--fp32-spmv", choices=["auto", "slots1", "slots2", "slots4", "group4", "group8"], default="auto",
+                    help="FP32 product shape: the autotuner's choice, the slot-value form with 1, 2 or 4 slots "
+                         "per thread, or the grouped form with 4 or 8 (for counter runs, whose profiling skews "
+                         "the autotuner's timings)")
     ap.add_argument("--max-iter", type=int, default=10000,
                     help="PCG iteration cap per solve (the reference's maxIteration; the mixed form's inner "
                          "iterations count against it together)")
     ap.add_argument("--fp64-spmv", default="auto",
-                    help="FP64 product shape 'R,L' (chunks per workgroup, lanes per row), or auto (the "
-                         "autotuner's choice); counter runs pin the shape the timed runs chose")
+                    help="FP64 product shape 'R,L' (chunks per workgroup, lanes per row), 'group4' or "
+                         "'group8' (the grouped form, slots per thread), or auto (the autotuner's choice); "
+                         "counter runs pin the shape the timed runs chose")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -427,9 +439,11 @@ def main():
     pre = {"jacobi": capi.OD_PRE_JACOBI, "ic0": capi.OD_PRE_IC0}[args.precond]
     tune_table, v64, v32 = od.mesh_autotune(mesh)
     if args.fp64_spmv != "auto":
-        v64 = tuple(int(x) for x in args.fp64_spmv.split(","))
+        v64 = ((int(args.fp64_spmv[len("group"):]), -1) if args.fp64_spmv.startswith("group")
+               else tuple(int(x) for x in args.fp64_spmv.split(",")))
     if args.fp32_spmv != "auto":
-        v32 = (int(args.fp32_spmv[len("slots"):]), 0)
+        v32 = ((int(args.fp32_spmv[len("group"):]), -1) if args.fp32_spmv.startswith("group")
+               else (int(args.fp32_spmv[len("slots"):]), 0))
     if args.fp64_spmv != "auto" or args.fp32_spmv != "auto":
         od.mesh_spmv_variant(mesh, v64, v32)      # the sweep forms stay as tuned
 
@@ -473,7 +487,7 @@ def main():
         del nm
         t_step = elapsed / args.steps
         sb = step_bytes(mesh, precision, float(np.mean(iters)), float(np.mean(outer)))
-        spmv = {"kernel": (f"k_spmv_dot<double, {v64[0]}, false, {v64[1]}>" if vb == 8 else spmv_kernel_f32(v32)),
+        spmv = {"kernel": (spmv_kernel_f64(v64) if vb == 8 else spmv_kernel_f32(v32)),
                 "launches": spmv_n, "iterations": real_spmv,
                 "avg_us_per_launch": round(spmv_ms / max(spmv_n, 1) * 1e3, 2),
                 "avg_us_per_iteration": round(avg_spmv_s * 1e6, 2),
@@ -540,10 +554,12 @@ def main():
         line["mixed_rel_err_vs_fp64_path"] = {"value": err,
                                               "at": f"displacement after step {args.warmup + 2 * args.steps} "
                                                     f"of each form's run"}
-    line["spmv_autotune"] = {"fp64": {"chunks_per_workgroup": v64[0], "lanes_per_row": v64[1]},
+    def tune_key(p, ls, r):
+        return f"{p}_grouped_K{r}" if ls == -1 else (f"{p}_L{ls}_R{r}" if ls else f"{p}_slotvalues_K{r}")
+    line["spmv_autotune"] = {"fp64": ({"form": "grouped", "slots_per_thread": v64[0]} if v64[1] == -1 else
+                                      {"chunks_per_workgroup": v64[0], "lanes_per_row": v64[1]}),
                              "fp32": shape_f32(v32),
-                             "cold_us": {(f"{p}_L{ls}_R{r}" if ls else f"{p}_slotvalues_K{r}"): round(v * 1e3, 2)
-                                         for (p, ls, r), v in tune_table.items()}}
+                             "cold_us": {tune_key(p, ls, r): round(v * 1e3, 2) for (p, ls, r), v in tune_table.items()}}
     info = mesh.info_raw()
     def forms(word):
         # per level: lanes per row of the lane-per-row kernel, or slots per thread of the slot kernel

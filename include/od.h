@@ -129,10 +129,18 @@ int od_pattern_export(od_mesh *mesh, int *blockPcol_host, int *blockColumnOris_h
  * solver created on a mesh runs the autotuner, which times each shape for FP64
  * and FP32 values with the matrix out of cache and keeps the fastest, and picks
  * the form of each IC(0) sweep level (od_mesh_sweep_form) the same way.
- * od_mesh_autotune reruns it (ms_host: 21 median times, [FP64, FP32] x [16, 8,
+ * The grouped form (lanes -1; FP64 and FP32) takes up to four chunks that
+ * share many blocks (on a 4-colour numbering whose chunks are tiles: one tile's
+ * four colour chunks) per workgroup and reads each upper block stored for the
+ * group once: a block between two of the group's rows serves both through LDS.
+ * R is then the slots of a row per thread (4 or 8).  It needs rows of at most
+ * 16 blocks; the solver keeps its own copy of the matrix in the group order,
+ * refreshed at every solve that runs it.
+ * od_mesh_autotune reruns it (ms_host: 25 median times, [FP64, FP32] x [16, 8,
  * 4 lanes] x [R = 1, 2, 4], then the FP32 slot-value form with 1, 2, 4 slots
- * per thread; chosen_host: R and lanes for FP64, then for FP32; either may be
- * NULL).  od_mesh_spmv_variant sets the SpMV shapes directly (no tuning of the
+ * per thread, then the grouped form [FP64, FP32] x [4, 8 slots per thread] (-1
+ * where the mesh has none); chosen_host: R and lanes for FP64, then for FP32;
+ * either may be NULL).  od_mesh_spmv_variant sets the SpMV shapes directly (no tuning of the
  * SpMV or the sweeps). */
 int od_mesh_autotune(od_mesh *mesh, double *ms_host, int *chosen_host);
 int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes32);

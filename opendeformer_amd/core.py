@@ -172,18 +172,24 @@ class Mesh:
 
 def mesh_autotune(mesh: "Mesh"):
     """Reruns the autotuner: ({(precision, lanes, R): ms}, (R, lanes) for FP64, (R, lanes) for FP32).
-    Lanes 0 is the FP32 slot-value form, whose R is the slots per thread."""
-    ms = np.zeros(21); ch = np.zeros(4, np.int32)
+    Lanes 0 is the FP32 slot-value form, whose R is the slots per thread; lanes -1 the grouped
+    form, whose R is the slots per thread (4 or 8), listed only where the mesh has it."""
+    ms = np.zeros(25); ch = np.zeros(4, np.int32)
     check(lib().od_mesh_autotune(mesh.h, dptr(ms), capi.iptr(ch)))
     table = {(p, ls, r): float(ms[9 * i + 3 * k + j]) for i, p in enumerate(("fp64", "fp32"))
              for k, ls in enumerate((16, 8, 4)) for j, r in enumerate((1, 2, 4))}
     table.update({("fp32", 0, r): float(ms[18 + j]) for j, r in enumerate((1, 2, 4))})
+    for i, p in enumerate(("fp64", "fp32")):
+        for j, k in enumerate((4, 8)):
+            if ms[21 + 2 * i + j] >= 0:
+                table[(p, -1, k)] = float(ms[21 + 2 * i + j])
     return table, (int(ch[0]), int(ch[1])), (int(ch[2]), int(ch[3]))
 
 
 def mesh_spmv_variant(mesh: "Mesh", v64, v32):
     """Sets the SpMV launch shapes: v64, v32 = (chunks per workgroup R, lanes per row); v32 may
-    be (slots per thread, 0), the FP32 slot-value form."""
+    be (slots per thread, 0), the FP32 slot-value form; either may be (4 or 8, -1), the grouped
+    form with that many slots per thread (where the mesh has it)."""
     check(lib().od_mesh_spmv_variant(mesh.h, int(v64[0]), int(v64[1]), int(v32[0]), int(v32[1])))
 
 

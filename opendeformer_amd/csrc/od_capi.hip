@@ -150,6 +150,7 @@ struct od_mesh {
     uint64_t formGen = 0;
     double tuneMs[2][3][3] = {};   // [FP64, FP32][16, 8, 4 lanes per row][R = 1, 2, 4]: cold launch time
     double tuneSvMs[3] = {};       // FP32 slot-value form, [1, 2, 4 slots per thread]: cold launch time
+    double tuneGrpMs[2][2] = {{-1, -1}, {-1, -1}};   // grouped form, [FP64, FP32] x [4, 8 slots per thread]: cold launch time (-1: not available)
     double *blkC = nullptr, *nodeC = nullptr;   // contribution arrays (lazily allocated)
     // Partially constrained nodes: the caller's DOF vectors hold topo.ndofFree entries, the
     // kernels' topo.ndof (the last block row padded).  Entry points that take device vectors
@@ -305,6 +306,8 @@ struct od_solver {
     float *r32 = nullptr, *ri32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
     float *invDiag32 = nullptr, *U32 = nullptr, *Ul32 = nullptr;
     float *sv32 = nullptr;      // FP32 slot values of the matrix (slot-value product form)
+    double *gv64 = nullptr;     // grouped values of the matrix (grouped product form), FP64
+    float *gv32 = nullptr;      // and FP32
     double *Ax = nullptr;
     double *partials = nullptr;
     int *doneIn = nullptr, *doneOut = nullptr;   // per-instance done flags (device)
@@ -442,8 +445,32 @@ static void autotune_spmv(od_mesh *mesh) {
         InstScalars *sc = (InstScalars *)get(sizeof(InstScalars) * t.nInst);   // zero: no instance done
         float *flush = (float *)get(4 * (size_t)flushN);
         float *sv = (float *)get(sizeof(float) * kSlotValFloats * (size_t)std::max(1, m.nChunks));
+        void *gvals = (m.grpOk64 || m.grpOk32) ? get(8 * (size_t)grp_values(m)) : nullptr;
         OD_HIP(hipEventCreate(&e0));
         OD_HIP(hipEventCreate(&e1));
+        // the grouped product (FP64 plain, FP32 with the fused direction update)
+        for (int prec = 0; prec < 2; prec++)
+            for (int kv = 0; kv < 2; kv++) {
+                const int K = kGrpSlotVariants[kv];
+                mesh->tuneGrpMs[prec][kv] = -1.0;
+                if (!(prec == 0 ? m.grpOk64 : m.grpOk32)) continue;
+                float ms[3];
+                for (int rep = 0; rep < 3; rep++) {
+                    hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, st, (const float4 *)flush, flushN / 4,
+                                       flush);
+                    if (prec == 0)
+                        launch_spmv_dot<double>(m, st, K, -1, (const double *)vals, (const double *)x, (double *)y,
+                                                partials, sc, e0, e1, nullptr, nullptr, (const double *)gvals);
+                    else
+                        launch_spmv_dot<float>(m, st, K, -1, (const float *)vals, (const float *)x, (float *)y,
+                                               partials, sc, e0, e1, (const float *)z, (float *)pNew,
+                                               (const float *)gvals);
+                    OD_HIP(hipEventSynchronize(e1));
+                    OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
+                }
+                std::sort(ms, ms + 3);
+                mesh->tuneGrpMs[prec][kv] = ms[1];
+            }
         for (int v = 0; v < 3; v++) {
             float ms[3];
             for (int rep = 0; rep < 3; rep++) {
@@ -494,8 +521,8 @@ static void autotune_spmv(od_mesh *mesh) {
                     best[prec][0] = lv;
                     best[prec][1] = v;
                 }
-    m.spmvR64 = kSpmvVariants[best[0][1]];
-    m.spmvL64 = kSpmvLaneVariants[best[0][0]];
+    m.spmvR64 = m.spmvR64d = kSpmvVariants[best[0][1]];
+    m.spmvL64 = m.spmvL64d = kSpmvLaneVariants[best[0][0]];
     m.spmvR32 = kSpmvVariants[best[1][1]];
     m.spmvL32 = kSpmvLaneVariants[best[1][0]];
     // the FP32 slot-value form (lanes 0) when one of its shapes is faster still
@@ -506,6 +533,20 @@ static void autotune_spmv(od_mesh *mesh) {
             m.spmvR32 = kSpmvVariants[v];
             m.spmvL32 = 0;
         }
+    // the grouped form (lanes -1, R = slots per thread) when one of its shapes is faster still
+    double best64 = mesh->tuneMs[0][best[0][0]][best[0][1]];
+    for (int kv = 0; kv < 2; kv++) {
+        if (mesh->tuneGrpMs[0][kv] >= 0 && mesh->tuneGrpMs[0][kv] < best64) {
+            best64 = mesh->tuneGrpMs[0][kv];
+            m.spmvR64 = kGrpSlotVariants[kv];
+            m.spmvL64 = -1;
+        }
+        if (mesh->tuneGrpMs[1][kv] >= 0 && mesh->tuneGrpMs[1][kv] < best32) {
+            best32 = mesh->tuneGrpMs[1][kv];
+            m.spmvR32 = kGrpSlotVariants[kv];
+            m.spmvL32 = -1;
+        }
+    }
     // lanes per row of the IC(0) sweeps, per level (meshes with few levels)
     if (t.nLevels <= 64) {
         tune_ic0_lanes(m, st, t.levelPtr, t.nValues, (int64_t)t.tRow.size(), mesh->levelLanesFwd.data(),
@@ -560,7 +601,10 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
             // allocates them at the next solve)
             if (mesh->dm.spmvL32 == 0)
                 s->sv32 = (float *)s->alloc(sizeof(float) * kSlotValFloats * (size_t)std::max(1, mesh->dm.nChunks));
+            if (mesh->dm.spmvL32 == -1) s->gv32 = (float *)s->alloc(sizeof(float) * (size_t)grp_values(mesh->dm));
         }
+        if (precision == OD_FP64 && mesh->dm.spmvL64 == -1)
+            s->gv64 = (double *)s->alloc(sizeof(double) * (size_t)grp_values(mesh->dm));
         s->partials = (double *)s->alloc(sizeof(double) * 3 * (size_t)t.chunkRow0.size());
         s->doneIn = (int *)s->alloc(sizeof(int) * t.nInst);
         s->doneOut = (int *)s->alloc(sizeof(int) * t.nInst);
@@ -691,7 +735,8 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     const DevMesh &m = mesh->dm;
     hipStream_t st = mesh->ctx->stream;
     PcgWork w{s->partials, sc, doneFlags, outer};
-    if (sizeof(T) == 4) w.slotVals = s->sv32;
+    if (sizeof(T) == 4) w.slotVals = m.spmvL32 == -1 ? (const void *)s->gv32 : (const void *)s->sv32;
+    else if (m.spmvL64 == -1) w.slotVals = s->gv64;
     const size_t n = (size_t)mesh->topo.ndof;
     if (xZero && fixedIters) {   // under step-graph capture: a kernel, not a memset node
         const size_t words = sizeof(T) * n / 4;
@@ -700,7 +745,7 @@ static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const
     } else if (xZero) {
         OD_HIP(hipMemsetAsync(q, 0, sizeof(T) * n, st));
     } else {
-        launch_spmv<T>(m, st, vals, x, q, 0, nullptr);
+        launch_spmv<T>(m, st, vals, x, q, 0, nullptr, (const T *)w.slotVals);
     }
     pcg_init_residual<T>(m, st, b, q, r, w);
     const bool ic = s->precond == OD_PRE_IC0;
@@ -822,6 +867,10 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     const uint8_t *pre = skip ? skip : s->zeroMask;
     if (s->precision == OD_FP64) {
         build_precond(s, A, pre);
+        if (m.spmvL64 == -1) {
+            if (!s->gv64) s->gv64 = (double *)s->alloc(sizeof(double) * (size_t)grp_values(m));
+            launch_fill_grp<double>(m, st, A->vals, s->gv64);
+        }
         run_pcg<double>(s, A->vals, s->U, s->Ul, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre,
                         s->scIn, s->doneIn, false, s->lastMaxIters, nullptr, fixed ? (*fixed)[0] : 0);
         if (fixed) return;
@@ -837,6 +886,9 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
     if (m.spmvL32 == 0) {
         if (!s->sv32) s->sv32 = (float *)s->alloc(sizeof(float) * kSlotValFloats * (size_t)std::max(1, m.nChunks));
         launch_fill_slot_values(m, st, A->vals32, s->sv32);
+    } else if (m.spmvL32 == -1) {
+        if (!s->gv32) s->gv32 = (float *)s->alloc(sizeof(float) * (size_t)grp_values(m));
+        launch_fill_grp<float>(m, st, A->vals32, s->gv32);
     }
     std::fill(s->iters.begin(), s->iters.end(), 0);
     PcgWork wo{s->partials, s->scOut, s->doneOut};
@@ -1223,6 +1275,34 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
             m.svCol = mesh->upload(vCol);
             m.runInfo = (const int4 *)mesh->upload(info);
         }
+        m.spmvR64d = m.spmvR64;
+        m.spmvL64d = m.spmvL64;
+        {
+            // grouped symmetric product: the layout, when the mesh allows it
+            GroupLayout g;
+            const std::string err = buildGroupLayout(t, kChunkRows, 8, g);
+            require(err.empty(), err.c_str(), OD_ERR_UNSUPPORTED);
+            m.grpOk64 = m.grpOk32 = 0;
+            m.nGroups = m.perRunG = m.grpMaxDeps = 0;
+            m.grpNu = m.grpNx = 0;
+            if (g.ok) {
+                m.nGroups = g.nGroups;
+                m.perRunG = g.perRun;
+                m.grpMaxDeps = g.maxDeps;
+                m.grpNu = g.nUp;
+                m.grpNx = g.nX;
+                m.grpChunks = (const int4 *)mesh->upload(g.runChunks);
+                m.grpX = (const int4 *)mesh->upload(g.runX);
+                m.grpChunkUp = (const int2 *)mesh->upload(g.chunkUp);
+                m.grpRowInfo = mesh->upload(g.rowInfo);
+                m.grpUpMeta = mesh->upload(g.upMeta);
+                m.grpUpSrc = mesh->upload(g.upSrc);
+                m.grpXMeta = (const int2 *)mesh->upload(g.xMeta);
+                m.grpXSrc = mesh->upload(g.xSrc);
+                m.grpOk64 = grp_lds_bytes<double>(m, 4) <= 65536 ? 1 : 0;   // 4 slots per thread: the most LDS
+                m.grpOk32 = grp_lds_bytes<float>(m, 4) <= 65536 ? 1 : 0;
+            }
+        }
         m.instChunkStart = mesh->upload(t.instChunkStart);
         m.instRowStart = mesh->upload(t.instRowStart);
         m.tetInst = mesh->upload(t.tetInst);
@@ -1365,9 +1445,16 @@ int od_mesh_spmv_variant(od_mesh *mesh, int r64, int lanes64, int r32, int lanes
     require(mesh != nullptr, "NULL argument");
     auto okR = [](int r) { return r == 1 || r == 2 || r == 4; };
     auto okL = [](int l) { return l == 4 || l == 8 || l == 16; };
-    require(okR(r64) && okR(r32), "chunks per SpMV workgroup must be 1, 2 or 4");
-    require(okL(lanes64) && (okL(lanes32) || lanes32 == 0),
-            "lanes per SpMV block row must be 4, 8 or 16 (or 0 for the FP32 slot-value form)");
+    auto okRK = [&](int r, int l) { return l == -1 ? (r == 4 || r == 8) : okR(r); };
+    require(okRK(r64, lanes64) && okRK(r32, lanes32),
+            "chunks per SpMV workgroup must be 1, 2 or 4 (slots per thread 4 or 8 for the grouped form)");
+    require((okL(lanes64) || lanes64 == -1) && (okL(lanes32) || lanes32 == 0 || lanes32 == -1),
+            "lanes per SpMV block row must be 4, 8 or 16 (or 0 for the FP32 slot-value form, -1 for the grouped form)");
+    require(lanes64 != -1 || mesh->dm.grpOk64, "the grouped FP64 product is not available on this mesh",
+            OD_ERR_UNSUPPORTED);
+    require(lanes32 != -1 || mesh->dm.grpOk32, "the grouped FP32 product is not available on this mesh",
+            OD_ERR_UNSUPPORTED);
+    if (lanes64 != -1) { mesh->dm.spmvR64d = r64; mesh->dm.spmvL64d = lanes64; }
     mesh->dm.spmvR64 = r64;
     mesh->dm.spmvL64 = lanes64;
     mesh->dm.spmvR32 = r32;
@@ -1403,6 +1490,9 @@ int od_mesh_autotune(od_mesh *mesh, double *ms, int *chosen) {
                 for (int v = 0; v < 3; v++) ms[9 * p + 3 * lv + v] = mesh->tuneMs[p][lv][v];
     if (ms)
         for (int v = 0; v < 3; v++) ms[18 + v] = mesh->tuneSvMs[v];
+    if (ms)
+        for (int p = 0; p < 2; p++)
+            for (int kv = 0; kv < 2; kv++) ms[21 + 2 * p + kv] = mesh->tuneGrpMs[p][kv];
     if (chosen) {
         chosen[0] = mesh->dm.spmvR64; chosen[1] = mesh->dm.spmvL64;
         chosen[2] = mesh->dm.spmvR32; chosen[3] = mesh->dm.spmvL32;

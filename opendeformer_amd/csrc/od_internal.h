@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -83,6 +84,19 @@ struct DevMesh {
     int spmvL64, spmvL32; // lanes per block row of the SpMV (16, 8 or 4), set by the autotuner
     int *slotOff, *slotCol;                      // kSpmvThreads per run position (padding: slotCol = -1)
     int *svCol;           // slotCol slot-major (position s * 64 + r: slot s of row r), the slot-value order
+    // best shape among the chunk-descriptor forms (the FP64 product when no grouped values
+    // are at hand, e.g. y = A x outside a solve)
+    int spmvR64d, spmvL64d;
+    // grouped symmetric product (GroupLayout in od_setup.h; lanes -1 selects it): per run
+    // position the group's chunks and {crossing base, crossing count, instance, 0}; per chunk
+    // {upper-stream base, width}; per row nUp | items << 4 | depBase << 9; per upper position
+    // its meta; per crossing position {column row, deposit}; the fill sources of both
+    int grpOk64, grpOk32;   // the form fits (rows of at most 16 blocks, LDS within 64 KB)
+    int nGroups, perRunG, grpMaxDeps;
+    int64_t grpNu, grpNx;
+    const int4 *grpChunks, *grpX;
+    const int2 *grpChunkUp, *grpXMeta;
+    const int *grpRowInfo, *grpUpMeta, *grpUpSrc, *grpXSrc;
     int *instChunkStart, *instRowStart;          // nInst+1
     int *tetInst, *rowInst;
     // material (per tet)
@@ -117,15 +131,18 @@ void launch_gather_rhs(const DevMesh &m, hipStream_t s, const double *nodeC, con
                        double cf, double *out, int mode, const uint8_t *instActive);
 void launch_body_force(const DevMesh &m, hipStream_t s, const double *bodyPerTet, double *nodeC, double *f);
 void launch_expand_map(const DevMesh &m, hipStream_t s, int *map78);
+// y = A x or y += A x.  FP64 y = A x runs the autotuned FP64 shape: the grouped form when it
+// is selected and its values (grpVals) are given, else the best chunk-descriptor shape.
 template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
-                 const InstScalars *sc);
+                 const InstScalars *sc, const T *grpVals = nullptr);
 // PCG SpMV with R chunks per workgroup (1, 2 or 4), timed by optional dispatch events
 // With z set (the PCG product), x is the previous direction and the product runs on the new
 // direction p = z + beta x of each instance, written to pNew (see spmv_chunks).
 // LS = 0 selects the FP32 slot-value form (slotVals, filled by launch_fill_slot_values from
 // vals; R is then the slots per thread, 1, 2 or 4); vals still serves rows longer than
-// kSpmvLanes blocks.
+// kSpmvLanes blocks.  LS = -1 selects the grouped form (FP64 or FP32; slotVals then holds the
+// grouped values of launch_fill_grp; R is the slots per thread, 4 or 8).
 template <typename T>
 void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *vals, const T *x, T *y, double *partials,
                      const InstScalars *sc, hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
@@ -138,6 +155,24 @@ constexpr int kSpmvLaneVariants[3] = {16, 8, 4};   // lanes per block row
 // entries 4-7, then entry 8
 constexpr int kSlotValFloats = 9 * kSpmvThreads;
 void launch_fill_slot_values(const DevMesh &m, hipStream_t s, const float *vals, float *slotVals);
+// Grouped product values (lanes -1): the upper stream (9 entries per position), the crossing
+// list (9 per position) and the packed diagonals (6 per position, 64 positions per chunk),
+// each part-major (entries in vector planes: FP64 pairs, FP32 quads, then the rest), refreshed
+// from the canonical values at every solve that runs the form.
+constexpr int kGrpThreads = kGroupChunks * kChunkRows;   // rows of a group (threads per 16 slots per thread)
+constexpr int kGrpSlotVariants[2] = {4, 8};               // slots per thread of the grouped product
+inline int64_t grp_values(const DevMesh &m) {
+    return 9 * (m.grpNu + m.grpNx) + 6 * (int64_t)m.nChunks * kChunkRows;
+}
+// LDS of the grouped product with K slots per thread: the group's x, then the deposits, whose
+// area afterwards holds the cross-thread row sums (16 / K - 1 per row)
+template <typename T>
+size_t grp_lds_bytes(const DevMesh &m, int K) {
+    const size_t red = (size_t)(16 / K - 1) * kGrpThreads;
+    return sizeof(T) * 3 * ((size_t)kGrpThreads + std::max((size_t)m.grpMaxDeps, red));
+}
+template <typename T>
+void launch_fill_grp(const DevMesh &m, hipStream_t s, const T *vals, T *grpVals);
 // FP32 IC(0) factor copies, part-major: position b of an array of nb positions has entries
 // 0-3 at [4 b, 4 b + 4), entries 4-7 at [4 nb + 4 b, ...) and entry 8 at [8 nb + b], so a
 // wave's loads of consecutive positions read consecutive memory.  U32 holds the diagonal

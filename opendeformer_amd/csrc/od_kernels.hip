@@ -1103,10 +1103,387 @@ __global__ __launch_bounds__(kSpmvThreads / K) void k_spmv_sv(DevMesh m, const f
     }
 }
 
+// ---- grouped symmetric product ---------------------------------------------------------
+// Part-major storage of n positions: FP64 entries in double2 planes (0-1, 2-3, ...), FP32 in
+// float4 planes (0-3, 4-7); the ninth entry of a block in its own plane, the diagonal's last
+// pair (FP32) in a float2 plane.  A wave's load of one plane over 64 consecutive positions
+// reads consecutive memory.
+__device__ __forceinline__ void pm_load9(const double *R, int64_t n, int64_t p, double (&e)[9]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double2 v = *(const double2 *)(R + 2 * q * n + 2 * p);
+        e[2 * q] = v.x; e[2 * q + 1] = v.y;
+    }
+    e[8] = R[8 * n + p];
+}
+__device__ __forceinline__ void pm_load9(const float *R, int64_t n, int64_t p, float (&e)[9]) {
+    const float4 a = *(const float4 *)(R + 4 * p), b = *(const float4 *)(R + 4 * n + 4 * p);
+    e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+    e[4] = b.x; e[5] = b.y; e[6] = b.z; e[7] = b.w;
+    e[8] = R[8 * n + p];
+}
+__device__ __forceinline__ void pm_store9(double *R, int64_t n, int64_t p, const double (&e)[9]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) *(double2 *)(R + 2 * q * n + 2 * p) = make_double2(e[2 * q], e[2 * q + 1]);
+    R[8 * n + p] = e[8];
+}
+__device__ __forceinline__ void pm_store9(float *R, int64_t n, int64_t p, const float (&e)[9]) {
+    *(float4 *)(R + 4 * p) = make_float4(e[0], e[1], e[2], e[3]);
+    *(float4 *)(R + 4 * n + 4 * p) = make_float4(e[4], e[5], e[6], e[7]);
+    R[8 * n + p] = e[8];
+}
+__device__ __forceinline__ void pm_load6(const double *R, int64_t n, int64_t p, double (&d)[6]) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double2 v = *(const double2 *)(R + 2 * q * n + 2 * p);
+        d[2 * q] = v.x; d[2 * q + 1] = v.y;
+    }
+}
+__device__ __forceinline__ void pm_load6(const float *R, int64_t n, int64_t p, float (&d)[6]) {
+    const float4 a = *(const float4 *)(R + 4 * p);
+    const float2 b = *(const float2 *)(R + 4 * n + 2 * p);
+    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = b.x; d[5] = b.y;
+}
+__device__ __forceinline__ void pm_store6(double *R, int64_t n, int64_t p, const double (&d)[6]) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) *(double2 *)(R + 2 * q * n + 2 * p) = make_double2(d[2 * q], d[2 * q + 1]);
+}
+__device__ __forceinline__ void pm_store6(float *R, int64_t n, int64_t p, const float (&d)[6]) {
+    *(float4 *)(R + 4 * p) = make_float4(d[0], d[1], d[2], d[3]);
+    *(float2 *)(R + 4 * n + 2 * p) = make_float2(d[4], d[5]);
+}
+
+// The planes of one part-major region of n blocks as uniform base pointers, so a lane's load
+// is a uniform base plus its 32-bit position.
+template <typename T>
+struct Planes9;
+template <>
+struct Planes9<double> {
+    const double2 *a, *b, *c, *d;
+    const double *e;
+    __device__ Planes9(const double *R, int64_t n)
+        : a((const double2 *)R), b((const double2 *)(R + 2 * n)), c((const double2 *)(R + 4 * n)),
+          d((const double2 *)(R + 6 * n)), e(R + 8 * n) {}
+    __device__ __forceinline__ void load(int p, double (&v)[9]) const {
+        const double2 x0 = a[p], x1 = b[p], x2 = c[p], x3 = d[p];
+        v[0] = x0.x; v[1] = x0.y; v[2] = x1.x; v[3] = x1.y; v[4] = x2.x; v[5] = x2.y; v[6] = x3.x; v[7] = x3.y;
+        v[8] = e[p];
+    }
+};
+template <>
+struct Planes9<float> {
+    const float4 *a, *b;
+    const float *e;
+    __device__ Planes9(const float *R, int64_t n) : a((const float4 *)R), b((const float4 *)(R + 4 * n)), e(R + 8 * n) {}
+    __device__ __forceinline__ void load(int p, float (&v)[9]) const {
+        const float4 x0 = a[p], x1 = b[p];
+        v[0] = x0.x; v[1] = x0.y; v[2] = x0.z; v[3] = x0.w; v[4] = x1.x; v[5] = x1.y; v[6] = x1.z; v[7] = x1.w;
+        v[8] = e[p];
+    }
+};
+
+// 0 + the product of one block row with c, rounded as block_mv rounds a slot's first term
+template <typename T>
+__device__ __forceinline__ T slot_term(T e0, T e1, T e2, const T (&c)[3]) {
+    T a = 0;
+    acc_add(a, row3(e0, e1, e2, c[0], c[1], c[2]));
+    return a;
+}
+
+// The grouped values from the canonical ones: one thread per position (upper stream, crossing
+// list, then the diagonals, 64 per chunk; padding positions zero).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_grp(DevMesh m, const T *__restrict__ vals, T *__restrict__ gv) {
+    const int64_t Nu = m.grpNu, Nx = m.grpNx, Nd = (int64_t)m.nChunks * kChunkRows;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < Nu + Nx) {
+        const bool up = p < Nu;
+        const int64_t q = up ? p : p - Nu;
+        const int src = up ? m.grpUpSrc[q] : m.grpXSrc[q];
+        T e[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) e[i] = src >= 0 ? vals[(int64_t)src + i] : (T)0;
+        pm_store9(up ? gv : gv + 9 * Nu, up ? Nu : Nx, q, e);
+    } else if (p < Nu + Nx + Nd) {
+        const int64_t q = p - Nu - Nx;
+        const int c = (int)(q / kChunkRows), r = m.chunkRow0[c] + (int)(q % kChunkRows);
+        T d[6];
+        const bool in = r < m.chunkRow1[c];
+        const int64_t src = in ? m.valOff[r] : 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++) d[i] = in ? vals[src + i] : (T)0;
+        pm_store6(gv + 9 * (Nu + Nx), Nd, q, d);
+    }
+}
+
+template <typename T>
+void launch_fill_grp(const DevMesh &m, hipStream_t s, const T *vals, T *grpVals) {
+    const int64_t n = m.grpNu + m.grpNx + (int64_t)m.nChunks * kChunkRows;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_grp<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, vals, grpVals);
+    OD_HIP(hipGetLastError());
+}
+template void launch_fill_grp<double>(const DevMesh &, hipStream_t, const double *, double *);
+template void launch_fill_grp<float>(const DevMesh &, hipStream_t, const float *, float *);
+
+// y = A x (or y += A x) over one group per workgroup, K of a row's 16 slots per thread and
+// S = 16 / K threads per row: wave v of the 4 S waves serves chunk v / S of the group (one row
+// per lane) with sub-index u = v mod S, and takes the row's slots u, u + S, ... (the slot-value
+// kernel's assignment).
+//   1. Every thread loads its slots' upper-stream entries (slot s >= 1 is stream position
+//      s - 1 of the row's chunk; a row's upper slots come first, so a position holding no
+//      block of this row, meta -1, is a lower or padding slot), its row's x (with FUSE the new
+//      direction z + beta p_old; sub-index 0 also writes it to pNew and to LDS), and its share
+//      of the group's crossing list.  Loads are unconditional (clamped positions, results
+//      dropped) so a thread's loads are in flight together.
+//   2. The upper terms A_ij x_j (x_j from LDS when row j is in the group, else gathered); for
+//      an internal block the column row's term A_ij^T x_i goes to that row's LDS deposit, and
+//      the crossing list fills the other deposits.
+//   3. The lower slots read their deposits; each thread sums its K slots in the 16-lane tree
+//      order, the S threads of a row combine through LDS in the same order, so the bits are
+//      those of the chunk-descriptor forms, and each chunk's dot partial is one wave's sum.
+template <typename T, int K>
+constexpr int grp_waves_per_eu() { return K == 4 ? (sizeof(T) == 8 ? 6 : 8) : (sizeof(T) == 8 ? 5 : 6); }
+template <typename T, bool FUSE, int K>
+__global__ __launch_bounds__(kGrpThreads * (16 / K)) __attribute__((amdgpu_waves_per_eu(grp_waves_per_eu<T, K>())))
+void k_spmv_grp(DevMesh m, const T *__restrict__ gv,
+                                                                     const T *__restrict__ x, T *__restrict__ y,
+                                                                     int accumulate, double *__restrict__ partials,
+                                                                     const InstScalars *__restrict__ sc,
+                                                                     const T *__restrict__ z, T *__restrict__ pNew) {
+    constexpr int S = 16 / K, NT = kGrpThreads * S;
+    constexpr int KB = sizeof(T) == 8 ? 2 : 4;   // slots whose loads are in flight together
+    extern __shared__ __align__(16) unsigned char grpLds[];
+    T *xs = reinterpret_cast<T *>(grpLds);   // 3 per local row
+    T *dep = xs + 3 * kGrpThreads;            // 3 per deposit; reused for the cross-thread sums
+    const bool single = m.nInst == 1;
+    if (single && sc && sc[0].done) return;
+    const int jb = (int)(blockIdx.x >> 3), pos = (int)(blockIdx.x & 7) * m.perRunG + jb;
+    if (jb >= m.perRunG || pos >= m.nGroups) return;   // whole workgroup
+    const int4 gc = m.grpChunks[pos], gx = m.grpX[pos];
+    const int t = (int)threadIdx.x, wv = t >> 6, lane = t & 63, ci = wv / S, u = wv % S;
+    const int chunk = ci == 0 ? gc.x : (ci == 1 ? gc.y : (ci == 2 ? gc.z : gc.w));
+    const int lr = ci * kChunkRows + lane;   // local row
+    if (!single && sc && sc[gx.z].done) {      // whole workgroup: the group's instance is done
+        if (partials && u == 0 && lane == 0 && chunk >= 0) partials[chunk] = 0.0;
+        return;
+    }
+    const int64_t Nu = m.grpNu, Nx = m.grpNx, Nd = (int64_t)m.nChunks * kChunkRows;
+    const Planes9<T> gUp(gv, Nu), gX(gv + 9 * Nu, Nx);
+    const T *gD = gv + 9 * (Nu + Nx);
+    int row = -1;
+    int2 cu = make_int2(0, 0);
+    if (chunk >= 0) {
+        const int r0 = single ? chunk * kChunkRows : m.chunkRow0[chunk];
+        const int r1 = single ? min(r0 + kChunkRows, m.nRows) : m.chunkRow1[chunk];
+        if (r0 + lane < r1) row = r0 + lane;
+        cu = m.grpChunkUp[chunk];
+    }
+    const T beta = FUSE ? (T)sc[single ? 0 : gx.z].beta : (T)0;
+    // 1. loads
+    const int rr = row >= 0 ? row : 0;
+    const int info = m.grpRowInfo[rr];
+    T px[3] = {x[3 * rr], x[3 * rr + 1], x[3 * rr + 2]};
+    if (FUSE) {
+        px[0] = fma(beta, px[0], z[3 * rr]);
+        px[1] = fma(beta, px[1], z[3 * rr + 1]);
+        px[2] = fma(beta, px[2], z[3 * rr + 2]);
+    }
+    // v[i] (i < K / 2) holds the thread's slot i and, once slot i + K / 2 (8 slots further) is
+    // one of the row's own upper slots, their sum: the first level of the row's tree, added as
+    // the terms come (a row's upper slots come first, so slot i is then its own term too)
+    constexpr int H = K / 2;
+    int meta[K];
+    T v[H][3];
+#pragma unroll
+    for (int i = 0; i < H; i++) v[i][0] = v[i][1] = v[i][2] = 0;
+    if (u == 0) {   // slot 0: the diagonal
+        T d[6];
+        pm_load6(gD, Nd, (int64_t)(chunk >= 0 ? chunk : 0) * kChunkRows + lane, d);
+        v[0][0] = slot_term(d[0], d[1], d[2], px);
+        v[0][1] = slot_term(d[1], d[3], d[4], px);
+        v[0][2] = slot_term(d[2], d[4], d[5], px);
+    }
+    auto load_batch = [&](int i0, T (&e)[KB][9]) {
+#pragma unroll
+        for (int i = 0; i < KB; i++) {
+            const int sl = u + S * (i0 + i), k = sl - 1;
+            const bool in = row >= 0 && sl >= 1 && k < cu.y;
+            const int p = in ? cu.x + k * kChunkRows + lane : 0;
+            meta[i0 + i] = m.grpUpMeta[p];
+            gUp.load(p, e[i]);
+            if (!in) meta[i0 + i] = -1;
+        }
+    };
+    // column entries and terms of a loaded batch (after the LDS x is written)
+    auto use_batch = [&](int i0, const T (&e)[KB][9]) {
+#pragma unroll
+        for (int i = 0; i < KB; i++) {
+            const int mt = meta[i0 + i];
+            const int g = mt >= 0 ? mt : 0, lc = (mt >> 14) & 0xFF;
+            T c[3] = {x[3 * g], x[3 * g + 1], x[3 * g + 2]};
+            if (FUSE) {
+                c[0] = fma(beta, c[0], z[3 * g]);
+                c[1] = fma(beta, c[1], z[3 * g + 1]);
+                c[2] = fma(beta, c[2], z[3 * g + 2]);
+            }
+            if (mt < 0) { c[0] = xs[3 * lc]; c[1] = xs[3 * lc + 1]; c[2] = xs[3 * lc + 2]; }
+            if (mt == -1) continue;   // not an upper slot of this row (the diagonal, a lower slot, padding)
+            const T o0 = slot_term(e[i][0], e[i][1], e[i][2], c);
+            const T o1 = slot_term(e[i][3], e[i][4], e[i][5], c);
+            const T o2 = slot_term(e[i][6], e[i][7], e[i][8], c);
+            if (i0 + i < H) {
+                v[i0 + i][0] = o0; v[i0 + i][1] = o1; v[i0 + i][2] = o2;
+            } else {
+                v[i0 + i - H][0] += o0; v[i0 + i - H][1] += o1; v[i0 + i - H][2] += o2;
+            }
+            if (mt < 0 && mt != -1) {
+                T *o = dep + 3 * (mt & 0x3FFF);
+                o[0] = slot_term(e[i][0], e[i][3], e[i][6], px);
+                o[1] = slot_term(e[i][1], e[i][4], e[i][7], px);
+                o[2] = slot_term(e[i][2], e[i][5], e[i][8], px);
+            }
+        }
+    };
+    T e0[KB][9];
+    load_batch(0, e0);
+    // the crossing list: at most two items per thread in flight
+    constexpr int XB = 1;
+    int2 xm[XB];
+    T xe[XB][9];
+    auto load_x = [&](int q0) {
+#pragma unroll
+        for (int i = 0; i < XB; i++) {
+            const int q = q0 + i * NT;
+            const int p = gx.x + (q < gx.y ? q : 0);
+            xm[i] = m.grpXMeta[p];
+            gX.load(p, xe[i]);
+            if (q >= gx.y) xm[i].y = -1;
+        }
+    };
+    auto use_x = [&]() {
+#pragma unroll
+        for (int i = 0; i < XB; i++) {
+            T c[3] = {x[3 * xm[i].x], x[3 * xm[i].x + 1], x[3 * xm[i].x + 2]};
+            if (FUSE) {
+                c[0] = fma(beta, c[0], z[3 * xm[i].x]);
+                c[1] = fma(beta, c[1], z[3 * xm[i].x + 1]);
+                c[2] = fma(beta, c[2], z[3 * xm[i].x + 2]);
+            }
+            if (xm[i].y < 0) continue;
+            T *o = dep + 3 * xm[i].y;
+            o[0] = slot_term(xe[i][0], xe[i][3], xe[i][6], c);
+            o[1] = slot_term(xe[i][1], xe[i][4], xe[i][7], c);
+            o[2] = slot_term(xe[i][2], xe[i][5], xe[i][8], c);
+        }
+    };
+    if (u == 0) {
+        xs[3 * lr] = px[0]; xs[3 * lr + 1] = px[1]; xs[3 * lr + 2] = px[2];
+        if (FUSE && row >= 0) { pNew[3 * row] = px[0]; pNew[3 * row + 1] = px[1]; pNew[3 * row + 2] = px[2]; }
+    }
+    __syncthreads();
+    // 2. terms and deposits
+    use_batch(0, e0);
+#pragma unroll
+    for (int i0 = KB; i0 < K; i0 += KB) {
+        T e1[KB][9];
+        load_batch(i0, e1);
+        use_batch(i0, e1);
+    }
+    for (int q0 = t; q0 < gx.y; q0 += XB * NT) {
+        load_x(q0);
+        use_x();
+    }
+    __syncthreads();
+    // 3. slot values, the thread's tree levels, then the row's S threads through LDS
+    const int nUp = info & 15, items = (info >> 4) & 31, db = info >> 9;
+    auto slot_value = [&](int sl, T (&a)[3]) {   // a lower slot's deposit, or padding
+        const bool has = sl < items && row >= 0;
+        const int o = 3 * (has ? db + sl - nUp - 1 : 0);
+        a[0] = has ? dep[o] : (T)0;
+        a[1] = has ? dep[o + 1] : (T)0;
+        a[2] = has ? dep[o + 2] : (T)0;
+    };
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+        const int sl = u + S * i;
+        if (sl != 0 && meta[i] == -1) slot_value(sl, v[i]);      // slot i is no own term
+        if (meta[i + H] == -1) {                                 // slot i + H is not folded in yet
+            T a[3];
+            slot_value(sl + 8, a);
+            v[i][0] += a[0]; v[i][1] += a[1]; v[i][2] += a[2];
+        }
+    }
+#pragma unroll
+    for (int o = H / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int i = 0; i < o; i++) {
+            v[i][0] += v[i + o][0]; v[i][1] += v[i + o][1]; v[i][2] += v[i + o][2];
+        }
+    T a0 = v[0][0], a1 = v[0][1], a2 = v[0][2];
+    if constexpr (S > 1) {
+        __syncthreads();                       // every deposit read: the area takes the sums
+        T *red = dep;                          // [S - 1][kGrpThreads][3]
+        if (u > 0) {
+            T *o = red + 3 * ((u - 1) * kGrpThreads + lr);
+            o[0] = a0; o[1] = a1; o[2] = a2;
+        }
+        __syncthreads();
+        if (u == 0) {
+            T r[S][3];
+            r[0][0] = a0; r[0][1] = a1; r[0][2] = a2;
+#pragma unroll
+            for (int j = 1; j < S; j++) {
+                const T *o = red + 3 * ((j - 1) * kGrpThreads + lr);
+                r[j][0] = o[0]; r[j][1] = o[1]; r[j][2] = o[2];
+            }
+#pragma unroll
+            for (int o = S / 2; o > 0; o >>= 1)
+#pragma unroll
+                for (int j = 0; j < o; j++) {
+                    r[j][0] += r[j + o][0]; r[j][1] += r[j + o][1]; r[j][2] += r[j + o][2];
+                }
+            a0 = r[0][0]; a1 = r[0][1]; a2 = r[0][2];
+        }
+    }
+    if (u != 0) return;                        // no barrier below
+    double rowDot = 0.0;
+    if (row >= 0) {
+        if (accumulate) { a0 += y[3 * row]; a1 += y[3 * row + 1]; a2 += y[3 * row + 2]; }
+        y[3 * row] = a0; y[3 * row + 1] = a1; y[3 * row + 2] = a2;
+        if (partials) rowDot = row3<double>(px[0], px[1], px[2], a0, a1, a2);
+    }
+    if (partials) {
+        const double sum = wave_sum(rowDot);
+        if (lane == 0 && chunk >= 0) partials[chunk] = sum;
+    }
+}
+
 template <typename T>
 void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *vals, const T *x, T *y, double *partials,
                      const InstScalars *sc, hipEvent_t start, hipEvent_t stop, const T *z, T *pNew,
                      const T *slotVals) {
+    if (LS == -1) {
+        if (!slotVals) throw HipError("the grouped product needs its grouped values");
+        if (!(sizeof(T) == 8 ? m.grpOk64 : m.grpOk32)) throw HipError("the grouped product is not available on this mesh");
+        if ((sizeof(T) == 4) != (z != nullptr)) throw HipError("the FP32 grouped product runs with the fused direction update");
+        if (z && !sc) throw HipError("the fused direction update needs the solver scalars");
+        if (R != 4 && R != 8) throw HipError("the grouped product runs 4 or 8 slots per thread");
+        const size_t lds = grp_lds_bytes<T>(m, R);
+        const dim3 grid(8 * (unsigned)m.perRunG);
+        auto go = [&](auto kernel, int threads) {
+            if (start)
+                hipExtLaunchKernelGGL(kernel, grid, dim3(threads), lds, s, start, stop, 0, m, slotVals, x, y, 0,
+                                      partials, sc, z, pNew);
+            else
+                hipLaunchKernelGGL(kernel, grid, dim3(threads), lds, s, m, slotVals, x, y, 0, partials, sc, z, pNew);
+        };
+        constexpr bool F = sizeof(T) == 4;
+        if (R == 4) go(k_spmv_grp<T, F, 4>, kGrpThreads * 4);
+        else go(k_spmv_grp<T, F, 8>, kGrpThreads * 2);
+        OD_HIP(hipGetLastError());
+        return;
+    }
     // With events, the launch records them from the dispatch's own begin/end timestamps,
     // so their elapsed time is the kernel's execution alone (measurement mode).
     auto go = [&](auto kernel, int threads) {
@@ -1166,11 +1543,17 @@ template void launch_spmv_dot<float>(const DevMesh &, hipStream_t, int, int, con
 
 template <typename T>
 void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *y, int accumulate,
-                 const InstScalars *sc) {
+                 const InstScalars *sc, const T *grpVals) {
     // an FP64 product y = A x runs the autotuned FP64 shape (same bits as every shape)
     if constexpr (sizeof(T) == 8) {
         if (!accumulate) {
-            launch_spmv_dot<T>(m, s, m.spmvR64, m.spmvL64, vals, x, y, nullptr, sc);
+            if (m.spmvL64 == -1 && grpVals)
+                launch_spmv_dot<T>(m, s, m.spmvR64, -1, vals, x, y, nullptr, sc, nullptr, nullptr, nullptr, nullptr,
+                                   grpVals);
+            else if (m.spmvL64 == -1)
+                launch_spmv_dot<T>(m, s, m.spmvR64d, m.spmvL64d, vals, x, y, nullptr, sc);
+            else
+                launch_spmv_dot<T>(m, s, m.spmvR64, m.spmvL64, vals, x, y, nullptr, sc);
             return;
         }
     }
@@ -1178,16 +1561,16 @@ void launch_spmv(const DevMesh &m, hipStream_t s, const T *vals, const T *x, T *
     OD_HIP(hipGetLastError());
 }
 template void launch_spmv<double>(const DevMesh &, hipStream_t, const double *, const double *, double *, int,
-                                  const InstScalars *);
+                                  const InstScalars *, const double *);
 template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, const float *, float *, int,
-                                 const InstScalars *);
+                                 const InstScalars *, const float *);
 
 template <typename T>
 void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *pOld, const T *z, T *p, T *q, PcgWork &w,
                hipEvent_t start, hipEvent_t stop) {
-    launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, sizeof(T) == 8 ? m.spmvL64 : m.spmvL32, vals,
-                       z ? pOld : p, q, w.partials, w.sc, start, stop, z, z ? p : nullptr,
-                       sizeof(T) == 4 ? (const T *)w.slotVals : nullptr);
+    const int L = sizeof(T) == 8 ? m.spmvL64 : m.spmvL32;
+    launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, L, vals, z ? pOld : p, q, w.partials, w.sc, start,
+                       stop, z, z ? p : nullptr, L <= 0 ? (const T *)w.slotVals : nullptr);
 }
 
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w) {

@@ -84,6 +84,34 @@ struct Topology {
 // Builds every derived structure.  Returns an empty string or an error text.
 std::string buildTopology(Topology &t, int chunkRows, int nXcd = 8);
 
+// Layout of the grouped symmetric product (od_groups.cpp).  Up to kGroupChunks chunks of one
+// instance that share many blocks form a group, one workgroup of the product with one wave
+// per chunk (local row = wave * chunkRows + row in chunk).  The group stores each upper block
+// of its rows once; a block whose column row is in the group is "internal": its product for
+// the column row (the transposed block times the row's x) goes to an LDS deposit, so the
+// block is read once for both rows.  The lower slots of the group's rows whose upper owner
+// lies outside the group come from the group's crossing list.  Every row then sums its slots
+// (diagonal, upper blocks, lower blocks in transposed-list order) exactly as the other SpMV
+// forms do.  Rows with more than 16 blocks are not supported (ok = false).
+constexpr int kGroupChunks = 4;
+struct GroupLayout {
+    bool ok = false;
+    std::string why;                 // reason when !ok
+    int nGroups = 0, perRun = 0;     // groups in run order, cut into nXcd runs of perRun
+    int maxDeps = 0;                 // most LDS deposits (lower slots) of any group
+    int64_t nUp = 0, nX = 0;         // upper-stream positions (padded per chunk), crossing positions (padded to 64)
+    std::vector<int> runChunks;      // 4 per run position: the group's chunks (-1: none)
+    std::vector<int> runX;           // 4 per run position: crossing base, crossing count, instance, 0
+    std::vector<int> chunkUp;        // 2 per chunk: upper-stream base, width (slots)
+    std::vector<int> rowInfo;        // per row: nUp | items << 4 | depBase << 9
+    std::vector<int> upMeta;         // per upper position: -1 padding; >= 0 crossing (column row);
+                                     // else 0x80000000 | local column << 14 | deposit
+    std::vector<int> upSrc;          // per upper position: value offset of the block (-1: padding)
+    std::vector<int> xMeta;          // 2 per crossing position: column row (upper owner), deposit (-1: padding)
+    std::vector<int> xSrc;           // per crossing position: value offset of the block (-1: padding)
+};
+std::string buildGroupLayout(const Topology &t, int chunkRows, int nXcd, GroupLayout &g);
+
 // GPS relabelling (od_relabel.cpp): new label per node, elements relabelled and sorted by
 // label sum in place; returns an error message or nullptr.
 const char *relabelGps(int nNodes, int nElems, int npe, int *elements, int *labels);

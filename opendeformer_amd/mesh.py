@@ -187,6 +187,59 @@ def multicolor_permutation(mesh: TetMesh, colors: int = 8) -> np.ndarray:
     return perm
 
 
+def tiled_color4_permutation(mesh: TetMesh, fixed_nodes: np.ndarray, chunk: int = 64,
+                             tile=(4, 8, 8)) -> np.ndarray:
+    """The 4-colour numbering ((i + j + k) mod 4, colours in order) with each colour's nodes
+    ordered so that every 64-row chunk is a compact tile of the grid, the same tile in every
+    colour.
+
+    Block rows are the free nodes in node order, and the library cuts them into chunks of
+    `chunk` consecutive rows.  A tile of 4 x 8 x 8 nodes (4 along i, so (i + j + k) mod 4 takes
+    each value equally often on every (j, k) line) holds exactly 64 nodes of each colour.
+    Within colour c the rows are: a head that completes the chunk left open by the previous
+    colour, then one chunk per full tile in tile order, then the nodes of partial tiles sorted
+    by tile (chunks of the boundary layers).  Chunk q of the full tiles then covers the same
+    tile in every colour, so the grouped SpMV's groups (the four chunks of one tile) hold most
+    of their rows' blocks.  Within a colour there are no edges, so the IC(0) factor is the
+    one of any other order inside the colours, up to rounding.
+    `fixed_nodes` (the nodes that will be constrained, hence not rows) are numbered last.
+    Returns perm with new_id = perm[old_id]."""
+    nx, ny, nz = mesh.shape
+    n = np.arange(mesh.n_nodes)
+    k = n % (nz + 1)
+    j = (n // (nz + 1)) % (ny + 1)
+    i = n // ((nz + 1) * (ny + 1))
+    color = (i + j + k) % 4
+    free = np.ones(mesh.n_nodes, bool)
+    free[np.asarray(fixed_nodes, dtype=np.int64)] = False
+    TI, TJ, TK = tile
+    i0 = int(i[free].min()) if free.any() else 0
+    ti, tj, tk = (i - i0) // TI, j // TJ, k // TK
+    # a tile is full when all of its TI*TJ*TK nodes exist and are free
+    ntj, ntk = (ny + TJ) // TJ, (nz + TK) // TK
+    tile_id = (ti * ntj + tj) * ntk + tk
+    cnt = np.bincount(tile_id[free], minlength=int(tile_id.max()) + 1)
+    full = free & (cnt[tile_id] == TI * TJ * TK)
+    out = []
+    rows = 0
+    for c in range(4):
+        sel = free & (color == c)
+        full_nodes = n[sel & full]
+        full_nodes = full_nodes[np.lexsort((full_nodes, tile_id[full_nodes]))]
+        part = n[sel & ~full]
+        part = part[np.lexsort((part, tile_id[part]))]
+        head = (-rows) % chunk
+        head = min(head, len(part))
+        seq = np.concatenate([part[:head], full_nodes, part[head:]])
+        out.append(seq)
+        rows += len(seq)
+    out.append(n[~free])
+    order = np.concatenate(out)
+    perm = np.empty(mesh.n_nodes, np.int64)
+    perm[order] = np.arange(mesh.n_nodes)
+    return perm
+
+
 def gps_relabel(mesh: TetMesh) -> TetMesh:
     """The mesh renumbered by the Gibbs-Poole-Stockmeyer relabeller (od_relabel_gps, host code
     in the native library; MeshRelabeler::getNewLables, core/meshRelabler.cpp:468-498): new node

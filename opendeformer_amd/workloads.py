@@ -82,9 +82,17 @@ class Workload:
         return self.node_to_free(self.impulse_node_forces)
 
 
+def _color_perm(m: M.TetMesh, ordering: str) -> np.ndarray:
+    """Node permutation of a multicolour ordering: color8 / color5 / color4, or color4t (4
+    colours, each colour's chunks compact tiles; the x = 0 face, fixed by the workloads, last)."""
+    if ordering == "color4t":
+        return M.tiled_color4_permutation(m, M.plane_fixed_nodes(m, axis=0, value=0.0))
+    return M.multicolor_permutation(m, int(ordering[len("color"):]))
+
+
 def _single(name, m: M.TetMesh, kind, E, nu, dt, steps, ordering="natural", impulse=None, meta=None):
-    if ordering in ("color8", "color5", "color4"):
-        perm = M.multicolor_permutation(m, int(ordering[len("color"):]))
+    if ordering in ("color8", "color5", "color4", "color4t"):
+        perm = _color_perm(m, ordering)
         m = M.relabel(m, perm)
         if impulse is not None:
             imp = np.zeros_like(impulse)
@@ -167,8 +175,8 @@ def batch(n_inst=256, seed=4, tet_lo=500, tet_hi=50000, cell=0.05, steps=120, or
         s = seed * 100003 + i
         nx, ny, nz, kind, E, nu = instance_shape(s, tet_lo, tet_hi)
         m = M.box_mesh(nx, ny, nz, nx * cell, ny * cell, nz * cell, seed=s)
-        if ordering in ("color8", "color5", "color4"):
-            m = M.relabel(m, M.multicolor_permutation(m, int(ordering[len("color"):])))
+        if ordering in ("color8", "color5", "color4", "color4t"):
+            m = M.relabel(m, _color_perm(m, ordering))
         off = node_start[-1]
         Xs.append(m.X); Ts.append(m.tets + off); Fs.append(m.facets + off)
         fixed.append(M.plane_fixed_nodes(m, axis=0, value=0.0) + off)
@@ -203,8 +211,8 @@ def served_pool(n_obj=1024, seed=5, tet_lo=1000, tet_hi=500000, cell=0.05, order
         s = seed * 100003 + i
         nx, ny, nz, kind, E, nu = instance_shape(s, tet_lo, tet_hi)
         m = M.box_mesh(nx, ny, nz, nx * cell, ny * cell, nz * cell, seed=s)
-        if ordering in ("color8", "color5", "color4"):
-            m = M.relabel(m, M.multicolor_permutation(m, int(ordering[len("color"):])))
+        if ordering in ("color8", "color5", "color4", "color4t"):
+            m = M.relabel(m, _color_perm(m, ordering))
         elif ordering != "natural":
             raise ValueError(f"unknown ordering {ordering}")
         fixed = M.plane_fixed_nodes(m, axis=0, value=0.0)

@@ -107,6 +107,17 @@ int main() {
     od::Topology t4 = t2;
     t4.constrained = {0, 1, 3 * b.nNodes + 2};   // ... and two instances with it are rejected
     if (od::buildTopology(t4, 64).empty()) return fail("partial constraint over two instances accepted");
+    // the grouped SpMV's layout: one box (a small chunk size, so groups hold several chunks),
+    // two instances, and the partial constraint (no layout)
+    {
+        od::Topology tg = od::Topology();
+        tg.nNodes = b.nNodes; tg.nTets = t.nTets; tg.X = b.X; tg.tets = b.tets; tg.constrained = b.fixed;
+        od::GroupLayout g;
+        if (!od::buildTopology(tg, 8).empty() || !od::buildGroupLayout(tg, 8, 8, g).empty() || !g.ok)
+            return fail("group layout");
+        if (!od::buildGroupLayout(t2, 64, 8, g).empty() || !g.ok || g.nGroups < 2) return fail("group layout, two instances");
+        if (!od::buildGroupLayout(t3, 64, 8, g).empty() || g.ok) return fail("group layout, partial constraint");
+    }
 
     // ---- GPS relabelling (library and oracle)
     std::vector<int> el = b.tets, el2 = b.tets, lab((size_t)b.nNodes), lab2((size_t)b.nNodes);

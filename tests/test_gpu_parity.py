@@ -271,20 +271,22 @@ def test_batch_equals_alone_bitwise_large_instances(ctx, precision):
 
 def test_spmv_variants_give_identical_pcg_bits(ctx):
     """The autotuned SpMV launch shapes (1, 2 or 4 chunks per workgroup, 16, 8 or 4 lanes per
-    block row; for FP32 also the slot-value form with 1, 2 or 4 slots per thread) compute every
-    row and every chunk partial in the same order: PCG results are bitwise equal."""
+    block row; for FP32 also the slot-value form with 1, 2 or 4 slots per thread; the grouped
+    form for both) compute every row and every chunk partial in the same order: PCG results
+    are bitwise equal."""
     import opendeformer_amd as od
     w = W.box_small("color8")
     mesh, s = make(ctx, w)
     table, v64, v32 = od.mesh_autotune(mesh)
-    assert v64[0] in (1, 2, 4) and v32[0] in (1, 2, 4) and v64[1] in (4, 8, 16) and v32[1] in (0, 4, 8, 16)
-    assert len(table) == 21 and all(v > 0 for v in table.values())
+    assert v64[0] in (1, 2, 4) and v32[0] in (1, 2, 4) and v64[1] in (-1, 4, 8, 16) and v32[1] in (-1, 0, 4, 8, 16)
+    assert len(table) == 25 and all(v > 0 for v in table.values())   # rows of <= 15 blocks: grouped form too
     A = newmark_system(s, w)
     Ad = od.Matrix(mesh); Ad.upload(A)
     b = np.random.default_rng(17).standard_normal(mesh.ndof)
     outs = []
     shapes = [((r, ls), (r, ls)) for r, ls in ((1, 16), (2, 16), (4, 16), (1, 8), (2, 8), (4, 8), (1, 4), (2, 4))]
     shapes += [((1, 16), (k, 0)) for k in (1, 2, 4)]
+    shapes += [((k, -1), (k, -1)) for k in (4, 8)]
     for v64s, v32s in shapes:
         od.mesh_spmv_variant(mesh, v64s, v32s)
         for prec in (capi.OD_FP64, capi.OD_MIXED):

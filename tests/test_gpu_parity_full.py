@@ -12,7 +12,7 @@
   served stream over a 64-object pool of 1k-50k tets: every object's state after the mix
       equals its requests replayed alone, bit for bit
 
-Both meshes use the 4-colour node numbering the benchmark runs.  The oracle trajectories run
+Both meshes and the batch use the tiled 4-colour node numbering the benchmark runs.  The oracle trajectories run
 on host threads (ctypes releases the GIL) while the GPU tests run; the oracle's element
 evaluation uses several threads, which changes none of its bits.
 
@@ -80,8 +80,8 @@ class OracleTrajectory(threading.Thread):
 def trajectories():
     O.lib()
     O.set_threads(8)
-    mid = W.box_mid_neo("color4", seed=2)
-    large = W.box_1m_dof("color4", seed=3)
+    mid = W.box_mid_neo("color4t", seed=2)
+    large = W.box_1m_dof("color4t", seed=3)
     runs = {"mid": (mid, OracleTrajectory(mid, MID_STEPS, mid.impulse)),
             "large": (large, OracleTrajectory(large, LARGE_STEPS, pattern=True))}
     for _, t in runs.values():
@@ -171,7 +171,7 @@ def test_batch_256_instances_equal_alone_and_oracle(ctx):
     O.lib()
     O.set_threads(8)
     steps = 120
-    wb = W.batch(256, seed=4, ordering="color4")
+    wb = W.batch(256, seed=4, ordering="color4t")
     mesh = od.Mesh(ctx, wb.X, wb.tets, wb.fixed_dofs, wb.inst_node_start, wb.inst_tet_start)
     mesh.set_material(wb.kind, wb.E, wb.nu, wb.rho)
     irs, _ = mesh.rows()
