@@ -167,7 +167,8 @@ struct od_mesh {
         }
         return (T *)b;
     }
-    std::vector<uint8_t> levelLanesFwd, levelLanesBwd;   // DevMesh points at these (host memory)
+    std::vector<uint8_t> levelLanesFwd, levelLanesBwd;       // DevMesh points at these (host memory)
+    std::vector<uint8_t> levelLanesFwd64, levelLanesBwd64;   // the same for the FP64 sweeps
     std::vector<int> levelChunkPtr;                      // per level, its range in dm.levelChunks
     template <typename T>
     T *upload(const std::vector<T> &v) {
@@ -302,7 +303,8 @@ struct od_solver {
     double innerTol = 1e-4;     // mixed: inner FP32 relative tolerance (floor; see kInnerOuterScale2)
     int maxOuter = 12;
     std::vector<void *> allocs;
-    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *invDiag = nullptr, *U = nullptr, *Ul = nullptr;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *invDiag = nullptr, *U = nullptr;
+    double *U64 = nullptr, *Ul64 = nullptr;   // IC(0): the FP64 sweeps' part-major copies of U
     float *r32 = nullptr, *ri32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
     float *invDiag32 = nullptr, *U32 = nullptr, *Ul32 = nullptr;
     float *sv32 = nullptr;      // FP32 slot values of the matrix (slot-value product form)
@@ -549,14 +551,25 @@ static void autotune_spmv(od_mesh *mesh) {
     }
     // lanes per row of the IC(0) sweeps, per level (meshes with few levels)
     if (t.nLevels <= 64) {
-        tune_ic0_lanes(m, st, t.levelPtr, t.nValues, (int64_t)t.tRow.size(), mesh->levelLanesFwd.data(),
-                       mesh->levelLanesBwd.data());
+        tune_ic0_lanes<float>(m, st, t.levelPtr, mesh->levelLanesFwd.data(), mesh->levelLanesBwd.data());
+        tune_ic0_lanes<double>(m, st, t.levelPtr, mesh->levelLanesFwd64.data(), mesh->levelLanesBwd64.data());
     }
     mesh->spmvTuned = true;
     mesh->formGen++;
 }
 
 static bool scalar_precond(int precond) { return precond == OD_PRE_ICT || precond == OD_PRE_LDLT; }
+
+// The FP64 sweeps' copies of the IC(0) factor (once per solver).
+static void alloc_fp64_sweeps(od_solver *s) {
+    if (s->U64) return;
+    const DevMesh &m = s->mesh->dm;
+    s->U64 = (double *)s->alloc(sizeof(double) * (size_t)ic0_u32_floats(m));
+    s->Ul64 = (double *)s->alloc(sizeof(double) * (size_t)std::max<int64_t>(1, ic0_ul32_floats(m)));
+    OD_HIP(hipMemsetAsync(s->U64, 0, sizeof(double) * (size_t)ic0_u32_floats(m), s->mesh->ctx->stream));
+    OD_HIP(hipMemsetAsync(s->Ul64, 0, sizeof(double) * (size_t)std::max<int64_t>(1, ic0_ul32_floats(m)),
+                          s->mesh->ctx->stream));
+}
 
 static od_solver *make_solver(od_mesh *mesh, int precond, double param, int precision) {
     require(precond == OD_PRE_NONE || precond == OD_PRE_JACOBI || precond == OD_PRE_IC0 || scalar_precond(precond),
@@ -580,12 +593,11 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
         s->p = (double *)s->alloc(8 * n); s->q = (double *)s->alloc(8 * n);
         s->Ax = (double *)s->alloc(8 * n);
         if (precond == OD_PRE_JACOBI) s->invDiag = (double *)s->alloc(8 * n);
-        const size_t nl = 9 * (size_t)t.tRow.size() + kValuePad;
         if (precond == OD_PRE_IC0) {
             s->U = (double *)s->alloc(8 * (nv + kValuePad));
-            // the FP64 lower copy serves only the FP64 sweeps: mixed solvers allocate it on
+            // the FP64 sweep copies serve only the FP64 sweeps: mixed solvers allocate them on
             // the first od_precond_apply
-            if (precision == OD_FP64) s->Ul = (double *)s->alloc(8 * nl);
+            if (precision == OD_FP64) alloc_fp64_sweeps(s);
         }
         if (precision == OD_MIXED) {
             s->r32 = (float *)s->alloc(4 * n); s->ri32 = (float *)s->alloc(4 * n); s->z32 = (float *)s->alloc(4 * n);
@@ -714,11 +726,12 @@ static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip,
         launch_jacobi_diag<double>(m, st, A->vals, s->invDiag);
         if (s->invDiag32) launch_jacobi_diag<float>(m, st, A->vals, s->invDiag32);
     } else if (s->precond == OD_PRE_IC0) {
-        if (s->U32)
-            launch_ic0_factor<float>(m, st, mesh->topo.levelPtr, A->vals, s->U, s->U32, fp64Sweeps ? s->Ul : nullptr,
-                                     s->Ul32, s->param, skip);
-        else launch_ic0_factor<double>(m, st, mesh->topo.levelPtr, A->vals, s->U, (double *)nullptr, s->Ul, (double *)nullptr,
-                                       s->param, skip);
+        if (fp64Sweeps) {
+            alloc_fp64_sweeps(s);
+            launch_ic0_factor<double>(m, st, mesh->topo.levelPtr, A->vals, s->U, s->U64, s->Ul64, s->param, skip);
+        } else {
+            launch_ic0_factor<float>(m, st, mesh->topo.levelPtr, A->vals, s->U, s->U32, s->Ul32, s->param, skip);
+        }
     }
 }
 
@@ -871,7 +884,7 @@ static void solve(od_solver *s, od_matrix *A, const double *b, double *x, double
             if (!s->gv64) s->gv64 = (double *)s->alloc(sizeof(double) * (size_t)grp_values(m));
             launch_fill_grp<double>(m, st, A->vals, s->gv64);
         }
-        run_pcg<double>(s, A->vals, s->U, s->Ul, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre,
+        run_pcg<double>(s, A->vals, s->U64, s->Ul64, s->invDiag, b, x, s->r, s->z, s->p, s->q, tol, maxIt, pre,
                         s->scIn, s->doneIn, false, s->lastMaxIters, nullptr, fixed ? (*fixed)[0] : 0);
         if (fixed) return;
         collect_stats(s, s->scIn, false);
@@ -1216,6 +1229,8 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         // the swept side, else 8
         mesh->levelLanesFwd.assign(std::max(1, t.nLevels), 8);
         mesh->levelLanesBwd.assign(std::max(1, t.nLevels), 8);
+        mesh->levelLanesFwd64.assign(std::max(1, t.nLevels), 8);
+        mesh->levelLanesBwd64.assign(std::max(1, t.nLevels), 8);
         for (int l = 0; l < t.nLevels; l++) {
             double nT = 0, nU = 0;
             for (int q = t.levelPtr[l]; q < t.levelPtr[l + 1]; q++) {
@@ -1224,11 +1239,13 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
                 nU += t.rowPtr[I + 1] - t.rowPtr[I] - 1;
             }
             const double cnt = std::max(1, t.levelPtr[l + 1] - t.levelPtr[l]);
-            mesh->levelLanesFwd[l] = nT / cnt > 8.0 ? 16 : 8;
-            mesh->levelLanesBwd[l] = nU / cnt > 8.0 ? 16 : 8;
+            mesh->levelLanesFwd[l] = mesh->levelLanesFwd64[l] = nT / cnt > 8.0 ? 16 : 8;
+            mesh->levelLanesBwd[l] = mesh->levelLanesBwd64[l] = nU / cnt > 8.0 ? 16 : 8;
         }
         m.levelLanesFwd = mesh->levelLanesFwd.data();
         m.levelLanesBwd = mesh->levelLanesBwd.data();
+        m.levelLanesFwd64 = mesh->levelLanesFwd64.data();
+        m.levelLanesBwd64 = mesh->levelLanesBwd64.data();
         m.levelsContiguous = t.levelsContiguous ? 1 : 0;
         m.chunkRow0 = mesh->upload(t.chunkRow0);
         m.chunkRow1 = mesh->upload(t.chunkRow1);
@@ -1474,6 +1491,8 @@ int od_mesh_sweep_form(od_mesh *mesh, int lanes, int slots) {
     const uint8_t code = (uint8_t)(lanes | (sel << 5));
     std::fill(mesh->levelLanesFwd.begin(), mesh->levelLanesFwd.end(), code);
     std::fill(mesh->levelLanesBwd.begin(), mesh->levelLanesBwd.end(), code);
+    std::fill(mesh->levelLanesFwd64.begin(), mesh->levelLanesFwd64.end(), code);
+    std::fill(mesh->levelLanesBwd64.begin(), mesh->levelLanesBwd64.end(), code);
     mesh->spmvTuned = true;
     mesh->formGen++;
     API_END
@@ -1859,13 +1878,14 @@ int od_pcg_solve_noalloc(od_matrix *A, int precond, const double *b, double *x, 
     API_END
 }
 
-// Applies the preconditioner built last (IC(0): U and, FP64 sweeps, Ul; scalar kinds: the
+// Applies the preconditioner built last (IC(0): the FP64 sweep copies of U; scalar kinds: the
 // uploaded sweeps; Jacobi: D^-1).
 static void precond_solve(od_solver *s, const double *r, double *z) {
     od_mesh *mesh = s->mesh;
     PcgWork w{s->partials, s->scIn, s->doneIn};
     if (s->precond == OD_PRE_IC0)
-        launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U, s->Ul, r, z, nullptr, nullptr);
+        launch_ic0_apply<double>(mesh->dm, mesh->ctx->stream, mesh->topo.levelPtr, s->U64, s->Ul64, r, z, nullptr,
+                                 nullptr);
     else if (s->scalar) launch_scalar_apply(s->scalar->dev, mesh->ctx->stream, r, z);
     else pcg_jacobi<double>(mesh->dm, mesh->ctx->stream, s->precond == OD_PRE_JACOBI ? s->invDiag : nullptr, r, z, w);
 }
@@ -1873,8 +1893,6 @@ static void precond_solve(od_solver *s, const double *r, double *z) {
 int od_precond_preprocess(od_solver *s, od_matrix *A) {
     API_BEGIN
     require(s && A && A->mesh == s->mesh, "bad arguments");
-    if (s->precond == OD_PRE_IC0 && !s->Ul)
-        s->Ul = (double *)s->alloc(8 * (9 * (size_t)s->mesh->topo.tRow.size() + kValuePad));
     build_precond(s, A, s->zeroMask);
     s->preprocessed = true;
     API_END
@@ -1897,8 +1915,6 @@ int od_precond_apply(od_solver *s, od_matrix *A, const double *r, double *z) {
     require(s && A && r && z && A->mesh == s->mesh, "bad arguments");
     require(r != z, "r and z must differ");
     od_mesh *mesh = s->mesh;
-    if (s->precond == OD_PRE_IC0 && !s->Ul)
-        s->Ul = (double *)s->alloc(8 * (9 * (size_t)mesh->topo.tRow.size() + kValuePad));
     build_precond(s, A, s->zeroMask);
     s->preprocessed = true;
     DofVec<double> rv(mesh, 0, r, true, false), zv(mesh, 1, z, false, true);

@@ -72,9 +72,10 @@ struct DevMesh {
     // per row, for the sweeps: {tPtr[r], tPtr[r+1], valOff[r], instance} and
     // {rowPtr[r], rowPtr[r+1], valOff[r], instance}
     const int4 *rowFwd, *rowBwd;
-    // per level, in host memory (read at launch, never on the device): lanes per row (4, 8 or
-    // 16) of the forward and backward sweeps
-    const uint8_t *levelLanesFwd, *levelLanesBwd;
+    // per level, in host memory (read at launch, never on the device): the form of the forward
+    // and backward sweeps (lanes per row 4, 8 or 16 | slot kernel selector << 5), for the FP32
+    // copies and for the FP64 copies
+    const uint8_t *levelLanesFwd, *levelLanesBwd, *levelLanesFwd64, *levelLanesBwd64;
     int *chunkRow0, *chunkRow1, *chunkInst;      // nChunks
     // SpMV run order: the chunks in spatial order, cut into 8 runs of perRun positions (one
     // run per XCD); per position {chunk, first row, end row, instance | kRunLong}
@@ -173,12 +174,14 @@ size_t grp_lds_bytes(const DevMesh &m, int K) {
 }
 template <typename T>
 void launch_fill_grp(const DevMesh &m, hipStream_t s, const T *vals, T *grpVals);
-// FP32 IC(0) factor copies, part-major: position b of an array of nb positions has entries
-// 0-3 at [4 b, 4 b + 4), entries 4-7 at [4 nb + 4 b, ...) and entry 8 at [8 nb + b], so a
-// wave's loads of consecutive positions read consecutive memory.  U32 holds the diagonal
-// blocks first (8 floats per row: the 6 packed entries, 2 unused), then the upper blocks at
-// their backward sweep slots (swNB positions); Ul32 holds the lower blocks at their forward
-// sweep slots (swNF positions).  Padding positions stay zero.
+// IC(0) factor copies of the sweeps (FP32 for the mixed form, FP64 for the FP64 sweeps),
+// part-major: FP32 position b of an array of nb positions has entries 0-3 at [4 b, 4 b + 4),
+// entries 4-7 at [4 nb + 4 b, ...) and entry 8 at [8 nb + b]; FP64 has the pairs (0, 1), (2,
+// 3), (4, 5), (6, 7) at [2 q nb + 2 b, ...) and entry 8 at [8 nb + b]; so a wave's loads of
+// consecutive positions read consecutive memory.  The upper copy holds the diagonal blocks
+// first (8 entries per row: the 6 packed entries, 2 unused), then the upper blocks at their
+// backward sweep slots (swNB positions); the lower copy holds the lower blocks at their
+// forward sweep slots (swNF positions).  Padding positions stay zero.  Sizes in entries:
 inline int64_t ic0_u32_floats(const DevMesh &m) { return 8 * (int64_t)m.nRows + 9 * m.swNB; }
 inline int64_t ic0_ul32_floats(const DevMesh &m) { return 9 * m.swNF; }
 
@@ -226,11 +229,17 @@ void pcg_update_xr_ic0(const DevMesh &m, hipStream_t s, const std::vector<int> &
                        T *r, const T *U, const T *Ul, T *z, PcgWork &w, const uint8_t *skip);
 template <typename T>
 void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *invDiag);
-void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, int64_t nValues, int64_t nT,
-                    uint8_t *fwd, uint8_t *bwd);
+// sweep form of each level for factor copies of type T (FP32: the mixed form's sweeps, FP64:
+// the FP64 sweeps); codes as launch_sweep reads them
+template <typename T>
+void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, uint8_t *fwd, uint8_t *bwd);
+// IC(0) factor U of A (canonical layout) and, when given, its part-major sweep copies of type T
+// (Uc: diagonal blocks, 8 entries per row, then the upper blocks at their backward sweep
+// slots; Ulc: the lower blocks at their forward sweep slots; sizes ic0_u32_floats /
+// ic0_ul32_floats entries), which the sweeps read
 template <typename T>
 void launch_ic0_factor(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const double *A, double *U,
-                       T *U32, double *Ul, T *Ul32, double pivotEps, const uint8_t *skip);
+                       T *Uc, T *Ulc, double pivotEps, const uint8_t *skip);
 template <typename T>
 void launch_ic0_apply(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const T *U, const T *Ul,
                       const T *r, T *z, const InstScalars *sc, const uint8_t *skip);

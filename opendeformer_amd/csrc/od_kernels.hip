@@ -1748,25 +1748,46 @@ __device__ __forceinline__ int64_t pm_at(int64_t nb, int64_t b, int q) {
     return q < 4 ? 4 * b + q : (q < 8 ? 4 * nb + 4 * b + (q - 4) : 8 * nb + b);
 }
 
-// The nine entries of block b of a part-major FP32 array: two float4 loads and one float.
+// The nine entries of block b of a part-major factor copy of nb blocks: FP32 two float4 loads
+// and one float, FP64 four double2 loads and one double.
 __device__ __forceinline__ void load_pm(const float *__restrict__ P, int64_t nb, int64_t b, float (&e)[9]) {
     const float4 a = ((const float4 *)P)[b], c = ((const float4 *)(P + 4 * nb))[b];
     e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
     e[4] = c.x; e[5] = c.y; e[6] = c.z; e[7] = c.w;
     e[8] = P[8 * nb + b];
 }
+__device__ __forceinline__ void load_pm(const double *__restrict__ P, int64_t nb, int64_t b, double (&e)[9]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double2 v = ((const double2 *)(P + 2 * q * nb))[b];
+        e[2 * q] = v.x; e[2 * q + 1] = v.y;
+    }
+    e[8] = P[8 * nb + b];
+}
+// Block b of a part-major copy from FP64 entries, rounded to the copy's type.
+__device__ __forceinline__ void store_pm(float *P, int64_t nb, int64_t b, const double (&X)[9]) {
+    ((float4 *)P)[b] = make_float4((float)X[0], (float)X[1], (float)X[2], (float)X[3]);
+    ((float4 *)(P + 4 * nb))[b] = make_float4((float)X[4], (float)X[5], (float)X[6], (float)X[7]);
+    P[8 * nb + b] = (float)X[8];
+}
+__device__ __forceinline__ void store_pm(double *P, int64_t nb, int64_t b, const double (&X)[9]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) ((double2 *)(P + 2 * q * nb))[b] = make_double2(X[2 * q], X[2 * q + 1]);
+    P[8 * nb + b] = X[8];
+}
 
 // One block row of U per 8-lane group: lane 0 forms and factors the diagonal block,
 // lanes 1..7 the row's upper off-diagonal blocks U_IJ = U_II^-T (A_IJ - sum_K U_KI^T U_KJ),
 // the terms K from the block's precomputed pair list (icPairs: no search).  Every U_KI is
-// read from U itself (transposed value offsets); the FP64 lower copy Ul is written only when
-// given (the FP64 sweeps read it).
+// read from U itself (transposed value offsets).  The sweeps' copies (T: FP32 for the mixed
+// form, FP64 for the FP64 sweeps) are written when given: Uc the diagonal blocks (8 entries
+// per row) then the upper blocks at their backward sweep slots, Ulc the lower blocks at their
+// forward sweep slots, both part-major.
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *__restrict__ rows, int count,
                                                           const double *__restrict__ A, double *__restrict__ U,
-                                                          T *__restrict__ U32, double *__restrict__ Ul,
-                                                          T *__restrict__ Ul32, double eps2,
+                                                          T *__restrict__ U32, T *__restrict__ Ul32, double eps2,
                                                           const uint8_t *__restrict__ skip) {
     const int wg = ((count + 31) >> 5) > 0 ? ((int)(blockIdx.x & 7) * ((((count + 31) >> 5) + 7) >> 3) + (int)(blockIdx.x >> 3)) : 0;
     const int t = wg * 32 + (threadIdx.x >> 3), lane = threadIdx.x & 7;
@@ -1829,36 +1850,24 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
         const int64_t bq = b0 + 1 + q, tq = m.tPos[bq];
 #pragma unroll
         for (int j = 0; j < 9; j++) U[vq + j] = X[j];
-        if (Ul) {
-#pragma unroll
-            for (int j = 0; j < 9; j++) Ul[9 * tq + j] = X[j];
-        }
-        if constexpr (sizeof(T) == 4) {
-            // part-major FP32 copies: one float4, float4, float store each
-            auto put = [&](float *P, int64_t nb, int64_t b) {
-                ((float4 *)P)[b] = make_float4((float)X[0], (float)X[1], (float)X[2], (float)X[3]);
-                ((float4 *)(P + 4 * nb))[b] = make_float4((float)X[4], (float)X[5], (float)X[6], (float)X[7]);
-                P[8 * nb + b] = (float)X[8];
-            };
-            // U_IJ at slot q of row I's backward list, and as block tq - tPtr[J] of row J's forward list
-            if (U32) put(U32 + 8 * (int64_t)m.nRows, m.swNB, m.swBaseB[I] + (int64_t)q * kChunkRows);
-            if (Ul32) {
-                const int J = m.colIdx[bq];
-                put(Ul32, m.swNF, m.swBaseF[J] + (tq - m.tPtr[J]) * kChunkRows);
-            }
+        // U_IJ at slot q of row I's backward list, and as block tq - tPtr[J] of row J's forward list
+        if (U32) store_pm(U32 + 8 * (int64_t)m.nRows, m.swNB, m.swBaseB[I] + (int64_t)q * kChunkRows, X);
+        if (Ul32) {
+            const int J = m.colIdx[bq];
+            store_pm(Ul32, m.swNF, m.swBaseF[J] + (tq - m.tPtr[J]) * kChunkRows, X);
         }
     }
 }
 
 template <typename T>
 void launch_ic0_factor(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, const double *A, double *U,
-                       T *U32, double *Ul, T *Ul32, double pivotEps, const uint8_t *skip) {
+                       T *U32, T *Ul32, double pivotEps, const uint8_t *skip) {
     const double eps2 = pivotEps * pivotEps;
     for (size_t l = 0; l + 1 < levelPtr.size(); l++) {
         int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
         hipLaunchKernelGGL(k_ic0_factor_level<T>, dim3(8 * ((((cnt + 31) / 32) + 7) / 8)), dim3(256), 0, s, m,
-                           m.levelRows + levelPtr[l], cnt, A, U, U32, Ul, Ul32, eps2, skip);
+                           m.levelRows + levelPtr[l], cnt, A, U, U32, Ul32, eps2, skip);
     }
     OD_HIP(hipGetLastError());
 }
@@ -1903,30 +1912,28 @@ __device__ __forceinline__ bool sweep_row_live(const DevMesh &m, int inst, const
     return (d | k) == 0;
 }
 
-// A factor block and the three entries of x it multiplies: the FP32 copies are part-major
-// (sweep slot b of nb), the FP64 factor is in the canonical layout (9 entries from offset aos).
+// A factor block and the three entries of x it multiplies, from a part-major sweep copy (slot
+// b of nb).
 template <typename T>
-__device__ __forceinline__ void load_factor_block(const T *__restrict__ P, int64_t nb, int64_t b, int64_t aos,
+__device__ __forceinline__ void load_factor_block(const T *__restrict__ P, int64_t nb, int64_t b,
                                                   const T *__restrict__ xc, BlockX<T> &blk) {
-    if constexpr (sizeof(T) == 4) {
-        load_pm(P, nb, b, blk.e);
-        blk.x[0] = xc[0]; blk.x[1] = xc[1]; blk.x[2] = xc[2];
-    } else {
-        load_block<T>(P + aos, xc, blk);
-    }
+    load_pm(P, nb, b, blk.e);
+    blk.x[0] = xc[0]; blk.x[1] = xc[1]; blk.x[2] = xc[2];
 }
 
-// The packed diagonal block of row I: FP32 from the diagonal part of U32 (8 floats per row),
-// FP64 from the canonical layout at valOff[I].
+// The packed diagonal block of row I from the diagonal part of a sweep copy (8 entries per row).
 template <typename T>
-__device__ __forceinline__ void load_diag(const T *__restrict__ U, int I, int v0, T (&d)[6]) {
+__device__ __forceinline__ void load_diag(const T *__restrict__ U, int I, T (&d)[6]) {
     if constexpr (sizeof(T) == 4) {
         const float4 a = ((const float4 *)U)[2 * (int64_t)I];
         const float2 c = ((const float2 *)U)[4 * (int64_t)I + 2];
         d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = c.x; d[5] = c.y;
     } else {
 #pragma unroll
-        for (int q = 0; q < 6; q++) d[q] = U[v0 + q];
+        for (int q = 0; q < 3; q++) {
+            const double2 v = ((const double2 *)U)[4 * (int64_t)I + q];
+            d[2 * q] = v.x; d[2 * q + 1] = v.y;
+        }
     }
 }
 
@@ -1986,10 +1993,10 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
         live = true;
     }
     int4 ri = make_int4(0, 0, 0, 0);
-    int sb = 0;                                 // FP32: the row's sweep slot of block 0
+    int sb = 0;                                 // the row's sweep slot of block 0
     if (live) {
         ri = m.rowFwd[I];
-        if constexpr (sizeof(T) == 4) sb = m.swBaseF[I];
+        sb = m.swBaseF[I];
     }
     // the flags of one instance, loaded with the row records (the first lane's record start,
     // never negative, joins the test so that those loads are issued before the branch)
@@ -2005,7 +2012,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
         const int kb = ri.x, ke = ri.y;
         // the diagonal block and the right-hand side, loaded with the blocks
         T d[6], rhs[3];
-        load_diag<T>(U, I, ri.z, d);
+        load_diag<T>(U, I, d);
         rhs[0] = r[3 * I]; rhs[1] = r[3 * I + 1]; rhs[2] = r[3 * I + 2];
         T acc[G][3];
 #pragma unroll
@@ -2016,8 +2023,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
                 const int k = kk + g * L;
                 if (k < ke) {
                     BlockX<T> b;
-                    load_factor_block<T>(Ul, m.swNF, sb + (int64_t)(k - kb) * kChunkRows, 9 * (int64_t)k,
-                                         y + 3 * m.tRow[k], b);
+                    load_factor_block<T>(Ul, m.swNF, sb + (int64_t)(k - kb) * kChunkRows, y + 3 * m.tRow[k], b);
                     acc_add(acc[g][0], row3(b.e[0], b.e[3], b.e[6], b.x[0], b.x[1], b.x[2]));
                     acc_add(acc[g][1], row3(b.e[1], b.e[4], b.e[7], b.x[0], b.x[1], b.x[2]));
                     acc_add(acc[g][2], row3(b.e[2], b.e[5], b.e[8], b.x[0], b.x[1], b.x[2]));
@@ -2061,10 +2067,10 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
         live = true;
     }
     int4 ri = make_int4(0, 0, 0, 0);
-    int sb = 0;                                 // FP32: the row's sweep slot of block 0
+    int sb = 0;                                 // the row's sweep slot of block 0
     if (live) {
         ri = m.rowBwd[I];
-        if constexpr (sizeof(T) == 4) sb = m.swBaseB[I];
+        sb = m.swBaseB[I];
     }
     // the flags of one instance, loaded with the row records (the first lane's record start,
     // never negative, joins the test so that those loads are issued before the branch)
@@ -2077,13 +2083,13 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
     }
     double v[3] = {0, 0, 0};
     if (live) {
-        const int v0 = ri.z, b0 = ri.x, nOff = ri.y - b0 - 1;
+        const int b0 = ri.x, nOff = ri.y - b0 - 1;
         // the diagonal block and the forward result, loaded with the blocks
         T d[6], yv[3], rv[3] = {0, 0, 0};
-        load_diag<T>(U, I, v0, d);
+        load_diag<T>(U, I, d);
         yv[0] = z[3 * I]; yv[1] = z[3 * I + 1]; yv[2] = z[3 * I + 2];
         if constexpr (DOT) { rv[0] = r[3 * I]; rv[1] = r[3 * I + 1]; rv[2] = r[3 * I + 2]; }
-        const T *Uo = sizeof(T) == 4 ? U + 8 * (int64_t)m.nRows : U;   // FP32: the part-major upper blocks
+        const T *Uo = U + 8 * (int64_t)m.nRows;   // the part-major upper blocks
         T acc[G][3];
 #pragma unroll
         for (int g = 0; g < G; g++) acc[g][0] = acc[g][1] = acc[g][2] = 0;
@@ -2093,8 +2099,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
                 const int k = kk + g * L;
                 if (k < nOff) {
                     BlockX<T> b;
-                    load_factor_block<T>(Uo, m.swNB, sb + (int64_t)k * kChunkRows, v0 + 6 + 9 * (int64_t)k,
-                                         z + 3 * m.colIdx[b0 + 1 + k], b);
+                    load_factor_block<T>(Uo, m.swNB, sb + (int64_t)k * kChunkRows, z + 3 * m.colIdx[b0 + 1 + k], b);
                     acc_add(acc[g][0], row3(b.e[0], b.e[1], b.e[2], b.x[0], b.x[1], b.x[2]));
                     acc_add(acc[g][1], row3(b.e[3], b.e[4], b.e[5], b.x[0], b.x[1], b.x[2]));
                     acc_add(acc[g][2], row3(b.e[6], b.e[7], b.e[8], b.x[0], b.x[1], b.x[2]));
@@ -2114,17 +2119,17 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
     if constexpr (DOT) sweep_dot(m, c, p0, count, lr, lane == 0, v, partials);
 }
 
-// FP32 sweeps over the sweep slots: one chunk of 64 rows per workgroup, 16 / K waves.  The
+// Sweeps over the sweep slots (FP32 or FP64 copies): one chunk of 64 rows per workgroup, 16 / K waves.  The
 // thread of row r in wave w keeps the virtual lanes v = w + (16 / K) i (i < K); virtual lane v
 // takes the row's blocks v, v + 16, ..., each wave load one slot position of all 64 rows
-// (1 KB of consecutive float4 per part).  The sums combine as the 16-lane tree combines
+// (1 KB of consecutive float4 or double2 per part).  The sums combine as the 16-lane tree combines
 // them (pairs at distance 8, 4, 2, 1: inside the thread first, then across the waves through
 // LDS), so every K gives the bits of the lane-per-row kernels above.  Rows of the chunk
 // outside the level load their slots and discard the sums.
-template <bool FWD, int K, bool DOT>
+template <typename T, bool FWD, int K, bool DOT>
 __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
-    DevMesh m, int p0, int count, int level, const int *__restrict__ chunks, const float *__restrict__ U,
-    const float *__restrict__ Ul, const float *__restrict__ r, float *__restrict__ y,
+    DevMesh m, int p0, int count, int level, const int *__restrict__ chunks, const T *__restrict__ U,
+    const T *__restrict__ Ul, const T *__restrict__ r, T *__restrict__ y,
     const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip, int lastLevel, int nwg,
     double *__restrict__ partials) {
     constexpr int W = 16 / K;
@@ -2144,22 +2149,22 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     if (!sweep_row_live(m, one ? 0 : m.chunkInst[c], sc, skip, one) | ((off | width) < 0)) return;
     const bool live = I < row1 && (m.levelsContiguous ? I >= p0 && I < p0 + count : m.rowLevel[I] == level);
     const int64_t nb = FWD ? m.swNF : m.swNB;
-    const float *P = FWD ? Ul : U + 8 * (int64_t)m.nRows;
+    const T *P = FWD ? Ul : U + 8 * (int64_t)m.nRows;
     const int *__restrict__ colA = FWD ? m.swColF : m.swColB;
     // the diagonal block and the right-hand side (forward: r; backward: the forward result),
     // loaded with the blocks
-    float d[6] = {1, 0, 0, 1, 0, 1}, rhs[3] = {0, 0, 0}, rv[3] = {0, 0, 0};
+    T d[6] = {1, 0, 0, 1, 0, 1}, rhs[3] = {0, 0, 0}, rv[3] = {0, 0, 0};
     if (w == 0 && live) {
-        load_diag<float>(U, I, 0, d);
-        const float *src = FWD ? r : y;
+        load_diag<T>(U, I, d);
+        const T *src = FWD ? r : y;
         rhs[0] = src[3 * I]; rhs[1] = src[3 * I + 1]; rhs[2] = src[3 * I + 2];
         if constexpr (DOT && !FWD) { rv[0] = r[3 * I]; rv[1] = r[3 * I + 1]; rv[2] = r[3 * I + 2]; }
     }
-    float acc[K][3];
+    T acc[K][3];
 #pragma unroll
-    for (int i = 0; i < K; i++) acc[i][0] = acc[i][1] = acc[i][2] = 0.f;
+    for (int i = 0; i < K; i++) acc[i][0] = acc[i][1] = acc[i][2] = 0;
     for (int kb = 0; kb < width; kb += 16) {
-        float e[K][9], xv[K][3];
+        T e[K][9], xv[K][3];
         bool use[K];
         // every load first: positions past the width repeat the last one (in the chunk's
         // slots), padding columns read the first entries of y
@@ -2170,19 +2175,19 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
             const int col = colA[pos];
             use[i] = live && k < width && col >= 0;
             load_pm(P, nb, pos, e[i]);
-            const float *xc = y + 3 * (int64_t)max(col, 0);
+            const T *xc = y + 3 * (int64_t)max(col, 0);
             xv[i][0] = xc[0]; xv[i][1] = xc[1]; xv[i][2] = xc[2];
         }
 #pragma unroll
         for (int i = 0; i < K; i++) {
             // forward: U_KI^T y_K (columns of the stored block); backward: U_IJ z_J (rows)
-            const float t0 = FWD ? row3(e[i][0], e[i][3], e[i][6], xv[i][0], xv[i][1], xv[i][2])
+            const T t0 = FWD ? row3(e[i][0], e[i][3], e[i][6], xv[i][0], xv[i][1], xv[i][2])
                                  : row3(e[i][0], e[i][1], e[i][2], xv[i][0], xv[i][1], xv[i][2]);
-            const float t1 = FWD ? row3(e[i][1], e[i][4], e[i][7], xv[i][0], xv[i][1], xv[i][2])
+            const T t1 = FWD ? row3(e[i][1], e[i][4], e[i][7], xv[i][0], xv[i][1], xv[i][2])
                                  : row3(e[i][3], e[i][4], e[i][5], xv[i][0], xv[i][1], xv[i][2]);
-            const float t2 = FWD ? row3(e[i][2], e[i][5], e[i][8], xv[i][0], xv[i][1], xv[i][2])
+            const T t2 = FWD ? row3(e[i][2], e[i][5], e[i][8], xv[i][0], xv[i][1], xv[i][2])
                                  : row3(e[i][6], e[i][7], e[i][8], xv[i][0], xv[i][1], xv[i][2]);
-            float a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];
+            T a0 = acc[i][0], a1 = acc[i][1], a2 = acc[i][2];
             acc_add(a0, t0); acc_add(a1, t1); acc_add(a2, t2);
             if (use[i]) { acc[i][0] = a0; acc[i][1] = a1; acc[i][2] = a2; }
         }
@@ -2196,13 +2201,13 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
             acc_add(acc[i][1], acc[i + dv / W][1]);
             acc_add(acc[i][2], acc[i + dv / W][2]);
         }
-    float a0 = acc[0][0], a1 = acc[0][1], a2 = acc[0][2];
+    T a0 = acc[0][0], a1 = acc[0][1], a2 = acc[0][2];
     if constexpr (W > 1) {
-        __shared__ float red[W][kChunkRows][3];
+        __shared__ T red[W][kChunkRows][3];
         red[w][rr][0] = a0; red[w][rr][1] = a1; red[w][rr][2] = a2;
         __syncthreads();
         if (w == 0) {
-            float t[W][3];
+            T t[W][3];
 #pragma unroll
             for (int j = 0; j < W; j++) { t[j][0] = red[j][rr][0]; t[j][1] = red[j][rr][1]; t[j][2] = red[j][rr][2]; }
 #pragma unroll
@@ -2219,16 +2224,16 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     double v[3] = {0, 0, 0};
     if (w == 0 && live) {
         if constexpr (FWD) {
-            float z0 = (rhs[0] - a0) / d[0];
-            float z1 = (rhs[1] - a1 - d[1] * z0) / d[3];
-            float z2 = (rhs[2] - a2 - d[2] * z0 - d[4] * z1) / d[5];
+            T z0 = (rhs[0] - a0) / d[0];
+            T z1 = (rhs[1] - a1 - d[1] * z0) / d[3];
+            T z2 = (rhs[2] - a2 - d[2] * z0 - d[4] * z1) / d[5];
             if (lastLevel) diag_bwd(d, z0, z1, z2);   // rows of the last level have no blocks above them
             y[3 * I] = z0; y[3 * I + 1] = z1; y[3 * I + 2] = z2;
             v[0] = (double)rhs[0] * (double)z0; v[1] = (double)rhs[1] * (double)z1; v[2] = (double)rhs[2] * (double)z2;
         } else {
-            const float x2 = (rhs[2] - a2) / d[5];
-            const float x1 = (rhs[1] - a1 - d[4] * x2) / d[3];
-            const float x0 = (rhs[0] - a0 - d[1] * x1 - d[2] * x2) / d[0];
+            const T x2 = (rhs[2] - a2) / d[5];
+            const T x1 = (rhs[1] - a1 - d[4] * x2) / d[3];
+            const T x0 = (rhs[0] - a0 - d[1] * x1 - d[2] * x2) / d[0];
             y[3 * I] = x0; y[3 * I + 1] = x1; y[3 * I + 2] = x2;
             v[0] = (double)rv[0] * (double)x0; v[1] = (double)rv[1] * (double)x1; v[2] = (double)rv[2] * (double)x2;
         }
@@ -2237,8 +2242,8 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
 }
 
 // One sweep launch of level l in the form `code` names: code & 31 is the lanes per row of
-// the lane-per-row kernels (4, 8 or 16; FP64 always runs these), code >> 5 (1-4) selects the
-// FP32 slot kernel with K = 2, 4, 8 or 16 slots per thread.  With partials set, the launch
+// the lane-per-row kernels (4, 8 or 16), code >> 5 (1-4) selects the slot kernel with K = 2,
+// 4, 8 or 16 slots per thread.  With partials set, the launch
 // also forms the r.z chunk partials of the rows whose z it finishes (one instance,
 // consecutive levels).  The optional events are recorded from the dispatch's own begin/end
 // timestamps (lane autotuner).
@@ -2248,7 +2253,7 @@ static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, in
                          double *partials, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const bool dot = partials != nullptr;
     if (dot && (m.nInst != 1 || !m.levelsContiguous)) throw HipError("sweep dot needs one instance, consecutive levels");
-    const int L = code & 31, sel = sizeof(T) == 4 ? code >> 5 : 0;
+    const int L = code & 31, sel = code >> 5;
     if (sel) {
         const int c0 = m.levelChunkPtr[level], nwg = m.levelChunkPtr[level + 1] - c0;
         if (nwg <= 0) return;
@@ -2256,17 +2261,16 @@ static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, in
         auto go = [&](auto kernel) {
             const dim3 grid(xcd_grid(nwg)), block(1024 / K);
             if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, m, p0, cnt, level, m.levelChunks + c0,
-                                          (const float *)U, (const float *)Ul, (const float *)r, (float *)z, sc, skip,
-                                          last, nwg, partials);
-            else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, p0, cnt, level, m.levelChunks + c0, (const float *)U,
-                                    (const float *)Ul, (const float *)r, (float *)z, sc, skip, last, nwg, partials);
+                                          U, Ul, r, z, sc, skip, last, nwg, partials);
+            else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, p0, cnt, level, m.levelChunks + c0, U, Ul, r, z, sc,
+                                    skip, last, nwg, partials);
         };
         auto pick = [&](auto fwdTag, auto dotTag) {
             constexpr bool F = decltype(fwdTag)::value, D = decltype(dotTag)::value;
-            if (K == 2) go(k_ic0_slots<F, 2, D>);
-            else if (K == 4) go(k_ic0_slots<F, 4, D>);
-            else if (K == 8) go(k_ic0_slots<F, 8, D>);
-            else go(k_ic0_slots<F, 16, D>);
+            if (K == 2) go(k_ic0_slots<T, F, 2, D>);
+            else if (K == 4) go(k_ic0_slots<T, F, 4, D>);
+            else if (K == 8) go(k_ic0_slots<T, F, 8, D>);
+            else go(k_ic0_slots<T, F, 16, D>);
         };
         if (fwd) { if (dot) pick(std::true_type{}, std::true_type{}); else pick(std::true_type{}, std::false_type{}); }
         else { if (dot) pick(std::false_type{}, std::true_type{}); else pick(std::false_type{}, std::false_type{}); }
@@ -2308,23 +2312,24 @@ static void ic0_levels(const DevMesh &m, hipStream_t s, const std::vector<int> &
         const int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
         const int last = (fuseLast && l == nl - 1) ? 1 : 0;
-        launch_sweep<T>(true, m.levelLanesFwd[l], m, s, l, levelPtr[l], cnt, U, Ul, r, z, sc, skip, last,
-                        last ? dotPartials : nullptr);
+        launch_sweep<T>(true, (sizeof(T) == 8 ? m.levelLanesFwd64 : m.levelLanesFwd)[l], m, s, l, levelPtr[l], cnt, U,
+                        Ul, r, z, sc, skip, last, last ? dotPartials : nullptr);
     }
     for (int l = b1 - 1; l >= 0; l--) {
         const int cnt = levelPtr[l + 1] - levelPtr[l];
         if (cnt <= 0) continue;
-        launch_sweep<T>(false, m.levelLanesBwd[l], m, s, l, levelPtr[l], cnt, U, Ul, r, z, sc, skip, 0, dotPartials);
+        launch_sweep<T>(false, (sizeof(T) == 8 ? m.levelLanesBwd64 : m.levelLanesBwd)[l], m, s, l, levelPtr[l], cnt, U,
+                        Ul, r, z, sc, skip, 0, dotPartials);
     }
     OD_HIP(hipGetLastError());
 }
 
-// Launch form of each level's FP32 sweeps: times 4, 8 and 16 lanes per row and the slot
-// kernel with 2, 4, 8 and 16 slots per thread (median of three launches each, on zero-filled
-// factors of this mesh's pattern) and keeps the fastest (launch_sweep's code).
+// Launch form of each level's sweeps in precision T: times 4, 8 and 16 lanes per row and the
+// slot kernel with 2, 4, 8 and 16 slots per thread (median of three launches each, on
+// zero-filled factors of this mesh's pattern) and keeps the fastest (launch_sweep's code).
 // The last level's forward launch is timed with its fused backward step, as the solver runs it.
-void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, int64_t nValues, int64_t nT,
-                    uint8_t *fwd, uint8_t *bwd) {
+template <typename T>
+void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, uint8_t *fwd, uint8_t *bwd) {
     const int nl = (int)levelPtr.size() - 1;
     const size_t n = 3 * (size_t)m.nRows;
     std::vector<void *> bufs;
@@ -2337,9 +2342,8 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
         return p;
     };
     try {
-        (void)nValues; (void)nT;
-        float *U = (float *)get(4 * (size_t)ic0_u32_floats(m)), *Ul = (float *)get(4 * (size_t)ic0_ul32_floats(m));
-        float *r = (float *)get(4 * n), *z = (float *)get(4 * n);
+        T *U = (T *)get(sizeof(T) * (size_t)ic0_u32_floats(m)), *Ul = (T *)get(sizeof(T) * (size_t)ic0_ul32_floats(m));
+        T *r = (T *)get(sizeof(T) * n), *z = (T *)get(sizeof(T) * n);
         // the launches that finish z form the r.z partials when the solver fuses that dot
         double *dots = ic0_dot_fused(m, levelPtr) ? (double *)get(8 * (size_t)std::max(1, m.nChunks)) : nullptr;
         OD_HIP(hipEventCreate(&e0));
@@ -2358,8 +2362,8 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
                     float ms[3];
                     for (int rep = 0; rep < 3; rep++) {
                         const bool last = dir == 0 && l == nl - 1, finishes = last || (dir == 1 && l < nl - 1);
-                        launch_sweep<float>(dir == 0, cv, m, s, l, levelPtr[l], cnt, U, Ul, r, z, nullptr, nullptr,
-                                            last ? 1 : 0, finishes ? dots : nullptr, e0, e1);
+                        launch_sweep<T>(dir == 0, cv, m, s, l, levelPtr[l], cnt, U, Ul, r, z, nullptr, nullptr,
+                                        last ? 1 : 0, finishes ? dots : nullptr, e0, e1);
                         OD_HIP(hipGetLastError());
                         OD_HIP(hipEventSynchronize(e1));
                         OD_HIP(hipEventElapsedTime(&ms[rep], e0, e1));
@@ -2405,7 +2409,7 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xr_ic0(DevMesh m, const 
     const bool lev0 = act && threadIdx.x < kChunkRows && row < row1 &&
                       (level0End >= 0 ? row < level0End : m.rowLevel[row] == 0);
     T d[6] = {1, 0, 0, 1, 0, 1};
-    if (lev0) load_diag<T>(U, row, sizeof(T) == 4 ? 0 : m.rowFwd[row].z, d);
+    if (lev0) load_diag<T>(U, row, d);
     double v[1] = {0};
     T ri = 0;
     if (in && act) {
@@ -2576,8 +2580,10 @@ void launch_set_done_from_active(const DevMesh &m, hipStream_t s, uint8_t *preDo
 INST_T(double)
 INST_T(float)
 template void launch_ic0_factor<double>(const DevMesh &, hipStream_t, const std::vector<int> &, const double *, double *,
-                                        double *, double *, double *, double, const uint8_t *);
+                                        double *, double *, double, const uint8_t *);
 template void launch_ic0_factor<float>(const DevMesh &, hipStream_t, const std::vector<int> &, const double *, double *,
-                                       float *, double *, float *, double, const uint8_t *);
+                                       float *, float *, double, const uint8_t *);
+template void tune_ic0_lanes<double>(const DevMesh &, hipStream_t, const std::vector<int> &, uint8_t *, uint8_t *);
+template void tune_ic0_lanes<float>(const DevMesh &, hipStream_t, const std::vector<int> &, uint8_t *, uint8_t *);
 
 }  // namespace od
