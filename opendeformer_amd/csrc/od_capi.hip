@@ -1133,6 +1133,15 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.pairBlk = mesh->upload(t.pairBlk);
         m.pairSlot = mesh->upload(t.pairSlot);
         m.pairT = mesh->upload(t.pairT);
+        m.slotSrc = nullptr; m.rotC = nullptr;
+        if (!t.partial) {
+            // the source of each contribution slot, for the compact corotational form
+            require(t.nTets < (1 << 26), "too many tets for the contribution source encoding", OD_ERR_UNSUPPORTED);
+            std::vector<int> src((size_t)std::max<int64_t>(1, t.nPairContrib), 0);
+            for (size_t k = 0; k < t.pairSlot.size(); k++)
+                if (t.pairSlot[k] >= 0) src[t.pairSlot[k]] = (int)((k / 10) << 5 | (k % 10) << 1 | t.pairT[k]);
+            m.slotSrc = mesh->upload(src);
+        }
         m.blkCStart = mesh->upload(t.blkCStart);
         m.nodeSlot = mesh->upload(t.nodeSlot);
         m.rowCStart = mesh->upload(t.rowCStart);
@@ -1548,6 +1557,8 @@ int od_mesh_set_material(od_mesh *mesh, const int *kind, const double *E, const 
     mesh->dm.thrNeo = neoThreshold;
     mesh->dm.hasKind[0] = mesh->dm.hasKind[1] = mesh->dm.hasKind[2] = 0;
     for (int i = 0; i < n; i++) mesh->dm.hasKind[kind[i]] = 1;
+    if (mesh->dm.hasKind[0] && mesh->dm.slotSrc && !mesh->dm.rotC)   // compact corotational form
+        mesh->dm.rotC = (double *)mesh->alloc(sizeof(double) * 16 * (size_t)n);
     if (mesh->dm.hasKind[2] && !mesh->dm.plStrain) {
         // plastic tets before od_mesh_set_plastic: never yielding (the corotational response)
         std::vector<double> inf(n, HUGE_VAL), zero(n, 0.0);
@@ -1657,7 +1668,7 @@ int od_mass_matrix(od_mesh *mesh, od_matrix *M) {
     mesh->ensureContrib();
     ElemParams P{1, 0, 0, 0, 0, 0, 0};
     launch_element(mesh->dm, mesh->ctx->stream, nullptr, nullptr, nullptr, P, mesh->blkC, mesh->nodeC, nullptr);
-    launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, M->vals, 1, nullptr, nullptr);
+    launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, M->vals, 1, nullptr, nullptr, false);
     API_END
 }
 
@@ -1669,7 +1680,7 @@ int od_assemble(od_mesh *mesh, const double *u_dev, od_matrix *K, double *fint_d
     ElemParams P{0, fint_dev ? 1 : 0, 0, 0, 0, 0, 0};
     DofVec<double> u(mesh, 0, u_dev, true, false), f(mesh, 1, fint_dev, true, true);
     launch_element(mesh->dm, mesh->ctx->stream, u.dev, nullptr, nullptr, P, mesh->blkC, mesh->nodeC, nullptr);
-    launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, K->vals, 1, nullptr, nullptr);
+    launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, K->vals, 1, nullptr, nullptr, true);
     if (fint_dev) launch_gather_rhs(mesh->dm, mesh->ctx->stream, mesh->nodeC, nullptr, nullptr, 1.0, f.dev, 1, nullptr);
     f.done();
     API_END
@@ -1966,7 +1977,8 @@ static od_integrator *make_integrator(od_mesh *mesh, int kind, int precond, doub
         it->lastRelres.assign(mesh->topo.nInst, 0.0);
         ElemParams P{1, 0, 0, 0, 0, 0, 0};
         launch_element(mesh->dm, mesh->ctx->stream, nullptr, nullptr, nullptr, P, mesh->blkC, mesh->nodeC, nullptr);
-        launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, it->M.vals, 0, nullptr, nullptr);
+        launch_gather_blocks(mesh->dm, mesh->ctx->stream, mesh->blkC, 1.0, nullptr, 0.0, it->M.vals, 0, nullptr, nullptr,
+                             false);
         OD_HIP(hipStreamSynchronize(mesh->ctx->stream));
     } catch (...) {
         delete it;
@@ -2096,7 +2108,7 @@ static void integrator_step_eager(od_integrator *it, const uint8_t *actOverride 
         ElemParams P{0, 1, h + aK, (0.5 - beta) * h * h + aK * (1.0 - gamma) * h, 1.0, (1.0 - gamma) * h, aM};
         launch_element(m, st, it->d, it->v, it->a, P, mesh->blkC, mesh->nodeC, act);
         const double cK = gamma * h * aK + beta * h * h, cM = 1.0 + gamma * h * aM;
-        launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act);
+        launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act, true);
         launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, 1.0, it->rhs, 0, act);
         // the solve runs in place on aNew, warm-started from a; a and aNew keep their
         // addresses, so the buffers baked into captured PCG graphs stay valid from step to step
@@ -2111,7 +2123,8 @@ static void integrator_step_eager(od_integrator *it, const uint8_t *actOverride 
         const double aM = it->alphaM, aK = it->alphaK;
         ElemParams P{0, 1, h + aK, 0.0, 1.0, 0.0, aM};
         launch_element(m, st, it->d, it->v, nullptr, P, mesh->blkC, mesh->nodeC, act);
-        launch_gather_blocks(m, st, mesh->blkC, h * (h + aK), it->M.vals, 1.0 + h * aM, it->A.vals, 0, nullptr, act);
+        launch_gather_blocks(m, st, mesh->blkC, h * (h + aK), it->M.vals, 1.0 + h * aM, it->A.vals, 0, nullptr, act,
+                             true);
         launch_gather_rhs(m, st, mesh->nodeC, it->fext, it->hasImpulse ? it->imp : nullptr, h, it->rhs, 0, act);
         if (it->hasDamp) {
             // the tangent (h + aK) K + aM M + D: A gains h D, the right-hand side -h D v
@@ -2184,7 +2197,7 @@ static void newmark_graph_body(od_integrator *it, const std::vector<int> &plan) 
     ElemParams P{0, 1, h + aK, (0.5 - beta) * h * h + aK * (1.0 - gamma) * h, 1.0, (1.0 - gamma) * h, aM};
     launch_element(m, st, it->d, it->v, it->a, P, mesh->blkC, mesh->nodeC, act);
     const double cK = gamma * h * aK + beta * h * h, cM = 1.0 + gamma * h * aM;
-    launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act);
+    launch_gather_blocks(m, st, mesh->blkC, cK, it->M.vals, cM, it->A.vals, 0, it->A.vals32, act, true);
     launch_gather_rhs(m, st, mesh->nodeC, it->fext, nullptr, 1.0, it->rhs, 0, act);
     const size_t n = (size_t)mesh->topo.ndof;
     hipLaunchKernelGGL(k_copy_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, it->aNew, (const double *)it->a, n);

@@ -60,6 +60,13 @@ struct DevMesh {
     int *tPtr, *tRow, *tBlk, *tOff;             // transposed index (+ value offset of each block)
     int *tPos;                                   // per block: its place in the transposed list (-1: diagonal)
     int *pairBlk, *pairSlot; uint8_t *pairT;    // 10 per tet
+    // corotational tets in compact form (no partially constrained nodes): per contribution slot
+    // its source (tet << 5 | pair << 1 | transposed); per tet, from the last stiffness pass, the
+    // rotated shape-function gradients g_q = R dN_q (12) and lambda / (36 V0), mu / (36 V0).
+    // The element pass stores these instead of the 10 blocks, and the block gather forms the
+    // blocks R K0_ab R^T from them.
+    int *slotSrc;
+    double *rotC;                                // 16 per tet (NULL: no compact form)
     int *blkCStart;                              // nBlocks+1
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
     int *levelRows;                              // nRows
@@ -126,8 +133,10 @@ struct ElemParams {
 // ---- kernel launchers (od_kernels.hip) ------------------------------------------
 void launch_element(const DevMesh &m, hipStream_t s, const double *d, const double *v, const double *a,
                     const ElemParams &p, double *blkC, double *nodeC, const uint8_t *instActive);
+// stiff: the contributions come from a stiffness pass (corotational tets in compact form are
+// formed from their rotations); otherwise (mass pass) every contribution is a stored block
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
-                          double *out, int accumulate, float *out32, const uint8_t *instActive);
+                          double *out, int accumulate, float *out32, const uint8_t *instActive, bool stiff);
 void launch_gather_rhs(const DevMesh &m, hipStream_t s, const double *nodeC, const double *fext, const double *imp,
                        double cf, double *out, int mode, const uint8_t *instActive);
 void launch_body_force(const DevMesh &m, hipStream_t s, const double *bodyPerTet, double *nodeC, double *f);

@@ -26,6 +26,33 @@ using namespace odk;
 // =====================================================================================
 // element kernel
 // =====================================================================================
+// The corotational stiffness block of node pair (a, b) at rest, K0_ab = V B_a^T D B_b for the
+// isotropic D (D0 = lambda + 2 mu, D1 = lambda, D2 = mu) with factor = 1 / (36 V) and the
+// gradients Ba, Bb scaled by 6 V (corotationalHyperelastic.cpp:20-41, tetelement.cpp:546-620).
+__device__ __forceinline__ void corot_k0(const v3 Ba, const v3 Bb, double D0, double D1, double D2, double factor,
+                                         double (&k)[9]) {
+    k[0] = (D0 * Ba.x * Bb.x + D2 * (Ba.y * Bb.y + Ba.z * Bb.z)) * factor;
+    k[1] = (D1 * Ba.x * Bb.y + D2 * Ba.y * Bb.x) * factor;
+    k[2] = (D1 * Ba.x * Bb.z + D2 * Ba.z * Bb.x) * factor;
+    k[3] = (D1 * Ba.y * Bb.x + D2 * Ba.x * Bb.y) * factor;
+    k[4] = (D0 * Ba.y * Bb.y + D2 * (Ba.x * Bb.x + Ba.z * Bb.z)) * factor;
+    k[5] = (D1 * Ba.y * Bb.z + D2 * Ba.z * Bb.y) * factor;
+    k[6] = (D1 * Ba.z * Bb.x + D2 * Ba.x * Bb.z) * factor;
+    k[7] = (D1 * Ba.z * Bb.y + D2 * Ba.y * Bb.z) * factor;
+    k[8] = (D0 * Ba.z * Bb.z + D2 * (Ba.x * Bb.x + Ba.y * Bb.y)) * factor;
+}
+
+// KK = R k R^T
+__device__ __forceinline__ void rotate_block(const double *R, const double *k, double *KK) {
+    double QK[9];
+    mul3(R, k, QK);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            KK[i * 3 + j] = QK[i * 3 + 0] * R[j * 3 + 0] + QK[i * 3 + 1] * R[j * 3 + 1] + QK[i * 3 + 2] * R[j * 3 + 2];
+}
+
 __device__ __forceinline__ void store_block(double *dst, const double *m, int transpose) {
     if (transpose) {
         dst[0] = m[0]; dst[1] = m[3]; dst[2] = m[6];
@@ -188,6 +215,19 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
                 h[q] = mk(0, 0, 0);
             }
         }
+        // compact form (stiffness pass): the rotated gradients g_q = R dN_q and the factors
+        // lambda / (36 V0), mu / (36 V0); the block gather forms the blocks from them
+        const bool compact = KIND == 0 && m.rotC != nullptr;
+        if (compact) {
+            double *o = m.rotC + 16 * (size_t)e;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const v3 gq = mulv(R, dN[q]);
+                o[3 * q] = gq.x; o[3 * q + 1] = gq.y; o[3 * q + 2] = gq.z;
+            }
+            o[12] = lambda * factor;
+            o[13] = mu * factor;
+        }
         // both pair loops unrolled: every per-node array index is a constant (registers)
 #pragma unroll
         for (int aI = 0; aI < 4; aI++) {
@@ -196,24 +236,13 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void k
                 const int p = 4 * aI - aI * (aI - 1) / 2 + (bI - aI);   // pair index (aI, bI), aI <= bI
                 const v3 Ba = dN[aI], Bb = dN[bI];
                 double k[9];
-                k[0] = (D0 * Ba.x * Bb.x + D2 * (Ba.y * Bb.y + Ba.z * Bb.z)) * factor;
-                k[1] = (D1 * Ba.x * Bb.y + D2 * Ba.y * Bb.x) * factor;
-                k[2] = (D1 * Ba.x * Bb.z + D2 * Ba.z * Bb.x) * factor;
-                k[3] = (D1 * Ba.y * Bb.x + D2 * Ba.x * Bb.y) * factor;
-                k[4] = (D0 * Ba.y * Bb.y + D2 * (Ba.x * Bb.x + Ba.z * Bb.z)) * factor;
-                k[5] = (D1 * Ba.y * Bb.z + D2 * Ba.z * Bb.y) * factor;
-                k[6] = (D1 * Ba.z * Bb.x + D2 * Ba.x * Bb.z) * factor;
-                k[7] = (D1 * Ba.z * Bb.y + D2 * Ba.y * Bb.z) * factor;
-                k[8] = (D0 * Ba.z * Bb.z + D2 * (Ba.x * Bb.x + Ba.y * Bb.y)) * factor;
-                double QK[9], KK[9];
-                mul3(R, k, QK);
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-#pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        KK[i * 3 + j] = QK[i * 3 + 0] * R[j * 3 + 0] + QK[i * 3 + 1] * R[j * 3 + 1] + QK[i * 3 + 2] * R[j * 3 + 2];
-                int slot = m.pairSlot[10 * e + p];
-                if (slot >= 0) store_block(blkC + (size_t)slot * 9, KK, m.pairT[10 * e + p]);
+                corot_k0(Ba, Bb, D0, D1, D2, factor, k);
+                double KK[9];
+                if (!compact) {
+                    rotate_block(R, k, KK);
+                    int slot = m.pairSlot[10 * e + p];
+                    if (slot >= 0) store_block(blkC + (size_t)slot * 9, KK, m.pairT[10 * e + p]);
+                }
                 if (!P.nodeTerms) continue;
                 if (KIND == 2) {   // the K_e y term; the forces come from the elastic stress below
                     g[aI] = g[aI] + mulv(KK, y[bI]);
@@ -389,8 +418,89 @@ __global__ void k_gather_values(DevMesh m, const double *__restrict__ C, double 
     if (out32) out32[vi] = (float)o;
 }
 
+// Node pair (a, b), a <= b, of pair index p (0-9).
+__constant__ int c_pairA[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+__constant__ int c_pairB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+
+// A = cK K + cM M with the compact corotational form, one thread per block: its contributions
+// in slot order.  A corotational tet's block is R K0_ab R^T; with K0_ab = f (lambda a b^T +
+// mu b a^T + mu (a . b) I) (the isotropic B^T D B of corot_k0, a = dN_a, b = dN_b, f =
+// 1 / (36 V0)) and R R^T = I that is f (lambda g_a g_b^T + mu g_b g_a^T + mu (g_a . g_b) I)
+// with the rotated gradients g = R dN the element pass stored; its transpose swaps a and b.
+// Any other tet's block is read from C.  Each value is summed in slot order, as
+// k_gather_blocks sums it.
+template <bool ALLCOROT>
+__global__ void k_gather_blocks_rot(DevMesh m, const double *__restrict__ C, double cK, const double *__restrict__ M,
+                                    double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
+                                    const uint8_t *__restrict__ instActive) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nBlocks) return;
+    const int r = m.blkRow[b], col = m.colIdx[b], k0 = m.blkCStart[b], k1 = m.blkCStart[b + 1];
+    if (instActive && !instActive[m.rowInst[r]]) return;
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // contributions in batches of four whose loads are issued together (past the end: slot k0,
+    // dropped); the sums keep the slot order
+    constexpr int NB = 4;
+    for (int k = k0; k < k1; k += NB) {
+        int src[NB];
+        double gv[NB][8];
+#pragma unroll
+        for (int j = 0; j < NB; j++) src[j] = m.slotSrc[k + j < k1 ? k + j : k0];
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const int e = src[j] >> 5, p = (src[j] >> 1) & 15, tr = src[j] & 1;
+            const int a = tr ? c_pairB[p] : c_pairA[p], bb = tr ? c_pairA[p] : c_pairB[p];
+            const double *g = m.rotC + 16 * (size_t)e;
+            gv[j][0] = g[3 * a]; gv[j][1] = g[3 * a + 1]; gv[j][2] = g[3 * a + 2];
+            gv[j][3] = g[3 * bb]; gv[j][4] = g[3 * bb + 1]; gv[j][5] = g[3 * bb + 2];
+            gv[j][6] = g[12]; gv[j][7] = g[13];
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            if (k + j >= k1) break;
+            double kk[9];
+            const int e = src[j] >> 5;
+            if (ALLCOROT || m.kind[e] == 0) {
+                const double *ga = gv[j], *gb = gv[j] + 3;
+                const double lf = gv[j][6], mf = gv[j][7];
+                const double dm = mf * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int q = 0; q < 3; q++) kk[3 * i + q] = lf * ga[i] * gb[q] + mf * gb[i] * ga[q] + (i == q ? dm : 0.0);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 9; i++) kk[i] = C[(size_t)(k + j) * 9 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < 9; i++) s[i] += kk[i];
+        }
+    }
+    const int64_t bo = m.blkOff[b];
+    const bool diag = col == r;
+    const int nv = diag ? 6 : 9;
+    for (int c = 0; c < nv; c++) {
+        const int64_t vi = bo + c;
+        double o = cK * s[diag ? c_diagSrc[c] : c];
+        if (M) o += cM * M[vi];
+        if (accumulate) o += out[vi];
+        out[vi] = o;
+        if (out32) out32[vi] = (float)o;
+    }
+}
+
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
-                          double *out, int accumulate, float *out32, const uint8_t *instActive) {
+                          double *out, int accumulate, float *out32, const uint8_t *instActive, bool stiff) {
+    if (stiff && m.rotC && m.hasKind[0] && !m.vPtr) {
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, dim3((unsigned)((m.nBlocks + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m, blkC,
+                               cK, M, cM, out, accumulate, out32, instActive);
+        };
+        if (!m.hasKind[1] && !m.hasKind[2]) go(k_gather_blocks_rot<true>);
+        else go(k_gather_blocks_rot<false>);
+        OD_HIP(hipGetLastError());
+        return;
+    }
     if (m.vPtr) {
         hipLaunchKernelGGL(k_gather_values, dim3((unsigned)((m.nValues + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m,
                            blkC, cK, M, cM, out, accumulate, out32, instActive);
