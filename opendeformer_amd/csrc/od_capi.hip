@@ -738,8 +738,9 @@ static void build_precond(od_solver *s, const od_matrix *A, const uint8_t *skip,
 // PCG on the chunked instances: x in/out, b fixed.  preDone marks instances to skip.
 // FP32 solves fuse the direction update into the product: p holds two direction buffers of
 // ndof each, iteration k reads buffer k & 1 and its product writes the new direction
-// z + beta p into the other one.  FP64 solves update p in place after the dot product
-// (the fused FP64 product needs more registers than two workgroups per CU allow).
+// z + beta p into the other one.  FP64 solves update p in place after the dot product (the
+// fused FP64 descriptor product needs more registers than two workgroups per CU allow, and
+// the fused grouped FP64 product measured 74 us against 63 us plus 6.6 us for this pass).
 template <typename T>
 static void run_pcg(od_solver *s, const T *vals, const T *Uf, const T *Ul, const T *invDiag, const T *b,
                     T *x, T *r, T *z, T *p, T *q, double tol, int maxIt, const uint8_t *preDone, InstScalars *sc,

@@ -1815,8 +1815,9 @@ void pcg_dot_rz(const DevMesh &m, hipStream_t s, const T *r, const T *z, PcgWork
 }
 
 // p = z + beta p (skipped for instances that are done): the direction update of the FP64
-// solve, whose product runs without the fused update (it would need more registers than two
-// workgroups per CU allow)
+// solve, whose products run without the fused update (the descriptor shapes would need more
+// registers than two workgroups per CU allow; the grouped product measured slower with it
+// than this separate pass)
 template <typename T>
 __global__ __launch_bounds__(kVecThreads) void k_update_p(DevMesh m, const T *z, T *p, const InstScalars *sc) {
     CHUNK_PROLOGUE
