@@ -1134,7 +1134,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.pairBlk = mesh->upload(t.pairBlk);
         m.pairSlot = mesh->upload(t.pairSlot);
         m.pairT = mesh->upload(t.pairT);
-        m.slotSrc = nullptr; m.rotC = nullptr;
+        m.slotSrc = nullptr; m.rotC = nullptr; m.gatherOrder = nullptr; m.gatherPerRun = 0;
         if (!t.partial) {
             // the source of each contribution slot, for the compact corotational form
             require(t.nTets < (1 << 26), "too many tets for the contribution source encoding", OD_ERR_UNSUPPORTED);
@@ -1304,6 +1304,23 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         }
         m.spmvR64d = m.spmvR64;
         m.spmvL64d = m.spmvL64;
+        if (m.slotSrc) {
+            // the compact gather's block order: the blocks of each XCD run's chunks, in order
+            const int nc = (int)t.chunkRow0.size();
+            std::vector<std::vector<int>> runs(8);
+            for (int p = 0; p < nc; p++) {
+                const int c = t.chunkSorted[p];
+                auto &run = runs[std::min(7, p / std::max(1, t.perRun))];
+                for (int r = t.chunkRow0[c]; r < t.chunkRow1[c]; r++)
+                    for (int b = t.rowPtr[r]; b < t.rowPtr[r + 1]; b++) run.push_back(b);
+            }
+            size_t per = 0;
+            for (const auto &run : runs) per = std::max(per, (run.size() + kBlock - 1) / kBlock);
+            std::vector<int> order(8 * per * kBlock, -1);
+            for (int x = 0; x < 8; x++) std::copy(runs[x].begin(), runs[x].end(), order.begin() + (size_t)x * per * kBlock);
+            m.gatherPerRun = (int)per;
+            m.gatherOrder = mesh->upload(order);
+        }
         {
             // grouped symmetric product: the layout, when the mesh allows it
             GroupLayout g;

@@ -67,6 +67,10 @@ struct DevMesh {
     // blocks R K0_ab R^T from them.
     int *slotSrc;
     double *rotC;                                // 16 per tet (NULL: no compact form)
+    // the compact gather's block order: 8 runs (one per XCD) of gatherPerRun workgroups of
+    // kBlock blocks each, the blocks of the SpMV runs' chunks in order, padded with -1
+    int *gatherOrder;
+    int gatherPerRun;
     int *blkCStart;                              // nBlocks+1
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
     int *levelRows;                              // nRows

@@ -433,8 +433,13 @@ template <bool ALLCOROT>
 __global__ void k_gather_blocks_rot(DevMesh m, const double *__restrict__ C, double cK, const double *__restrict__ M,
                                     double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
                                     const uint8_t *__restrict__ instActive) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= m.nBlocks) return;
+    // blocks in the SpMV's run order (chunks sorted in space, one run per XCD): the j-th
+    // workgroup on XCD x takes the next 256 blocks of run x, so an XCD's L2 keeps the tets of
+    // its slab while all their blocks are formed
+    const int j = (int)(blockIdx.x >> 3);
+    if (j >= m.gatherPerRun) return;
+    const int b = m.gatherOrder[((int64_t)(blockIdx.x & 7) * m.gatherPerRun + j) * kBlock + threadIdx.x];
+    if (b < 0) return;
     const int r = m.blkRow[b], col = m.colIdx[b], k0 = m.blkCStart[b], k1 = m.blkCStart[b + 1];
     if (instActive && !instActive[m.rowInst[r]]) return;
     double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -493,8 +498,8 @@ void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, d
                           double *out, int accumulate, float *out32, const uint8_t *instActive, bool stiff) {
     if (stiff && m.rotC && m.hasKind[0] && !m.vPtr) {
         auto go = [&](auto kernel) {
-            hipLaunchKernelGGL(kernel, dim3((unsigned)((m.nBlocks + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m, blkC,
-                               cK, M, cM, out, accumulate, out32, instActive);
+            hipLaunchKernelGGL(kernel, dim3(8 * (unsigned)m.gatherPerRun), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out,
+                               accumulate, out32, instActive);
         };
         if (!m.hasKind[1] && !m.hasKind[2]) go(k_gather_blocks_rot<true>);
         else go(k_gather_blocks_rot<false>);
