@@ -77,6 +77,7 @@ struct od_server {
     int precision = OD_MIXED, precond = OD_PRE_IC0;
     double tol = 1e-6;
     int spmvR64 = 0, spmvL64 = 0, spmvR32 = 0, spmvL32 = 0;   // SpMV variant tuned on the first object, reused for the rest
+    int fbR64 = 1, fbL64 = 4, fbR32 = 4, fbL32 = 0;           // the best shapes without the grouped form (same bits)
     std::vector<od_ctx *> slotCtx;
     std::vector<Object> objs;
     // run state
@@ -139,7 +140,10 @@ void load(od_server *S, Object &o, od_ctx *ctx) {
           "od_mesh_create");
     check(od_mesh_set_material(o.mesh, o.kind.data(), o.E.data(), o.nu.data(), o.rho.data(), 0.0, 0.1),
           "od_mesh_set_material");
-    check(od_mesh_spmv_variant(o.mesh, S->spmvR64, S->spmvL64, S->spmvR32, S->spmvL32), "od_mesh_spmv_variant");
+    int sv = od_mesh_spmv_variant(o.mesh, S->spmvR64, S->spmvL64, S->spmvR32, S->spmvL32);
+    if (sv == OD_ERR_UNSUPPORTED)   // an object without the grouped form (rows of more than 16 blocks)
+        sv = od_mesh_spmv_variant(o.mesh, S->fbR64, S->fbL64, S->fbR32, S->fbL32);
+    check(sv, "od_mesh_spmv_variant");
     check(od_newmark_create(o.mesh, 0.25, 0.5, o.dt, 0.0, 0.0, S->precond, 1e-3, S->tol, 10000, S->precision, 1,
                             &o.it),
           "od_newmark_create");
@@ -416,11 +420,22 @@ int od_server_run(od_server *S, int nReq, const int *obj, const int *nSteps, con
                              o.constrained.empty() ? nullptr : o.constrained.data(), 1, nullptr, nullptr, &tm),
               "od_mesh_create");
         int chosen[4];
-        const int st = od_mesh_autotune(tm, nullptr, chosen);
+        double ms[25];
+        const int st = od_mesh_autotune(tm, ms, chosen);
         od_mesh_destroy(tm);
         check(st, "od_mesh_autotune");
         S->spmvR64 = chosen[0]; S->spmvL64 = chosen[1];
         S->spmvR32 = chosen[2]; S->spmvL32 = chosen[3];
+        // the fastest descriptor / slot-value shapes, for objects without the grouped form
+        static const int kR[3] = {1, 2, 4}, kL[3] = {16, 8, 4};
+        double b64 = 1e300, b32 = 1e300;
+        for (int lv = 0; lv < 3; lv++)
+            for (int v = 0; v < 3; v++) {
+                if (ms[3 * lv + v] < b64) { b64 = ms[3 * lv + v]; S->fbR64 = kR[v]; S->fbL64 = kL[lv]; }
+                if (ms[9 + 3 * lv + v] < b32) { b32 = ms[9 + 3 * lv + v]; S->fbR32 = kR[v]; S->fbL32 = kL[lv]; }
+            }
+        for (int v = 0; v < 3; v++)
+            if (ms[18 + v] < b32) { b32 = ms[18 + v]; S->fbR32 = kR[v]; S->fbL32 = 0; }
     }
     S->evictions = S->loads = 0;
     S->peak = S->allocated;
