@@ -139,6 +139,9 @@ static void prof_pair(od_ctx *c, hipEvent_t *e0, hipEvent_t *e1) {
 
 struct od_mesh {
     od_ctx *ctx = nullptr;
+    // references: the handle itself and every solver, matrix handle and integrator built on the
+    // mesh; the last release frees it, so destroying them in any order is safe
+    std::atomic<int> refs{1};
     Topology topo;
     DevMesh dm{};
     std::vector<void *> allocs;
@@ -268,13 +271,18 @@ static void values_from_kernel(const od_mesh *mesh, const double *in, double *re
         if (t.refPos[v] >= 0) ref[t.refPos[v]] = in[v];
 }
 
+static void mesh_retain(od_mesh *mesh);
+static void mesh_release(od_mesh *mesh);
+
 struct od_matrix {
     od_mesh *mesh = nullptr;
+    bool retained = false;   // holds a mesh reference (a handle; not a matrix inside an integrator)
     double *vals = nullptr;
     float *vals32 = nullptr;
     ~od_matrix() {
         if (vals) (void)hipFree(vals);
         if (vals32) (void)hipFree(vals32);
+        if (retained) mesh_release(mesh);
     }
 };
 
@@ -292,6 +300,7 @@ struct ScalarPrecond {
 
 struct od_solver {
     od_mesh *mesh = nullptr;
+    bool retained = false;      // holds a mesh reference (not the no-allocation solve's view)
     int precond = OD_PRE_JACOBI;
     double param = 1e-3;        // IC(0) pivot epsilon; threshold IC epsilon; SAINV epsilon
     double param2 = 1.0;        // threshold IC relaxedScale; LDL^T entry epsilon
@@ -321,10 +330,15 @@ struct od_solver {
     int lastMaxIters = 0;       // iterations of the previous solve (sizes the first launch batch)
     std::vector<int> sweepIters; // mixed form: iterations of each inner sweep of the previous solve
     size_t bytes = 0;
+    // Zero-filled in the mesh's stream order (the first product of a solve forms p = z + 0 *
+    // p_old, which a stale NaN or infinity in a fresh direction buffer would poison).  A
+    // synchronous hipMemset runs on the null stream, which the non-blocking context streams do
+    // not wait for: under concurrent streams it could land after the kernels that fill the buffer.
     void *alloc(size_t b) {
         void *ptr = nullptr;
         OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
         allocs.push_back(ptr);
+        OD_HIP(hipMemsetAsync(ptr, 0, std::max<size_t>(b, 16), mesh->ctx->stream));
         bytes += b;
         return ptr;
     }
@@ -341,6 +355,7 @@ struct od_solver {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (void *ptr : allocs) (void)hipFree(ptr);
         delete scalar;
+        if (retained) mesh_release(mesh);
     }
 };
 
@@ -357,6 +372,7 @@ constexpr int kGraphMaxLaunches = 4096;
 
 struct od_integrator {
     od_mesh *mesh = nullptr;
+    bool retained = false;   // holds a mesh reference
     int kind = 0;  // 0 Newmark, 1 backward Euler
     double beta = 0.25, gamma = 0.5, h = 1.0 / 60, alphaM = 0, alphaK = 0, tol = 1e-6;
     int maxIter = 10000;
@@ -388,11 +404,12 @@ struct od_integrator {
     uint8_t *commit = nullptr, *redo = nullptr;   // per instance: committed by the graph; to complete eagerly
     long long graphSteps = 0, graphFallbacks = 0, graphCaptures = 0;
     bool stepGraph = false;   // graph mode 2: whole-step graphs (mode 1: PCG iteration graphs only)
+    // zero-filled in the mesh's stream order (the state starts at rest; see od_solver::alloc)
     void *alloc(size_t b) {
         void *ptr = nullptr;
         OD_HIP(hipMalloc(&ptr, std::max<size_t>(b, 16)));
-        OD_HIP(hipMemset(ptr, 0, std::max<size_t>(b, 16)));
         allocs.push_back(ptr);
+        OD_HIP(hipMemsetAsync(ptr, 0, std::max<size_t>(b, 16), mesh->ctx->stream));
         bytes += b;
         return ptr;
     }
@@ -402,6 +419,7 @@ struct od_integrator {
         M.vals = nullptr; A.vals = nullptr; A.vals32 = nullptr; Dm.vals = nullptr;   // owned by allocs
         for (void *ptr : allocs) (void)hipFree(ptr);
         delete solver;
+        if (retained) mesh_release(mesh);
     }
 };
 
@@ -566,9 +584,6 @@ static void alloc_fp64_sweeps(od_solver *s) {
     const DevMesh &m = s->mesh->dm;
     s->U64 = (double *)s->alloc(sizeof(double) * (size_t)ic0_u32_floats(m));
     s->Ul64 = (double *)s->alloc(sizeof(double) * (size_t)std::max<int64_t>(1, ic0_ul32_floats(m)));
-    OD_HIP(hipMemsetAsync(s->U64, 0, sizeof(double) * (size_t)ic0_u32_floats(m), s->mesh->ctx->stream));
-    OD_HIP(hipMemsetAsync(s->Ul64, 0, sizeof(double) * (size_t)std::max<int64_t>(1, ic0_ul32_floats(m)),
-                          s->mesh->ctx->stream));
 }
 
 static od_solver *make_solver(od_mesh *mesh, int precond, double param, int precision) {
@@ -581,6 +596,8 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
     od_solver *s = new od_solver();
     try {
         s->mesh = mesh;
+        mesh_retain(mesh);
+        s->retained = true;
         s->precond = precond;
         s->param = scalar_precond(precond) ? param : (param > 0 ? param : 1e-3);
         s->param2 = precond == OD_PRE_LDLT ? 0.0 : 1.0;
@@ -620,15 +637,10 @@ static od_solver *make_solver(od_mesh *mesh, int precond, double param, int prec
         s->partials = (double *)s->alloc(sizeof(double) * 3 * (size_t)t.chunkRow0.size());
         s->doneIn = (int *)s->alloc(sizeof(int) * t.nInst);
         s->doneOut = (int *)s->alloc(sizeof(int) * t.nInst);
-        OD_HIP(hipMemset(s->doneIn, 0, sizeof(int) * t.nInst));
-        OD_HIP(hipMemset(s->doneOut, 0, sizeof(int) * t.nInst));
         s->scIn = (InstScalars *)s->alloc(sizeof(InstScalars) * t.nInst);
         s->scOut = (InstScalars *)s->alloc(sizeof(InstScalars) * t.nInst);
-        OD_HIP(hipMemset(s->scIn, 0, sizeof(InstScalars) * t.nInst));
-        OD_HIP(hipMemset(s->scOut, 0, sizeof(InstScalars) * t.nInst));
         s->mask = (uint8_t *)s->alloc(t.nInst);
         s->zeroMask = (uint8_t *)s->alloc(t.nInst);
-        OD_HIP(hipMemset(s->zeroMask, 0, t.nInst));
         s->iters.assign(t.nInst, 0);
         s->relres.assign(t.nInst, 0.0);
     } catch (...) {
@@ -997,6 +1009,18 @@ static void ctx_free(od_ctx *ctx) {
 // A mesh stops using ctx (destroyed or rebound): the last one frees a closed context.
 static void ctx_release(od_ctx *ctx) {
     if (ctx->meshes.fetch_sub(1) == 1 && ctx->closed.load()) ctx_free(ctx);
+}
+
+static void mesh_retain(od_mesh *mesh) { mesh->refs.fetch_add(1); }
+
+// Drops one reference; the last one waits for the mesh's stream, frees it and releases its
+// context.
+static void mesh_release(od_mesh *mesh) {
+    if (mesh->refs.fetch_sub(1) != 1) return;
+    od_ctx *ctx = mesh->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
+    delete mesh;
+    ctx_release(ctx);
 }
 
 int od_ctx_destroy(od_ctx *ctx) {
@@ -1390,12 +1414,7 @@ int od_mesh_bind_ctx(od_mesh *mesh, od_ctx *ctx) {
 
 int od_mesh_destroy(od_mesh *mesh) {
     API_BEGIN
-    if (mesh) {
-        od_ctx *ctx = mesh->ctx;
-        (void)hipStreamSynchronize(ctx->stream);
-        delete mesh;
-        ctx_release(ctx);
-    }
+    if (mesh) mesh_release(mesh);   // the mesh lives on while solvers, matrices or integrators use it
     API_END
 }
 
@@ -1612,6 +1631,8 @@ int od_matrix_create(od_mesh *mesh, od_matrix **out) {
     require(mesh && out, "NULL argument");
     od_matrix *m = new od_matrix();
     m->mesh = mesh;
+    mesh_retain(mesh);
+    m->retained = true;
     size_t b = sizeof(double) * ((size_t)mesh->topo.nValues + kValuePad);
     if (hipMalloc(&m->vals, b) != hipSuccess) { delete m; throw ApiError(OD_ERR_HIP, "hipMalloc failed for matrix values"); }
     OD_HIP(hipMemsetAsync(m->vals, 0, b, mesh->ctx->stream));
@@ -1976,6 +1997,8 @@ static od_integrator *make_integrator(od_mesh *mesh, int kind, int precond, doub
     od_integrator *it = new od_integrator();
     try {
         it->mesh = mesh;
+        mesh_retain(mesh);
+        it->retained = true;
         it->kind = kind;
         it->solver = make_solver(mesh, precond, param, precision);
         mesh->ensureContrib();

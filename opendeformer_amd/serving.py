@@ -4,7 +4,9 @@ A pool of seeded boxes (workloads.served_pool) is registered with an od_server;
 Poisson arrivals (workloads.served_requests) each advance one object 1..8 Newmark
 steps with an impulse on its first step.  Latency is finish - arrival (queueing
 included); throughput counts Newmark steps over the span from the first arrival
-to the last finish.
+to the last finish.  Before the streams every object advances one step, 128
+objects per batch, so that the objects the budget holds are resident (--no-warm
+starts with nothing resident).
 
     python -m opendeformer_amd.serving --objects 1024 --tets 1000 500000 \\
         --rates 50,200,1000,2000 --duration 60 --budget-gb 200
@@ -56,14 +58,29 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--no-warm", action="store_true",
+                    help="start the streams with nothing resident (default: every object first advances one step, "
+                         "in batches, so the objects that fit the budget are resident before the timed streams)")
+    ap.add_argument("--durations", default=None,
+                    help="per-rate stream durations in s (comma-separated; default --duration for every rate)")
     args = ap.parse_args()
     t0 = time.perf_counter()
     pool = W.served_pool(args.objects, seed=args.seed, tet_lo=args.tets[0], tet_hi=args.tets[1])
     srv = build_server(pool, int(args.budget_gb * 2 ** 30), slots=args.slots)
     print(json.dumps({"pool_objects": len(pool), "pool_tets": int(sum(w.n_tets for w in pool)),
+                      "pool_dof": int(sum(3 * w.X.shape[0] - len(w.fixed_dofs) for w in pool)),
                       "build_s": round(time.perf_counter() - t0, 1)}), flush=True)
-    for i, rate in enumerate(float(r) for r in args.rates.split(",")):
-        print(json.dumps(run_stream(srv, pool, rate, args.duration, seed=args.seed + 1 + i)), flush=True)
+    if not args.no_warm:
+        for b0 in range(0, len(pool), 128):
+            ids = np.arange(b0, min(b0 + 128, len(pool)), dtype=np.int32)
+            t1 = time.perf_counter()
+            out = srv.run(ids, np.ones(ids.size, np.int32), np.zeros(ids.size))
+            print(json.dumps({"warm_objects": [int(ids[0]), int(ids[-1])], "s": round(time.perf_counter() - t1, 2),
+                              "server": out["stats"]}), flush=True)
+    rates = [float(r) for r in args.rates.split(",")]
+    durs = [float(d) for d in args.durations.split(",")] if args.durations else [args.duration] * len(rates)
+    for i, (rate, dur) in enumerate(zip(rates, durs)):
+        print(json.dumps(run_stream(srv, pool, rate, dur, seed=args.seed + 1 + i)), flush=True)
 
 
 if __name__ == "__main__":

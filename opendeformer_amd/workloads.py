@@ -201,11 +201,11 @@ def subset(w: Workload, i: int) -> Workload:
                     rho=w.rho[t0:t1], dt=w.dt, steps=w.steps, shapes=[w.shapes[i]], meta=dict(w.meta))
 
 
-def served_pool(n_obj=1024, seed=5, tet_lo=1000, tet_hi=500000, cell=0.05, ordering="color4"):
+def served_pool(n_obj=1024, seed=5, tet_lo=1000, tet_hi=500000, cell=0.05, ordering="color4t"):
     """Pool of independent boxes for the serving stream: tets log-uniform in [tet_lo, tet_hi],
     aspect U(1, 6), corotational or neo-Hookean, E log-uniform [1e4, 1e6], nu U(0.3, 0.48),
-    the x = 0 face fixed, gravity.  Nodes are numbered by a 4-colouring by default (4 IC(0)
-    dependency levels per object)."""
+    the x = 0 face fixed, gravity.  Nodes are numbered by the tiled 4-colouring by default (4
+    IC(0) dependency levels per object, 64-row chunks that group by tile)."""
     objs = []
     for i in range(n_obj):
         s = seed * 100003 + i

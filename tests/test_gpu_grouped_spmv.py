@@ -94,3 +94,45 @@ def test_newmark_tiled_colour4_grouped_matches_oracle(ctx):
         err = np.linalg.norm(d - do) / np.linalg.norm(do)
         assert err < 1e-6, (step, err)
         assert abs(int(nm.stats()[0][0]) - int(it_o[0])) <= 1, step
+
+
+# launch forms: (SpMV shapes FP64, FP32), sweep form (lanes, slots) or None (the mesh's own tuning)
+FORMS = [("descriptor", ((1, 16), (1, 16)), (16, 0)),
+         ("grouped8_4", ((8, -1), (4, -1)), (8, 0)),
+         ("grouped4_8", ((4, -1), (8, -1)), (4, 4)),
+         ("slotvalues", ((2, 8), (4, 0)), (8, 2)),
+         ("tuned", None, None)]
+
+
+@pytest.mark.parametrize("precision", [capi.OD_FP64, capi.OD_MIXED], ids=["fp64", "mixed"])
+def test_newmark_forms_same_bits_on_served_pool_objects(ctx, precision):
+    """Newmark on the served stream's pool objects (tiled 4-colour numbering, 1k-50k tets) with
+    an impulse: every SpMV and sweep form, set by the caller or tuned on the mesh, gives the
+    bits of the plain descriptor form, step by step."""
+    import opendeformer_amd as od
+    pool = W.served_pool(8, seed=5, tet_lo=1000, tet_hi=50000)
+    rng = np.random.default_rng(3)
+    for i, w in enumerate(pool):
+        f = np.zeros((w.n_nodes, 3))
+        f[rng.integers(0, w.n_nodes, 8)] = rng.standard_normal((8, 3)) * 20
+        imp = w.node_to_free(f)
+        ref = None
+        for name, spmv, sweep in FORMS:
+            print(f"pool object {i} ({w.n_tets} tets), form {name}", flush=True)
+            mesh = make(ctx, w)
+            if spmv is not None:
+                od.mesh_spmv_variant(mesh, *spmv)
+                od.mesh_sweep_form(mesh, *sweep)
+            nm = od.NewmarkIntegrator(mesh, w.dt, precond=capi.OD_PRE_IC0, precision=precision, tol=1e-6)
+            nm.set_external_virtual_work(mesh.body_force(w.body))
+            nm.set_impulse(imp)
+            states = []
+            for _ in range(6):
+                nm.run_one_time_step()
+                states.append((nm.state()[0].copy(), int(nm.stats()[0][0])))
+            del nm, mesh
+            if ref is None:
+                ref = states
+                continue
+            for step, ((da, ia), (db, ib)) in enumerate(zip(ref, states)):
+                assert ia == ib and np.array_equal(da.view(np.uint64), db.view(np.uint64)), (i, name, step)
