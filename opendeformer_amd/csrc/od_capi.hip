@@ -426,16 +426,6 @@ struct od_integrator {
 // =====================================================================================
 // solver internals
 // =====================================================================================
-// Reads a large buffer once (one partial per workgroup): evicts the memory-side cache without
-// leaving dirty lines behind, as the read-mostly kernels before a PCG SpMV do.
-__global__ static void k_stream_read(const float4 *p, int64_t n, float *out) {
-    float s = 0.f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const float4 v = p[i];
-        s += v.x + v.y + v.z + v.w;
-    }
-    if (s == 12345.f) out[blockIdx.x] = s;   // keeps the loads; the buffer never holds this sum
-}
 
 // Autotuner of the PCG SpMV: times each chunks-per-workgroup variant (1, 2, 4) for FP64 and
 // FP32 values on this mesh's pattern, with the matrix out of every cache as it is inside a
@@ -476,8 +466,7 @@ static void autotune_spmv(od_mesh *mesh) {
                 if (!(prec == 0 ? m.grpOk64 : m.grpOk32)) continue;
                 float ms[3];
                 for (int rep = 0; rep < 3; rep++) {
-                    hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, st, (const float4 *)flush, flushN / 4,
-                                       flush);
+                    launch_stream_read(st, flush, flushN);
                     if (prec == 0)
                         launch_spmv_dot<double>(m, st, K, -1, (const double *)vals, (const double *)x, (double *)y,
                                                 partials, sc, e0, e1, nullptr, nullptr, (const double *)gvals);
@@ -494,8 +483,7 @@ static void autotune_spmv(od_mesh *mesh) {
         for (int v = 0; v < 3; v++) {
             float ms[3];
             for (int rep = 0; rep < 3; rep++) {
-                hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, st, (const float4 *)flush, flushN / 4,
-                                   flush);
+                launch_stream_read(st, flush, flushN);
                 launch_spmv_dot<float>(m, st, kSpmvVariants[v], 0, (const float *)vals, (const float *)x, (float *)y,
                                        partials, sc, e0, e1, (const float *)z, (float *)pNew, sv);
                 OD_HIP(hipEventSynchronize(e1));
@@ -510,8 +498,7 @@ static void autotune_spmv(od_mesh *mesh) {
                     const int R = kSpmvVariants[v], LS = kSpmvLaneVariants[lv];
                     float ms[3];
                     for (int rep = 0; rep < 3; rep++) {
-                        hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, st, (const float4 *)flush,
-                                           flushN / 4, flush);
+                        launch_stream_read(st, flush, flushN);
                         if (prec == 0)
                             launch_spmv_dot<double>(m, st, R, LS, (const double *)vals, (const double *)x, (double *)y,
                                                     partials, sc, e0, e1);

@@ -246,6 +246,8 @@ void launch_jacobi_diag(const DevMesh &m, hipStream_t s, const double *vals, T *
 // the FP64 sweeps); codes as launch_sweep reads them
 template <typename T>
 void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, uint8_t *fwd, uint8_t *bwd);
+// one read of a large buffer (nFloats floats) that evicts the caches before a timed launch
+void launch_stream_read(hipStream_t s, float *buf, int64_t nFloats);
 // IC(0) factor U of A (canonical layout) and, when given, its part-major sweep copies of type T
 // (Uc: diagonal blocks, 8 entries per row, then the upper blocks at their backward sweep
 // slots; Ulc: the lower blocks at their forward sweep slots; sizes ic0_u32_floats /

@@ -2440,14 +2440,33 @@ static void ic0_levels(const DevMesh &m, hipStream_t s, const std::vector<int> &
     OD_HIP(hipGetLastError());
 }
 
+// Reads a large buffer once (one partial per workgroup): evicts the memory-side cache without
+// leaving dirty lines behind, so that a timed launch after it reads its data from HBM, as the
+// solver's launches do (one PCG iteration's data exceeds the 256 MB memory-side cache).
+__global__ void k_stream_read(const float4 *p, int64_t n, float *out) {
+    float s = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float4 v = p[i];
+        s += v.x + v.y + v.z + v.w;
+    }
+    if (s == 12345.f) out[blockIdx.x] = s;   // keeps the loads; the buffer never holds this sum
+}
+
+void launch_stream_read(hipStream_t s, float *buf, int64_t nFloats) {
+    hipLaunchKernelGGL(k_stream_read, dim3(2048), dim3(256), 0, s, (const float4 *)buf, nFloats / 4, buf);
+    OD_HIP(hipGetLastError());
+}
+
 // Launch form of each level's sweeps in precision T: times 4, 8 and 16 lanes per row and the
 // slot kernel with 2, 4, 8 and 16 slots per thread (median of three launches each, on
-// zero-filled factors of this mesh's pattern) and keeps the fastest (launch_sweep's code).
+// zero-filled factors of this mesh's pattern, each after a read that evicts the caches, as
+// the solver's sweeps find them) and keeps the fastest (launch_sweep's code).
 // The last level's forward launch is timed with its fused backward step, as the solver runs it.
 template <typename T>
 void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, uint8_t *fwd, uint8_t *bwd) {
     const int nl = (int)levelPtr.size() - 1;
     const size_t n = 3 * (size_t)m.nRows;
+    const int64_t flushN = (int64_t)128 << 20;    // floats: 512 MB
     std::vector<void *> bufs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto get = [&](size_t bytes) {
@@ -2460,6 +2479,7 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
     try {
         T *U = (T *)get(sizeof(T) * (size_t)ic0_u32_floats(m)), *Ul = (T *)get(sizeof(T) * (size_t)ic0_ul32_floats(m));
         T *r = (T *)get(sizeof(T) * n), *z = (T *)get(sizeof(T) * n);
+        float *flush = (float *)get(4 * (size_t)flushN);
         // the launches that finish z form the r.z partials when the solver fuses that dot
         double *dots = ic0_dot_fused(m, levelPtr) ? (double *)get(8 * (size_t)std::max(1, m.nChunks)) : nullptr;
         OD_HIP(hipEventCreate(&e0));
@@ -2478,6 +2498,7 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
                     float ms[3];
                     for (int rep = 0; rep < 3; rep++) {
                         const bool last = dir == 0 && l == nl - 1, finishes = last || (dir == 1 && l < nl - 1);
+                        launch_stream_read(s, flush, flushN);
                         launch_sweep<T>(dir == 0, cv, m, s, l, levelPtr[l], cnt, U, Ul, r, z, nullptr, nullptr,
                                         last ? 1 : 0, finishes ? dots : nullptr, e0, e1);
                         OD_HIP(hipGetLastError());
