@@ -1145,13 +1145,15 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.pairBlk = mesh->upload(t.pairBlk);
         m.pairSlot = mesh->upload(t.pairSlot);
         m.pairT = mesh->upload(t.pairT);
-        m.slotSrc = nullptr; m.rotC = nullptr; m.gatherOrder = nullptr; m.gatherPerRun = 0;
+        m.slotSrc = nullptr; m.rotC = nullptr; m.blkCompact = nullptr; m.gatherOrder = nullptr; m.gatherPerRun = 0;
         if (!t.partial) {
             // the source of each contribution slot, for the compact corotational form
-            require(t.nTets < (1 << 26), "too many tets for the contribution source encoding", OD_ERR_UNSUPPORTED);
+            require(t.nTets < (1 << (31 - kSrcTetShift)), "too many tets for the contribution source encoding",
+                    OD_ERR_UNSUPPORTED);
             std::vector<int> src((size_t)std::max<int64_t>(1, t.nPairContrib), 0);
             for (size_t k = 0; k < t.pairSlot.size(); k++)
-                if (t.pairSlot[k] >= 0) src[t.pairSlot[k]] = (int)((k / 10) << 5 | (k % 10) << 1 | t.pairT[k]);
+                if (t.pairSlot[k] >= 0)
+                    src[t.pairSlot[k]] = (int)((k / 10) << kSrcTetShift | (k % 10) << 1 | t.pairT[k]);
             m.slotSrc = mesh->upload(src);
         }
         m.blkCStart = mesh->upload(t.blkCStart);
@@ -1581,8 +1583,14 @@ int od_mesh_set_material(od_mesh *mesh, const int *kind, const double *E, const 
     mesh->dm.thrNeo = neoThreshold;
     mesh->dm.hasKind[0] = mesh->dm.hasKind[1] = mesh->dm.hasKind[2] = 0;
     for (int i = 0; i < n; i++) mesh->dm.hasKind[kind[i]] = 1;
-    if (mesh->dm.hasKind[0] && mesh->dm.slotSrc && !mesh->dm.rotC)   // compact corotational form
+    if (mesh->dm.hasKind[0] && mesh->dm.slotSrc && !mesh->dm.rotC) {   // compact corotational form
         mesh->dm.rotC = (double *)mesh->alloc(sizeof(double) * 16 * (size_t)n);
+        mesh->dm.blkCompact = (uint8_t *)mesh->alloc(std::max<int64_t>(1, mesh->dm.nBlocks));
+    }
+    if (mesh->dm.slotSrc) {
+        launch_mark_stored(mesh->dm, st, mesh->topo.nPairContrib);
+        OD_HIP(hipStreamSynchronize(st));
+    }
     if (mesh->dm.hasKind[2] && !mesh->dm.plStrain) {
         // plastic tets before od_mesh_set_plastic: never yielding (the corotational response)
         std::vector<double> inf(n, HUGE_VAL), zero(n, 0.0);

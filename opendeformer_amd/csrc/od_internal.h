@@ -61,12 +61,15 @@ struct DevMesh {
     int *tPos;                                   // per block: its place in the transposed list (-1: diagonal)
     int *pairBlk, *pairSlot; uint8_t *pairT;    // 10 per tet
     // corotational tets in compact form (no partially constrained nodes): per contribution slot
-    // its source (tet << 5 | pair << 1 | transposed); per tet, from the last stiffness pass, the
+    // its source (tet << kSrcTetShift | stored flag | pair << 1 | transposed; the flag marks a
+    // tet that is not corotational, whose block the element pass stores); per tet, from the
+    // last stiffness pass, the
     // rotated shape-function gradients g_q = R dN_q (12) and lambda / (36 V0), mu / (36 V0).
     // The element pass stores these instead of the 10 blocks, and the block gather forms the
     // blocks R K0_ab R^T from them.
     int *slotSrc;
     double *rotC;                                // 16 per tet (NULL: no compact form)
+    uint8_t *blkCompact;                         // per block: has a corotational contribution (with rotC)
     // the compact gather's block order: 8 runs (one per XCD) of gatherPerRun workgroups of
     // kBlock blocks each, the blocks of the SpMV runs' chunks in order, padded with -1
     int *gatherOrder;
@@ -134,9 +137,14 @@ struct ElemParams {
     double czv, cza, cm;  // z = czv*v + cza*a  (multiplied by cm*M_e)
 };
 
+// contribution source encoding of the compact gather (DevMesh::slotSrc)
+constexpr int kSrcTetShift = 6, kSrcStored = 32;
+
 // ---- kernel launchers (od_kernels.hip) ------------------------------------------
 void launch_element(const DevMesh &m, hipStream_t s, const double *d, const double *v, const double *a,
                     const ElemParams &p, double *blkC, double *nodeC, const uint8_t *instActive);
+// sets the stored flag of each of the nContrib contribution sources from the tets' kinds
+void launch_mark_stored(const DevMesh &m, hipStream_t s, int64_t nContrib);
 // stiff: the contributions come from a stiffness pass (corotational tets in compact form are
 // formed from their rotations); otherwise (mass pass) every contribution is a stored block
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,

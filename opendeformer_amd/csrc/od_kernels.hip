@@ -366,12 +366,17 @@ void launch_element(const DevMesh &m, hipStream_t s, const double *d, const doub
 // =====================================================================================
 __constant__ int c_diagSrc[6] = {0, 1, 2, 4, 5, 8};
 
+// One thread per stored value, its contributions in slot order.  SKIPC: a stiffness pass on a
+// mesh with corotational tets in compact form and tets of other kinds, whose blocks with a
+// corotational contribution the compact gather forms; this pass takes the others.
+template <bool SKIPC>
 __global__ void k_gather_blocks(DevMesh m, const double *__restrict__ C, double cK, const double *__restrict__ M,
                                 double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
                                 const uint8_t *__restrict__ instActive) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= m.nBlocks * 9) return;
     const int b = (int)(gid / 9), c = (int)(gid % 9);
+    if (SKIPC && m.blkCompact[b]) return;
     const int r = m.blkRow[b], col = m.colIdx[b], k0 = m.blkCStart[b], k1 = m.blkCStart[b + 1];
     const int64_t bo = m.blkOff[b];
     if (instActive && !instActive[m.rowInst[r]]) return;
@@ -440,32 +445,40 @@ __global__ void k_gather_blocks_rot(DevMesh m, const double *__restrict__ C, dou
     if (j >= m.gatherPerRun) return;
     const int b = m.gatherOrder[((int64_t)(blockIdx.x & 7) * m.gatherPerRun + j) * kBlock + threadIdx.x];
     if (b < 0) return;
+    if (!ALLCOROT && !m.blkCompact[b]) return;   // no corotational contribution: the stored-block pass
     const int r = m.blkRow[b], col = m.colIdx[b], k0 = m.blkCStart[b], k1 = m.blkCStart[b + 1];
     if (instActive && !instActive[m.rowInst[r]]) return;
     double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     // contributions in batches of four whose loads are issued together (past the end: slot k0,
-    // dropped); the sums keep the slot order
+    // dropped); the sums keep the slot order.  A corotational contribution loads g_a, g_b and
+    // the two factors, any other its stored block (the source's stored flag, set from the
+    // tets' kinds, picks which)
     constexpr int NB = 4;
     for (int k = k0; k < k1; k += NB) {
         int src[NB];
-        double gv[NB][8];
+        double gv[NB][9];
 #pragma unroll
         for (int j = 0; j < NB; j++) src[j] = m.slotSrc[k + j < k1 ? k + j : k0];
 #pragma unroll
         for (int j = 0; j < NB; j++) {
-            const int e = src[j] >> 5, p = (src[j] >> 1) & 15, tr = src[j] & 1;
-            const int a = tr ? c_pairB[p] : c_pairA[p], bb = tr ? c_pairA[p] : c_pairB[p];
-            const double *g = m.rotC + 16 * (size_t)e;
-            gv[j][0] = g[3 * a]; gv[j][1] = g[3 * a + 1]; gv[j][2] = g[3 * a + 2];
-            gv[j][3] = g[3 * bb]; gv[j][4] = g[3 * bb + 1]; gv[j][5] = g[3 * bb + 2];
-            gv[j][6] = g[12]; gv[j][7] = g[13];
+            if (ALLCOROT || !(src[j] & kSrcStored)) {
+                const int e = src[j] >> kSrcTetShift, p = (src[j] >> 1) & 15, tr = src[j] & 1;
+                const int a = tr ? c_pairB[p] : c_pairA[p], bb = tr ? c_pairA[p] : c_pairB[p];
+                const double *g = m.rotC + 16 * (size_t)e;
+                gv[j][0] = g[3 * a]; gv[j][1] = g[3 * a + 1]; gv[j][2] = g[3 * a + 2];
+                gv[j][3] = g[3 * bb]; gv[j][4] = g[3 * bb + 1]; gv[j][5] = g[3 * bb + 2];
+                gv[j][6] = g[12]; gv[j][7] = g[13];
+            } else {
+                const double *c = C + (size_t)(k + j < k1 ? k + j : k0) * 9;
+#pragma unroll
+                for (int i = 0; i < 9; i++) gv[j][i] = c[i];
+            }
         }
 #pragma unroll
         for (int j = 0; j < NB; j++) {
             if (k + j >= k1) break;
             double kk[9];
-            const int e = src[j] >> 5;
-            if (ALLCOROT || m.kind[e] == 0) {
+            if (ALLCOROT || !(src[j] & kSrcStored)) {
                 const double *ga = gv[j], *gb = gv[j] + 3;
                 const double lf = gv[j][6], mf = gv[j][7];
                 const double dm = mf * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
@@ -475,7 +488,7 @@ __global__ void k_gather_blocks_rot(DevMesh m, const double *__restrict__ C, dou
                     for (int q = 0; q < 3; q++) kk[3 * i + q] = lf * ga[i] * gb[q] + mf * gb[i] * ga[q] + (i == q ? dm : 0.0);
             } else {
 #pragma unroll
-                for (int i = 0; i < 9; i++) kk[i] = C[(size_t)(k + j) * 9 + i];
+                for (int i = 0; i < 9; i++) kk[i] = gv[j][i];
             }
 #pragma unroll
             for (int i = 0; i < 9; i++) s[i] += kk[i];
@@ -494,6 +507,36 @@ __global__ void k_gather_blocks_rot(DevMesh m, const double *__restrict__ C, dou
     }
 }
 
+// The stored flag of every contribution source from its tet's material kind (set with the
+// materials): the compact gather reads the stored block of any tet that is not corotational.
+__global__ void k_mark_stored(int *slotSrc, int64_t n, const int *__restrict__ kind) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int sv = slotSrc[k];
+    slotSrc[k] = (sv & ~kSrcStored) | (kind[sv >> kSrcTetShift] != 0 ? kSrcStored : 0);
+}
+
+// Per block, whether any of its contributions is corotational (formed by the compact gather).
+__global__ void k_mark_compact_blocks(DevMesh m) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= m.nBlocks) return;
+    uint8_t f = 0;
+    for (int k = m.blkCStart[b]; k < m.blkCStart[b + 1]; k++)
+        if (!(m.slotSrc[k] & kSrcStored)) f = 1;
+    m.blkCompact[b] = f;
+}
+
+void launch_mark_stored(const DevMesh &m, hipStream_t s, int64_t nContrib) {
+    if (!m.slotSrc || nContrib <= 0) return;
+    hipLaunchKernelGGL(k_mark_stored, dim3((unsigned)((nContrib + 255) / 256)), dim3(256), 0, s, m.slotSrc, nContrib,
+                       m.kind);
+    OD_HIP(hipGetLastError());
+    if (m.blkCompact) {
+        hipLaunchKernelGGL(k_mark_compact_blocks, dim3((unsigned)((m.nBlocks + 255) / 256)), dim3(256), 0, s, m);
+        OD_HIP(hipGetLastError());
+    }
+}
+
 void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, double cK, const double *M, double cM,
                           double *out, int accumulate, float *out32, const uint8_t *instActive, bool stiff) {
     if (stiff && m.rotC && m.hasKind[0] && !m.vPtr) {
@@ -501,8 +544,16 @@ void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, d
             hipLaunchKernelGGL(kernel, dim3(8 * (unsigned)m.gatherPerRun), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out,
                                accumulate, out32, instActive);
         };
-        if (!m.hasKind[1] && !m.hasKind[2]) go(k_gather_blocks_rot<true>);
-        else go(k_gather_blocks_rot<false>);
+        if (!m.hasKind[1] && !m.hasKind[2]) {
+            go(k_gather_blocks_rot<true>);
+        } else {
+            // other kinds too: their blocks value by value (coalesced reads of the stored
+            // blocks), then the blocks with a corotational contribution in compact form
+            const int64_t n = m.nBlocks * 9;
+            hipLaunchKernelGGL(k_gather_blocks<true>, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m,
+                               blkC, cK, M, cM, out, accumulate, out32, instActive);
+            go(k_gather_blocks_rot<false>);
+        }
         OD_HIP(hipGetLastError());
         return;
     }
@@ -514,8 +565,8 @@ void launch_gather_blocks(const DevMesh &m, hipStream_t s, const double *blkC, d
     }
     int64_t n = m.nBlocks * 9;
     int grid = (int)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_gather_blocks, dim3(grid), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out, accumulate, out32,
-                       instActive);
+    hipLaunchKernelGGL(k_gather_blocks<false>, dim3(grid), dim3(kBlock), 0, s, m, blkC, cK, M, cM, out, accumulate,
+                       out32, instActive);
     OD_HIP(hipGetLastError());
 }
 
