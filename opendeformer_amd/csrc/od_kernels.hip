@@ -2510,14 +2510,17 @@ void launch_stream_read(hipStream_t s, float *buf, int64_t nFloats) {
 
 // Launch form of each level's sweeps in precision T: times 4, 8 and 16 lanes per row and the
 // slot kernel with 2, 4, 8 and 16 slots per thread (median of three launches each, on
-// zero-filled factors of this mesh's pattern, each after a read that evicts the caches, as
-// the solver's sweeps find them) and keeps the fastest (launch_sweep's code).
-// The last level's forward launch is timed with its fused backward step, as the solver runs it.
+// zero-filled factors of this mesh's pattern) and keeps the fastest (launch_sweep's code).
+// Each launch finds its data as the solver's sweeps find it: when a PCG iteration's factor
+// copies and matrix values exceed the 256 MB memory-side cache, after a read that evicts the
+// caches, else warm.  The last level's forward launch is timed with its fused backward step,
+// as the solver runs it.
 template <typename T>
 void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &levelPtr, uint8_t *fwd, uint8_t *bwd) {
     const int nl = (int)levelPtr.size() - 1;
     const size_t n = 3 * (size_t)m.nRows;
-    const int64_t flushN = (int64_t)128 << 20;    // floats: 512 MB
+    const bool cold = (int64_t)sizeof(T) * (ic0_u32_floats(m) + ic0_ul32_floats(m) + m.nValues) > ((int64_t)192 << 20);
+    const int64_t flushN = cold ? (int64_t)128 << 20 : 4;    // floats: 512 MB
     std::vector<void *> bufs;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto get = [&](size_t bytes) {
@@ -2549,7 +2552,7 @@ void tune_ic0_lanes(const DevMesh &m, hipStream_t s, const std::vector<int> &lev
                     float ms[3];
                     for (int rep = 0; rep < 3; rep++) {
                         const bool last = dir == 0 && l == nl - 1, finishes = last || (dir == 1 && l < nl - 1);
-                        launch_stream_read(s, flush, flushN);
+                        if (cold) launch_stream_read(s, flush, flushN);
                         launch_sweep<T>(dir == 0, cv, m, s, l, levelPtr[l], cnt, U, Ul, r, z, nullptr, nullptr,
                                         last ? 1 : 0, finishes ? dots : nullptr, e0, e1);
                         OD_HIP(hipGetLastError());
