@@ -1340,12 +1340,13 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
             const std::string err = buildGroupLayout(t, kChunkRows, 8, g);
             require(err.empty(), err.c_str(), OD_ERR_UNSUPPORTED);
             m.grpOk64 = m.grpOk32 = 0;
-            m.nGroups = m.perRunG = m.grpMaxDeps = 0;
+            m.nGroups = m.perRunG = m.grpMaxDeps = m.grpMaxHalo = 0;
             m.grpNu = m.grpNx = 0;
             if (g.ok) {
                 m.nGroups = g.nGroups;
                 m.perRunG = g.perRun;
                 m.grpMaxDeps = g.maxDeps;
+                m.grpMaxHalo = g.maxHalo;
                 m.grpNu = g.nUp;
                 m.grpNx = g.nX;
                 m.grpChunks = (const int4 *)mesh->upload(g.runChunks);
@@ -1356,8 +1357,10 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
                 m.grpUpSrc = mesh->upload(g.upSrc);
                 m.grpXMeta = (const int2 *)mesh->upload(g.xMeta);
                 m.grpXSrc = mesh->upload(g.xSrc);
-                m.grpOk64 = grp_lds_bytes<double>(m, 4) <= 65536 ? 1 : 0;   // 4 slots per thread: the most LDS
-                m.grpOk32 = grp_lds_bytes<float>(m, 4) <= 65536 ? 1 : 0;
+                m.grpHalo = (const int2 *)mesh->upload(g.grpHalo);
+                m.grpHaloRows = mesh->upload(g.haloRows);
+                m.grpOk64 = grp_lds_bytes<double>(m, 4) <= lds_max() ? 1 : 0;   // 4 slots per thread: the most LDS
+                m.grpOk32 = grp_lds_bytes<float>(m, 4) <= lds_max() ? 1 : 0;
             }
         }
         m.instChunkStart = mesh->upload(t.instChunkStart);

@@ -174,6 +174,35 @@ std::string buildGroupLayout(const Topology &t, int chunkRows, int nXcd, GroupLa
         g.runX[4 * (size_t)p + 1] = (int)(nX - base);
         g.runX[4 * (size_t)p + 2] = t.chunkInst[mem[0]];
     }
+    // ---- halos: per group the outside rows whose x it multiplies (the column rows of its
+    // crossing upper blocks and the upper owners of its crossing list), in row order; the
+    // metas name a row by its slot in the group's halo, whose x the product stages in LDS
+    g.grpHalo.assign(2 * (size_t)ng, 0);
+    {
+        std::vector<int> slotOf(t.nRows, -1), ext;
+        for (int p = 0; p < ng; p++) {
+            const int gi = order[p];
+            ext.clear();
+            for (int c : groups[gi])
+                for (int64_t q = g.chunkUp[2 * (size_t)c]; q < g.chunkUp[2 * (size_t)c] + (int64_t)g.chunkUp[2 * (size_t)c + 1] * chunkRows; q++)
+                    if (g.upMeta[q] >= 0) ext.push_back(g.upMeta[q]);
+            const int xb = g.runX[4 * (size_t)p], xn = g.runX[4 * (size_t)p + 1];
+            for (int q = xb; q < xb + xn; q++) ext.push_back(g.xMeta[2 * (size_t)q]);
+            std::sort(ext.begin(), ext.end());
+            ext.erase(std::unique(ext.begin(), ext.end()), ext.end());
+            for (size_t h = 0; h < ext.size(); h++) slotOf[ext[h]] = (int)h;
+            for (int c : groups[gi])
+                for (int64_t q = g.chunkUp[2 * (size_t)c]; q < g.chunkUp[2 * (size_t)c] + (int64_t)g.chunkUp[2 * (size_t)c + 1] * chunkRows; q++)
+                    if (g.upMeta[q] >= 0) g.upMeta[q] = slotOf[g.upMeta[q]];
+            for (int q = xb; q < xb + xn; q++) g.xMeta[2 * (size_t)q] = slotOf[g.xMeta[2 * (size_t)q]];
+            g.grpHalo[2 * (size_t)p] = (int)g.haloRows.size();
+            g.grpHalo[2 * (size_t)p + 1] = (int)ext.size();
+            g.maxHalo = std::max(g.maxHalo, (int)ext.size());
+            g.haloRows.insert(g.haloRows.end(), ext.begin(), ext.end());
+            for (int r : ext) slotOf[r] = -1;
+        }
+        if (g.haloRows.empty()) g.haloRows.assign(1, 0);
+    }
     while (nX % chunkRows) { g.xMeta.push_back(0); g.xMeta.push_back(-1); g.xSrc.push_back(-1); nX++; }
     if (nX == 0) { g.xMeta.assign(2, 0); g.xMeta[1] = -1; g.xSrc.assign(1, -1); }
     g.nX = nX;

@@ -107,11 +107,11 @@ struct DevMesh {
     // {upper-stream base, width}; per row nUp | items << 4 | depBase << 9; per upper position
     // its meta; per crossing position {column row, deposit}; the fill sources of both
     int grpOk64, grpOk32;   // the form fits (rows of at most 16 blocks, LDS within 64 KB)
-    int nGroups, perRunG, grpMaxDeps;
+    int nGroups, perRunG, grpMaxDeps, grpMaxHalo;
     int64_t grpNu, grpNx;
     const int4 *grpChunks, *grpX;
-    const int2 *grpChunkUp, *grpXMeta;
-    const int *grpRowInfo, *grpUpMeta, *grpUpSrc, *grpXSrc;
+    const int2 *grpChunkUp, *grpXMeta, *grpHalo;
+    const int *grpRowInfo, *grpUpMeta, *grpUpSrc, *grpXSrc, *grpHaloRows;
     int *instChunkStart, *instRowStart;          // nInst+1
     int *tetInst, *rowInst;
     // material (per tet)
@@ -186,12 +186,15 @@ constexpr int kGrpSlotVariants[2] = {4, 8};               // slots per thread of
 inline int64_t grp_values(const DevMesh &m) {
     return 9 * (m.grpNu + m.grpNx) + 6 * (int64_t)m.nChunks * kChunkRows;
 }
-// LDS of the grouped product with K slots per thread: the group's x, then the deposits, whose
-// area afterwards holds the cross-thread row sums (16 / K - 1 per row)
+// LDS a workgroup may hold on the current device (the device's per-block limit; 64 KB if the
+// query fails)
+size_t lds_max();
+// LDS of the grouped product with K slots per thread: the group's x, its halo's x, then the
+// deposits, whose area afterwards holds the cross-thread row sums (16 / K - 1 per row)
 template <typename T>
 size_t grp_lds_bytes(const DevMesh &m, int K) {
     const size_t red = (size_t)(16 / K - 1) * kGrpThreads;
-    return sizeof(T) * 3 * ((size_t)kGrpThreads + std::max((size_t)m.grpMaxDeps, red));
+    return sizeof(T) * 3 * ((size_t)kGrpThreads + (size_t)m.grpMaxHalo + std::max((size_t)m.grpMaxDeps, red));
 }
 template <typename T>
 void launch_fill_grp(const DevMesh &m, hipStream_t s, const T *vals, T *grpVals);

@@ -1392,6 +1392,14 @@ void launch_fill_grp(const DevMesh &m, hipStream_t s, const T *vals, T *grpVals)
 template void launch_fill_grp<double>(const DevMesh &, hipStream_t, const double *, double *);
 template void launch_fill_grp<float>(const DevMesh &, hipStream_t, const float *, float *);
 
+size_t lds_max() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        return 65536;
+    return (size_t)v;
+}
+
 // y = A x (or y += A x) over one group per workgroup, K of a row's 16 slots per thread and
 // S = 16 / K threads per row: wave v of the 4 S waves serves chunk v / S of the group (one row
 // per lane) with sub-index u = v mod S, and takes the row's slots u, u + S, ... (the slot-value
@@ -1400,11 +1408,12 @@ template void launch_fill_grp<float>(const DevMesh &, hipStream_t, const float *
 //      s - 1 of the row's chunk; a row's upper slots come first, so a position holding no
 //      block of this row, meta -1, is a lower or padding slot), its row's x (with FUSE the new
 //      direction z + beta p_old; sub-index 0 also writes it to pNew and to LDS), and its share
-//      of the group's crossing list.  Loads are unconditional (clamped positions, results
-//      dropped) so a thread's loads are in flight together.
-//   2. The upper terms A_ij x_j (x_j from LDS when row j is in the group, else gathered); for
-//      an internal block the column row's term A_ij^T x_i goes to that row's LDS deposit, and
-//      the crossing list fills the other deposits.
+//      of the group's halo (the x of the outside rows its blocks multiply) into LDS.  Loads
+//      are unconditional (clamped positions, results dropped) so a thread's loads are in
+//      flight together.
+//   2. The upper terms A_ij x_j (x_j from LDS: the group's rows or its halo); for an internal
+//      block the column row's term A_ij^T x_i goes to that row's LDS deposit, and the
+//      crossing list (its owners' x from the halo) fills the other deposits.
 //   3. The lower slots read their deposits; each thread sums its K slots in the 16-lane tree
 //      order, the S threads of a row combine through LDS in the same order, so the bits are
 //      those of the chunk-descriptor forms, and each chunk's dot partial is one wave's sum.
@@ -1421,12 +1430,14 @@ void k_spmv_grp(DevMesh m, const T *__restrict__ gv,
     constexpr int KB = sizeof(T) == 8 ? 2 : 4;   // slots whose loads are in flight together
     extern __shared__ __align__(16) unsigned char grpLds[];
     T *xs = reinterpret_cast<T *>(grpLds);   // 3 per local row
-    T *dep = xs + 3 * kGrpThreads;            // 3 per deposit; reused for the cross-thread sums
+    T *hx = xs + 3 * kGrpThreads;             // 3 per halo row
+    T *dep = hx + 3 * m.grpMaxHalo;           // 3 per deposit; reused for the cross-thread sums
     const bool single = m.nInst == 1;
     if (single && sc && sc[0].done) return;
     const int jb = (int)(blockIdx.x >> 3), pos = (int)(blockIdx.x & 7) * m.perRunG + jb;
     if (jb >= m.perRunG || pos >= m.nGroups) return;   // whole workgroup
     const int4 gc = m.grpChunks[pos], gx = m.grpX[pos];
+    const int2 gh = m.grpHalo[pos];
     const int t = (int)threadIdx.x, wv = t >> 6, lane = t & 63, ci = wv / S, u = wv % S;
     const int chunk = ci == 0 ? gc.x : (ci == 1 ? gc.y : (ci == 2 ? gc.z : gc.w));
     const int lr = ci * kChunkRows + lane;   // local row
@@ -1486,15 +1497,10 @@ void k_spmv_grp(DevMesh m, const T *__restrict__ gv,
 #pragma unroll
         for (int i = 0; i < KB; i++) {
             const int mt = meta[i0 + i];
-            const int g = mt >= 0 ? mt : 0, lc = (mt >> 14) & 0xFF;
-            T c[3] = {x[3 * g], x[3 * g + 1], x[3 * g + 2]};
-            if (FUSE) {
-                c[0] = fma(beta, c[0], z[3 * g]);
-                c[1] = fma(beta, c[1], z[3 * g + 1]);
-                c[2] = fma(beta, c[2], z[3 * g + 2]);
-            }
-            if (mt < 0) { c[0] = xs[3 * lc]; c[1] = xs[3 * lc + 1]; c[2] = xs[3 * lc + 2]; }
             if (mt == -1) continue;   // not an upper slot of this row (the diagonal, a lower slot, padding)
+            // the column's x: a halo row's, or (internal block) a row of the group
+            const T *cx = mt >= 0 ? hx + 3 * mt : xs + 3 * ((mt >> 14) & 0xFF);
+            const T c[3] = {cx[0], cx[1], cx[2]};
             const T o0 = slot_term(e[i][0], e[i][1], e[i][2], c);
             const T o1 = slot_term(e[i][3], e[i][4], e[i][5], c);
             const T o2 = slot_term(e[i][6], e[i][7], e[i][8], c);
@@ -1530,13 +1536,8 @@ void k_spmv_grp(DevMesh m, const T *__restrict__ gv,
     auto use_x = [&]() {
 #pragma unroll
         for (int i = 0; i < XB; i++) {
-            T c[3] = {x[3 * xm[i].x], x[3 * xm[i].x + 1], x[3 * xm[i].x + 2]};
-            if (FUSE) {
-                c[0] = fma(beta, c[0], z[3 * xm[i].x]);
-                c[1] = fma(beta, c[1], z[3 * xm[i].x + 1]);
-                c[2] = fma(beta, c[2], z[3 * xm[i].x + 2]);
-            }
             if (xm[i].y < 0) continue;
+            const T c[3] = {hx[3 * xm[i].x], hx[3 * xm[i].x + 1], hx[3 * xm[i].x + 2]};
             T *o = dep + 3 * xm[i].y;
             o[0] = slot_term(xe[i][0], xe[i][3], xe[i][6], c);
             o[1] = slot_term(xe[i][1], xe[i][4], xe[i][7], c);
@@ -1546,6 +1547,17 @@ void k_spmv_grp(DevMesh m, const T *__restrict__ gv,
     if (u == 0) {
         xs[3 * lr] = px[0]; xs[3 * lr + 1] = px[1]; xs[3 * lr + 2] = px[2];
         if (FUSE && row >= 0) { pNew[3 * row] = px[0]; pNew[3 * row + 1] = px[1]; pNew[3 * row + 2] = px[2]; }
+    }
+    // the halo's x (with FUSE its direction z + beta p_old, formed as the rows' own)
+    for (int h = t; h < gh.y; h += NT) {
+        const int hr = m.grpHaloRows[gh.x + h];
+        T c[3] = {x[3 * hr], x[3 * hr + 1], x[3 * hr + 2]};
+        if (FUSE) {
+            c[0] = fma(beta, c[0], z[3 * hr]);
+            c[1] = fma(beta, c[1], z[3 * hr + 1]);
+            c[2] = fma(beta, c[2], z[3 * hr + 2]);
+        }
+        hx[3 * h] = c[0]; hx[3 * h + 1] = c[1]; hx[3 * h + 2] = c[2];
     }
     __syncthreads();
     // 2. terms and deposits
@@ -1638,6 +1650,13 @@ void launch_spmv_dot(const DevMesh &m, hipStream_t s, int R, int LS, const T *va
         const size_t lds = grp_lds_bytes<T>(m, R);
         const dim3 grid(8 * (unsigned)m.perRunG);
         auto go = [&](auto kernel, int threads) {
+            // LDS beyond 64 KB (gfx950 gives a workgroup up to the CU's 160 KB): the kernel's
+            // dynamic limit is raised to it once
+            static bool raised = false;
+            if (lds > 65536 && !raised) {
+                OD_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max()));
+                raised = true;
+            }
             if (start)
                 hipExtLaunchKernelGGL(kernel, grid, dim3(threads), lds, s, start, stop, 0, m, slotVals, x, y, 0,
                                       partials, sc, z, pNew);

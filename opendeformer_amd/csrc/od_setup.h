@@ -99,15 +99,21 @@ struct GroupLayout {
     std::string why;                 // reason when !ok
     int nGroups = 0, perRun = 0;     // groups in run order, cut into nXcd runs of perRun
     int maxDeps = 0;                 // most LDS deposits (lower slots) of any group
+    int maxHalo = 0;                 // most halo rows of any group
     int64_t nUp = 0, nX = 0;         // upper-stream positions (padded per chunk), crossing positions (padded to 64)
     std::vector<int> runChunks;      // 4 per run position: the group's chunks (-1: none)
     std::vector<int> runX;           // 4 per run position: crossing base, crossing count, instance, 0
     std::vector<int> chunkUp;        // 2 per chunk: upper-stream base, width (slots)
     std::vector<int> rowInfo;        // per row: nUp | items << 4 | depBase << 9
-    std::vector<int> upMeta;         // per upper position: -1 padding; >= 0 crossing (column row);
-                                     // else 0x80000000 | local column << 14 | deposit
+    std::vector<int> upMeta;         // per upper position: -1 padding; >= 0 crossing (the column
+                                     // row's slot in the group's halo); else 0x80000000 | local
+                                     // column << 14 | deposit
     std::vector<int> upSrc;          // per upper position: value offset of the block (-1: padding)
-    std::vector<int> xMeta;          // 2 per crossing position: column row (upper owner), deposit (-1: padding)
+    std::vector<int> xMeta;          // 2 per crossing position: halo slot of the column row (the upper
+                                     // owner), deposit (-1: padding)
+    std::vector<int> grpHalo;        // 2 per run position: base in haloRows, count
+    std::vector<int> haloRows;       // per group, in row order: the rows outside the group whose x
+                                     // its upper blocks and crossing blocks multiply
     std::vector<int> xSrc;           // per crossing position: value offset of the block (-1: padding)
 };
 std::string buildGroupLayout(const Topology &t, int chunkRows, int nXcd, GroupLayout &g);

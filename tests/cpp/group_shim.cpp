@@ -62,9 +62,11 @@ int gs_build(int nNodes, const double *X, int nTets, const int *tets, int nCon, 
     return 0;
 }
 const char *gs_error(void) { return g_err.c_str(); }
+int gs_max_halo(void) { return g_g.maxHalo; }
 int gs_nvalues(void) { return (int)g_t.nValues; }
 // y_slot and y_grp (3 nRows each) from values (nValues, the canonical layout) and x (3 nRows).
-// Returns the number of deposits not written exactly once (0 when the layout is complete).
+// Returns the number of deposits not written exactly once plus the halo slots out of range, out
+// of order or naming a row of the group (0 when the layout is complete).
 long long gs_products(const double *vals, const double *x, double *ySlot, double *yGrp) {
     const od::Topology &t = g_t;
     for (int r = 0; r < t.nRows; r++) {
@@ -85,6 +87,18 @@ long long gs_products(const double *vals, const double *x, double *ySlot, double
         std::vector<int> hit((size_t)g.maxDeps + 1, 0);
         int nDep = 0;
         std::vector<std::vector<V3>> up(4 * 64, std::vector<V3>(15, V3{0, 0, 0}));
+        // the group's halo: outside rows, strictly increasing; a meta slot outside it is an error
+        const int hb = g.grpHalo[2 * p], hn = g.grpHalo[2 * p + 1];
+        for (int h = 1; h < hn; h++) bad += g.haloRows[hb + h] <= g.haloRows[hb + h - 1];
+        for (int h = 0; h < hn; h++)
+            for (int w = 0; w < 4; w++) {
+                const int c = g.runChunks[4 * p + w];
+                if (c >= 0) bad += g.haloRows[hb + h] >= t.chunkRow0[c] && g.haloRows[hb + h] < t.chunkRow1[c];
+            }
+        auto haloRow = [&](int h) {
+            if (h < 0 || h >= hn) { bad++; return 0; }
+            return g.haloRows[hb + h];
+        };
         for (int w = 0; w < 4; w++) {
             const int c = g.runChunks[4 * p + w];
             if (c < 0) continue;
@@ -105,13 +119,13 @@ long long gs_products(const double *vals, const double *x, double *ySlot, double
                         dep[d] = prod(e, x + 3 * r, 1);
                         hit[d]++;
                     } else {
-                        up[w * 64 + lane][k] = prod(e, x + 3 * meta, 0);
+                        up[w * 64 + lane][k] = prod(e, x + 3 * haloRow(meta), 0);
                     }
                 }
         }
         const int xb = g.runX[4 * p], xn = g.runX[4 * p + 1];
         for (int q = xb; q < xb + xn; q++) {
-            const int K = g.xMeta[2 * q], d = g.xMeta[2 * q + 1];
+            const int K = haloRow(g.xMeta[2 * q]), d = g.xMeta[2 * q + 1];
             dep[d] = prod(vals + g.xSrc[q], x + 3 * K, 1);
             hit[d]++;
         }

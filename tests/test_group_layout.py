@@ -5,7 +5,8 @@ same group feeds the column row through an LDS deposit, and the group's crossing
 the lower blocks whose upper owner lies outside.  tests/cpp/group_shim.cpp replays both device
 products on the host with the kernels' rounding (fused multiply-adds, the 16-lane slot tree):
 the grouped product must equal the slot product bit for bit, every deposit must be written
-exactly once, and both must match the oracle's SpMDV (core/numerMethod.h:134-262) and a dense
+exactly once, every halo (the outside rows whose x a group stages in LDS) must hold distinct
+outside rows in order, and both must match the oracle's SpMDV (core/numerMethod.h:134-262) and a dense
 product.  The 4-colour tiled numbering must make most upper blocks internal (the property the
 product's bytes rest on), and meshes with rows of more than 16 blocks must be refused.
 """
@@ -75,7 +76,7 @@ def test_grouped_product_equals_slot_product_bitwise(gs, name, make):
     assert st["ok"] == 1
     nvals = gs.gs_nvalues()
     vals, x, ys, yg, bad = products(gs, st, nvals, seed=7)
-    assert bad == 0, f"{bad} deposits not written exactly once"
+    assert bad == 0, f"{bad} deposits not written exactly once or halo slots out of place"
     assert np.array_equal(ys.view(np.uint64), yg.view(np.uint64))
     if st["rows"] <= 30000:
         s = O.System(w.X, w.tets, w.fixed_dofs)
