@@ -1753,9 +1753,13 @@ template void launch_spmv<float>(const DevMesh &, hipStream_t, const float *, co
 template <typename T>
 void pcg_alpha(const DevMesh &m, hipStream_t s, const T *vals, const T *pOld, const T *z, T *p, T *q, PcgWork &w,
                hipEvent_t start, hipEvent_t stop) {
-    const int L = sizeof(T) == 8 ? m.spmvL64 : m.spmvL32;
-    launch_spmv_dot<T>(m, s, sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, L, vals, z ? pOld : p, q, w.partials, w.sc, start,
-                       stop, z, z ? p : nullptr, L <= 0 ? (const T *)w.slotVals : nullptr);
+    int R = sizeof(T) == 8 ? m.spmvR64 : m.spmvR32, L = sizeof(T) == 8 ? m.spmvL64 : m.spmvL32;
+    if (sizeof(T) == 8 && L == -1 && !w.slotVals) {   // no grouped values (the allocation-free solve)
+        R = m.spmvR64d;
+        L = m.spmvL64d;
+    }
+    launch_spmv_dot<T>(m, s, R, L, vals, z ? pOld : p, q, w.partials, w.sc, start, stop, z, z ? p : nullptr,
+                       L <= 0 ? (const T *)w.slotVals : nullptr);
 }
 
 void pcg_alpha_finalize(const DevMesh &m, hipStream_t s, PcgWork &w) {

@@ -26,12 +26,17 @@ def system(ctx, w):
     return mesh, A
 
 
+@pytest.mark.parametrize("grouped", [False, True], ids=["tuned", "grouped"])
 @pytest.mark.parametrize("pre", [capi.OD_PRE_JACOBI, capi.OD_PRE_NONE])
-def test_noalloc_solve_equals_pcg_solve_on_two_calls(ctx, pre):
+def test_noalloc_solve_equals_pcg_solve_on_two_calls(ctx, pre, grouped):
+    """With grouped, the mesh runs the grouped SpMV form, whose values the allocation-free
+    solve cannot hold: it runs the chunk-descriptor shape instead (the same bits)."""
     import opendeformer_amd as od
     small = W._single("box_8x3x3", M.box_mesh(8, 3, 3, 0.4, 0.15, 0.15, seed=1), 0, 1e5, 0.45, 1.0 / 60, 1, "color4")
-    for call, w in enumerate([W.box_small("color4"), small]):
+    for call, w in enumerate([W.box_small("color4t" if grouped else "color4"), small]):
         mesh, A = system(ctx, w)
+        if grouped:
+            od.mesh_spmv_variant(mesh, (8, -1), (8, -1))
         rng = np.random.default_rng(10 + call)
         b = od.DeviceArray.from_host(ctx, rng.standard_normal(mesh.ndof))
         x0 = rng.standard_normal(mesh.ndof) * 1e-3
