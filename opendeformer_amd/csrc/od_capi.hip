@@ -1225,6 +1225,31 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
                 for (int q = 0; q < t.rowPtr[I + 1] - t.rowPtr[I] - 1; q++)
                     colB[(size_t)baseB[I] + (size_t)q * kChunkRows] = t.colIdx[t.rowPtr[I] + 1 + q];
             }
+            // per chunk and direction, the halo: the rows its slots' columns name, sorted; the
+            // columns become halo slots
+            auto halo = [&](std::vector<int> &col, int oi, int wi, int &H) {
+                std::vector<std::vector<int>> lists((size_t)std::max(1, nc));
+                H = 1;
+                for (int c = 0; c < nc; c++) {
+                    auto &L = lists[c];
+                    const int64_t o = ch[4 * (size_t)c + oi], e = o + (int64_t)ch[4 * (size_t)c + wi] * kChunkRows;
+                    for (int64_t q = o; q < e; q++)
+                        if (col[q] >= 0) L.push_back(col[q]);
+                    std::sort(L.begin(), L.end());
+                    L.erase(std::unique(L.begin(), L.end()), L.end());
+                    H = std::max(H, (int)L.size());
+                    for (int64_t q = o; q < e; q++)
+                        if (col[q] >= 0) col[q] = (int)(std::lower_bound(L.begin(), L.end(), col[q]) - L.begin());
+                }
+                std::vector<int> rows((size_t)std::max(1, nc) * H, 0);
+                for (int c = 0; c < nc; c++) std::copy(lists[c].begin(), lists[c].end(), rows.begin() + (size_t)c * H);
+                return rows;
+            };
+            int hF = 1, hB = 1;
+            const std::vector<int> haloF = halo(colF, 0, 1, hF), haloB = halo(colB, 2, 3, hB);
+            m.swHF = hF; m.swHB = hB;
+            m.swHaloF = mesh->upload(haloF);
+            m.swHaloB = mesh->upload(haloB);
             m.swChunk = (const int4 *)mesh->upload(ch);
             m.swColF = mesh->upload(colF);
             m.swColB = mesh->upload(colB);

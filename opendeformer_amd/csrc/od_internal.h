@@ -43,7 +43,12 @@ struct DevMesh {
     // chunk as wide as its longest row; swNF / swNB positions in all
     int64_t swNF, swNB;
     const int4 *swChunk;  // per chunk: {forward offset, forward width, backward offset, backward width}
-    int *swColF, *swColB; // column of each position (-1: padding)
+    int *swColF, *swColB; // per position: the column's slot in the chunk's halo (-1: padding)
+    // per chunk and direction its halo: the rows whose x its swept-side blocks multiply, in
+    // row order, at a fixed stride (swHF / swHB rows per chunk, padded with row 0), so that a
+    // sweep workgroup loads them with the chunk's record and stages their x in LDS
+    int *swHaloF, *swHaloB;
+    int swHF, swHB;
     int *swBaseF, *swBaseB;   // per row: its position of block 0 (offset + r)
     int *levelChunks;     // per level, the chunks holding its rows, in chunk order
     const int *levelChunkPtr;   // host memory: level l's chunks are levelChunks[ptr[l], ptr[l + 1])

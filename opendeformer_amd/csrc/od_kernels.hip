@@ -2315,7 +2315,9 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
 // (1 KB of consecutive float4 or double2 per part).  The sums combine as the 16-lane tree combines
 // them (pairs at distance 8, 4, 2, 1: inside the thread first, then across the waves through
 // LDS), so every K gives the bits of the lane-per-row kernels above.  Rows of the chunk
-// outside the level load their slots and discard the sums.
+// outside the level load their slots and discard the sums.  The x the blocks multiply come
+// from the chunk's halo, staged in LDS while the blocks load, so no block waits on an x
+// gather.
 template <typename T, bool FWD, int K, bool DOT>
 __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     DevMesh m, int p0, int count, int level, const int *__restrict__ chunks, const T *__restrict__ U,
@@ -2331,6 +2333,16 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     const int c = one && m.levelsContiguous ? p0 / kChunkRows + wg : chunks[wg];
     const int rr = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
     const int4 ch = m.swChunk[c];
+    // the chunk's halo rows (fixed stride per chunk: no record needed), loaded with the record
+    constexpr int NT = 1024 / K, HR = 2;
+    const int H = FWD ? m.swHF : m.swHB;
+    const int *__restrict__ hrows = (FWD ? m.swHaloF : m.swHaloB) + (int64_t)c * H;
+    int hr[HR];
+#pragma unroll
+    for (int i = 0; i < HR; i++) {
+        const int h = (int)threadIdx.x + i * NT;
+        hr[i] = hrows[h < H ? h : 0];
+    }
     const int row0 = one ? c * kChunkRows : m.chunkRow0[c], I = row0 + rr;
     const int row1 = one ? min(row0 + kChunkRows, m.nRows) : m.chunkRow1[c];
     const int off = FWD ? ch.x : ch.z, width = FWD ? ch.y : ch.w;
@@ -2353,11 +2365,12 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     T acc[K][3];
 #pragma unroll
     for (int i = 0; i < K; i++) acc[i][0] = acc[i][1] = acc[i][2] = 0;
-    for (int kb = 0; kb < width; kb += 16) {
-        T e[K][9], xv[K][3];
-        bool use[K];
-        // every load first: positions past the width repeat the last one (in the chunk's
-        // slots), padding columns read the first entries of y
+    T e[K][9];
+    int hs[K];
+    bool use[K];
+    // every load first: positions past the width repeat the last one (in the chunk's slots),
+    // padding columns name halo slot 0
+    auto load = [&](int kb) {
 #pragma unroll
         for (int i = 0; i < K; i++) {
             const int k = kb + w + W * i;
@@ -2365,9 +2378,29 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
             const int col = colA[pos];
             use[i] = live && k < width && col >= 0;
             load_pm(P, nb, pos, e[i]);
-            const T *xc = y + 3 * (int64_t)max(col, 0);
-            xv[i][0] = xc[0]; xv[i][1] = xc[1]; xv[i][2] = xc[2];
+            hs[i] = max(col, 0);
         }
+    };
+    if (width > 0) load(0);
+    // the halo's x into LDS (y of earlier levels, final; a row of this launch's level is
+    // named only by rows outside the level, whose sums are dropped)
+    extern __shared__ __align__(16) unsigned char swLds[];
+    T *hx = reinterpret_cast<T *>(swLds);
+#pragma unroll
+    for (int i = 0; i < HR; i++) {
+        const int h = (int)threadIdx.x + i * NT;
+        if (h < H) { hx[3 * h] = y[3 * hr[i]]; hx[3 * h + 1] = y[3 * hr[i] + 1]; hx[3 * h + 2] = y[3 * hr[i] + 2]; }
+    }
+    for (int h = (int)threadIdx.x + HR * NT; h < H; h += NT) {
+        const int hrw = hrows[h];
+        hx[3 * h] = y[3 * hrw]; hx[3 * h + 1] = y[3 * hrw + 1]; hx[3 * h + 2] = y[3 * hrw + 2];
+    }
+    __syncthreads();
+    for (int kb = 0; kb < width; kb += 16) {
+        if (kb > 0) load(kb);
+        T xv[K][3];
+#pragma unroll
+        for (int i = 0; i < K; i++) { xv[i][0] = hx[3 * hs[i]]; xv[i][1] = hx[3 * hs[i] + 1]; xv[i][2] = hx[3 * hs[i] + 2]; }
 #pragma unroll
         for (int i = 0; i < K; i++) {
             // forward: U_KI^T y_K (columns of the stored block); backward: U_IJ z_J (rows)
@@ -2448,12 +2481,19 @@ static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, in
         const int c0 = m.levelChunkPtr[level], nwg = m.levelChunkPtr[level + 1] - c0;
         if (nwg <= 0) return;
         const int K = 1 << sel;
+        const size_t lds = 3 * sizeof(T) * (size_t)(fwd ? m.swHF : m.swHB);   // the chunk's halo
         auto go = [&](auto kernel) {
             const dim3 grid(xcd_grid(nwg)), block(1024 / K);
-            if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, m, p0, cnt, level, m.levelChunks + c0,
-                                          U, Ul, r, z, sc, skip, last, nwg, partials);
-            else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, p0, cnt, level, m.levelChunks + c0, U, Ul, r, z, sc,
-                                    skip, last, nwg, partials);
+            static bool raised = false;   // a halo beyond 48 KB: the kernel's dynamic LDS limit raised once
+            if (lds > 49152 && !raised) {
+                OD_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds_max() - 16384));
+                raised = true;
+            }
+            if (e0) hipExtLaunchKernelGGL(kernel, grid, block, lds, s, e0, e1, 0, m, p0, cnt, level,
+                                          m.levelChunks + c0, U, Ul, r, z, sc, skip, last, nwg, partials);
+            else hipLaunchKernelGGL(kernel, grid, block, lds, s, m, p0, cnt, level, m.levelChunks + c0, U, Ul, r, z,
+                                    sc, skip, last, nwg, partials);
         };
         auto pick = [&](auto fwdTag, auto dotTag) {
             constexpr bool F = decltype(fwdTag)::value, D = decltype(dotTag)::value;
