@@ -1227,6 +1227,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
             }
             // per chunk and direction, the halo: the rows its slots' columns name, sorted; the
             // columns become halo slots
+            std::vector<int> hcnt(2 * (size_t)std::max(1, nc), 0);
             auto halo = [&](std::vector<int> &col, int oi, int wi, int &H) {
                 std::vector<std::vector<int>> lists((size_t)std::max(1, nc));
                 H = 1;
@@ -1238,6 +1239,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
                     std::sort(L.begin(), L.end());
                     L.erase(std::unique(L.begin(), L.end()), L.end());
                     H = std::max(H, (int)L.size());
+                    hcnt[2 * (size_t)c + (oi == 0 ? 0 : 1)] = (int)L.size();
                     for (int64_t q = o; q < e; q++)
                         if (col[q] >= 0) col[q] = (int)(std::lower_bound(L.begin(), L.end(), col[q]) - L.begin());
                 }
@@ -1250,6 +1252,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
             m.swHF = hF; m.swHB = hB;
             m.swHaloF = mesh->upload(haloF);
             m.swHaloB = mesh->upload(haloB);
+            m.swHaloCnt = (const int2 *)mesh->upload(hcnt);
             m.swChunk = (const int4 *)mesh->upload(ch);
             m.swColF = mesh->upload(colF);
             m.swColB = mesh->upload(colB);

@@ -49,6 +49,7 @@ struct DevMesh {
     // sweep workgroup loads them with the chunk's record and stages their x in LDS
     int *swHaloF, *swHaloB;
     int swHF, swHB;
+    const int2 *swHaloCnt;   // per chunk: the halo's rows forward, backward
     int *swBaseF, *swBaseB;   // per row: its position of block 0 (offset + r)
     int *levelChunks;     // per level, the chunks holding its rows, in chunk order
     const int *levelChunkPtr;   // host memory: level l's chunks are levelChunks[ptr[l], ptr[l + 1])

@@ -2335,6 +2335,7 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     const int4 ch = m.swChunk[c];
     // the chunk's halo rows (fixed stride per chunk: no record needed), loaded with the record
     constexpr int NT = 1024 / K, HR = 2;
+    const int2 hcn = m.swHaloCnt[c];
     const int H = FWD ? m.swHF : m.swHB;
     const int *__restrict__ hrows = (FWD ? m.swHaloF : m.swHaloB) + (int64_t)c * H;
     int hr[HR];
@@ -2386,12 +2387,13 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     // named only by rows outside the level, whose sums are dropped)
     extern __shared__ __align__(16) unsigned char swLds[];
     T *hx = reinterpret_cast<T *>(swLds);
+    const int hc = FWD ? hcn.x : hcn.y;   // the chunk's own halo rows (the rest is padding)
 #pragma unroll
     for (int i = 0; i < HR; i++) {
         const int h = (int)threadIdx.x + i * NT;
-        if (h < H) { hx[3 * h] = y[3 * hr[i]]; hx[3 * h + 1] = y[3 * hr[i] + 1]; hx[3 * h + 2] = y[3 * hr[i] + 2]; }
+        if (h < hc) { hx[3 * h] = y[3 * hr[i]]; hx[3 * h + 1] = y[3 * hr[i] + 1]; hx[3 * h + 2] = y[3 * hr[i] + 2]; }
     }
-    for (int h = (int)threadIdx.x + HR * NT; h < H; h += NT) {
+    for (int h = (int)threadIdx.x + HR * NT; h < hc; h += NT) {
         const int hrw = hrows[h];
         hx[3 * h] = y[3 * hrw]; hx[3 * h + 1] = y[3 * hrw + 1]; hx[3 * h + 2] = y[3 * hrw + 2];
     }
