@@ -1987,18 +1987,26 @@ __global__ __launch_bounds__(256) void k_ic0_factor_level(DevMesh m, const int *
     const int b0 = m.rowPtr[I], nOff = m.rowPtr[I + 1] - b0 - 1, v0 = m.valOff[I];
     const int k0 = m.tPtr[I], k1 = m.tPtr[I + 1];
     double u[6] = {0, 0, 0, 0, 0, 0};
+    // sum of U_KI^T U_KI over the row's lower blocks, spread over the row's 8 lanes (lane l takes
+    // blocks l, l + 8, ...) and combined in a fixed tree (pairs at distance 4, 2, 1)
+    double g[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = k0 + lane; k < k1; k += 8) {
+        const double *W = U + m.tOff[k];
+        const double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5], w6 = W[6], w7 = W[7], w8 = W[8];
+        g[0] += w0 * w0 + w3 * w3 + w6 * w6;
+        g[1] += w0 * w1 + w3 * w4 + w6 * w7;
+        g[2] += w0 * w2 + w3 * w5 + w6 * w8;
+        g[3] += w1 * w1 + w4 * w4 + w7 * w7;
+        g[4] += w1 * w2 + w4 * w5 + w7 * w8;
+        g[5] += w2 * w2 + w5 * w5 + w8 * w8;
+    }
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1)
+#pragma unroll
+        for (int j = 0; j < 6; j++) g[j] += __shfl_xor(g[j], o, 8);
     if (lane == 0) {
-        double s00 = A[v0], s01 = A[v0 + 1], s02 = A[v0 + 2], s11 = A[v0 + 3], s12 = A[v0 + 4], s22 = A[v0 + 5];
-        for (int k = k0; k < k1; k++) {
-            const double *W = U + m.tOff[k];
-            const double w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3], w4 = W[4], w5 = W[5], w6 = W[6], w7 = W[7], w8 = W[8];
-            s00 -= w0 * w0 + w3 * w3 + w6 * w6;
-            s01 -= w0 * w1 + w3 * w4 + w6 * w7;
-            s02 -= w0 * w2 + w3 * w5 + w6 * w8;
-            s11 -= w1 * w1 + w4 * w4 + w7 * w7;
-            s12 -= w1 * w2 + w4 * w5 + w7 * w8;
-            s22 -= w2 * w2 + w5 * w5 + w8 * w8;
-        }
+        const double s00 = A[v0] - g[0], s01 = A[v0 + 1] - g[1], s02 = A[v0 + 2] - g[2];
+        const double s11 = A[v0 + 3] - g[3], s12 = A[v0 + 4] - g[4], s22 = A[v0 + 5] - g[5];
         // dense Cholesky of the 3x3 diagonal block, pivot rule d <= 0 -> eps^2 d^2
         u[0] = sqrt(pivot_fix(s00, eps2));
         u[1] = s01 / u[0];
