@@ -1145,7 +1145,7 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.pairBlk = mesh->upload(t.pairBlk);
         m.pairSlot = mesh->upload(t.pairSlot);
         m.pairT = mesh->upload(t.pairT);
-        m.slotSrc = nullptr; m.rotC = nullptr; m.blkCompact = nullptr; m.gatherOrder = nullptr; m.gatherPerRun = 0;
+        m.slotSrc = nullptr; m.rotC = nullptr; m.blkCompact = nullptr; m.gatherSeg = nullptr; m.gatherPerRun = 0;
         if (!t.partial) {
             // the source of each contribution slot, for the compact corotational form
             require(t.nTets < (1 << (31 - kSrcTetShift)), "too many tets for the contribution source encoding",
@@ -1346,21 +1346,35 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
         m.spmvR64d = m.spmvR64;
         m.spmvL64d = m.spmvL64;
         if (m.slotSrc) {
-            // the compact gather's block order: the blocks of each XCD run's chunks, in order
+            // the compact gather's work: the blocks of each XCD run's chunks, in order, each cut
+            // into segments of at most kGatherSeg contributions, a block's segments within one
+            // workgroup
             const int nc = (int)t.chunkRow0.size();
-            std::vector<std::vector<int>> runs(8);
+            std::vector<std::vector<int>> runs(8);   // 4 ints per segment
             for (int p = 0; p < nc; p++) {
                 const int c = t.chunkSorted[p];
                 auto &run = runs[std::min(7, p / std::max(1, t.perRun))];
                 for (int r = t.chunkRow0[c]; r < t.chunkRow1[c]; r++)
-                    for (int b = t.rowPtr[r]; b < t.rowPtr[r + 1]; b++) run.push_back(b);
+                    for (int b = t.rowPtr[r]; b < t.rowPtr[r + 1]; b++) {
+                        const int k0 = t.blkCStart[b], k1 = t.blkCStart[b + 1];
+                        const int ns = std::max(1, (k1 - k0 + kGatherSeg - 1) / kGatherSeg);
+                        require(ns <= kBlock, "a block has too many contributions for the compact gather",
+                                OD_ERR_UNSUPPORTED);
+                        const size_t used = (run.size() / 4) % kBlock;
+                        if (used + ns > (size_t)kBlock)   // the block's segments start the next workgroup
+                            for (size_t q = used; q < (size_t)kBlock; q++) run.insert(run.end(), {-1, 0, 0, -1});
+                        for (int i = 0; i < ns; i++)
+                            run.insert(run.end(), {b, k0 + i * kGatherSeg, std::min(k1, k0 + (i + 1) * kGatherSeg),
+                                                   i == 0 ? ns - 1 : -1});
+                    }
             }
             size_t per = 0;
-            for (const auto &run : runs) per = std::max(per, (run.size() + kBlock - 1) / kBlock);
-            std::vector<int> order(8 * per * kBlock, -1);
-            for (int x = 0; x < 8; x++) std::copy(runs[x].begin(), runs[x].end(), order.begin() + (size_t)x * per * kBlock);
+            for (const auto &run : runs) per = std::max(per, (run.size() / 4 + kBlock - 1) / kBlock);
+            std::vector<int> seg(4 * 8 * per * kBlock);
+            for (size_t q = 0; q < seg.size(); q += 4) { seg[q] = -1; seg[q + 1] = 0; seg[q + 2] = 0; seg[q + 3] = -1; }
+            for (int x = 0; x < 8; x++) std::copy(runs[x].begin(), runs[x].end(), seg.begin() + 4 * (size_t)x * per * kBlock);
             m.gatherPerRun = (int)per;
-            m.gatherOrder = mesh->upload(order);
+            m.gatherSeg = (const int4 *)mesh->upload(seg);
         }
         {
             // grouped symmetric product: the layout, when the mesh allows it

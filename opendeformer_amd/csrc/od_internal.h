@@ -76,9 +76,11 @@ struct DevMesh {
     int *slotSrc;
     double *rotC;                                // 16 per tet (NULL: no compact form)
     uint8_t *blkCompact;                         // per block: has a corotational contribution (with rotC)
-    // the compact gather's block order: 8 runs (one per XCD) of gatherPerRun workgroups of
-    // kBlock blocks each, the blocks of the SpMV runs' chunks in order, padded with -1
-    int *gatherOrder;
+    // the compact gather's work: 8 runs (one per XCD) of gatherPerRun workgroups of kBlock
+    // segments each, the blocks of the SpMV runs' chunks in order, each block cut into segments
+    // of at most kGatherSeg contributions that one workgroup holds: {block, first and end
+    // contribution, segments that follow (first segment of the block) or -1}; padding block -1
+    const int4 *gatherSeg;
     int gatherPerRun;
     int *blkCStart;                              // nBlocks+1
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
@@ -145,6 +147,8 @@ struct ElemParams {
 
 // contribution source encoding of the compact gather (DevMesh::slotSrc)
 constexpr int kSrcTetShift = 6, kSrcStored = 32;
+// contributions per segment of the compact gather (a diagonal block has about 24)
+constexpr int kGatherSeg = 4;
 
 // ---- kernel launchers (od_kernels.hip) ------------------------------------------
 void launch_element(const DevMesh &m, hipStream_t s, const double *d, const double *v, const double *a,

@@ -439,22 +439,27 @@ __global__ void k_gather_blocks_rot(DevMesh m, const double *__restrict__ C, dou
                                     double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
                                     const uint8_t *__restrict__ instActive) {
     // blocks in the SpMV's run order (chunks sorted in space, one run per XCD): the j-th
-    // workgroup on XCD x takes the next 256 blocks of run x, so an XCD's L2 keeps the tets of
-    // its slab while all their blocks are formed
+    // workgroup on XCD x takes the next 256 segments of run x, so an XCD's L2 keeps the tets of
+    // its slab while all their blocks are formed.  A thread sums one segment (at most
+    // kGatherSeg contributions); the block's first segment adds the others' sums in order.
     const int j = (int)(blockIdx.x >> 3);
-    if (j >= m.gatherPerRun) return;
-    const int b = m.gatherOrder[((int64_t)(blockIdx.x & 7) * m.gatherPerRun + j) * kBlock + threadIdx.x];
-    if (b < 0) return;
-    if (!ALLCOROT && !m.blkCompact[b]) return;   // no corotational contribution: the stored-block pass
-    const int r = m.blkRow[b], col = m.colIdx[b], k0 = m.blkCStart[b], k1 = m.blkCStart[b + 1];
-    if (instActive && !instActive[m.rowInst[r]]) return;
+    if (j >= m.gatherPerRun) return;   // the whole workgroup
+    const int4 sg = m.gatherSeg[((int64_t)(blockIdx.x & 7) * m.gatherPerRun + j) * kBlock + threadIdx.x];
+    const int b = sg.x, k0 = sg.y, k1 = sg.z;
+    bool act = b >= 0 && (ALLCOROT || m.blkCompact[b]);   // else padding, or the stored-block pass's
+    int r = 0, col = 0;
+    if (act) {
+        r = m.blkRow[b];
+        col = m.colIdx[b];
+        if (instActive && !instActive[m.rowInst[r]]) act = false;
+    }
     double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     // contributions in batches of four whose loads are issued together (past the end: slot k0,
     // dropped); the sums keep the slot order.  A corotational contribution loads g_a, g_b and
     // the two factors, any other its stored block (the source's stored flag, set from the
     // tets' kinds, picks which)
-    constexpr int NB = 4;
-    for (int k = k0; k < k1; k += NB) {
+    constexpr int NB = kGatherSeg < 4 ? kGatherSeg : 4;
+    for (int k = k0; act && k < k1; k += NB) {
         int src[NB];
         double gv[NB][9];
 #pragma unroll
@@ -494,6 +499,15 @@ __global__ void k_gather_blocks_rot(DevMesh m, const double *__restrict__ C, dou
             for (int i = 0; i < 9; i++) s[i] += kk[i];
         }
     }
+    // the block's segments combine in order through LDS
+    __shared__ double part[kBlock][9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) part[threadIdx.x][i] = s[i];
+    __syncthreads();
+    if (!act || sg.w < 0) return;
+    for (int q = 1; q <= sg.w; q++)
+#pragma unroll
+        for (int i = 0; i < 9; i++) s[i] += part[threadIdx.x + q][i];
     const int64_t bo = m.blkOff[b];
     const bool diag = col == r;
     const int nv = diag ? 6 : 9;

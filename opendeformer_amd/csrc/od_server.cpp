@@ -101,10 +101,11 @@ struct od_server {
 namespace {
 
 // Device bytes an object needs before it has been loaded once (mesh, contribution slots,
-// matrices, factor, vectors, the FP32 slot values): measured 1.49-1.81 KB per tet on
-// 4-colour boxes from 960 to 1.9M tets, mixed precision the larger; 2 KB per tet bounds it.
+// matrices, factor and sweep copies, grouped values, halos, vectors): measured 1.73-3.03 KB
+// per tet on tiled 4-colour boxes from 264 to 500k tets in either precision, the smallest
+// boxes the most (structures of fixed size per chunk); 2.5 KB per tet plus 256 KB bounds it.
 // The first load replaces it with the measured size.
-long long estimate_bytes(const Object &o) { return 2048LL * o.nTets + (64LL << 10); }
+long long estimate_bytes(const Object &o) { return 2560LL * o.nTets + (256LL << 10); }
 
 // Free DOF of (node, axis), -1 when constrained (core/nodeIndexer.cpp:12-25).
 int free_dof(const Object &o, int node, int axis) {
