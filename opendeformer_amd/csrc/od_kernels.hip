@@ -427,13 +427,14 @@ __global__ void k_gather_values(DevMesh m, const double *__restrict__ C, double 
 __constant__ int c_pairA[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
 __constant__ int c_pairB[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
 
-// A = cK K + cM M with the compact corotational form, one thread per block: its contributions
-// in slot order.  A corotational tet's block is R K0_ab R^T; with K0_ab = f (lambda a b^T +
-// mu b a^T + mu (a . b) I) (the isotropic B^T D B of corot_k0, a = dN_a, b = dN_b, f =
-// 1 / (36 V0)) and R R^T = I that is f (lambda g_a g_b^T + mu g_b g_a^T + mu (g_a . g_b) I)
-// with the rotated gradients g = R dN the element pass stored; its transpose swaps a and b.
-// Any other tet's block is read from C.  Each value is summed in slot order, as
-// k_gather_blocks sums it.
+// A = cK K + cM M with the compact corotational form.  A corotational tet's block is
+// R K0_ab R^T; with K0_ab = f (lambda a b^T + mu b a^T + mu (a . b) I) (the isotropic B^T D B
+// of corot_k0, a = dN_a, b = dN_b, f = 1 / (36 V0)) and R R^T = I that is
+// f (lambda g_a g_b^T + mu g_b g_a^T + mu (g_a . g_b) I) with the rotated gradients g = R dN
+// the element pass stored; its transpose swaps a and b.  Any other tet's block is read from C.
+// A block's contributions are cut, in slot order, into segments of at most kGatherSeg; each
+// segment is summed in slot order by one thread and the segment sums are added in segment
+// order, so the rounding is fixed by the mesh, not by the launch.
 template <bool ALLCOROT>
 __global__ void k_gather_blocks_rot(DevMesh m, const double *__restrict__ C, double cK, const double *__restrict__ M,
                                     double cM, double *__restrict__ out, int accumulate, float *__restrict__ out32,
