@@ -1130,7 +1130,11 @@ struct CscCholesky : Preconditioner {
 
 // solver/inCholeskyPreconditioner.cpp:8-130 — left-looking threshold IC with
 // drop-and-compensate (the reference's "IC").  Parity pinned at epsilon = 0
-// (complete Cholesky) and by the compensation property LL^T - A PSD.
+// (complete Cholesky), by the compensation property LL^T - A PSD, and at epsilon > 0
+// without compensation by the drop rule recomputed densely from the factor, column by
+// column.  Parity unpinned: which entries are dropped when compensation is on (each
+// compensation raises the pivot before the column's later entries are tested, so the
+// decisions follow the sparse vector's visiting order).
 struct ThresholdIC : CscCholesky {
     Scalar epsilon, relaxedScale;
     ThresholdIC(Scalar eps, Scalar relax) : epsilon(eps), relaxedScale(relax) {}
@@ -1293,7 +1297,9 @@ static void SpMSV(const Scalar *values, const std::vector<std::vector<std::pair<
 
 // solver/inLDLTPreconditioner.cpp:8-184 — SAINV/RIF incomplete LDL^T with two
 // drop tolerances (2-argument form; the 3-argument constructor self-initialises
-// its members, inLDLTPreconditioner.cpp:9).  Pinned at both eps = 0 (exact LDL^T).
+// its members, inLDLTPreconditioner.cpp:9).  Pinned at both eps = 0 (exact LDL^T) and,
+// with sainvEpsilon = 0, at ldltEpsilon > 0 (the exact factor with its small entries
+// dropped).  Parity unpinned: the inverse-factor drops at sainvEpsilon > 0.
 struct SainvLDLT : Preconditioner {
     Scalar sainvEpsilon, ldltEpsilon;
     std::vector<Scalar> values, invDiagonal;
@@ -1731,6 +1737,8 @@ static Materials makeMaterials(int nTets, const int *kind, const Scalar *E, cons
 //      degrees in push order;
 //  (4) the sorts of components by size (:334-336) and of elements by label sum (:496)
 //      are stable.
+// Parity unpinned: the exact labels of the reference binary, which depend on those four
+// library orders (pinned instead by hand-traced graphs and the level-structure properties).
 // A disconnected mesh has no level structure over all nodes; it is rejected (the
 // reference leaves such nodes unnumbered).
 // ---------------------------------------------------------------------------

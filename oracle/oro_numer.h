@@ -265,7 +265,9 @@ inline void Orthonormalize(Scalar mat[8]) {
 
 // numerMethod.h:570-1016 — polar decomposition through the dominant eigenvector
 // of the 4x4 quaternion matrix B (Higham & Noferini).  ortho receives the
-// transposed rotation, sspd (optional) the symmetric factor, as the reference.
+// transposed rotation, sspd (optional) the symmetric factor, as the reference.  Pinned
+// against numpy's SVD polar factor and special cases; parity unpinned: which branch is
+// taken for inputs exactly at its thresholds (numerMethod.h:603-604, 715, 873-877).
 inline void polarDecompostion3x3(const Scalar *mat, Scalar *ortho, Scalar *sspd) {
     Scalar A[9], B[16];
     Scalar fnorm = 0;

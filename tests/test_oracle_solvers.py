@@ -138,6 +138,93 @@ def test_threshold_ic_complete_and_compensated(oracle):
     assert np.count_nonzero(L) < np.count_nonzero(L0)
 
 
+def ict_rule_violations(Ad, L, eps, rule):
+    """Checks a threshold-IC factor column by column against the drop rule of
+    InCholeskyPreconditioner::proccessSingleColumn (inCholeskyPreconditioner.cpp:18-64) with no
+    compensation (relaxedScale 0, so no decision depends on the order the column's entries are
+    visited): column i's candidates are w_r = A_ri - sum_{j<i} L_rj L_ij (r > i, w_r != 0); with
+    d_i = A_ii - sum_{j<i} L_ij^2 and d_r = A_rr - sum_{j<i} L_rj^2, an entry is kept iff
+    w_r^2 >= rule(eps, d_i, d_r), and then L_ri = w_r / L_ii with L_ii = sqrt(d_i), or
+    sqrt(eps^2 d_i^2) where d_i <= 0 (:48).  The last column is not tested, only rooted, with
+    no guard (:122-124): a negative last pivot gives NaN, as in the reference.  Everything is recomputed densely from L's
+    earlier columns, independently of the oracle's linked-list left-looking updates.  Returns
+    (wrong decisions, kept, dropped, largest relative value error); decisions within 1e-8 of
+    the threshold are not counted."""
+    n = Ad.shape[0]
+    wrong = kept = dropped = 0
+    err = 0.0
+    for i in range(n):
+        Li = L[i, :i]
+        di = Ad[i, i] - Li @ Li
+        if i == n - 1:
+            if di < 0:
+                assert np.isnan(L[i, i])
+            else:
+                err = max(err, abs(L[i, i] - np.sqrt(di)) / np.sqrt(di))
+            break
+        root = np.sqrt(di) if di > 0 else np.sqrt(eps * eps * di * di)
+        assert np.isfinite(L[i, i])
+        err = max(err, abs(L[i, i] - root) / root)
+        w = Ad[i + 1:, i] - L[i + 1:, :i] @ Li
+        dr = np.diag(Ad)[i + 1:] - np.einsum("ij,ij->i", L[i + 1:, :i], L[i + 1:, :i])
+        thr = rule(eps, di, dr)
+        for k in np.nonzero((w != 0) | (L[i + 1:, i] != 0))[0]:
+            if abs(w[k] * w[k] - thr[k]) <= 1e-8 * thr[k]:
+                continue
+            keep = w[k] * w[k] >= thr[k]
+            if keep != (L[i + 1 + k, i] != 0):
+                wrong += 1
+            elif keep:
+                kept += 1
+                err = max(err, abs(L[i + 1 + k, i] - w[k] / root) / abs(w[k] / root))
+            else:
+                dropped += 1
+    return wrong, kept, dropped, err
+
+
+@pytest.mark.parametrize("stiff", [1.0, 300.0])
+@pytest.mark.parametrize("eps", [0.02, 0.1, 0.3])
+def test_threshold_ic_drop_rule_reproduced_from_the_factor(oracle, eps, stiff):
+    """The threshold IC's drop decisions at epsilon > 0: each of the oracle's columns keeps
+    exactly the entries the rule keeps, with the right values, recomputed densely.  A plausible
+    misreading of the rule (the row's diagonal replaced by the column's) contradicts the factor."""
+    s, A = small_system(oracle)
+    K, _ = s.assemble(0, 1e5, 0.45)
+    A = A + (stiff - 1.0) * 0.25 / 3600 * K      # stiffer matrices: more fill, more drops
+    Ad = dense(s, A)
+    L = s.ict_dense(A, eps, 0.0)
+    rule = lambda e, di, dr: np.abs(e * e * di * dr)
+    wrong, kept, dropped, err = ict_rule_violations(Ad, L, eps, rule)
+    assert wrong == 0
+    assert kept > 0 and dropped > 0
+    assert err < 1e-9
+    misread = lambda e, di, dr: np.abs(e * e * di * di) * np.ones_like(dr)
+    assert ict_rule_violations(Ad, L, eps, misread)[0] > 0
+
+
+@pytest.mark.parametrize("ldlt_eps", [1e-3, 1e-2, 5e-2])
+def test_sainv_ldlt_factor_drop_against_exact_ldlt(oracle, ldlt_eps):
+    """SAINV with no inverse-factor drops (sainvEpsilon 0) forms the exact A-orthogonal inverse
+    factor, so the stored L is the exact unit LDL^T factor with the entries of magnitude at most
+    ldltEpsilon dropped (inLDLTPreconditioner.cpp:83-91) and D is exact: checked against
+    numpy's Cholesky factor C (L = C diag(C)^-1, D = diag(C)^2)."""
+    s, A = small_system(oracle, nx=3, ny=2, nz=2)
+    Ad = dense(s, A)
+    C = np.linalg.cholesky(Ad)
+    c = np.diag(C)
+    Lx, Dx = C / c, c * c
+    L, D = s.ldlt_dense(A, 0.0, ldlt_eps)
+    low = np.tril(np.ones_like(Lx, dtype=bool), -1)
+    clear = low & (np.abs(np.abs(Lx) - ldlt_eps) > 1e-9 * ldlt_eps)   # not at the threshold
+    keep = clear & (np.abs(Lx) > ldlt_eps)
+    drop = clear & ~keep
+    assert keep.sum() > 0 and (drop & (Lx != 0)).sum() > 0
+    np.testing.assert_allclose(L[keep], Lx[keep], rtol=1e-10, atol=1e-13)
+    assert np.all(L[drop] == 0.0)
+    np.testing.assert_allclose(np.diag(L), 1.0)
+    np.testing.assert_allclose(D, Dx, rtol=1e-11)
+
+
 def test_sainv_ldlt_exact_at_zero_tolerances(oracle):
     s, A = small_system(oracle, nx=3, ny=1, nz=1)
     Ad = dense(s, A)
