@@ -1268,6 +1268,15 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X, int nTets, const in
             }
             std::stable_sort(lcPairs.begin(), lcPairs.end(),
                              [](const std::pair<int, int> &a, const std::pair<int, int> &b) { return a.first < b.first; });
+            // several instances whose chunks mostly lie in one level (at most 1.25 level entries
+            // per chunk): the sweeps that finish z form the r.z chunk partials, each chunk's
+            // first at its highest level (levels finish in decreasing order)
+            m.sweepDotLists = t.nInst > 1 && 4 * (int64_t)lcPairs.size() <= 5 * (int64_t)nc ? 1 : 0;
+            {
+                std::vector<int> top((size_t)std::max(1, nc), 0);
+                for (const auto &pr : lcPairs) top[pr.second] = std::max(top[pr.second], pr.first);
+                m.chunkTopLevel = mesh->upload(top);
+            }
             mesh->levelChunkPtr.assign(t.nLevels + 1, 0);
             std::vector<int> lc;
             for (const auto &pr : lcPairs) { lc.push_back(pr.second); mesh->levelChunkPtr[pr.first + 1]++; }

@@ -86,6 +86,9 @@ struct DevMesh {
     int *nodeSlot, *rowCStart;                   // 4 per tet / nRows+1
     int *levelRows;                              // nRows
     int levelsContiguous;                        // levels are consecutive row ranges (levelRows is the identity)
+    int sweepDotLists;                           // several instances, short level chunk lists: the sweeps
+                                                 // that finish z form the r.z chunk partials
+    const int *chunkTopLevel;                    // per chunk, the highest level of its rows
     int *rowLevel;                               // IC(0) dependency level of each row
     // IC(0) update pairs of each upper block b = (I, J): the lower neighbours K of I (in
     // transposed-list order) whose row also holds column J, as value offsets of U_KI and U_KJ

@@ -2151,9 +2151,10 @@ __device__ __forceinline__ void load_diag(const T *__restrict__ U, int I, T (&d)
 }
 
 // Rows of a sweep workgroup.  Plain launches: kIcThreads / L consecutive rows of the level
-// (level-relative).  Launches that also form the r.z dot (DOT; one instance, levels as
-// consecutive row ranges): the rows of one 64-row chunk that lie in the level, so the
-// workgroup can reduce the chunk's r.z terms.  xcd_wg spreads either over the XCDs.
+// (level-relative).  Launches that also form the r.z dot (DOT; one instance with levels as
+// consecutive row ranges, or several instances, over the level's chunk list): the rows of one
+// 64-row chunk that lie in the level, so the workgroup can reduce the chunk's r.z terms.
+// xcd_wg spreads either over the XCDs.
 __device__ __forceinline__ int sweep_chunks(int p0, int count) {
     return (p0 + count - 1) / kChunkRows - p0 / kChunkRows + 1;
 }
@@ -2161,9 +2162,9 @@ __device__ __forceinline__ int sweep_chunks(int p0, int count) {
 // The chunk partial of r.z from the final z of the rows of chunk c in this level: the 192
 // products r_i z_i of the chunk (zero outside the level) summed as k_dot_rz sums them (one
 // wave per 64 entries, then the three wave sums in order).  The chunk's first launch (the
-// highest level it touches: levels finish in decreasing order) stores the sum, later ones
-// add theirs to it.  Every thread of the workgroup calls this.
-__device__ __forceinline__ void sweep_dot(const DevMesh &m, int c, int p0, int count, int lr, bool writer,
+// highest level it touches: levels finish in decreasing order) stores the sum, later ones add
+// theirs to it.  Every thread of the workgroup calls this.
+__device__ __forceinline__ void sweep_dot(const DevMesh &m, int c, int p0, int count, int level, int lr, bool writer,
                                           const double (&v)[3], double *__restrict__ partials) {
     __shared__ double rz[3 * kChunkRows];
     __shared__ double ws[3];
@@ -2178,7 +2179,8 @@ __device__ __forceinline__ void sweep_dot(const DevMesh &m, int c, int p0, int c
         double s = 0;
         for (int k = 0; k < 3; k++) s += ws[k];
         const int cEnd = min((c + 1) * kChunkRows, m.nRows);
-        partials[c] = cEnd <= p0 + count ? s : partials[c] + s;
+        const bool first = m.nInst != 1 ? level == m.chunkTopLevel[c] : cEnd <= p0 + count;
+        partials[c] = first ? s : partials[c] + s;
     }
 }
 
@@ -2188,17 +2190,25 @@ template <typename T, int L, bool DOT>
 __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
     DevMesh m, int p0, int count, const T *__restrict__ U, const T *__restrict__ Ul, const T *__restrict__ r,
     T *__restrict__ y, const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip, int lastLevel,
-    double *__restrict__ partials) {
+    double *__restrict__ partials, int level) {
     constexpr int kRows = DOT ? kChunkRows : kIcThreads / L, G = 16 / L;
     const int lane = threadIdx.x % L, lr = threadIdx.x / L;
     int I, c = 0;
     bool live;
     if constexpr (DOT) {
-        const int nwg = sweep_chunks(p0, count), wg = xcd_wg(nwg);
-        if (wg >= nwg) return;                  // grid padding: the whole workgroup
-        c = p0 / kChunkRows + wg;
-        I = c * kChunkRows + lr;
-        live = I >= p0 && I < p0 + count;
+        if (m.nInst == 1) {
+            const int nwg = sweep_chunks(p0, count), wg = xcd_wg(nwg);
+            if (wg >= nwg) return;              // grid padding: the whole workgroup
+            c = p0 / kChunkRows + wg;
+            I = c * kChunkRows + lr;
+            live = I >= p0 && I < p0 + count;
+        } else {                                // several instances: p0 and count name the
+            const int wg = xcd_wg(count);       // level's range of the chunk list
+            if (wg >= count) return;
+            c = m.levelChunks[p0 + wg];
+            I = m.chunkRow0[c] + lr;
+            live = I < m.chunkRow1[c] && m.rowLevel[I] == level;
+        }
     } else {
         const int pl = xcd_wg((count + kRows - 1) / kRows) * kRows + lr;
         if (pl >= count) return;
@@ -2254,7 +2264,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
             v[0] = (double)rhs[0] * (double)z0; v[1] = (double)rhs[1] * (double)z1; v[2] = (double)rhs[2] * (double)z2;
         }
     }
-    if constexpr (DOT) sweep_dot(m, c, p0, count, lr, lane == 0, v, partials);
+    if constexpr (DOT) sweep_dot(m, c, p0, count, level, lr, lane == 0, v, partials);
 }
 
 // backward: U z = y  ->  z_I = U_II^-1 (y_I - sum_{J>I} U_IJ z_J); with DOT r is the residual
@@ -2262,17 +2272,25 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
 template <typename T, int L, bool DOT>
 __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
     DevMesh m, int p0, int count, const T *__restrict__ U, T *__restrict__ z, const InstScalars *__restrict__ sc,
-    const uint8_t *__restrict__ skip, const T *__restrict__ r, double *__restrict__ partials) {
+    const uint8_t *__restrict__ skip, const T *__restrict__ r, double *__restrict__ partials, int level) {
     constexpr int kRows = DOT ? kChunkRows : kIcThreads / L, G = 16 / L;
     const int lane = threadIdx.x % L, lr = threadIdx.x / L;
     int I, c = 0;
     bool live;
     if constexpr (DOT) {
-        const int nwg = sweep_chunks(p0, count), wg = xcd_wg(nwg);
-        if (wg >= nwg) return;                  // grid padding: the whole workgroup
-        c = p0 / kChunkRows + wg;
-        I = c * kChunkRows + lr;
-        live = I >= p0 && I < p0 + count;
+        if (m.nInst == 1) {
+            const int nwg = sweep_chunks(p0, count), wg = xcd_wg(nwg);
+            if (wg >= nwg) return;              // grid padding: the whole workgroup
+            c = p0 / kChunkRows + wg;
+            I = c * kChunkRows + lr;
+            live = I >= p0 && I < p0 + count;
+        } else {                                // several instances: p0 and count name the
+            const int wg = xcd_wg(count);       // level's range of the chunk list
+            if (wg >= count) return;
+            c = m.levelChunks[p0 + wg];
+            I = m.chunkRow0[c] + lr;
+            live = I < m.chunkRow1[c] && m.rowLevel[I] == level;
+        }
     } else {
         const int pl = xcd_wg((count + kRows - 1) / kRows) * kRows + lr;
         if (pl >= count) return;
@@ -2329,7 +2347,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
             v[0] = (double)rv[0] * (double)x0; v[1] = (double)rv[1] * (double)x1; v[2] = (double)rv[2] * (double)x2;
         }
     }
-    if constexpr (DOT) sweep_dot(m, c, p0, count, lr, lane == 0, v, partials);
+    if constexpr (DOT) sweep_dot(m, c, p0, count, level, lr, lane == 0, v, partials);
 }
 
 // Sweeps over the sweep slots (FP32 or FP64 copies): one chunk of 64 rows per workgroup, 16 / K waves.  The
@@ -2486,7 +2504,7 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
             v[0] = (double)rv[0] * (double)x0; v[1] = (double)rv[1] * (double)x1; v[2] = (double)rv[2] * (double)x2;
         }
     }
-    if constexpr (DOT) sweep_dot(m, c, p0, count, rr, w == 0, v, partials);
+    if constexpr (DOT) sweep_dot(m, c, p0, count, level, rr, w == 0, v, partials);
 }
 
 // One sweep launch of level l in the form `code` names: code & 31 is the lanes per row of
@@ -2500,7 +2518,8 @@ static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, in
                          const T *Ul, const T *r, T *z, const InstScalars *sc, const uint8_t *skip, int last,
                          double *partials, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     const bool dot = partials != nullptr;
-    if (dot && (m.nInst != 1 || !m.levelsContiguous)) throw HipError("sweep dot needs one instance, consecutive levels");
+    if (dot && !(m.nInst == 1 ? m.levelsContiguous : m.sweepDotLists))
+        throw HipError("sweep dot needs one instance with consecutive levels, or short level chunk lists");
     const int L = code & 31, sel = code >> 5;
     if (sel) {
         const int c0 = m.levelChunkPtr[level], nwg = m.levelChunkPtr[level + 1] - c0;
@@ -2531,23 +2550,29 @@ static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, in
         else { if (dot) pick(std::false_type{}, std::true_type{}); else pick(std::false_type{}, std::false_type{}); }
         return;
     }
+    // a dot launch over several instances covers the level's chunks: its range of the chunk list
+    const bool chunkList = dot && m.nInst != 1;
+    const int q0 = chunkList ? m.levelChunkPtr[level] : p0;
+    const int qn = chunkList ? m.levelChunkPtr[level + 1] - q0 : cnt;
+    if (chunkList && qn <= 0) return;
     auto go = [&](auto kernel, int lanes, auto... args) {
-        const int nwg = dot ? (p0 + cnt - 1) / kChunkRows - p0 / kChunkRows + 1
-                            : (cnt + kIcThreads / lanes - 1) / (kIcThreads / lanes);
+        const int nwg = chunkList ? qn
+                        : dot    ? (p0 + cnt - 1) / kChunkRows - p0 / kChunkRows + 1
+                                 : (cnt + kIcThreads / lanes - 1) / (kIcThreads / lanes);
         const dim3 grid(xcd_grid(nwg)), block(dot ? kChunkRows * lanes : kIcThreads);
-        if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, m, p0, cnt, args...);
-        else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, p0, cnt, args...);
+        if (e0) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, m, q0, qn, args...);
+        else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, q0, qn, args...);
     };
     auto pick = [&](auto dotTag) {
         constexpr bool D = decltype(dotTag)::value;
         if (fwd) {
-            if (L == 16) go(k_ic0_fwd<T, 16, D>, 16, U, Ul, r, z, sc, skip, last, partials);
-            else if (L == 4) go(k_ic0_fwd<T, 4, D>, 4, U, Ul, r, z, sc, skip, last, partials);
-            else go(k_ic0_fwd<T, 8, D>, 8, U, Ul, r, z, sc, skip, last, partials);
+            if (L == 16) go(k_ic0_fwd<T, 16, D>, 16, U, Ul, r, z, sc, skip, last, partials, level);
+            else if (L == 4) go(k_ic0_fwd<T, 4, D>, 4, U, Ul, r, z, sc, skip, last, partials, level);
+            else go(k_ic0_fwd<T, 8, D>, 8, U, Ul, r, z, sc, skip, last, partials, level);
         } else {
-            if (L == 16) go(k_ic0_bwd<T, 16, D>, 16, U, z, sc, skip, r, partials);
-            else if (L == 4) go(k_ic0_bwd<T, 4, D>, 4, U, z, sc, skip, r, partials);
-            else go(k_ic0_bwd<T, 8, D>, 8, U, z, sc, skip, r, partials);
+            if (L == 16) go(k_ic0_bwd<T, 16, D>, 16, U, z, sc, skip, r, partials, level);
+            else if (L == 4) go(k_ic0_bwd<T, 4, D>, 4, U, z, sc, skip, r, partials, level);
+            else go(k_ic0_bwd<T, 8, D>, 8, U, z, sc, skip, r, partials, level);
         }
     };
     if (dot) pick(std::true_type{});
@@ -2723,7 +2748,7 @@ void pcg_update_xr_ic0(const DevMesh &m, hipStream_t s, const std::vector<int> &
 }
 
 bool ic0_dot_fused(const DevMesh &m, const std::vector<int> &levelPtr) {
-    return m.nInst == 1 && m.levelsContiguous && levelPtr.size() >= 3;
+    return (m.nInst == 1 ? m.levelsContiguous : m.sweepDotLists) && levelPtr.size() >= 3;
 }
 
 void pcg_finalize_iter(const DevMesh &m, hipStream_t s, PcgWork &w) {
