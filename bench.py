@@ -67,8 +67,7 @@ def spmv_bytes(mesh, value_bytes=8, vec_bytes=8, fused=False):
 
 
 def step_bytes(mesh, precision, iters, outer=0.0):
-    """Algorithmic HBM bytes of one Newmark step (the survey's per-unit figures, from the mesh's
-    counts): fixed work = element pass (130.7 B per tet: connectivity, material, node positions
+    """Algorithmic HBM bytes of one Newmark step (per-unit figures times the mesh's counts): fixed work = element pass (130.7 B per tet: connectivity, material, node positions
     and state, K0 recomputed), the M-combine read (8 B per stored value), two right-hand-side
     SpMVs and the IC(0) factorisation (read A, write the factor); per PCG iteration = one SpMV,
     one IC(0) apply (two sweeps over a factor with A's pattern: twice the SpMV's bytes) and
@@ -476,7 +475,7 @@ def main():
         spmv_ms, spmv_n = ctx.profile_read()
         ctx.profile(False)
         vb = 4 if precision == "mixed" else 8
-        alg = spmv_bytes(mesh, vb, vb)                       # the survey's SpMV unit
+        alg = spmv_bytes(mesh, vb, vb)                       # the SpMV's algorithmic bytes
         fused = spmv_bytes(mesh, vb, vb, fused=(vb == 4))    # what the FP32 product also moves
         # Launches of a batch that finds its instance converged exit at once; their (small) time
         # stays in the timed total, but only the iterations that ran count as SpMV work.
