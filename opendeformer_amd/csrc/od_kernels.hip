@@ -2151,9 +2151,10 @@ __device__ __forceinline__ void load_diag(const T *__restrict__ U, int I, T (&d)
 }
 
 // Rows of a sweep workgroup.  Plain launches: kIcThreads / L consecutive rows of the level
-// (level-relative).  Launches that also form the r.z dot (DOT; one instance with levels as
-// consecutive row ranges, or several instances, over the level's chunk list): the rows of one
-// 64-row chunk that lie in the level, so the workgroup can reduce the chunk's r.z terms.
+// (level-relative).  Launches that also form the r.z dot (DOT = 1: one instance with levels
+// as consecutive row ranges; DOT = 2: several instances, over the level's chunk list): the
+// rows of one 64-row chunk that lie in the level, so the workgroup can reduce the chunk's r.z
+// terms.
 // xcd_wg spreads either over the XCDs.
 __device__ __forceinline__ int sweep_chunks(int p0, int count) {
     return (p0 + count - 1) / kChunkRows - p0 / kChunkRows + 1;
@@ -2164,6 +2165,7 @@ __device__ __forceinline__ int sweep_chunks(int p0, int count) {
 // wave per 64 entries, then the three wave sums in order).  The chunk's first launch (the
 // highest level it touches: levels finish in decreasing order) stores the sum, later ones add
 // theirs to it.  Every thread of the workgroup calls this.
+template <int DOT>
 __device__ __forceinline__ void sweep_dot(const DevMesh &m, int c, int p0, int count, int level, int lr, bool writer,
                                           const double (&v)[3], double *__restrict__ partials) {
     __shared__ double rz[3 * kChunkRows];
@@ -2179,14 +2181,16 @@ __device__ __forceinline__ void sweep_dot(const DevMesh &m, int c, int p0, int c
         double s = 0;
         for (int k = 0; k < 3; k++) s += ws[k];
         const int cEnd = min((c + 1) * kChunkRows, m.nRows);
-        const bool first = m.nInst != 1 ? level == m.chunkTopLevel[c] : cEnd <= p0 + count;
+        bool first;
+        if constexpr (DOT == 1) first = cEnd <= p0 + count;
+        else first = level == m.chunkTopLevel[c];
         partials[c] = first ? s : partials[c] + s;
     }
 }
 
 // forward: U^T y = r  ->  y_I = U_II^-T (r_I - sum_{K<I} U_KI^T y_K), the U_KI from the
 // lower copy (FP64: contiguous per row I; FP32: row I's forward sweep slots)
-template <typename T, int L, bool DOT>
+template <typename T, int L, int DOT>
 __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
     DevMesh m, int p0, int count, const T *__restrict__ U, const T *__restrict__ Ul, const T *__restrict__ r,
     T *__restrict__ y, const InstScalars *__restrict__ sc, const uint8_t *__restrict__ skip, int lastLevel,
@@ -2196,7 +2200,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
     int I, c = 0;
     bool live;
     if constexpr (DOT) {
-        if (m.nInst == 1) {
+        if constexpr (DOT == 1) {
             const int nwg = sweep_chunks(p0, count), wg = xcd_wg(nwg);
             if (wg >= nwg) return;              // grid padding: the whole workgroup
             c = p0 / kChunkRows + wg;
@@ -2264,12 +2268,12 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_fwd(
             v[0] = (double)rhs[0] * (double)z0; v[1] = (double)rhs[1] * (double)z1; v[2] = (double)rhs[2] * (double)z2;
         }
     }
-    if constexpr (DOT) sweep_dot(m, c, p0, count, level, lr, lane == 0, v, partials);
+    if constexpr (DOT) sweep_dot<DOT>(m, c, p0, count, level, lr, lane == 0, v, partials);
 }
 
 // backward: U z = y  ->  z_I = U_II^-1 (y_I - sum_{J>I} U_IJ z_J); with DOT r is the residual
 // (the r.z terms of the final z)
-template <typename T, int L, bool DOT>
+template <typename T, int L, int DOT>
 __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
     DevMesh m, int p0, int count, const T *__restrict__ U, T *__restrict__ z, const InstScalars *__restrict__ sc,
     const uint8_t *__restrict__ skip, const T *__restrict__ r, double *__restrict__ partials, int level) {
@@ -2278,7 +2282,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
     int I, c = 0;
     bool live;
     if constexpr (DOT) {
-        if (m.nInst == 1) {
+        if constexpr (DOT == 1) {
             const int nwg = sweep_chunks(p0, count), wg = xcd_wg(nwg);
             if (wg >= nwg) return;              // grid padding: the whole workgroup
             c = p0 / kChunkRows + wg;
@@ -2347,7 +2351,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
             v[0] = (double)rv[0] * (double)x0; v[1] = (double)rv[1] * (double)x1; v[2] = (double)rv[2] * (double)x2;
         }
     }
-    if constexpr (DOT) sweep_dot(m, c, p0, count, level, lr, lane == 0, v, partials);
+    if constexpr (DOT) sweep_dot<DOT>(m, c, p0, count, level, lr, lane == 0, v, partials);
 }
 
 // Sweeps over the sweep slots (FP32 or FP64 copies): one chunk of 64 rows per workgroup, 16 / K waves.  The
@@ -2359,7 +2363,7 @@ __global__ __launch_bounds__(DOT ? kChunkRows * L : kIcThreads) void k_ic0_bwd(
 // outside the level load their slots and discard the sums.  The x the blocks multiply come
 // from the chunk's halo, staged in LDS while the blocks load, so no block waits on an x
 // gather.
-template <typename T, bool FWD, int K, bool DOT>
+template <typename T, bool FWD, int K, int DOT>
 __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
     DevMesh m, int p0, int count, int level, const int *__restrict__ chunks, const T *__restrict__ U,
     const T *__restrict__ Ul, const T *__restrict__ r, T *__restrict__ y,
@@ -2504,7 +2508,7 @@ __global__ __launch_bounds__(1024 / K) void k_ic0_slots(
             v[0] = (double)rv[0] * (double)x0; v[1] = (double)rv[1] * (double)x1; v[2] = (double)rv[2] * (double)x2;
         }
     }
-    if constexpr (DOT) sweep_dot(m, c, p0, count, level, rr, w == 0, v, partials);
+    if constexpr (DOT) sweep_dot<DOT>(m, c, p0, count, level, rr, w == 0, v, partials);
 }
 
 // One sweep launch of level l in the form `code` names: code & 31 is the lanes per row of
@@ -2540,14 +2544,25 @@ static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, in
                                     sc, skip, last, nwg, partials);
         };
         auto pick = [&](auto fwdTag, auto dotTag) {
-            constexpr bool F = decltype(fwdTag)::value, D = decltype(dotTag)::value;
+            constexpr bool F = decltype(fwdTag)::value;
+            constexpr int D = decltype(dotTag)::value;
             if (K == 2) go(k_ic0_slots<T, F, 2, D>);
             else if (K == 4) go(k_ic0_slots<T, F, 4, D>);
             else if (K == 8) go(k_ic0_slots<T, F, 8, D>);
             else go(k_ic0_slots<T, F, 16, D>);
         };
-        if (fwd) { if (dot) pick(std::true_type{}, std::true_type{}); else pick(std::true_type{}, std::false_type{}); }
-        else { if (dot) pick(std::false_type{}, std::true_type{}); else pick(std::false_type{}, std::false_type{}); }
+        using One = std::integral_constant<int, 1>;
+        using Many = std::integral_constant<int, 2>;
+        using None = std::integral_constant<int, 0>;
+        if (fwd) {
+            if (!dot) pick(std::true_type{}, None{});
+            else if (m.nInst == 1) pick(std::true_type{}, One{});
+            else pick(std::true_type{}, Many{});
+        } else {
+            if (!dot) pick(std::false_type{}, None{});
+            else if (m.nInst == 1) pick(std::false_type{}, One{});
+            else pick(std::false_type{}, Many{});
+        }
         return;
     }
     // a dot launch over several instances covers the level's chunks: its range of the chunk list
@@ -2564,7 +2579,7 @@ static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, in
         else hipLaunchKernelGGL(kernel, grid, block, 0, s, m, q0, qn, args...);
     };
     auto pick = [&](auto dotTag) {
-        constexpr bool D = decltype(dotTag)::value;
+        constexpr int D = decltype(dotTag)::value;
         if (fwd) {
             if (L == 16) go(k_ic0_fwd<T, 16, D>, 16, U, Ul, r, z, sc, skip, last, partials, level);
             else if (L == 4) go(k_ic0_fwd<T, 4, D>, 4, U, Ul, r, z, sc, skip, last, partials, level);
@@ -2575,8 +2590,9 @@ static void launch_sweep(bool fwd, int code, const DevMesh &m, hipStream_t s, in
             else go(k_ic0_bwd<T, 8, D>, 8, U, z, sc, skip, r, partials, level);
         }
     };
-    if (dot) pick(std::true_type{});
-    else pick(std::false_type{});
+    if (!dot) pick(std::integral_constant<int, 0>{});
+    else if (m.nInst == 1) pick(std::integral_constant<int, 1>{});
+    else pick(std::integral_constant<int, 2>{});
 }
 
 // Sweep launches for forward levels [f0, nl) (the last one also solving its backward step
