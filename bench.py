@@ -564,7 +564,7 @@ def main():
         # per level: lanes per row of the lane-per-row kernel, or slots per thread of the slot kernel
         codes = [(int(word) >> (4 * l)) & 15 for l in range(min(8, mesh.n_levels))]
         return [f"slots{1 << (c - 8)}" if c >= 8 else f"lanes{1 << c}" for c in codes]
-    line["ic0_sweeps"] = {"contiguous_levels": bool(info[12]), "form_fwd": forms(info[13]),
+    line["ic0_sweeps"] = {"contiguous_levels": bool(info[12] & 1), "form_fwd": forms(info[13]),
                           "form_bwd": forms(info[14])}
     if not args.no_damping:
         line["plane_damping"] = damping_leg(ctx, mesh, w)

@@ -104,8 +104,9 @@ int od_mesh_create(od_ctx *ctx, int nNodes, const double *X_host, int nTets, con
                    const int *instNodeStart_host, const int *instTetStart_host, od_mesh **out);
 int od_mesh_destroy(od_mesh *mesh);
 /* info[16]: ndof, nBlockRows, nBlocks, nValues, nLevels, nInstances, nTets,
- * nNodes, nChunks, nPairContrib, nNodeContrib, device bytes, 1 when the IC(0)
- * levels are consecutive row ranges, then the sweep form of the first 8 levels
+ * nNodes, nChunks, nPairContrib, nNodeContrib, device bytes, flags (bit 0: the
+ * IC(0) levels are consecutive row ranges; bit 1: several instances whose sweeps
+ * form the PCG's r.z partials), then the sweep form of the first 8 levels
  * (forward, backward; 4 bits per level: log2 of the lanes per row, or 8 +
  * log2(slots) for the FP32 slot kernel), 1 when some node is partially
  * constrained.  ndof and nValues are the reference's (vector entries, canonical

@@ -1469,7 +1469,7 @@ int od_mesh_info(od_mesh *mesh, long long *info) {
     info[0] = t.ndofFree; info[1] = t.nRows; info[2] = t.nBlocks; info[3] = t.nValuesRef; info[4] = t.nLevels;
     info[5] = t.nInst; info[6] = t.nTets; info[7] = t.nNodes; info[8] = (long long)t.chunkRow0.size();
     info[9] = t.nPairContrib; info[10] = t.nNodeContrib; info[11] = (long long)mesh->bytes;
-    info[12] = t.levelsContiguous ? 1 : 0;
+    info[12] = (t.levelsContiguous ? 1 : 0) | (mesh->dm.sweepDotLists ? 2 : 0);
     // per level of the first 8, 4 bits each (forward, backward): log2 of the sweep lanes per
     // row, or 8 + log2(K) when the FP32 sweep runs the slot kernel with K slots per thread
     auto lg = [](int v) { return (v >> 5) ? 8 + (v >> 5) : ((v & 31) == 16 ? 4 : ((v & 31) == 4 ? 2 : 3)); };

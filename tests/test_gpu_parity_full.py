@@ -179,6 +179,7 @@ def test_batch_256_instances_equal_alone_and_oracle(ctx):
     mesh = od.Mesh(ctx, wb.X, wb.tets, wb.fixed_dofs, wb.inst_node_start, wb.inst_tet_start)
     mesh.set_material(wb.kind, wb.E, wb.nu, wb.rho)
     irs, _ = mesh.rows()
+    assert mesh.info_raw()[12] & 2, "the batch's sweeps form the r.z partials"
     sizes = np.diff(wb.inst_tet_start)
     smallest = [int(i) for i in np.argsort(sizes, kind="stable")[:3]]
     sampled = sorted(set(list(range(0, 256, 51)) + smallest + [int(np.argmax(sizes))]))
